@@ -1,0 +1,158 @@
+/*
+ * peclet_dem.h — C-ABI of the MI355X-native peclet-dem substep library (libpecletdem_hip.so).
+ *
+ * Plain C types only (opaque handle, pointers, sizes).  Each entry point replaces one method of
+ * the reference C++ facade peclet::dem::Simulation (src/sim.hpp) as bound to Python by
+ * src/dem_bindings.cpp; the citation on each declaration names the reference interface it
+ * replaces.  Arrays cross the boundary as host FP64 buffers, row-major (N,3) / (N,4) / (N,).
+ *
+ * Status codes: 0 = OK, -1 = runtime error, -2 = invalid argument (the reference throws
+ * std::runtime_error / std::invalid_argument in those cases); pd_last_error() has the message.
+ * Handles are not thread-safe; one handle owns one HIP stream on the current device.
+ */
+#ifndef PECLET_DEM_H
+#define PECLET_DEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pd_sim pd_sim;
+
+/* Per-substep counters of the last pd_step (no reference counterpart: the reference exposes only
+ * num_contacts / num_manifolds / max_overlap, src/dem_bindings.cpp:358-366). */
+typedef struct pd_stats {
+  int num_real;
+  int num_particles;       /* real + periodic ghost slots */
+  int num_pairs;           /* broadphase candidate pairs */
+  int num_contacts;        /* narrowphase contacts */
+  int num_manifolds;       /* contact manifolds */
+  int active_manifolds;    /* manifolds solved by the velocity sweeps (colour >= 0) */
+  int num_colors_vel;
+  int num_colors_pos;
+  int rounds_vel;          /* Jones-Plassmann rounds, manifold graph */
+  int rounds_pos;          /* Jones-Plassmann rounds, contact graph */
+  int sweeps_vel;          /* executed main velocity sweeps */
+  int sweeps_stab;         /* executed stabilization sweeps */
+  int sweeps_pos;          /* executed position sweeps */
+  int stab_ran;
+  long long uncolored_vel; /* sum over colouring rounds of uncoloured manifolds */
+  long long uncolored_pos; /* sum over colouring rounds of uncoloured contacts */
+  double max_overlap;
+  double max_approach;
+} pd_stats;
+
+/* --- lifetime --------------------------------------------------------------------------- */
+pd_sim* pd_create(int capacity);      /* Simulation(int capacity), src/sim.hpp:911 */
+void pd_destroy(pd_sim* s);           /* ~Simulation / releaseViews, src/sim.hpp:917-936 */
+const char* pd_last_error(void);      /* exception message of the failed call (thread-local) */
+const char* pd_execution_space(void); /* module attr execution_space, src/dem_bindings.cpp:70 */
+int pd_synchronize(pd_sim* s);        /* Kokkos::fence equivalent: wait for the handle's stream */
+
+/* --- shape ------------------------------------------------------------------------------ */
+/* initializeShape(shape_type, radius, height, thickness), src/sim.hpp:949-1021 */
+int pd_initialize_shape(pd_sim* s, int shape_type, double radius, double height,
+                        double thickness);
+/* Extension (no reference counterpart): replace the surface point shell of shape 0 (N,3). */
+int pd_set_shell_points(pd_sim* s, const double* pts, int n);
+int pd_get_shell_points(pd_sim* s, double* out, int cap);
+
+/* --- domain and parameters -------------------------------------------------------------- */
+int pd_set_domain(pd_sim* s, double lx, double ly, double lz, int px, int py,
+                  int pz);                                     /* setDomain, src/sim.hpp:1086 */
+int pd_set_domain_minmax(pd_sim* s, double mnx, double mny, double mnz, double mxx, double mxy,
+                         double mxz);                          /* setDomainMinMax, :1091 */
+int pd_enable_periodicity(pd_sim* s, int x, int y, int z);    /* enablePeriodicity, :1101 */
+int pd_get_domain(pd_sim* s, double* mn3, double* mx3);       /* getDomainMin/Max, :1106-1111 */
+int pd_set_gravity(pd_sim* s, double gx, double gy, double gz);          /* setGravity, :1112 */
+int pd_set_thermostat(pd_sim* s, double temperature, double tau, double kB); /* :1113-1117 */
+int pd_set_solver_iterations(pd_sim* s, int pos, int vel);   /* setSolverIterations, :1117 */
+int pd_set_velocity_use_gs(pd_sim* s, int use_gs);           /* setVelocityUseGS, :1123 */
+int pd_set_stabilization_mode(pd_sim* s, int mode);  /* setStabilizationMode, :1136-1152 */
+int pd_set_global_scale(pd_sim* s, double gs);                /* setGlobalScale, :1171 */
+int pd_set_dt(pd_sim* s, double dt);                          /* setDt, :1175 */
+int pd_set_material_params(pd_sim* s, double e_n, double e_t, double mu); /* :1179-1183 */
+int pd_set_material_ids(pd_sim* s, const int* ids, int n);    /* setMaterialIds, :1187 */
+int pd_set_pair_material(pd_sim* s, int a, int b, double e, double mu); /* :1196-1215 */
+int pd_add_plane(pd_sim* s, double px, double py, double pz, double nx, double ny,
+                 double nz);                                  /* addPlane, :1226-1232 */
+
+/* --- particle arrays (host FP64) -------------------------------------------------------- */
+int pd_set_positions(pd_sim* s, const double* xyz, int n);    /* setPositions, :1290-1330 */
+int pd_set_velocities(pd_sim* s, const double* v, int n);     /* setVelocities, :1348 */
+int pd_set_external_forces(pd_sim* s, const double* f, int n); /* setExternalForces, :1358 */
+int pd_clear_external_forces(pd_sim* s);                      /* clearExternalForces, :1367 */
+int pd_set_quaternions(pd_sim* s, const double* q, int n);    /* setQuaternions, :1372 */
+int pd_set_angular_velocities(pd_sim* s, const double* w, int n); /* :1383 */
+int pd_set_inv_inertia(pd_sim* s, const double* ii, int n);   /* setInvInertia, :1392 */
+int pd_set_inv_mass(pd_sim* s, const double* im, int n);      /* setInvMass, :1401 */
+int pd_set_scales(pd_sim* s, const double* sc, int n);        /* setScales, :1340 */
+int pd_set_scales_uniform(pd_sim* s, double sc);              /* setScalesUniform, :1332 */
+int pd_set_growth_params(pd_sim* s, double rate, double new_factor); /* :1417-1425 */
+double pd_get_growth_factor(pd_sim* s);                       /* growthFactor, :1426 */
+double pd_get_growth_rate(pd_sim* s);                         /* getGrowthRate, :1427 */
+int pd_num_particles(pd_sim* s);                              /* numParticles, :1542 */
+int pd_get_positions(pd_sim* s, double* out, int n);          /* getPositions, :1440 */
+int pd_get_velocities(pd_sim* s, double* out, int n);         /* getVelocities, :1444 */
+int pd_get_quaternions(pd_sim* s, double* out, int n);        /* getQuaternions, :1448 */
+int pd_get_angular_velocities(pd_sim* s, double* out, int n); /* :1407 */
+int pd_get_inv_inertia(pd_sim* s, double* out, int n);        /* getInvInertia, :1411 */
+int pd_get_inv_mass(pd_sim* s, double* out, int n);           /* (getMasses input), :1430 */
+int pd_get_scales(pd_sim* s, double* out, int n);             /* getScales, :1452 */
+/* Device pointer of a live FP64 state array, the zero-copy get_*_view exports of
+ * src/dem_bindings.cpp:302-335.  Row strides in doubles: "pos" 4 (x, y, z, pad), "ext_force" 4,
+ * "vel" 8 (the {v, w} record: v at 0, angular velocity at 4), "inv_mass" 8 (packed record). */
+void* pd_get_device_pointer(pd_sim* s, const char* name);
+
+/* --- the substep ------------------------------------------------------------------------ */
+int pd_step(pd_sim* s, double dt);        /* Simulation::step -> demStep, src/sim.hpp:1456, :97 */
+/* Extension: n substeps of size dt without returning to the caller in between. */
+int pd_step_n(pd_sim* s, double dt, int n);
+int pd_compute_overlaps(pd_sim* s, double* out); /* computeOverlapsKokkos, src/sim.hpp:494 */
+int pd_num_contacts(pd_sim* s);                  /* numContacts, src/sim.hpp:1548 */
+int pd_num_manifolds(pd_sim* s);                 /* numManifolds, src/sim.hpp:1549 */
+double pd_max_overlap(pd_sim* s);                /* maxOverlap, src/sim.hpp:1550 */
+int pd_get_stats(pd_sim* s, pd_stats* out);
+
+/* --- diagnostics of the last substep (count returned; NULL output = count only) --------- */
+int pd_get_pairs(pd_sim* s, int* out, int cap);                      /* (i, j) pairs */
+int pd_get_contacts(pd_sim* s, int* ids, double* data, int cap);     /* n, rA, rB, dist */
+int pd_get_contact_colors(pd_sim* s, int* out, int cap);
+int pd_get_manifolds(pd_sim* s, int* ids, double* data, int cap);    /* A, B, points; sums */
+int pd_get_manifold_colors(pd_sim* s, int* out, int cap);
+int pd_get_ledger(pd_sim* s, uint64_t* keys, double* lambda, double* lambdaT, double* posImp,
+                  int cap);                                          /* warm-start store */
+int pd_get_grounded(pd_sim* s, int* out, int n);
+
+/* --- host-side export ------------------------------------------------------------------- */
+int pd_export_lammps(pd_sim* s, const char* filename, int step); /* exportLammps, :1463 */
+int pd_write_vtp(pd_sim* s, const char* filename);               /* writeVtp, :1560-1601 */
+/* module-level export_lammps, src/dem_bindings.cpp:436-464 / writeLammpsDump, src/io.hpp:23 */
+int pd_write_lammps_dump(const char* filename, int step, const double* pos, const double* vel,
+                         const double* quat, const double* radii, int n, const double* box_min,
+                         const double* box_max, int pbc);
+
+/* --- workload utility (no reference counterpart) ------------------------------------- */
+/* Random sequential addition of n spheres (radii[i]) from a caller-drawn candidate stream (3
+ * doubles per candidate, centres already inside [lo+r, hi-r]); see pd_sim.hip for the contract. */
+long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, int start,
+                  const double* lo, const double* hi, double* out, int* placed);
+
+/* --- measurement ------------------------------------------------------------------------ */
+/* Timing with HIP events on the handle's stream: level 0 off, 1 per-stage times, 2 also one
+ * event pair around every launch of the velocity-sweep (k_vsweep, PGS) and position-sweep
+ * (k_psweep) kernels.  Values accumulate until pd_set_timing is called again. */
+int pd_set_timing(pd_sim* s, int level);
+int pd_get_stage_times(pd_sim* s, double* out_ms, int cap);
+const char* pd_stage_name(int stage);
+/* per kernel (0 = k_vsweep PGS, 1 = k_psweep): summed ms, launch count, rows (manifolds or
+ * contacts) processed; returns the kernel count */
+int pd_get_kernel_times(pd_sim* s, double* ms, long* launches, long* rows, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PECLET_DEM_H */

@@ -1,0 +1,1711 @@
+// pd_kernels.hpp — HIP kernels of the XPBD substep (gfx950, FP64, wave64).
+//
+// Data layout in HBM (all FP64, 32-byte double4 rows):
+//   per slot (reals then periodic ghosts):
+//     pos[s]      committed position            vw0[s]  committed {v, w} (64 B record)
+//     bc[s]       {q, (invI.xyz, invMass)}      vw[s]   predicted {v, w} (64 B record)
+//     posPred[s]  predicted position, .w = broadphase radius       quatPred[s]
+//   contacts (narrowphase order): cAB int2, cN (n, dist), cRA (rA, e_b), cRB (rB, mu_b)
+//   manifolds (key order): mAB, mReal int2, mRange int2, mNs (Nsum, e_sum), mTA (tauA, mu_sum),
+//                          mTB, mRA, mRB double4, mNp int
+//   velocity rows (colour-sorted): vAB int2, vN, vTA, vTB, vRA, vRB, vVt0, vLamT double4, vVn0,
+//                                  vLam double, vFlag uchar
+//   position rows (colour-sorted): pAB int2, pN, pRA, pRB double4, pLam double
+// Every kernel cites the reference function it implements; the canonical orders are those of the
+// CPU oracle (ghost slots by real index, pairs lexicographic, contacts pair-major then planes,
+// manifolds by key, warm start by colour class, double-buffered grounded sweeps).
+#pragma once
+
+#include "pd_math.hpp"
+
+namespace pd {
+
+constexpr int TPB = 256;
+
+struct Body64 {
+  double4 v;
+  double4 w;
+};
+struct BodyC {
+  double4 q;
+  double4 iim;  // (invI.x, invI.y, invI.z, invMass)
+};
+
+struct Params {
+  double dt, gs, baseRadius, margin, band, boxCap, growthRate;
+  double restN, restT, mu;
+  d3 g;
+  d3 dmin, dmax, dsize;
+  int px, py, pz;
+  int numReal, numParticles;
+  int numPlanes;
+  int hasPairTable;
+  ShapeP shape;
+  double4 planeP[8];
+  double4 planeN[8];
+};
+
+struct Grid {
+  d3 lo;
+  double h;
+  int dim[3];
+};
+
+// device pointers of one simulation
+struct Dev {
+  // per slot
+  double4* pos;
+  Body64* vw0;
+  BodyC* bc;
+  double4* posPred;
+  double4* quatPred;
+  Body64* vw;
+  double* scale;
+  double* targetScale;
+  double4* extForce;
+  int* realIdx;
+  unsigned char* matId;
+  int* ground;
+  int* ground2;
+  double4* shell;
+  double* pairTable;
+  // broadphase
+  int* cellOf;
+  int* cellCnt;
+  int* cellStart;
+  int* order;
+  double4* sbox;
+  int* pairCnt;
+  int* pairStart;
+  int2* pairs;
+  // narrowphase
+  int* ncnt;  // [numPairs + numReal] contact counts (pairs, then planes per real)
+  int* noff;  // exclusive scan of ncnt
+  int2* cAB;
+  double4* cN;
+  double4* cRA;
+  double4* cRB;
+  int* cSlot;
+  // manifolds
+  int* mCnt;
+  int* mStart;
+  int2* mAB;
+  int2* mReal;
+  int2* mRange;
+  double4* mNs;
+  double4* mTA;
+  double4* mTB;
+  double4* mRA;
+  double4* mRB;
+  int* mNp;
+  unsigned long long* mKey;
+  unsigned char* mCode;
+  // colouring
+  int* mColor;
+  int* cColor;
+  unsigned long long* winner;
+  unsigned long long* mask;
+  int* vPerm;
+  int* vRowOf;
+  int* pPerm;
+  int* pRowOf;
+  // velocity rows
+  int2* vAB;
+  double4* vN;
+  double4* vTA;
+  double4* vTB;
+  double4* vRA;
+  double4* vRB;
+  double4* vVt0;
+  double4* vLamT;
+  double* vVn0;
+  double* vLam;
+  unsigned char* vFlag;
+  // position rows
+  int2* pAB;
+  double4* pN;
+  double4* pRA;
+  double4* pRB;
+  double* pLam;
+  // warm-start ledger (sorted by key)
+  unsigned long long* lKey;
+  double* lLam;
+  double4* lLamT;  // (lambdaT.xyz, position-channel impulse)
+  int* lIdx;
+  unsigned long long* uKey;
+  int* bucketCnt;
+  int* bucketStart;
+  // scalars
+  unsigned long long* red;  // reduction scratch (ordered keys)
+  int* counters;            // [0] remaining JP edges, [1] uncoloured entering, [2..] misc
+  int* colCnt;              // [64]
+  int* colFill;             // [64]
+  int* colStart;            // [65]
+  unsigned long long* maxOverlapBits;
+  unsigned long long* maxApproachBits;
+  double* thermo;           // partial sums
+  // Jacobi fallback accumulators (per slot)
+  double4* dVel;
+  double4* dAng;
+  double4* dPos;
+  int* dCnt;
+};
+
+// velocity-row flag bits
+constexpr unsigned char F_PERSIST = 1, F_SIDE1 = 2, F_SIDE2 = 4, F_UP = 8, F_DOWN = 16;
+
+__device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ void atomicMaxD(unsigned long long* a, double v) {
+  // non-negative doubles order like their bit patterns
+  atomicMax(a, dbits(v));
+}
+
+// ============================================================================================
+// generic helpers: fills, exclusive scan
+// ============================================================================================
+__global__ void k_fill_i32(int* a, int v, int n) {
+  const int i = gtid();
+  if (i < n) a[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* a, unsigned long long v, int n) {
+  const int i = gtid();
+  if (i < n) a[i] = v;
+}
+
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = TPB * SCAN_ITEMS;
+
+__device__ int blockExclusiveScan(int v, int* sh, int* total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < TPB; off <<= 1) {
+    const int x = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  const int incl = sh[t];
+  *total = sh[TPB - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ void k_scan_reduce(const int* in, int n, int* blockSums) {
+  __shared__ int sh[TPB];
+  const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int s = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k)
+    if (base + k < n) s += in[base + k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = TPB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blockSums[blockIdx.x] = sh[0];
+}
+// single block: exclusive scan of the block sums in place, total -> *total
+__global__ void k_scan_top(int* blockSums, int nb, int* total) {
+  __shared__ int sh[TPB];
+  int carry = 0;
+  for (int base = 0; base < nb; base += TPB) {
+    const int i = base + threadIdx.x;
+    const int v = (i < nb) ? blockSums[i] : 0;
+    int tot;
+    const int ex = blockExclusiveScan(v, sh, &tot);
+    if (i < nb) blockSums[i] = ex + carry;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+__global__ void k_scan_down(const int* in, int n, const int* blockSums, int* out) {
+  __shared__ int sh[TPB];
+  const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int vals[SCAN_ITEMS];
+  int s = 0;
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    vals[k] = (base + k < n) ? in[base + k] : 0;
+    s += vals[k];
+  }
+  int tot;
+  int ex = blockExclusiveScan(s, sh, &tot) + blockSums[blockIdx.x];
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = ex;
+    ex += vals[k];
+  }
+}
+
+// ============================================================================================
+// integration — src/integration.hpp
+// ============================================================================================
+// maxOwnedRadius reduction, src/sim.hpp:61-70 (ordered-key max, NaN skipped)
+__global__ void k_max_scale(const double* scale, int n, unsigned long long* out) {
+  __shared__ unsigned long long sh[TPB];
+  const int i = gtid();
+  unsigned long long k = 0;
+  if (i < n) {
+    const double s = scale[i];
+    if (s == s) k = ordKey(s);
+  }
+  sh[threadIdx.x] = k;
+  __syncthreads();
+  for (int off = TPB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off && sh[threadIdx.x + off] > sh[threadIdx.x])
+      sh[threadIdx.x] = sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sh[0]) atomicMax(out, sh[0]);
+}
+
+// updateGrowthScalesKokkos, src/integration.hpp:275-283
+__global__ void k_growth(double* scale, const double* target, double factor, int n) {
+  const int i = gtid();
+  if (i < n) scale[i] = target[i] * factor;
+}
+
+// predictVelocityKokkos, src/integration.hpp:44-93 (+ realIndices and broadphase radius fill)
+__global__ void k_predict(Dev D, Params P) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  const BodyC b = D.bc[i];
+  const Body64 v0 = D.vw0[i];
+  const double invM = b.iim.w;
+  const double dt = P.dt;
+  d3 v = xyz(v0.v);
+  if (invM > 0.0) {
+    v = add3(v, scale3(P.g, dt));
+    v = add3(v, scale3(xyz(D.extForce[i]), invM * dt));
+  }
+  d3 wpred = xyz(v0.w);
+  if (invM > 0.0) {
+    const d3 invI = xyz(b.iim);
+    if (invI.x > 0.0 || invI.y > 0.0 || invI.z > 0.0) {
+      const q4 q = toq(b.q);
+      d3 wb = invRotate(q, xyz(v0.w));
+      const d3 Ib{(invI.x > 1e-9) ? 1.0 / invI.x : 0.0, (invI.y > 1e-9) ? 1.0 / invI.y : 0.0,
+                  (invI.z > 1e-9) ? 1.0 / invI.z : 0.0};
+      const d3 Lb{Ib.x * wb.x, Ib.y * wb.y, Ib.z * wb.z};
+      const d3 wxL = cross3(wb, Lb);
+      const d3 alpha{-invI.x * wxL.x, -invI.y * wxL.y, -invI.z * wxL.z};
+      wb = add3(wb, scale3(alpha, dt));
+      wpred = rotate(q, wb);
+    }
+  }
+  Body64 out;
+  out.v = tod4(v, 0.0);
+  out.w = tod4(wpred, 0.0);
+  D.vw[i] = out;
+  d3 x = xyz(D.pos[i]);
+  if (invM > 0.0) x = add3(x, scale3(v, dt));
+  const double rad = D.scale[i] * P.gs * P.baseRadius;
+  D.posPred[i] = tod4(x, rad);
+  D.quatPred[i] = b.q;
+  D.realIdx[i] = i;
+}
+
+// computeOverlapsKokkos preparation, src/sim.hpp:494-510: predicted := committed
+__global__ void k_overlap_prep(Dev D, Params P) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  const double rad = D.scale[i] * P.gs * P.baseRadius;
+  D.posPred[i] = tod4(xyz(D.pos[i]), rad);
+  D.quatPred[i] = D.bc[i].q;
+  D.vw[i] = D.vw0[i];
+  D.realIdx[i] = i;
+}
+
+__device__ __forceinline__ void ghostShifts(const Params& P, d3 p, int& sx, int& sy, int& sz) {
+  const double skin = P.band;
+  sx = P.px ? ((p.x < P.dmin.x + skin) ? 1 : ((p.x > P.dmax.x - skin) ? -1 : 0)) : 0;
+  sy = P.py ? ((p.y < P.dmin.y + skin) ? 1 : ((p.y > P.dmax.y - skin) ? -1 : 0)) : 0;
+  sz = P.pz ? ((p.z < P.dmin.z + skin) ? 1 : ((p.z > P.dmax.z - skin) ? -1 : 0)) : 0;
+}
+// generateGhostsKokkos, src/periodicity.hpp:43-94 — pass 1: ghost count per real
+__global__ void k_ghost_count(Dev D, Params P, int* cnt) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  int sx, sy, sz;
+  ghostShifts(P, xyz(D.pos[i]), sx, sy, sz);
+  cnt[i] = (sx ? 2 : 1) * (sy ? 2 : 1) * (sz ? 2 : 1) - 1;
+}
+// pass 2: emit ghosts at numReal + off[i] in the (a,b,c) loop order
+__global__ void k_ghost_emit(Dev D, Params P, const int* off) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  const d3 p = xyz(D.pos[i]);
+  int sx, sy, sz;
+  ghostShifts(P, p, sx, sy, sz);
+  if (!sx && !sy && !sz) return;
+  const int ax[2] = {0, sx}, ay[2] = {0, sy}, az[2] = {0, sz};
+  const int nx = sx == 0 ? 1 : 2, ny = sy == 0 ? 1 : 2, nz = sz == 0 ? 1 : 2;
+  const double4 pp = D.posPred[i];
+  const d3 pPred = xyz(pp);
+  const BodyC b = D.bc[i];
+  BodyC bg;
+  bg.q = b.q;
+  bg.iim = make_double4(0.0, 0.0, 0.0, b.iim.w);
+  const Body64 v0 = D.vw0[i], v1 = D.vw[i];
+  const double4 qp = D.quatPred[i];
+  const double sc = D.scale[i];
+  const unsigned char mat = D.matId[i];
+  int slot = P.numReal + off[i];
+  for (int a = 0; a < nx; ++a)
+    for (int bb = 0; bb < ny; ++bb)
+      for (int c = 0; c < nz; ++c) {
+        const int ix = ax[a], iy = ay[bb], iz = az[c];
+        if (ix == 0 && iy == 0 && iz == 0) continue;
+        const d3 shift{ix * P.dsize.x, iy * P.dsize.y, iz * P.dsize.z};
+        D.pos[slot] = tod4(add3(p, shift), 0.0);
+        D.posPred[slot] = tod4(add3(pPred, shift), pp.w);
+        D.bc[slot] = bg;
+        D.vw0[slot] = v0;
+        D.vw[slot] = v1;
+        D.quatPred[slot] = qp;
+        D.scale[slot] = sc;
+        D.matId[slot] = mat;
+        D.realIdx[slot] = i;
+        ++slot;
+      }
+}
+
+// applyVelocityAndPredictPositionKokkos, src/integration.hpp:112-148 (reals and ghosts)
+__global__ void k_reintegrate(Dev D, Params P) {
+  const int i = gtid();
+  if (i >= P.numParticles) return;
+  const double dt = P.dt;
+  const Body64 vs = D.vw0[i];
+  const Body64 vf = D.vw[i];
+  const d3 vStart = xyz(vs.v);
+  const d3 vFinal = xyz(vf.v);
+  D.vw0[i] = vf;  // persist v_{n+1} and omega
+  const BodyC b = D.bc[i];
+  const double invM = b.iim.w;
+  d3 x = xyz(D.pos[i]);
+  if (invM > 0.0) x = add3(x, scale3(add3(vStart, vFinal), 0.5 * dt));
+  D.posPred[i].x = x.x;
+  D.posPred[i].y = x.y;
+  D.posPred[i].z = x.z;
+  const q4 q = toq(b.q);
+  const d3 omega = xyz(vf.w);
+  q4 qp = q;
+  if (invM > 0.0) {
+    qp.x += 0.5 * dt * (omega.x * q.w + omega.y * q.z - omega.z * q.y);
+    qp.y += 0.5 * dt * (omega.y * q.w + omega.z * q.x - omega.x * q.z);
+    qp.z += 0.5 * dt * (omega.z * q.w + omega.x * q.y - omega.y * q.x);
+    qp.w += 0.5 * dt * (-omega.x * q.x - omega.y * q.y - omega.z * q.z);
+    const double len = sqrt(qp.x * qp.x + qp.y * qp.y + qp.z * qp.z + qp.w * qp.w);
+    if (len > 1e-9) {
+      const double s = 1.0 / len;
+      qp = q4{qp.x * s, qp.y * s, qp.z * s, qp.w * s};
+    }
+  }
+  D.quatPred[i] = tod4(qp);
+}
+
+// finalCommitKokkos, src/integration.hpp:187-215
+__global__ void k_final_commit(Dev D, Params P) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  d3 p = xyz(D.posPred[i]);
+  if (P.px) {
+    if (p.x < P.dmin.x) p.x += P.dsize.x;
+    else if (p.x >= P.dmax.x) p.x -= P.dsize.x;
+  }
+  if (P.py) {
+    if (p.y < P.dmin.y) p.y += P.dsize.y;
+    else if (p.y >= P.dmax.y) p.y -= P.dsize.y;
+  }
+  if (P.pz) {
+    if (p.z < P.dmin.z) p.z += P.dsize.z;
+    else if (p.z >= P.dmax.z) p.z -= P.dsize.z;
+  }
+  D.pos[i] = tod4(p, 0.0);
+  D.bc[i].q = D.quatPred[i];
+}
+
+// applyThermostatKokkos, src/integration.hpp:229-271 — per-block partial energies (fixed order)
+__global__ void k_thermo_partial(Dev D, Params P, double* part) {
+  __shared__ double st[TPB], sr[TPB];
+  const int i = gtid();
+  double kt = 0.0, kr = 0.0;
+  if (i < P.numReal) {
+    const BodyC b = D.bc[i];
+    const Body64 v0 = D.vw0[i];
+    const double invM = b.iim.w;
+    const d3 v = xyz(v0.v);
+    if (invM > 0.0) kt = 0.5 * (1.0 / invM) * (v.x * v.x + v.y * v.y + v.z * v.z);
+    const d3 invI = xyz(b.iim);
+    if (invI.x > 0.0 || invI.y > 0.0 || invI.z > 0.0) {
+      const d3 wb = invRotate(toq(b.q), xyz(v0.w));
+      const double Ix = (invI.x > 0.0) ? 1.0 / invI.x : 0.0;
+      const double Iy = (invI.y > 0.0) ? 1.0 / invI.y : 0.0;
+      const double Iz = (invI.z > 0.0) ? 1.0 / invI.z : 0.0;
+      kr = 0.5 * (Ix * wb.x * wb.x + Iy * wb.y * wb.y + Iz * wb.z * wb.z);
+    }
+  }
+  st[threadIdx.x] = kt;
+  sr[threadIdx.x] = kr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    for (int t = 0; t < TPB; ++t) {
+      a += st[t];
+      c += sr[t];
+    }
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = c;
+  }
+}
+__global__ void k_thermo_scale(Dev D, Params P, double lt, double lr) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  Body64 v = D.vw0[i];
+  v.v = tod4(scale3(xyz(v.v), lt), 0.0);
+  v.w = tod4(scale3(xyz(v.w), lr), 0.0);
+  D.vw0[i] = v;
+}
+
+// ============================================================================================
+// broadphase — src/broadphase_arborx.hpp:36-107, uniform grid, pairs (i<j) lexicographic
+// ============================================================================================
+// PECLET_DEM_SANE_BOX, src/broadphase_arborx.hpp:41-52
+__device__ __forceinline__ double4 saneBox(double4 pp, double margin, double boxCap) {
+  double x = pp.x, y = pp.y, z = pp.z, b = pp.w + margin;
+  if (!(b == b) || !(x == x) || !(y == y) || !(z == z) || (boxCap > 0.0 && b > boxCap)) {
+    x = y = z = -1.0e30;
+    b = 0.0;
+  }
+  return make_double4(x, y, z, b);
+}
+__device__ __forceinline__ bool boxOverlap(const double4& a, const double4& c) {
+  if (a.x - a.w > c.x + c.w || a.x + a.w < c.x - c.w) return false;
+  if (a.y - a.w > c.y + c.w || a.y + a.w < c.y - c.w) return false;
+  if (a.z - a.w > c.z + c.w || a.z + a.w < c.z - c.w) return false;
+  return true;
+}
+
+// bounds of the finite, non-parked box centres and the largest half-width
+// red[0..2] = ordered min, red[3..5] = ordered max, red[6] = ordered max half-width
+__global__ void k_bp_bounds(Dev D, Params P, unsigned long long* red) {
+  __shared__ unsigned long long sh[7][TPB];
+  const int s = gtid();
+  unsigned long long mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0, 0, 0}, mb = 0;
+  if (s < P.numParticles) {
+    const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
+    if (isfinite(B.w)) mb = ordKey(B.w);
+    if (B.x != -1.0e30) {
+      const double c[3] = {B.x, B.y, B.z};
+      for (int d = 0; d < 3; ++d)
+        if (isfinite(c[d])) {
+          mn[d] = ordKey(c[d]);
+          mx[d] = mn[d];
+        }
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    sh[d][threadIdx.x] = mn[d];
+    sh[3 + d][threadIdx.x] = mx[d];
+  }
+  sh[6][threadIdx.x] = mb;
+  __syncthreads();
+  for (int off = TPB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      for (int d = 0; d < 3; ++d) {
+        if (sh[d][threadIdx.x + off] < sh[d][threadIdx.x]) sh[d][threadIdx.x] = sh[d][threadIdx.x + off];
+        if (sh[3 + d][threadIdx.x + off] > sh[3 + d][threadIdx.x])
+          sh[3 + d][threadIdx.x] = sh[3 + d][threadIdx.x + off];
+      }
+      if (sh[6][threadIdx.x + off] > sh[6][threadIdx.x]) sh[6][threadIdx.x] = sh[6][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&red[d], sh[d][0]);
+      atomicMax(&red[3 + d], sh[3 + d][0]);
+    }
+    atomicMax(&red[6], sh[6][0]);
+  }
+}
+
+__device__ __forceinline__ int cellCoord(double c, double lo, double h, int dim) {
+  const double f = (c - lo) / h;
+  if (!(f >= 0.0)) return 0;
+  if (f >= (double)(dim - 1)) return dim - 1;
+  return (int)f;
+}
+__global__ void k_cell_count(Dev D, Params P, Grid G) {
+  const int s = gtid();
+  if (s >= P.numParticles) return;
+  const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
+  const int cx = cellCoord(B.x, G.lo.x, G.h, G.dim[0]);
+  const int cy = cellCoord(B.y, G.lo.y, G.h, G.dim[1]);
+  const int cz = cellCoord(B.z, G.lo.z, G.h, G.dim[2]);
+  const int c = (cz * G.dim[1] + cy) * G.dim[0] + cx;
+  D.cellOf[s] = c;
+  atomicAdd(&D.cellCnt[c], 1);
+}
+// cellCnt is reused as the fill cursor (zeroed by the host before this kernel)
+__global__ void k_cell_fill(Dev D, Params P) {
+  const int s = gtid();
+  if (s >= P.numParticles) return;
+  const int c = D.cellOf[s];
+  const int k = D.cellStart[c] + atomicAdd(&D.cellCnt[c], 1);
+  D.order[k] = s;
+  D.sbox[k] = saneBox(D.posPred[s], P.margin, P.boxCap);
+}
+template <bool WRITE>
+__global__ void k_bp_query(Dev D, Params P, Grid G) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  const double4 Bi = saneBox(D.posPred[i], P.margin, P.boxCap);
+  const int cx = cellCoord(Bi.x, G.lo.x, G.h, G.dim[0]);
+  const int cy = cellCoord(Bi.y, G.lo.y, G.h, G.dim[1]);
+  const int cz = cellCoord(Bi.z, G.lo.z, G.h, G.dim[2]);
+  int n = 0;
+  const int base = WRITE ? D.pairStart[i] : 0;
+  for (int z = cz - 1; z <= cz + 1; ++z) {
+    if (z < 0 || z >= G.dim[2]) continue;
+    for (int y = cy - 1; y <= cy + 1; ++y) {
+      if (y < 0 || y >= G.dim[1]) continue;
+      const int row = (z * G.dim[1] + y) * G.dim[0];
+      const int x0 = cx > 0 ? cx - 1 : 0;
+      const int x1 = cx + 1 < G.dim[0] ? cx + 1 : G.dim[0] - 1;
+      // cells x0..x1 of one row are contiguous in the cell-sorted order
+      const int k0 = D.cellStart[row + x0], k1 = D.cellStart[row + x1 + 1];
+      for (int k = k0; k < k1; ++k) {
+        const int j = D.order[k];
+        if (j > i && boxOverlap(Bi, D.sbox[k])) {
+          if (WRITE) D.pairs[base + n] = make_int2(i, j);
+          ++n;
+        }
+      }
+    }
+  }
+  if (!WRITE) {
+    D.pairCnt[i] = n;
+    return;
+  }
+  // sort this real's partners ascending (segments are short)
+  int2* seg = D.pairs + base;
+  for (int a = 1; a < n; ++a) {
+    const int2 x = seg[a];
+    int b = a - 1;
+    while (b >= 0 && seg[b].y > x.y) {
+      seg[b + 1] = seg[b];
+      --b;
+    }
+    seg[b + 1] = x;
+  }
+}
+
+// ============================================================================================
+// narrowphase — src/narrowphase.hpp:179-268 (pairs) and :371-449 (planes)
+// ============================================================================================
+__device__ __forceinline__ void writeContact(Dev& D, int slot, int a, int b, d3 n, d3 rA, d3 rB,
+                                             double dist, double eb, double mub) {
+  D.cAB[slot] = make_int2(a, b);
+  D.cN[slot] = tod4(n, dist);
+  D.cRA[slot] = tod4(rA, eb);
+  D.cRB[slot] = tod4(rB, mub);
+}
+
+template <bool WRITE>
+__global__ void k_np_pairs(Dev D, Params P, int numPairs) {
+  const int p = gtid();
+  if (p >= numPairs) return;
+  const int2 ab = D.pairs[p];
+  const int idA = ab.x, idB = ab.y;
+  const d3 posA = xyz(D.posPred[idA]), posB = xyz(D.posPred[idB]);
+  const q4 qA = toq(D.quatPred[idA]), qB = toq(D.quatPred[idB]);
+  const double effScaleA = D.scale[idA] * P.gs, effScaleB = D.scale[idB] * P.gs;
+  const int countA = P.shape.numShell;
+  const bool sphereA = (P.shape.type == SHAPE_SPHERE);
+  const int iter = countA > 0 ? countA : 1;
+  double eb = -1.0, mub = -1.0;
+  if (WRITE && P.hasPairTable) {
+    const int t = ((int)D.matId[idA] * 8 + (int)D.matId[idB]) * 2;
+    eb = D.pairTable[t];
+    mub = D.pairTable[t + 1];
+  }
+  int n = 0;
+  const int base = WRITE ? D.noff[p] : 0;
+  for (int k = 0; k < iter; ++k) {
+    d3 pLocalA{0.0, 0.0, 0.0};
+    double pointRadius = 0.0;
+    if (countA > 0) pLocalA = xyz(D.shell[k]);
+    else if (sphereA) pointRadius = P.shape.p0 * effScaleA;
+    const d3 pWorld = add3(posA, rotate(qA, scale3(pLocalA, effScaleA)));
+    const d3 pLocalB = invRotate(qB, sub3(pWorld, posB));
+    const d3 pCanB = scale3(pLocalB, 1.0 / effScaleB);
+    const double dist = sdfShape(P.shape, pCanB) * effScaleB;
+    const double effDist = dist - pointRadius;
+    if (effDist >= P.margin) continue;
+    if (!WRITE) {
+      ++n;
+      continue;
+    }
+    if (effDist < 0.0) atomicMaxD(D.maxOverlapBits, -effDist);
+    const double eps = 1e-4;
+    d3 nLoc{sdfShape(P.shape, d3{pCanB.x + eps, pCanB.y, pCanB.z}) -
+                sdfShape(P.shape, d3{pCanB.x - eps, pCanB.y, pCanB.z}),
+            sdfShape(P.shape, d3{pCanB.x, pCanB.y + eps, pCanB.z}) -
+                sdfShape(P.shape, d3{pCanB.x, pCanB.y - eps, pCanB.z}),
+            sdfShape(P.shape, d3{pCanB.x, pCanB.y, pCanB.z + eps}) -
+                sdfShape(P.shape, d3{pCanB.x, pCanB.y, pCanB.z - eps})};
+    const double len = len3(nLoc);
+    nLoc = (len > 1e-9) ? scale3(nLoc, 1.0 / len) : d3{0.0, 1.0, 0.0};
+    const d3 nWorld = rotate(qB, nLoc);
+    const d3 pSurfA = sub3(pWorld, scale3(nWorld, pointRadius));
+    const d3 rA = sub3(pSurfA, posA);
+    const d3 rB = sub3(sub3(pSurfA, scale3(nWorld, effDist)), posB);
+    writeContact(D, base + n, idA, idB, nWorld, rA, rB, effDist, eb, mub);
+    ++n;
+  }
+  if (!WRITE) D.ncnt[p] = n;
+}
+
+template <bool WRITE>
+__global__ void k_np_planes(Dev D, Params P, int numPairs) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  const d3 posA = xyz(D.posPred[i]);
+  const double s = D.scale[i] * P.gs;
+  double baseR = P.shape.p0;
+  if (baseR == 0.0) baseR = 1.0;
+  const double radius = baseR * s;
+  const int numPts = P.shape.numShell;
+  const q4 qA = toq(D.quatPred[i]);
+  int n = 0;
+  const int base = WRITE ? D.noff[numPairs + i] : 0;
+  for (int pi = 0; pi < P.numPlanes; ++pi) {
+    const d3 pp = xyz(P.planeP[pi]), pn = xyz(P.planeN[pi]);
+    if (numPts > 0) {
+      for (int k = 0; k < numPts; ++k) {
+        const d3 rA = rotate(qA, scale3(xyz(D.shell[k]), s));
+        const d3 pwk = add3(posA, rA);
+        const double dist = dot3(sub3(pwk, pp), pn);
+        if (dist >= P.margin) continue;
+        if (WRITE) {
+          if (dist < 0.0) atomicMaxD(D.maxOverlapBits, -dist);
+          writeContact(D, base + n, i, -1, pn, rA, pp, dist, -1.0, -1.0);
+        }
+        ++n;
+      }
+    } else {
+      const double dist = dot3(sub3(posA, pp), pn) - radius;
+      if (dist >= P.margin) continue;
+      if (WRITE) {
+        if (dist < 0.0) atomicMaxD(D.maxOverlapBits, -dist);
+        writeContact(D, base + n, i, -1, pn, scale3(pn, -radius), pp, dist, -1.0, -1.0);
+      }
+      ++n;
+    }
+  }
+  if (!WRITE) D.ncnt[numPairs + i] = n;
+}
+
+// ============================================================================================
+// contact -> manifold reduction — src/contact_preprocessing.hpp:320-471
+// ============================================================================================
+// manifolds per real body: its pairs with >= 1 contact, then one boundary manifold
+__global__ void k_mcount(Dev D, Params P, int numPairs) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  int m = 0;
+  for (int p = D.pairStart[i]; p < D.pairStart[i + 1]; ++p) m += D.ncnt[p] > 0;
+  m += D.ncnt[numPairs + i] > 0;
+  D.mCnt[i] = m;
+}
+__global__ void k_mbuild(Dev D, Params P, int numPairs) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  int idx = D.mStart[i];
+  for (int p = D.pairStart[i]; p < D.pairStart[i + 1]; ++p) {
+    const int c = D.ncnt[p];
+    if (c > 0) {
+      D.mAB[idx] = D.pairs[p];
+      D.mRange[idx] = make_int2(D.noff[p], c);
+      ++idx;
+    }
+  }
+  const int cb = D.ncnt[numPairs + i];
+  if (cb > 0) {
+    D.mAB[idx] = make_int2(i, -1);
+    D.mRange[idx] = make_int2(D.noff[numPairs + i], cb);
+  }
+}
+// transformContact + ordered segmented sum, src/contact_preprocessing.hpp:339-466
+__global__ void k_mreduce(Dev D, int numManifolds) {
+  const int m = gtid();
+  if (m >= numManifolds) return;
+  const int2 ab = D.mAB[m];
+  const int2 rg = D.mRange[m];
+  d3 ns{0, 0, 0}, ta{0, 0, 0}, tb{0, 0, 0}, ra{0, 0, 0}, rb{0, 0, 0};
+  double es = 0.0, fs = 0.0;
+  int np = 0;
+  for (int c = rg.x; c < rg.x + rg.y; ++c) {
+    D.cSlot[c] = m;
+    const double4 cn = D.cN[c];
+    // an inactive contact (dist > 0) contributes a zero manifold with num_points 0
+    d3 na{0, 0, 0}, tca{0, 0, 0}, tcb{0, 0, 0}, rcan{0, 0, 0}, roth{0, 0, 0};
+    double e = 0.0, f = 0.0;
+    int pts = 0;
+    if (!(cn.w > 0.0)) {
+      const int2 cab = D.cAB[c];
+      const double4 cra = D.cRA[c], crb = D.cRB[c];
+      const bool flip = (cab.y >= 0 && cab.y < cab.x);
+      const d3 nvec = xyz(cn);
+      na = flip ? nvec : neg3(nvec);
+      const d3 shift{cn.x * cn.w * 0.5, cn.y * cn.w * 0.5, cn.z * cn.w * 0.5};
+      const d3 rAmid = sub3(xyz(cra), shift);
+      const d3 rBmid = add3(xyz(crb), shift);
+      rcan = flip ? rBmid : rAmid;
+      roth = flip ? rAmid : rBmid;
+      tca = cross3(rcan, na);
+      tcb = cross3(roth, neg3(na));
+      e = cra.w;
+      f = crb.w;
+      pts = 1;
+    }
+    ns = add3(ns, na);
+    ta = add3(ta, tca);
+    tb = add3(tb, tcb);
+    ra = add3(ra, rcan);
+    rb = add3(rb, roth);
+    es += e;
+    fs += f;
+    np += pts;
+  }
+  D.mNs[m] = tod4(ns, es);
+  D.mTA[m] = tod4(ta, fs);
+  D.mTB[m] = tod4(tb, 0.0);
+  D.mRA[m] = tod4(ra, 0.0);
+  D.mRB[m] = tod4(rb, 0.0);
+  D.mNp[m] = np;
+  const int realA = D.realIdx[ab.x];
+  const int realB = ab.y >= 0 ? D.realIdx[ab.y] : -1;
+  D.mReal[m] = make_int2(realA, realB);
+}
+
+// ============================================================================================
+// Jones-Plassmann colouring — src/solver_velocity.hpp:277-374, src/solver_position.hpp:155-241
+// ============================================================================================
+__global__ void k_jp_init_manifolds(Dev D, int numManifolds, int* uncol) {
+  const int m = gtid();
+  if (m >= numManifolds) return;
+  const int2 r = D.mReal[m];
+  int c = -1;
+  if (D.mNp[m] <= 0) c = -2;
+  else if (D.mAB[m].y >= 0 && r.x > r.y) c = -2;
+  D.mColor[m] = c;
+  if (c == -1) atomicAdd(uncol, 1);
+}
+__global__ void k_jp_contend(const int2* ends, const int* colour, int numEdges,
+                             unsigned long long* winner) {
+  const int e = gtid();
+  if (e >= numEdges || colour[e] != -1) return;
+  const int2 ab = ends[e];
+  const unsigned long long key = colorKey1(e);
+  atomicMax(&winner[ab.x], key);
+  if (ab.y >= 0) atomicMax(&winner[ab.y], key);
+}
+__global__ void k_jp_commit(const int2* ends, int* colour, int numEdges,
+                            const unsigned long long* winner, unsigned long long* mask,
+                            int* remaining) {
+  const int e = gtid();
+  if (e >= numEdges || colour[e] != -1) return;
+  const int2 ab = ends[e];
+  const unsigned long long key = colorKey1(e);
+  if (winner[ab.x] != key || (ab.y >= 0 && winner[ab.y] != key)) {
+    atomicAdd(remaining, 1);
+    return;
+  }
+  unsigned long long forbidden = mask[ab.x];
+  if (ab.y >= 0) forbidden |= mask[ab.y];
+  int c = 0;
+  while (c < 62 && (forbidden & (1ull << c))) ++c;
+  colour[e] = c;
+  mask[ab.x] |= 1ull << c;
+  if (ab.y >= 0) mask[ab.y] |= 1ull << c;
+}
+__global__ void k_color_hist(const int* colour, int numEdges, int* colCnt) {
+  const int e = gtid();
+  if (e >= numEdges) return;
+  const int c = colour[e];
+  if (c >= 0) atomicAdd(&colCnt[c], 1);
+}
+__global__ void k_color_perm(const int* colour, int numEdges, const int* colStart, int* colFill,
+                             int* perm, int* rowOf) {
+  const int e = gtid();
+  if (e >= numEdges) return;
+  const int c = colour[e];
+  if (c < 0) {
+    rowOf[e] = -1;
+    return;
+  }
+  const int k = colStart[c] + atomicAdd(&colFill[c], 1);
+  perm[k] = e;
+  rowOf[e] = k;
+}
+
+// ============================================================================================
+// velocity solve — src/solver_velocity.hpp, src/contact_preprocessing.hpp:70-253
+// ============================================================================================
+__device__ __forceinline__ int lowerBound(const unsigned long long* keys, int n,
+                                          unsigned long long k) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (keys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ unsigned long long realKey(int realA, int realB) {
+  const unsigned a = (unsigned)realA;
+  const unsigned b = realB >= 0 ? (unsigned)realB : 0xFFFFFFFFu;
+  const unsigned hi = a < b ? a : b, lo = a < b ? b : a;
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// per manifold: ledger key (markPersistentManifoldsKokkos semantics, duplicates keep their key)
+// and the support relation of updateGroundedLevelsKokkos, src/contact_preprocessing.hpp:78-253
+__global__ void k_mkeys(Dev D, int numManifolds, d3 gHat) {
+  const int m = gtid();
+  if (m >= numManifolds) return;
+  if (D.mNp[m] <= 0) {
+    D.mKey[m] = ~0ull;
+    D.mCode[m] = 0;
+    return;
+  }
+  const int2 ab = D.mAB[m];
+  const int2 r = D.mReal[m];
+  D.mKey[m] = realKey(r.x, r.y);
+  if (ab.y < 0) {
+    D.mCode[m] = 3;
+    return;
+  }
+  const d3 dx = sub3(xyz(D.posPred[ab.x]), xyz(D.posPred[ab.y]));
+  const double up = -(dx.x * gHat.x + dx.y * gHat.y + dx.z * gHat.z);
+  const double thr = 0.3 * sqrt(dot3(dx, dx));
+  D.mCode[m] = (up > thr) ? 1 : ((up < -thr) ? 2 : 0);
+}
+
+// velocity rows: gather the manifold (colour-sorted), warm start, persistence, vn0 / vt0
+// (gatherWarmLambdaKokkos :113, markPersistentManifoldsKokkos :78, computeVn0Kokkos :390)
+__global__ void k_vrows(Dev D, Params P, int numRows, int prevCount, int usePGS) {
+  const int k = gtid();
+  if (k >= numRows) return;
+  const int m = D.vPerm[k];
+  const int2 r = D.mReal[m];
+  const int realA = r.x, realB = r.y;
+  const int np = D.mNp[m];
+  const double invN = 1.0 / (double)np;
+  const double4 ns4 = D.mNs[m], ta4 = D.mTA[m], tb4 = D.mTB[m];
+  const d3 Nsum = xyz(ns4), TauA = xyz(ta4), TauB = xyz(tb4);
+  double restitution = P.restN;
+  {
+    const double ra = ns4.w * invN;
+    if (ra >= 0.0) restitution = ra;
+  }
+  double mu = P.mu;
+  {
+    const double fa = ta4.w * invN;
+    if (fa >= 0.0) mu = fa;
+  }
+  const d3 rAavg = scale3(xyz(D.mRA[m]), invN);
+  const d3 rBavg = scale3(xyz(D.mRB[m]), invN);
+  const double lenN = sqrt(dot3(Nsum, Nsum));
+  const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
+  const double alignment = dot3(Nsum, diffCenters);
+  const double sgn = (alignment > 0.0) ? 1.0 : -1.0;
+  D.vAB[k] = r;
+  D.vN[k] = tod4(Nsum, restitution);
+  D.vTA[k] = tod4(TauA, mu);
+  D.vTB[k] = tod4(TauB, invN);
+  D.vRA[k] = tod4(rAavg, lenN);
+  D.vRB[k] = tod4(rBavg, sgn);
+  unsigned char flag = 0;
+  const unsigned char code = D.mCode[m];
+  if (code == 1) flag |= F_UP;
+  if (code == 2) flag |= F_DOWN;
+  double lam = 0.0;
+  double4 lamT = make_double4(0, 0, 0, 0);
+  double pimp = 0.0;
+  if (usePGS) {
+    const unsigned long long key = realKey(realA, realB);
+    const int lo = lowerBound(D.lKey, prevCount, key);
+    const bool hit = (lo < prevCount && D.lKey[lo] == key);
+    if (hit) {
+      lam = D.lLam[lo];
+      const double4 lt = D.lLamT[lo];
+      lamT = make_double4(lt.x, lt.y, lt.z, 0.0);
+      pimp = lt.w;
+      flag |= F_PERSIST;
+    }
+  }
+  D.vLam[k] = lam;
+  D.vLamT[k] = lamT;
+  D.vFlag[k] = flag;
+  // computeVn0Kokkos, src/solver_velocity.hpp:390-447
+  const Body64 ba = D.vw[realA];
+  const d3 vA = xyz(ba.v), wA = xyz(ba.w);
+  d3 vB{0, 0, 0}, wB{0, 0, 0};
+  if (realB >= 0) {
+    const Body64 bb = D.vw[realB];
+    vB = xyz(bb.v);
+    wB = xyz(bb.w);
+  } else {
+    vB = scale3(d3{0.0, 0.0, 0.0}, invN);
+  }
+  const d3 vGrowth = scale3(diffCenters, P.growthRate);
+  double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
+  vn += dot3(vGrowth, Nsum);
+  D.vVn0[k] = vn;
+  d3 vt{0, 0, 0};
+  if (lenN > 1e-9) {
+    const d3 nhat = scale3(Nsum, 1.0 / lenN);
+    const d3 wBv = (realB >= 0) ? wB : d3{0.0, 0.0, 0.0};
+    const d3 vrel = sub3(add3(vA, cross3(wA, rAavg)), add3(vB, cross3(wBv, rBavg)));
+    vt = sub3(vrel, scale3(nhat, dot3(vrel, nhat)));
+  }
+  D.vVt0[k] = tod4(vt, pimp);
+}
+
+// Guendelman support levels, double-buffered sweeps (src/contact_preprocessing.hpp:213-253)
+__global__ void k_ground_decay(int* g, int n, int decay) {
+  const int i = gtid();
+  if (i >= n) return;
+  const int v = g[i] - decay;
+  g[i] = v > 0 ? v : 0;
+}
+__global__ void k_copy_i32(int* dst, const int* src, int n) {
+  const int i = gtid();
+  if (i < n) dst[i] = src[i];
+}
+__global__ void k_ground_sweep(Dev D, int numManifolds, const int* gOld, int* gNew) {
+  const int m = gtid();
+  if (m >= numManifolds) return;
+  const unsigned char code = D.mCode[m];
+  if (code == 0) return;
+  const int2 r = D.mReal[m];
+  if (code == 3) {
+    atomicMax(&gNew[r.x], 255);
+  } else if (code == 1) {
+    const int lvl = gOld[r.y] - 1;
+    if (lvl > 0) atomicMax(&gNew[r.x], lvl);
+  } else {
+    const int lvl = gOld[r.x] - 1;
+    if (lvl > 0) atomicMax(&gNew[r.y], lvl);
+  }
+}
+
+__device__ __forceinline__ void addVW(Body64& b, d3 dv, d3 dw) {
+  b.v.x += dv.x;
+  b.v.y += dv.y;
+  b.v.z += dv.z;
+  b.w.x += dw.x;
+  b.w.y += dw.y;
+  b.w.z += dw.z;
+}
+
+// warmStartApplyKokkos, src/solver_velocity.hpp:498-605 (one row)
+__device__ void warmApplyRow(Dev& D, int k) {
+  const double p = D.vLam[k];
+  const double4 lt4 = D.vLamT[k];
+  d3 lt = xyz(lt4);
+  const bool hasT = nonzero3(lt);
+  if (p == 0.0 && !hasT) return;
+  const int2 r = D.vAB[k];
+  const int realA = r.x, realB = r.y;
+  const d3 Nsum = xyz(D.vN[k]), TauA = xyz(D.vTA[k]), TauB = xyz(D.vTB[k]);
+  const d3 rAavg = xyz(D.vRA[k]);
+  const double4 rb4 = D.vRB[k];
+  const d3 rBavg = xyz(rb4);
+  const double sgn = rb4.w;
+  const double lambda = -sgn * p;
+  const d3 Jlin = scale3(Nsum, lambda);
+  const BodyC ca = D.bc[realA];
+  const double invMassA = ca.iim.w;
+  Body64 ba = D.vw[realA];
+  const q4 qA = toq(ca.q);
+  const d3 invIA = xyz(ca.iim);
+  ba.v.x += Jlin.x * invMassA;
+  ba.v.y += Jlin.y * invMassA;
+  ba.v.z += Jlin.z * invMassA;
+  {
+    const d3 dw = angDelta(qA, scale3(TauA, lambda), invIA);
+    ba.w.x += dw.x;
+    ba.w.y += dw.y;
+    ba.w.z += dw.z;
+  }
+  Body64 bb;
+  BodyC cb;
+  double invMassB = 0.0;
+  if (realB >= 0) {
+    cb = D.bc[realB];
+    invMassB = cb.iim.w;
+    bb = D.vw[realB];
+    bb.v.x += -Jlin.x * invMassB;
+    bb.v.y += -Jlin.y * invMassB;
+    bb.v.z += -Jlin.z * invMassB;
+    const d3 dw = angDelta(toq(cb.q), scale3(TauB, lambda), xyz(cb.iim));
+    bb.w.x += dw.x;
+    bb.w.y += dw.y;
+    bb.w.z += dw.z;
+  }
+  if (hasT) {
+    const double lenN2 = sqrt(dot3(Nsum, Nsum));
+    if (lenN2 > 1e-9) {
+      const d3 nhat = scale3(Nsum, 1.0 / lenN2);
+      lt = sub3(lt, scale3(nhat, dot3(lt, nhat)));
+    }
+    D.vLamT[k] = tod4(lt, 0.0);
+    ba.v.x += lt.x * invMassA;
+    ba.v.y += lt.y * invMassA;
+    ba.v.z += lt.z * invMassA;
+    {
+      const d3 dw = angDelta(qA, cross3(rAavg, lt), invIA);
+      ba.w.x += dw.x;
+      ba.w.y += dw.y;
+      ba.w.z += dw.z;
+    }
+    if (realB >= 0) {
+      bb.v.x += -lt.x * invMassB;
+      bb.v.y += -lt.y * invMassB;
+      bb.v.z += -lt.z * invMassB;
+      const d3 dw = angDelta(toq(cb.q), cross3(rBavg, scale3(lt, -1.0)), xyz(cb.iim));
+      bb.w.x += dw.x;
+      bb.w.y += dw.y;
+      bb.w.z += dw.z;
+    }
+  }
+  D.vw[realA] = ba;
+  if (realB >= 0) D.vw[realB] = bb;
+}
+
+// PGSManifoldSweep::solveOne, src/solver_velocity.hpp:641-858 (one row)
+__device__ void pgsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
+  const int2 r = D.vAB[k];
+  const int realA = r.x, realB = r.y;
+  const BodyC ca = D.bc[realA];
+  const double invMassA = ca.iim.w;
+  const d3 invIA = xyz(ca.iim);
+  const q4 qA = toq(ca.q);
+  BodyC cb;
+  double invMassB = 0.0;
+  d3 invIB{0, 0, 0};
+  q4 qB{0, 0, 0, 1};
+  Body64 ba = D.vw[realA];
+  Body64 bb;
+  d3 vB{0, 0, 0}, wB{0, 0, 0};
+  const double4 tb4 = D.vTB[k];
+  const double invN = tb4.w;
+  if (realB >= 0) {
+    cb = D.bc[realB];
+    invMassB = cb.iim.w;
+    invIB = xyz(cb.iim);
+    qB = toq(cb.q);
+    bb = D.vw[realB];
+    vB = xyz(bb.v);
+    wB = xyz(bb.w);
+  } else {
+    vB = scale3(d3{0.0, 0.0, 0.0}, invN);
+  }
+  const d3 vA = xyz(ba.v), wA = xyz(ba.w);
+  const double4 n4 = D.vN[k], ta4 = D.vTA[k], ra4 = D.vRA[k], rb4 = D.vRB[k];
+  double restitution = n4.w;
+  const d3 Nsum = xyz(n4), TauA = xyz(ta4), TauB = xyz(tb4);
+  const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
+  const double lenN = ra4.w;
+  if (lenN < 1e-9) return;
+  const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
+  const d3 vGrowth = scale3(diffCenters, P.growthRate);
+  double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
+  vn += dot3(vGrowth, Nsum);
+  const double sgn = rb4.w;
+  const double Nsq = dot3(Nsum, Nsum);
+  double wA_n = Nsq * invMassA + genInvMass(TauA, invIA, qA);
+  double wB_n = Nsq * invMassB + genInvMass(TauB, invIB, qB);
+  bool applyA = true, applyB = true;
+  const unsigned char fl = D.vFlag[k];
+  const int sf = (fl & F_SIDE1) ? 1 : ((fl & F_SIDE2) ? 2 : 0);
+  if (sf == 1) {
+    wB_n = 0.0;
+    applyB = false;
+    restitution = 0.0;
+  } else if (sf == 2) {
+    wA_n = 0.0;
+    applyA = false;
+    restitution = 0.0;
+  }
+  const double wTotal = wA_n + wB_n;
+  if (wTotal <= 0.0) return;
+  const double vn0 = D.vVn0[k];
+  const double v0til = sgn * vn0;
+  if (fabs(vn0) < restVelThreshold * lenN) restitution = 0.0;
+  const double target = (v0til > 0.0) ? -restitution * v0til : 0.0;
+  const double vtil = sgn * vn;
+  const double dp = (vtil - target) / wTotal;
+  const double pOld = D.vLam[k];
+  double pNew = pOld + dp;
+  if (pNew < 0.0) pNew = 0.0;
+  const double dApplied = pNew - pOld;
+  bool dirtyA = false, dirtyB = false;
+  if (dApplied != 0.0) {
+    D.vLam[k] = pNew;
+    atomicMaxD(D.maxApproachBits, fabs(dApplied) * wTotal / lenN);
+    const double lambda = -sgn * dApplied;
+    const d3 Jlin = scale3(Nsum, lambda);
+    if (applyA) {
+      ba.v.x += Jlin.x * invMassA;
+      ba.v.y += Jlin.y * invMassA;
+      ba.v.z += Jlin.z * invMassA;
+      const d3 dw = angDelta(qA, scale3(TauA, lambda), invIA);
+      ba.w.x += dw.x;
+      ba.w.y += dw.y;
+      ba.w.z += dw.z;
+      dirtyA = true;
+    }
+    if (realB >= 0 && applyB) {
+      bb.v.x += -Jlin.x * invMassB;
+      bb.v.y += -Jlin.y * invMassB;
+      bb.v.z += -Jlin.z * invMassB;
+      const d3 dw = angDelta(qB, scale3(TauB, lambda), invIB);
+      bb.w.x += dw.x;
+      bb.w.y += dw.y;
+      bb.w.z += dw.z;
+      dirtyB = true;
+    }
+  }
+  // friction cone, src/solver_velocity.hpp:760-855
+  double mu = ta4.w;
+  const double4 lto4 = D.vLamT[k];
+  const d3 ltOld = xyz(lto4);
+  const bool haveOld = nonzero3(ltOld);
+  if (mu > 0.0 || haveOld) {
+    const d3 nhat = scale3(Nsum, 1.0 / lenN);
+    const d3 vA2 = xyz(ba.v), wA2 = xyz(ba.w);
+    d3 vB2{0, 0, 0}, wB2{0, 0, 0};
+    if (realB >= 0) {
+      vB2 = xyz(bb.v);
+      wB2 = xyz(bb.w);
+    } else {
+      vB2 = scale3(d3{0.0, 0.0, 0.0}, invN);
+    }
+    const d3 vrel = sub3(add3(vA2, cross3(wA2, rAavg)), add3(vB2, cross3(wB2, rBavg)));
+    const d3 vt = sub3(vrel, scale3(nhat, dot3(vrel, nhat)));
+    double beta = P.restT;
+    if (beta != 0.0 && (sf != 0 || fabs(vn0) < restVelThreshold * lenN)) beta = 0.0;
+    d3 vtErr = vt;
+    const double4 vt04 = D.vVt0[k];
+    if (beta != 0.0) {
+      d3 v0 = xyz(vt04);
+      v0 = sub3(v0, scale3(nhat, dot3(v0, nhat)));
+      vtErr = add3(vt, scale3(v0, beta));
+    }
+    const double vtLen = sqrt(dot3(vtErr, vtErr));
+    d3 ltNew = ltOld;
+    double wT = invMassA + invMassB;
+    if (vtLen > 1e-9) {
+      const d3 that = scale3(vtErr, 1.0 / vtLen);
+      const double wAt = (sf == 2) ? 0.0 : invMassA + genInvMass(cross3(rAavg, that), invIA, qA);
+      const double wBt =
+          (sf == 1 || realB < 0) ? 0.0 : invMassB + genInvMass(cross3(rBavg, that), invIB, qB);
+      wT = wAt + wBt;
+      if (wT > 1e-9) ltNew = add3(ltOld, scale3(that, -vtLen / wT));
+    }
+    ltNew = sub3(ltNew, scale3(nhat, dot3(ltNew, nhat)));
+    double nTot = fmax(D.vLam[k], 0.0) * lenN;
+    if (fabs(vn0) < restVelThreshold * lenN) nTot += fmax(vt04.w, 0.0);
+    const double bound = mu * nTot;
+    const double ltLen = sqrt(dot3(ltNew, ltNew));
+    if (ltLen > bound) ltNew = (bound > 0.0) ? scale3(ltNew, bound / ltLen) : d3{0.0, 0.0, 0.0};
+    const d3 dApp = sub3(ltNew, ltOld);
+    if (nonzero3(dApp)) {
+      D.vLamT[k] = tod4(ltNew, 0.0);
+      const double dLen = sqrt(dot3(dApp, dApp));
+      if (wT > 1e-9) atomicMaxD(D.maxApproachBits, dLen * wT);
+      if (applyA) {
+        ba.v.x += dApp.x * invMassA;
+        ba.v.y += dApp.y * invMassA;
+        ba.v.z += dApp.z * invMassA;
+        const d3 dw = angDelta(qA, cross3(rAavg, dApp), invIA);
+        ba.w.x += dw.x;
+        ba.w.y += dw.y;
+        ba.w.z += dw.z;
+        dirtyA = true;
+      }
+      if (realB >= 0 && applyB) {
+        bb.v.x += -dApp.x * invMassB;
+        bb.v.y += -dApp.y * invMassB;
+        bb.v.z += -dApp.z * invMassB;
+        const d3 dw = angDelta(qB, cross3(rBavg, scale3(dApp, -1.0)), invIB);
+        bb.w.x += dw.x;
+        bb.w.y += dw.y;
+        bb.w.z += dw.z;
+        dirtyB = true;
+      }
+    }
+  }
+  if (dirtyA) D.vw[realA] = ba;
+  if (dirtyB) D.vw[realB] = bb;
+}
+
+// solveVelocityColoredGSKokkos one-shot restitution (g = 0), src/solver_velocity.hpp:989-1133
+__device__ void gsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
+  const int2 r = D.vAB[k];
+  const int realA = r.x, realB = r.y;
+  const BodyC ca = D.bc[realA];
+  const double invMassA = ca.iim.w;
+  const d3 invIA = xyz(ca.iim);
+  const q4 qA = toq(ca.q);
+  BodyC cb;
+  double invMassB = 0.0;
+  d3 invIB{0, 0, 0};
+  q4 qB{0, 0, 0, 1};
+  Body64 ba = D.vw[realA];
+  Body64 bb;
+  d3 vB{0, 0, 0}, wB{0, 0, 0};
+  const double4 tb4 = D.vTB[k];
+  const double invN = tb4.w;
+  if (realB >= 0) {
+    cb = D.bc[realB];
+    invMassB = cb.iim.w;
+    invIB = xyz(cb.iim);
+    qB = toq(cb.q);
+    bb = D.vw[realB];
+    vB = xyz(bb.v);
+    wB = xyz(bb.w);
+  } else {
+    vB = scale3(d3{0.0, 0.0, 0.0}, invN);
+  }
+  const d3 vA = xyz(ba.v), wA = xyz(ba.w);
+  const double4 n4 = D.vN[k], ta4 = D.vTA[k], ra4 = D.vRA[k], rb4 = D.vRB[k];
+  double restitution = n4.w;
+  const d3 Nsum = xyz(n4), TauA = xyz(ta4), TauB = xyz(tb4);
+  const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
+  const double lenN = ra4.w;
+  if (lenN < 1e-9) return;
+  const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
+  const d3 vGrowth = scale3(diffCenters, P.growthRate);
+  double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
+  vn += dot3(vGrowth, Nsum);
+  const double alignment = dot3(Nsum, diffCenters);
+  if (alignment > 0.0) {
+    if (vn < 0.0) return;
+  } else {
+    if (vn > 0.0) return;
+  }
+  const double Nsq = dot3(Nsum, Nsum);
+  const double wA_n = Nsq * invMassA + genInvMass(TauA, invIA, qA);
+  const double wB_n = Nsq * invMassB + genInvMass(TauB, invIB, qB);
+  const double wTotal = wA_n + wB_n;
+  if (wTotal <= 0.0) return;
+  atomicMaxD(D.maxApproachBits, fabs(vn) / lenN);
+  if (fabs(vn) < restVelThreshold * lenN) restitution = 0.0;
+  const double lambda = (-restitution * vn - vn) / wTotal;
+  const d3 Jlin = scale3(Nsum, lambda);
+  ba.v.x += Jlin.x * invMassA;
+  ba.v.y += Jlin.y * invMassA;
+  ba.v.z += Jlin.z * invMassA;
+  {
+    const d3 dw = angDelta(qA, scale3(TauA, lambda), invIA);
+    ba.w.x += dw.x;
+    ba.w.y += dw.y;
+    ba.w.z += dw.z;
+  }
+  D.vw[realA] = ba;
+  if (realB >= 0) {
+    bb.v.x += -Jlin.x * invMassB;
+    bb.v.y += -Jlin.y * invMassB;
+    bb.v.z += -Jlin.z * invMassB;
+    const d3 dw = angDelta(qB, scale3(TauB, lambda), invIB);
+    bb.w.x += dw.x;
+    bb.w.y += dw.y;
+    bb.w.z += dw.z;
+    D.vw[realB] = bb;
+  }
+}
+
+// computeSideFlagsKokkos, src/solver_velocity.hpp:456-495 (rows are active, non-duplicate)
+__global__ void k_side_flags(Dev D, int numRows, d3 gHat, double riseThr, double approachThr) {
+  const int k = gtid();
+  if (k >= numRows) return;
+  unsigned char fl = D.vFlag[k] & (unsigned char)~(F_SIDE1 | F_SIDE2);
+  const int2 r = D.vAB[k];
+  if (r.y >= 0 && (fl & F_PERSIST) && !(fabs(D.vVn0[k]) > approachThr)) {
+    if ((fl & F_UP) && -dot3(xyz(D.vw[r.y].v), gHat) <= riseThr && D.ground[r.y] > 0)
+      fl |= F_SIDE1;
+    else if ((fl & F_DOWN) && -dot3(xyz(D.vw[r.x].v), gHat) <= riseThr && D.ground[r.x] > 0)
+      fl |= F_SIDE2;
+  }
+  D.vFlag[k] = fl;
+}
+
+// mode: 0 = warm start, 1 = PGS sweep, 2 = one-shot GS sweep
+__global__ void k_vsweep(Dev D, Params P, int rowBegin, int rowEnd, int mode, double thr) {
+  const int k = rowBegin + gtid();
+  if (k >= rowEnd) return;
+  if (mode == 0) warmApplyRow(D, k);
+  else if (mode == 1) pgsRow(D, P, k, thr);
+  else gsRow(D, P, k, thr);
+}
+// colour 62 may hold manifolds sharing a body (mask saturation): one thread, manifold order
+__global__ void k_vsweep_serial(Dev D, Params P, int numManifolds, int colour, int mode,
+                                double thr) {
+  if (gtid() != 0) return;
+  for (int m = 0; m < numManifolds; ++m) {
+    if (D.mColor[m] != colour) continue;
+    const int k = D.vRowOf[m];
+    if (mode == 0) warmApplyRow(D, k);
+    else if (mode == 1) pgsRow(D, P, k, thr);
+    else gsRow(D, P, k, thr);
+  }
+}
+
+// commitPairKeysLambdaKokkos, src/contact_preprocessing.hpp:160-192: bucket by the low real index
+// (key >> 32), then sort each bucket by (key, manifold index) — a stable key sort
+__global__ void k_ledger_count(Dev D, int numManifolds, int numReal) {
+  const int m = gtid();
+  if (m >= numManifolds) return;
+  const unsigned long long key = D.mKey[m];
+  const int b = (key == ~0ull) ? numReal : (int)(key >> 32);
+  atomicAdd(&D.bucketCnt[b], 1);
+}
+__global__ void k_ledger_scatter(Dev D, int numManifolds, int numReal) {
+  const int m = gtid();
+  if (m >= numManifolds) return;
+  const unsigned long long key = D.mKey[m];
+  const int b = (key == ~0ull) ? numReal : (int)(key >> 32);
+  const int pos = D.bucketStart[b] + atomicAdd(&D.bucketCnt[b], 1);
+  D.lKey[pos] = key;
+  D.lIdx[pos] = m;
+}
+__global__ void k_ledger_sort(Dev D, int numReal) {
+  const int b = gtid();
+  if (b >= numReal) return;
+  const int s = D.bucketStart[b], e = D.bucketStart[b + 1];
+  for (int a = s + 1; a < e; ++a) {
+    const unsigned long long k = D.lKey[a];
+    const int ix = D.lIdx[a];
+    int c = a - 1;
+    while (c >= s && (D.lKey[c] > k || (D.lKey[c] == k && D.lIdx[c] > ix))) {
+      D.lKey[c + 1] = D.lKey[c];
+      D.lIdx[c + 1] = D.lIdx[c];
+      --c;
+    }
+    D.lKey[c + 1] = k;
+    D.lIdx[c + 1] = ix;
+  }
+}
+__global__ void k_ledger_values(Dev D, int numManifolds) {
+  const int pos = gtid();
+  if (pos >= numManifolds) return;
+  const int m = D.lIdx[pos];
+  const int k = D.vRowOf[m];
+  if (k >= 0) {
+    D.lLam[pos] = D.vLam[k];
+    const double4 lt = D.vLamT[k];
+    D.lLamT[pos] = make_double4(lt.x, lt.y, lt.z, 0.0);
+  } else {
+    D.lLam[pos] = 0.0;
+    D.lLamT[pos] = make_double4(0, 0, 0, 0);
+  }
+}
+
+// ============================================================================================
+// position solve — src/solver_position.hpp:248-331
+// ============================================================================================
+__global__ void k_jp_init_contacts(int* colour, int n, int* uncol) {
+  const int c = gtid();
+  if (c >= n) return;
+  colour[c] = -1;
+  atomicAdd(uncol, 1);
+}
+__global__ void k_prows(Dev D, int numRows) {
+  const int k = gtid();
+  if (k >= numRows) return;
+  const int c = D.pPerm[k];
+  D.pAB[k] = D.cAB[c];
+  D.pN[k] = D.cN[c];
+  D.pRA[k] = D.cRA[c];
+  D.pRB[k] = D.cRB[c];
+  D.pLam[k] = 0.0;
+}
+__device__ void posRow(Dev& D, int k) {
+  const int2 ab = D.pAB[k];
+  const int idA = ab.x, idB = ab.y;
+  const BodyC ca = D.bc[idA];
+  const double invMassA = ca.iim.w;
+  const double4 pa4 = D.posPred[idA];
+  const d3 pA = xyz(pa4);
+  const q4 qA = toq(D.quatPred[idA]);
+  d3 pB{0, 0, 0};
+  q4 qB{0, 0, 0, 1};
+  BodyC cb;
+  double invMassB = 0.0;
+  double4 pb4;
+  if (idB >= 0) {
+    cb = D.bc[idB];
+    invMassB = cb.iim.w;
+    pb4 = D.posPred[idB];
+    pB = xyz(pb4);
+    qB = toq(D.quatPred[idB]);
+  }
+  const double4 n4 = D.pN[k], ra4 = D.pRA[k], rb4 = D.pRB[k];
+  const q4 qAdelta = quatMult(qA, quatInverse(toq(ca.q)));
+  const d3 rA = rotate(qAdelta, xyz(ra4));
+  d3 rB = xyz(rb4);
+  d3 n = xyz(n4);
+  if (idB >= 0) {
+    const q4 qBdelta = quatMult(qB, quatInverse(toq(cb.q)));
+    rB = rotate(qBdelta, rB);
+    n = rotate(qBdelta, n);
+  }
+  double C;
+  if (idB < 0) {
+    n = xyz(n4);
+    C = dot3(sub3(add3(pA, rA), xyz(rb4)), n);
+  } else {
+    C = dot3(sub3(add3(pA, rA), add3(pB, rB)), n);
+  }
+  if (C >= 0.0) return;
+  const d3 invIA = xyz(ca.iim);
+  const d3 invIB = (idB >= 0) ? xyz(cb.iim) : d3{0, 0, 0};
+  const double wTotal = computeW(rA, n, invMassA, invIA) + computeW(rB, n, invMassB, invIB);
+  if (wTotal < 1e-6) return;
+  const double dLambda = -C / wTotal;
+  D.pLam[k] += dLambda;
+  D.posPred[idA].x = pa4.x + n.x * dLambda * invMassA;
+  D.posPred[idA].y = pa4.y + n.y * dLambda * invMassA;
+  D.posPred[idA].z = pa4.z + n.z * dLambda * invMassA;
+  if (idB >= 0) {
+    D.posPred[idB].x = pb4.x + -n.x * dLambda * invMassB;
+    D.posPred[idB].y = pb4.y + -n.y * dLambda * invMassB;
+    D.posPred[idB].z = pb4.z + -n.z * dLambda * invMassB;
+  }
+  atomicMaxD(D.maxOverlapBits, -C);
+}
+__global__ void k_psweep(Dev D, int rowBegin, int rowEnd) {
+  const int k = rowBegin + gtid();
+  if (k >= rowEnd) return;
+  posRow(D, k);
+}
+__global__ void k_psweep_serial(Dev D, int numContacts, int colour) {
+  if (gtid() != 0) return;
+  for (int c = 0; c < numContacts; ++c)
+    if (D.cColor[c] == colour) posRow(D, D.pRowOf[c]);
+}
+
+// commitPosImpulseKokkos, src/contact_preprocessing.hpp:501-538 (sum in contact order; among
+// manifolds sharing a key the highest manifold index writes, as a serial loop would)
+__global__ void k_commit_pos_impulse(Dev D, int numManifolds, double dt) {
+  const int m = gtid();
+  if (m >= numManifolds) return;
+  const int2 rg = D.mRange[m];
+  double s = 0.0;
+  for (int c = rg.x; c < rg.x + rg.y; ++c) {
+    const int row = D.pRowOf[c];  // -1: uncoloured contact, solved by the Jacobi fallback only
+    const double l = row >= 0 ? D.pLam[row] : 0.0;
+    if (l != 0.0) s += l / dt;
+  }
+  const unsigned long long key = D.mKey[m];
+  if (key == ~0ull) return;
+  const int lo = lowerBound(D.lKey, numManifolds, key);
+  if (!(lo < numManifolds && D.lKey[lo] == key)) return;
+  int hi = lo + 1;
+  while (hi < numManifolds && D.lKey[hi] == key) ++hi;
+  if (D.lIdx[hi - 1] == m) D.lLamT[lo].w = s;
+}
+
+// ============================================================================================
+// colour-saturation fallbacks (colour == -1 after the round cap), src/sim.hpp:245-256,454-462
+// Accumulation is serial in manifold / contact order (the canonical order of the oracle); the
+// averaged apply is parallel (one body per thread).
+// ============================================================================================
+// solveVelocityKokkos restricted to uncoloured manifolds, src/solver_velocity.hpp:38-224
+__global__ void k_vel_jacobi_accum(Dev D, Params P, int numManifolds, double thr) {
+  if (gtid() != 0) return;
+  for (int m = 0; m < numManifolds; ++m) {
+    if (D.mColor[m] != -1 || D.mNp[m] <= 0) continue;
+    const int2 r = D.mReal[m];
+    const int realA = r.x, realB = r.y;
+    if (realB >= 0 && realA > realB) continue;
+    const BodyC ca = D.bc[realA];
+    const double invMassA = ca.iim.w;
+    const d3 invIA = xyz(ca.iim);
+    const q4 qA = toq(ca.q);
+    double invMassB = 0.0;
+    d3 invIB{0, 0, 0};
+    q4 qB{0, 0, 0, 1};
+    const Body64 ba = D.vw[realA];
+    const d3 vA = xyz(ba.v), wA = xyz(ba.w);
+    d3 vB{0, 0, 0}, wB{0, 0, 0};
+    if (realB >= 0) {
+      const BodyC cb = D.bc[realB];
+      invMassB = cb.iim.w;
+      invIB = xyz(cb.iim);
+      qB = toq(cb.q);
+      const Body64 bb = D.vw[realB];
+      vB = xyz(bb.v);
+      wB = xyz(bb.w);
+    }
+    const double4 ns4 = D.mNs[m];
+    const d3 Nsum = xyz(ns4), TauA = xyz(D.mTA[m]), TauB = xyz(D.mTB[m]);
+    const double invN = 1.0 / (double)D.mNp[m];
+    double restitution = P.restN;
+    if (realB < 0) vB = scale3(d3{0.0, 0.0, 0.0}, invN);
+    {
+      const double ra = ns4.w * invN;
+      if (ra >= 0.0) restitution = ra;
+    }
+    const d3 rAavg = scale3(xyz(D.mRA[m]), invN);
+    const d3 rBavg = scale3(xyz(D.mRB[m]), invN);
+    const double lenN = sqrt(dot3(Nsum, Nsum));
+    if (lenN < 1e-9) continue;
+    const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
+    const d3 vGrowth = scale3(diffCenters, P.growthRate);
+    double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
+    vn += dot3(vGrowth, Nsum);
+    const double alignment = dot3(Nsum, diffCenters);
+    if (alignment > 0.0) {
+      if (vn < 0.0) continue;
+    } else {
+      if (vn > 0.0) continue;
+    }
+    const double Nsq = dot3(Nsum, Nsum);
+    const double wTotal = (Nsq * invMassA + genInvMass(TauA, invIA, qA)) +
+                          (Nsq * invMassB + genInvMass(TauB, invIB, qB));
+    if (wTotal <= 0.0) continue;
+    if (fabs(vn) < thr * lenN) restitution = 0.0;
+    const double lambda = (-restitution * vn - vn) / wTotal;
+    const d3 Jlin = scale3(Nsum, lambda);
+    double4& dva = D.dVel[realA];
+    dva.x += Jlin.x * invMassA;
+    dva.y += Jlin.y * invMassA;
+    dva.z += Jlin.z * invMassA;
+    {
+      const d3 dw = angDelta(qA, scale3(TauA, lambda), invIA);
+      double4& dwa = D.dAng[realA];
+      dwa.x += dw.x;
+      dwa.y += dw.y;
+      dwa.z += dw.z;
+    }
+    if (realB >= 0) {
+      double4& dvb = D.dVel[realB];
+      dvb.x += -Jlin.x * invMassB;
+      dvb.y += -Jlin.y * invMassB;
+      dvb.z += -Jlin.z * invMassB;
+      const d3 dw = angDelta(qB, scale3(TauB, lambda), invIB);
+      double4& dwb = D.dAng[realB];
+      dwb.x += dw.x;
+      dwb.y += dw.y;
+      dwb.z += dw.z;
+      D.dCnt[realB] += 1;
+    }
+    D.dCnt[realA] += 1;
+  }
+}
+// applyVelocityDeltasAveragedKokkos, src/solver_velocity.hpp:233-255
+__global__ void k_apply_vel_avg(Dev D, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const int count = D.dCnt[i];
+  if (count <= 0) return;
+  const double f = fmin(1.0, 2.0 / (double)count);
+  Body64 b = D.vw[i];
+  const double4 dv = D.dVel[i], dw = D.dAng[i];
+  b.v.x += dv.x * f;
+  b.w.x += dw.x * f;
+  b.v.y += dv.y * f;
+  b.w.y += dw.y * f;
+  b.v.z += dv.z * f;
+  b.w.z += dw.z * f;
+  D.vw[i] = b;
+  D.dVel[i] = make_double4(0, 0, 0, 0);
+  D.dAng[i] = make_double4(0, 0, 0, 0);
+  D.dCnt[i] = 0;
+}
+// solvePositionKokkos restricted to uncoloured contacts, src/solver_position.hpp:34-132
+__global__ void k_pos_jacobi_accum(Dev D, int numContacts) {
+  if (gtid() != 0) return;
+  for (int c = 0; c < numContacts; ++c) {
+    if (D.cColor[c] != -1) continue;
+    const int2 ab = D.cAB[c];
+    const int idA = ab.x, idB = ab.y;
+    const BodyC ca = D.bc[idA];
+    const double invMassA = ca.iim.w;
+    const d3 pA = xyz(D.posPred[idA]);
+    const q4 qA = toq(D.quatPred[idA]);
+    d3 pB{0, 0, 0};
+    q4 qB{0, 0, 0, 1};
+    BodyC cb;
+    double invMassB = 0.0;
+    if (idB >= 0) {
+      cb = D.bc[idB];
+      invMassB = cb.iim.w;
+      pB = xyz(D.posPred[idB]);
+      qB = toq(D.quatPred[idB]);
+    }
+    const double4 n4 = D.cN[c], ra4 = D.cRA[c], rb4 = D.cRB[c];
+    const q4 qAdelta = quatMult(qA, quatInverse(toq(ca.q)));
+    const d3 rA = rotate(qAdelta, xyz(ra4));
+    d3 rB = xyz(rb4);
+    d3 n = xyz(n4);
+    if (idB >= 0) {
+      const q4 qBdelta = quatMult(qB, quatInverse(toq(cb.q)));
+      rB = rotate(qBdelta, rB);
+      n = rotate(qBdelta, n);
+    }
+    double C;
+    if (idB < 0) {
+      n = xyz(n4);
+      C = dot3(sub3(add3(pA, rA), xyz(rb4)), n);
+    } else {
+      C = dot3(sub3(add3(pA, rA), add3(pB, rB)), n);
+    }
+    if (C >= 0.0) continue;
+    const d3 invIA = xyz(ca.iim);
+    const d3 invIB = (idB >= 0) ? xyz(cb.iim) : d3{0, 0, 0};
+    const double wTotal = computeW(rA, n, invMassA, invIA) + computeW(rB, n, invMassB, invIB);
+    if (wTotal < 1e-6) continue;
+    const double dLambda = -C / wTotal;
+    double4& da = D.dPos[idA];
+    da.x += n.x * dLambda * invMassA;
+    da.y += n.y * dLambda * invMassA;
+    da.z += n.z * dLambda * invMassA;
+    if (idB >= 0) {
+      double4& db = D.dPos[idB];
+      db.x += -n.x * dLambda * invMassB;
+      db.y += -n.y * dLambda * invMassB;
+      db.z += -n.z * dLambda * invMassB;
+      D.dCnt[idB] += 1;
+    }
+    D.dCnt[idA] += 1;
+    atomicMaxD(D.maxOverlapBits, -C);
+  }
+}
+// applyUpdatesKokkos, src/integration.hpp:168-184
+__global__ void k_apply_pos_avg(Dev D, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const int count = D.dCnt[i];
+  if (count <= 0) return;
+  const double f = 1.0 / (double)count;
+  const double4 dp = D.dPos[i], dv = D.dVel[i];
+  D.posPred[i].x = D.posPred[i].x + dp.x * f;
+  D.posPred[i].y = D.posPred[i].y + dp.y * f;
+  D.posPred[i].z = D.posPred[i].z + dp.z * f;
+  D.vw[i].v.x = D.vw[i].v.x + dv.x * f;
+  D.vw[i].v.y = D.vw[i].v.y + dv.y * f;
+  D.vw[i].v.z = D.vw[i].v.z + dv.z * f;
+  D.dPos[i] = make_double4(0, 0, 0, 0);
+  D.dVel[i] = make_double4(0, 0, 0, 0);
+  D.dCnt[i] = 0;
+}
+
+}  // namespace pd

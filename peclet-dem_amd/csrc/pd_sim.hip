@@ -1,0 +1,1582 @@
+// pd_sim.hip — host orchestration of the XPBD substep and the C-ABI of libpecletdem_hip.so.
+//
+// One pd_sim owns one HIP stream and every device buffer of one particle system.  pd_step runs
+// the substep of the reference demStep (src/sim.hpp:97-488) as a sequence of kernels from
+// pd_kernels.hpp on that stream; buffer counts that size the next launch are read back to the
+// host at the same points where the reference reads its device counters.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pd_kernels.hpp"
+#include "peclet_dem.h"
+
+using namespace pd;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(x)                                                                 \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + \
+                               " at " #x);                                          \
+  } while (0)
+
+// every kernel bounds-checks its index, so an empty range still launches one block
+inline int nblk(long n) { return n > 0 ? (int)((n + TPB - 1) / TPB) : 1; }
+
+// grow-only device buffer; growth keeps the first `keep` elements
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  void ensure(size_t n, hipStream_t st, size_t keep = 0) {
+    if (n <= cap) return;
+    size_t nc = n + n / 4 + 64;
+    T* q = nullptr;
+    HIPCHECK(hipMalloc(&q, nc * sizeof(T)));
+    HIPCHECK(hipMemsetAsync(q, 0, nc * sizeof(T), st));
+    if (p && keep) HIPCHECK(hipMemcpyAsync(q, p, std::min(keep, cap) * sizeof(T),
+                                           hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (p) HIPCHECK(hipFree(p));
+    p = q;
+    cap = nc;
+  }
+};
+
+enum Stage {
+  ST_PREDICT,
+  ST_BROAD,
+  ST_NARROW,
+  ST_MANIFOLD,
+  ST_COLOR_V,
+  ST_PREP,
+  ST_VSWEEP,
+  ST_STAB,
+  ST_LEDGER,
+  ST_REINT,
+  ST_COLOR_P,
+  ST_PSWEEP,
+  ST_COMMIT,
+  ST_COUNT
+};
+const char* kStageNames[ST_COUNT] = {"predict_ghosts", "broadphase",   "narrowphase",
+                                     "manifolds",      "color_vel",    "vel_prepare_warm",
+                                     "vel_sweeps",     "stabilization", "ledger",
+                                     "reintegrate",    "color_pos",    "pos_sweeps",
+                                     "commit"};
+
+}  // namespace
+
+struct pd_sim {
+  hipStream_t st = nullptr;
+  int capacity = 0;
+  int numReal = 0, numParticles = 0;
+  // shape 0 (src/sim.hpp:949-1021)
+  int shapeType = SHAPE_SPHERE;
+  double shapeParams[4] = {1.0, 0, 0, 0};
+  std::vector<double> shellHost;  // 3 per point
+  double baseRadius = 1.0;
+  double defaultInvI[3] = {2.5, 2.5, 2.5};
+  // host parameters (src/particles.hpp:173-191)
+  double dmin[3] = {0, 0, 0}, dmax[3] = {0, 0, 0}, dsize[3] = {0, 0, 0};
+  int per[3] = {0, 0, 0};
+  double g[3] = {0, 0, 0};
+  double dt = 1e-3, globalScale = 1.0, growthRate = 0.0, growthFactor = -1.0;
+  double thermoTau = 0.0, thermoTemp = 0.0, thermoKB = 1.0;
+  double frictionDynamic = 0.0, restitutionNormal = 0.0, restitutionTangent = 0.0;
+  int posIters = 10, velIters = 0, useGS = 1, stabMode = 1;
+  double planeP[8][3], planeN[8][3];
+  int numPlanes = 0;
+  int hasPairTable = 0;
+  double pairTable[128];
+  // results
+  int np = 0, nc = 0, nm = 0, prevCount = 0, nRowsV = 0;
+  double maxOverlap = 0.0, maxApproach = 0.0;
+  pd_stats stats{};
+  // timing: stage events (level >= 1) and per-launch events of the sweep kernels (level >= 2)
+  int timing = 0;
+  double stageMs[ST_COUNT] = {};
+  hipEvent_t ev[ST_COUNT + 1] = {};
+  enum { K_VSWEEP, K_PSWEEP, K_COUNT };
+  double kernMs[K_COUNT] = {};
+  long kernLaunches[K_COUNT] = {};
+  long kernRows[K_COUNT] = {};
+  std::vector<hipEvent_t> kev;
+  std::vector<std::pair<int, int>> kevUse;  // (kernel id, rows) per recorded pair
+  size_t kevNext = 0;
+  void kernBegin() {
+    if (timing < 2) return;
+    if (kevNext + 2 > kev.size()) {
+      for (int k = 0; k < 64; ++k) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        kev.push_back(e);
+      }
+    }
+    HIPCHECK(hipEventRecord(kev[kevNext], st));
+  }
+  void kernEnd(int kid, int rows) {
+    if (timing < 2) return;
+    HIPCHECK(hipEventRecord(kev[kevNext + 1], st));
+    kevUse.push_back({kid, rows});
+    kevNext += 2;
+  }
+  void kernCollect() {
+    if (timing < 2) return;
+    sync();
+    for (size_t u = 0; u < kevUse.size(); ++u) {
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, kev[2 * u], kev[2 * u + 1]));
+      kernMs[kevUse[u].first] += ms;
+      kernLaunches[kevUse[u].first] += 1;
+      kernRows[kevUse[u].first] += kevUse[u].second;
+    }
+    kevUse.clear();
+    kevNext = 0;
+  }
+  // device buffers
+  DBuf<double4> pos, posPred, quatPred, extForce, shell, sbox;
+  DBuf<Body64> vw0, vw;
+  DBuf<BodyC> bc;
+  DBuf<double> scale, targetScale, pairTableD, thermo;
+  DBuf<double4> dVel, dAng, dPos;
+  DBuf<int> dCnt;
+  DBuf<int> realIdx, ground, ground2, cellOf, cellCnt, cellStart, order, pairCnt, pairStart;
+  DBuf<unsigned char> matId, mCode, vFlag;
+  DBuf<int2> pairs, cAB, mAB, mReal, mRange, vAB, pAB;
+  DBuf<int> ncnt, noff, cSlot, mCnt, mStart, mNp, mColor, cColor, vPerm, vRowOf, pPerm, pRowOf;
+  DBuf<double4> cN, cRA, cRB, mNs, mTA, mTB, mRA, mRB, vN, vTA, vTB, vRA, vRB, vVt0, vLamT, pN,
+      pRA, pRB, lLamT;
+  DBuf<double> vVn0, vLam, pLam, lLam;
+  DBuf<unsigned long long> mKey, winner, mask, lKey, uKey, red, bits;
+  DBuf<int> lIdx, bucketCnt, bucketStart, counters, colCnt, colFill, colStart, scanTmp, ghostCnt,
+      ghostOff;
+  // pinned host scratch
+  void* hostScratch = nullptr;
+
+  Dev D{};
+
+  pd_sim(int cap) : capacity(cap) {
+    HIPCHECK(hipStreamCreate(&st));
+    HIPCHECK(hipHostMalloc(&hostScratch, 4096, hipHostMallocDefault));
+    for (int k = 0; k <= ST_COUNT; ++k) HIPCHECK(hipEventCreate(&ev[k]));
+    ensureSlots(cap > 0 ? cap : 1, 0);
+    red.ensure(8, st);
+    bits.ensure(4, st);
+    counters.ensure(16, st);
+    colCnt.ensure(64, st);
+    colFill.ensure(64, st);
+    colStart.ensure(65, st);
+    pairTableD.ensure(128, st);
+    shell.ensure(1, st);
+    initializeShape(SHAPE_SPHERE, 1.0, 0.0, 0.0);
+  }
+  ~pd_sim() {
+    (void)hipStreamSynchronize(st);
+    DBufReleaseAll();
+    for (int k = 0; k <= ST_COUNT; ++k) (void)hipEventDestroy(ev[k]);
+    for (hipEvent_t e : kev) (void)hipEventDestroy(e);
+    if (hostScratch) (void)hipHostFree(hostScratch);
+    (void)hipStreamDestroy(st);
+  }
+  void DBufReleaseAll() {
+    pos.release(); posPred.release(); quatPred.release(); extForce.release(); shell.release();
+    sbox.release(); vw0.release(); vw.release(); bc.release(); scale.release();
+    targetScale.release(); pairTableD.release(); thermo.release(); realIdx.release();
+    ground.release(); ground2.release(); cellOf.release(); cellCnt.release(); cellStart.release();
+    order.release(); pairCnt.release(); pairStart.release(); matId.release(); mCode.release();
+    vFlag.release(); pairs.release(); cAB.release(); mAB.release(); mReal.release();
+    mRange.release(); vAB.release(); pAB.release(); ncnt.release(); noff.release();
+    cSlot.release(); mCnt.release(); mStart.release(); mNp.release(); mColor.release();
+    cColor.release(); vPerm.release(); vRowOf.release(); pPerm.release(); pRowOf.release();
+    cN.release(); cRA.release(); cRB.release(); mNs.release(); mTA.release(); mTB.release();
+    mRA.release(); mRB.release(); vN.release(); vTA.release(); vTB.release(); vRA.release();
+    vRB.release(); vVt0.release(); vLamT.release(); pN.release(); pRA.release(); pRB.release();
+    lLamT.release(); vVn0.release(); vLam.release(); pLam.release(); lLam.release();
+    mKey.release(); winner.release(); mask.release(); lKey.release(); uKey.release();
+    red.release(); bits.release(); lIdx.release(); bucketCnt.release(); bucketStart.release();
+    counters.release(); colCnt.release(); colFill.release(); colStart.release();
+    scanTmp.release(); ghostCnt.release(); ghostOff.release();
+    dVel.release(); dAng.release(); dPos.release(); dCnt.release();
+  }
+
+  // ---------------------------------------------------------------- small helpers
+  void sync() { HIPCHECK(hipStreamSynchronize(st)); }
+  template <class T>
+  T read1(const T* dptr) {
+    HIPCHECK(hipMemcpyAsync(hostScratch, dptr, sizeof(T), hipMemcpyDeviceToHost, st));
+    sync();
+    T v;
+    std::memcpy(&v, hostScratch, sizeof(T));
+    return v;
+  }
+  template <class T>
+  void readN(T* host, const T* dptr, size_t n) {
+    if (!n) return;
+    HIPCHECK(hipMemcpyAsync(host, dptr, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    sync();
+  }
+  template <class T>
+  void writeN(T* dptr, const T* host, size_t n) {
+    if (!n) return;
+    HIPCHECK(hipMemcpyAsync(dptr, host, n * sizeof(T), hipMemcpyHostToDevice, st));
+    sync();
+  }
+  template <class T>
+  void zero(T* dptr, size_t n) {
+    if (n) HIPCHECK(hipMemsetAsync(dptr, 0, n * sizeof(T), st));
+  }
+  void mark(int stage) {
+    if (timing) HIPCHECK(hipEventRecord(ev[stage], st));
+  }
+
+  // exclusive scan of n ints into out[0..n), total into out[n]; returns the total
+  int scan(const int* in, int* out, int n) {
+    if (n <= 0) {
+      zero(out, 1);
+      return 0;
+    }
+    const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    scanTmp.ensure(nb + 1, st);
+    k_scan_reduce<<<nb, TPB, 0, st>>>(in, n, scanTmp.p);
+    k_scan_top<<<1, TPB, 0, st>>>(scanTmp.p, nb, out + n);
+    k_scan_down<<<nb, TPB, 0, st>>>(in, n, scanTmp.p, out);
+    HIPCHECK(hipGetLastError());
+    return read1(out + n);
+  }
+
+  // ---------------------------------------------------------------- capacity
+  void ensureSlots(int n, int keep) {
+    pos.ensure(n, st, keep);
+    posPred.ensure(n, st, keep);
+    quatPred.ensure(n, st, keep);
+    extForce.ensure(n, st, keep);
+    vw0.ensure(n, st, keep);
+    vw.ensure(n, st, keep);
+    bc.ensure(n, st, keep);
+    scale.ensure(n, st, keep);
+    targetScale.ensure(n, st, keep);
+    realIdx.ensure(n, st, keep);
+    matId.ensure(n, st, keep);
+    ground.ensure(n, st, keep);
+    ground2.ensure(n, st, keep);
+  }
+
+  void refreshDev() {
+    Dev& d = D;
+    d.pos = pos.p; d.vw0 = vw0.p; d.bc = bc.p; d.posPred = posPred.p; d.quatPred = quatPred.p;
+    d.vw = vw.p; d.scale = scale.p; d.targetScale = targetScale.p; d.extForce = extForce.p;
+    d.realIdx = realIdx.p; d.matId = matId.p; d.ground = ground.p; d.ground2 = ground2.p;
+    d.shell = shell.p; d.pairTable = pairTableD.p;
+    d.cellOf = cellOf.p; d.cellCnt = cellCnt.p; d.cellStart = cellStart.p; d.order = order.p;
+    d.sbox = sbox.p; d.pairCnt = pairCnt.p; d.pairStart = pairStart.p; d.pairs = pairs.p;
+    d.ncnt = ncnt.p; d.noff = noff.p; d.cAB = cAB.p; d.cN = cN.p; d.cRA = cRA.p; d.cRB = cRB.p;
+    d.cSlot = cSlot.p; d.mCnt = mCnt.p; d.mStart = mStart.p; d.mAB = mAB.p; d.mReal = mReal.p;
+    d.mRange = mRange.p; d.mNs = mNs.p; d.mTA = mTA.p; d.mTB = mTB.p; d.mRA = mRA.p;
+    d.mRB = mRB.p; d.mNp = mNp.p; d.mKey = mKey.p; d.mCode = mCode.p; d.mColor = mColor.p;
+    d.cColor = cColor.p; d.winner = winner.p; d.mask = mask.p; d.vPerm = vPerm.p;
+    d.vRowOf = vRowOf.p; d.pPerm = pPerm.p; d.pRowOf = pRowOf.p; d.vAB = vAB.p; d.vN = vN.p;
+    d.vTA = vTA.p; d.vTB = vTB.p; d.vRA = vRA.p; d.vRB = vRB.p; d.vVt0 = vVt0.p;
+    d.vLamT = vLamT.p; d.vVn0 = vVn0.p; d.vLam = vLam.p; d.vFlag = vFlag.p; d.pAB = pAB.p;
+    d.pN = pN.p; d.pRA = pRA.p; d.pRB = pRB.p; d.pLam = pLam.p; d.lKey = lKey.p;
+    d.lLam = lLam.p; d.lLamT = lLamT.p; d.lIdx = lIdx.p; d.uKey = uKey.p;
+    d.bucketCnt = bucketCnt.p; d.bucketStart = bucketStart.p; d.red = red.p;
+    d.counters = counters.p; d.colCnt = colCnt.p; d.colFill = colFill.p; d.colStart = colStart.p;
+    d.maxOverlapBits = bits.p; d.maxApproachBits = bits.p + 1; d.thermo = thermo.p;
+    d.dVel = dVel.p; d.dAng = dAng.p; d.dPos = dPos.p; d.dCnt = dCnt.p;
+  }
+  // zeroed Jacobi accumulators for the colour-saturation fallback
+  void prepareFallback() {
+    dVel.ensure(numParticles + 1, st);
+    dAng.ensure(numParticles + 1, st);
+    dPos.ensure(numParticles + 1, st);
+    dCnt.ensure(numParticles + 1, st);
+    refreshDev();
+    zero(dVel.p, numParticles);
+    zero(dAng.p, numParticles);
+    zero(dPos.p, numParticles);
+    zero(dCnt.p, numParticles);
+  }
+
+  Params params(double margin, double band) const {
+    Params P{};
+    P.dt = dt;
+    P.gs = globalScale;
+    P.baseRadius = baseRadius;
+    P.margin = margin;
+    P.band = band;
+    P.boxCap = std::max(std::max(dsize[0], dsize[1]), dsize[2]);
+    P.growthRate = growthRate;
+    P.restN = restitutionNormal;
+    P.restT = restitutionTangent;
+    P.mu = frictionDynamic;
+    P.g = d3{g[0], g[1], g[2]};
+    P.dmin = d3{dmin[0], dmin[1], dmin[2]};
+    P.dmax = d3{dmax[0], dmax[1], dmax[2]};
+    P.dsize = d3{dsize[0], dsize[1], dsize[2]};
+    P.px = per[0];
+    P.py = per[1];
+    P.pz = per[2];
+    P.numReal = numReal;
+    P.numParticles = numParticles;
+    P.numPlanes = numPlanes;
+    P.hasPairTable = hasPairTable;
+    P.shape.type = shapeType;
+    P.shape.numShell = (int)(shellHost.size() / 3);
+    P.shape.p0 = shapeParams[0];
+    P.shape.p1 = shapeParams[1];
+    P.shape.p2 = shapeParams[2];
+    P.shape.p3 = shapeParams[3];
+    for (int k = 0; k < numPlanes; ++k) {
+      P.planeP[k] = make_double4(planeP[k][0], planeP[k][1], planeP[k][2], 0.0);
+      P.planeN[k] = make_double4(planeN[k][0], planeN[k][1], planeN[k][2], 0.0);
+    }
+    return P;
+  }
+
+  // ---------------------------------------------------------------- shape
+  void uploadShell() {
+    const size_t n = shellHost.size() / 3;
+    shell.ensure(n > 0 ? n : 1, st);
+    std::vector<double4> h(n);
+    for (size_t i = 0; i < n; ++i)
+      h[i] = make_double4(shellHost[3 * i], shellHost[3 * i + 1], shellHost[3 * i + 2], 0.0);
+    writeN(shell.p, h.data(), n);
+  }
+  // src/sim.hpp:949-1021 + shell generators src/shapes_portable.hpp:21-111 (FP64)
+  void initializeShape(int type, double radius, double height, double thickness) {
+    baseRadius = radius;
+    double prm[4] = {radius, 0, 0, 0};
+    std::vector<double> pts;
+    auto push = [&](double x, double y, double z) {
+      pts.push_back(x);
+      pts.push_back(y);
+      pts.push_back(z);
+    };
+    if (type == SHAPE_HOLLOW_CYLINDER) {
+      prm[1] = height;
+      prm[2] = thickness;
+      double min_dim = std::min(radius, thickness);
+      if (min_dim < 1e-4) min_dim = radius;
+      double spacing = std::min(radius * 0.3, min_dim * 0.5);
+      if (spacing < 1e-3) spacing = 1e-3;
+      const double r_outer = radius, r_inner = radius - thickness, h = height;
+      double circumference = 2.0 * M_PI * r_outer;
+      int n_angular = (int)std::ceil(circumference / spacing);
+      const int n_vertical = (int)std::ceil(h / spacing);
+      for (int i = 0; i < n_angular; ++i) {
+        const double theta = 2.0 * M_PI * i / n_angular;
+        for (int j = 0; j <= n_vertical; ++j)
+          push(r_outer * std::cos(theta), -0.5 * h + h * j / n_vertical, r_outer * std::sin(theta));
+      }
+      if (thickness > 0.0) {
+        circumference = 2.0 * M_PI * r_inner;
+        n_angular = (int)std::ceil(circumference / spacing);
+        for (int i = 0; i < n_angular; ++i) {
+          const double theta = 2.0 * M_PI * i / n_angular;
+          for (int j = 0; j <= n_vertical; ++j)
+            push(r_inner * std::cos(theta), -0.5 * h + h * j / n_vertical,
+                 r_inner * std::sin(theta));
+        }
+      }
+      const int n_radial = (int)std::ceil(thickness / spacing);
+      for (int i = 0; i <= n_radial; ++i) {
+        const double r = r_inner + (r_outer - r_inner) * i / n_radial;
+        circumference = 2.0 * M_PI * r;
+        n_angular = (int)std::ceil(circumference / spacing);
+        for (int j = 0; j < n_angular; ++j) {
+          const double theta = 2.0 * M_PI * j / n_angular;
+          const double x = r * std::cos(theta), z = r * std::sin(theta);
+          push(x, 0.5 * h, z);
+          push(x, -0.5 * h, z);
+        }
+      }
+    } else if (type == SHAPE_BOX) {
+      prm[1] = radius;
+      prm[2] = radius;
+      const double spacing = std::max(radius * 0.5, 1e-3);
+      auto nseg = [&](double half) {
+        int n = (int)std::ceil(2.0 * half / spacing);
+        return n < 1 ? 1 : n;
+      };
+      auto lin = [](double half, int i, int n) { return -half + 2.0 * half * (double)i / (double)n; };
+      const double hx = radius, hy = radius, hz = radius;
+      const int nx = nseg(hx), ny = nseg(hy), nz = nseg(hz);
+      for (int j = 0; j <= ny; ++j)
+        for (int k = 0; k <= nz; ++k) {
+          push(hx, lin(hy, j, ny), lin(hz, k, nz));
+          push(-hx, lin(hy, j, ny), lin(hz, k, nz));
+        }
+      for (int i = 0; i <= nx; ++i)
+        for (int k = 0; k <= nz; ++k) {
+          push(lin(hx, i, nx), hy, lin(hz, k, nz));
+          push(lin(hx, i, nx), -hy, lin(hz, k, nz));
+        }
+      for (int i = 0; i <= nx; ++i)
+        for (int j = 0; j <= ny; ++j) {
+          push(lin(hx, i, nx), lin(hy, j, ny), hz);
+          push(lin(hx, i, nx), lin(hy, j, ny), -hz);
+        }
+    } else {
+      type = SHAPE_SPHERE;
+    }
+    shapeType = type;
+    std::memcpy(shapeParams, prm, sizeof prm);
+    shellHost = pts;
+    uploadShell();
+    double ix = 1.0, iy = 1.0, iz = 1.0;
+    if (type == SHAPE_SPHERE) {
+      if (baseRadius > 0.0) ix = iy = iz = 2.5 / (baseRadius * baseRadius);
+    } else if (type == SHAPE_HOLLOW_CYLINDER) {
+      const double r_out = baseRadius;
+      double r_in = baseRadius - thickness;
+      if (r_in < 0) r_in = 0;
+      const double term_r = r_out * r_out + r_in * r_in;
+      const double I_zz = 0.5 * term_r;
+      const double I_xx = (1.0 / 12.0) * (3.0 * term_r + height * height);
+      if (I_xx > 1e-6) ix = iy = 1.0 / I_xx;
+      if (I_zz > 1e-6) iz = 1.0 / I_zz;
+    } else if (type == SHAPE_BOX) {
+      const double L = 2.0 * baseRadius;
+      const double I = (1.0 / 6.0) * L * L;
+      if (I > 1e-6) ix = iy = iz = 1.0 / I;
+    }
+    defaultInvI[0] = ix;
+    defaultInvI[1] = iy;
+    defaultInvI[2] = iz;
+  }
+
+  // ---------------------------------------------------------------- packed-field access
+  // field ids: 0 pos, 1 vel, 2 angVel, 3 quat, 4 invI, 5 invMass, 6 extForce
+  void fieldLayout(int f, double*& base, int& stride, int& off, int& comps) {
+    switch (f) {
+      case 0: base = (double*)pos.p; stride = 4; off = 0; comps = 3; break;
+      case 1: base = (double*)vw0.p; stride = 8; off = 0; comps = 3; break;
+      case 2: base = (double*)vw0.p; stride = 8; off = 4; comps = 3; break;
+      case 3: base = (double*)bc.p; stride = 8; off = 0; comps = 4; break;
+      case 4: base = (double*)bc.p; stride = 8; off = 4; comps = 3; break;
+      case 5: base = (double*)bc.p; stride = 8; off = 7; comps = 1; break;
+      default: base = (double*)extForce.p; stride = 4; off = 0; comps = 3; break;
+    }
+  }
+  void setField(int f, const double* src, int n) {
+    n = std::min(n, numReal);
+    if (n <= 0) return;
+    double* base;
+    int stride, off, comps;
+    fieldLayout(f, base, stride, off, comps);
+    std::vector<double> h((size_t)numReal * stride);
+    readN(h.data(), base, h.size());
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < comps; ++c) h[(size_t)i * stride + off + c] = src[(size_t)i * comps + c];
+    writeN(base, h.data(), h.size());
+  }
+  void getField(int f, double* out, int n) {
+    n = std::min(n, numReal);
+    if (n <= 0) return;
+    double* base;
+    int stride, off, comps;
+    fieldLayout(f, base, stride, off, comps);
+    std::vector<double> h((size_t)n * stride);
+    readN(h.data(), base, h.size());
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < comps; ++c) out[(size_t)i * comps + c] = h[(size_t)i * stride + off + c];
+  }
+
+  // src/sim.hpp:1290-1330
+  void setPositions(const double* xyz, int n) {
+    if (n < 0) throw ArgError("negative particle count");
+    // headroom for the periodic ghosts (src/periodicity.hpp:25-38); grown on demand
+    const long est = (long)n + (long)n / 4 + 4096;
+    ensureSlots((int)std::max<long>(est, capacity), numReal);
+    numReal = n;
+    numParticles = n;
+    std::vector<double4> P((size_t)n);
+    std::vector<BodyC> B((size_t)n);
+    std::vector<Body64> V((size_t)n);
+    std::vector<double> one((size_t)n, 1.0);
+    for (int i = 0; i < n; ++i) {
+      P[i] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0);
+      B[i].q = make_double4(0, 0, 0, 1);
+      B[i].iim = make_double4(defaultInvI[0], defaultInvI[1], defaultInvI[2], 1.0);
+      V[i].v = make_double4(0, 0, 0, 0);
+      V[i].w = make_double4(0, 0, 0, 0);
+    }
+    writeN(pos.p, P.data(), n);
+    writeN(bc.p, B.data(), n);
+    writeN(vw0.p, V.data(), n);
+    writeN(scale.p, one.data(), n);
+    writeN(targetScale.p, one.data(), n);
+  }
+
+  // ---------------------------------------------------------------- the substep
+  void broadAndNarrow(double margin) {
+    Params P = params(margin, 0.0);
+    // bounds of the box centres -> uniform grid with cell edge >= 2 * max half-width
+    const unsigned long long init[7] = {~0ull, ~0ull, ~0ull, 0, 0, 0, 0};
+    writeN(red.p, init, 7);
+    k_bp_bounds<<<nblk(numParticles), TPB, 0, st>>>(D, P, red.p);
+    unsigned long long rk[7];
+    readN(rk, red.p, 7);
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = rk[d] == ~0ull ? 0.0 : ordVal(rk[d]);
+      hi[d] = rk[3 + d] == 0 ? 0.0 : ordVal(rk[3 + d]);
+      if (lo[d] > hi[d]) lo[d] = hi[d] = 0.0;
+    }
+    const double maxB = rk[6] ? ordVal(rk[6]) : 0.0;
+    Grid G{};
+    G.lo = d3{lo[0], lo[1], lo[2]};
+    G.h = 2.0 * maxB * 1.0001 + 1e-300;
+    double cells;
+    for (;;) {
+      cells = 1.0;
+      for (int d = 0; d < 3; ++d) {
+        double nd = std::floor((hi[d] - lo[d]) / G.h) + 1.0;
+        if (!(nd >= 1.0)) nd = 1.0;
+        if (nd > 1e9) nd = 1e9;
+        G.dim[d] = (int)nd;
+        cells *= nd;
+      }
+      if (cells <= 4.0 * numParticles + 64.0) break;
+      G.h *= 1.5;
+    }
+    const int ncell = G.dim[0] * G.dim[1] * G.dim[2];
+    cellCnt.ensure(ncell + 1, st);
+    cellStart.ensure(ncell + 1, st);
+    cellOf.ensure(numParticles, st);
+    order.ensure(numParticles, st);
+    sbox.ensure(numParticles, st);
+    pairCnt.ensure(numReal + 1, st);
+    pairStart.ensure(numReal + 1, st);
+    refreshDev();
+    zero(cellCnt.p, ncell);
+    k_cell_count<<<nblk(numParticles), TPB, 0, st>>>(D, P, G);
+    scan(cellCnt.p, cellStart.p, ncell);
+    zero(cellCnt.p, ncell);
+    k_cell_fill<<<nblk(numParticles), TPB, 0, st>>>(D, P);
+    k_bp_query<false><<<nblk(numReal), TPB, 0, st>>>(D, P, G);
+    np = scan(pairCnt.p, pairStart.p, numReal);
+    pairs.ensure(np + 1, st);
+    refreshDev();
+    k_bp_query<true><<<nblk(numReal), TPB, 0, st>>>(D, P, G);
+    HIPCHECK(hipGetLastError());
+    mark(ST_NARROW);
+
+    // narrowphase: counts, scan, write (pairs first, then planes per real)
+    const int nslots = np + numReal;
+    ncnt.ensure(nslots + 1, st);
+    noff.ensure(nslots + 1, st);
+    refreshDev();
+    k_np_pairs<false><<<nblk(np), TPB, 0, st>>>(D, P, np);
+    k_np_planes<false><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
+    nc = scan(ncnt.p, noff.p, nslots);
+    cAB.ensure(nc + 1, st);
+    cN.ensure(nc + 1, st);
+    cRA.ensure(nc + 1, st);
+    cRB.ensure(nc + 1, st);
+    cSlot.ensure(nc + 1, st);
+    refreshDev();
+    zero(bits.p, 1);
+    k_np_pairs<true><<<nblk(np), TPB, 0, st>>>(D, P, np);
+    k_np_planes<true><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
+    HIPCHECK(hipGetLastError());
+  }
+
+  void buildManifolds() {
+    Params P = params(0.0, 0.0);
+    mCnt.ensure(numReal + 1, st);
+    mStart.ensure(numReal + 1, st);
+    refreshDev();
+    k_mcount<<<nblk(numReal), TPB, 0, st>>>(D, P, np);
+    nm = scan(mCnt.p, mStart.p, numReal);
+    const size_t M = nm + 1;
+    mAB.ensure(M, st); mReal.ensure(M, st); mRange.ensure(M, st); mNs.ensure(M, st);
+    mTA.ensure(M, st); mTB.ensure(M, st); mRA.ensure(M, st); mRB.ensure(M, st);
+    mNp.ensure(M, st); mKey.ensure(M, st); mCode.ensure(M, st); mColor.ensure(M, st);
+    vRowOf.ensure(M, st);
+    refreshDev();
+    k_mbuild<<<nblk(numReal), TPB, 0, st>>>(D, P, np);
+    k_mreduce<<<nblk(nm), TPB, 0, st>>>(D, nm);
+    HIPCHECK(hipGetLastError());
+  }
+
+  // Jones-Plassmann rounds (host-driven like the reference); returns the uncoloured leftovers
+  int jpColor(const int2* ends, int* colour, int numEdges, int numBodies, int& rounds,
+              long long& uncol) {
+    winner.ensure(numBodies + 1, st);
+    mask.ensure(numBodies + 1, st);
+    refreshDev();
+    zero(mask.p, numBodies);
+    int remaining = read1(counters.p + 1);
+    uncol = 0;
+    rounds = 0;
+    int prevRemaining = -1;
+    const int maxRounds = numBodies + 2;
+    for (int round = 0; round < maxRounds && remaining > 0; ++round) {
+      uncol += remaining;
+      zero(winner.p, numBodies);
+      zero(counters.p, 1);
+      k_jp_contend<<<nblk(numEdges), TPB, 0, st>>>(ends, colour, numEdges, winner.p);
+      k_jp_commit<<<nblk(numEdges), TPB, 0, st>>>(ends, colour, numEdges, winner.p, mask.p,
+                                                  counters.p);
+      HIPCHECK(hipGetLastError());
+      const int rem = read1(counters.p);
+      ++rounds;
+      if (rem == prevRemaining) break;
+      prevRemaining = rem;
+      remaining = rem;
+    }
+    return remaining > 0 ? remaining : 0;
+  }
+  // colour classes: counts, starts, rows; returns maxColour+1 and fills colStartHost
+  int colourClasses(const int* colour, int numEdges, int* perm, int* rowOf,
+                    std::vector<int>& starts) {
+    zero(colCnt.p, 64);
+    zero(colFill.p, 64);
+    k_color_hist<<<nblk(numEdges), TPB, 0, st>>>(colour, numEdges, colCnt.p);
+    int cnt[64];
+    readN(cnt, colCnt.p, 64);
+    starts.assign(65, 0);
+    int maxc = -1;
+    for (int c = 0; c < 64; ++c) {
+      starts[c + 1] = starts[c] + cnt[c];
+      if (cnt[c] > 0) maxc = c;
+    }
+    writeN(colStart.p, starts.data(), 65);
+    k_color_perm<<<nblk(numEdges), TPB, 0, st>>>(colour, numEdges, colStart.p, colFill.p, perm,
+                                                 rowOf);
+    HIPCHECK(hipGetLastError());
+    return maxc + 1;
+  }
+
+  double readMaxApproach() {
+    return bitsd(read1(bits.p + 1));
+  }
+  void velocitySweep(const std::vector<int>& vs, int numColors, int mode, double thr) {
+    Params P = params(0.0, 0.0);
+    for (int c = 0; c < numColors; ++c) {
+      const int b = vs[c], e = vs[c + 1];
+      if (e <= b) continue;
+      if (c == 62) {
+        k_vsweep_serial<<<1, 1, 0, st>>>(D, P, nm, c, mode, thr);
+      } else {
+        if (mode == 1) kernBegin();
+        k_vsweep<<<nblk(e - b), TPB, 0, st>>>(D, P, b, e, mode, thr);
+        if (mode == 1) kernEnd(K_VSWEEP, e - b);
+      }
+    }
+    HIPCHECK(hipGetLastError());
+  }
+
+  void demStep() {
+    std::memset(&stats, 0, sizeof stats);
+    if (stabMode == 2 || stabMode == 4)
+      throw std::runtime_error("stabilization modes 'multilevel' and 'ordered' are not implemented");
+    if (!useGS)
+      throw std::runtime_error("the Jacobi (velocity_use_gs=False) solve is not implemented on HIP");
+    const bool frictionOn = frictionDynamic > 0.0;
+    const bool usePersist = (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0);
+    if (frictionOn && !usePersist)
+      throw std::runtime_error(
+          "the legacy count-averaged friction path (gravity 0 with friction) is not implemented "
+          "on HIP");
+    mark(ST_PREDICT);
+    refreshDev();
+    // growth ramp, src/sim.hpp:100-108
+    if (growthFactor != -1.0 && growthRate != 0.0) {
+      growthFactor *= std::exp(growthRate * dt);
+      if (growthFactor > 1.0) growthFactor = 1.0;
+    }
+    if (growthFactor > 0.0 && numReal > 0)
+      k_growth<<<nblk(numReal), TPB, 0, st>>>(scale.p, targetScale.p, growthFactor, numReal);
+    // maxOwnedRadius, src/sim.hpp:61-70
+    double maxRad = globalScale * baseRadius;
+    if (numReal > 0) {
+      zero(red.p, 1);
+      k_max_scale<<<nblk(numReal), TPB, 0, st>>>(scale.p, numReal, red.p);
+      const unsigned long long k = read1(red.p);
+      const double mx = k ? ordVal(k) : -std::numeric_limits<double>::max();
+      maxRad = mx * globalScale * baseRadius;
+    }
+    const double margin = 0.1 * maxRad;
+    numParticles = numReal;
+    Params P = params(margin, maxRad);
+    k_predict<<<nblk(numReal), TPB, 0, st>>>(D, P);
+    // periodic ghosts, src/periodicity.hpp:43-94
+    ghostCnt.ensure(numReal + 1, st);
+    ghostOff.ensure(numReal + 1, st);
+    k_ghost_count<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostCnt.p);
+    const int G = scan(ghostCnt.p, ghostOff.p, numReal);
+    if ((size_t)(numReal + G) > pos.cap) ensureSlots(numReal + G, numReal);
+    refreshDev();
+    k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostOff.p);
+    numParticles = numReal + G;
+    HIPCHECK(hipGetLastError());
+    mark(ST_BROAD);
+    broadAndNarrow(margin);
+    mark(ST_MANIFOLD);
+    buildManifolds();
+    mark(ST_COLOR_V);
+
+    // manifold colouring, src/solver_velocity.hpp:277-374
+    zero(counters.p, 2);
+    vPerm.ensure(nm + 1, st);
+    refreshDev();
+    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(D, nm, counters.p + 1);
+    std::vector<int> vs;
+    int numColors = 0;
+    int velLeftover = 0;
+    if (nm > 0 && numReal > 0) {
+      velLeftover = jpColor(mReal.p, mColor.p, nm, numReal, stats.rounds_vel, stats.uncolored_vel);
+      numColors = colourClasses(mColor.p, nm, vPerm.p, vRowOf.p, vs);
+    } else {
+      vs.assign(65, 0);
+    }
+    nRowsV = vs[numColors];
+    mark(ST_PREP);
+
+    const bool usePGS = usePersist && useGS;
+    const double gMagP = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    const d3 gHat = usePersist ? d3{g[0] / gMagP, g[1] / gMagP, g[2] / gMagP} : d3{0, 0, 0};
+    const size_t R = nRowsV + 1;
+    vAB.ensure(R, st); vN.ensure(R, st); vTA.ensure(R, st); vTB.ensure(R, st); vRA.ensure(R, st);
+    vRB.ensure(R, st); vVt0.ensure(R, st); vLamT.ensure(R, st); vVn0.ensure(R, st);
+    vLam.ensure(R, st); vFlag.ensure(R, st);
+    refreshDev();
+    if (nm > 0) k_mkeys<<<nblk(nm), TPB, 0, st>>>(D, nm, gHat);
+    if (nRowsV > 0)
+      k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(D, P, nRowsV, prevCount, usePGS ? 1 : 0);
+    if (usePGS) {
+      // grounded levels, double-buffered (src/contact_preprocessing.hpp:213-253)
+      k_ground_decay<<<nblk(numReal), TPB, 0, st>>>(ground.p, numReal, 8);
+      int* ga = ground.p;
+      int* gb = ground2.p;
+      for (int s = 0; s < 8; ++s) {
+        k_copy_i32<<<nblk(numReal), TPB, 0, st>>>(gb, ga, numReal);
+        if (nm > 0) k_ground_sweep<<<nblk(nm), TPB, 0, st>>>(D, nm, ga, gb);
+        std::swap(ga, gb);
+      }
+      // 8 swaps: the result is back in `ground`
+      velocitySweep(vs, numColors, 0, 0.0);
+    }
+    HIPCHECK(hipGetLastError());
+    mark(ST_VSWEEP);
+    for (int it = 0; it < velIters; ++it) {
+      const double vRest = 2.0 * dt * std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+      zero(bits.p + 1, 1);
+      velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest);
+      if (velLeftover > 0) {
+        prepareFallback();
+        Params PF = params(margin, maxRad);
+        k_vel_jacobi_accum<<<1, 1, 0, st>>>(D, PF, nm, vRest);
+        k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+      }
+      stats.sweeps_vel++;
+      maxApproach = readMaxApproach();
+      if (maxApproach <= (usePGS ? 0.02 * vRest : vRest)) break;
+    }
+    mark(ST_STAB);
+    if (usePGS) {
+      const double vRestS = 2.0 * dt * gMagP;
+      if (stabMode != 0 && maxApproach > vRestS) {
+        stats.stab_ran = 1;
+        if (stabMode == 1 && nRowsV > 0)
+          k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
+                                                     8.0 * dt * gMagP);
+        const int cap = stabMode == 1 ? 2 * velIters : 256;
+        for (int it = 0; it < cap; ++it) {
+          zero(bits.p + 1, 1);
+          velocitySweep(vs, numColors, 1, vRestS);
+          stats.sweeps_stab++;
+          maxApproach = readMaxApproach();
+          if (maxApproach <= vRestS) break;
+        }
+      }
+    }
+    mark(ST_LEDGER);
+    if (usePGS && nm > 0) {
+      // commitPairKeysLambdaKokkos, src/contact_preprocessing.hpp:160-192
+      bucketCnt.ensure(numReal + 2, st);
+      bucketStart.ensure(numReal + 2, st);
+      lKey.ensure(nm + 1, st);
+      lIdx.ensure(nm + 1, st);
+      lLam.ensure(nm + 1, st);
+      lLamT.ensure(nm + 1, st);
+      refreshDev();
+      zero(bucketCnt.p, numReal + 1);
+      k_ledger_count<<<nblk(nm), TPB, 0, st>>>(D, nm, numReal);
+      scan(bucketCnt.p, bucketStart.p, numReal + 1);
+      zero(bucketCnt.p, numReal + 1);
+      k_ledger_scatter<<<nblk(nm), TPB, 0, st>>>(D, nm, numReal);
+      k_ledger_sort<<<nblk(numReal), TPB, 0, st>>>(D, numReal);
+      k_ledger_values<<<nblk(nm), TPB, 0, st>>>(D, nm);
+      HIPCHECK(hipGetLastError());
+      prevCount = nm;
+    } else if (usePGS) {
+      prevCount = 0;
+    }
+    mark(ST_REINT);
+    P = params(margin, maxRad);
+    k_reintegrate<<<nblk(numParticles), TPB, 0, st>>>(D, P);
+    mark(ST_COLOR_P);
+
+    // contact colouring over raw slots, src/solver_position.hpp:155-241
+    cColor.ensure(nc + 1, st);
+    pPerm.ensure(nc + 1, st);
+    pRowOf.ensure(nc + 1, st);
+    refreshDev();
+    zero(counters.p, 2);
+    std::vector<int> ps;
+    int numPosColors = 0;
+    int posLeftover = 0;
+    if (nc > 0 && numParticles > 0) {
+      k_jp_init_contacts<<<nblk(nc), TPB, 0, st>>>(cColor.p, nc, counters.p + 1);
+      posLeftover = jpColor(cAB.p, cColor.p, nc, numParticles, stats.rounds_pos,
+                            stats.uncolored_pos);
+      numPosColors = colourClasses(cColor.p, nc, pPerm.p, pRowOf.p, ps);
+    } else {
+      ps.assign(65, 0);
+    }
+    const int nRowsP = ps[numPosColors];
+    pAB.ensure(nRowsP + 1, st);
+    pN.ensure(nRowsP + 1, st);
+    pRA.ensure(nRowsP + 1, st);
+    pRB.ensure(nRowsP + 1, st);
+    pLam.ensure(nRowsP + 1, st);
+    refreshDev();
+    if (nRowsP > 0) k_prows<<<nblk(nRowsP), TPB, 0, st>>>(D, nRowsP);
+    mark(ST_PSWEEP);
+    const double posTol = 1e-4 * baseRadius * globalScale;
+    for (int it = 0; it < posIters; ++it) {
+      zero(bits.p, 1);
+      for (int c = 0; c < numPosColors; ++c) {
+        const int b = ps[c], e = ps[c + 1];
+        if (e <= b) continue;
+        if (c == 62) {
+          k_psweep_serial<<<1, 1, 0, st>>>(D, nc, c);
+        } else {
+          kernBegin();
+          k_psweep<<<nblk(e - b), TPB, 0, st>>>(D, b, e);
+          kernEnd(K_PSWEEP, e - b);
+        }
+      }
+      if (posLeftover > 0) {
+        prepareFallback();
+        k_pos_jacobi_accum<<<1, 1, 0, st>>>(D, nc);
+        k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+      }
+      HIPCHECK(hipGetLastError());
+      stats.sweeps_pos++;
+      maxOverlap = bitsd(read1(bits.p));
+      if (maxOverlap < posTol) break;
+    }
+    if (posIters <= 0) maxOverlap = bitsd(read1(bits.p));
+    mark(ST_COMMIT);
+    if (usePGS && nm > 0) k_commit_pos_impulse<<<nblk(nm), TPB, 0, st>>>(D, nm, dt);
+    k_final_commit<<<nblk(numReal), TPB, 0, st>>>(D, P);
+    if (thermoTau > 0.0 && dt > 0.0) applyThermostat();
+    HIPCHECK(hipGetLastError());
+    mark(ST_COUNT);
+    kernCollect();
+    if (timing) {
+      sync();
+      for (int k = 0; k < ST_COUNT; ++k) {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        stageMs[k] += ms;
+      }
+    }
+    stats.num_real = numReal;
+    stats.num_particles = numParticles;
+    stats.num_pairs = np;
+    stats.num_contacts = nc;
+    stats.num_manifolds = nm;
+    stats.active_manifolds = nRowsV;
+    stats.num_colors_vel = numColors;
+    stats.num_colors_pos = numPosColors;
+    stats.max_overlap = maxOverlap;
+    stats.max_approach = maxApproach;
+  }
+
+  // applyThermostatKokkos, src/integration.hpp:229-271 (block partials summed in block order)
+  void applyThermostat() {
+    const int nb = nblk(numReal);
+    thermo.ensure(2 * nb + 2, st);
+    refreshDev();
+    Params P = params(0.0, 0.0);
+    k_thermo_partial<<<nb, TPB, 0, st>>>(D, P, thermo.p);
+    std::vector<double> part(2 * nb);
+    readN(part.data(), thermo.p, part.size());
+    double ke[2] = {0.0, 0.0};
+    for (int b = 0; b < nb; ++b) {
+      ke[0] += part[2 * b];
+      ke[1] += part[2 * b + 1];
+    }
+    const double ndof = 3.0 * numReal;
+    double lam[2];
+    for (int k = 0; k < 2; ++k) {
+      lam[k] = 1.0;
+      if (ndof <= 0 || thermoKB <= 0) continue;
+      const double Tcur = (2.0 * ke[k]) / (ndof * thermoKB);
+      if (Tcur < 1e-6) continue;
+      double f = 1.0 + (dt / thermoTau) * (thermoTemp / Tcur - 1.0);
+      if (f < 0.0) f = 0.0;
+      lam[k] = std::sqrt(f);
+    }
+    k_thermo_scale<<<nb, TPB, 0, st>>>(D, P, lam[0], lam[1]);
+  }
+
+  // computeOverlapsKokkos, src/sim.hpp:494-541
+  double computeOverlaps() {
+    refreshDev();
+    double maxRad = globalScale * baseRadius;
+    if (numReal > 0) {
+      zero(red.p, 1);
+      k_max_scale<<<nblk(numReal), TPB, 0, st>>>(scale.p, numReal, red.p);
+      const unsigned long long k = read1(red.p);
+      maxRad = (k ? ordVal(k) : -std::numeric_limits<double>::max()) * globalScale * baseRadius;
+    }
+    const double margin = 0.1 * maxRad;
+    numParticles = numReal;
+    Params P = params(margin, maxRad);
+    k_overlap_prep<<<nblk(numReal), TPB, 0, st>>>(D, P);
+    ghostCnt.ensure(numReal + 1, st);
+    ghostOff.ensure(numReal + 1, st);
+    k_ghost_count<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostCnt.p);
+    const int G = scan(ghostCnt.p, ghostOff.p, numReal);
+    if ((size_t)(numReal + G) > pos.cap) ensureSlots(numReal + G, numReal);
+    refreshDev();
+    k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostOff.p);
+    numParticles = numReal + G;
+    broadAndNarrow(margin);
+    numParticles = numReal;
+    maxOverlap = bitsd(read1(bits.p));
+    return maxOverlap;
+  }
+};
+
+// ================================================================================================
+// C-ABI
+// ================================================================================================
+namespace {
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const ArgError& e) {
+    g_error = e.what();
+    return -2;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -1;
+  }
+}
+void writeLammps(const std::string& filename, int step, const double* pos, const double* vel,
+                 const double* quat, const double* radii, int n, const double* bmn,
+                 const double* bmx, bool pbc) {
+  // writeLammpsDump, src/io.hpp:23-82
+  double bmin[3], bmax[3];
+  if (bmn && bmx) {
+    for (int d = 0; d < 3; ++d) {
+      bmin[d] = bmn[d];
+      bmax[d] = bmx[d];
+    }
+  } else if (n == 0) {
+    for (int d = 0; d < 3; ++d) bmin[d] = bmax[d] = 0.0;
+  } else {
+    for (int d = 0; d < 3; ++d) {
+      bmin[d] = std::numeric_limits<double>::max();
+      bmax[d] = -std::numeric_limits<double>::max();
+    }
+    for (int i = 0; i < n; ++i) {
+      const double r = radii ? radii[i] : 0.0;
+      for (int d = 0; d < 3; ++d) {
+        bmin[d] = std::min(bmin[d], pos[3 * i + d] - r);
+        bmax[d] = std::max(bmax[d], pos[3 * i + d] + r);
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      const double m = 0.05 * (bmax[d] - bmin[d]);
+      bmin[d] -= m;
+      bmax[d] += m;
+    }
+  }
+  std::ofstream file(filename);
+  if (!file) throw std::runtime_error("Could not open file for writing: " + filename);
+  file << "ITEM: TIMESTEP\n" << step << "\n";
+  file << "ITEM: NUMBER OF ATOMS\n" << n << "\n";
+  file << "ITEM: BOX BOUNDS " << (pbc ? "pp pp pp" : "ff ff ff") << "\n";
+  for (int d = 0; d < 3; ++d) file << bmin[d] << " " << bmax[d] << "\n";
+  file << "ITEM: ATOMS id type x y z radius vx vy vz qw qx qy qz\n";
+  file << std::fixed << std::setprecision(6);
+  for (int i = 0; i < n; ++i) {
+    const double r = radii ? radii[i] : 1.0;
+    const double vx = vel ? vel[3 * i] : 0.0, vy = vel ? vel[3 * i + 1] : 0.0,
+                 vz = vel ? vel[3 * i + 2] : 0.0;
+    double qx = 0, qy = 0, qz = 0, qw = 1;
+    if (quat) {
+      qx = quat[4 * i];
+      qy = quat[4 * i + 1];
+      qz = quat[4 * i + 2];
+      qw = quat[4 * i + 3];
+    }
+    file << (i + 1) << " 1 " << pos[3 * i] << " " << pos[3 * i + 1] << " " << pos[3 * i + 2]
+         << " " << r << " " << vx << " " << vy << " " << vz << " " << qw << " " << qx << " " << qy
+         << " " << qz << "\n";
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* pd_last_error(void) { return g_error.c_str(); }
+
+const char* pd_execution_space(void) {
+  static std::string name;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    name = std::string("HIP(") + prop.gcnArchName + ")";
+  else
+    name = "HIP(no device)";
+  return name.c_str();
+}
+
+pd_sim* pd_create(int capacity) {
+  if (capacity < 0) {
+    g_error = "capacity must be >= 0";
+    return nullptr;
+  }
+  try {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+      throw std::runtime_error("no HIP device available (the substep runs only on the GPU)");
+    return new pd_sim(capacity);
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return nullptr;
+  }
+}
+void pd_destroy(pd_sim* s) { delete s; }
+int pd_synchronize(pd_sim* s) {
+  return guard([&] { s->sync(); });
+}
+
+int pd_initialize_shape(pd_sim* s, int t, double r, double h, double th) {
+  return guard([&] { s->initializeShape(t, r, h, th); });
+}
+int pd_set_shell_points(pd_sim* s, const double* pts, int n) {
+  return guard([&] {
+    if (n < 0) throw ArgError("set_shell_points: negative count");
+    if (s->shapeType == SHAPE_SPHERE && n > 0)
+      throw ArgError("set_shell_points: the analytic sphere uses a single centre probe");
+    s->shellHost.assign(pts, pts + 3 * (size_t)n);
+    s->uploadShell();
+  });
+}
+int pd_get_shell_points(pd_sim* s, double* out, int cap) {
+  const int n = (int)(s->shellHost.size() / 3);
+  if (out)
+    for (int i = 0; i < n && i < cap; ++i)
+      for (int c = 0; c < 3; ++c) out[3 * i + c] = s->shellHost[3 * i + c];
+  return n;
+}
+int pd_set_domain(pd_sim* s, double lx, double ly, double lz, int px, int py, int pz) {
+  s->dmin[0] = s->dmin[1] = s->dmin[2] = 0.0;
+  s->dmax[0] = lx; s->dmax[1] = ly; s->dmax[2] = lz;
+  s->dsize[0] = lx; s->dsize[1] = ly; s->dsize[2] = lz;
+  s->per[0] = px != 0; s->per[1] = py != 0; s->per[2] = pz != 0;
+  return 0;
+}
+int pd_set_domain_minmax(pd_sim* s, double a, double b, double c, double d, double e, double f) {
+  s->dmin[0] = a; s->dmin[1] = b; s->dmin[2] = c;
+  s->dmax[0] = d; s->dmax[1] = e; s->dmax[2] = f;
+  s->dsize[0] = d - a; s->dsize[1] = e - b; s->dsize[2] = f - c;
+  return 0;
+}
+int pd_enable_periodicity(pd_sim* s, int x, int y, int z) {
+  s->per[0] = x != 0; s->per[1] = y != 0; s->per[2] = z != 0;
+  return 0;
+}
+int pd_get_domain(pd_sim* s, double* mn, double* mx) {
+  for (int d = 0; d < 3; ++d) {
+    mn[d] = s->dmin[d];
+    mx[d] = s->dmax[d];
+  }
+  return 0;
+}
+int pd_set_gravity(pd_sim* s, double gx, double gy, double gz) {
+  s->g[0] = gx; s->g[1] = gy; s->g[2] = gz;
+  return 0;
+}
+int pd_set_thermostat(pd_sim* s, double T, double tau, double kB) {
+  s->thermoTemp = T; s->thermoTau = tau; s->thermoKB = kB;
+  return 0;
+}
+int pd_set_solver_iterations(pd_sim* s, int pos, int vel) {
+  s->posIters = pos; s->velIters = vel;
+  return 0;
+}
+int pd_set_velocity_use_gs(pd_sim* s, int use_gs) {
+  s->useGS = use_gs != 0;
+  return 0;
+}
+int pd_set_stabilization_mode(pd_sim* s, int mode) {
+  return guard([&] {
+    if (mode < 0 || mode > 4) throw ArgError("stabilization mode must be 0..4");
+    s->stabMode = mode;
+  });
+}
+int pd_set_global_scale(pd_sim* s, double v) { s->globalScale = v; return 0; }
+int pd_set_dt(pd_sim* s, double v) { s->dt = v; return 0; }
+int pd_set_material_params(pd_sim* s, double en, double et, double mu) {
+  s->restitutionNormal = en; s->frictionDynamic = mu; s->restitutionTangent = et;
+  return 0;
+}
+int pd_set_material_ids(pd_sim* s, const int* ids, int n) {
+  return guard([&] {
+    const int m = std::min(n, s->numReal);
+    if (m <= 0) return;
+    std::vector<unsigned char> h(m);
+    for (int i = 0; i < m; ++i) h[i] = (unsigned char)ids[i];
+    s->writeN(s->matId.p, h.data(), m);
+  });
+}
+int pd_set_pair_material(pd_sim* s, int a, int b, double e, double mu) {
+  return guard([&] {
+    if (a < 0 || b < 0 || a >= 8 || b >= 8) throw ArgError("material id out of range");
+    if (!s->hasPairTable) {
+      for (int i = 0; i < 64; ++i) {
+        s->pairTable[2 * i] = s->restitutionNormal;
+        s->pairTable[2 * i + 1] = s->frictionDynamic;
+      }
+      s->hasPairTable = 1;
+    }
+    s->pairTable[(a * 8 + b) * 2] = e; s->pairTable[(a * 8 + b) * 2 + 1] = mu;
+    s->pairTable[(b * 8 + a) * 2] = e; s->pairTable[(b * 8 + a) * 2 + 1] = mu;
+    s->writeN(s->pairTableD.p, s->pairTable, 128);
+  });
+}
+int pd_add_plane(pd_sim* s, double px, double py, double pz, double nx, double ny, double nz) {
+  if (s->numPlanes < 8) {
+    s->planeP[s->numPlanes][0] = px; s->planeP[s->numPlanes][1] = py;
+    s->planeP[s->numPlanes][2] = pz; s->planeN[s->numPlanes][0] = nx;
+    s->planeN[s->numPlanes][1] = ny; s->planeN[s->numPlanes][2] = nz;
+    s->numPlanes++;
+  }
+  return 0;
+}
+int pd_set_positions(pd_sim* s, const double* xyz, int n) {
+  return guard([&] { s->setPositions(xyz, n); });
+}
+int pd_set_velocities(pd_sim* s, const double* v, int n) {
+  return guard([&] { s->setField(1, v, n); });
+}
+int pd_set_external_forces(pd_sim* s, const double* f, int n) {
+  return guard([&] { s->setField(6, f, n); });
+}
+int pd_clear_external_forces(pd_sim* s) {
+  return guard([&] {
+    s->zero(s->extForce.p, s->extForce.cap);
+    s->sync();
+  });
+}
+int pd_set_quaternions(pd_sim* s, const double* q, int n) {
+  return guard([&] { s->setField(3, q, n); });
+}
+int pd_set_angular_velocities(pd_sim* s, const double* w, int n) {
+  return guard([&] { s->setField(2, w, n); });
+}
+int pd_set_inv_inertia(pd_sim* s, const double* ii, int n) {
+  return guard([&] { s->setField(4, ii, n); });
+}
+int pd_set_inv_mass(pd_sim* s, const double* im, int n) {
+  return guard([&] { s->setField(5, im, n); });
+}
+int pd_set_scales(pd_sim* s, const double* sc, int n) {
+  // src/sim.hpp:1340-1346: target := given, scale := target
+  return guard([&] {
+    const int m = std::min(n, s->numReal);
+    if (m > 0) s->writeN(s->targetScale.p, sc, m);
+    HIPCHECK(hipMemcpyAsync(s->scale.p, s->targetScale.p, s->scale.cap * sizeof(double),
+                            hipMemcpyDeviceToDevice, s->st));
+    s->sync();
+  });
+}
+int pd_set_scales_uniform(pd_sim* s, double v) {
+  // src/sim.hpp:1332-1338
+  return guard([&] {
+    std::vector<double> h(s->numReal, v);
+    if (s->numReal > 0) s->writeN(s->scale.p, h.data(), s->numReal);
+    HIPCHECK(hipMemcpyAsync(s->targetScale.p, s->scale.p, s->scale.cap * sizeof(double),
+                            hipMemcpyDeviceToDevice, s->st));
+    s->sync();
+  });
+}
+int pd_set_growth_params(pd_sim* s, double rate, double nf) {
+  // src/sim.hpp:1417-1425
+  return guard([&] {
+    if (s->growthFactor == -1.0) s->growthFactor = (nf > 0.0) ? nf : 0.01;
+    else if (nf > 0.0) s->growthFactor = nf;
+    s->growthRate = rate;
+    if (s->growthFactor > 0.0 && s->numReal > 0) {
+      k_growth<<<nblk(s->numReal), TPB, 0, s->st>>>(s->scale.p, s->targetScale.p,
+                                                    s->growthFactor, s->numReal);
+      s->sync();
+    }
+  });
+}
+double pd_get_growth_factor(pd_sim* s) { return s->growthFactor; }
+double pd_get_growth_rate(pd_sim* s) { return s->growthRate; }
+int pd_num_particles(pd_sim* s) { return s->numReal; }
+int pd_get_positions(pd_sim* s, double* out, int n) {
+  guard([&] { s->getField(0, out, n); });
+  return s->numReal;
+}
+int pd_get_velocities(pd_sim* s, double* out, int n) {
+  guard([&] { s->getField(1, out, n); });
+  return s->numReal;
+}
+int pd_get_quaternions(pd_sim* s, double* out, int n) {
+  guard([&] { s->getField(3, out, n); });
+  return s->numReal;
+}
+int pd_get_angular_velocities(pd_sim* s, double* out, int n) {
+  guard([&] { s->getField(2, out, n); });
+  return s->numReal;
+}
+int pd_get_inv_inertia(pd_sim* s, double* out, int n) {
+  guard([&] { s->getField(4, out, n); });
+  return s->numReal;
+}
+int pd_get_inv_mass(pd_sim* s, double* out, int n) {
+  guard([&] { s->getField(5, out, n); });
+  return s->numReal;
+}
+int pd_get_scales(pd_sim* s, double* out, int n) {
+  guard([&] { s->readN(out, s->scale.p, std::min(n, s->numReal)); });
+  return s->numReal;
+}
+void* pd_get_device_pointer(pd_sim* s, const char* name) {
+  const std::string n(name ? name : "");
+  if (n == "pos") return s->pos.p;
+  if (n == "vel") return s->vw0.p;
+  if (n == "ext_force") return s->extForce.p;
+  if (n == "inv_mass") return ((double*)s->bc.p) + 7;
+  g_error = "unknown device array: " + n;
+  return nullptr;
+}
+
+int pd_step(pd_sim* s, double dt) {
+  return guard([&] {
+    s->dt = dt;
+    s->demStep();
+  });
+}
+int pd_step_n(pd_sim* s, double dt, int n) {
+  return guard([&] {
+    for (int k = 0; k < n; ++k) {
+      s->dt = dt;
+      s->demStep();
+    }
+  });
+}
+int pd_compute_overlaps(pd_sim* s, double* out) {
+  return guard([&] {
+    const double v = s->computeOverlaps();
+    if (out) *out = v;
+  });
+}
+int pd_num_contacts(pd_sim* s) { return s->nc; }
+int pd_num_manifolds(pd_sim* s) { return s->nm; }
+double pd_max_overlap(pd_sim* s) { return s->maxOverlap; }
+int pd_get_stats(pd_sim* s, pd_stats* out) {
+  *out = s->stats;
+  return 0;
+}
+
+int pd_get_pairs(pd_sim* s, int* out, int cap) {
+  if (out) {
+    const int n = std::min(cap, s->np);
+    std::vector<int2> h(n);
+    guard([&] { s->readN(h.data(), s->pairs.p, n); });
+    for (int k = 0; k < n; ++k) {
+      out[2 * k] = h[k].x;
+      out[2 * k + 1] = h[k].y;
+    }
+  }
+  return s->np;
+}
+int pd_get_contacts(pd_sim* s, int* ids, double* data, int cap) {
+  const int n = std::min(cap, s->nc);
+  if (n > 0 && (ids || data)) {
+    std::vector<int2> ab(n);
+    std::vector<double4> cn(n), ra(n), rb(n);
+    guard([&] {
+      s->readN(ab.data(), s->cAB.p, n);
+      s->readN(cn.data(), s->cN.p, n);
+      s->readN(ra.data(), s->cRA.p, n);
+      s->readN(rb.data(), s->cRB.p, n);
+    });
+    for (int k = 0; k < n; ++k) {
+      if (ids) {
+        ids[2 * k] = ab[k].x;
+        ids[2 * k + 1] = ab[k].y;
+      }
+      if (data) {
+        double* d = data + 10 * k;
+        d[0] = cn[k].x; d[1] = cn[k].y; d[2] = cn[k].z;
+        d[3] = ra[k].x; d[4] = ra[k].y; d[5] = ra[k].z;
+        d[6] = rb[k].x; d[7] = rb[k].y; d[8] = rb[k].z;
+        d[9] = cn[k].w;
+      }
+    }
+  }
+  return s->nc;
+}
+int pd_get_contact_colors(pd_sim* s, int* out, int cap) {
+  if (out) guard([&] { s->readN(out, s->cColor.p, std::min(cap, s->nc)); });
+  return s->nc;
+}
+int pd_get_manifolds(pd_sim* s, int* ids, double* data, int cap) {
+  const int n = std::min(cap, s->nm);
+  if (n > 0 && (ids || data)) {
+    std::vector<int2> ab(n);
+    std::vector<int> npt(n);
+    std::vector<double4> f[5];
+    for (auto& v : f) v.resize(n);
+    guard([&] {
+      s->readN(ab.data(), s->mAB.p, n);
+      s->readN(npt.data(), s->mNp.p, n);
+      s->readN(f[0].data(), s->mNs.p, n);
+      s->readN(f[1].data(), s->mTA.p, n);
+      s->readN(f[2].data(), s->mTB.p, n);
+      s->readN(f[3].data(), s->mRA.p, n);
+      s->readN(f[4].data(), s->mRB.p, n);
+    });
+    for (int k = 0; k < n; ++k) {
+      if (ids) {
+        ids[3 * k] = ab[k].x;
+        ids[3 * k + 1] = ab[k].y;
+        ids[3 * k + 2] = npt[k];
+      }
+      if (data)
+        for (int j = 0; j < 5; ++j) {
+          data[15 * k + 3 * j] = f[j][k].x;
+          data[15 * k + 3 * j + 1] = f[j][k].y;
+          data[15 * k + 3 * j + 2] = f[j][k].z;
+        }
+    }
+  }
+  return s->nm;
+}
+int pd_get_manifold_colors(pd_sim* s, int* out, int cap) {
+  if (out) guard([&] { s->readN(out, s->mColor.p, std::min(cap, s->nm)); });
+  return s->nm;
+}
+int pd_get_ledger(pd_sim* s, uint64_t* keys, double* lambda, double* lambdaT, double* posImp,
+                  int cap) {
+  const int n = std::min(cap, s->prevCount);
+  if (n > 0 && (keys || lambda || lambdaT || posImp)) {
+    std::vector<unsigned long long> k(n);
+    std::vector<double> l(n);
+    std::vector<double4> lt(n);
+    guard([&] {
+      s->readN(k.data(), s->lKey.p, n);
+      s->readN(l.data(), s->lLam.p, n);
+      s->readN(lt.data(), s->lLamT.p, n);
+    });
+    for (int i = 0; i < n; ++i) {
+      if (keys) keys[i] = k[i];
+      if (lambda) lambda[i] = l[i];
+      if (lambdaT) {
+        lambdaT[3 * i] = lt[i].x;
+        lambdaT[3 * i + 1] = lt[i].y;
+        lambdaT[3 * i + 2] = lt[i].z;
+      }
+      if (posImp) posImp[i] = lt[i].w;
+    }
+  }
+  return s->prevCount;
+}
+int pd_get_grounded(pd_sim* s, int* out, int n) {
+  if (out) guard([&] { s->readN(out, s->ground.p, std::min(n, s->numReal)); });
+  return s->numReal;
+}
+
+int pd_export_lammps(pd_sim* s, const char* filename, int step) {
+  // exportLammps, src/sim.hpp:1463-1474
+  return guard([&] {
+    const int n = s->numReal;
+    std::vector<double> p(3 * (size_t)n), v(3 * (size_t)n), q(4 * (size_t)n), sc(n), r(n);
+    s->getField(0, p.data(), n);
+    s->getField(1, v.data(), n);
+    s->getField(3, q.data(), n);
+    s->readN(sc.data(), s->scale.p, n);
+    for (int i = 0; i < n; ++i) r[i] = sc[i] * s->globalScale * s->baseRadius;
+    const bool pbc = s->per[0] || s->per[1] || s->per[2];
+    writeLammps(filename, step, p.data(), v.data(), q.data(), r.data(), n, nullptr, nullptr, pbc);
+  });
+}
+int pd_write_lammps_dump(const char* filename, int step, const double* pos, const double* vel,
+                         const double* quat, const double* radii, int n, const double* bmn,
+                         const double* bmx, int pbc) {
+  return guard([&] { writeLammps(filename, step, pos, vel, quat, radii, n, bmn, bmx, pbc != 0); });
+}
+int pd_write_vtp(pd_sim* s, const char* filename) {
+  // writeVtp, src/sim.hpp:1560-1601
+  return guard([&] {
+    const int n = s->numReal;
+    std::vector<double> p(3 * (size_t)n), v(3 * (size_t)n), sc(n);
+    s->getField(0, p.data(), n);
+    s->getField(1, v.data(), n);
+    s->readN(sc.data(), s->scale.p, n);
+    std::ofstream out(filename);
+    if (!out) throw std::runtime_error(std::string("Could not open file for writing: ") + filename);
+    out << "<?xml version=\"1.0\"?>\n";
+    out << "<VTKFile type=\"PolyData\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+    out << "  <PolyData>\n";
+    out << "    <Piece NumberOfPoints=\"" << n << "\" NumberOfVerts=\"0\" "
+        << "NumberOfLines=\"0\" NumberOfStrips=\"0\" NumberOfPolys=\"0\">\n";
+    out << "      <Points>\n";
+    out << "        <DataArray type=\"Float32\" Name=\"Position\" NumberOfComponents=\"3\" "
+           "format=\"ascii\">\n";
+    for (int i = 0; i < n; ++i) out << p[3 * i] << " " << p[3 * i + 1] << " " << p[3 * i + 2] << " ";
+    out << "\n        </DataArray>\n";
+    out << "      </Points>\n";
+    out << "      <PointData Scalars=\"Radius\">\n";
+    out << "        <DataArray type=\"Float32\" Name=\"Radius\" NumberOfComponents=\"1\" "
+           "format=\"ascii\">\n";
+    for (int i = 0; i < n; ++i) out << sc[i] * s->globalScale * s->baseRadius << " ";
+    out << "\n        </DataArray>\n";
+    out << "        <DataArray type=\"Float32\" Name=\"Velocity\" NumberOfComponents=\"3\" "
+           "format=\"ascii\">\n";
+    for (int i = 0; i < n; ++i) out << v[3 * i] << " " << v[3 * i + 1] << " " << v[3 * i + 2] << " ";
+    out << "\n        </DataArray>\n";
+    out << "      </PointData>\n";
+    out << "    </Piece>\n";
+    out << "  </PolyData>\n";
+    out << "</VTKFile>\n";
+    std::printf("Exported VTP: %s\n", filename);
+  });
+}
+
+// Host utility for seeded workloads (no reference counterpart): random sequential addition.
+// Spheres i = 0..n-1 (radius radii[i]) take candidates from `cand` (3 per candidate, in draw
+// order) until one overlaps no placed sphere; returns the number of candidates consumed, or -1 if
+// the stream ran out (then *placed tells how many were placed; call again with more candidates
+// and `start` = *placed to continue).
+long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, int start,
+                  const double* lo, const double* hi, double* out, int* placed) {
+  static thread_local std::vector<int> head, next;
+  double rmax = 0.0;
+  for (int i = 0; i < n; ++i) rmax = std::max(rmax, radii[i]);
+  const double h = 2.0 * rmax > 0.0 ? 2.0 * rmax : 1.0;
+  int dim[3];
+  for (int d = 0; d < 3; ++d) dim[d] = std::max(1, (int)((hi[d] - lo[d]) / h));
+  const long ncell = (long)dim[0] * dim[1] * dim[2];
+  head.assign(ncell, -1);
+  next.assign(n, -1);
+  auto cellOf = [&](const double* p, int c[3]) {
+    for (int d = 0; d < 3; ++d) c[d] = std::min(dim[d] - 1, std::max(0, (int)((p[d] - lo[d]) / h)));
+  };
+  for (int i = 0; i < start; ++i) {
+    int c[3];
+    cellOf(out + 3 * i, c);
+    const long k = ((long)c[2] * dim[1] + c[1]) * dim[0] + c[0];
+    next[i] = head[k];
+    head[k] = i;
+  }
+  long used = 0;
+  int i = start;
+  while (i < n) {
+    if (used >= ncand) {
+      *placed = i;
+      return -1;
+    }
+    const double* p = cand + 3 * used;
+    ++used;
+    const double r = radii[i];
+    int c[3];
+    cellOf(p, c);
+    bool ok = true;
+    for (int dz = -1; dz <= 1 && ok; ++dz)
+      for (int dy = -1; dy <= 1 && ok; ++dy)
+        for (int dx = -1; dx <= 1 && ok; ++dx) {
+          const int x = c[0] + dx, y = c[1] + dy, z = c[2] + dz;
+          if (x < 0 || y < 0 || z < 0 || x >= dim[0] || y >= dim[1] || z >= dim[2]) continue;
+          for (int j = head[((long)z * dim[1] + y) * dim[0] + x]; j >= 0; j = next[j]) {
+            const double ex = out[3 * j] - p[0], ey = out[3 * j + 1] - p[1],
+                         ez = out[3 * j + 2] - p[2];
+            const double rr = r + radii[j];
+            if (ex * ex + ey * ey + ez * ez < rr * rr) {
+              ok = false;
+              break;
+            }
+          }
+        }
+    if (!ok) continue;
+    out[3 * i] = p[0];
+    out[3 * i + 1] = p[1];
+    out[3 * i + 2] = p[2];
+    const long k = ((long)c[2] * dim[1] + c[1]) * dim[0] + c[0];
+    next[i] = head[k];
+    head[k] = i;
+    ++i;
+  }
+  *placed = n;
+  return used;
+}
+
+int pd_set_timing(pd_sim* s, int level) {
+  s->timing = level;
+  for (double& t : s->stageMs) t = 0.0;
+  for (int k = 0; k < pd_sim::K_COUNT; ++k) {
+    s->kernMs[k] = 0.0;
+    s->kernLaunches[k] = 0;
+    s->kernRows[k] = 0;
+  }
+  return 0;
+}
+int pd_get_kernel_times(pd_sim* s, double* ms, long* launches, long* rows, int cap) {
+  for (int k = 0; k < pd_sim::K_COUNT && k < cap; ++k) {
+    ms[k] = s->kernMs[k];
+    launches[k] = s->kernLaunches[k];
+    rows[k] = s->kernRows[k];
+  }
+  return pd_sim::K_COUNT;
+}
+int pd_get_stage_times(pd_sim* s, double* out, int cap) {
+  for (int k = 0; k < ST_COUNT && k < cap; ++k) out[k] = s->stageMs[k];
+  return ST_COUNT;
+}
+const char* pd_stage_name(int stage) {
+  return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : "";
+}
+
+}  // extern "C"

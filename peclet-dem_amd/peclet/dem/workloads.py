@@ -1,0 +1,154 @@
+"""Seeded synthetic workloads: non-overlapping random sphere packings in a box.
+
+``rsa_positions`` is random sequential addition (rejection sampling): candidates are drawn from
+``numpy.random.default_rng(seed)`` and accepted in draw order when they overlap no accepted sphere
+and clear the floor.  ``order="cell"`` returns the accepted spheres sorted along the cells of a
+uniform grid (z-major), an input ordering with spatial locality; ``order="draw"`` keeps draw order.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def _rsa_native(n, radii, lo, hi, rng, max_tries):
+    """Same acceptance rule as the Python loop below, run by the library's pd_rsa_place."""
+    import ctypes as C
+
+    from . import _product_binding
+
+    lib = _product_binding().lib
+    fn = lib.pd_rsa_place
+    dp = C.POINTER(C.c_double)
+    fn.restype = C.c_long
+    fn.argtypes = [dp, C.c_long, dp, C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int)]
+    out = np.zeros((n, 3), np.float64)
+    lo3 = np.ascontiguousarray(lo, np.float64)
+    hi3 = np.ascontiguousarray(hi, np.float64)
+    placed = C.c_int(0)
+    rmax = float(radii.max())
+    total = 0
+    while placed.value < n:
+        m = max(4 * (n - placed.value), n // 2, 4096)
+        total += m
+        if total > max_tries * max(n, 1):
+            raise RuntimeError(f"rsa_positions: could only place {placed.value} of {n} spheres")
+        cand = lo3 + rmax + rng.random((m, 3)) * (hi3 - lo3 - 2.0 * rmax)
+        cand = np.ascontiguousarray(cand)
+        fn(cand.ctypes.data_as(dp), m, radii.ctypes.data_as(dp), n, placed.value,
+           lo3.ctypes.data_as(dp), hi3.ctypes.data_as(dp), out.ctypes.data_as(dp),
+           C.byref(placed))
+    return out
+
+
+def rsa_positions(n, radii, lo, hi, seed, max_tries=200, order="cell", native=None):
+    """Place ``n`` spheres with the given radii (array of n) uniformly in the box [lo, hi].
+
+    Candidates are drawn in the box shrunk by the largest radius.  ``native`` (default: when
+    n > 20000) runs the acceptance loop in the C library.
+    """
+    radii = np.ascontiguousarray(radii, np.float64)
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    rng = np.random.default_rng(seed)
+    rmax = float(radii.max()) if n else 0.0
+    h = 2.0 * rmax
+    dims = np.maximum(((hi - lo) / h).astype(np.int64), 1)
+    if native is None:
+        native = n > 20000
+    if native and n:
+        out = _rsa_native(n, radii, lo, hi, rng, max_tries)
+        if order == "cell":
+            k = np.minimum(((out - lo) / h).astype(np.int64), dims - 1)
+            key = (k[:, 2] * dims[1] + k[:, 1]) * dims[0] + k[:, 0]
+            perm = np.argsort(key, kind="stable")
+            return out[perm], perm
+        return out, np.arange(n)
+    cells: dict = {}
+    out = np.empty((n, 3), np.float64)
+    placed = 0
+    batch = 4096
+    tries = 0
+    while placed < n:
+        if tries > max_tries * max(n, 1):
+            raise RuntimeError(f"rsa_positions: could only place {placed} of {n} spheres")
+        cand = lo + rmax + rng.random((batch, 3)) * (hi - lo - 2.0 * rmax)
+        for c in cand:
+            tries += 1
+            r = radii[placed]
+            key = tuple(np.minimum(((c - lo) / h).astype(np.int64), dims - 1))
+            ok = True
+            for dz in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        lst = cells.get((key[0] + dx, key[1] + dy, key[2] + dz))
+                        if not lst:
+                            continue
+                        for j in lst:
+                            d = out[j] - c
+                            if d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < (r + radii[j]) ** 2:
+                                ok = False
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                out[placed] = c
+                cells.setdefault(key, []).append(placed)
+                placed += 1
+                if placed == n:
+                    break
+    if order == "cell":
+        k = np.minimum(((out - lo) / h).astype(np.int64), dims - 1)
+        key = (k[:, 2] * dims[1] + k[:, 1]) * dims[0] + k[:, 0]
+        perm = np.argsort(key, kind="stable")
+        return out[perm], perm
+    return out, np.arange(n)
+
+
+def sphere_bed(n, r_lo, r_hi, box, seed, rho=2500.0, base_radius=1e-3, column=None,
+               order="cell"):
+    """Polydisperse sphere bed in SI units.
+
+    Returns dict(pos, scale, inv_mass, inv_inertia, radius, domain) for a bed of ``n`` spheres
+    with radius ~ U[r_lo, r_hi] (metres) seeded in ``[0, box[0]] x [0, box[1]] x [0, column]``.
+    """
+    rng = np.random.default_rng(seed + 7919)
+    radius = rng.uniform(r_lo, r_hi, n) if r_hi > r_lo else np.full(n, float(r_lo))
+    col = box[2] if column is None else column
+    pos, perm = rsa_positions(n, radius, (0.0, 0.0, 0.0), (box[0], box[1], col), seed,
+                              order=order)
+    radius = radius[perm]
+    mass = rho * 4.0 / 3.0 * math.pi * radius ** 3
+    inv_mass = 1.0 / mass
+    inv_i = 1.0 / (0.4 * mass * radius ** 2)
+    return {
+        "pos": pos,
+        "radius": radius,
+        "scale": radius / base_radius,
+        "inv_mass": inv_mass,
+        "inv_inertia": np.repeat(inv_i[:, None], 3, axis=1),
+        "domain": (float(box[0]), float(box[1]), float(box[2])),
+        "base_radius": base_radius,
+    }
+
+
+def setup_gravity_bed(sim, bed, dt, mu, e, iters=(12, 8), periodic=(True, True, False)):
+    """Configure a Simulation for a gravity-settling bed (floor at z = 0)."""
+    sim.set_sphere_shape(bed["base_radius"])
+    lx, ly, lz = bed["domain"]
+    sim.set_domain(lx, ly, lz, *periodic)
+    sim.add_plane(0.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+    sim.set_positions(bed["pos"])
+    sim.set_scales(bed["scale"])
+    sim.set_inv_mass(bed["inv_mass"])
+    sim.set_inv_inertia(bed["inv_inertia"])
+    sim.set_gravity(0.0, 0.0, -9.81)
+    sim.set_material_params(e, 0.0, mu)
+    sim.set_solver_iterations(*iters)
+    sim.set_dt(dt)
+    return sim
