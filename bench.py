@@ -47,9 +47,11 @@ def substep_bytes(st):
     return b
 
 
-def make_workload(n, seed, column):
-    from peclet.dem.workloads import sphere_bed
+def make_workload(kind, n, seed, column):
+    from peclet.dem.workloads import lattice_bed, sphere_bed
 
+    if kind == "lattice":
+        return lattice_bed(n, 0.16, seed)
     return sphere_bed(n, 0.8e-3, 1.2e-3, (0.16, 0.16, 0.4), seed, column=column)
 
 
@@ -76,6 +78,9 @@ def main():
     ap.add_argument("--column", type=float, default=0.7, help="seeding column height (m)")
     ap.add_argument("--cpu-substeps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["pour", "lattice"], default="pour",
+                    help="pour: the 1M-sphere gravity pour (default); lattice: a contact-rich "
+                         "bed of spheres on a slightly overlapping cubic lattice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
@@ -91,7 +96,7 @@ def main():
 
     dt = 5e-6
     t_gen = time.perf_counter()
-    bed = make_workload(args.n, args.seed, args.column)
+    bed = make_workload(args.workload, args.n, args.seed, args.column)
     t_gen = time.perf_counter() - t_gen
     sim = setup_gravity_bed(Simulation(args.n), bed, dt, 0.5, 0.3)
     print(f"# workload ready: n={args.n} (generation {t_gen:.1f}s)", file=sys.stderr, flush=True)
@@ -124,6 +129,9 @@ def main():
     avg_launch_s = ms[dom] / 1e3 / max(launches[dom], 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     total_bytes = sum(substep_bytes(s) for s in stats)
+    stage = np.zeros(32)
+    nst = sim._b.get_stage_times(sim._h, stage.ctypes.data_as(C.POINTER(C.c_double)), 32)
+    stage_ms = {sim._b.stage_name(k).decode(): float(stage[k]) / args.steps for k in range(nst)}
     mean = {k: float(np.mean([s[k] for s in stats])) for k in stats[0]}
 
     cpu = None
@@ -149,7 +157,11 @@ def main():
         "config": {
             "workload": (f"plain gravity pour, {args.n} polydisperse spheres r~U[0.8,1.2] mm, "
                          "160x160 mm periodic xy + floor, RSA start in a 700 mm column, "
-                         "dt=5e-6 s, mu=0.5, e=0.3, iters pos 12 / vel 8"),
+                         "dt=5e-6 s, mu=0.5, e=0.3, iters pos 12 / vel 8"
+                         if args.workload == "pour" else
+                         f"lattice bed, {args.n} spheres r~U[0.99,1.01] mm on a 1.996 r cubic "
+                         "lattice, 160x160 mm periodic xy + floor, dt=5e-6 s, mu=0.5, e=0.3, "
+                         "iters pos 12 / vel 8"),
             "form": "plain",
             "substeps_timed": args.steps,
         },
@@ -172,6 +184,7 @@ def main():
             "frac": total_bytes / t / 1e9 / HBM_PEAK_GBS,
         },
         "kernel_ms": {names[k]: float(ms[k]) for k in range(2)},
+        "stage_ms_per_substep": stage_ms,
         "counters_mean": mean,
     }
     print(json.dumps(line))

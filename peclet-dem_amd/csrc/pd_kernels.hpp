@@ -155,6 +155,42 @@ struct Dev {
 constexpr unsigned char F_PERSIST = 1, F_SIDE1 = 2, F_SIDE2 = 4, F_UP = 8, F_DOWN = 16;
 
 __device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ---- wave64 / block reductions: one global atomic per block instead of one per thread ----
+constexpr int WAVES = TPB / 64;
+__device__ __forceinline__ unsigned long long waveMaxU64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ int waveSumI(int v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+// every thread of the block must call these (no early return before them)
+__device__ __forceinline__ void blockAtomicMaxU64(unsigned long long* addr, unsigned long long v,
+                                                  unsigned long long* sh) {
+  v = waveMaxU64(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = sh[0];
+    for (int w = 1; w < WAVES; ++w) m = sh[w] > m ? sh[w] : m;
+    if (m) atomicMax(addr, m);
+  }
+}
+__device__ __forceinline__ void blockAtomicAddI(int* addr, int v, int* sh) {
+  v = waveSumI(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = 0;
+    for (int w = 0; w < WAVES; ++w) m += sh[w];
+    if (m) atomicAdd(addr, m);
+  }
+}
 __device__ __forceinline__ void atomicMaxD(unsigned long long* a, double v) {
   // non-negative doubles order like their bit patterns
   atomicMax(a, dbits(v));
@@ -613,9 +649,7 @@ __device__ __forceinline__ void writeContact(Dev& D, int slot, int a, int b, d3 
 }
 
 template <bool WRITE>
-__global__ void k_np_pairs(Dev D, Params P, int numPairs) {
-  const int p = gtid();
-  if (p >= numPairs) return;
+__device__ __forceinline__ double npPair(Dev& D, const Params& P, int p) {
   const int2 ab = D.pairs[p];
   const int idA = ab.x, idB = ab.y;
   const d3 posA = xyz(D.posPred[idA]), posB = xyz(D.posPred[idB]);
@@ -631,6 +665,7 @@ __global__ void k_np_pairs(Dev D, Params P, int numPairs) {
     mub = D.pairTable[t + 1];
   }
   int n = 0;
+  double ovl = 0.0;
   const int base = WRITE ? D.noff[p] : 0;
   for (int k = 0; k < iter; ++k) {
     d3 pLocalA{0.0, 0.0, 0.0};
@@ -647,7 +682,7 @@ __global__ void k_np_pairs(Dev D, Params P, int numPairs) {
       ++n;
       continue;
     }
-    if (effDist < 0.0) atomicMaxD(D.maxOverlapBits, -effDist);
+    if (effDist < 0.0 && -effDist > ovl) ovl = -effDist;
     const double eps = 1e-4;
     d3 nLoc{sdfShape(P.shape, d3{pCanB.x + eps, pCanB.y, pCanB.z}) -
                 sdfShape(P.shape, d3{pCanB.x - eps, pCanB.y, pCanB.z}),
@@ -665,12 +700,18 @@ __global__ void k_np_pairs(Dev D, Params P, int numPairs) {
     ++n;
   }
   if (!WRITE) D.ncnt[p] = n;
+  return ovl;
+}
+template <bool WRITE>
+__global__ void k_np_pairs(Dev D, Params P, int numPairs) {
+  __shared__ unsigned long long sh[WAVES];
+  const int p = gtid();
+  const double ovl = (p < numPairs) ? npPair<WRITE>(D, P, p) : 0.0;
+  if (WRITE) blockAtomicMaxU64(D.maxOverlapBits, dbits(ovl), sh);
 }
 
 template <bool WRITE>
-__global__ void k_np_planes(Dev D, Params P, int numPairs) {
-  const int i = gtid();
-  if (i >= P.numReal) return;
+__device__ __forceinline__ double npPlanes(Dev& D, const Params& P, int numPairs, int i) {
   const d3 posA = xyz(D.posPred[i]);
   const double s = D.scale[i] * P.gs;
   double baseR = P.shape.p0;
@@ -679,6 +720,7 @@ __global__ void k_np_planes(Dev D, Params P, int numPairs) {
   const int numPts = P.shape.numShell;
   const q4 qA = toq(D.quatPred[i]);
   int n = 0;
+  double ovl = 0.0;
   const int base = WRITE ? D.noff[numPairs + i] : 0;
   for (int pi = 0; pi < P.numPlanes; ++pi) {
     const d3 pp = xyz(P.planeP[pi]), pn = xyz(P.planeN[pi]);
@@ -689,7 +731,7 @@ __global__ void k_np_planes(Dev D, Params P, int numPairs) {
         const double dist = dot3(sub3(pwk, pp), pn);
         if (dist >= P.margin) continue;
         if (WRITE) {
-          if (dist < 0.0) atomicMaxD(D.maxOverlapBits, -dist);
+          if (dist < 0.0 && -dist > ovl) ovl = -dist;
           writeContact(D, base + n, i, -1, pn, rA, pp, dist, -1.0, -1.0);
         }
         ++n;
@@ -698,13 +740,21 @@ __global__ void k_np_planes(Dev D, Params P, int numPairs) {
       const double dist = dot3(sub3(posA, pp), pn) - radius;
       if (dist >= P.margin) continue;
       if (WRITE) {
-        if (dist < 0.0) atomicMaxD(D.maxOverlapBits, -dist);
+        if (dist < 0.0 && -dist > ovl) ovl = -dist;
         writeContact(D, base + n, i, -1, pn, scale3(pn, -radius), pp, dist, -1.0, -1.0);
       }
       ++n;
     }
   }
   if (!WRITE) D.ncnt[numPairs + i] = n;
+  return ovl;
+}
+template <bool WRITE>
+__global__ void k_np_planes(Dev D, Params P, int numPairs) {
+  __shared__ unsigned long long sh[WAVES];
+  const int i = gtid();
+  const double ovl = (i < P.numReal) ? npPlanes<WRITE>(D, P, numPairs, i) : 0.0;
+  if (WRITE) blockAtomicMaxU64(D.maxOverlapBits, dbits(ovl), sh);
 }
 
 // ============================================================================================
@@ -794,14 +844,17 @@ __global__ void k_mreduce(Dev D, int numManifolds) {
 // Jones-Plassmann colouring — src/solver_velocity.hpp:277-374, src/solver_position.hpp:155-241
 // ============================================================================================
 __global__ void k_jp_init_manifolds(Dev D, int numManifolds, int* uncol) {
+  __shared__ int sh[WAVES];
   const int m = gtid();
-  if (m >= numManifolds) return;
-  const int2 r = D.mReal[m];
-  int c = -1;
-  if (D.mNp[m] <= 0) c = -2;
-  else if (D.mAB[m].y >= 0 && r.x > r.y) c = -2;
-  D.mColor[m] = c;
-  if (c == -1) atomicAdd(uncol, 1);
+  int c = -2;
+  if (m < numManifolds) {
+    const int2 r = D.mReal[m];
+    c = -1;
+    if (D.mNp[m] <= 0) c = -2;
+    else if (D.mAB[m].y >= 0 && r.x > r.y) c = -2;
+    D.mColor[m] = c;
+  }
+  blockAtomicAddI(uncol, c == -1 ? 1 : 0, sh);
 }
 __global__ void k_jp_contend(const int2* ends, const int* colour, int numEdges,
                              unsigned long long* winner) {
@@ -815,40 +868,62 @@ __global__ void k_jp_contend(const int2* ends, const int* colour, int numEdges,
 __global__ void k_jp_commit(const int2* ends, int* colour, int numEdges,
                             const unsigned long long* winner, unsigned long long* mask,
                             int* remaining) {
+  __shared__ int sh[WAVES];
   const int e = gtid();
-  if (e >= numEdges || colour[e] != -1) return;
-  const int2 ab = ends[e];
-  const unsigned long long key = colorKey1(e);
-  if (winner[ab.x] != key || (ab.y >= 0 && winner[ab.y] != key)) {
-    atomicAdd(remaining, 1);
-    return;
+  int rem = 0;
+  if (e < numEdges && colour[e] == -1) {
+    const int2 ab = ends[e];
+    const unsigned long long key = colorKey1(e);
+    if (winner[ab.x] != key || (ab.y >= 0 && winner[ab.y] != key)) {
+      rem = 1;
+    } else {
+      unsigned long long forbidden = mask[ab.x];
+      if (ab.y >= 0) forbidden |= mask[ab.y];
+      int c = 0;
+      while (c < 62 && (forbidden & (1ull << c))) ++c;
+      colour[e] = c;
+      mask[ab.x] |= 1ull << c;
+      if (ab.y >= 0) mask[ab.y] |= 1ull << c;
+    }
   }
-  unsigned long long forbidden = mask[ab.x];
-  if (ab.y >= 0) forbidden |= mask[ab.y];
-  int c = 0;
-  while (c < 62 && (forbidden & (1ull << c))) ++c;
-  colour[e] = c;
-  mask[ab.x] |= 1ull << c;
-  if (ab.y >= 0) mask[ab.y] |= 1ull << c;
+  blockAtomicAddI(remaining, rem, sh);
 }
 __global__ void k_color_hist(const int* colour, int numEdges, int* colCnt) {
+  __shared__ int cnt[64];
+  if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+  __syncthreads();
   const int e = gtid();
-  if (e >= numEdges) return;
-  const int c = colour[e];
-  if (c >= 0) atomicAdd(&colCnt[c], 1);
+  if (e < numEdges) {
+    const int c = colour[e];
+    if (c >= 0) atomicAdd(&cnt[c], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64 && cnt[threadIdx.x]) atomicAdd(&colCnt[threadIdx.x], cnt[threadIdx.x]);
 }
+// rows of one colour class are contiguous; the order inside a class is free (body-disjoint)
 __global__ void k_color_perm(const int* colour, int numEdges, const int* colStart, int* colFill,
                              int* perm, int* rowOf) {
+  __shared__ int cnt[64];
+  __shared__ int base[64];
+  if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+  __syncthreads();
   const int e = gtid();
-  if (e >= numEdges) return;
-  const int c = colour[e];
-  if (c < 0) {
-    rowOf[e] = -1;
-    return;
+  const int c = (e < numEdges) ? colour[e] : -1;
+  int local = 0;
+  if (c >= 0) local = atomicAdd(&cnt[c], 1);
+  __syncthreads();
+  if (threadIdx.x < 64 && cnt[threadIdx.x])
+    base[threadIdx.x] = colStart[threadIdx.x] + atomicAdd(&colFill[threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (e < numEdges) {
+    if (c >= 0) {
+      const int k = base[c] + local;
+      perm[k] = e;
+      rowOf[e] = k;
+    } else {
+      rowOf[e] = -1;
+    }
   }
-  const int k = colStart[c] + atomicAdd(&colFill[c], 1);
-  perm[k] = e;
-  rowOf[e] = k;
 }
 
 // ============================================================================================
@@ -1088,7 +1163,8 @@ __device__ void warmApplyRow(Dev& D, int k) {
 }
 
 // PGSManifoldSweep::solveOne, src/solver_velocity.hpp:641-858 (one row)
-__device__ void pgsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
+__device__ double pgsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
+  double amax = 0.0;
   const int2 r = D.vAB[k];
   const int realA = r.x, realB = r.y;
   const BodyC ca = D.bc[realA];
@@ -1121,7 +1197,7 @@ __device__ void pgsRow(Dev& D, const Params& P, int k, double restVelThreshold) 
   const d3 Nsum = xyz(n4), TauA = xyz(ta4), TauB = xyz(tb4);
   const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
   const double lenN = ra4.w;
-  if (lenN < 1e-9) return;
+  if (lenN < 1e-9) return 0.0;
   const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
   const d3 vGrowth = scale3(diffCenters, P.growthRate);
   double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
@@ -1143,7 +1219,7 @@ __device__ void pgsRow(Dev& D, const Params& P, int k, double restVelThreshold) 
     restitution = 0.0;
   }
   const double wTotal = wA_n + wB_n;
-  if (wTotal <= 0.0) return;
+  if (wTotal <= 0.0) return 0.0;
   const double vn0 = D.vVn0[k];
   const double v0til = sgn * vn0;
   if (fabs(vn0) < restVelThreshold * lenN) restitution = 0.0;
@@ -1157,7 +1233,7 @@ __device__ void pgsRow(Dev& D, const Params& P, int k, double restVelThreshold) 
   bool dirtyA = false, dirtyB = false;
   if (dApplied != 0.0) {
     D.vLam[k] = pNew;
-    atomicMaxD(D.maxApproachBits, fabs(dApplied) * wTotal / lenN);
+    amax = fabs(dApplied) * wTotal / lenN;
     const double lambda = -sgn * dApplied;
     const d3 Jlin = scale3(Nsum, lambda);
     if (applyA) {
@@ -1228,7 +1304,7 @@ __device__ void pgsRow(Dev& D, const Params& P, int k, double restVelThreshold) 
     if (nonzero3(dApp)) {
       D.vLamT[k] = tod4(ltNew, 0.0);
       const double dLen = sqrt(dot3(dApp, dApp));
-      if (wT > 1e-9) atomicMaxD(D.maxApproachBits, dLen * wT);
+      if (wT > 1e-9 && dLen * wT > amax) amax = dLen * wT;
       if (applyA) {
         ba.v.x += dApp.x * invMassA;
         ba.v.y += dApp.y * invMassA;
@@ -1253,10 +1329,11 @@ __device__ void pgsRow(Dev& D, const Params& P, int k, double restVelThreshold) 
   }
   if (dirtyA) D.vw[realA] = ba;
   if (dirtyB) D.vw[realB] = bb;
+  return amax;
 }
 
 // solveVelocityColoredGSKokkos one-shot restitution (g = 0), src/solver_velocity.hpp:989-1133
-__device__ void gsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
+__device__ double gsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
   const int2 r = D.vAB[k];
   const int realA = r.x, realB = r.y;
   const BodyC ca = D.bc[realA];
@@ -1289,23 +1366,23 @@ __device__ void gsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
   const d3 Nsum = xyz(n4), TauA = xyz(ta4), TauB = xyz(tb4);
   const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
   const double lenN = ra4.w;
-  if (lenN < 1e-9) return;
+  if (lenN < 1e-9) return 0.0;
   const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
   const d3 vGrowth = scale3(diffCenters, P.growthRate);
   double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
   vn += dot3(vGrowth, Nsum);
   const double alignment = dot3(Nsum, diffCenters);
   if (alignment > 0.0) {
-    if (vn < 0.0) return;
+    if (vn < 0.0) return 0.0;
   } else {
-    if (vn > 0.0) return;
+    if (vn > 0.0) return 0.0;
   }
   const double Nsq = dot3(Nsum, Nsum);
   const double wA_n = Nsq * invMassA + genInvMass(TauA, invIA, qA);
   const double wB_n = Nsq * invMassB + genInvMass(TauB, invIB, qB);
   const double wTotal = wA_n + wB_n;
-  if (wTotal <= 0.0) return;
-  atomicMaxD(D.maxApproachBits, fabs(vn) / lenN);
+  if (wTotal <= 0.0) return 0.0;
+  const double amax = fabs(vn) / lenN;
   if (fabs(vn) < restVelThreshold * lenN) restitution = 0.0;
   const double lambda = (-restitution * vn - vn) / wTotal;
   const d3 Jlin = scale3(Nsum, lambda);
@@ -1329,6 +1406,7 @@ __device__ void gsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
     bb.w.z += dw.z;
     D.vw[realB] = bb;
   }
+  return amax;
 }
 
 // computeSideFlagsKokkos, src/solver_velocity.hpp:456-495 (rows are active, non-duplicate)
@@ -1348,11 +1426,15 @@ __global__ void k_side_flags(Dev D, int numRows, d3 gHat, double riseThr, double
 
 // mode: 0 = warm start, 1 = PGS sweep, 2 = one-shot GS sweep
 __global__ void k_vsweep(Dev D, Params P, int rowBegin, int rowEnd, int mode, double thr) {
+  __shared__ unsigned long long sh[WAVES];
   const int k = rowBegin + gtid();
-  if (k >= rowEnd) return;
-  if (mode == 0) warmApplyRow(D, k);
-  else if (mode == 1) pgsRow(D, P, k, thr);
-  else gsRow(D, P, k, thr);
+  double a = 0.0;
+  if (k < rowEnd) {
+    if (mode == 0) warmApplyRow(D, k);
+    else if (mode == 1) a = pgsRow(D, P, k, thr);
+    else a = gsRow(D, P, k, thr);
+  }
+  if (mode != 0) blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
 }
 // colour 62 may hold manifolds sharing a body (mask saturation): one thread, manifold order
 __global__ void k_vsweep_serial(Dev D, Params P, int numManifolds, int colour, int mode,
@@ -1362,28 +1444,50 @@ __global__ void k_vsweep_serial(Dev D, Params P, int numManifolds, int colour, i
     if (D.mColor[m] != colour) continue;
     const int k = D.vRowOf[m];
     if (mode == 0) warmApplyRow(D, k);
-    else if (mode == 1) pgsRow(D, P, k, thr);
-    else gsRow(D, P, k, thr);
+    else if (mode == 1) atomicMaxD(D.maxApproachBits, pgsRow(D, P, k, thr));
+    else atomicMaxD(D.maxApproachBits, gsRow(D, P, k, thr));
   }
 }
 
 // commitPairKeysLambdaKokkos, src/contact_preprocessing.hpp:160-192: bucket by the low real index
 // (key >> 32), then sort each bucket by (key, manifold index) — a stable key sort
 __global__ void k_ledger_count(Dev D, int numManifolds, int numReal) {
+  __shared__ int sh[WAVES];
   const int m = gtid();
-  if (m >= numManifolds) return;
-  const unsigned long long key = D.mKey[m];
-  const int b = (key == ~0ull) ? numReal : (int)(key >> 32);
-  atomicAdd(&D.bucketCnt[b], 1);
+  int dead = 0;
+  if (m < numManifolds) {
+    const unsigned long long key = D.mKey[m];
+    if (key == ~0ull) dead = 1;
+    else atomicAdd(&D.bucketCnt[(int)(key >> 32)], 1);
+  }
+  blockAtomicAddI(&D.bucketCnt[numReal], dead, sh);
 }
 __global__ void k_ledger_scatter(Dev D, int numManifolds, int numReal) {
+  __shared__ int ndead, baseDead;
+  if (threadIdx.x == 0) ndead = 0;
+  __syncthreads();
   const int m = gtid();
-  if (m >= numManifolds) return;
-  const unsigned long long key = D.mKey[m];
-  const int b = (key == ~0ull) ? numReal : (int)(key >> 32);
-  const int pos = D.bucketStart[b] + atomicAdd(&D.bucketCnt[b], 1);
-  D.lKey[pos] = key;
-  D.lIdx[pos] = m;
+  unsigned long long key = ~0ull;
+  int local = -1;
+  if (m < numManifolds) {
+    key = D.mKey[m];
+    if (key == ~0ull) {
+      local = atomicAdd(&ndead, 1);
+    } else {
+      const int b = (int)(key >> 32);
+      const int pos = D.bucketStart[b] + atomicAdd(&D.bucketCnt[b], 1);
+      D.lKey[pos] = key;
+      D.lIdx[pos] = m;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && ndead)
+    baseDead = D.bucketStart[numReal] + atomicAdd(&D.bucketCnt[numReal], ndead);
+  __syncthreads();
+  if (local >= 0) {
+    D.lKey[baseDead + local] = key;
+    D.lIdx[baseDead + local] = m;
+  }
 }
 __global__ void k_ledger_sort(Dev D, int numReal) {
   const int b = gtid();
@@ -1421,10 +1525,10 @@ __global__ void k_ledger_values(Dev D, int numManifolds) {
 // position solve — src/solver_position.hpp:248-331
 // ============================================================================================
 __global__ void k_jp_init_contacts(int* colour, int n, int* uncol) {
+  __shared__ int sh[WAVES];
   const int c = gtid();
-  if (c >= n) return;
-  colour[c] = -1;
-  atomicAdd(uncol, 1);
+  if (c < n) colour[c] = -1;
+  blockAtomicAddI(uncol, c < n ? 1 : 0, sh);
 }
 __global__ void k_prows(Dev D, int numRows) {
   const int k = gtid();
@@ -1436,7 +1540,7 @@ __global__ void k_prows(Dev D, int numRows) {
   D.pRB[k] = D.cRB[c];
   D.pLam[k] = 0.0;
 }
-__device__ void posRow(Dev& D, int k) {
+__device__ double posRow(Dev& D, int k) {
   const int2 ab = D.pAB[k];
   const int idA = ab.x, idB = ab.y;
   const BodyC ca = D.bc[idA];
@@ -1473,11 +1577,11 @@ __device__ void posRow(Dev& D, int k) {
   } else {
     C = dot3(sub3(add3(pA, rA), add3(pB, rB)), n);
   }
-  if (C >= 0.0) return;
+  if (C >= 0.0) return 0.0;
   const d3 invIA = xyz(ca.iim);
   const d3 invIB = (idB >= 0) ? xyz(cb.iim) : d3{0, 0, 0};
   const double wTotal = computeW(rA, n, invMassA, invIA) + computeW(rB, n, invMassB, invIB);
-  if (wTotal < 1e-6) return;
+  if (wTotal < 1e-6) return 0.0;
   const double dLambda = -C / wTotal;
   D.pLam[k] += dLambda;
   D.posPred[idA].x = pa4.x + n.x * dLambda * invMassA;
@@ -1488,17 +1592,18 @@ __device__ void posRow(Dev& D, int k) {
     D.posPred[idB].y = pb4.y + -n.y * dLambda * invMassB;
     D.posPred[idB].z = pb4.z + -n.z * dLambda * invMassB;
   }
-  atomicMaxD(D.maxOverlapBits, -C);
+  return -C;
 }
 __global__ void k_psweep(Dev D, int rowBegin, int rowEnd) {
+  __shared__ unsigned long long sh[WAVES];
   const int k = rowBegin + gtid();
-  if (k >= rowEnd) return;
-  posRow(D, k);
+  const double o = (k < rowEnd) ? posRow(D, k) : 0.0;
+  blockAtomicMaxU64(D.maxOverlapBits, dbits(o), sh);
 }
 __global__ void k_psweep_serial(Dev D, int numContacts, int colour) {
   if (gtid() != 0) return;
   for (int c = 0; c < numContacts; ++c)
-    if (D.cColor[c] == colour) posRow(D, D.pRowOf[c]);
+    if (D.cColor[c] == colour) atomicMaxD(D.maxOverlapBits, posRow(D, D.pRowOf[c]));
 }
 
 // commitPosImpulseKokkos, src/contact_preprocessing.hpp:501-538 (sum in contact order; among

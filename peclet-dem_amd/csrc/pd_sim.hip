@@ -631,13 +631,15 @@ struct pd_sim {
     mask.ensure(numBodies + 1, st);
     refreshDev();
     zero(mask.p, numBodies);
-    int remaining = read1(counters.p + 1);
+    // the reference starts with remaining = 1, so at least one round always runs
+    int entering = read1(counters.p + 1);
+    int remaining = 1;
     uncol = 0;
     rounds = 0;
     int prevRemaining = -1;
     const int maxRounds = numBodies + 2;
     for (int round = 0; round < maxRounds && remaining > 0; ++round) {
-      uncol += remaining;
+      uncol += entering;
       zero(winner.p, numBodies);
       zero(counters.p, 1);
       k_jp_contend<<<nblk(numEdges), TPB, 0, st>>>(ends, colour, numEdges, winner.p);
@@ -646,11 +648,12 @@ struct pd_sim {
       HIPCHECK(hipGetLastError());
       const int rem = read1(counters.p);
       ++rounds;
+      entering = rem;
       if (rem == prevRemaining) break;
       prevRemaining = rem;
       remaining = rem;
     }
-    return remaining > 0 ? remaining : 0;
+    return entering;
   }
   // colour classes: counts, starts, rows; returns maxColour+1 and fills colStartHost
   int colourClasses(const int* colour, int numEdges, int* perm, int* rowOf,

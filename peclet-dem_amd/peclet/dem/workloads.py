@@ -152,3 +152,32 @@ def setup_gravity_bed(sim, bed, dt, mu, e, iters=(12, 8), periodic=(True, True, 
     sim.set_solver_iterations(*iters)
     sim.set_dt(dt)
     return sim
+
+
+def lattice_bed(n, box_xy, seed, r=1e-3, spacing=1.996, jitter=0.004, spread=0.01, rho=2500.0,
+                base_radius=1e-3):
+    """Contact-rich bed: spheres on a cubic lattice (pitch ``spacing * r``) over the floor, filling
+    a ``box_xy`` x ``box_xy`` periodic footprint layer by layer; radii ~ r * U[1-spread, 1+spread]
+    and centres jittered by ``jitter * r``, so most lattice neighbours overlap slightly."""
+    rng = np.random.default_rng(seed)
+    pitch = spacing * r
+    side = max(1, int(box_xy / pitch))
+    k = np.arange(n)
+    iz, rem = np.divmod(k, side * side)
+    iy, ix = np.divmod(rem, side)
+    pos = np.stack([(ix + 0.5) * pitch, (iy + 0.5) * pitch, r + iz * pitch], 1).astype(np.float64)
+    pos += rng.uniform(-jitter * r, jitter * r, pos.shape)
+    radius = r * rng.uniform(1.0 - spread, 1.0 + spread, n)
+    pos[:, 2] = np.maximum(pos[:, 2], radius * 0.999)
+    mass = rho * 4.0 / 3.0 * math.pi * radius ** 3
+    inv_i = 1.0 / (0.4 * mass * radius ** 2)
+    L = side * pitch
+    return {
+        "pos": pos,
+        "radius": radius,
+        "scale": radius / base_radius,
+        "inv_mass": 1.0 / mass,
+        "inv_inertia": np.repeat(inv_i[:, None], 3, axis=1),
+        "domain": (L, L, float(pos[:, 2].max() + 4 * r)),
+        "base_radius": base_radius,
+    }

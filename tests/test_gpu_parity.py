@@ -179,3 +179,12 @@ def test_thermostat(hip_lib):
         h.step(scene["dt"])
         r.step(scene["dt"])
     assert rel_err(h.get_velocities(), r.get_velocities()) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_bed_bit_exact_state(hip_lib):
+    """With -ffp-contract=off on both sides the FP64 state agrees to the last bit."""
+    h, r = run_pair("bed_periodic", 300, 12, seed=4, check_topology_every=0)
+    for name in ["get_positions", "get_velocities", "get_quaternions", "get_angular_velocities"]:
+        a, b = getattr(h, name)(), getattr(r, name)()
+        assert np.array_equal(a, b), f"{name}: {int((a != b).sum())} of {a.size} differ"
