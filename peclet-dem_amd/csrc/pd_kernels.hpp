@@ -157,6 +157,17 @@ constexpr unsigned char F_PERSIST = 1, F_SIDE1 = 2, F_SIDE2 = 4, F_UP = 8, F_DOW
 __device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
 // ---- wave64 / block reductions: one global atomic per block instead of one per thread ----
+// block-aggregated append of `item` (when `take`) to list[*count]; every thread must call it
+__device__ __forceinline__ void blockAppend(bool take, int item, int* list, int* count, int* shCnt,
+                                            int* shBase) {
+  if (threadIdx.x == 0) *shCnt = 0;
+  __syncthreads();
+  const int local = take ? atomicAdd(shCnt, 1) : 0;
+  __syncthreads();
+  if (threadIdx.x == 0 && *shCnt) *shBase = atomicAdd(count, *shCnt);
+  __syncthreads();
+  if (take) list[*shBase + local] = item;
+}
 constexpr int WAVES = TPB / 64;
 __device__ __forceinline__ unsigned long long waveMaxU64(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -202,6 +213,10 @@ __device__ __forceinline__ void atomicMaxD(unsigned long long* a, double v) {
 __global__ void k_fill_i32(int* a, int v, int n) {
   const int i = gtid();
   if (i < n) a[i] = v;
+}
+__global__ void k_set_counts(int* a, int first, int n) {
+  const int i = gtid();
+  if (i < n) a[i] = i == 0 ? first : 0;
 }
 __global__ void k_fill_u64(unsigned long long* a, unsigned long long v, int n) {
   const int i = gtid();
@@ -276,23 +291,40 @@ __global__ void k_scan_down(const int* in, int n, const int* blockSums, int* out
 // ============================================================================================
 // integration — src/integration.hpp
 // ============================================================================================
-// maxOwnedRadius reduction, src/sim.hpp:61-70 (ordered-key max, NaN skipped)
-__global__ void k_max_scale(const double* scale, int n, unsigned long long* out) {
-  __shared__ unsigned long long sh[TPB];
-  const int i = gtid();
+// maxOwnedRadius reduction, src/sim.hpp:61-70 (ordered-key max, NaN skipped): per-block partials
+// then k_max_final (no contended atomics)
+__global__ __launch_bounds__(TPB) void k_max_scale(const double* scale, int n,
+                                                   unsigned long long* part) {
+  __shared__ unsigned long long sh[WAVES];
   unsigned long long k = 0;
-  if (i < n) {
+  for (int i = gtid(); i < n; i += gridDim.x * blockDim.x) {
     const double s = scale[i];
-    if (s == s) k = ordKey(s);
+    if (s == s) {
+      const unsigned long long o = ordKey(s);
+      k = o > k ? o : k;
+    }
   }
-  sh[threadIdx.x] = k;
+  k = waveMaxU64(k);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = k;
   __syncthreads();
-  for (int off = TPB / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off && sh[threadIdx.x + off] > sh[threadIdx.x])
-      sh[threadIdx.x] = sh[threadIdx.x + off];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = sh[0];
+    for (int w = 1; w < WAVES; ++w) m = sh[w] > m ? sh[w] : m;
+    part[blockIdx.x] = m;
   }
-  if (threadIdx.x == 0 && sh[0]) atomicMax(out, sh[0]);
+}
+__global__ void k_max_final(const unsigned long long* part, int nb, unsigned long long* out) {
+  __shared__ unsigned long long sh[WAVES];
+  unsigned long long k = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) k = part[b] > k ? part[b] : k;
+  k = waveMaxU64(k);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = sh[0];
+    for (int w = 1; w < WAVES; ++w) m = sh[w] > m ? sh[w] : m;
+    *out = m;
+  }
 }
 
 // updateGrowthScalesKokkos, src/integration.hpp:275-283
@@ -524,45 +556,65 @@ __device__ __forceinline__ bool boxOverlap(const double4& a, const double4& c) {
 
 // bounds of the finite, non-parked box centres and the largest half-width
 // red[0..2] = ordered min, red[3..5] = ordered max, red[6] = ordered max half-width
-__global__ void k_bp_bounds(Dev D, Params P, unsigned long long* red) {
-  __shared__ unsigned long long sh[7][TPB];
-  const int s = gtid();
-  unsigned long long mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0, 0, 0}, mb = 0;
-  if (s < P.numParticles) {
+__device__ __forceinline__ unsigned long long waveMinU64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+constexpr int RED_BLOCKS = 256;
+// part[b * 8 + 0..2] = ordered min, [3..5] = ordered max, [6] = ordered max half-width
+__global__ __launch_bounds__(TPB) void k_bp_bounds(Dev D, Params P, unsigned long long* part) {
+  __shared__ unsigned long long sh[7][WAVES];
+  unsigned long long v[7] = {~0ull, ~0ull, ~0ull, 0, 0, 0, 0};
+  for (int s = gtid(); s < P.numParticles; s += gridDim.x * blockDim.x) {
     const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
-    if (isfinite(B.w)) mb = ordKey(B.w);
+    if (isfinite(B.w)) {
+      const unsigned long long k = ordKey(B.w);
+      v[6] = k > v[6] ? k : v[6];
+    }
     if (B.x != -1.0e30) {
       const double c[3] = {B.x, B.y, B.z};
       for (int d = 0; d < 3; ++d)
         if (isfinite(c[d])) {
-          mn[d] = ordKey(c[d]);
-          mx[d] = mn[d];
+          const unsigned long long k = ordKey(c[d]);
+          v[d] = k < v[d] ? k : v[d];
+          v[3 + d] = k > v[3 + d] ? k : v[3 + d];
         }
     }
   }
-  for (int d = 0; d < 3; ++d) {
-    sh[d][threadIdx.x] = mn[d];
-    sh[3 + d][threadIdx.x] = mx[d];
-  }
-  sh[6][threadIdx.x] = mb;
+  for (int d = 0; d < 3; ++d) v[d] = waveMinU64(v[d]);
+  for (int d = 3; d < 7; ++d) v[d] = waveMaxU64(v[d]);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 7; ++d) sh[d][w] = v[d];
   __syncthreads();
-  for (int off = TPB / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      for (int d = 0; d < 3; ++d) {
-        if (sh[d][threadIdx.x + off] < sh[d][threadIdx.x]) sh[d][threadIdx.x] = sh[d][threadIdx.x + off];
-        if (sh[3 + d][threadIdx.x + off] > sh[3 + d][threadIdx.x])
-          sh[3 + d][threadIdx.x] = sh[3 + d][threadIdx.x + off];
-      }
-      if (sh[6][threadIdx.x + off] > sh[6][threadIdx.x]) sh[6][threadIdx.x] = sh[6][threadIdx.x + off];
-    }
-    __syncthreads();
+  if (threadIdx.x < 7) {
+    const int d = threadIdx.x;
+    unsigned long long r = sh[d][0];
+    for (int k = 1; k < WAVES; ++k) r = (d < 3) ? (sh[d][k] < r ? sh[d][k] : r) : (sh[d][k] > r ? sh[d][k] : r);
+    part[blockIdx.x * 8 + d] = r;
   }
-  if (threadIdx.x == 0) {
-    for (int d = 0; d < 3; ++d) {
-      atomicMin(&red[d], sh[d][0]);
-      atomicMax(&red[3 + d], sh[3 + d][0]);
+}
+__global__ void k_bp_bounds_final(const unsigned long long* part, int nb, unsigned long long* red) {
+  __shared__ unsigned long long sh[7][WAVES];
+  unsigned long long v[7] = {~0ull, ~0ull, ~0ull, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int d = 0; d < 7; ++d) {
+      const unsigned long long x = part[b * 8 + d];
+      v[d] = (d < 3) ? (x < v[d] ? x : v[d]) : (x > v[d] ? x : v[d]);
     }
-    atomicMax(&red[6], sh[6][0]);
+  for (int d = 0; d < 3; ++d) v[d] = waveMinU64(v[d]);
+  for (int d = 3; d < 7; ++d) v[d] = waveMaxU64(v[d]);
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 7; ++d) sh[d][threadIdx.x >> 6] = v[d];
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    const int d = threadIdx.x;
+    unsigned long long r = sh[d][0];
+    for (int k = 1; k < WAVES; ++k) r = (d < 3) ? (sh[d][k] < r ? sh[d][k] : r) : (sh[d][k] > r ? sh[d][k] : r);
+    red[d] = r;
   }
 }
 
@@ -703,7 +755,7 @@ __device__ __forceinline__ double npPair(Dev& D, const Params& P, int p) {
   return ovl;
 }
 template <bool WRITE>
-__global__ void k_np_pairs(Dev D, Params P, int numPairs) {
+__global__ __launch_bounds__(TPB) void k_np_pairs(Dev D, Params P, int numPairs) {
   __shared__ unsigned long long sh[WAVES];
   const int p = gtid();
   const double ovl = (p < numPairs) ? npPair<WRITE>(D, P, p) : 0.0;
@@ -788,7 +840,7 @@ __global__ void k_mbuild(Dev D, Params P, int numPairs) {
   }
 }
 // transformContact + ordered segmented sum, src/contact_preprocessing.hpp:339-466
-__global__ void k_mreduce(Dev D, int numManifolds) {
+__global__ __launch_bounds__(TPB) void k_mreduce(Dev D, int numManifolds) {
   const int m = gtid();
   if (m >= numManifolds) return;
   const int2 ab = D.mAB[m];
@@ -843,8 +895,9 @@ __global__ void k_mreduce(Dev D, int numManifolds) {
 // ============================================================================================
 // Jones-Plassmann colouring — src/solver_velocity.hpp:277-374, src/solver_position.hpp:155-241
 // ============================================================================================
-__global__ void k_jp_init_manifolds(Dev D, int numManifolds, int* uncol) {
-  __shared__ int sh[WAVES];
+// uncoloured manifolds (-1) are appended to the first work list; inactive / duplicate ones get -2
+__global__ void k_jp_init_manifolds(Dev D, int numManifolds, int* work, int* count) {
+  __shared__ int shCnt, shBase;
   const int m = gtid();
   int c = -2;
   if (m < numManifolds) {
@@ -854,28 +907,37 @@ __global__ void k_jp_init_manifolds(Dev D, int numManifolds, int* uncol) {
     else if (D.mAB[m].y >= 0 && r.x > r.y) c = -2;
     D.mColor[m] = c;
   }
-  blockAtomicAddI(uncol, c == -1 ? 1 : 0, sh);
+  blockAppend(c == -1, m, work, count, &shCnt, &shBase);
 }
-__global__ void k_jp_contend(const int2* ends, const int* colour, int numEdges,
+// One JP round over the compacted work list.  winner[] holds, per body, 1 + the largest priority
+// among its uncoloured edges; a committing edge clears the slots of its two bodies, a losing edge
+// keeps its key there (it re-contends with the same key), so no per-round reset is needed.
+__global__ void k_jp_contend(const int2* ends, const int* work, const int* count,
                              unsigned long long* winner) {
-  const int e = gtid();
-  if (e >= numEdges || colour[e] != -1) return;
+  const int t = gtid();
+  if (t >= *count) return;
+  const int e = work[t];
   const int2 ab = ends[e];
   const unsigned long long key = colorKey1(e);
   atomicMax(&winner[ab.x], key);
   if (ab.y >= 0) atomicMax(&winner[ab.y], key);
 }
-__global__ void k_jp_commit(const int2* ends, int* colour, int numEdges,
-                            const unsigned long long* winner, unsigned long long* mask,
-                            int* remaining) {
-  __shared__ int sh[WAVES];
-  const int e = gtid();
-  int rem = 0;
-  if (e < numEdges && colour[e] == -1) {
+__global__ void k_jp_commit(const int2* ends, int* colour, const int* work, const int* count,
+                            unsigned long long* winner, unsigned long long* mask, int* next,
+                            int* nextCount) {
+  __shared__ int shCnt, shBase;
+  const int t = gtid();
+  const int n = *count;
+  int e = -1;
+  bool lose = false;
+  if (t < n) {
+    e = work[t];
     const int2 ab = ends[e];
     const unsigned long long key = colorKey1(e);
-    if (winner[ab.x] != key || (ab.y >= 0 && winner[ab.y] != key)) {
-      rem = 1;
+    const unsigned long long wa = winner[ab.x];
+    const unsigned long long wb = ab.y >= 0 ? winner[ab.y] : key;
+    if (wa != key || wb != key) {
+      lose = true;
     } else {
       unsigned long long forbidden = mask[ab.x];
       if (ab.y >= 0) forbidden |= mask[ab.y];
@@ -883,41 +945,75 @@ __global__ void k_jp_commit(const int2* ends, int* colour, int numEdges,
       while (c < 62 && (forbidden & (1ull << c))) ++c;
       colour[e] = c;
       mask[ab.x] |= 1ull << c;
-      if (ab.y >= 0) mask[ab.y] |= 1ull << c;
+      winner[ab.x] = 0;
+      if (ab.y >= 0) {
+        mask[ab.y] |= 1ull << c;
+        winner[ab.y] = 0;
+      }
     }
   }
-  blockAtomicAddI(remaining, rem, sh);
+  blockAppend(lose, e, next, nextCount, &shCnt, &shBase);
 }
-__global__ void k_color_hist(const int* colour, int numEdges, int* colCnt) {
+// Colour classes: pass 1 counts per (block, colour) with LDS atomics; k_color_offsets turns the
+// counts into per-block bases; pass 2 places every edge at base + its rank inside the block.  Both
+// passes walk the same grid-stride chunks, so the per-block counts match.  The order inside a
+// class is free (the class is body-disjoint).
+constexpr int CP_BLOCKS = 1024;
+// blockCnt is colour-major: blockCnt[c * CP_BLOCKS + b]
+__global__ void k_color_count(const int* colour, int numEdges, int* blockCnt) {
   __shared__ int cnt[64];
   if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int e = gtid();
-  if (e < numEdges) {
+  for (int e = gtid(); e < numEdges; e += gridDim.x * blockDim.x) {
     const int c = colour[e];
     if (c >= 0) atomicAdd(&cnt[c], 1);
   }
   __syncthreads();
-  if (threadIdx.x < 64 && cnt[threadIdx.x]) atomicAdd(&colCnt[threadIdx.x], cnt[threadIdx.x]);
+  if (threadIdx.x < 64) blockCnt[threadIdx.x * CP_BLOCKS + blockIdx.x] = cnt[threadIdx.x];
 }
-// rows of one colour class are contiguous; the order inside a class is free (body-disjoint)
-__global__ void k_color_perm(const int* colour, int numEdges, const int* colStart, int* colFill,
-                             int* perm, int* rowOf) {
+// block c: exclusive scan of colour c's per-block counts; its total -> colTot[c]
+__global__ void k_color_offsets(int* blockCnt, int nb, int* colTot) {
+  __shared__ int sh[TPB];
+  int* row = blockCnt + blockIdx.x * CP_BLOCKS;
+  constexpr int PER = CP_BLOCKS / TPB;
+  int v[PER];
+  int sum = 0;
+  for (int k = 0; k < PER; ++k) {
+    const int b = threadIdx.x * PER + k;
+    v[k] = b < nb ? row[b] : 0;
+    sum += v[k];
+  }
+  int tot;
+  int ex = blockExclusiveScan(sum, sh, &tot);
+  for (int k = 0; k < PER; ++k) {
+    const int b = threadIdx.x * PER + k;
+    if (b < nb) row[b] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) colTot[blockIdx.x] = tot;
+}
+__global__ void k_color_starts(const int* colTot, int* colStart) {
+  if (threadIdx.x != 0) return;
+  int run = 0;
+  for (int c = 0; c < 64; ++c) {
+    colStart[c] = run;
+    run += colTot[c];
+  }
+  colStart[64] = run;
+}
+__global__ void k_color_place(const int* colour, int numEdges, const int* blockBase,
+                              const int* colStart, int* perm, int* rowOf) {
   __shared__ int cnt[64];
   __shared__ int base[64];
-  if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 64) {
+    cnt[threadIdx.x] = 0;
+    base[threadIdx.x] = colStart[threadIdx.x] + blockBase[threadIdx.x * CP_BLOCKS + blockIdx.x];
+  }
   __syncthreads();
-  const int e = gtid();
-  const int c = (e < numEdges) ? colour[e] : -1;
-  int local = 0;
-  if (c >= 0) local = atomicAdd(&cnt[c], 1);
-  __syncthreads();
-  if (threadIdx.x < 64 && cnt[threadIdx.x])
-    base[threadIdx.x] = colStart[threadIdx.x] + atomicAdd(&colFill[threadIdx.x], cnt[threadIdx.x]);
-  __syncthreads();
-  if (e < numEdges) {
+  for (int e = gtid(); e < numEdges; e += gridDim.x * blockDim.x) {
+    const int c = colour[e];
     if (c >= 0) {
-      const int k = base[c] + local;
+      const int k = base[c] + atomicAdd(&cnt[c], 1);
       perm[k] = e;
       rowOf[e] = k;
     } else {
@@ -938,6 +1034,21 @@ __device__ __forceinline__ int lowerBound(const unsigned long long* keys, int n,
     else hi = mid;
   }
   return lo;
+}
+// first position of `key` in the sorted ledger, or -1.  Keys with the same low real index are
+// contiguous (bucket b = key >> 32 spans [bstart[b], bstart[b+1])), so the search stays inside one
+// short bucket; the result equals a lower_bound over the whole ledger.
+__device__ __forceinline__ int ledgerFind(const unsigned long long* keys, const int* bstart,
+                                         int nbuckets, unsigned long long key) {
+  const unsigned long long hi = key >> 32;
+  if (hi >= (unsigned long long)nbuckets) return -1;
+  const int e = bstart[hi + 1];
+  for (int p = bstart[hi]; p < e; ++p) {
+    const unsigned long long k = keys[p];
+    if (k == key) return p;
+    if (k > key) return -1;
+  }
+  return -1;
 }
 __device__ __forceinline__ unsigned long long realKey(int realA, int realB) {
   const unsigned a = (unsigned)realA;
@@ -971,7 +1082,8 @@ __global__ void k_mkeys(Dev D, int numManifolds, d3 gHat) {
 
 // velocity rows: gather the manifold (colour-sorted), warm start, persistence, vn0 / vt0
 // (gatherWarmLambdaKokkos :113, markPersistentManifoldsKokkos :78, computeVn0Kokkos :390)
-__global__ void k_vrows(Dev D, Params P, int numRows, int prevCount, int usePGS) {
+__global__ __launch_bounds__(TPB) void k_vrows(Dev D, Params P, int numRows, int ledgerBuckets,
+                                               int usePGS) {
   const int k = gtid();
   if (k >= numRows) return;
   const int m = D.vPerm[k];
@@ -1012,9 +1124,8 @@ __global__ void k_vrows(Dev D, Params P, int numRows, int prevCount, int usePGS)
   double pimp = 0.0;
   if (usePGS) {
     const unsigned long long key = realKey(realA, realB);
-    const int lo = lowerBound(D.lKey, prevCount, key);
-    const bool hit = (lo < prevCount && D.lKey[lo] == key);
-    if (hit) {
+    const int lo = ledgerFind(D.lKey, D.bucketStart, ledgerBuckets, key);
+    if (lo >= 0) {
       lam = D.lLam[lo];
       const double4 lt = D.lLamT[lo];
       lamT = make_double4(lt.x, lt.y, lt.z, 0.0);
@@ -1410,9 +1521,10 @@ __device__ double gsRow(Dev& D, const Params& P, int k, double restVelThreshold)
 }
 
 // computeSideFlagsKokkos, src/solver_velocity.hpp:456-495 (rows are active, non-duplicate)
-__global__ void k_side_flags(Dev D, int numRows, d3 gHat, double riseThr, double approachThr) {
+__global__ void k_side_flags(Dev D, int numRows, d3 gHat, double riseThr, double approachThr,
+                             const int* run) {
   const int k = gtid();
-  if (k >= numRows) return;
+  if (k >= numRows || !*run) return;
   unsigned char fl = D.vFlag[k] & (unsigned char)~(F_SIDE1 | F_SIDE2);
   const int2 r = D.vAB[k];
   if (r.y >= 0 && (fl & F_PERSIST) && !(fabs(D.vVn0[k]) > approachThr)) {
@@ -1425,21 +1537,45 @@ __global__ void k_side_flags(Dev D, int numRows, d3 gHat, double riseThr, double
 }
 
 // mode: 0 = warm start, 1 = PGS sweep, 2 = one-shot GS sweep
-__global__ void k_vsweep(Dev D, Params P, int rowBegin, int rowEnd, int mode, double thr) {
+// ---- device-side loop control: ctl[0..] ints (see pd_sim.hip), sweeps after a stop are no-ops
+enum { CTL_STOP_V = 0, CTL_SWEEPS_V, CTL_STAB_RUN, CTL_STOP_S, CTL_SWEEPS_S, CTL_STOP_P,
+       CTL_SWEEPS_P, CTL_NONE, CTL_COUNT };
+__global__ void k_sweep_begin(unsigned long long* bits, const int* stop) {
+  if (!*stop) *bits = 0ull;
+}
+// after one sweep: count it and stop once its max is within the tolerance
+__global__ void k_sweep_check(const unsigned long long* bits, int* stop, int* count, double thr,
+                              int strict) {
+  if (*stop) return;
+  *count += 1;
+  const double m = bitsd(*bits);
+  if (strict ? (m < thr) : (m <= thr)) *stop = 1;
+}
+// stabilization gate (src/sim.hpp:279-283): run iff mode != 0 and the residual exceeds vRestS
+__global__ void k_stab_gate(const unsigned long long* bits, int* ctl, int mode, double vRestS) {
+  const int run = (mode != 0 && bitsd(*bits) > vRestS) ? 1 : 0;
+  ctl[CTL_STAB_RUN] = run;
+  ctl[CTL_STOP_S] = run ? 0 : 1;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(TPB) void k_vsweep(Dev D, Params P, int rowBegin, int rowEnd,
+                                                double thr, const int* stop) {
+  if (*stop) return;
   __shared__ unsigned long long sh[WAVES];
   const int k = rowBegin + gtid();
   double a = 0.0;
   if (k < rowEnd) {
-    if (mode == 0) warmApplyRow(D, k);
-    else if (mode == 1) a = pgsRow(D, P, k, thr);
+    if (MODE == 0) warmApplyRow(D, k);
+    else if (MODE == 1) a = pgsRow(D, P, k, thr);
     else a = gsRow(D, P, k, thr);
   }
-  if (mode != 0) blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
+  if (MODE != 0) blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
 }
 // colour 62 may hold manifolds sharing a body (mask saturation): one thread, manifold order
 __global__ void k_vsweep_serial(Dev D, Params P, int numManifolds, int colour, int mode,
-                                double thr) {
-  if (gtid() != 0) return;
+                                double thr, const int* stop) {
+  if (gtid() != 0 || *stop) return;
   for (int m = 0; m < numManifolds; ++m) {
     if (D.mColor[m] != colour) continue;
     const int k = D.vRowOf[m];
@@ -1524,65 +1660,72 @@ __global__ void k_ledger_values(Dev D, int numManifolds) {
 // ============================================================================================
 // position solve — src/solver_position.hpp:248-331
 // ============================================================================================
-__global__ void k_jp_init_contacts(int* colour, int n, int* uncol) {
-  __shared__ int sh[WAVES];
+__global__ void k_jp_init_contacts(int* colour, int n, int* work) {
   const int c = gtid();
-  if (c < n) colour[c] = -1;
-  blockAtomicAddI(uncol, c < n ? 1 : 0, sh);
+  if (c >= n) return;
+  colour[c] = -1;
+  work[c] = c;
 }
-__global__ void k_prows(Dev D, int numRows) {
+// Position rows (colour-sorted).  The lever arms and normal rotated into the predicted frame and
+// the effective inverse mass depend only on quatPred / quat / invMass / invI, which the position
+// sweeps never change, so they are evaluated once here (same expressions as per sweep in
+// src/solver_position.hpp:276-312).  Row: pN = (n, wTotal), pRA = (rA, invMassA),
+// pRB = (rB or plane point, invMassB).
+__global__ __launch_bounds__(TPB) void k_prows(Dev D, int numRows) {
   const int k = gtid();
   if (k >= numRows) return;
   const int c = D.pPerm[k];
-  D.pAB[k] = D.cAB[c];
-  D.pN[k] = D.cN[c];
-  D.pRA[k] = D.cRA[c];
-  D.pRB[k] = D.cRB[c];
-  D.pLam[k] = 0.0;
-}
-__device__ double posRow(Dev& D, int k) {
-  const int2 ab = D.pAB[k];
+  const int2 ab = D.cAB[c];
   const int idA = ab.x, idB = ab.y;
+  const double4 n4 = D.cN[c], ra4 = D.cRA[c], rb4 = D.cRB[c];
   const BodyC ca = D.bc[idA];
-  const double invMassA = ca.iim.w;
-  const double4 pa4 = D.posPred[idA];
-  const d3 pA = xyz(pa4);
   const q4 qA = toq(D.quatPred[idA]);
-  d3 pB{0, 0, 0};
-  q4 qB{0, 0, 0, 1};
-  BodyC cb;
-  double invMassB = 0.0;
-  double4 pb4;
-  if (idB >= 0) {
-    cb = D.bc[idB];
-    invMassB = cb.iim.w;
-    pb4 = D.posPred[idB];
-    pB = xyz(pb4);
-    qB = toq(D.quatPred[idB]);
-  }
-  const double4 n4 = D.pN[k], ra4 = D.pRA[k], rb4 = D.pRB[k];
   const q4 qAdelta = quatMult(qA, quatInverse(toq(ca.q)));
   const d3 rA = rotate(qAdelta, xyz(ra4));
   d3 rB = xyz(rb4);
   d3 n = xyz(n4);
+  double invMassB = 0.0;
+  d3 invIB{0, 0, 0};
   if (idB >= 0) {
+    const BodyC cb = D.bc[idB];
+    const q4 qB = toq(D.quatPred[idB]);
     const q4 qBdelta = quatMult(qB, quatInverse(toq(cb.q)));
     rB = rotate(qBdelta, rB);
     n = rotate(qBdelta, n);
-  }
-  double C;
-  if (idB < 0) {
-    n = xyz(n4);
-    C = dot3(sub3(add3(pA, rA), xyz(rb4)), n);
+    invMassB = cb.iim.w;
+    invIB = xyz(cb.iim);
   } else {
-    C = dot3(sub3(add3(pA, rA), add3(pB, rB)), n);
+    n = xyz(n4);
+  }
+  const double invMassA = ca.iim.w;
+  const double wTotal = computeW(rA, n, invMassA, xyz(ca.iim)) + computeW(rB, n, invMassB, invIB);
+  D.pAB[k] = ab;
+  D.pN[k] = tod4(n, wTotal);
+  D.pRA[k] = tod4(rA, invMassA);
+  D.pRB[k] = tod4(rB, invMassB);
+  D.pLam[k] = 0.0;
+}
+// solvePositionColoredGSKokkos, one contact (src/solver_position.hpp:259-326)
+__device__ double posRow(Dev& D, int k) {
+  const int2 ab = D.pAB[k];
+  const int idA = ab.x, idB = ab.y;
+  const double4 n4 = D.pN[k], ra4 = D.pRA[k], rb4 = D.pRB[k];
+  const d3 n = xyz(n4), rA = xyz(ra4), rB = xyz(rb4);
+  const double4 pa4 = D.posPred[idA];
+  const d3 pA = xyz(pa4);
+  double C;
+  double4 pb4;
+  if (idB < 0) {
+    C = dot3(sub3(add3(pA, rA), rB), n);
+  } else {
+    pb4 = D.posPred[idB];
+    C = dot3(sub3(add3(pA, rA), add3(xyz(pb4), rB)), n);
   }
   if (C >= 0.0) return 0.0;
-  const d3 invIA = xyz(ca.iim);
-  const d3 invIB = (idB >= 0) ? xyz(cb.iim) : d3{0, 0, 0};
-  const double wTotal = computeW(rA, n, invMassA, invIA) + computeW(rB, n, invMassB, invIB);
+  const double wTotal = n4.w;
   if (wTotal < 1e-6) return 0.0;
   const double dLambda = -C / wTotal;
+  const double invMassA = ra4.w, invMassB = rb4.w;
   D.pLam[k] += dLambda;
   D.posPred[idA].x = pa4.x + n.x * dLambda * invMassA;
   D.posPred[idA].y = pa4.y + n.y * dLambda * invMassA;
@@ -1594,21 +1737,22 @@ __device__ double posRow(Dev& D, int k) {
   }
   return -C;
 }
-__global__ void k_psweep(Dev D, int rowBegin, int rowEnd) {
+__global__ __launch_bounds__(TPB) void k_psweep(Dev D, int rowBegin, int rowEnd, const int* stop) {
+  if (*stop) return;
   __shared__ unsigned long long sh[WAVES];
   const int k = rowBegin + gtid();
   const double o = (k < rowEnd) ? posRow(D, k) : 0.0;
   blockAtomicMaxU64(D.maxOverlapBits, dbits(o), sh);
 }
-__global__ void k_psweep_serial(Dev D, int numContacts, int colour) {
-  if (gtid() != 0) return;
+__global__ void k_psweep_serial(Dev D, int numContacts, int colour, const int* stop) {
+  if (gtid() != 0 || *stop) return;
   for (int c = 0; c < numContacts; ++c)
     if (D.cColor[c] == colour) atomicMaxD(D.maxOverlapBits, posRow(D, D.pRowOf[c]));
 }
 
 // commitPosImpulseKokkos, src/contact_preprocessing.hpp:501-538 (sum in contact order; among
 // manifolds sharing a key the highest manifold index writes, as a serial loop would)
-__global__ void k_commit_pos_impulse(Dev D, int numManifolds, double dt) {
+__global__ void k_commit_pos_impulse(Dev D, int numManifolds, int numReal, double dt) {
   const int m = gtid();
   if (m >= numManifolds) return;
   const int2 rg = D.mRange[m];
@@ -1620,8 +1764,8 @@ __global__ void k_commit_pos_impulse(Dev D, int numManifolds, double dt) {
   }
   const unsigned long long key = D.mKey[m];
   if (key == ~0ull) return;
-  const int lo = lowerBound(D.lKey, numManifolds, key);
-  if (!(lo < numManifolds && D.lKey[lo] == key)) return;
+  const int lo = ledgerFind(D.lKey, D.bucketStart, numReal, key);
+  if (lo < 0) return;
   int hi = lo + 1;
   while (hi < numManifolds && D.lKey[hi] == key) ++hi;
   if (D.lIdx[hi - 1] == m) D.lLamT[lo].w = s;

@@ -114,6 +114,7 @@ struct pd_sim {
   double pairTable[128];
   // results
   int np = 0, nc = 0, nm = 0, prevCount = 0, nRowsV = 0;
+  int ledgerBuckets = 0;  // real-index buckets of the sorted ledger (bucketStart has +1 entries)
   double maxOverlap = 0.0, maxApproach = 0.0;
   pd_stats stats{};
   // timing: stage events (level >= 1) and per-launch events of the sweep kernels (level >= 2)
@@ -171,7 +172,9 @@ struct pd_sim {
   DBuf<double4> cN, cRA, cRB, mNs, mTA, mTB, mRA, mRB, vN, vTA, vTB, vRA, vRB, vVt0, vLamT, pN,
       pRA, pRB, lLamT;
   DBuf<double> vVn0, vLam, pLam, lLam;
-  DBuf<unsigned long long> mKey, winner, mask, lKey, uKey, red, bits;
+  DBuf<unsigned long long> mKey, winner, mask, lKey, uKey, red, bits, redPart;
+  DBuf<int> blockCnt;
+  DBuf<int> ctl;  // device loop control (CTL_* in pd_kernels.hpp)
   DBuf<int> lIdx, bucketCnt, bucketStart, counters, colCnt, colFill, colStart, scanTmp, ghostCnt,
       ghostOff;
   // pinned host scratch
@@ -187,6 +190,7 @@ struct pd_sim {
     red.ensure(8, st);
     bits.ensure(4, st);
     counters.ensure(16, st);
+    ctl.ensure(CTL_COUNT + 1, st);
     colCnt.ensure(64, st);
     colFill.ensure(64, st);
     colStart.ensure(65, st);
@@ -221,6 +225,8 @@ struct pd_sim {
     counters.release(); colCnt.release(); colFill.release(); colStart.release();
     scanTmp.release(); ghostCnt.release(); ghostOff.release();
     dVel.release(); dAng.release(); dPos.release(); dCnt.release();
+    jpWork[0].release(); jpWork[1].release(); jpCnt.release(); redPart.release();
+    blockCnt.release(); ctl.release();
   }
 
   // ---------------------------------------------------------------- small helpers
@@ -536,9 +542,10 @@ struct pd_sim {
   void broadAndNarrow(double margin) {
     Params P = params(margin, 0.0);
     // bounds of the box centres -> uniform grid with cell edge >= 2 * max half-width
-    const unsigned long long init[7] = {~0ull, ~0ull, ~0ull, 0, 0, 0, 0};
-    writeN(red.p, init, 7);
-    k_bp_bounds<<<nblk(numParticles), TPB, 0, st>>>(D, P, red.p);
+    const int nbb = std::min(nblk(numParticles), RED_BLOCKS);
+    redPart.ensure(8 * RED_BLOCKS, st);
+    k_bp_bounds<<<nbb, TPB, 0, st>>>(D, P, redPart.p);
+    k_bp_bounds_final<<<1, TPB, 0, st>>>(redPart.p, nbb, red.p);
     unsigned long long rk[7];
     readN(rk, red.p, 7);
     double lo[3], hi[3];
@@ -624,72 +631,110 @@ struct pd_sim {
     HIPCHECK(hipGetLastError());
   }
 
-  // Jones-Plassmann rounds (host-driven like the reference); returns the uncoloured leftovers
-  int jpColor(const int2* ends, int* colour, int numEdges, int numBodies, int& rounds,
-              long long& uncol) {
+  // Jones-Plassmann rounds over a shrinking work list (the uncoloured edges, already in jpWork[0]
+  // with `entering` entries).  Rounds run in batches: round j of a batch reads its list size from
+  // jpCnt[j] and appends the losers to the other list, counting into jpCnt[j + 1]; the host reads
+  // the counts once per batch and replays the reference loop (src/solver_velocity.hpp:305-355) on
+  // them.  Every round colours at least the globally highest-priority edge, so rounds after the
+  // list empties are no-ops.  Returns the uncoloured leftovers after the round cap.
+  DBuf<int> jpWork[2];
+  DBuf<int> jpCnt;
+  int jpColor(const int2* ends, int* colour, int numEdges, int numBodies, int entering,
+              int& rounds, long long& uncol) {
     winner.ensure(numBodies + 1, st);
     mask.ensure(numBodies + 1, st);
+    jpCnt.ensure(17, st);
     refreshDev();
     zero(mask.p, numBodies);
-    // the reference starts with remaining = 1, so at least one round always runs
-    int entering = read1(counters.p + 1);
-    int remaining = 1;
+    zero(winner.p, numBodies);
+    int remaining = 1;  // the reference starts with remaining = 1: one round always runs
     uncol = 0;
     rounds = 0;
     int prevRemaining = -1;
-    const int maxRounds = numBodies + 2;
-    for (int round = 0; round < maxRounds && remaining > 0; ++round) {
-      uncol += entering;
-      zero(winner.p, numBodies);
-      zero(counters.p, 1);
-      k_jp_contend<<<nblk(numEdges), TPB, 0, st>>>(ends, colour, numEdges, winner.p);
-      k_jp_commit<<<nblk(numEdges), TPB, 0, st>>>(ends, colour, numEdges, winner.p, mask.p,
-                                                  counters.p);
+    int cur = 0;
+    const long maxRounds = (long)numBodies + 2;
+    int batch = 2;
+    int hostCnt[17];
+    while (remaining > 0 && rounds < maxRounds) {
+      const int B = (int)std::min<long>(batch, maxRounds - rounds);
+      k_set_counts<<<1, 64, 0, st>>>(jpCnt.p, entering, B + 1);
+      const int grid = nblk(entering);
+      for (int j = 0; j < B; ++j) {
+        const int* work = jpWork[(cur + j) & 1].p;
+        int* next = jpWork[(cur + j + 1) & 1].p;
+        k_jp_contend<<<grid, TPB, 0, st>>>(ends, work, jpCnt.p + j, winner.p);
+        k_jp_commit<<<grid, TPB, 0, st>>>(ends, colour, work, jpCnt.p + j, winner.p, mask.p, next,
+                                          jpCnt.p + j + 1);
+      }
       HIPCHECK(hipGetLastError());
-      const int rem = read1(counters.p);
-      ++rounds;
-      entering = rem;
-      if (rem == prevRemaining) break;
-      prevRemaining = rem;
-      remaining = rem;
+      readN(hostCnt, jpCnt.p, B + 1);
+      for (int j = 0; j < B && remaining > 0; ++j) {
+        uncol += hostCnt[j];
+        ++rounds;
+        const int rem = hostCnt[j + 1];
+        if (rem == prevRemaining && rem > 0)
+          throw std::runtime_error("colouring stalled: a round coloured no edge");
+        prevRemaining = rem;
+        remaining = rem;
+      }
+      entering = hostCnt[B];
+      cur = (cur + B) & 1;
+      batch = std::min(batch * 2, 16);
     }
-    return entering;
+    return remaining > 0 ? entering : 0;
   }
-  // colour classes: counts, starts, rows; returns maxColour+1 and fills colStartHost
+  // colour classes: counts, starts, rows; returns maxColour+1 and fills the class starts
   int colourClasses(const int* colour, int numEdges, int* perm, int* rowOf,
                     std::vector<int>& starts) {
-    zero(colCnt.p, 64);
-    zero(colFill.p, 64);
-    k_color_hist<<<nblk(numEdges), TPB, 0, st>>>(colour, numEdges, colCnt.p);
-    int cnt[64];
-    readN(cnt, colCnt.p, 64);
-    starts.assign(65, 0);
-    int maxc = -1;
-    for (int c = 0; c < 64; ++c) {
-      starts[c + 1] = starts[c] + cnt[c];
-      if (cnt[c] > 0) maxc = c;
-    }
-    writeN(colStart.p, starts.data(), 65);
-    k_color_perm<<<nblk(numEdges), TPB, 0, st>>>(colour, numEdges, colStart.p, colFill.p, perm,
-                                                 rowOf);
+    const int nb = std::min(nblk(numEdges), CP_BLOCKS);
+    blockCnt.ensure(64 * CP_BLOCKS, st);
+    k_color_count<<<nb, TPB, 0, st>>>(colour, numEdges, blockCnt.p);
+    k_color_offsets<<<64, TPB, 0, st>>>(blockCnt.p, nb, colCnt.p);
+    k_color_starts<<<1, 64, 0, st>>>(colCnt.p, colStart.p);
+    k_color_place<<<nb, TPB, 0, st>>>(colour, numEdges, blockCnt.p, colStart.p, perm, rowOf);
     HIPCHECK(hipGetLastError());
+    starts.assign(65, 0);
+    readN(starts.data(), colStart.p, 65);
+    int maxc = -1;
+    for (int c = 0; c < 64; ++c)
+      if (starts[c + 1] > starts[c]) maxc = c;
     return maxc + 1;
   }
 
+  // Enqueue up to `iters` device-gated sweeps in geometric batches (1, 2, 4, ...), reading the
+  // stop flag once per batch so a loop that converges early does not pay for launching the rest.
+  template <class F>
+  void sweepLoop(int iters, const int* stop, F&& one) {
+    int done = 0, batch = 1;
+    while (done < iters) {
+      const int b = std::min(batch, iters - done);
+      for (int j = 0; j < b; ++j) one();
+      done += b;
+      if (done < iters && read1(stop)) break;
+      batch *= 2;
+    }
+  }
   double readMaxApproach() {
     return bitsd(read1(bits.p + 1));
   }
-  void velocitySweep(const std::vector<int>& vs, int numColors, int mode, double thr) {
+  void velocitySweep(const std::vector<int>& vs, int numColors, int mode, double thr,
+                     const int* stop) {
     Params P = params(0.0, 0.0);
     for (int c = 0; c < numColors; ++c) {
       const int b = vs[c], e = vs[c + 1];
       if (e <= b) continue;
       if (c == 62) {
-        k_vsweep_serial<<<1, 1, 0, st>>>(D, P, nm, c, mode, thr);
+        k_vsweep_serial<<<1, 1, 0, st>>>(D, P, nm, c, mode, thr, stop);
       } else {
-        if (mode == 1) kernBegin();
-        k_vsweep<<<nblk(e - b), TPB, 0, st>>>(D, P, b, e, mode, thr);
-        if (mode == 1) kernEnd(K_VSWEEP, e - b);
+        if (mode == 0) {
+          k_vsweep<0><<<nblk(e - b), TPB, 0, st>>>(D, P, b, e, thr, stop);
+        } else if (mode == 1) {
+          kernBegin();
+          k_vsweep<1><<<nblk(e - b), TPB, 0, st>>>(D, P, b, e, thr, stop);
+          kernEnd(K_VSWEEP, e - b);
+        } else {
+          k_vsweep<2><<<nblk(e - b), TPB, 0, st>>>(D, P, b, e, thr, stop);
+        }
       }
     }
     HIPCHECK(hipGetLastError());
@@ -719,8 +764,10 @@ struct pd_sim {
     // maxOwnedRadius, src/sim.hpp:61-70
     double maxRad = globalScale * baseRadius;
     if (numReal > 0) {
-      zero(red.p, 1);
-      k_max_scale<<<nblk(numReal), TPB, 0, st>>>(scale.p, numReal, red.p);
+      const int nbs = std::min(nblk(numReal), RED_BLOCKS);
+      redPart.ensure(8 * RED_BLOCKS, st);
+      k_max_scale<<<nbs, TPB, 0, st>>>(scale.p, numReal, redPart.p);
+      k_max_final<<<1, TPB, 0, st>>>(redPart.p, nbs, red.p);
       const unsigned long long k = read1(red.p);
       const double mx = k ? ordVal(k) : -std::numeric_limits<double>::max();
       maxRad = mx * globalScale * baseRadius;
@@ -748,13 +795,17 @@ struct pd_sim {
     // manifold colouring, src/solver_velocity.hpp:277-374
     zero(counters.p, 2);
     vPerm.ensure(nm + 1, st);
+    jpWork[0].ensure(std::max(nm, nc) + 1, st);
+    jpWork[1].ensure(std::max(nm, nc) + 1, st);
     refreshDev();
-    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(D, nm, counters.p + 1);
+    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(D, nm, jpWork[0].p, counters.p + 1);
     std::vector<int> vs;
     int numColors = 0;
     int velLeftover = 0;
     if (nm > 0 && numReal > 0) {
-      velLeftover = jpColor(mReal.p, mColor.p, nm, numReal, stats.rounds_vel, stats.uncolored_vel);
+      const int entering = read1(counters.p + 1);
+      velLeftover = jpColor(mReal.p, mColor.p, nm, numReal, entering, stats.rounds_vel,
+                            stats.uncolored_vel);
       numColors = colourClasses(mColor.p, nm, vPerm.p, vRowOf.p, vs);
     } else {
       vs.assign(65, 0);
@@ -772,7 +823,8 @@ struct pd_sim {
     refreshDev();
     if (nm > 0) k_mkeys<<<nblk(nm), TPB, 0, st>>>(D, nm, gHat);
     if (nRowsV > 0)
-      k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(D, P, nRowsV, prevCount, usePGS ? 1 : 0);
+      k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(D, P, nRowsV, prevCount > 0 ? ledgerBuckets : 0,
+                                            usePGS ? 1 : 0);
     if (usePGS) {
       // grounded levels, double-buffered (src/contact_preprocessing.hpp:213-253)
       k_ground_decay<<<nblk(numReal), TPB, 0, st>>>(ground.p, numReal, 8);
@@ -784,39 +836,72 @@ struct pd_sim {
         std::swap(ga, gb);
       }
       // 8 swaps: the result is back in `ground`
-      velocitySweep(vs, numColors, 0, 0.0);
+      velocitySweep(vs, numColors, 0, 0.0, ctl.p + CTL_NONE);
     }
     HIPCHECK(hipGetLastError());
     mark(ST_VSWEEP);
-    for (int it = 0; it < velIters; ++it) {
-      const double vRest = 2.0 * dt * std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-      zero(bits.p + 1, 1);
-      velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest);
-      if (velLeftover > 0) {
-        prepareFallback();
-        Params PF = params(margin, maxRad);
-        k_vel_jacobi_accum<<<1, 1, 0, st>>>(D, PF, nm, vRest);
-        k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+    const double vRest = 2.0 * dt * std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    const double vRestS = 2.0 * dt * gMagP;
+    const bool hostLoop = velLeftover > 0 || (usePGS && stabMode == 3);
+    if (!hostLoop) {
+      // device-controlled adaptive stops: no host round trip per sweep
+      zero(ctl.p, CTL_COUNT);
+      const double stopThr = usePGS ? 0.02 * vRest : vRest;
+      sweepLoop(velIters, ctl.p + CTL_STOP_V, [&] {
+        k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V);
+        velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest, ctl.p + CTL_STOP_V);
+        k_sweep_check<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V, ctl.p + CTL_SWEEPS_V,
+                                       stopThr, 0);
+      });
+      mark(ST_STAB);
+      if (usePGS) {
+        k_stab_gate<<<1, 1, 0, st>>>(bits.p + 1, ctl.p, stabMode, vRestS);
+        if (stabMode == 1 && velIters > 0 && read1(ctl.p + CTL_STAB_RUN)) {
+          if (nRowsV > 0)
+            k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
+                                                       8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
+          sweepLoop(2 * velIters, ctl.p + CTL_STOP_S, [&] {
+            k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S);
+            velocitySweep(vs, numColors, 1, vRestS, ctl.p + CTL_STOP_S);
+            k_sweep_check<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S, ctl.p + CTL_SWEEPS_S,
+                                           vRestS, 0);
+          });
+        }
       }
-      stats.sweeps_vel++;
-      maxApproach = readMaxApproach();
-      if (maxApproach <= (usePGS ? 0.02 * vRest : vRest)) break;
-    }
-    mark(ST_STAB);
-    if (usePGS) {
-      const double vRestS = 2.0 * dt * gMagP;
-      if (stabMode != 0 && maxApproach > vRestS) {
-        stats.stab_ran = 1;
-        if (stabMode == 1 && nRowsV > 0)
-          k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
-                                                     8.0 * dt * gMagP);
-        const int cap = stabMode == 1 ? 2 * velIters : 256;
-        for (int it = 0; it < cap; ++it) {
-          zero(bits.p + 1, 1);
-          velocitySweep(vs, numColors, 1, vRestS);
-          stats.sweeps_stab++;
-          maxApproach = readMaxApproach();
-          if (maxApproach <= vRestS) break;
+      HIPCHECK(hipGetLastError());
+    } else {
+      zero(ctl.p, CTL_COUNT);
+      for (int it = 0; it < velIters; ++it) {
+        zero(bits.p + 1, 1);
+        velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest, ctl.p + CTL_NONE);
+        if (velLeftover > 0) {
+          prepareFallback();
+          Params PF = params(margin, maxRad);
+          k_vel_jacobi_accum<<<1, 1, 0, st>>>(D, PF, nm, vRest);
+          k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+        }
+        stats.sweeps_vel++;
+        maxApproach = readMaxApproach();
+        if (maxApproach <= (usePGS ? 0.02 * vRest : vRest)) break;
+      }
+      mark(ST_STAB);
+      if (usePGS) {
+        maxApproach = readMaxApproach();
+        if (stabMode != 0 && maxApproach > vRestS) {
+          stats.stab_ran = 1;
+          const int one = 1;
+          writeN(ctl.p + CTL_STAB_RUN, &one, 1);
+          if (stabMode == 1 && nRowsV > 0)
+            k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
+                                                       8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
+          const int cap = stabMode == 1 ? 2 * velIters : 256;
+          for (int it = 0; it < cap; ++it) {
+            zero(bits.p + 1, 1);
+            velocitySweep(vs, numColors, 1, vRestS, ctl.p + CTL_NONE);
+            stats.sweeps_stab++;
+            maxApproach = readMaxApproach();
+            if (maxApproach <= vRestS) break;
+          }
         }
       }
     }
@@ -839,6 +924,7 @@ struct pd_sim {
       k_ledger_values<<<nblk(nm), TPB, 0, st>>>(D, nm);
       HIPCHECK(hipGetLastError());
       prevCount = nm;
+      ledgerBuckets = numReal;
     } else if (usePGS) {
       prevCount = 0;
     }
@@ -857,8 +943,8 @@ struct pd_sim {
     int numPosColors = 0;
     int posLeftover = 0;
     if (nc > 0 && numParticles > 0) {
-      k_jp_init_contacts<<<nblk(nc), TPB, 0, st>>>(cColor.p, nc, counters.p + 1);
-      posLeftover = jpColor(cAB.p, cColor.p, nc, numParticles, stats.rounds_pos,
+      k_jp_init_contacts<<<nblk(nc), TPB, 0, st>>>(cColor.p, nc, jpWork[0].p);
+      posLeftover = jpColor(cAB.p, cColor.p, nc, numParticles, nc, stats.rounds_pos,
                             stats.uncolored_pos);
       numPosColors = colourClasses(cColor.p, nc, pPerm.p, pRowOf.p, ps);
     } else {
@@ -874,32 +960,60 @@ struct pd_sim {
     if (nRowsP > 0) k_prows<<<nblk(nRowsP), TPB, 0, st>>>(D, nRowsP);
     mark(ST_PSWEEP);
     const double posTol = 1e-4 * baseRadius * globalScale;
-    for (int it = 0; it < posIters; ++it) {
-      zero(bits.p, 1);
-      for (int c = 0; c < numPosColors; ++c) {
-        const int b = ps[c], e = ps[c + 1];
-        if (e <= b) continue;
-        if (c == 62) {
-          k_psweep_serial<<<1, 1, 0, st>>>(D, nc, c);
-        } else {
-          kernBegin();
-          k_psweep<<<nblk(e - b), TPB, 0, st>>>(D, b, e);
-          kernEnd(K_PSWEEP, e - b);
+    if (posLeftover == 0) {
+      sweepLoop(posIters, ctl.p + CTL_STOP_P, [&] {
+        k_sweep_begin<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P);
+        for (int c = 0; c < numPosColors; ++c) {
+          const int b = ps[c], e = ps[c + 1];
+          if (e <= b) continue;
+          if (c == 62) {
+            k_psweep_serial<<<1, 1, 0, st>>>(D, nc, c, ctl.p + CTL_STOP_P);
+          } else {
+            kernBegin();
+            k_psweep<<<nblk(e - b), TPB, 0, st>>>(D, b, e, ctl.p + CTL_STOP_P);
+            kernEnd(K_PSWEEP, e - b);
+          }
         }
-      }
-      if (posLeftover > 0) {
+        k_sweep_check<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P, ctl.p + CTL_SWEEPS_P, posTol,
+                                       1);
+      });
+      HIPCHECK(hipGetLastError());
+    } else {
+      for (int it = 0; it < posIters; ++it) {
+        zero(bits.p, 1);
+        for (int c = 0; c < numPosColors; ++c) {
+          const int b = ps[c], e = ps[c + 1];
+          if (e <= b) continue;
+          if (c == 62)
+            k_psweep_serial<<<1, 1, 0, st>>>(D, nc, c, ctl.p + CTL_NONE);
+          else
+            k_psweep<<<nblk(e - b), TPB, 0, st>>>(D, b, e, ctl.p + CTL_NONE);
+        }
         prepareFallback();
         k_pos_jacobi_accum<<<1, 1, 0, st>>>(D, nc);
         k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+        HIPCHECK(hipGetLastError());
+        stats.sweeps_pos++;
+        if (bitsd(read1(bits.p)) < posTol) break;
       }
-      HIPCHECK(hipGetLastError());
-      stats.sweeps_pos++;
-      maxOverlap = bitsd(read1(bits.p));
-      if (maxOverlap < posTol) break;
     }
-    if (posIters <= 0) maxOverlap = bitsd(read1(bits.p));
+    {
+      // one read-back of the loop counters and the residuals
+      int c[CTL_COUNT];
+      readN(c, ctl.p, CTL_COUNT);
+      unsigned long long b2[2];
+      readN(b2, bits.p, 2);
+      if (!hostLoop) {
+        stats.sweeps_vel = c[CTL_SWEEPS_V];
+        stats.sweeps_stab = c[CTL_SWEEPS_S];
+        stats.stab_ran = usePGS ? c[CTL_STAB_RUN] : 0;
+      }
+      if (posLeftover == 0) stats.sweeps_pos = c[CTL_SWEEPS_P];
+      maxOverlap = bitsd(b2[0]);
+      maxApproach = bitsd(b2[1]);
+    }
     mark(ST_COMMIT);
-    if (usePGS && nm > 0) k_commit_pos_impulse<<<nblk(nm), TPB, 0, st>>>(D, nm, dt);
+    if (usePGS && nm > 0) k_commit_pos_impulse<<<nblk(nm), TPB, 0, st>>>(D, nm, numReal, dt);
     k_final_commit<<<nblk(numReal), TPB, 0, st>>>(D, P);
     if (thermoTau > 0.0 && dt > 0.0) applyThermostat();
     HIPCHECK(hipGetLastError());
@@ -958,8 +1072,10 @@ struct pd_sim {
     refreshDev();
     double maxRad = globalScale * baseRadius;
     if (numReal > 0) {
-      zero(red.p, 1);
-      k_max_scale<<<nblk(numReal), TPB, 0, st>>>(scale.p, numReal, red.p);
+      const int nbs = std::min(nblk(numReal), RED_BLOCKS);
+      redPart.ensure(8 * RED_BLOCKS, st);
+      k_max_scale<<<nbs, TPB, 0, st>>>(scale.p, numReal, redPart.p);
+      k_max_final<<<1, TPB, 0, st>>>(redPart.p, nbs, red.p);
       const unsigned long long k = read1(red.p);
       maxRad = (k ? ordVal(k) : -std::numeric_limits<double>::max()) * globalScale * baseRadius;
     }
