@@ -120,7 +120,7 @@ int pd_get_stats(pd_sim* s, pd_stats* out);
 int pd_get_pairs(pd_sim* s, int* out, int cap);                      /* (i, j) pairs */
 int pd_get_contacts(pd_sim* s, int* ids, double* data, int cap);     /* n, rA, rB, dist */
 int pd_get_contact_colors(pd_sim* s, int* out, int cap);
-int pd_get_manifolds(pd_sim* s, int* ids, double* data, int cap);    /* A, B, points; sums */
+int pd_get_manifolds(pd_sim* s, int* ids, double* data, int cap);    /* A,B,pts,realA,realB */
 int pd_get_manifold_colors(pd_sim* s, int* out, int cap);
 int pd_get_ledger(pd_sim* s, uint64_t* keys, double* lambda, double* lambdaT, double* posImp,
                   int cap);                                          /* warm-start store */

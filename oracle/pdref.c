@@ -38,6 +38,11 @@ static int fail(const char* msg) {
   snprintf(g_err, sizeof g_err, "%s", msg);
   return -1;
 }
+/* invalid arguments (the reference's std::invalid_argument, a ValueError in Python) */
+static int badarg(const char* msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return -2;
+}
 const char* pdref_last_error(void) { return g_err; }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -385,9 +390,9 @@ int pdref_initialize_shape(void* s, int shape_type, double radius, double height
 /* Extension (no reference counterpart): replace shape 0's surface point shell. */
 int pdref_set_shell_points(void* s, const double* pts, int n) {
   Sim* S = asSim(s);
-  if (n < 0) return fail("set_shell_points: negative count");
+  if (n < 0) return badarg("set_shell_points: negative count");
   if (S->shapeType == SHAPE_SPHERE && n > 0)
-    return fail("set_shell_points: the analytic sphere uses a single centre probe");
+    return badarg("set_shell_points: the analytic sphere uses a single centre probe");
   free(S->shell);
   S->shell = (V3*)xrealloc(NULL, sizeof(V3) * (size_t)(n > 0 ? n : 1));
   for (int i = 0; i < n; ++i) S->shell[i] = mk3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
@@ -487,7 +492,7 @@ int pdref_set_solver_iterations(void* s, int pos, int vel) {
 }
 int pdref_set_velocity_use_gs(void* s, int use_gs) { asSim(s)->useGS = use_gs != 0; return 0; }
 int pdref_set_stabilization_mode(void* s, int mode) {
-  if (mode < 0 || mode > 4) return fail("stabilization mode must be 0..4");
+  if (mode < 0 || mode > 4) return badarg("stabilization mode must be 0..4");
   asSim(s)->stabMode = mode;
   return 0;
 }
@@ -505,7 +510,7 @@ int pdref_set_material_ids(void* s, const int* ids, int n) {
 }
 int pdref_set_pair_material(void* s, int a, int b, double e, double mu) {
   Sim* S = asSim(s); /* src/sim.hpp:1196-1215 */
-  if (a < 0 || b < 0 || a >= KMAXMAT || b >= KMAXMAT) return fail("material id out of range");
+  if (a < 0 || b < 0 || a >= KMAXMAT || b >= KMAXMAT) return badarg("material id out of range");
   if (!S->hasPairTable) {
     for (int i = 0; i < KMAXMAT * KMAXMAT; ++i) {
       S->pairTable[2 * i] = S->restitutionNormal;
@@ -528,7 +533,7 @@ int pdref_add_plane(void* s, double px, double py, double pz, double nx, double 
 }
 int pdref_set_positions(void* s, const double* xyz, int n) {
   Sim* S = asSim(s); /* src/sim.hpp:1290-1330 */
-  if (n < 0) return fail("negative particle count");
+  if (n < 0) return badarg("negative particle count");
   ensureSlots(S, n);
   ensureGrounded(S, n);
   S->numReal = n;
@@ -2291,12 +2296,18 @@ int pdref_get_contact_colors(void* s, int* out, int cap) {
     for (int k = 0; k < S->nc && k < cap; ++k) out[k] = S->cColor[k];
   return S->nc;
 }
-/* ids: (bodyA, bodyB, numPoints); data: normalSum, tauA, tauB, rASum, rBSum (15) */
+/* ids: (bodyA, bodyB, numPoints, realA, realB); data: normalSum, tauA, tauB, rASum, rBSum (15) */
 int pdref_get_manifolds(void* s, int* ids, double* data, int cap) {
   Sim* S = asSim(s);
   for (int k = 0; k < S->nm && k < cap; ++k) {
     const Manifold* m = &S->manifolds[k];
-    if (ids) { ids[3 * k] = m->bodyA; ids[3 * k + 1] = m->bodyB; ids[3 * k + 2] = m->numPoints; }
+    if (ids) {
+      ids[5 * k] = m->bodyA;
+      ids[5 * k + 1] = m->bodyB;
+      ids[5 * k + 2] = m->numPoints;
+      ids[5 * k + 3] = S->realIdx[m->bodyA];
+      ids[5 * k + 4] = m->bodyB >= 0 ? S->realIdx[m->bodyB] : -1;
+    }
     if (data) {
       double* d = data + 15 * k;
       const V3 f[5] = {m->normalSum, m->tauA, m->tauB, m->rASum, m->rBSum};

@@ -1485,12 +1485,13 @@ int pd_get_contact_colors(pd_sim* s, int* out, int cap) {
 int pd_get_manifolds(pd_sim* s, int* ids, double* data, int cap) {
   const int n = std::min(cap, s->nm);
   if (n > 0 && (ids || data)) {
-    std::vector<int2> ab(n);
+    std::vector<int2> ab(n), rr(n);
     std::vector<int> npt(n);
     std::vector<double4> f[5];
     for (auto& v : f) v.resize(n);
     guard([&] {
       s->readN(ab.data(), s->mAB.p, n);
+      s->readN(rr.data(), s->mReal.p, n);
       s->readN(npt.data(), s->mNp.p, n);
       s->readN(f[0].data(), s->mNs.p, n);
       s->readN(f[1].data(), s->mTA.p, n);
@@ -1500,9 +1501,11 @@ int pd_get_manifolds(pd_sim* s, int* ids, double* data, int cap) {
     });
     for (int k = 0; k < n; ++k) {
       if (ids) {
-        ids[3 * k] = ab[k].x;
-        ids[3 * k + 1] = ab[k].y;
-        ids[3 * k + 2] = npt[k];
+        ids[5 * k] = ab[k].x;
+        ids[5 * k + 1] = ab[k].y;
+        ids[5 * k + 2] = npt[k];
+        ids[5 * k + 3] = rr[k].x;
+        ids[5 * k + 4] = rr[k].y;
       }
       if (data)
         for (int j = 0; j < 5; ++j) {

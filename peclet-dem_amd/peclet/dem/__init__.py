@@ -377,7 +377,7 @@ class Simulation:
 
     def debug_manifolds(self):
         n = self._b.get_manifolds(self._h, None, None, 0)
-        ids = np.empty((n, 3), np.int32)
+        ids = np.empty((n, 5), np.int32)
         data = np.empty((n, 15), np.float64)
         self._b.get_manifolds(self._h, ids.ctypes.data_as(C.POINTER(C.c_int)), _ptr(data), n)
         return ids, data

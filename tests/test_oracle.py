@@ -1,0 +1,394 @@
+"""Pins the CPU oracle: reference test inputs, brute force, closed forms and independent algorithms.
+
+Each check is chosen so a plausible mistake (dropped term, wrong sign or index, swapped operand)
+fails it; none re-types the oracle's own formula.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle_binding import oracle_binding, oracle_sim
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def ptr(a, ct):
+    import ctypes as C
+
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def oracle_broadphase(pos, rad, n_real, margin, box_cap, brute=False):
+    import ctypes as C
+
+    b = oracle_binding().lib
+    fn = b.pdref_broadphase_bruteforce if brute else b.pdref_broadphase
+    fn.restype = C.c_int
+    dp = C.POINTER(C.c_double)
+    fn.argtypes = [dp, dp, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int), C.c_int]
+    pos = np.ascontiguousarray(pos, np.float64)
+    rad = np.ascontiguousarray(rad, np.float64)
+    n = fn(ptr(pos, C.c_double), ptr(rad, C.c_double), len(pos), n_real, margin, box_cap, None, 0)
+    out = np.zeros((max(n, 1), 2), np.int32)
+    fn(ptr(pos, C.c_double), ptr(rad, C.c_double), len(pos), n_real, margin, box_cap,
+       ptr(out, C.c_int), n)
+    return out[:n]
+
+
+# --------------------------------------------------------------------------- broadphase
+def test_broadphase_reference_inputs_match_brute_force():
+    """tests/arborx/test_broadphase_arborx.cpp:19-89: 4000 particles (3600 real), margin 0.05."""
+    d = np.fromfile(os.path.join(GOLD, "broadphase.bin"), np.float32).reshape(-1, 4)
+    pos, rad = d[:, :3].astype(np.float64), d[:, 3].astype(np.float64)
+    grid = oracle_broadphase(pos, rad, 3600, 0.05, 0.0)
+    brute = oracle_broadphase(pos, rad, 3600, 0.05, 0.0, brute=True)
+    assert np.array_equal(grid, brute)
+    # the reference test's own criterion (|dx| <= bi + bj per axis), evaluated in numpy
+    b = rad + 0.05
+    expect = []
+    for i in range(3600):
+        j = np.arange(i + 1, 4000)
+        s = b[i] + b[j]
+        ok = (np.abs(pos[i, 0] - pos[j, 0]) <= s) & (np.abs(pos[i, 1] - pos[j, 1]) <= s) & \
+             (np.abs(pos[i, 2] - pos[j, 2]) <= s)
+        expect += [(i, k) for k in j[ok]]
+    assert len(expect) > 1000
+    assert set(map(tuple, grid.tolist())) == set(expect)
+    # canonical order: lexicographic, unique
+    keys = grid[:, 0].astype(np.int64) * 10**6 + grid[:, 1]
+    assert np.all(np.diff(keys) > 0)
+
+
+def test_broadphase_parks_non_finite_and_oversized_boxes():
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, 10, (500, 3))
+    rad = rng.uniform(0.2, 0.5, 500)
+    pos[7] = np.nan
+    pos[11, 1] = np.nan
+    rad[13] = 1e6          # larger than the box cap -> parked
+    pos[17] = [1e12, 0, 0]  # far away, finite
+    grid = oracle_broadphase(pos, rad, 450, 0.05, 10.0)
+    brute = oracle_broadphase(pos, rad, 450, 0.05, 10.0, brute=True)
+    assert np.array_equal(grid, brute)
+    # the parked NaN boxes coincide at -1e30 with zero half-width, so they pair with each other
+    assert (7, 11) in set(map(tuple, grid.tolist()))
+    assert not any(13 in p and 7 not in p and 11 not in p for p in grid.tolist())
+
+
+# --------------------------------------------------------------------------- ghosts
+def test_ghost_counts_reference_inputs():
+    """tests/kokkos/test_periodicity.cpp:20-61: 600 reals in [-5,5]^3, band 0.5, all periodic."""
+    p = np.fromfile(os.path.join(GOLD, "periodicity.bin"), np.float32).reshape(-1, 3).astype(float)
+    s = oracle_sim(600)
+    s.set_sphere_shape(0.5)
+    s.set_domain((-5, -5, -5), (5, 5, 5))
+    s.enable_periodicity(True, True, True)
+    s.set_positions(p)
+    s.set_solver_iterations(0, 0)
+    s.step(1e-9)
+    near = (p < -5 + 0.5) | (p > 5 - 0.5)
+    expected = int(sum(2 ** k - 1 for k in near.sum(1)))
+    assert s.get_stats()["num_particles"] == 600 + expected
+    assert expected > 100
+
+
+# --------------------------------------------------------------------------- integration
+def test_ballistic_trapezoid_is_exact_for_constant_gravity():
+    s = oracle_sim(1)
+    s.set_sphere_shape(0.1)
+    s.set_domain(100, 100, 100, False, False, False)
+    x0 = np.array([[50.0, 50.0, 80.0]])
+    v0 = np.array([[0.3, -0.2, 1.5]])
+    s.set_positions(x0)
+    s.set_velocities(v0)
+    g = np.array([0.0, 0.0, -9.81])
+    s.set_gravity(*g)
+    dt, n = 1e-3, 200
+    for _ in range(n):
+        s.step(dt)
+    t = n * dt
+    np.testing.assert_allclose(s.get_positions()[0], x0[0] + v0[0] * t + 0.5 * g * t * t,
+                               rtol=0, atol=1e-12)
+    np.testing.assert_allclose(s.get_velocities()[0], v0[0] + g * t, rtol=0, atol=1e-12)
+
+
+def test_periodic_wrap():
+    s = oracle_sim(1)
+    s.set_sphere_shape(0.1)
+    s.set_domain(10, 10, 10, True, True, False)
+    s.set_positions(np.array([[9.95, 0.02, 5.0]]))
+    s.set_velocities(np.array([[1.0, -1.0, 0.0]]))
+    s.step(0.1)
+    np.testing.assert_allclose(s.get_positions()[0], [0.05, 9.92, 5.0], atol=1e-12)
+
+
+def test_growth_ramp():
+    s = oracle_sim(3)
+    s.set_positions(np.array([[0, 0, 0], [5, 0, 0], [10, 0, 0.0]]))
+    s.set_scales(np.array([1.0, 2.0, 3.0]))
+    s.set_growth_params(2.0, 0.25)
+    np.testing.assert_allclose(s.get_scales(), [0.25, 0.5, 0.75])
+    f = 0.25
+    for _ in range(3):
+        s.step(0.1)
+        f = min(1.0, f * math.exp(2.0 * 0.1))
+    assert s.get_growth_factor() == pytest.approx(f, rel=1e-15)
+    np.testing.assert_allclose(s.get_scales(), np.array([1.0, 2.0, 3.0]) * f, rtol=1e-15)
+
+
+def test_thermostat_berendsen_factor():
+    rng = np.random.default_rng(1)
+    n = 50
+    s = oracle_sim(n)
+    s.set_sphere_shape(0.01)
+    s.set_domain(100, 100, 100, False, False, False)
+    s.set_positions(rng.uniform(0, 100, (n, 3)))
+    v = rng.normal(0, 1.0, (n, 3))
+    s.set_velocities(v)
+    s.set_thermostat(0.3, 0.05, 1.0)
+    dt = 0.01
+    s.step(dt)
+    t_cur = (2.0 * 0.5 * (v ** 2).sum()) / (3.0 * n)
+    lam = math.sqrt(1.0 + dt / 0.05 * (0.3 / t_cur - 1.0))
+    np.testing.assert_allclose(s.get_velocities(), v * lam, rtol=1e-13)
+
+
+# --------------------------------------------------------------------------- contacts
+def two_spheres(gap, e=0.5, mu=0.0, g=(0, 0, 0), v=((0, 0, 0), (0, 0, 0)), iters=(1, 0)):
+    s = oracle_sim(2)
+    s.set_sphere_shape(0.5)
+    s.set_domain((-10, -10, -10), (10, 10, 10))
+    s.enable_periodicity(False, False, False)
+    s.set_positions(np.array([[-0.5 - gap / 2, 0.1, 0.2], [0.5 + gap / 2, 0.1, 0.2]]))
+    s.set_velocities(np.array(v, float))
+    s.set_gravity(*g)
+    s.set_material_params(e, 0.0, mu)
+    s.set_solver_iterations(*iters)
+    return s
+
+
+def test_overlap_removal_splits_equally():
+    delta = 0.02
+    s = two_spheres(-delta, iters=(1, 0))
+    s.step(1e-4)
+    p = s.get_positions()
+    np.testing.assert_allclose(p[0], [-0.5, 0.1, 0.2], atol=1e-12)
+    np.testing.assert_allclose(p[1], [0.5, 0.1, 0.2], atol=1e-12)
+    ids, data = s.debug_contacts()
+    assert ids.tolist() == [[0, 1]]
+    assert data[0, 9] == pytest.approx(-delta, abs=1e-12)           # signed distance
+    np.testing.assert_allclose(data[0, :3], [-1, 0, 0], atol=1e-9)   # normal on B's surface
+
+
+@pytest.mark.parametrize("e", [0.2, 0.5, 0.8, 0.95])
+@pytest.mark.parametrize("g", [0.0, -1e-6])
+def test_binary_restitution_exact(e, g):
+    """test_colored_gs.py:42-57 (g = 0, one-shot GS) and test_cone_friction.py:102-119 (PGS)."""
+    s = two_spheres(3.0, e=e, mu=0.5 if g else 0.0, g=(0, 0, g), v=((1, 0, 0), (-1, 0, 0)),
+                    iters=(10, 12))
+    for _ in range(400):
+        s.step(0.01)
+    v = s.get_velocities()
+    assert (v[1, 0] - v[0, 0]) / 2.0 == pytest.approx(e, abs=1e-6)
+    assert abs(v[:, 0].sum()) < 1e-12
+
+
+def sphere_on_floor(v0=(0, 0, 0), w0=(0, 0, 0), mu=0.5, e=0.0, beta=0.0, g=(0, 0, -10.0),
+                    inv_i=None, z=0.5):
+    s = oracle_sim(1)
+    s.set_sphere_shape(0.5)
+    s.set_domain((0, 0, -1), (200, 8, 6))
+    s.enable_periodicity(False, False, False)
+    s.add_plane(0, 0, 0, 0, 0, 1)
+    s.set_positions(np.array([[5.0, 4.0, z]]))
+    s.set_inv_mass(np.ones(1))
+    s.set_inv_inertia(np.full((1, 3), 1.0 / (0.4 * 0.25) if inv_i is None else inv_i))
+    s.set_velocities(np.array([v0], float))
+    s.set_angular_velocities(np.array([w0], float))
+    s.set_gravity(*g)
+    s.set_material_params(e, beta, mu)
+    s.set_solver_iterations(12, 8)
+    return s
+
+
+def test_resting_sphere_stays_put():
+    s = sphere_on_floor()
+    for _ in range(300):
+        s.step(0.005)
+    assert abs(s.get_velocities()[0, 2]) < 1e-9
+    assert s.get_positions()[0, 2] == pytest.approx(0.5, abs=1e-6)
+
+
+def test_slide_to_roll_five_sevenths():
+    """test_cone_friction.py:74-99,137-140: terminal v = 5/7 v0 and w r = v."""
+    s = sphere_on_floor(v0=(4.0, 0, 0))
+    for _ in range(1200):
+        s.step(0.005)
+    v = s.get_velocities()[0]
+    w = s.get_angular_velocities()[0]
+    assert v[0] / 4.0 == pytest.approx(5 / 7, abs=0.01)
+    assert w[1] * 0.5 / 4.0 == pytest.approx(5 / 7, abs=0.01)
+
+
+@pytest.mark.parametrize("mu,theory", [(0.3, 0.5 * 5 * 64), (0.6, 0.5 * 2 * 64)])
+def test_kinetic_slide_distance(mu, theory):
+    """test_cone_friction.py:49-71,129-135: locked rotation, a = gx - mu gz."""
+    s = sphere_on_floor(mu=mu, g=(8.0, 0, -10.0), inv_i=0.0)
+    for _ in range(800):
+        s.step(0.01)
+    d = s.get_positions()[0, 0] - 5.0
+    assert abs(d / theory - 1) < 0.05
+
+
+def test_walton_oblique_impact():
+    """test_cone_friction.py:142-188 (stick and slide regimes of the (e, mu, beta) law)."""
+    def oblique(vx, vz, e=0.6, mu=0.4, beta=0.5):
+        s = sphere_on_floor(v0=(vx, 0, -vz), mu=mu, e=e, beta=beta, g=(0, 0, -0.02), z=0.62)
+        for _ in range(400):
+            s.step(0.002)
+        v = s.get_velocities()[0]
+        w = s.get_angular_velocities()[0]
+        return v[0], v[2], v[0] - w[1] * 0.5
+
+    vxp, vzp, utp = oblique(1.0, 2.0)
+    assert abs(vzp - 1.2) < 0.03 and abs(utp + 0.5) < 0.04
+    assert abs(vxp - (1 - (2 / 7) * 1.5)) < 0.04
+    vxp, vzp, _ = oblique(6.0, 2.0)
+    assert abs(vzp - 1.2) < 0.03 and abs(vxp - (6.0 - 0.4 * 1.6 * 2.0)) < 0.06
+    vxp, _, utp = oblique(1.0, 2.0, beta=0.0)
+    assert abs(utp) < 0.03 and abs(vxp - (1 - 2 / 7)) < 0.04
+
+
+def test_free_cylinder_cloud_conserves_momentum():
+    """test_colored_gs.py:60-106: g = 0 cloud of cylinders; total linear momentum is conserved."""
+    n = 24
+    rng = np.random.default_rng(0)
+    s = oracle_sim(n)
+    s.initialize(shape_type=2, radius=0.5, height=1.0, thickness=0.2)
+    s.set_domain((-20, -20, -20), (20, 20, 20))
+    s.enable_periodicity(False, False, False)
+    s.set_material_params(0.7, 0.0, 0.0)
+    s.set_solver_iterations(8, 10)
+    s.set_positions(rng.uniform(-3, 3, (n, 3)))
+    v = rng.normal(0, 2.0, (n, 3))
+    s.set_velocities(v)
+    p0 = v.sum(0)
+    spun = 0.0
+    for i in range(300):
+        s.step(0.004)
+        spun = max(spun, np.abs(s.get_angular_velocities()).max())
+    assert spun > 0.1  # off-centre hits happened
+    np.testing.assert_allclose(s.get_velocities().sum(0), p0, atol=1e-11)
+
+
+# --------------------------------------------------------------------------- colouring
+def splitmix_priority(idx):
+    z = (idx + 0x9E3779B9) & 0xFFFFFFFF
+    z = ((z ^ (z >> 16)) * 0x21F0AAAD) & 0xFFFFFFFF
+    z = ((z ^ (z >> 15)) * 0x735A2D97) & 0xFFFFFFFF
+    z ^= z >> 15
+    return ((z & 0x7FFFFFFF) << 32) | idx
+
+
+def greedy_in_priority_order(ends, active):
+    """Sequential greedy colouring in decreasing priority: a different algorithm that must give the
+    Jones-Plassmann result (an edge commits only after every higher-priority neighbour)."""
+    colour = np.full(len(ends), -2, np.int64)
+    masks = {}
+    order = sorted((i for i in range(len(ends)) if active[i]), key=splitmix_priority, reverse=True)
+    for e in order:
+        a, b = ends[e]
+        forbidden = masks.get(a, 0) | (masks.get(b, 0) if b >= 0 else 0)
+        c = 0
+        while c < 62 and forbidden & (1 << c):
+            c += 1
+        colour[e] = c
+        masks[a] = masks.get(a, 0) | (1 << c)
+        if b >= 0:
+            masks[b] = masks.get(b, 0) | (1 << c)
+    return colour
+
+
+@pytest.fixture(scope="module")
+def dense_bed():
+    from scenes import apply_scene, make_scene
+
+    sc = make_scene("bed_periodic", 300, 9)
+    s = apply_scene(oracle_sim(300), sc)
+    for _ in range(3):
+        s.step(sc["dt"])
+    return s
+
+
+def test_contact_colouring_equals_priority_greedy(dense_bed):
+    ids, _ = dense_bed.debug_contacts()
+    col = dense_bed.debug_contact_colors()
+    ref = greedy_in_priority_order([tuple(x) for x in ids.tolist()], [True] * len(ids))
+    assert np.array_equal(col, ref)
+    assert col.max() >= 4
+
+
+def test_manifold_colouring_equals_priority_greedy(dense_bed):
+    ids, _ = dense_bed.debug_manifolds()
+    col = dense_bed.debug_manifold_colors()
+    assert dense_bed.get_stats()["num_particles"] > dense_bed.num_particles()  # ghosts present
+    ends_real = [(ra, rb) for _, _, _, ra, rb in ids.tolist()]
+    active = (col != -2).tolist()
+    # the oracle's own rule for -2: empty manifold or periodic duplicate (realA > realB)
+    for (a, b, npts, ra, rb), act in zip(ids.tolist(), active):
+        assert act == (npts > 0 and not (b >= 0 and ra > rb))
+    ref = greedy_in_priority_order(ends_real, active)
+    assert np.array_equal(col, ref)
+
+
+def test_manifold_sums_regrouped_independently(dense_bed):
+    """Group the contacts by canonical pair key in numpy and re-sum the manifold fields."""
+    cids, cd = dense_bed.debug_contacts()
+    mids, md = dense_bed.debug_manifolds()
+    a, b = cids[:, 0].astype(np.int64), cids[:, 1].astype(np.int64)
+    key = np.where(b < 0, (a << 32) | 0xFFFFFFFF, (np.minimum(a, b) << 32) | np.maximum(a, b))
+    order = np.argsort(key, kind="stable")
+    uk, start = np.unique(key[order], return_index=True)
+    assert len(uk) == len(mids)
+    n, ra, rb, dist = cd[:, 0:3], cd[:, 3:6], cd[:, 6:9], cd[:, 9]
+    act = dist <= 0
+    flip = (b >= 0) & (b < a)
+    na = np.where(flip[:, None], n, -n)
+    shift = n * dist[:, None] * 0.5
+    rcan = np.where(flip[:, None], rb + shift, ra - shift)
+    roth = np.where(flip[:, None], ra - shift, rb + shift)
+    fields = [na, np.cross(rcan, na), np.cross(roth, -na), rcan, roth]
+    bounds = list(start) + [len(order)]
+    for m in range(len(uk)):
+        sel = order[bounds[m]:bounds[m + 1]]
+        sel = sel[act[sel]]
+        assert mids[m, 2] == len(sel)
+        for f, arr in enumerate(fields):
+            np.testing.assert_allclose(md[m, 3 * f:3 * f + 3], arr[sel].sum(0), rtol=0,
+                                       atol=1e-15 + 1e-12 * np.abs(arr[sel]).sum())
+
+
+def test_grounded_levels_double_buffered_chain():
+    """A vertical stack on the floor: each sweep lifts the support level one body further up."""
+    n = 12
+    s = oracle_sim(n)
+    s.set_sphere_shape(0.5)
+    s.set_domain(10, 10, 40, False, False, False)
+    s.add_plane(0, 0, 0, 0, 0, 1)
+    s.set_positions(np.array([[5.0, 5.0, 0.5 + 0.999 * k] for k in range(n)]))
+    s.set_gravity(0, 0, -10.0)
+    s.set_solver_iterations(1, 1)
+    s.step(1e-4)
+    lv = s.debug_grounded()
+    expect = [256 - (k + 1) if k < 8 else 0 for k in range(n)]
+    assert lv.tolist() == expect
+
+
+def test_api_errors():
+    s = oracle_sim(2)
+    with pytest.raises(ValueError):
+        s.set_stabilization_mode("sideways")
+    with pytest.raises(ValueError):
+        s.set_pair_material(0, 9, 0.1, 0.1)
