@@ -1,24 +1,36 @@
 #!/usr/bin/env python3
 """bench.py — particle-substeps/s of the MI355X HIP XPBD substep (plain form, one GPU).
 
-Workload (default): gravity pour of 1,048,576 polydisperse spheres, radius ~ U[0.8, 1.2] mm,
-rho = 2500 kg/m^3, 160 x 160 mm periodic in x and y with a floor, seeded random non-overlapping
-start (random sequential addition in a 700 mm column), dt = 5e-6 s, mu = 0.5, e = 0.3, solver
-iterations (pos 12, vel 8), FP64 state.  A "step" is one substep (one demStep); W untimed
-substeps, then K timed substeps bracketed by device synchronisation.
+Workload (default, the 1M-sphere gravity pour): 1,048,576 polydisperse spheres, radius
+~ U[0.8, 1.2] mm, rho = 2500 kg/m^3, 160 x 160 mm periodic in x and y with a floor, seeded random
+non-overlapping start (random sequential addition in a 700 mm column, since the 400 mm column of
+the specification is above the RSA jamming density), dt = 5e-6 s, mu = 0.5, e = 0.3, solver
+iterations (pos 12, vel 8), FP64 state.  A "step" is one substep (one demStep).  The default K =
+4000 substeps is the whole job (500 steps x 8 substeps); W untimed substeps run first.
 
 The JSON line carries:
-  roofline     — the dominant kernel (the one with the largest summed time: the PGS velocity
-                 sweep k_vsweep or the position sweep k_psweep), achieved = algorithmic bytes it
-                 moved / its summed launch time from HIP events on its own stream, against the
-                 8 TB/s HBM peak.  Algorithmic bytes: 601 B per active manifold per velocity
-                 sweep, 388 B per contact per position sweep.
-  cpu_baseline — the CPU oracle (serial C restatement, 1 thread) timed on the same seeded input
-                 for a few substeps, rank 0 only.
+  roofline      the kernel with the largest summed time over the timed substeps, timed with HIP
+                events on the simulation's stream around each launch; achieved = its algorithmic
+                bytes per launch / its average launch time, against the 8 TB/s HBM peak.
+                Algorithmic bytes per unit (reads + writes once, FP64, int32 indices):
+                  k_vsweep (PGS)      601 per active manifold row
+                  k_psweep            388 per contact row
+                  k_bp_query1         36 per slot (order + box) + 4 per real (count)
+                                      + 8 per candidate pair staged
+                  k_np_pairs          156 per pair (pair, offset, both bodies' pose and scale)
+                                      + 104 per contact written
+                  k_predict           240 per real;  k_reintegrate  264 per slot
+                traffic = measured HBM bytes per launch of that kernel (rocprofv3 --pmc FETCH_SIZE
+                and WRITE_SIZE, scaled by the same counters on a plain copy of known size), read
+                from profiles/traffic_<workload>.json when present, else null.
+  cpu_baseline  the CPU oracle (serial C, one thread) on the same seeded input for a bounded
+                number of substeps.
+  substep_roofline  whole-substep algorithmic bytes from the per-substep counters.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -30,7 +42,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "peclet-dem_amd"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES = {"vsweep": 601, "psweep": 388}
 METRIC = "particle-substeps/s at 1M spheres: plain vs batched vs fwd+adjoint, %HBM roof"
 
 
@@ -38,13 +49,31 @@ def substep_bytes(st):
     """Algorithmic bytes of one substep from its counters (FP64 SoA, spheres)."""
     N = st["num_real"]
     G = st["num_particles"] - N
-    P, C, M = st["num_pairs"], st["num_contacts"], st["num_manifolds"]
-    b = 684 * N + 460 * G + 152 * P + 236 * C + 1661 * M
+    P, Cn, M = st["num_pairs"], st["num_contacts"], st["num_manifolds"]
+    b = 684 * N + 460 * G + 152 * P + 236 * Cn + 1661 * M
     b += 88 * st["uncolored_vel"] + 8 * N * st["rounds_vel"]
     b += 80 * st["uncolored_pos"] + 8 * (N + G) * st["rounds_pos"]
     b += 601 * (st["sweeps_vel"] + st["sweeps_stab"]) * st["active_manifolds"]
-    b += 388 * st["sweeps_pos"] * C
+    b += 388 * st["sweeps_pos"] * Cn
     return b
+
+
+def kernel_bytes(name, units, st):
+    """Algorithmic bytes of all launches of one kernel in one substep (units = rows processed)."""
+    N = st["num_real"]
+    if name == "k_vsweep":
+        return 601 * units
+    if name == "k_psweep":
+        return 388 * units
+    if name == "k_bp_query1":
+        return 36 * st["num_particles"] + 4 * N + 8 * st["num_pairs"]
+    if name == "k_np_pairs":
+        return 156 * st["num_pairs"] + 104 * st["num_contacts"]
+    if name == "k_predict":
+        return 240 * units
+    if name == "k_reintegrate":
+        return 264 * units
+    raise KeyError(name)
 
 
 def make_workload(kind, n, seed, column):
@@ -68,23 +97,68 @@ def run_cpu_baseline(bed, dt, substeps):
     return len(bed["pos"]) * substeps / t, t
 
 
+class KernelClock:
+    """Per-kernel HIP-event times of the library, differenced per substep."""
+
+    def __init__(self, sim):
+        self.sim = sim
+        self.count = sim._b.get_kernel_times(sim._h, None, None, None, 0)
+        self.names = [sim._b.kernel_name(k).decode() for k in range(self.count)]
+
+    def read(self):
+        ms = np.zeros(self.count)
+        launches = np.zeros(self.count, np.int64)
+        rows = np.zeros(self.count, np.int64)
+        self.sim._b.get_kernel_times(self.sim._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                     launches.ctypes.data_as(C.POINTER(C.c_long)),
+                                     rows.ctypes.data_as(C.POINTER(C.c_long)), self.count)
+        return ms, launches, rows
+
+
+def load_traffic(workload, kernel):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else float(k["hbm_bytes_per_launch"])
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of known size, for scaling the HBM byte counters."""
+    import torch
+
+    x = torch.empty(mib << 18, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"# copy probe: 3 copies of {mib} MiB", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32, help="timed substeps")
+    ap.add_argument("--steps", type=int, default=4000, help="timed substeps")
     ap.add_argument("--warmup", type=int, default=8, help="untimed substeps")
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--column", type=float, default=0.7, help="seeding column height (m)")
-    ap.add_argument("--cpu-substeps", type=int, default=2)
+    ap.add_argument("--cpu-substeps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run a plain copy of MIB MiB (scales the HBM byte counters)")
     ap.add_argument("--workload", choices=["pour", "lattice"], default="pour",
                     help="pour: the 1M-sphere gravity pour (default); lattice: a contact-rich "
                          "bed of spheres on a slightly overlapping cubic lattice")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
-              "multi-GPU spatial decomposition is a design, not an implementation", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in the design notes, not implemented",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -104,35 +178,43 @@ def main():
         sim.step(dt)
     sim._b.synchronize(sim._h)
     sim._b.set_timing(sim._h, 2)
+    clock = KernelClock(sim)
+    names = clock.names
+    kbytes = np.zeros(len(names))
+    prev_ms, prev_l, prev_r = clock.read()
     stats = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    report = max(1, args.steps // 8)
+    for k in range(args.steps):
         sim.step(dt)
-        stats.append(sim.get_stats())
+        st = sim.get_stats()
+        stats.append(st)
+        ms, launches, rows = clock.read()
+        for q, name in enumerate(names):
+            if launches[q] > prev_l[q]:
+                kbytes[q] += kernel_bytes(name, rows[q] - prev_r[q], st)
+        prev_ms, prev_l, prev_r = ms, launches, rows
+        if (k + 1) % report == 0:
+            print(f"# substep {k + 1}/{args.steps}", file=sys.stderr, flush=True)
     sim._b.synchronize(sim._h)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    ms = np.zeros(2)
-    launches = np.zeros(2, np.int64)
-    rows = np.zeros(2, np.int64)
-    import ctypes as C
-
-    sim._b.get_kernel_times(sim._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
-                            launches.ctypes.data_as(C.POINTER(C.c_long)),
-                            rows.ctypes.data_as(C.POINTER(C.c_long)), 2)
+    ms, launches, rows = clock.read()
     value = args.n * args.steps / t
-    names = ["k_vsweep", "k_psweep"]
     dom = int(np.argmax(ms))
-    per_row = BYTES["vsweep"] if dom == 0 else BYTES["psweep"]
-    bytes_per_launch = per_row * rows[dom] / max(launches[dom], 1)
-    avg_launch_s = ms[dom] / 1e3 / max(launches[dom], 1)
+    nl = max(int(launches[dom]), 1)
+    bytes_per_launch = kbytes[dom] / nl
+    avg_launch_s = ms[dom] / 1e3 / nl
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     total_bytes = sum(substep_bytes(s) for s in stats)
     stage = np.zeros(32)
     nst = sim._b.get_stage_times(sim._h, stage.ctypes.data_as(C.POINTER(C.c_double)), 32)
     stage_ms = {sim._b.stage_name(k).decode(): float(stage[k]) / args.steps for k in range(nst)}
     mean = {k: float(np.mean([s[k] for s in stats])) for k in stats[0]}
+    sim._b.set_timing(sim._h, 0)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
 
     cpu = None
     if not args.no_cpu:
@@ -156,8 +238,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"plain gravity pour, {args.n} polydisperse spheres r~U[0.8,1.2] mm, "
-                         "160x160 mm periodic xy + floor, RSA start in a 700 mm column, "
-                         "dt=5e-6 s, mu=0.5, e=0.3, iters pos 12 / vel 8"
+                         "160x160 mm periodic xy + floor, RSA start in a "
+                         f"{args.column * 1e3:.0f} mm column, dt=5e-6 s, mu=0.5, e=0.3, "
+                         "iters pos 12 / vel 8"
                          if args.workload == "pour" else
                          f"lattice bed, {args.n} spheres r~U[0.99,1.01] mm on a 1.996 r cubic "
                          "lattice, 160x160 mm periodic xy + floor, dt=5e-6 s, mu=0.5, e=0.3, "
@@ -172,7 +255,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": load_traffic(args.workload, names[dom]),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_launch_s * 1e6,
             "launches": int(launches[dom]),
@@ -183,7 +266,10 @@ def main():
             "achieved_GBs": total_bytes / t / 1e9,
             "frac": total_bytes / t / 1e9 / HBM_PEAK_GBS,
         },
-        "kernel_ms": {names[k]: float(ms[k]) for k in range(2)},
+        "kernels": {names[q]: {"ms": float(ms[q]), "launches": int(launches[q]),
+                               "avg_us": float(ms[q]) * 1e3 / max(int(launches[q]), 1),
+                               "GBs": (kbytes[q] / (ms[q] / 1e3) / 1e9) if ms[q] > 0 else 0.0}
+                    for q in range(len(names))},
         "stage_ms_per_substep": stage_ms,
         "counters_mean": mean,
     }

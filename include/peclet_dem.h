@@ -141,15 +141,18 @@ long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, in
                   const double* lo, const double* hi, double* out, int* placed);
 
 /* --- measurement ------------------------------------------------------------------------ */
-/* Timing with HIP events on the handle's stream: level 0 off, 1 per-stage times, 2 also one
- * event pair around every launch of the velocity-sweep (k_vsweep, PGS) and position-sweep
- * (k_psweep) kernels.  Values accumulate until pd_set_timing is called again. */
+/* Timing with HIP events on the handle's stream (no reference counterpart; the reference's
+ * get_profiling_info returns an empty dict, src/dem_bindings.cpp:385): level 0 off, 1 per-stage
+ * times, 2 also one event pair around every launch of the main kernels (PGS velocity sweep,
+ * position sweep, broadphase query, narrowphase, predict, re-integrate).  Values accumulate
+ * until pd_set_timing is called again. */
 int pd_set_timing(pd_sim* s, int level);
 int pd_get_stage_times(pd_sim* s, double* out_ms, int cap);
 const char* pd_stage_name(int stage);
-/* per kernel (0 = k_vsweep PGS, 1 = k_psweep): summed ms, launch count, rows (manifolds or
- * contacts) processed; returns the kernel count */
+/* per timed kernel k: summed ms, launch count and units processed (manifold rows, contact rows,
+ * particles, pairs, particles, slots); returns the number of timed kernels */
 int pd_get_kernel_times(pd_sim* s, double* ms, long* launches, long* rows, int cap);
+const char* pd_kernel_name(int k);
 
 #ifdef __cplusplus
 }

@@ -644,40 +644,59 @@ __global__ void k_cell_fill(Dev D, Params P) {
   D.order[k] = s;
   D.sbox[k] = saneBox(D.posPred[s], P.margin, P.boxCap);
 }
-template <bool WRITE>
-__global__ void k_bp_query(Dev D, Params P, Grid G) {
-  const int i = gtid();
-  if (i >= P.numReal) return;
-  const double4 Bi = saneBox(D.posPred[i], P.margin, P.boxCap);
+// Single-pass query.  Threads walk the cell-sorted order, so a wave's particles share their
+// 27-cell neighbourhood (cache-resident reads, little divergence).  Each real stages up to
+// BP_CAP partners in pairTmp[i * BP_CAP ...] and records its full count; k_bp_compact then
+// moves every segment to its scanned offset in ascending partner order.  A real with more than
+// BP_CAP partners is queried again by k_bp_compact, writing straight into the final list.
+constexpr int BP_CAP = 32;
+template <bool STAGE>
+__device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, const double4 Bi,
+                                          int2* out) {
   const int cx = cellCoord(Bi.x, G.lo.x, G.h, G.dim[0]);
   const int cy = cellCoord(Bi.y, G.lo.y, G.h, G.dim[1]);
   const int cz = cellCoord(Bi.z, G.lo.z, G.h, G.dim[2]);
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx + 1 < G.dim[0] ? cx + 1 : G.dim[0] - 1;
   int n = 0;
-  const int base = WRITE ? D.pairStart[i] : 0;
   for (int z = cz - 1; z <= cz + 1; ++z) {
     if (z < 0 || z >= G.dim[2]) continue;
     for (int y = cy - 1; y <= cy + 1; ++y) {
       if (y < 0 || y >= G.dim[1]) continue;
       const int row = (z * G.dim[1] + y) * G.dim[0];
-      const int x0 = cx > 0 ? cx - 1 : 0;
-      const int x1 = cx + 1 < G.dim[0] ? cx + 1 : G.dim[0] - 1;
-      // cells x0..x1 of one row are contiguous in the cell-sorted order
       const int k0 = D.cellStart[row + x0], k1 = D.cellStart[row + x1 + 1];
       for (int k = k0; k < k1; ++k) {
         const int j = D.order[k];
         if (j > i && boxOverlap(Bi, D.sbox[k])) {
-          if (WRITE) D.pairs[base + n] = make_int2(i, j);
+          if (!STAGE || n < BP_CAP) out[n] = make_int2(i, j);
           ++n;
         }
       }
     }
   }
-  if (!WRITE) {
-    D.pairCnt[i] = n;
-    return;
+  return n;
+}
+__global__ __launch_bounds__(TPB) void k_bp_query1(Dev D, Params P, Grid G, int2* pairTmp) {
+  const int k = gtid();
+  if (k >= P.numParticles) return;
+  const int i = D.order[k];
+  if (i >= P.numReal) return;
+  D.pairCnt[i] = bpQueryOne<true>(D, G, i, D.sbox[k], pairTmp + (size_t)i * BP_CAP);
+}
+__global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, Grid G,
+                                                    const int2* pairTmp) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  const int n = D.pairCnt[i];
+  if (n == 0) return;
+  int2* seg = D.pairs + D.pairStart[i];
+  if (n <= BP_CAP) {
+    const int2* src = pairTmp + (size_t)i * BP_CAP;
+    for (int a = 0; a < n; ++a) seg[a] = src[a];
+  } else {
+    bpQueryOne<false>(D, G, i, saneBox(D.posPred[i], P.margin, P.boxCap), seg);
   }
-  // sort this real's partners ascending (segments are short)
-  int2* seg = D.pairs + base;
+  // ascending partner order (segments are short)
   for (int a = 1; a < n; ++a) {
     const int2 x = seg[a];
     int b = a - 1;

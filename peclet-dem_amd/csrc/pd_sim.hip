@@ -121,7 +121,8 @@ struct pd_sim {
   int timing = 0;
   double stageMs[ST_COUNT] = {};
   hipEvent_t ev[ST_COUNT + 1] = {};
-  enum { K_VSWEEP, K_PSWEEP, K_COUNT };
+  // kernels timed per launch at level 2; rows = the units each launch processed
+  enum { K_VSWEEP, K_PSWEEP, K_BPQUERY, K_NPPAIRS, K_PREDICT, K_REINTEGRATE, K_COUNT };
   double kernMs[K_COUNT] = {};
   long kernLaunches[K_COUNT] = {};
   long kernRows[K_COUNT] = {};
@@ -167,7 +168,7 @@ struct pd_sim {
   DBuf<int> dCnt;
   DBuf<int> realIdx, ground, ground2, cellOf, cellCnt, cellStart, order, pairCnt, pairStart;
   DBuf<unsigned char> matId, mCode, vFlag;
-  DBuf<int2> pairs, cAB, mAB, mReal, mRange, vAB, pAB;
+  DBuf<int2> pairs, pairTmp, cAB, mAB, mReal, mRange, vAB, pAB;
   DBuf<int> ncnt, noff, cSlot, mCnt, mStart, mNp, mColor, cColor, vPerm, vRowOf, pPerm, pRowOf;
   DBuf<double4> cN, cRA, cRB, mNs, mTA, mTB, mRA, mRB, vN, vTA, vTB, vRA, vRB, vVt0, vLamT, pN,
       pRA, pRB, lLamT;
@@ -211,7 +212,7 @@ struct pd_sim {
     sbox.release(); vw0.release(); vw.release(); bc.release(); scale.release();
     targetScale.release(); pairTableD.release(); thermo.release(); realIdx.release();
     ground.release(); ground2.release(); cellOf.release(); cellCnt.release(); cellStart.release();
-    order.release(); pairCnt.release(); pairStart.release(); matId.release(); mCode.release();
+    order.release(); pairCnt.release(); pairStart.release(); pairTmp.release(); matId.release(); mCode.release();
     vFlag.release(); pairs.release(); cAB.release(); mAB.release(); mReal.release();
     mRange.release(); vAB.release(); pAB.release(); ncnt.release(); noff.release();
     cSlot.release(); mCnt.release(); mStart.release(); mNp.release(); mColor.release();
@@ -585,11 +586,14 @@ struct pd_sim {
     scan(cellCnt.p, cellStart.p, ncell);
     zero(cellCnt.p, ncell);
     k_cell_fill<<<nblk(numParticles), TPB, 0, st>>>(D, P);
-    k_bp_query<false><<<nblk(numReal), TPB, 0, st>>>(D, P, G);
+    pairTmp.ensure((size_t)numReal * BP_CAP + 1, st);
+    kernBegin();
+    k_bp_query1<<<nblk(numParticles), TPB, 0, st>>>(D, P, G, pairTmp.p);
+    kernEnd(K_BPQUERY, numParticles);
     np = scan(pairCnt.p, pairStart.p, numReal);
     pairs.ensure(np + 1, st);
     refreshDev();
-    k_bp_query<true><<<nblk(numReal), TPB, 0, st>>>(D, P, G);
+    k_bp_compact<<<nblk(numReal), TPB, 0, st>>>(D, P, G, pairTmp.p);
     HIPCHECK(hipGetLastError());
     mark(ST_NARROW);
 
@@ -608,7 +612,9 @@ struct pd_sim {
     cSlot.ensure(nc + 1, st);
     refreshDev();
     zero(bits.p, 1);
+    kernBegin();
     k_np_pairs<true><<<nblk(np), TPB, 0, st>>>(D, P, np);
+    kernEnd(K_NPPAIRS, np);
     k_np_planes<true><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
     HIPCHECK(hipGetLastError());
   }
@@ -775,7 +781,9 @@ struct pd_sim {
     const double margin = 0.1 * maxRad;
     numParticles = numReal;
     Params P = params(margin, maxRad);
+    kernBegin();
     k_predict<<<nblk(numReal), TPB, 0, st>>>(D, P);
+    kernEnd(K_PREDICT, numReal);
     // periodic ghosts, src/periodicity.hpp:43-94
     ghostCnt.ensure(numReal + 1, st);
     ghostOff.ensure(numReal + 1, st);
@@ -930,7 +938,9 @@ struct pd_sim {
     }
     mark(ST_REINT);
     P = params(margin, maxRad);
+    kernBegin();
     k_reintegrate<<<nblk(numParticles), TPB, 0, st>>>(D, P);
+    kernEnd(K_REINTEGRATE, numParticles);
     mark(ST_COLOR_P);
 
     // contact colouring over raw slots, src/solver_position.hpp:155-241
@@ -1696,6 +1706,11 @@ int pd_get_kernel_times(pd_sim* s, double* ms, long* launches, long* rows, int c
 int pd_get_stage_times(pd_sim* s, double* out, int cap) {
   for (int k = 0; k < ST_COUNT && k < cap; ++k) out[k] = s->stageMs[k];
   return ST_COUNT;
+}
+const char* pd_kernel_name(int k) {
+  static const char* names[pd_sim::K_COUNT] = {"k_vsweep",  "k_psweep",  "k_bp_query1",
+                                               "k_np_pairs", "k_predict", "k_reintegrate"};
+  return (k >= 0 && k < pd_sim::K_COUNT) ? names[k] : "";
 }
 const char* pd_stage_name(int stage) {
   return (stage >= 0 && stage < ST_COUNT) ? kStageNames[stage] : "";

@@ -117,6 +117,7 @@ PRODUCT_SIGNATURES = {
     "get_stage_times": (_i, [_p, _dp, _i]),
     "stage_name": (C.c_char_p, [_i]),
     "get_kernel_times": (_i, [_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long), _i]),
+    "kernel_name": (C.c_char_p, [_i]),
     "rsa_place": (C.c_long, [_dp, C.c_long, _dp, _i, _i, _dp, _dp, _dp, _ip]),
 }
 

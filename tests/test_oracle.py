@@ -392,3 +392,74 @@ def test_api_errors():
         s.set_stabilization_mode("sideways")
     with pytest.raises(ValueError):
         s.set_pair_material(0, 9, 0.1, 0.1)
+
+
+# --------------------------------------------------------------------------- shapes
+def cylinder_count(r, h, t):
+    """Point count of the reference generator (src/shapes_portable.hpp:21-76, spacing rule
+    src/sim.hpp:958-964) evaluated from its ceil formulas."""
+    md = min(r, t) if min(r, t) >= 1e-4 else r
+    sp = max(min(r * 0.3, md * 0.5), 1e-3)
+    nv = math.ceil(h / sp)
+    n = math.ceil(2 * math.pi * r / sp) * (nv + 1)
+    n += math.ceil(2 * math.pi * (r - t) / sp) * (nv + 1) if t > 0 else 0
+    nr = math.ceil(t / sp)
+    for i in range(nr + 1):
+        n += 2 * math.ceil(2 * math.pi * (r - t + t * i / nr) / sp)
+    return n
+
+
+@pytest.mark.parametrize("dims,count", [((2e-3, 4e-3, 0.8e-3), 147), ((2.0, 4.0, 0.8), 715)])
+def test_hollow_cylinder_shell_on_surface(dims, count):
+    r, h, t = dims
+    s = oracle_sim(1)
+    s.initialize(shape_type=2, radius=r, height=h, thickness=t)
+    pts = s.get_shell_points()
+    assert len(pts) == count == cylinder_count(r, h, t)
+    rho = np.hypot(pts[:, 0], pts[:, 2])
+    on_wall = (np.isclose(rho, r, rtol=1e-6) | np.isclose(rho, r - t, rtol=1e-6)) & \
+        (np.abs(pts[:, 1]) <= h / 2 * (1 + 1e-9))
+    on_cap = np.isclose(np.abs(pts[:, 1]), h / 2, rtol=1e-9) & (rho >= (r - t) * (1 - 1e-6)) & \
+        (rho <= r * (1 + 1e-6))
+    assert np.all(on_wall | on_cap)
+
+
+def test_box_shell_on_surface_with_corners():
+    s = oracle_sim(1)
+    s.initialize(shape_type=3, radius=1.0)
+    pts = s.get_shell_points()
+    m = np.abs(pts).max(1)
+    np.testing.assert_allclose(m, 1.0, rtol=1e-12)
+    corners = {tuple(c) for c in np.sign(pts[np.all(np.isclose(np.abs(pts), 1.0), 1)]).tolist()}
+    assert len(corners) == 8
+
+
+def test_cylinder_standing_on_plane_contacts_bottom_ring():
+    """Axis along the plane normal: exactly the shell points at y = -h/2 (bottom cap annuli and
+    the j = 0 rows of both walls) lie within the margin, each at the same signed distance."""
+    r, h, t = 2.0, 4.0, 0.8
+    s = oracle_sim(1)
+    s.initialize(shape_type=2, radius=r, height=h, thickness=t)
+    s.set_domain((-10, -10, -10), (10, 10, 10))
+    s.enable_periodicity(False, False, False)
+    s.add_plane(0, 0, 0, 0, 1, 0)
+    sink = 0.01
+    s.set_positions(np.array([[0.0, h / 2 - sink, 0.0]]))
+    s.set_solver_iterations(0, 0)
+    s.step(1e-9)
+    ids, data = s.debug_contacts()
+    pts = s.get_shell_points()
+    bottom = np.isclose(pts[:, 1], -h / 2)
+    assert len(ids) == int(bottom.sum()) > 20
+    assert np.all(ids[:, 1] == -1)
+    np.testing.assert_allclose(data[:, 9], -sink, atol=1e-12)
+    np.testing.assert_allclose(data[:, 0:3], np.tile([0, 1, 0], (len(ids), 1)), atol=1e-12)
+
+
+def test_ledger_sorted_and_cone_bounded(dense_bed):
+    keys, lam, lamT, pimp = dense_bed.debug_ledger()
+    live = keys != np.uint64(0xFFFFFFFFFFFFFFFF)
+    assert live.sum() > 50
+    assert np.all(np.diff(keys.astype(np.float64)) >= 0)
+    assert np.all(lam[live] >= 0)
+    assert np.all(np.isfinite(lamT[live]))
