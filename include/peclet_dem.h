@@ -140,6 +140,23 @@ int pd_write_lammps_dump(const char* filename, int step, const double* pos, cons
 long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, int start,
                   const double* lo, const double* hi, double* out, int* placed);
 
+/* --- SDF walls and the Hertz–Mindlin engine --------------------------------------------- */
+/* addSdfWall(grid, nx, ny, nz, origin, spacing, restitution, friction), src/sim.hpp:1242-1277:
+ * flat x-fastest samples, positive in the void; returns the wall index (>= 0) or a status */
+int pd_add_sdf_wall(pd_sim* s, const double* grid, int nx, int ny, int nz, double ox, double oy,
+                    double oz, double sx, double sy, double sz, double restitution,
+                    double friction);
+/* setWallVelocity(wall, linVel, angVel, center), src/sim.hpp:1280-1287 */
+int pd_set_wall_velocity(pd_sim* s, int wid, double lx, double ly, double lz, double ax,
+                         double ay, double az, double cx, double cy, double cz);
+int pd_set_wall_material_id(pd_sim* s, int wid, int mat); /* setWallMaterialId, :1217-1224 */
+int pd_num_walls(pd_sim* s);
+int pd_set_hertz_material(pd_sim* s, int mat, double youngs, double poisson); /* :1153-1165 */
+/* stepHertz(dt, substeps, skin_frac) -> demStepHertz, src/sim.hpp:1167-1170, :628-750 */
+int pd_step_hertz(pd_sim* s, double dt, int nsteps, double skin_frac);
+/* diagnostic: the cached Hertz pair list (N,2) and shear histories (N,3); returns the count */
+int pd_get_hertz_pairs(pd_sim* s, int* pairs, double* xi, int cap);
+
 /* --- measurement ------------------------------------------------------------------------ */
 /* Timing with HIP events on the handle's stream (no reference counterpart; the reference's
  * get_profiling_info returns an empty dict, src/dem_bindings.cpp:385): level 0 off, 1 per-stage

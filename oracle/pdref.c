@@ -136,6 +136,18 @@ typedef struct {
 
 #define KMAXMAT 8
 #define MAXPLANES 8
+#define MAXWALLS 8
+#define HZ_MAXWALLS 4 /* Particles::kHertzMaxWalls, src/particles.hpp:130 */
+
+/* static world-space SDF wall, src/narrowphase.hpp:59-76 */
+typedef struct {
+  int nx, ny, nz;
+  long off;
+  V3 origin, invSpacing;
+  V3 linVel, angVel, center;
+  real restitution, friction;
+  int materialId;
+} Wall;
 
 typedef struct Sim {
   int capacity;
@@ -185,6 +197,23 @@ typedef struct Sim {
   real maxOverlap, maxApproach;
   int numColorsVel, numColorsPos;
   pdref_stats stats;
+  /* SDF walls (src/sim.hpp:1242-1287) */
+  Wall walls[MAXWALLS];
+  int numWalls;
+  real* wallGrid;
+  long wallGridN;
+  /* Hertz–Mindlin engine state (src/particles.hpp:113-130) */
+  real hertzE[KMAXMAT], hertzNu[KMAXMAT];
+  real hertzContactRadiusFrac;
+  int hNumPairs, hPrevCount, hPairCap, hSlotCap, hCandCap;
+  int* hPairs;
+  uint64_t *hKeys, *hPrevKeys;
+  V3 *hXi, *hPrevXi;
+  real* hSn;
+  V3 *hRefPos, *hForce, *hTorque, *hXiWall;
+  real* hSnWall;
+  int* hWallCand;
+  int hNumWallCand;
 } Sim;
 
 static void* xrealloc(void* p, size_t n) {
@@ -427,6 +456,9 @@ void* pdref_create(int capacity) {
   S->useGS = 1;
   S->stabMode = 1;
   S->defaultInvI = mk3(2.5, 2.5, 2.5);
+  for (int m = 0; m < KMAXMAT; ++m) { S->hertzE[m] = 1.0e9; S->hertzNu[m] = 0.25; }
+  S->hertzContactRadiusFrac = 0.5;
+  S->hNumPairs = -1;
   ensureSlots(S, capacity > 0 ? capacity : 1);
   pdref_initialize_shape(S, SHAPE_SPHERE, 1.0, 0.0, 0.0);
   return S;
@@ -446,6 +478,9 @@ void pdref_destroy(void* s) {
   free(S->sideFlags); free(S->lambdaAcc); free(S->posImpulse); free(S->vn0);
   free(S->posLambda); free(S->lambdaT); free(S->vt0); free(S->prevKeys);
   free(S->prevLambda); free(S->prevPosImpulse); free(S->prevLambdaT);
+  free(S->wallGrid); free(S->hPairs); free(S->hKeys); free(S->hPrevKeys); free(S->hXi);
+  free(S->hPrevXi); free(S->hSn); free(S->hRefPos); free(S->hForce); free(S->hTorque);
+  free(S->hXiWall); free(S->hSnWall); free(S->hWallCand);
   free(S);
 }
 
@@ -538,6 +573,8 @@ int pdref_set_positions(void* s, const double* xyz, int n) {
   ensureGrounded(S, n);
   S->numReal = n;
   S->numParticles = n;
+  S->hNumPairs = -1; /* particle indices changed: the Hertz pair cache is stale, :1294-1295 */
+  S->hPrevCount = 0;
   for (int i = 0; i < n; ++i) {
     S->pos[i] = mk3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
     S->quat[i].x = 0; S->quat[i].y = 0; S->quat[i].z = 0; S->quat[i].w = 1;
@@ -2065,6 +2102,7 @@ static int demStep(Sim* S) {
   memset(&S->stats, 0, sizeof S->stats);
   if (S->stabMode == 2 || S->stabMode == 4)
     return fail("stabilization modes 'multilevel' and 'ordered' are not implemented");
+  if (S->numWalls > 0) return fail("SDF walls in the XPBD substep are not implemented");
   /* growth ramp, src/sim.hpp:100-108 */
   if (S->growthFactor != -1.0 && S->growthRate != 0.0) {
     S->growthFactor *= exp(S->growthRate * S->dt);
@@ -2341,4 +2379,601 @@ int pdref_get_grounded(void* s, int* out, int n) {
   Sim* S = asSim(s);
   for (int i = 0; i < S->numReal && i < n && i < S->groundedCap; ++i) out[i] = S->grounded[i];
   return S->numReal;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* SDF walls — src/sim.hpp:1216-1287; trilinear sampling src/narrowphase.hpp:137-165            */
+/* ------------------------------------------------------------------------------------------ */
+int pdref_add_sdf_wall(void* s, const double* grid, int nx, int ny, int nz, double ox, double oy,
+                       double oz, double sx, double sy, double sz, double restitution,
+                       double friction) {
+  Sim* S = asSim(s);
+  if (nx < 2 || ny < 2 || nz < 2) return fail("addSdfWall: grid dims must be >= 2 on each axis");
+  if (S->numWalls >= MAXWALLS) return fail("addSdfWall: at most 8 walls");
+  const long cells = (long)nx * ny * nz;
+  Wall* w = &S->walls[S->numWalls];
+  memset(w, 0, sizeof *w);
+  w->nx = nx; w->ny = ny; w->nz = nz;
+  w->off = S->wallGridN;
+  w->origin = mk3(ox, oy, oz);
+  w->invSpacing = mk3(sx > 0 ? 1.0 / sx : 0.0, sy > 0 ? 1.0 / sy : 0.0, sz > 0 ? 1.0 / sz : 0.0);
+  w->restitution = restitution;
+  w->friction = friction;
+  w->materialId = -1;
+  S->wallGrid = (real*)xrealloc(S->wallGrid, sizeof(real) * (size_t)(S->wallGridN + cells));
+  memcpy(S->wallGrid + S->wallGridN, grid, sizeof(real) * (size_t)cells);
+  S->wallGridN += cells;
+  return S->numWalls++;
+}
+int pdref_set_wall_velocity(void* s, int wid, double lx, double ly, double lz, double ax,
+                            double ay, double az, double cx, double cy, double cz) {
+  Sim* S = asSim(s); /* src/sim.hpp:1280-1287 */
+  if (wid < 0 || wid >= S->numWalls) return fail("setWallVelocity: wall index out of range");
+  S->walls[wid].linVel = mk3(lx, ly, lz);
+  S->walls[wid].angVel = mk3(ax, ay, az);
+  S->walls[wid].center = mk3(cx, cy, cz);
+  return 0;
+}
+int pdref_set_wall_material_id(void* s, int wid, int mat) {
+  Sim* S = asSim(s); /* src/sim.hpp:1217-1224 */
+  if (wid < 0 || wid >= S->numWalls) return badarg("wall index out of range");
+  S->walls[wid].materialId = mat;
+  return 0;
+}
+int pdref_num_walls(void* s) { return asSim(s)->numWalls; }
+
+static real sampleWallSdf(V3 p, const Wall* w, const real* grid) {
+  const real fx = (p.x - w->origin.x) * w->invSpacing.x;
+  const real fy = (p.y - w->origin.y) * w->invSpacing.y;
+  const real fz = (p.z - w->origin.z) * w->invSpacing.z;
+  const real cx = fmin(fmax(fx, 0.0), (real)(w->nx - 1));
+  const real cy = fmin(fmax(fy, 0.0), (real)(w->ny - 1));
+  const real cz = fmin(fmax(fz, 0.0), (real)(w->nz - 1));
+  const int ix = (int)cx, iy = (int)cy, iz = (int)cz;
+  const int ix1 = ix < w->nx - 1 ? ix + 1 : ix;
+  const int iy1 = iy < w->ny - 1 ? iy + 1 : iy;
+  const int iz1 = iz < w->nz - 1 ? iz + 1 : iz;
+  const real tx = cx - ix, ty = cy - iy, tz = cz - iz;
+  const long nxny = (long)w->nx * w->ny;
+  const real* g = grid + w->off;
+#define AT(x, y, z) g[(long)(z) * nxny + (long)(y) * w->nx + (x)]
+  const real c00 = AT(ix, iy, iz) * (1.0 - tx) + AT(ix1, iy, iz) * tx;
+  const real c10 = AT(ix, iy1, iz) * (1.0 - tx) + AT(ix1, iy1, iz) * tx;
+  const real c01 = AT(ix, iy, iz1) * (1.0 - tx) + AT(ix1, iy, iz1) * tx;
+  const real c11 = AT(ix, iy1, iz1) * (1.0 - tx) + AT(ix1, iy1, iz1) * tx;
+#undef AT
+  const real c0 = c00 * (1.0 - ty) + c10 * ty;
+  const real c1 = c01 * (1.0 - ty) + c11 * ty;
+  const real val = c0 * (1.0 - tz) + c1 * tz;
+  const real rx = (w->invSpacing.x > 0.0) ? (fx - cx) / w->invSpacing.x : 0.0;
+  const real ry = (w->invSpacing.y > 0.0) ? (fy - cy) / w->invSpacing.y : 0.0;
+  const real rz = (w->invSpacing.z > 0.0) ? (fz - cz) / w->invSpacing.z : 0.0;
+  return val - sqrt(rx * rx + ry * ry + rz * rz);
+}
+static V3 wallGradient(V3 p, const Wall* w, const real* grid) {
+  const real hx = 0.5 / w->invSpacing.x;
+  return mk3(sampleWallSdf(mk3(p.x + hx, p.y, p.z), w, grid) -
+                 sampleWallSdf(mk3(p.x - hx, p.y, p.z), w, grid),
+             sampleWallSdf(mk3(p.x, p.y + hx, p.z), w, grid) -
+                 sampleWallSdf(mk3(p.x, p.y - hx, p.z), w, grid),
+             sampleWallSdf(mk3(p.x, p.y, p.z + hx), w, grid) -
+                 sampleWallSdf(mk3(p.x, p.y, p.z - hx), w, grid));
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Hertz–Mindlin soft-sphere engine — src/solver_hertz.hpp, driver src/sim.hpp:543-750          */
+/* FP64 constants: sqrt(5/6) and pi^2 in full precision (the reference writes them as floats). */
+/* ------------------------------------------------------------------------------------------ */
+#define HZ_SQRT56 0.91287092917527690
+#define HZ_PI2 9.8696044010893586
+
+int pdref_set_hertz_material(void* s, int mat, double youngs, double poisson) {
+  Sim* S = asSim(s); /* src/sim.hpp:1153-1165 */
+  if (mat < 0 || mat >= KMAXMAT) return badarg("material id out of range");
+  S->hertzE[mat] = youngs;
+  S->hertzNu[mat] = poisson;
+  return 0;
+}
+
+/* src/solver_hertz.hpp:27-37 */
+static void hertzPairEMu(const Sim* S, int ma, int mb, real* e, real* mu) {
+  if (S->hasPairTable) {
+    const int t = (ma * KMAXMAT + mb) * 2;
+    *e = S->pairTable[t];
+    *mu = S->pairTable[t + 1];
+  } else {
+    *e = S->restitutionNormal;
+    *mu = S->frictionDynamic;
+  }
+}
+/* src/solver_hertz.hpp:41-48 */
+static real hertzBetaD(real e) {
+  if (e <= 0.0) return -1.0;
+  if (e >= 1.0) return 0.0;
+  const real le = log(e);
+  return le / sqrt(le * le + HZ_PI2);
+}
+static void hertzStars(const Sim* S, int ma, int mb, real* eStar, real* gStar) {
+  const real Ei = S->hertzE[ma], Ej = S->hertzE[mb];
+  const real ni = S->hertzNu[ma], nj = S->hertzNu[mb];
+  *eStar = 1.0 / ((1.0 - ni * ni) / Ei + (1.0 - nj * nj) / Ej);
+  *gStar = 1.0 / (2.0 * (2.0 - ni) * (1.0 + ni) / Ei + 2.0 * (2.0 - nj) * (1.0 + nj) / Ej);
+}
+/* src/solver_hertz.hpp:53-84: force on A at the contact point; updates the shear history */
+static V3 hertzForce(real delta, V3 nhat, V3 vrel, real rStar, real mStar, real eStar,
+                     real gStar, real bd, real mu, real dt, V3* xi) {
+  const real sq = sqrt(rStar * delta);
+  const real sn = 2.0 * eStar * sq;
+  const real st = 8.0 * gStar * sq;
+  const real kn = (4.0 / 3.0) * eStar * sq;
+  const real etaN = -2.0 * HZ_SQRT56 * bd * sqrt(sn * mStar);
+  const real etaT = -2.0 * HZ_SQRT56 * bd * sqrt(st * mStar);
+  const real vn = dot3(vrel, nhat);
+  const real fnSpring = kn * delta;
+  const real fn = fnSpring - etaN * vn;
+  const V3 vt = sub3(vrel, scale3(nhat, vn));
+  V3 x = add3(*xi, scale3(vt, dt));
+  x = sub3(x, scale3(nhat, dot3(x, nhat)));
+  V3 ft = add3(scale3(x, -st), scale3(vt, -etaT));
+  const real ftLen = sqrt(dot3(ft, ft));
+  const real ftMax = mu * fabs(fnSpring);
+  if (ftLen > ftMax) {
+    const real sc = (ftLen > 1e-20) ? ftMax / ftLen : 0.0;
+    ft = scale3(ft, sc);
+    if (st > 1e-20) x = scale3(add3(ft, scale3(vt, etaT)), -1.0 / st);
+  }
+  *xi = x;
+  return add3(scale3(nhat, fn), ft);
+}
+static void hzAccum(Sim* S, int i, V3 f, V3 t) {
+  S->hForce[i] = add3(S->hForce[i], f);
+  S->hTorque[i] = add3(S->hTorque[i], t);
+}
+static V3 neg3(V3 a) { return mk3(-a.x, -a.y, -a.z); }
+
+/* spheres: src/solver_hertz.hpp:88-144 (accumulation in pair order) */
+static void hertzPairForces(Sim* S, real dt) {
+  for (int idx = 0; idx < S->hNumPairs; ++idx) {
+    const int i = S->hPairs[2 * idx], j = S->hPairs[2 * idx + 1];
+    const V3 dx = sub3(S->pos[i], S->pos[j]);
+    const real d = sqrt(dot3(dx, dx));
+    const real ri = S->rad[i], rj = S->rad[j];
+    const real delta = ri + rj - d;
+    if (delta <= 0.0 || d < 1e-12) {
+      S->hXi[idx] = mk3(0, 0, 0);
+      continue;
+    }
+    const V3 nhat = scale3(dx, 1.0 / d);
+    const V3 rA = scale3(nhat, -(ri - 0.5 * delta));
+    const V3 rB = scale3(nhat, (rj - 0.5 * delta));
+    const V3 vrel = sub3(add3(S->vel[i], cross3(S->angVel[i], rA)),
+                         add3(S->vel[j], cross3(S->angVel[j], rB)));
+    const int ma = S->materialId[i], mb = S->materialId[j];
+    real e, mu, eStar, gStar;
+    hertzPairEMu(S, ma, mb, &e, &mu);
+    hertzStars(S, ma, mb, &eStar, &gStar);
+    const real rStar = ri * rj / (ri + rj);
+    const real imSum = S->invMass[i] + S->invMass[j];
+    const real mStar = (imSum > 0.0) ? 1.0 / imSum : 0.0;
+    const V3 f = hertzForce(delta, nhat, vrel, rStar, mStar, eStar, gStar, hertzBetaD(e), mu, dt,
+                            &S->hXi[idx]);
+    hzAccum(S, i, f, cross3(rA, f));
+    hzAccum(S, j, neg3(f), cross3(rB, scale3(f, -1.0)));
+  }
+}
+
+/* point-shell shapes: src/solver_hertz.hpp:454-606 (one Mindlin patch history per pair) */
+static void hertzShapePairForces(Sim* S, real dt) {
+  const real frac = S->hertzContactRadiusFrac;
+  const int sphere = S->shapeType == SHAPE_SPHERE;
+  for (int idx = 0; idx < S->hNumPairs; ++idx) {
+    const int idA = S->hPairs[2 * idx], idB = S->hPairs[2 * idx + 1];
+    const V3 posA = S->pos[idA], posB = S->pos[idB];
+    const Q4 qA = S->quat[idA], qB = S->quat[idB];
+    const real effScaleA = S->scale[idA] * S->globalScale, effScaleB = S->scale[idB] * S->globalScale;
+    const int countA = S->numShell;
+    const int iter = countA > 0 ? countA : 1;
+    const int ma = S->materialId[idA], mb = S->materialId[idB];
+    real e, mu, eStar, gStar;
+    hertzPairEMu(S, ma, mb, &e, &mu);
+    hertzStars(S, ma, mb, &eStar, &gStar);
+    const real rcA = (sphere ? 1.0 : frac) * S->shapeParams[0] * effScaleA;
+    const real rcB = (sphere ? 1.0 : frac) * S->shapeParams[0] * effScaleB;
+    const real rStar = rcA * rcB / (rcA + rcB);
+    const real imSum = S->invMass[idA] + S->invMass[idB];
+    const real mStar = (imSum > 0.0) ? 1.0 / imSum : 0.0;
+    const real bd = hertzBetaD(e);
+    const V3 vA = S->vel[idA], wA = S->angVel[idA], vB = S->vel[idB], wB = S->angVel[idB];
+    real fnSum = 0.0, snSum = 0.0, dMax = 0.0;
+    V3 cSum = mk3(0, 0, 0), nSum = mk3(0, 0, 0), fAcc = mk3(0, 0, 0);
+    V3 tAccA = mk3(0, 0, 0), tAccB = mk3(0, 0, 0);
+    const real snTotPrev = S->hSn[idx];
+    const real dampC = (mStar > 0.0) ? -2.0 * HZ_SQRT56 * bd * sqrt(mStar) : 0.0;
+    for (int k = 0; k < iter; ++k) {
+      V3 pLocalA = mk3(0, 0, 0);
+      real pointRadius = 0.0;
+      if (countA > 0) pLocalA = S->shell[k];
+      else if (sphere) pointRadius = S->shapeParams[0] * effScaleA;
+      const V3 pWorld = add3(posA, rotate(qA, scale3(pLocalA, effScaleA)));
+      const V3 pCanB = scale3(invRotate(qB, sub3(pWorld, posB)), 1.0 / effScaleB);
+      const real dist = sdfShape(S, pCanB) * effScaleB;
+      const real effDist = dist - pointRadius;
+      if (effDist >= 0.0) continue;
+      const real delta = -effDist;
+      const real eps = 1e-4;
+      V3 nLoc = mk3(sdfShape(S, mk3(pCanB.x + eps, pCanB.y, pCanB.z)) -
+                        sdfShape(S, mk3(pCanB.x - eps, pCanB.y, pCanB.z)),
+                    sdfShape(S, mk3(pCanB.x, pCanB.y + eps, pCanB.z)) -
+                        sdfShape(S, mk3(pCanB.x, pCanB.y - eps, pCanB.z)),
+                    sdfShape(S, mk3(pCanB.x, pCanB.y, pCanB.z + eps)) -
+                        sdfShape(S, mk3(pCanB.x, pCanB.y, pCanB.z - eps)));
+      const real len = len3(nLoc);
+      nLoc = (len > 1e-9) ? scale3(nLoc, 1.0 / len) : mk3(0, 1, 0);
+      const V3 nW = rotate(qB, nLoc);
+      const V3 pSurfA = sub3(pWorld, scale3(nW, pointRadius));
+      const V3 rA = sub3(pSurfA, posA);
+      const V3 rB = sub3(add3(pSurfA, scale3(nW, delta)), posB);
+      const V3 vrelP = sub3(add3(vA, cross3(wA, rA)), add3(vB, cross3(wB, rB)));
+      const real vnP = dot3(vrelP, nW);
+      const real sq = sqrt(rStar * delta);
+      const real fnSpring = (4.0 / 3.0) * eStar * sq * delta;
+      const real snI = 2.0 * eStar * sq;
+      const real snTot = (snTotPrev > snI) ? snTotPrev : snI;
+      const real etaI = dampC * snI / sqrt(snTot);
+      const V3 f = scale3(nW, fnSpring - etaI * vnP);
+      fAcc = add3(fAcc, f);
+      tAccA = add3(tAccA, cross3(rA, f));
+      tAccB = add3(tAccB, cross3(rB, scale3(f, -1.0)));
+      fnSum += fnSpring;
+      snSum += 2.0 * eStar * sq;
+      cSum = add3(cSum, scale3(pSurfA, fnSpring));
+      nSum = add3(nSum, scale3(nW, fnSpring));
+      if (delta > dMax) dMax = delta;
+    }
+    if (fnSum <= 0.0) {
+      S->hXi[idx] = mk3(0, 0, 0);
+      S->hSn[idx] = 0.0;
+      continue;
+    }
+    const V3 cpt = scale3(cSum, 1.0 / fnSum);
+    V3 nhat = nSum;
+    const real nl = len3(nhat);
+    if (nl > 1e-9) nhat = scale3(nhat, 1.0 / nl);
+    const V3 rA = sub3(cpt, posA), rB = sub3(cpt, posB);
+    const V3 vrel = sub3(add3(vA, cross3(wA, rA)), add3(vB, cross3(wB, rB)));
+    S->hSn[idx] = snSum;
+    const V3 vt = sub3(vrel, scale3(nhat, dot3(vrel, nhat)));
+    const real st = 8.0 * gStar * sqrt(rStar * dMax);
+    const real etaT = -2.0 * HZ_SQRT56 * bd * sqrt(st * mStar);
+    V3 x = S->hXi[idx];
+    x = add3(x, scale3(vt, dt));
+    x = sub3(x, scale3(nhat, dot3(x, nhat)));
+    V3 ft = add3(scale3(x, -st), scale3(vt, -etaT));
+    const real ftLen = len3(ft);
+    const real ftMax = mu * fnSum;
+    if (ftLen > ftMax) {
+      const real sc = (ftLen > 1e-20) ? ftMax / ftLen : 0.0;
+      ft = scale3(ft, sc);
+      if (st > 1e-20) x = scale3(add3(ft, scale3(vt, etaT)), -1.0 / st);
+    }
+    S->hXi[idx] = x;
+    fAcc = add3(fAcc, ft);
+    tAccA = add3(tAccA, cross3(rA, ft));
+    tAccB = add3(tAccB, cross3(rB, scale3(ft, -1.0)));
+    hzAccum(S, idA, fAcc, tAccA);
+    hzAccum(S, idB, neg3(fAcc), tAccB);
+  }
+}
+
+/* wall (e, mu, E*, G*) of particle material ma against wall w, src/solver_hertz.hpp:187-199 */
+static void hertzWallMaterial(const Sim* S, int ma, const Wall* w, real* e, real* mu,
+                              real* eStar, real* gStar) {
+  if (w->materialId >= 0 && S->hasPairTable) {
+    hertzPairEMu(S, ma, w->materialId, e, mu);
+  } else {
+    *e = w->restitution;
+    *mu = w->friction;
+  }
+  const int mw = (w->materialId >= 0) ? w->materialId : ma;
+  hertzStars(S, ma, mw, eStar, gStar);
+}
+
+/* src/solver_hertz.hpp:148-338, candidates in ascending (particle, wall) order */
+static void hertzWallForces(Sim* S, real dt, int hasShapes) {
+  const real frac = S->hertzContactRadiusFrac;
+  for (int ci = 0; ci < S->hNumWallCand; ++ci) {
+    const int enc = S->hWallCand[ci];
+    const int i = enc / 8, wi = enc % 8;
+    const V3 p = S->pos[i];
+    const real r = S->rad[i];
+    const Wall* w = &S->walls[wi];
+    const int slot = i * HZ_MAXWALLS + wi;
+    const int ma = S->materialId[i];
+    real e, mu, eStar, gStar;
+    hertzWallMaterial(S, ma, w, &e, &mu, &eStar, &gStar);
+    const real mStar = (S->invMass[i] > 0.0) ? 1.0 / S->invMass[i] : 0.0;
+    if (hasShapes) {
+      const Q4 qP = S->quat[i];
+      const real effScale = S->scale[i] * S->globalScale;
+      const real rc = frac * S->shapeParams[0] * effScale;
+      const real bd = hertzBetaD(e);
+      const V3 vP = S->vel[i], wP = S->angVel[i];
+      const real snTotPrev = S->hSnWall[slot];
+      const real dampC = (mStar > 0.0) ? -2.0 * HZ_SQRT56 * bd * sqrt(mStar) : 0.0;
+      real fnSum = 0.0, snSum = 0.0, dMax = 0.0;
+      V3 cSum = mk3(0, 0, 0), nSum = mk3(0, 0, 0), fAcc = mk3(0, 0, 0), tAcc = mk3(0, 0, 0);
+      for (int k = 0; k < S->numShell; ++k) {
+        const V3 pW = add3(p, rotate(qP, scale3(S->shell[k], effScale)));
+        const real sd = sampleWallSdf(pW, w, S->wallGrid);
+        if (sd >= 0.0) continue;
+        const real delta = -sd;
+        const V3 g = wallGradient(pW, w, S->wallGrid);
+        const real gl = len3(g);
+        if (gl < 1e-12) continue;
+        const V3 nhat = scale3(g, 1.0 / gl);
+        const V3 rA = sub3(pW, p);
+        const V3 vWall = add3(w->linVel, cross3(w->angVel, sub3(pW, w->center)));
+        const V3 vrelP = sub3(add3(vP, cross3(wP, rA)), vWall);
+        const real vnP = dot3(vrelP, nhat);
+        const real sq = sqrt(rc * delta);
+        const real fnSpring = (4.0 / 3.0) * eStar * sq * delta;
+        const real snI = 2.0 * eStar * sq;
+        const real snTot = (snTotPrev > snI) ? snTotPrev : snI;
+        const real etaI = dampC * snI / sqrt(snTot);
+        const V3 f = scale3(nhat, fnSpring - etaI * vnP);
+        fAcc = add3(fAcc, f);
+        tAcc = add3(tAcc, cross3(rA, f));
+        fnSum += fnSpring;
+        snSum += 2.0 * eStar * sq;
+        cSum = add3(cSum, scale3(pW, fnSpring));
+        nSum = add3(nSum, scale3(nhat, fnSpring));
+        if (delta > dMax) dMax = delta;
+      }
+      if (fnSum <= 0.0) {
+        S->hXiWall[slot] = mk3(0, 0, 0);
+        S->hSnWall[slot] = 0.0;
+        continue;
+      }
+      const V3 cpt = scale3(cSum, 1.0 / fnSum);
+      V3 nhat = nSum;
+      const real nl = len3(nhat);
+      if (nl > 1e-9) nhat = scale3(nhat, 1.0 / nl);
+      const V3 rA = sub3(cpt, p);
+      const V3 vWall = add3(w->linVel, cross3(w->angVel, sub3(cpt, w->center)));
+      const V3 vrel = sub3(add3(vP, cross3(wP, rA)), vWall);
+      S->hSnWall[slot] = snSum;
+      const V3 vt = sub3(vrel, scale3(nhat, dot3(vrel, nhat)));
+      const real st = 8.0 * gStar * sqrt(rc * dMax);
+      const real etaT = -2.0 * HZ_SQRT56 * bd * sqrt(st * mStar);
+      V3 x = S->hXiWall[slot];
+      x = add3(x, scale3(vt, dt));
+      x = sub3(x, scale3(nhat, dot3(x, nhat)));
+      V3 ft = add3(scale3(x, -st), scale3(vt, -etaT));
+      const real ftLen = len3(ft);
+      const real ftMax = mu * fnSum;
+      if (ftLen > ftMax) {
+        const real sc = (ftLen > 1e-20) ? ftMax / ftLen : 0.0;
+        ft = scale3(ft, sc);
+        if (st > 1e-20) x = scale3(add3(ft, scale3(vt, etaT)), -1.0 / st);
+      }
+      S->hXiWall[slot] = x;
+      fAcc = add3(fAcc, ft);
+      tAcc = add3(tAcc, cross3(rA, ft));
+      hzAccum(S, i, fAcc, tAcc);
+      continue;
+    }
+    const real sd = sampleWallSdf(p, w, S->wallGrid);
+    const real delta = r - sd;
+    if (delta <= 0.0) {
+      S->hXiWall[slot] = mk3(0, 0, 0);
+      continue;
+    }
+    const V3 g = wallGradient(p, w, S->wallGrid);
+    const real gl = sqrt(dot3(g, g));
+    if (gl < 1e-12) continue;
+    const V3 nhat = scale3(g, 1.0 / gl);
+    const V3 rA = scale3(nhat, -(r - 0.5 * delta));
+    const V3 cpt = add3(p, rA);
+    const V3 vWall = add3(w->linVel, cross3(w->angVel, sub3(cpt, w->center)));
+    const V3 vrel = sub3(add3(S->vel[i], cross3(S->angVel[i], rA)), vWall);
+    const V3 f = hertzForce(delta, nhat, vrel, r, mStar, eStar, gStar, hertzBetaD(e), mu, dt,
+                            &S->hXiWall[slot]);
+    hzAccum(S, i, f, cross3(rA, f));
+  }
+}
+
+/* src/solver_hertz.hpp:374-422: symplectic Euler, forces then cleared */
+static void hertzIntegrate(Sim* S, real dt, int integrateOrientation) {
+  const V3 g = S->gravity;
+  for (int i = 0; i < S->numReal; ++i) {
+    const real im = S->invMass[i];
+    if (im > 0.0) {
+      const V3 iI = S->invI[i];
+      const V3 T = S->hTorque[i];
+      V3 dw;
+      if (iI.x == iI.y && iI.y == iI.z) {
+        dw = mk3(T.x * iI.x, T.y * iI.x, T.z * iI.x);
+      } else {
+        const Q4 q = S->quat[i];
+        const V3 tl = invRotate(q, T);
+        dw = rotate(q, mk3(tl.x * iI.x, tl.y * iI.y, tl.z * iI.z));
+      }
+      const V3 F = S->hForce[i];
+      S->vel[i].x += (F.x * im + g.x) * dt;
+      S->vel[i].y += (F.y * im + g.y) * dt;
+      S->vel[i].z += (F.z * im + g.z) * dt;
+      S->angVel[i].x += dw.x * dt;
+      S->angVel[i].y += dw.y * dt;
+      S->angVel[i].z += dw.z * dt;
+      S->pos[i].x += S->vel[i].x * dt;
+      S->pos[i].y += S->vel[i].y * dt;
+      S->pos[i].z += S->vel[i].z * dt;
+      if (integrateOrientation) {
+        Q4 q = S->quat[i];
+        const V3 w = S->angVel[i];
+        const real h = 0.5 * dt;
+        Q4 dq;
+        dq.x = h * (w.x * q.w + w.y * q.z - w.z * q.y);
+        dq.y = h * (w.y * q.w + w.z * q.x - w.x * q.z);
+        dq.z = h * (w.z * q.w + w.x * q.y - w.y * q.x);
+        dq.w = h * (-w.x * q.x - w.y * q.y - w.z * q.z);
+        q.x = q.x + dq.x; q.y = q.y + dq.y; q.z = q.z + dq.z; q.w = q.w + dq.w;
+        const real qn = sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+        if (qn > 1e-12) {
+          S->quat[i].x = q.x / qn;
+          S->quat[i].y = q.y / qn;
+          S->quat[i].z = q.z / qn;
+          S->quat[i].w = q.w / qn;
+        }
+      }
+    }
+    S->hForce[i] = mk3(0, 0, 0);
+    S->hTorque[i] = mk3(0, 0, 0);
+  }
+}
+
+static void hzEnsureSlots(Sim* S, int n) {
+  if (n <= S->hSlotCap) return;
+  const int nc = n + n / 4 + 64;
+  S->hRefPos = (V3*)xrealloc(S->hRefPos, sizeof(V3) * (size_t)nc);
+  S->hForce = (V3*)xrealloc(S->hForce, sizeof(V3) * (size_t)nc);
+  S->hTorque = (V3*)xrealloc(S->hTorque, sizeof(V3) * (size_t)nc);
+  S->hXiWall = (V3*)xrealloc(S->hXiWall, sizeof(V3) * (size_t)nc * HZ_MAXWALLS);
+  S->hSnWall = (real*)xrealloc(S->hSnWall, sizeof(real) * (size_t)nc * HZ_MAXWALLS);
+  S->hWallCand = (int*)xrealloc(S->hWallCand, sizeof(int) * (size_t)nc * HZ_MAXWALLS);
+  const size_t old = (size_t)S->hSlotCap;
+  memset(S->hRefPos + old, 0, sizeof(V3) * (nc - old));
+  memset(S->hForce + old, 0, sizeof(V3) * (nc - old));
+  memset(S->hTorque + old, 0, sizeof(V3) * (nc - old));
+  memset(S->hXiWall + old * HZ_MAXWALLS, 0, sizeof(V3) * (nc - old) * HZ_MAXWALLS);
+  memset(S->hSnWall + old * HZ_MAXWALLS, 0, sizeof(real) * (nc - old) * HZ_MAXWALLS);
+  S->hSlotCap = nc;
+}
+
+/* the key of the Hertz pair cache is (hi << 32) | lo, src/sim.hpp:571-573 */
+static uint64_t hzKey(int a, int b) {
+  const uint64_t lo = (uint64_t)(unsigned)(a < b ? a : b), hi = (uint64_t)(unsigned)(a < b ? b : a);
+  return (hi << 32) | lo;
+}
+typedef struct { uint64_t key; int idx; } HzKI;
+static int cmpHzKI(const void* a, const void* b) {
+  const HzKI* x = (const HzKI*)a;
+  const HzKI* y = (const HzKI*)b;
+  return x->key < y->key ? -1 : (x->key > y->key ? 1 : 0);
+}
+/* src/sim.hpp:596-624: save the current list sorted by key for the next rebuild */
+static void hertzCommitHistory(Sim* S) {
+  const int n = S->hNumPairs;
+  if (n <= 0) {
+    S->hPrevCount = 0;
+    return;
+  }
+  HzKI* v = (HzKI*)xrealloc(NULL, sizeof(HzKI) * (size_t)n);
+  for (int i = 0; i < n; ++i) { v[i].key = S->hKeys[i]; v[i].idx = i; }
+  qsort(v, (size_t)n, sizeof(HzKI), cmpHzKI); /* keys are unique */
+  S->hPrevKeys = (uint64_t*)xrealloc(S->hPrevKeys, sizeof(uint64_t) * (size_t)n);
+  S->hPrevXi = (V3*)xrealloc(S->hPrevXi, sizeof(V3) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    S->hPrevKeys[i] = v[i].key;
+    S->hPrevXi[i] = S->hXi[v[i].idx];
+  }
+  free(v);
+  S->hPrevCount = n;
+}
+/* src/sim.hpp:545-594: Verlet list with margin = skin, history carried by key */
+static void hertzRebuildPairs(Sim* S, real skin) {
+  for (int i = 0; i < S->numReal; ++i) S->posPred[i] = S->pos[i];
+  S->numParticles = S->numReal;
+  findCollisions(S, skin);
+  const int np = S->np;
+  if (np > S->hPairCap) {
+    const int nc = np + np / 4 + 64;
+    S->hPairs = (int*)xrealloc(S->hPairs, sizeof(int) * 2 * (size_t)nc);
+    S->hKeys = (uint64_t*)xrealloc(S->hKeys, sizeof(uint64_t) * (size_t)nc);
+    S->hXi = (V3*)xrealloc(S->hXi, sizeof(V3) * (size_t)nc);
+    S->hSn = (real*)xrealloc(S->hSn, sizeof(real) * (size_t)nc);
+    S->hPairCap = nc;
+  }
+  for (int idx = 0; idx < np; ++idx) {
+    const int a = S->pairs[2 * idx], b = S->pairs[2 * idx + 1];
+    S->hPairs[2 * idx] = a;
+    S->hPairs[2 * idx + 1] = b;
+    const uint64_t k = hzKey(a, b);
+    S->hKeys[idx] = k;
+    const int l = lowerBound(S->hPrevKeys, S->hPrevCount, k);
+    const int hit = (l < S->hPrevCount && S->hPrevKeys[l] == k);
+    S->hXi[idx] = hit ? S->hPrevXi[l] : mk3(0, 0, 0);
+    S->hSn[idx] = 0.0;
+  }
+  S->hNumPairs = np;
+  for (int i = 0; i < S->numReal; ++i) S->hRefPos[i] = S->pos[i];
+}
+/* src/solver_hertz.hpp:343-370 */
+static void hertzBuildWallCandidates(Sim* S, real skin) {
+  int n = 0;
+  for (int i = 0; i < S->numReal; ++i)
+    for (int wi = 0; wi < S->numWalls; ++wi) {
+      const real sd = sampleWallSdf(S->pos[i], &S->walls[wi], S->wallGrid);
+      if (sd < S->rad[i] + skin) S->hWallCand[n++] = i * 8 + wi;
+    }
+  S->hNumWallCand = n;
+}
+static real hertzMaxDisp2(const Sim* S) {
+  real m = 0.0;
+  for (int i = 0; i < S->numReal; ++i) {
+    const V3 d = sub3(S->pos[i], S->hRefPos[i]);
+    const real dd = dot3(d, d);
+    if (dd > m) m = dd;
+  }
+  return m;
+}
+
+/* demStepHertz, src/sim.hpp:628-750 */
+int pdref_step_hertz(void* s, double dt, int nsteps, double skinFrac) {
+  Sim* S = asSim(s);
+  S->dt = dt;
+  if (S->dom.px || S->dom.py || S->dom.pz)
+    return fail("step_hertz: periodic domains not supported");
+  if (S->numPlanes > 0)
+    return fail("step_hertz: analytic planes not supported (use an SDF wall)");
+  if (S->numWalls > HZ_MAXWALLS) return fail("step_hertz: too many SDF walls");
+  const int n = S->numReal;
+  hzEnsureSlots(S, n > 0 ? n : 1);
+  fillRad(S, n);
+  real minRad = 3.4e38;
+  for (int i = 0; i < n; ++i)
+    if (S->rad[i] < minRad) minRad = S->rad[i];
+  const real skin = skinFrac * minRad;
+  const real rebuildAt = 0.25 * skin * skin;
+  int rebuild = (S->hNumPairs < 0);
+  const int hasShapes = S->numShell > 0;
+  for (int k = 0; k < nsteps; ++k) {
+    if (rebuild) {
+      hertzCommitHistory(S);
+      hertzRebuildPairs(S, skin);
+      if (S->numWalls > 0) hertzBuildWallCandidates(S, skin);
+      rebuild = 0;
+    }
+    if (hasShapes) hertzShapePairForces(S, dt);
+    else hertzPairForces(S, dt);
+    if (S->numWalls > 0 && S->hNumWallCand > 0) hertzWallForces(S, dt, hasShapes);
+    hertzIntegrate(S, dt, hasShapes);
+    if ((k & 15) == 15 || k == nsteps - 1)
+      if (hertzMaxDisp2(S) > rebuildAt) rebuild = 1;
+  }
+  for (int i = 0; i < n; ++i) {
+    S->posPred[i] = S->pos[i];
+    S->velPred[i] = S->vel[i];
+  }
+  return 0;
+}
+/* debug: the cached Hertz pair list with its shear history */
+int pdref_get_hertz_pairs(void* s, int* pairs, double* xi, int cap) {
+  Sim* S = asSim(s);
+  const int n = S->hNumPairs > 0 ? S->hNumPairs : 0;
+  for (int k = 0; k < n && k < cap; ++k) {
+    if (pairs) { pairs[2 * k] = S->hPairs[2 * k]; pairs[2 * k + 1] = S->hPairs[2 * k + 1]; }
+    if (xi) { xi[3 * k] = S->hXi[k].x; xi[3 * k + 1] = S->hXi[k].y; xi[3 * k + 2] = S->hXi[k].z; }
+  }
+  return n;
 }

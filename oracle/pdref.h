@@ -88,6 +88,17 @@ int pdref_get_inv_mass(void* s, double* out, int n);
 int pdref_get_scales(void* s, double* out, int n);
 
 int pdref_step(void* s, double dt);
+/* SDF walls and the Hertz–Mindlin engine (src/sim.hpp:1153-1287, :628-750) */
+int pdref_add_sdf_wall(void* s, const double* grid, int nx, int ny, int nz, double ox, double oy,
+                       double oz, double sx, double sy, double sz, double restitution,
+                       double friction);
+int pdref_set_wall_velocity(void* s, int wid, double lx, double ly, double lz, double ax,
+                            double ay, double az, double cx, double cy, double cz);
+int pdref_set_wall_material_id(void* s, int wid, int mat);
+int pdref_num_walls(void* s);
+int pdref_set_hertz_material(void* s, int mat, double youngs, double poisson);
+int pdref_step_hertz(void* s, double dt, int nsteps, double skin_frac);
+int pdref_get_hertz_pairs(void* s, int* pairs, double* xi, int cap);
 int pdref_compute_overlaps(void* s, double* out);
 int pdref_num_contacts(void* s);
 int pdref_num_manifolds(void* s);

@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "pd_hertz.hpp"
 #include "pd_kernels.hpp"
 #include "peclet_dem.h"
 
@@ -117,6 +118,20 @@ struct pd_sim {
   int ledgerBuckets = 0;  // real-index buckets of the sorted ledger (bucketStart has +1 entries)
   double maxOverlap = 0.0, maxApproach = 0.0;
   pd_stats stats{};
+  // SDF walls (src/sim.hpp:1242-1287) and the Hertz–Mindlin engine (src/particles.hpp:113-130)
+  std::vector<WallP> wallsHost;
+  std::vector<double> wallGridHost;
+  DBuf<double> wallGridD;
+  double hertzE[8] = {1e9, 1e9, 1e9, 1e9, 1e9, 1e9, 1e9, 1e9};
+  double hertzNu[8] = {0.25, 0.25, 0.25, 0.25, 0.25, 0.25, 0.25, 0.25};
+  double hertzFrac = 0.5;
+  int hNumPairs = -1, hPrevCount = 0, hNumWallCand = 0;
+  DBuf<int2> hPairs;
+  DBuf<int> hIStart, hJStart, hJList, hJCnt, hWStart, hWCand, hWCnt;
+  DBuf<double4> hXi, hF, hTA, hTB, hPrevXi, hRefPos, hXiWall, hWF, hWT;
+  DBuf<unsigned long long> hPrevKeys;
+  DBuf<HzTab> hTab;
+  HzDev Z{};
   // timing: stage events (level >= 1) and per-launch events of the sweep kernels (level >= 2)
   int timing = 0;
   double stageMs[ST_COUNT] = {};
@@ -226,6 +241,11 @@ struct pd_sim {
     counters.release(); colCnt.release(); colFill.release(); colStart.release();
     scanTmp.release(); ghostCnt.release(); ghostOff.release();
     dVel.release(); dAng.release(); dPos.release(); dCnt.release();
+    wallGridD.release(); hPairs.release(); hIStart.release(); hJStart.release();
+    hJList.release(); hJCnt.release(); hWStart.release(); hWCand.release(); hWCnt.release();
+    hXi.release(); hF.release(); hTA.release(); hTB.release(); hPrevXi.release();
+    hRefPos.release(); hXiWall.release(); hWF.release(); hWT.release(); hPrevKeys.release();
+    hTab.release();
     jpWork[0].release(); jpWork[1].release(); jpCnt.release(); redPart.release();
     blockCnt.release(); ctl.release();
   }
@@ -521,6 +541,8 @@ struct pd_sim {
     ensureSlots((int)std::max<long>(est, capacity), numReal);
     numReal = n;
     numParticles = n;
+    hNumPairs = -1;  // particle indices changed: the Hertz pair cache is stale, :1294-1295
+    hPrevCount = 0;
     std::vector<double4> P((size_t)n);
     std::vector<BodyC> B((size_t)n);
     std::vector<Body64> V((size_t)n);
@@ -542,6 +564,14 @@ struct pd_sim {
   // ---------------------------------------------------------------- the substep
   void broadAndNarrow(double margin) {
     Params P = params(margin, 0.0);
+    broadphase(P);
+    mark(ST_NARROW);
+    narrowphase(P);
+  }
+
+  // uniform-grid broadphase over posPred (.w = radius) of numParticles slots with P.margin;
+  // fills pairs / pairStart / np
+  void broadphase(const Params& P) {
     // bounds of the box centres -> uniform grid with cell edge >= 2 * max half-width
     const int nbb = std::min(nblk(numParticles), RED_BLOCKS);
     redPart.ensure(8 * RED_BLOCKS, st);
@@ -595,9 +625,10 @@ struct pd_sim {
     refreshDev();
     k_bp_compact<<<nblk(numReal), TPB, 0, st>>>(D, P, G, pairTmp.p);
     HIPCHECK(hipGetLastError());
-    mark(ST_NARROW);
+  }
 
-    // narrowphase: counts, scan, write (pairs first, then planes per real)
+  // narrowphase: counts, scan, write (pairs first, then planes per real)
+  void narrowphase(const Params& P) {
     const int nslots = np + numReal;
     ncnt.ensure(nslots + 1, st);
     noff.ensure(nslots + 1, st);
@@ -750,6 +781,8 @@ struct pd_sim {
     std::memset(&stats, 0, sizeof stats);
     if (stabMode == 2 || stabMode == 4)
       throw std::runtime_error("stabilization modes 'multilevel' and 'ordered' are not implemented");
+    if (!wallsHost.empty())
+      throw std::runtime_error("SDF walls in the XPBD substep are not implemented");
     if (!useGS)
       throw std::runtime_error("the Jacobi (velocity_use_gs=False) solve is not implemented on HIP");
     const bool frictionOn = frictionDynamic > 0.0;
@@ -1050,6 +1083,166 @@ struct pd_sim {
   }
 
   // applyThermostatKokkos, src/integration.hpp:229-271 (block partials summed in block order)
+  // ---------------------------------------------------------------- Hertz–Mindlin engine
+  void refreshHz() {
+    Z.pairs = hPairs.p; Z.iStart = hIStart.p; Z.jStart = hJStart.p; Z.jList = hJList.p;
+    Z.xi = hXi.p; Z.f = hF.p; Z.tA = hTA.p; Z.tB = hTB.p; Z.prevKeys = hPrevKeys.p;
+    Z.prevXi = hPrevXi.p; Z.refPos = hRefPos.p; Z.wStart = hWStart.p; Z.wCand = hWCand.p;
+    Z.xiWall = hXiWall.p; Z.wf = hWF.p; Z.wt = hWT.p; Z.wallGrid = wallGridD.p; Z.tab = hTab.p;
+  }
+  HzParams hzParams(double dtH) const {
+    HzParams H{};
+    H.dt = dtH;
+    H.gs = globalScale;
+    H.baseR = baseRadius;
+    H.frac = hertzFrac;
+    H.g = d3{g[0], g[1], g[2]};
+    H.numReal = numReal;
+    H.numWalls = (int)wallsHost.size();
+    H.hasShapes = shellHost.size() > 0;
+    H.numPairs = hNumPairs > 0 ? hNumPairs : 0;
+    H.shape = params(0.0, 0.0).shape;
+    for (int w = 0; w < H.numWalls && w < HZ_MAXWALLS; ++w) H.walls[w] = wallsHost[w];
+    return H;
+  }
+  // per material pair / (material, wall): beta_d, mu, E*, G* (src/solver_hertz.hpp:27-48,
+  // :187-199); computed on the host so the logarithm is the same one the oracle uses
+  void uploadHertzTable() {
+    HzTab T{};
+    for (int a = 0; a < 8; ++a)
+      for (int b = 0; b < 8; ++b) {
+        double e = restitutionNormal, mu = frictionDynamic;
+        if (hasPairTable) {
+          e = pairTable[(a * 8 + b) * 2];
+          mu = pairTable[(a * 8 + b) * 2 + 1];
+        }
+        double* c = T.pair[a * 8 + b];
+        c[0] = hzBetaD(e);
+        c[1] = mu;
+        hzStars(hertzE[a], hertzE[b], hertzNu[a], hertzNu[b], &c[2], &c[3]);
+      }
+    for (int a = 0; a < 8; ++a)
+      for (int w = 0; w < (int)wallsHost.size() && w < 8; ++w) {
+        const WallP& W = wallsHost[w];
+        double e = W.restitution, mu = W.friction;
+        if (W.materialId >= 0 && hasPairTable) {
+          e = pairTable[(a * 8 + W.materialId) * 2];
+          mu = pairTable[(a * 8 + W.materialId) * 2 + 1];
+        }
+        const int mw = W.materialId >= 0 ? W.materialId : a;
+        double* c = T.wall[a * 8 + w];
+        c[0] = hzBetaD(e);
+        c[1] = mu;
+        hzStars(hertzE[a], hertzE[mw], hertzNu[a], hertzNu[mw], &c[2], &c[3]);
+      }
+    hTab.ensure(1, st);
+    writeN(hTab.p, &T, 1);
+  }
+  // src/sim.hpp:596-624 then :545-594, and the wall candidates (src/solver_hertz.hpp:343-370)
+  void hertzRebuild(double skin) {
+    const int n = numReal;
+    if (hNumPairs > 0) {
+      hPrevKeys.ensure(hNumPairs, st);
+      hPrevXi.ensure(hNumPairs, st);
+      refreshHz();
+      k_hz_commit<<<nblk(hNumPairs), TPB, 0, st>>>(Z, hNumPairs);
+      hPrevCount = hNumPairs;
+    } else {
+      hPrevCount = 0;
+    }
+    numParticles = n;
+    {  // the broadphase reads posPred: current positions, radius in .w
+      HzParams H0 = hzParams(dt);
+      k_hz_prep<<<nblk(n), TPB, 0, st>>>(D, H0, red.p + 1);
+    }
+    Params P = params(skin, 0.0);
+    broadphase(P);
+    hPairs.ensure(np + 1, st);
+    hXi.ensure(np + 1, st);
+    hF.ensure(np + 1, st);
+    hTA.ensure(np + 1, st);
+    hTB.ensure(np + 1, st);
+    hJList.ensure(np + 1, st);
+    hIStart.ensure(n + 1, st);
+    hJStart.ensure(n + 1, st);
+    hJCnt.ensure(n + 1, st);
+    refreshDev();
+    refreshHz();
+    k_hz_carry<<<nblk(np), TPB, 0, st>>>(D, Z, np, hPrevCount);
+    HIPCHECK(hipMemcpyAsync(hIStart.p, pairStart.p, (n + 1) * sizeof(int),
+                            hipMemcpyDeviceToDevice, st));
+    zero(hJCnt.p, n + 1);
+    k_hz_jcount<<<nblk(np), TPB, 0, st>>>(Z, np, hJCnt.p);
+    scan(hJCnt.p, hJStart.p, n);
+    zero(hJCnt.p, n + 1);
+    k_hz_jfill<<<nblk(np), TPB, 0, st>>>(Z, np, hJCnt.p);
+    hNumPairs = np;
+    HzParams H = hzParams(dt);
+    k_hz_jsort<<<nblk(n), TPB, 0, st>>>(Z, H);
+    k_hz_ref<<<nblk(n), TPB, 0, st>>>(D, Z, H);
+    if (!wallsHost.empty()) {
+      hWStart.ensure(n + 1, st);
+      hWCnt.ensure(n + 1, st);
+      refreshHz();
+      k_hz_wallcand<false><<<nblk(n), TPB, 0, st>>>(D, Z, H, skin, hWCnt.p);
+      hNumWallCand = scan(hWCnt.p, hWStart.p, n);
+      hWCand.ensure(hNumWallCand + 1, st);
+      hWF.ensure(hNumWallCand + 1, st);
+      hWT.ensure(hNumWallCand + 1, st);
+      refreshHz();
+      k_hz_wallcand<true><<<nblk(n), TPB, 0, st>>>(D, Z, H, skin, hWCnt.p);
+    } else {
+      hNumWallCand = 0;
+    }
+    HIPCHECK(hipGetLastError());
+  }
+  // demStepHertz, src/sim.hpp:628-750
+  void stepHertz(double dtH, int nsteps, double skinFrac) {
+    dt = dtH;
+    if (per[0] || per[1] || per[2])
+      throw std::runtime_error("step_hertz: periodic domains not supported");
+    if (numPlanes > 0)
+      throw std::runtime_error("step_hertz: analytic planes not supported (use an SDF wall)");
+    if ((int)wallsHost.size() > HZ_MAXWALLS)
+      throw std::runtime_error("step_hertz: too many SDF walls");
+    const int n = numReal;
+    hXiWall.ensure((size_t)(n > 0 ? n : 1) * HZ_MAXWALLS, st, hXiWall.cap);
+    hRefPos.ensure(n + 1, st);
+    uploadHertzTable();
+    refreshDev();
+    refreshHz();
+    HzParams H = hzParams(dtH);
+    HIPCHECK(hipMemsetAsync(red.p, 0xff, sizeof(unsigned long long), st));
+    k_hz_prep<<<nblk(n), TPB, 0, st>>>(D, H, red.p);
+    const unsigned long long mk = read1(red.p);
+    const double minRad = (mk == ~0ull) ? 3.4e38 : ordVal(mk);
+    const double skin = skinFrac * minRad;
+    const double rebuildAt = 0.25 * skin * skin;
+    bool rebuild = hNumPairs < 0;
+    for (int k = 0; k < nsteps; ++k) {
+      if (rebuild) {
+        hertzRebuild(skin);
+        rebuild = false;
+      }
+      H = hzParams(dtH);
+      if (H.numPairs > 0) {
+        if (H.hasShapes) k_hz_shape_pairs<<<nblk(H.numPairs), TPB, 0, st>>>(D, Z, H);
+        else k_hz_pairs<<<nblk(H.numPairs), TPB, 0, st>>>(D, Z, H);
+      }
+      if (H.numWalls > 0 && hNumWallCand > 0)
+        k_hz_walls<<<nblk(hNumWallCand), TPB, 0, st>>>(D, Z, H, hNumWallCand);
+      if (H.numWalls == 0 || hNumWallCand == 0) H.numWalls = 0;
+      k_hz_integrate<<<nblk(n), TPB, 0, st>>>(D, Z, H);
+      if ((k & 15) == 15 || k == nsteps - 1) {
+        zero(bits.p + 2, 1);
+        k_hz_disp<<<std::min(nblk(n), RED_BLOCKS), TPB, 0, st>>>(D, Z, H, bits.p + 2);
+        if (bitsd(read1(bits.p + 2)) > rebuildAt) rebuild = true;
+      }
+    }
+    HIPCHECK(hipGetLastError());
+    sync();
+  }
+
   void applyThermostat() {
     const int nb = nblk(numReal);
     thermo.ensure(2 * nb + 2, st);
@@ -1426,6 +1619,79 @@ int pd_step(pd_sim* s, double dt) {
     s->dt = dt;
     s->demStep();
   });
+}
+int pd_set_hertz_material(pd_sim* s, int mat, double youngs, double poisson) {
+  return guard([&] {
+    if (mat < 0 || mat >= 8) throw ArgError("material id out of range");
+    s->hertzE[mat] = youngs;
+    s->hertzNu[mat] = poisson;
+  });
+}
+int pd_step_hertz(pd_sim* s, double dt, int nsteps, double skin_frac) {
+  return guard([&] { s->stepHertz(dt, nsteps, skin_frac); });
+}
+int pd_add_sdf_wall(pd_sim* s, const double* grid, int nx, int ny, int nz, double ox, double oy,
+                    double oz, double sx, double sy, double sz, double restitution,
+                    double friction) {
+  int idx = -1;
+  const int rc = guard([&] {
+    if (nx < 2 || ny < 2 || nz < 2)
+      throw std::runtime_error("addSdfWall: grid dims must be >= 2 on each axis");
+    if (s->wallsHost.size() >= 8) throw std::runtime_error("addSdfWall: at most 8 walls");
+    const long cells = (long)nx * ny * nz;
+    WallP w{};
+    w.nx = nx;
+    w.ny = ny;
+    w.nz = nz;
+    w.materialId = -1;
+    w.off = (long)s->wallGridHost.size();
+    w.origin = d3{ox, oy, oz};
+    w.invSpacing = d3{sx > 0 ? 1.0 / sx : 0.0, sy > 0 ? 1.0 / sy : 0.0, sz > 0 ? 1.0 / sz : 0.0};
+    w.restitution = restitution;
+    w.friction = friction;
+    s->wallGridHost.insert(s->wallGridHost.end(), grid, grid + cells);
+    s->wallGridD.ensure(s->wallGridHost.size(), s->st);
+    s->writeN(s->wallGridD.p, s->wallGridHost.data(), s->wallGridHost.size());
+    idx = (int)s->wallsHost.size();
+    s->wallsHost.push_back(w);
+  });
+  return rc == 0 ? idx : rc;
+}
+int pd_set_wall_velocity(pd_sim* s, int wid, double lx, double ly, double lz, double ax,
+                         double ay, double az, double cx, double cy, double cz) {
+  return guard([&] {
+    if (wid < 0 || wid >= (int)s->wallsHost.size())
+      throw std::runtime_error("setWallVelocity: wall index out of range");
+    WallP& w = s->wallsHost[wid];
+    w.linVel = d3{lx, ly, lz};
+    w.angVel = d3{ax, ay, az};
+    w.center = d3{cx, cy, cz};
+  });
+}
+int pd_set_wall_material_id(pd_sim* s, int wid, int mat) {
+  return guard([&] {
+    if (wid < 0 || wid >= (int)s->wallsHost.size()) throw ArgError("wall index out of range");
+    s->wallsHost[wid].materialId = mat;
+  });
+}
+int pd_num_walls(pd_sim* s) { return (int)s->wallsHost.size(); }
+int pd_get_hertz_pairs(pd_sim* s, int* pairs, double* xi, int cap) {
+  const int n = s->hNumPairs > 0 ? s->hNumPairs : 0;
+  const int m = std::min(n, cap);
+  if (m <= 0) return n;
+  guard([&] {
+    if (pairs) s->readN((int2*)pairs, s->hPairs.p, m);
+    if (xi) {
+      std::vector<double4> h(m);
+      s->readN(h.data(), s->hXi.p, m);
+      for (int k = 0; k < m; ++k) {
+        xi[3 * k] = h[k].x;
+        xi[3 * k + 1] = h[k].y;
+        xi[3 * k + 2] = h[k].z;
+      }
+    }
+  });
+  return n;
 }
 int pd_step_n(pd_sim* s, double dt, int n) {
   return guard([&] {

@@ -16,9 +16,11 @@ import os
 import numpy as np
 
 from ._binding import Binding, Stats
+from .walls import WallSDF, build_wall_sdf
 
 __version__ = "0.3.2"
-__all__ = ["Simulation", "execution_space", "finalize", "export_lammps", "library_path"]
+__all__ = ["Simulation", "execution_space", "finalize", "export_lammps", "library_path",
+           "build_wall_sdf", "WallSDF"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "libpecletdem_hip.so"
@@ -209,10 +211,13 @@ class Simulation:
                                             float(friction)))
 
     def set_hertz_material(self, mat, youngs, poisson):
-        raise NotImplementedError("the Hertz-Mindlin engine is not implemented in this build")
+        """Young's modulus and Poisson ratio of material ``mat`` (defaults 1e9, 0.25)."""
+        self._chk(self._b.set_hertz_material(self._h, int(mat), float(youngs), float(poisson)))
 
     def step_hertz(self, dt, substeps=1, skin_frac=0.3):
-        raise NotImplementedError("the Hertz-Mindlin engine is not implemented in this build")
+        """``substeps`` explicit Hertz–Mindlin steps (spheres or point-shell shapes, SDF walls,
+        non-periodic, no planes) on a Verlet pair list with skin ``skin_frac`` x min radius."""
+        self._chk(self._b.step_hertz(self._h, float(dt), int(substeps), float(skin_frac)))
 
     def add_plane(self, *args, **kw):
         """``add_plane(px, py, pz, nx, ny, nz)`` or ``add_plane(point, normal)``."""
@@ -225,14 +230,25 @@ class Simulation:
         self._chk(self._b.add_plane(self._h, *vals))
 
     def add_sdf_wall(self, grid, nx, ny, nz, origin, spacing, restitution=0.0, friction=0.0):
-        raise NotImplementedError("SDF walls are not implemented in this build")
+        """Static world-space SDF wall: flat ``nx*ny*nz`` samples, x fastest, positive in the
+        void; returns the wall index."""
+        g = np.ascontiguousarray(np.asarray(grid, dtype=np.float64).ravel())
+        if g.size != int(nx) * int(ny) * int(nz):
+            raise RuntimeError("addSdfWall: grid.size() must equal nx*ny*nz")
+        rc = self._b.add_sdf_wall(self._h, _ptr(g), int(nx), int(ny), int(nz),
+                                  *map(float, origin), *map(float, spacing), float(restitution),
+                                  float(friction))
+        if rc < 0:
+            self._chk(rc)
+        return rc
 
     def set_wall_velocity(self, wall_index, lin_vel=(0.0, 0.0, 0.0), ang_vel=(0.0, 0.0, 0.0),
                           center=(0.0, 0.0, 0.0)):
-        raise NotImplementedError("SDF walls are not implemented in this build")
+        self._chk(self._b.set_wall_velocity(self._h, int(wall_index), *map(float, lin_vel),
+                                            *map(float, ang_vel), *map(float, center)))
 
     def set_wall_material_id(self, wid, mat):
-        raise NotImplementedError("SDF walls are not implemented in this build")
+        self._chk(self._b.set_wall_material_id(self._h, int(wid), int(mat)))
 
     # ------------------------------------------------------------------ particle arrays
     def set_positions(self, a):
@@ -397,6 +413,13 @@ class Simulation:
         self._b.get_ledger(self._h, keys.ctypes.data_as(C.POINTER(C.c_uint64)), _ptr(lam),
                            _ptr(lamT), _ptr(pimp), n)
         return keys, lam, lamT, pimp
+
+    def debug_hertz_pairs(self):
+        n = self._b.get_hertz_pairs(self._h, None, None, 0)
+        pairs = np.empty((n, 2), np.int32)
+        xi = np.empty((n, 3), np.float64)
+        self._b.get_hertz_pairs(self._h, pairs.ctypes.data_as(C.POINTER(C.c_int)), _ptr(xi), n)
+        return pairs, xi
 
     def debug_grounded(self):
         n = self._b.num_particles(self._h)

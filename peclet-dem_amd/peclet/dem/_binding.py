@@ -102,6 +102,13 @@ SIGNATURES = {
     "get_manifold_colors": (_i, [_p, _ip, _i]),
     "get_ledger": (_i, [_p, _u64p, _dp, _dp, _dp, _i]),
     "get_grounded": (_i, [_p, _ip, _i]),
+    "add_sdf_wall": (_i, [_p, _dp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d]),
+    "set_wall_velocity": (_i, [_p, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d]),
+    "set_wall_material_id": (_i, [_p, _i, _i]),
+    "num_walls": (_i, [_p]),
+    "set_hertz_material": (_i, [_p, _i, _d, _d]),
+    "step_hertz": (_i, [_p, _d, _i, _d]),
+    "get_hertz_pairs": (_i, [_p, _ip, _dp, _i]),
 }
 
 # Product-only entry points (no counterpart in a restatement library).
