@@ -149,6 +149,9 @@ struct Dev {
   double4* dAng;
   double4* dPos;
   int* dCnt;
+  // legacy count-averaged friction (per contact normal impulse, per body contact count)
+  double* cLamN;
+  int* frCnt;
 };
 
 // velocity-row flag bits
@@ -1796,10 +1799,11 @@ __global__ void k_commit_pos_impulse(Dev D, int numManifolds, int numReal, doubl
 // averaged apply is parallel (one body per thread).
 // ============================================================================================
 // solveVelocityKokkos restricted to uncoloured manifolds, src/solver_velocity.hpp:38-224
-__global__ void k_vel_jacobi_accum(Dev D, Params P, int numManifolds, double thr) {
+__global__ void k_vel_jacobi_accum(Dev D, Params P, int numManifolds, double thr,
+                                   int onlyUncoloured) {
   if (gtid() != 0) return;
   for (int m = 0; m < numManifolds; ++m) {
-    if (D.mColor[m] != -1 || D.mNp[m] <= 0) continue;
+    if ((onlyUncoloured && D.mColor[m] != -1) || D.mNp[m] <= 0) continue;
     const int2 r = D.mReal[m];
     const int realA = r.x, realB = r.y;
     if (realB >= 0 && realA > realB) continue;
@@ -1899,10 +1903,10 @@ __global__ void k_apply_vel_avg(Dev D, int n) {
   D.dCnt[i] = 0;
 }
 // solvePositionKokkos restricted to uncoloured contacts, src/solver_position.hpp:34-132
-__global__ void k_pos_jacobi_accum(Dev D, int numContacts) {
+__global__ void k_pos_jacobi_accum(Dev D, int numContacts, int onlyUncoloured) {
   if (gtid() != 0) return;
   for (int c = 0; c < numContacts; ++c) {
-    if (D.cColor[c] != -1) continue;
+    if (onlyUncoloured && D.cColor[c] != -1) continue;
     const int2 ab = D.cAB[c];
     const int idA = ab.x, idB = ab.y;
     const BodyC ca = D.bc[idA];
@@ -1974,6 +1978,158 @@ __global__ void k_apply_pos_avg(Dev D, int n) {
   D.dPos[i] = make_double4(0, 0, 0, 0);
   D.dVel[i] = make_double4(0, 0, 0, 0);
   D.dCnt[i] = 0;
+}
+
+// ============================================================================================
+// legacy count-averaged Coulomb friction (zero gravity, or the Jacobi velocity path):
+// src/sim.hpp:173-176,224-226,423-429 and src/solver_friction.hpp:32-220
+// ============================================================================================
+// plane contacts: initial normal impulse from the approach speed (src/sim.hpp:173-176)
+__global__ void k_fr_plane_load(Dev D, int nc) {
+  const int c = gtid();
+  if (c >= nc) return;
+  const int2 ab = D.cAB[c];
+  double lam = 0.0;
+  if (ab.y < 0) {
+    const int idA = ab.x;
+    const BodyC ca = D.bc[idA];
+    const double invMA = ca.iim.w;
+    if (invMA > 0.0) {
+      const Body64 b = D.vw[idA];
+      const d3 n = xyz(D.cN[c]), rA = xyz(D.cRA[c]);
+      const d3 vAc = add3(xyz(b.v), cross3(xyz(b.w), rA));
+      const double approach = -dot3(sub3(vAc, d3{0.0, 0.0, 0.0}), n);
+      if (approach > 0.0) {
+        const double w_n = computeW(rA, n, invMA, xyz(ca.iim));
+        lam = (w_n > 1e-6) ? approach / w_n : 0.0;
+      }
+    }
+  }
+  D.cLamN[c] = lam;
+}
+// src/solver_friction.hpp:32-85, once per velocity iteration (each contact its own field)
+__global__ void k_fr_accum_normal(Dev D, int nc, double growthRate) {
+  const int c = gtid();
+  if (c >= nc) return;
+  const int2 ab = D.cAB[c];
+  const d3 n = xyz(D.cN[c]), rA = xyz(D.cRA[c]);
+  if (ab.y < 0) {
+    const int idA = ab.x;
+    const BodyC ca = D.bc[idA];
+    const double invMA = ca.iim.w;
+    if (invMA <= 0.0) return;
+    const Body64 b = D.vw[idA];
+    const d3 vAc = add3(xyz(b.v), cross3(xyz(b.w), rA));
+    const double approach = -dot3(sub3(vAc, d3{0.0, 0.0, 0.0}), n);
+    if (approach <= 0.0) return;
+    const double w_n = computeW(rA, n, invMA, xyz(ca.iim));
+    if (w_n > 1e-6) D.cLamN[c] += approach / w_n;
+    return;
+  }
+  const int realA = D.realIdx[ab.x], realB = D.realIdx[ab.y];
+  if (realA > realB) return;
+  const d3 rB = xyz(D.cRB[c]);
+  const BodyC ca = D.bc[realA], cb = D.bc[realB];
+  const Body64 ba = D.vw[realA], bb = D.vw[realB];
+  const d3 vAc = add3(xyz(ba.v), cross3(xyz(ba.w), rA));
+  const d3 vBc = add3(xyz(bb.v), cross3(xyz(bb.w), rB));
+  const d3 vrel = add3(sub3(vAc, vBc), scale3(sub3(rA, rB), growthRate));
+  const double approach = -dot3(vrel, n);
+  if (approach <= 0.0) return;
+  const double w_n = computeW(rA, n, ca.iim.w, xyz(ca.iim)) + computeW(rB, n, cb.iim.w, xyz(cb.iim));
+  if (w_n > 1e-6) D.cLamN[c] += approach / w_n;
+}
+// src/solver_friction.hpp:123-140 (integer counts: atomics are exact)
+__global__ void k_fr_count(Dev D, int nc) {
+  const int c = gtid();
+  if (c >= nc) return;
+  if (!(D.cLamN[c] > 0.0)) return;
+  const int2 ab = D.cAB[c];
+  atomicAdd(&D.frCnt[D.realIdx[ab.x]], 1);
+  if (ab.y >= 0) atomicAdd(&D.frCnt[D.realIdx[ab.y]], 1);
+}
+// src/solver_friction.hpp:142-220: impulses accumulated in contact order (one lane), then
+// applied unaveraged by k_fr_apply (src/integration.hpp:97-109)
+__global__ void k_fr_solve_serial(Dev D, int nc, double frictionDynamic) {
+  if (gtid() != 0) return;
+  for (int c = 0; c < nc; ++c) {
+    const double lambda_n = D.cLamN[c];
+    if (lambda_n <= 0.0) continue;
+    const int2 ab = D.cAB[c];
+    const int idA = ab.x, idB = ab.y;
+    const int realA = D.realIdx[idA];
+    const int realB = (idB >= 0) ? D.realIdx[idB] : -1;
+    const BodyC ca = D.bc[realA];
+    const double invMA = ca.iim.w;
+    const d3 invIA = xyz(ca.iim);
+    double invMB = 0.0;
+    d3 invIB{0.0, 0.0, 0.0};
+    const d3 rA = xyz(D.cRA[c]), n = xyz(D.cN[c]);
+    const double4 rb4 = D.cRB[c];
+    const d3 rB = xyz(rb4);
+    const Body64 ba = D.vw[realA];
+    const d3 vAc = add3(xyz(ba.v), cross3(xyz(ba.w), rA));
+    d3 vBc{0.0, 0.0, 0.0};
+    if (idB >= 0) {
+      const BodyC cb = D.bc[realB];
+      invMB = cb.iim.w;
+      invIB = xyz(cb.iim);
+      const Body64 bb = D.vw[realB];
+      vBc = add3(xyz(bb.v), cross3(xyz(bb.w), rB));
+    }
+    const d3 vrel = sub3(vAc, vBc);
+    const double vn = dot3(vrel, n);
+    const d3 vt = sub3(vrel, scale3(n, vn));
+    const double vt_len = sqrt(dot3(vt, vt));
+    if (vt_len < 1e-7) continue;
+    const d3 t = scale3(vt, 1.0 / vt_len);
+    const d3 rnA = cross3(rA, t), rnB = cross3(rB, t);
+    const double w_t = invMA + invMB + rnA.x * rnA.x * invIA.x + rnA.y * rnA.y * invIA.y +
+                       rnA.z * rnA.z * invIA.z + rnB.x * rnB.x * invIB.x +
+                       rnB.y * rnB.y * invIB.y + rnB.z * rnB.z * invIB.z;
+    if (w_t < 1e-6) continue;
+    const double nA = (double)D.frCnt[realA];
+    const double nB = (idB >= 0) ? (double)D.frCnt[realB] : 0.0;
+    const double inv_n = 1.0 / fmax(fmax(nA, nB), 1.0);
+    double lt = -vt_len / w_t;
+    const double mu = (rb4.w >= 0.0) ? rb4.w : frictionDynamic;
+    const double maxf = mu * lambda_n;
+    if (lt < -maxf) lt = -maxf;
+    lt *= inv_n;
+    double4& dva = D.dVel[realA];
+    dva.x += t.x * lt * invMA;
+    dva.y += t.y * lt * invMA;
+    dva.z += t.z * lt * invMA;
+    double4& dwa = D.dAng[realA];
+    dwa.x += rnA.x * invIA.x * lt;
+    dwa.y += rnA.y * invIA.y * lt;
+    dwa.z += rnA.z * invIA.z * lt;
+    if (idB >= 0) {
+      double4& dvb = D.dVel[realB];
+      dvb.x += -t.x * lt * invMB;
+      dvb.y += -t.y * lt * invMB;
+      dvb.z += -t.z * lt * invMB;
+      double4& dwb = D.dAng[realB];
+      dwb.x += -rnB.x * invIB.x * lt;
+      dwb.y += -rnB.y * invIB.y * lt;
+      dwb.z += -rnB.z * invIB.z * lt;
+    }
+  }
+}
+__global__ void k_fr_apply(Dev D, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  Body64 b = D.vw[i];
+  const double4 dv = D.dVel[i], dw = D.dAng[i];
+  b.v.x = b.v.x + dv.x;
+  b.v.y = b.v.y + dv.y;
+  b.v.z = b.v.z + dv.z;
+  b.w.x = b.w.x + dw.x;
+  b.w.y = b.w.y + dw.y;
+  b.w.z = b.w.z + dw.z;
+  D.vw[i] = b;
+  D.dVel[i] = make_double4(0, 0, 0, 0);
+  D.dAng[i] = make_double4(0, 0, 0, 0);
 }
 
 }  // namespace pd

@@ -181,6 +181,8 @@ struct pd_sim {
   DBuf<double> scale, targetScale, pairTableD, thermo;
   DBuf<double4> dVel, dAng, dPos;
   DBuf<int> dCnt;
+  DBuf<double> cLamN;
+  DBuf<int> frCnt;
   DBuf<int> realIdx, ground, ground2, cellOf, cellCnt, cellStart, order, pairCnt, pairStart;
   DBuf<unsigned char> matId, mCode, vFlag;
   DBuf<int2> pairs, pairTmp, cAB, mAB, mReal, mRange, vAB, pAB;
@@ -240,7 +242,8 @@ struct pd_sim {
     red.release(); bits.release(); lIdx.release(); bucketCnt.release(); bucketStart.release();
     counters.release(); colCnt.release(); colFill.release(); colStart.release();
     scanTmp.release(); ghostCnt.release(); ghostOff.release();
-    dVel.release(); dAng.release(); dPos.release(); dCnt.release();
+    dVel.release(); dAng.release(); dPos.release(); dCnt.release(); cLamN.release();
+    frCnt.release();
     wallGridD.release(); hPairs.release(); hIStart.release(); hJStart.release();
     hJList.release(); hJCnt.release(); hWStart.release(); hWCand.release(); hWCnt.release();
     hXi.release(); hF.release(); hTA.release(); hTB.release(); hPrevXi.release();
@@ -334,6 +337,7 @@ struct pd_sim {
     d.counters = counters.p; d.colCnt = colCnt.p; d.colFill = colFill.p; d.colStart = colStart.p;
     d.maxOverlapBits = bits.p; d.maxApproachBits = bits.p + 1; d.thermo = thermo.p;
     d.dVel = dVel.p; d.dAng = dAng.p; d.dPos = dPos.p; d.dCnt = dCnt.p;
+    d.cLamN = cLamN.p; d.frCnt = frCnt.p;
   }
   // zeroed Jacobi accumulators for the colour-saturation fallback
   void prepareFallback() {
@@ -783,14 +787,10 @@ struct pd_sim {
       throw std::runtime_error("stabilization modes 'multilevel' and 'ordered' are not implemented");
     if (!wallsHost.empty())
       throw std::runtime_error("SDF walls in the XPBD substep are not implemented");
-    if (!useGS)
-      throw std::runtime_error("the Jacobi (velocity_use_gs=False) solve is not implemented on HIP");
     const bool frictionOn = frictionDynamic > 0.0;
     const bool usePersist = (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0);
-    if (frictionOn && !usePersist)
-      throw std::runtime_error(
-          "the legacy count-averaged friction path (gravity 0 with friction) is not implemented "
-          "on HIP");
+    // count-averaged Coulomb friction when the PGS cone is off (src/sim.hpp:173-176)
+    const bool legacyFriction = frictionOn && !(usePersist && useGS);
     mark(ST_PREDICT);
     refreshDev();
     // growth ramp, src/sim.hpp:100-108
@@ -831,6 +831,12 @@ struct pd_sim {
     broadAndNarrow(margin);
     mark(ST_MANIFOLD);
     buildManifolds();
+    if (legacyFriction) {
+      cLamN.ensure(nc + 1, st);
+      frCnt.ensure(numParticles + 1, st);
+      refreshDev();
+      if (nc > 0) k_fr_plane_load<<<nblk(nc), TPB, 0, st>>>(D, nc);
+    }
     mark(ST_COLOR_V);
 
     // manifold colouring, src/solver_velocity.hpp:277-374
@@ -843,7 +849,7 @@ struct pd_sim {
     std::vector<int> vs;
     int numColors = 0;
     int velLeftover = 0;
-    if (nm > 0 && numReal > 0) {
+    if (useGS && nm > 0 && numReal > 0) {
       const int entering = read1(counters.p + 1);
       velLeftover = jpColor(mReal.p, mColor.p, nm, numReal, entering, stats.rounds_vel,
                             stats.uncolored_vel);
@@ -883,7 +889,7 @@ struct pd_sim {
     mark(ST_VSWEEP);
     const double vRest = 2.0 * dt * std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
     const double vRestS = 2.0 * dt * gMagP;
-    const bool hostLoop = velLeftover > 0 || (usePGS && stabMode == 3);
+    const bool hostLoop = velLeftover > 0 || (usePGS && stabMode == 3) || !useGS || legacyFriction;
     if (!hostLoop) {
       // device-controlled adaptive stops: no host round trip per sweep
       zero(ctl.p, CTL_COUNT);
@@ -912,13 +918,23 @@ struct pd_sim {
       HIPCHECK(hipGetLastError());
     } else {
       zero(ctl.p, CTL_COUNT);
+      Params PF = params(margin, maxRad);
       for (int it = 0; it < velIters; ++it) {
+        if (legacyFriction && nc > 0)
+          k_fr_accum_normal<<<nblk(nc), TPB, 0, st>>>(D, nc, growthRate);
+        if (!useGS) {
+          // count-averaged Jacobi over every manifold (src/solver_velocity.hpp:38-255)
+          prepareFallback();
+          k_vel_jacobi_accum<<<1, 1, 0, st>>>(D, PF, nm, vRest, 0);
+          k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+          stats.sweeps_vel++;
+          continue;
+        }
         zero(bits.p + 1, 1);
         velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest, ctl.p + CTL_NONE);
         if (velLeftover > 0) {
           prepareFallback();
-          Params PF = params(margin, maxRad);
-          k_vel_jacobi_accum<<<1, 1, 0, st>>>(D, PF, nm, vRest);
+          k_vel_jacobi_accum<<<1, 1, 0, st>>>(D, PF, nm, vRest, 1);
           k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
         }
         stats.sweeps_vel++;
@@ -969,6 +985,15 @@ struct pd_sim {
     } else if (usePGS) {
       prevCount = 0;
     }
+    if (legacyFriction && nc > 0) {
+      // src/solver_friction.hpp:123-220 then applyVelocityDeltas (src/integration.hpp:97-109)
+      zero(frCnt.p, numParticles);
+      k_fr_count<<<nblk(nc), TPB, 0, st>>>(D, nc);
+      prepareFallback();
+      k_fr_solve_serial<<<1, 1, 0, st>>>(D, nc, frictionDynamic);
+      k_fr_apply<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+      HIPCHECK(hipGetLastError());
+    }
     mark(ST_REINT);
     P = params(margin, maxRad);
     kernBegin();
@@ -985,7 +1010,7 @@ struct pd_sim {
     std::vector<int> ps;
     int numPosColors = 0;
     int posLeftover = 0;
-    if (nc > 0 && numParticles > 0) {
+    if (useGS && nc > 0 && numParticles > 0) {
       k_jp_init_contacts<<<nblk(nc), TPB, 0, st>>>(cColor.p, nc, jpWork[0].p);
       posLeftover = jpColor(cAB.p, cColor.p, nc, numParticles, nc, stats.rounds_pos,
                             stats.uncolored_pos);
@@ -1003,7 +1028,16 @@ struct pd_sim {
     if (nRowsP > 0) k_prows<<<nblk(nRowsP), TPB, 0, st>>>(D, nRowsP);
     mark(ST_PSWEEP);
     const double posTol = 1e-4 * baseRadius * globalScale;
-    if (posLeftover == 0) {
+    if (!useGS) {
+      // Jacobi XPBD over every contact, no adaptive stop (src/solver_position.hpp:34-134)
+      for (int it = 0; it < posIters; ++it) {
+        prepareFallback();
+        k_pos_jacobi_accum<<<1, 1, 0, st>>>(D, nc, 0);
+        k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+        stats.sweeps_pos++;
+      }
+      HIPCHECK(hipGetLastError());
+    } else if (posLeftover == 0) {
       sweepLoop(posIters, ctl.p + CTL_STOP_P, [&] {
         k_sweep_begin<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P);
         for (int c = 0; c < numPosColors; ++c) {
@@ -1033,7 +1067,7 @@ struct pd_sim {
             k_psweep<<<nblk(e - b), TPB, 0, st>>>(D, b, e, ctl.p + CTL_NONE);
         }
         prepareFallback();
-        k_pos_jacobi_accum<<<1, 1, 0, st>>>(D, nc);
+        k_pos_jacobi_accum<<<1, 1, 0, st>>>(D, nc, 1);
         k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
         HIPCHECK(hipGetLastError());
         stats.sweeps_pos++;
@@ -1051,7 +1085,7 @@ struct pd_sim {
         stats.sweeps_stab = c[CTL_SWEEPS_S];
         stats.stab_ran = usePGS ? c[CTL_STAB_RUN] : 0;
       }
-      if (posLeftover == 0) stats.sweeps_pos = c[CTL_SWEEPS_P];
+      if (useGS && posLeftover == 0) stats.sweeps_pos = c[CTL_SWEEPS_P];
       maxOverlap = bitsd(b2[0]);
       maxApproach = bitsd(b2[1]);
     }

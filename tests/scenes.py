@@ -64,6 +64,8 @@ def apply_scene(sim, scene):
     sim.set_solver_iterations(*s["iters"])
     if "stab" in s:
         sim.set_stabilization_mode(s["stab"])
+    if "use_gs" in s:
+        sim.set_velocity_use_gs(s["use_gs"])
     if "pair_materials" in s:
         sim.set_material_ids(s["mat_ids"])
         for a, b, e, mu in s["pair_materials"]:
