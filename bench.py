@@ -139,6 +139,38 @@ def copy_probe(mib):
     print(f"# copy probe: 3 copies of {mib} MiB", file=sys.stderr, flush=True)
 
 
+def run_batched(num, seed, steps, seq_steps, workers):
+    """Config-4 style ensemble: aggregate particle-substeps/s of one batch, against the same
+    systems stepped one after another (a single worker)."""
+    from peclet.dem import Batch
+    from peclet.dem.workloads import ensemble, setup_gravity_bed
+
+    beds = ensemble(num, seed)
+    dt = 1e-5
+    batch = Batch([len(b["pos"]) for b in beds], workers=workers)
+    for sim, bed in zip(batch.systems, beds):
+        setup_gravity_bed(sim, bed, dt, bed["mu"], bed["e"])
+    total = sum(len(b["pos"]) for b in beds)
+    batch.step(dt, 2)  # warm-up: buffers reach their working size
+    batch.set_workers(1)
+    t0 = time.perf_counter()
+    batch.step(dt, seq_steps)
+    t_seq = time.perf_counter() - t0
+    batch.set_workers(workers)
+    t0 = time.perf_counter()
+    batch.step(dt, steps)
+    t_b = time.perf_counter() - t0
+    batch.close()
+    v_b = total * steps / t_b
+    v_s = total * seq_steps / t_seq
+    return {"value": v_b, "unit": "particle-substeps/s", "systems": num, "particles": total,
+            "substeps_timed": steps, "workers": workers, "ms_per_substep": 1e3 * t_b / steps,
+            "sequential_value": v_s, "sequential_substeps_timed": seq_steps,
+            "speedup_vs_sequential": v_b / v_s,
+            "workload": f"{num} independent periodic-xy sphere beds, N~U[500,20000], "
+                        "r~U[0.8,1.2] mm, mu~U[0.2,0.8], e~U[0.1,0.6], phi0=0.1, dt=1e-5 s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +183,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (scales the HBM byte counters)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=24)
+    ap.add_argument("--batch-workers", type=int, default=16)
     ap.add_argument("--workload", choices=["pour", "lattice"], default="pour",
                     help="pour: the 1M-sphere gravity pour (default); lattice: a contact-rich "
                          "bed of spheres on a slightly overlapping cubic lattice")
@@ -216,6 +252,14 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
 
+    batched = None
+    if not args.no_batched:
+        del sim
+        batched = run_batched(args.batch_systems, 4, args.batch_steps,
+                              max(2, args.batch_steps // 6), args.batch_workers)
+        print(f"# batched: {batched['value']:.3e} particle-substeps/s "
+              f"({batched['speedup_vs_sequential']:.2f}x sequential)", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cps, tc = run_cpu_baseline(bed, dt, args.cpu_substeps)
@@ -261,6 +305,7 @@ def main():
             "launches": int(launches[dom]),
         },
         "cpu_baseline": cpu,
+        "batched": batched,
         "substep_roofline": {
             "algorithmic_bytes_per_substep": total_bytes / args.steps,
             "achieved_GBs": total_bytes / t / 1e9,

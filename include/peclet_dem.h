@@ -157,6 +157,17 @@ int pd_step_hertz(pd_sim* s, double dt, int nsteps, double skin_frac);
 /* diagnostic: the cached Hertz pair list (N,2) and shear histories (N,3); returns the count */
 int pd_get_hertz_pairs(pd_sim* s, int* pairs, double* xi, int cap);
 
+/* --- batched ensembles (no reference counterpart: the reference steps one system per
+ * Simulation).  Independent systems, each a full pd_sim configured through the calls above,
+ * advanced together; every system's result equals its own pd_step sequence. */
+typedef struct pd_batch pd_batch;
+pd_batch* pd_batch_create(int n, const int* capacities);
+void pd_batch_destroy(pd_batch* b);
+int pd_batch_size(pd_batch* b);
+pd_sim* pd_batch_system(pd_batch* b, int k); /* owned by the batch */
+int pd_batch_set_workers(pd_batch* b, int workers);
+int pd_batch_step(pd_batch* b, double dt, int nsteps); /* nsteps substeps of every system */
+
 /* --- measurement ------------------------------------------------------------------------ */
 /* Timing with HIP events on the handle's stream (no reference counterpart; the reference's
  * get_profiling_info returns an empty dict, src/dem_bindings.cpp:385): level 0 off, 1 per-stage

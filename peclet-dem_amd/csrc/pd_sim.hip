@@ -7,14 +7,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
 #include <limits>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pd_hertz.hpp"
@@ -2017,3 +2020,84 @@ const char* pd_stage_name(int stage) {
 }
 
 }  // extern "C"
+
+// ============================================================================================
+// batched ensembles: independent systems, each with its own stream and buffers, advanced
+// concurrently by a pool of host workers (each worker drives one system at a time, so the
+// GPU overlaps the kernels and read-backs of many small systems)
+// ============================================================================================
+struct pd_batch {
+  std::vector<pd_sim*> sims;
+  int workers = 16;
+};
+
+pd_batch* pd_batch_create(int n, const int* capacities) {
+  if (n < 0) {
+    g_error = "batch size must be >= 0";
+    return nullptr;
+  }
+  pd_batch* b = new pd_batch();
+  for (int k = 0; k < n; ++k) {
+    pd_sim* s = pd_create(capacities ? capacities[k] : 0);
+    if (!s) {
+      for (pd_sim* q : b->sims) delete q;
+      delete b;
+      return nullptr;
+    }
+    b->sims.push_back(s);
+  }
+  return b;
+}
+void pd_batch_destroy(pd_batch* b) {
+  if (!b) return;
+  for (pd_sim* s : b->sims) delete s;
+  delete b;
+}
+int pd_batch_size(pd_batch* b) { return (int)b->sims.size(); }
+pd_sim* pd_batch_system(pd_batch* b, int k) {
+  if (k < 0 || k >= (int)b->sims.size()) {
+    g_error = "system index out of range";
+    return nullptr;
+  }
+  return b->sims[k];
+}
+int pd_batch_set_workers(pd_batch* b, int workers) {
+  if (workers < 1) {
+    g_error = "workers must be >= 1";
+    return -2;
+  }
+  b->workers = workers;
+  return 0;
+}
+int pd_batch_step(pd_batch* b, double dt, int nsteps) {
+  const int S = (int)b->sims.size();
+  std::atomic<int> next{0};
+  std::mutex mtx;
+  std::string err;
+  auto work = [&] {
+    for (;;) {
+      const int k = next.fetch_add(1);
+      if (k >= S) break;
+      pd_sim* s = b->sims[k];
+      try {
+        for (int t = 0; t < nsteps; ++t) {
+          s->dt = dt;
+          s->demStep();
+        }
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mtx);
+        if (err.empty()) err = "system " + std::to_string(k) + ": " + e.what();
+      }
+    }
+  };
+  const int nt = std::max(1, std::min(b->workers, S));
+  std::vector<std::thread> pool;
+  for (int w = 1; w < nt; ++w) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  if (!err.empty()) {
+    g_error = err;
+    return -1;
+  }
+  return 0;
+}

@@ -19,7 +19,7 @@ from ._binding import Binding, Stats
 from .walls import WallSDF, build_wall_sdf
 
 __version__ = "0.3.2"
-__all__ = ["Simulation", "execution_space", "finalize", "export_lammps", "library_path",
+__all__ = ["Simulation", "Batch", "execution_space", "finalize", "export_lammps", "library_path",
            "build_wall_sdf", "WallSDF"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -57,10 +57,64 @@ def __getattr__(name):
     raise AttributeError(name)
 
 
+class Batch:
+    """Independent systems advanced together (no reference counterpart).
+
+    ``systems[k]`` is a full :class:`Simulation` view of system ``k``; configure each through
+    the usual API, then :meth:`step` advances every system by the same substeps.  Each system
+    runs on its own HIP stream and a pool of host workers keeps many of them in flight, so a
+    batch of small systems fills the GPU where one system cannot.  Every system's result is
+    the same as stepping it alone.
+    """
+
+    _live: "set[Batch]" = set()
+
+    def __init__(self, capacities, workers: int = 16):
+        self._b = _product_binding()
+        caps = np.ascontiguousarray(np.asarray(capacities, dtype=np.int32))
+        h = self._b.batch_create(int(caps.size), caps.ctypes.data_as(C.POINTER(C.c_int)))
+        if not h:
+            raise RuntimeError(self._b.error())
+        self._h = C.c_void_p(h)
+        self.systems = [Simulation(0, _binding=self._b,
+                                   _handle=self._b.batch_system(self._h, k))
+                        for k in range(int(caps.size))]
+        self.set_workers(workers)
+        Batch._live.add(self)
+
+    def __len__(self):
+        return len(self.systems)
+
+    def set_workers(self, workers: int):
+        rc = self._b.batch_set_workers(self._h, int(workers))
+        if rc:
+            raise ValueError(self._b.error())
+
+    def step(self, dt, substeps: int = 1):
+        if self._b.batch_step(self._h, float(dt), int(substeps)) != 0:
+            raise RuntimeError(self._b.error())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for s in self.systems:
+                s._h = None
+            self._b.batch_destroy(self._h)
+            self._h = None
+        Batch._live.discard(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def finalize() -> None:
     """Release every live Simulation (reference module ``finalize``, src/dem_bindings.cpp:61-69)."""
     for sim in list(Simulation._live):
         sim.close()
+    for batch in list(Batch._live):
+        batch.close()
 
 
 def _f64(a, cols=None) -> np.ndarray:
@@ -79,13 +133,15 @@ class Simulation:
 
     _live: "set[Simulation]" = set()
 
-    def __init__(self, capacity: int, _binding: Binding | None = None):
+    def __init__(self, capacity: int, _binding: Binding | None = None, _handle=None):
         self._b = _binding if _binding is not None else _product_binding()
-        h = self._b.create(int(capacity))
+        self._owned = _handle is None
+        h = self._b.create(int(capacity)) if _handle is None else _handle
         if not h:
             raise RuntimeError(self._b.error())
         self._h = C.c_void_p(h)
-        Simulation._live.add(self)
+        if self._owned:
+            Simulation._live.add(self)
 
     # ------------------------------------------------------------------ plumbing
     def _chk(self, rc: int):
@@ -97,7 +153,8 @@ class Simulation:
 
     def close(self):
         if getattr(self, "_h", None):
-            self._b.destroy(self._h)
+            if getattr(self, "_owned", True):
+                self._b.destroy(self._h)
             self._h = None
         Simulation._live.discard(self)
 

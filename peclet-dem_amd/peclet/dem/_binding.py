@@ -125,6 +125,12 @@ PRODUCT_SIGNATURES = {
     "stage_name": (C.c_char_p, [_i]),
     "get_kernel_times": (_i, [_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long), _i]),
     "kernel_name": (C.c_char_p, [_i]),
+    "batch_create": (_p, [_i, _ip]),
+    "batch_destroy": (None, [_p]),
+    "batch_size": (_i, [_p]),
+    "batch_system": (_p, [_p, _i]),
+    "batch_set_workers": (_i, [_p, _i]),
+    "batch_step": (_i, [_p, _d, _i]),
     "rsa_place": (C.c_long, [_dp, C.c_long, _dp, _i, _i, _dp, _dp, _dp, _ip]),
 }
 

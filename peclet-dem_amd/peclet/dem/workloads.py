@@ -111,7 +111,7 @@ def rsa_positions(n, radii, lo, hi, seed, max_tries=200, order="cell", native=No
 
 
 def sphere_bed(n, r_lo, r_hi, box, seed, rho=2500.0, base_radius=1e-3, column=None,
-               order="cell"):
+               order="cell", native=None):
     """Polydisperse sphere bed in SI units.
 
     Returns dict(pos, scale, inv_mass, inv_inertia, radius, domain) for a bed of ``n`` spheres
@@ -121,7 +121,7 @@ def sphere_bed(n, r_lo, r_hi, box, seed, rho=2500.0, base_radius=1e-3, column=No
     radius = rng.uniform(r_lo, r_hi, n) if r_hi > r_lo else np.full(n, float(r_lo))
     col = box[2] if column is None else column
     pos, perm = rsa_positions(n, radius, (0.0, 0.0, 0.0), (box[0], box[1], col), seed,
-                              order=order)
+                              order=order, native=native)
     radius = radius[perm]
     mass = rho * 4.0 / 3.0 * math.pi * radius ** 3
     inv_mass = 1.0 / mass
@@ -181,3 +181,24 @@ def lattice_bed(n, box_xy, seed, r=1e-3, spacing=1.996, jitter=0.004, spread=0.0
         "domain": (L, L, float(pos[:, 2].max() + 4 * r)),
         "base_radius": base_radius,
     }
+
+
+def ensemble(num, seed, n_lo=500, n_hi=20000, r_lo=0.8e-3, r_hi=1.2e-3, mu=(0.2, 0.8),
+             e=(0.1, 0.6), phi0=0.1):
+    """Independent sphere packings for a batched run: per system (drawn from ``seed`` and the
+    system index) a particle count ~ U[n_lo, n_hi], radii ~ U[r_lo, r_hi], friction ~ U[mu],
+    restitution ~ U[e], in a cubic box of solid fraction ``phi0`` (periodic in x and y, floor at
+    z = 0).  Returns a list of dicts with the bed (see ``sphere_bed``) plus ``mu`` and ``e``."""
+    out = []
+    for k in range(num):
+        rng = np.random.default_rng([seed, k])
+        n = int(rng.integers(n_lo, n_hi + 1))
+        m_mu = float(rng.uniform(*mu))
+        m_e = float(rng.uniform(*e))
+        mean_r3 = (r_hi ** 4 - r_lo ** 4) / (4.0 * (r_hi - r_lo))
+        side = (n * 4.0 / 3.0 * math.pi * mean_r3 / phi0) ** (1.0 / 3.0)
+        bed = sphere_bed(n, r_lo, r_hi, (side, side, side), seed * 100003 + k, native=True)
+        bed["mu"] = m_mu
+        bed["e"] = m_e
+        out.append(bed)
+    return out
