@@ -2098,11 +2098,13 @@ static void fillRad(Sim* S, int n) {
   for (int i = 0; i < n; ++i) S->rad[i] = S->scale[i] * S->globalScale * S->baseRadius;
 }
 
+static void detectWallSdf(Sim* S, real margin);
+static real wallFrictionMax(const Sim* S);
+
 static int demStep(Sim* S) {
   memset(&S->stats, 0, sizeof S->stats);
   if (S->stabMode == 2 || S->stabMode == 4)
     return fail("stabilization modes 'multilevel' and 'ordered' are not implemented");
-  if (S->numWalls > 0) return fail("SDF walls in the XPBD substep are not implemented");
   /* growth ramp, src/sim.hpp:100-108 */
   if (S->growthFactor != -1.0 && S->growthRate != 0.0) {
     S->growthFactor *= exp(S->growthRate * S->dt);
@@ -2124,12 +2126,14 @@ static int demStep(Sim* S) {
   S->maxOverlap = 0.0;
   detectContacts(S, margin);
   detectBoundary(S, margin);
+  detectWallSdf(S, margin);
   ensureContacts(S, S->nc);
   reduceToManifolds(S);
   ensureManifolds(S, S->nm);
   const int nm = S->nm, nc = S->nc;
 
-  const int friction = (S->frictionDynamic > 0.0);
+  /* a frictional wall drives friction even for frictionless bodies, src/sim.hpp:169-170 */
+  const int friction = (S->frictionDynamic > 0.0 || wallFrictionMax(S) > 0.0);
   const int usePersist = (S->gravity.x != 0.0 || S->gravity.y != 0.0 || S->gravity.z != 0.0);
   const int legacyFriction = friction && !(usePersist && S->useGS);
   real* pfLoad = NULL;
@@ -2287,6 +2291,7 @@ int pdref_compute_overlaps(void* s, double* out) {
   S->maxOverlap = 0.0;
   detectContacts(S, margin);
   detectBoundary(S, margin);
+  detectWallSdf(S, margin);
   S->numParticles = S->numReal;
   if (out) *out = S->maxOverlap;
   return 0;
@@ -2976,4 +2981,66 @@ int pdref_get_hertz_pairs(void* s, int* pairs, double* xi, int cap) {
     if (xi) { xi[3 * k] = S->hXi[k].x; xi[3 * k + 1] = S->hXi[k].y; xi[3 * k + 2] = S->hXi[k].z; }
   }
   return n;
+}
+
+/* wall contacts of the XPBD substep, src/narrowphase.hpp:278-367: per real, per wall, per probe */
+static real wallFrictionMax(const Sim* S) {
+  real m = 0.0;
+  for (int w = 0; w < S->numWalls; ++w)
+    if (S->walls[w].friction > m) m = S->walls[w].friction;
+  return m;
+}
+static void detectWallSdf(Sim* S, real margin) {
+  const real eps = 1e-4;
+  for (int i = 0; i < S->numReal; ++i) {
+    const V3 posA = S->posPred[i];
+    const real s = S->scale[i] * S->globalScale;
+    real baseR = S->shapeParams[0];
+    if (baseR == 0.0) baseR = 1.0;
+    const real radius = baseR * s;
+    const int numPts = S->numShell;
+    const Q4 qA = S->quatPred[i];
+    for (int wi = 0; wi < S->numWalls; ++wi) {
+      const Wall* w = &S->walls[wi];
+      const int iter = numPts > 0 ? numPts : 1;
+      for (int k = 0; k < iter; ++k) {
+        V3 rA = mk3(0, 0, 0);
+        if (numPts > 0) rA = rotate(qA, scale3(S->shell[k], s));
+        const V3 pw = add3(posA, rA);
+        const real sdf = sampleWallSdf(pw, w, S->wallGrid);
+        V3 n = mk3(sampleWallSdf(mk3(pw.x + eps, pw.y, pw.z), w, S->wallGrid) -
+                       sampleWallSdf(mk3(pw.x - eps, pw.y, pw.z), w, S->wallGrid),
+                   sampleWallSdf(mk3(pw.x, pw.y + eps, pw.z), w, S->wallGrid) -
+                       sampleWallSdf(mk3(pw.x, pw.y - eps, pw.z), w, S->wallGrid),
+                   sampleWallSdf(mk3(pw.x, pw.y, pw.z + eps), w, S->wallGrid) -
+                       sampleWallSdf(mk3(pw.x, pw.y, pw.z - eps), w, S->wallGrid));
+        const real ln = len3(n);
+        n = (ln > 1e-9) ? scale3(n, 1.0 / ln) : mk3(0, 1, 0);
+        const real dist = (numPts > 0) ? sdf : sdf - radius;
+        if (dist >= margin) continue;
+        const V3 rAeff = (numPts > 0) ? rA : scale3(n, -radius);
+        const V3 pSurfA = add3(posA, rAeff);
+        const V3 pWall = sub3(pSurfA, scale3(n, dist));
+        if (dist < 0.0 && -dist > S->maxOverlap) S->maxOverlap = -dist;
+        Contact c;
+        memset(&c, 0, sizeof c);
+        c.bodyA = i;
+        c.bodyB = -1;
+        c.normal = n;
+        c.rA = rAeff;
+        c.rB = pWall;
+        c.dist = dist;
+        c.boundaryVel = add3(w->linVel, cross3(w->angVel, sub3(pWall, w->center)));
+        if (w->materialId >= 0 && S->hasPairTable) {
+          const int t = ((int)S->materialId[i] * KMAXMAT + w->materialId) * 2;
+          c.boundaryRestitution = S->pairTable[t];
+          c.boundaryFriction = S->pairTable[t + 1];
+        } else {
+          c.boundaryRestitution = w->restitution;
+          c.boundaryFriction = w->friction;
+        }
+        pushContact(S, &c);
+      }
+    }
+  }
 }

@@ -561,4 +561,72 @@ __global__ __launch_bounds__(TPB) void k_hz_disp(Dev D, HzDev Z, HzParams H,
   blockAtomicMaxU64(out, m, sh);
 }
 
+// ============================================================================================
+// wall contacts of the XPBD substep, src/narrowphase.hpp:278-367: per real body, per wall, per
+// surface probe (shell point, or the centre for an analytic sphere); appended after the planes
+// ============================================================================================
+struct WallSet {
+  int n;
+  WallP w[8];
+  const double* grid;
+};
+template <bool WRITE>
+__global__ __launch_bounds__(TPB) void k_np_walls(Dev D, Params P, WallSet W, int slotBase) {
+  __shared__ unsigned long long sh[WAVES];
+  const int i = gtid();
+  double ovl = 0.0;
+  if (i < P.numReal) {
+    const d3 posA = xyz(D.posPred[i]);
+    const double s = D.scale[i] * P.gs;
+    double baseR = P.shape.p0;
+    if (baseR == 0.0) baseR = 1.0;
+    const double radius = baseR * s;
+    const int numPts = P.shape.numShell;
+    const q4 qA = toq(D.quatPred[i]);
+    const double eps = 1e-4;
+    int n = 0;
+    const int base = WRITE ? D.noff[slotBase + i] : 0;
+    const int ma = (int)D.matId[i];
+    for (int wi = 0; wi < W.n; ++wi) {
+      const WallP& w = W.w[wi];
+      const int iter = numPts > 0 ? numPts : 1;
+      for (int k = 0; k < iter; ++k) {
+        d3 rA{0.0, 0.0, 0.0};
+        if (numPts > 0) rA = rotate(qA, scale3(xyz(D.shell[k]), s));
+        const d3 pw = add3(posA, rA);
+        const double sdf = sampleWallSdf(pw, w, W.grid);
+        d3 nrm{sampleWallSdf(d3{pw.x + eps, pw.y, pw.z}, w, W.grid) -
+                   sampleWallSdf(d3{pw.x - eps, pw.y, pw.z}, w, W.grid),
+               sampleWallSdf(d3{pw.x, pw.y + eps, pw.z}, w, W.grid) -
+                   sampleWallSdf(d3{pw.x, pw.y - eps, pw.z}, w, W.grid),
+               sampleWallSdf(d3{pw.x, pw.y, pw.z + eps}, w, W.grid) -
+                   sampleWallSdf(d3{pw.x, pw.y, pw.z - eps}, w, W.grid)};
+        const double ln = len3(nrm);
+        nrm = (ln > 1e-9) ? scale3(nrm, 1.0 / ln) : d3{0.0, 1.0, 0.0};
+        const double dist = (numPts > 0) ? sdf : sdf - radius;
+        if (dist >= P.margin) continue;
+        if (!WRITE) {
+          ++n;
+          continue;
+        }
+        const d3 rAeff = (numPts > 0) ? rA : scale3(nrm, -radius);
+        const d3 pSurfA = add3(posA, rAeff);
+        const d3 pWall = sub3(pSurfA, scale3(nrm, dist));
+        if (dist < 0.0 && -dist > ovl) ovl = -dist;
+        const d3 vWall = add3(w.linVel, cross3(w.angVel, sub3(pWall, w.center)));
+        double eb = w.restitution, mub = w.friction;
+        if (w.materialId >= 0 && P.hasPairTable) {
+          const int t = (ma * 8 + w.materialId) * 2;
+          eb = D.pairTable[t];
+          mub = D.pairTable[t + 1];
+        }
+        writeContact(D, base + n, i, -1, nrm, rAeff, pWall, dist, eb, mub, vWall);
+        ++n;
+      }
+    }
+    if (!WRITE) D.ncnt[slotBase + i] = n;
+  }
+  if (WRITE) blockAtomicMaxU64(D.maxOverlapBits, dbits(ovl), sh);
+}
+
 }  // namespace pd

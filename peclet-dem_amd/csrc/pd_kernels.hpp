@@ -85,6 +85,7 @@ struct Dev {
   double4* cN;
   double4* cRA;
   double4* cRB;
+  double4* cBV;   // boundary (wall) surface velocity at the contact, zero otherwise
   int* cSlot;
   // manifolds
   int* mCnt;
@@ -92,11 +93,13 @@ struct Dev {
   int2* mAB;
   int2* mReal;
   int2* mRange;
+  int2* mRange2;  // boundary manifolds: the wall-contact range (planes are mRange)
   double4* mNs;
   double4* mTA;
   double4* mTB;
   double4* mRA;
   double4* mRB;
+  double4* mWV;   // sum of the boundary velocities of the active contacts
   int* mNp;
   unsigned long long* mKey;
   unsigned char* mCode;
@@ -118,6 +121,7 @@ struct Dev {
   double4* vRB;
   double4* vVt0;
   double4* vLamT;
+  double4* vWV;   // boundary rows: wall velocity sum (the body-B velocity is vWV / points)
   double* vVn0;
   double* vLam;
   unsigned char* vFlag;
@@ -715,11 +719,13 @@ __global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, Grid G,
 // narrowphase — src/narrowphase.hpp:179-268 (pairs) and :371-449 (planes)
 // ============================================================================================
 __device__ __forceinline__ void writeContact(Dev& D, int slot, int a, int b, d3 n, d3 rA, d3 rB,
-                                             double dist, double eb, double mub) {
+                                             double dist, double eb, double mub,
+                                             d3 bv = d3{0.0, 0.0, 0.0}) {
   D.cAB[slot] = make_int2(a, b);
   D.cN[slot] = tod4(n, dist);
   D.cRA[slot] = tod4(rA, eb);
   D.cRB[slot] = tod4(rB, mub);
+  D.cBV[slot] = tod4(bv, 0.0);
 }
 
 template <bool WRITE>
@@ -840,7 +846,7 @@ __global__ void k_mcount(Dev D, Params P, int numPairs) {
   if (i >= P.numReal) return;
   int m = 0;
   for (int p = D.pairStart[i]; p < D.pairStart[i + 1]; ++p) m += D.ncnt[p] > 0;
-  m += D.ncnt[numPairs + i] > 0;
+  m += (D.ncnt[numPairs + i] + D.ncnt[numPairs + P.numReal + i]) > 0;
   D.mCnt[i] = m;
 }
 __global__ void k_mbuild(Dev D, Params P, int numPairs) {
@@ -852,13 +858,16 @@ __global__ void k_mbuild(Dev D, Params P, int numPairs) {
     if (c > 0) {
       D.mAB[idx] = D.pairs[p];
       D.mRange[idx] = make_int2(D.noff[p], c);
+      D.mRange2[idx] = make_int2(0, 0);
       ++idx;
     }
   }
-  const int cb = D.ncnt[numPairs + i];
-  if (cb > 0) {
+  // boundary manifold: plane contacts, then wall contacts (contact order)
+  const int cb = D.ncnt[numPairs + i], cw = D.ncnt[numPairs + P.numReal + i];
+  if (cb + cw > 0) {
     D.mAB[idx] = make_int2(i, -1);
     D.mRange[idx] = make_int2(D.noff[numPairs + i], cb);
+    D.mRange2[idx] = make_int2(D.noff[numPairs + P.numReal + i], cw);
   }
 }
 // transformContact + ordered segmented sum, src/contact_preprocessing.hpp:339-466
@@ -866,15 +875,17 @@ __global__ __launch_bounds__(TPB) void k_mreduce(Dev D, int numManifolds) {
   const int m = gtid();
   if (m >= numManifolds) return;
   const int2 ab = D.mAB[m];
-  const int2 rg = D.mRange[m];
-  d3 ns{0, 0, 0}, ta{0, 0, 0}, tb{0, 0, 0}, ra{0, 0, 0}, rb{0, 0, 0};
+  const int2 rg = D.mRange[m], rg2 = D.mRange2[m];
+  d3 ns{0, 0, 0}, ta{0, 0, 0}, tb{0, 0, 0}, ra{0, 0, 0}, rb{0, 0, 0}, wv{0, 0, 0};
   double es = 0.0, fs = 0.0;
   int np = 0;
-  for (int c = rg.x; c < rg.x + rg.y; ++c) {
+  const int nTot = rg.y + rg2.y;
+  for (int q = 0; q < nTot; ++q) {
+    const int c = q < rg.y ? rg.x + q : rg2.x + (q - rg.y);
     D.cSlot[c] = m;
     const double4 cn = D.cN[c];
     // an inactive contact (dist > 0) contributes a zero manifold with num_points 0
-    d3 na{0, 0, 0}, tca{0, 0, 0}, tcb{0, 0, 0}, rcan{0, 0, 0}, roth{0, 0, 0};
+    d3 na{0, 0, 0}, tca{0, 0, 0}, tcb{0, 0, 0}, rcan{0, 0, 0}, roth{0, 0, 0}, bv{0, 0, 0};
     double e = 0.0, f = 0.0;
     int pts = 0;
     if (!(cn.w > 0.0)) {
@@ -892,6 +903,7 @@ __global__ __launch_bounds__(TPB) void k_mreduce(Dev D, int numManifolds) {
       tcb = cross3(roth, neg3(na));
       e = cra.w;
       f = crb.w;
+      bv = xyz(D.cBV[c]);
       pts = 1;
     }
     ns = add3(ns, na);
@@ -899,6 +911,7 @@ __global__ __launch_bounds__(TPB) void k_mreduce(Dev D, int numManifolds) {
     tb = add3(tb, tcb);
     ra = add3(ra, rcan);
     rb = add3(rb, roth);
+    wv = add3(wv, bv);
     es += e;
     fs += f;
     np += pts;
@@ -908,6 +921,7 @@ __global__ __launch_bounds__(TPB) void k_mreduce(Dev D, int numManifolds) {
   D.mTB[m] = tod4(tb, 0.0);
   D.mRA[m] = tod4(ra, 0.0);
   D.mRB[m] = tod4(rb, 0.0);
+  D.mWV[m] = tod4(wv, 0.0);
   D.mNp[m] = np;
   const int realA = D.realIdx[ab.x];
   const int realB = ab.y >= 0 ? D.realIdx[ab.y] : -1;
@@ -1132,6 +1146,7 @@ __global__ __launch_bounds__(TPB) void k_vrows(Dev D, Params P, int numRows, int
   const double alignment = dot3(Nsum, diffCenters);
   const double sgn = (alignment > 0.0) ? 1.0 : -1.0;
   D.vAB[k] = r;
+  D.vWV[k] = D.mWV[m];
   D.vN[k] = tod4(Nsum, restitution);
   D.vTA[k] = tod4(TauA, mu);
   D.vTB[k] = tod4(TauB, invN);
@@ -1167,7 +1182,7 @@ __global__ __launch_bounds__(TPB) void k_vrows(Dev D, Params P, int numRows, int
     vB = xyz(bb.v);
     wB = xyz(bb.w);
   } else {
-    vB = scale3(d3{0.0, 0.0, 0.0}, invN);
+    vB = scale3(xyz(D.mWV[m]), invN);
   }
   const d3 vGrowth = scale3(diffCenters, P.growthRate);
   double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
@@ -1322,7 +1337,7 @@ __device__ double pgsRow(Dev& D, const Params& P, int k, double restVelThreshold
     vB = xyz(bb.v);
     wB = xyz(bb.w);
   } else {
-    vB = scale3(d3{0.0, 0.0, 0.0}, invN);
+    vB = scale3(xyz(D.vWV[k]), invN);
   }
   const d3 vA = xyz(ba.v), wA = xyz(ba.w);
   const double4 n4 = D.vN[k], ta4 = D.vTA[k], ra4 = D.vRA[k], rb4 = D.vRB[k];
@@ -1403,7 +1418,7 @@ __device__ double pgsRow(Dev& D, const Params& P, int k, double restVelThreshold
       vB2 = xyz(bb.v);
       wB2 = xyz(bb.w);
     } else {
-      vB2 = scale3(d3{0.0, 0.0, 0.0}, invN);
+      vB2 = scale3(xyz(D.vWV[k]), invN);
     }
     const d3 vrel = sub3(add3(vA2, cross3(wA2, rAavg)), add3(vB2, cross3(wB2, rBavg)));
     const d3 vt = sub3(vrel, scale3(nhat, dot3(vrel, nhat)));
@@ -1491,7 +1506,7 @@ __device__ double gsRow(Dev& D, const Params& P, int k, double restVelThreshold)
     vB = xyz(bb.v);
     wB = xyz(bb.w);
   } else {
-    vB = scale3(d3{0.0, 0.0, 0.0}, invN);
+    vB = scale3(xyz(D.vWV[k]), invN);
   }
   const d3 vA = xyz(ba.v), wA = xyz(ba.w);
   const double4 n4 = D.vN[k], ta4 = D.vTA[k], ra4 = D.vRA[k], rb4 = D.vRB[k];
@@ -1777,9 +1792,10 @@ __global__ void k_psweep_serial(Dev D, int numContacts, int colour, const int* s
 __global__ void k_commit_pos_impulse(Dev D, int numManifolds, int numReal, double dt) {
   const int m = gtid();
   if (m >= numManifolds) return;
-  const int2 rg = D.mRange[m];
+  const int2 rg = D.mRange[m], rg2 = D.mRange2[m];
   double s = 0.0;
-  for (int c = rg.x; c < rg.x + rg.y; ++c) {
+  for (int q = 0; q < rg.y + rg2.y; ++q) {
+    const int c = q < rg.y ? rg.x + q : rg2.x + (q - rg.y);
     const int row = D.pRowOf[c];  // -1: uncoloured contact, solved by the Jacobi fallback only
     const double l = row >= 0 ? D.pLam[row] : 0.0;
     if (l != 0.0) s += l / dt;
@@ -1830,7 +1846,7 @@ __global__ void k_vel_jacobi_accum(Dev D, Params P, int numManifolds, double thr
     const d3 Nsum = xyz(ns4), TauA = xyz(D.mTA[m]), TauB = xyz(D.mTB[m]);
     const double invN = 1.0 / (double)D.mNp[m];
     double restitution = P.restN;
-    if (realB < 0) vB = scale3(d3{0.0, 0.0, 0.0}, invN);
+    if (realB < 0) vB = scale3(xyz(D.mWV[m]), invN);
     {
       const double ra = ns4.w * invN;
       if (ra >= 0.0) restitution = ra;
@@ -1998,7 +2014,7 @@ __global__ void k_fr_plane_load(Dev D, int nc) {
       const Body64 b = D.vw[idA];
       const d3 n = xyz(D.cN[c]), rA = xyz(D.cRA[c]);
       const d3 vAc = add3(xyz(b.v), cross3(xyz(b.w), rA));
-      const double approach = -dot3(sub3(vAc, d3{0.0, 0.0, 0.0}), n);
+      const double approach = -dot3(sub3(vAc, xyz(D.cBV[c])), n);
       if (approach > 0.0) {
         const double w_n = computeW(rA, n, invMA, xyz(ca.iim));
         lam = (w_n > 1e-6) ? approach / w_n : 0.0;
@@ -2020,7 +2036,7 @@ __global__ void k_fr_accum_normal(Dev D, int nc, double growthRate) {
     if (invMA <= 0.0) return;
     const Body64 b = D.vw[idA];
     const d3 vAc = add3(xyz(b.v), cross3(xyz(b.w), rA));
-    const double approach = -dot3(sub3(vAc, d3{0.0, 0.0, 0.0}), n);
+    const double approach = -dot3(sub3(vAc, xyz(D.cBV[c])), n);
     if (approach <= 0.0) return;
     const double w_n = computeW(rA, n, invMA, xyz(ca.iim));
     if (w_n > 1e-6) D.cLamN[c] += approach / w_n;
@@ -2069,7 +2085,7 @@ __global__ void k_fr_solve_serial(Dev D, int nc, double frictionDynamic) {
     const d3 rB = xyz(rb4);
     const Body64 ba = D.vw[realA];
     const d3 vAc = add3(xyz(ba.v), cross3(xyz(ba.w), rA));
-    d3 vBc{0.0, 0.0, 0.0};
+    d3 vBc = xyz(D.cBV[c]);
     if (idB >= 0) {
       const BodyC cb = D.bc[realB];
       invMB = cb.iim.w;

@@ -188,9 +188,10 @@ struct pd_sim {
   DBuf<int> frCnt;
   DBuf<int> realIdx, ground, ground2, cellOf, cellCnt, cellStart, order, pairCnt, pairStart;
   DBuf<unsigned char> matId, mCode, vFlag;
-  DBuf<int2> pairs, pairTmp, cAB, mAB, mReal, mRange, vAB, pAB;
+  DBuf<int2> pairs, pairTmp, cAB, mAB, mReal, mRange, mRange2, vAB, pAB;
   DBuf<int> ncnt, noff, cSlot, mCnt, mStart, mNp, mColor, cColor, vPerm, vRowOf, pPerm, pRowOf;
-  DBuf<double4> cN, cRA, cRB, mNs, mTA, mTB, mRA, mRB, vN, vTA, vTB, vRA, vRB, vVt0, vLamT, pN,
+  DBuf<double4> cN, cRA, cRB, cBV, mNs, mTA, mTB, mRA, mRB, mWV, vN, vTA, vTB, vRA, vRB, vVt0,
+      vLamT, vWV, pN,
       pRA, pRB, lLamT;
   DBuf<double> vVn0, vLam, pLam, lLam;
   DBuf<unsigned long long> mKey, winner, mask, lKey, uKey, red, bits, redPart;
@@ -234,7 +235,8 @@ struct pd_sim {
     ground.release(); ground2.release(); cellOf.release(); cellCnt.release(); cellStart.release();
     order.release(); pairCnt.release(); pairStart.release(); pairTmp.release(); matId.release(); mCode.release();
     vFlag.release(); pairs.release(); cAB.release(); mAB.release(); mReal.release();
-    mRange.release(); vAB.release(); pAB.release(); ncnt.release(); noff.release();
+    mRange.release(); mRange2.release(); cBV.release(); mWV.release(); vWV.release();
+    vAB.release(); pAB.release(); ncnt.release(); noff.release();
     cSlot.release(); mCnt.release(); mStart.release(); mNp.release(); mColor.release();
     cColor.release(); vPerm.release(); vRowOf.release(); pPerm.release(); pRowOf.release();
     cN.release(); cRA.release(); cRB.release(); mNs.release(); mTA.release(); mTB.release();
@@ -288,9 +290,14 @@ struct pd_sim {
 
   // exclusive scan of n ints into out[0..n), total into out[n]; returns the total
   int scan(const int* in, int* out, int n) {
+    scanQuiet(in, out, n);
+    return n > 0 ? read1(out + n) : 0;
+  }
+  // the same scan without reading the total back (no host synchronisation)
+  void scanQuiet(const int* in, int* out, int n) {
     if (n <= 0) {
       zero(out, 1);
-      return 0;
+      return;
     }
     const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     scanTmp.ensure(nb + 1, st);
@@ -298,7 +305,6 @@ struct pd_sim {
     k_scan_top<<<1, TPB, 0, st>>>(scanTmp.p, nb, out + n);
     k_scan_down<<<nb, TPB, 0, st>>>(in, n, scanTmp.p, out);
     HIPCHECK(hipGetLastError());
-    return read1(out + n);
   }
 
   // ---------------------------------------------------------------- capacity
@@ -328,7 +334,8 @@ struct pd_sim {
     d.sbox = sbox.p; d.pairCnt = pairCnt.p; d.pairStart = pairStart.p; d.pairs = pairs.p;
     d.ncnt = ncnt.p; d.noff = noff.p; d.cAB = cAB.p; d.cN = cN.p; d.cRA = cRA.p; d.cRB = cRB.p;
     d.cSlot = cSlot.p; d.mCnt = mCnt.p; d.mStart = mStart.p; d.mAB = mAB.p; d.mReal = mReal.p;
-    d.mRange = mRange.p; d.mNs = mNs.p; d.mTA = mTA.p; d.mTB = mTB.p; d.mRA = mRA.p;
+    d.mRange = mRange.p; d.mRange2 = mRange2.p; d.cBV = cBV.p; d.mWV = mWV.p; d.vWV = vWV.p;
+    d.mNs = mNs.p; d.mTA = mTA.p; d.mTB = mTB.p; d.mRA = mRA.p;
     d.mRB = mRB.p; d.mNp = mNp.p; d.mKey = mKey.p; d.mCode = mCode.p; d.mColor = mColor.p;
     d.cColor = cColor.p; d.winner = winner.p; d.mask = mask.p; d.vPerm = vPerm.p;
     d.vRowOf = vRowOf.p; d.pPerm = pPerm.p; d.pRowOf = pRowOf.p; d.vAB = vAB.p; d.vN = vN.p;
@@ -620,7 +627,7 @@ struct pd_sim {
     refreshDev();
     zero(cellCnt.p, ncell);
     k_cell_count<<<nblk(numParticles), TPB, 0, st>>>(D, P, G);
-    scan(cellCnt.p, cellStart.p, ncell);
+    scanQuiet(cellCnt.p, cellStart.p, ncell);
     zero(cellCnt.p, ncell);
     k_cell_fill<<<nblk(numParticles), TPB, 0, st>>>(D, P);
     pairTmp.ensure((size_t)numReal * BP_CAP + 1, st);
@@ -635,15 +642,26 @@ struct pd_sim {
   }
 
   // narrowphase: counts, scan, write (pairs first, then planes per real)
+  WallSet wallSet() const {
+    WallSet W{};
+    W.n = (int)wallsHost.size();
+    for (int k = 0; k < W.n && k < 8; ++k) W.w[k] = wallsHost[k];
+    W.grid = wallGridD.p;
+    return W;
+  }
+  // contact slots: pairs, then planes per real, then walls per real (the reference's order)
   void narrowphase(const Params& P) {
-    const int nslots = np + numReal;
+    const int nslots = np + 2 * numReal;
+    const WallSet W = wallSet();
     ncnt.ensure(nslots + 1, st);
     noff.ensure(nslots + 1, st);
     refreshDev();
     k_np_pairs<false><<<nblk(np), TPB, 0, st>>>(D, P, np);
     k_np_planes<false><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
+    k_np_walls<false><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
     nc = scan(ncnt.p, noff.p, nslots);
     cAB.ensure(nc + 1, st);
+    cBV.ensure(nc + 1, st);
     cN.ensure(nc + 1, st);
     cRA.ensure(nc + 1, st);
     cRB.ensure(nc + 1, st);
@@ -654,6 +672,7 @@ struct pd_sim {
     k_np_pairs<true><<<nblk(np), TPB, 0, st>>>(D, P, np);
     kernEnd(K_NPPAIRS, np);
     k_np_planes<true><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
+    if (W.n > 0) k_np_walls<true><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
     HIPCHECK(hipGetLastError());
   }
 
@@ -665,7 +684,8 @@ struct pd_sim {
     k_mcount<<<nblk(numReal), TPB, 0, st>>>(D, P, np);
     nm = scan(mCnt.p, mStart.p, numReal);
     const size_t M = nm + 1;
-    mAB.ensure(M, st); mReal.ensure(M, st); mRange.ensure(M, st); mNs.ensure(M, st);
+    mAB.ensure(M, st); mReal.ensure(M, st); mRange.ensure(M, st); mRange2.ensure(M, st);
+    mWV.ensure(M, st); mNs.ensure(M, st);
     mTA.ensure(M, st); mTB.ensure(M, st); mRA.ensure(M, st); mRB.ensure(M, st);
     mNp.ensure(M, st); mKey.ensure(M, st); mCode.ensure(M, st); mColor.ensure(M, st);
     vRowOf.ensure(M, st);
@@ -788,9 +808,10 @@ struct pd_sim {
     std::memset(&stats, 0, sizeof stats);
     if (stabMode == 2 || stabMode == 4)
       throw std::runtime_error("stabilization modes 'multilevel' and 'ordered' are not implemented");
-    if (!wallsHost.empty())
-      throw std::runtime_error("SDF walls in the XPBD substep are not implemented");
-    const bool frictionOn = frictionDynamic > 0.0;
+    // a frictional wall drives friction even for frictionless bodies (src/sim.hpp:169-170)
+    double wallFrictionMax = 0.0;
+    for (const WallP& w : wallsHost) wallFrictionMax = std::max(wallFrictionMax, w.friction);
+    const bool frictionOn = frictionDynamic > 0.0 || wallFrictionMax > 0.0;
     const bool usePersist = (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0);
     // count-averaged Coulomb friction when the PGS cone is off (src/sim.hpp:173-176)
     const bool legacyFriction = frictionOn && !(usePersist && useGS);
@@ -869,7 +890,7 @@ struct pd_sim {
     const size_t R = nRowsV + 1;
     vAB.ensure(R, st); vN.ensure(R, st); vTA.ensure(R, st); vTB.ensure(R, st); vRA.ensure(R, st);
     vRB.ensure(R, st); vVt0.ensure(R, st); vLamT.ensure(R, st); vVn0.ensure(R, st);
-    vLam.ensure(R, st); vFlag.ensure(R, st);
+    vLam.ensure(R, st); vFlag.ensure(R, st); vWV.ensure(R, st);
     refreshDev();
     if (nm > 0) k_mkeys<<<nblk(nm), TPB, 0, st>>>(D, nm, gHat);
     if (nRowsV > 0)
@@ -977,7 +998,7 @@ struct pd_sim {
       refreshDev();
       zero(bucketCnt.p, numReal + 1);
       k_ledger_count<<<nblk(nm), TPB, 0, st>>>(D, nm, numReal);
-      scan(bucketCnt.p, bucketStart.p, numReal + 1);
+      scanQuiet(bucketCnt.p, bucketStart.p, numReal + 1);
       zero(bucketCnt.p, numReal + 1);
       k_ledger_scatter<<<nblk(nm), TPB, 0, st>>>(D, nm, numReal);
       k_ledger_sort<<<nblk(numReal), TPB, 0, st>>>(D, numReal);
@@ -1210,7 +1231,7 @@ struct pd_sim {
                             hipMemcpyDeviceToDevice, st));
     zero(hJCnt.p, n + 1);
     k_hz_jcount<<<nblk(np), TPB, 0, st>>>(Z, np, hJCnt.p);
-    scan(hJCnt.p, hJStart.p, n);
+    scanQuiet(hJCnt.p, hJStart.p, n);
     zero(hJCnt.p, n + 1);
     k_hz_jfill<<<nblk(np), TPB, 0, st>>>(Z, np, hJCnt.p);
     hNumPairs = np;

@@ -46,6 +46,15 @@ def apply_scene(sim, scene):
         sim.set_domain(*s["domain"], *s["periodic"])
     for pl in s.get("planes", []):
         sim.add_plane(*pl)
+    for w in s.get("walls", []):
+        from peclet.dem import build_wall_sdf
+
+        wid = build_wall_sdf(w["f"], w["bounds"], resolution=w["res"]).add_to(
+            sim, restitution=w["e"], friction=w["mu"])
+        if "vel" in w:
+            sim.set_wall_velocity(wid, *w["vel"])
+        if "mat" in w:
+            sim.set_wall_material_id(wid, w["mat"])
     sim.set_positions(s["pos"])
     if "scale" in s:
         sim.set_scales(s["scale"])
@@ -96,6 +105,19 @@ def make_scene(kind, n=256, seed=0):
         if kind == "bed_small_box":
             sc["pos"][:, 0] %= L
             sc["pos"][:, 1] %= L
+        return sc
+    if kind == "bed_walls":
+        # non-periodic bed in an SDF box whose floor moves like a belt
+        sc = make_scene("bed", n, seed)
+        L = sc["domain"][0]
+        lo, hi = (-0.1 * L, -0.1 * L, -2e-3), (1.1 * L, 1.1 * L, sc["domain"][2])
+        sc["periodic"] = (False, False, False)
+        sc["planes"] = []
+        sc["walls"] = [dict(f=lambda p: np.minimum.reduce([p[:, 0], L - p[:, 0], p[:, 1],
+                                                           L - p[:, 1], p[:, 2]]),
+                            bounds=(lo, hi), res=(24, 24, 40), e=0.2, mu=0.6,
+                            vel=((1e-3, 0.0, 0.0), (0.0, 0.0, 2.0), (L / 2, L / 2, 0.0)))]
+        sc["pos"][:, :2] = np.clip(sc["pos"][:, :2], 1.05e-3, L - 1.05e-3)
         return sc
     if kind == "gas":
         # granular gas in a closed box, g = 0: one-shot coloured GS path

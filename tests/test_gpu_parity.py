@@ -207,3 +207,23 @@ def test_bed_jacobi_velocity_with_friction(hip_lib):
 @pytest.mark.gpu
 def test_bed_jacobi_frictionless(hip_lib):
     run_pair("bed", 300, 10, use_gs=False, material=(0.3, 0.0, 0.0))
+
+
+@pytest.mark.gpu
+def test_bed_in_sdf_box_with_moving_floor(hip_lib):
+    run_pair("bed_walls", 300, 15)
+
+
+@pytest.mark.gpu
+def test_cylinders_in_sdf_box(hip_lib):
+    sc_walls = [dict(f=lambda p: np.minimum(p[:, 2], 20.0 - p[:, 0]), bounds=((-1, -1, -1),
+                     (21, 21, 21)), res=(24, 24, 24), e=0.3, mu=0.4)]
+    run_pair("cylinders", 27, 6, walls=sc_walls, planes=[])
+
+
+@pytest.mark.gpu
+def test_frictional_wall_drives_legacy_friction(hip_lib):
+    # zero gravity, frictionless bodies, frictional walls: the count-averaged friction path
+    sc_walls = [dict(f=lambda p: 5.4 - np.abs(p[:, 0] - 6.0), bounds=((-1, -1, -1), (13, 13, 13)),
+                     res=(20, 20, 20), e=0.9, mu=0.5)]
+    run_pair("gas", 300, 12, walls=sc_walls)

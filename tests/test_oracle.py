@@ -463,3 +463,50 @@ def test_ledger_sorted_and_cone_bounded(dense_bed):
     assert np.all(np.diff(keys.astype(np.float64)) >= 0)
     assert np.all(lam[live] >= 0)
     assert np.all(np.isfinite(lamT[live]))
+
+
+# --------------------------------------------------------------------------- SDF walls (XPBD)
+def sphere_on(floor, v0=(0, 0, 0), mu=0.5, steps=1200, dt=0.005, belt=None):
+    s = oracle_sim(1)
+    s.set_sphere_shape(0.5)
+    lo, hi = (0, 0, -1), (200, 8, 6)
+    s.set_domain(lo, hi)
+    s.enable_periodicity(False, False, False)
+    if floor == "plane":
+        s.add_plane(0, 0, 0, 0, 0, 1)
+    else:
+        from peclet.dem import build_wall_sdf
+
+        w = build_wall_sdf(lambda p: p[:, 2], (lo, hi), resolution=(64, 8, 16))
+        wid = w.add_to(s, restitution=-1.0, friction=mu)
+        if belt is not None:
+            s.set_wall_velocity(wid, (belt, 0, 0))
+    s.set_positions(np.array([[5.0, 4.0, 0.5]]))
+    s.set_inv_mass(np.ones(1))
+    s.set_inv_inertia(np.full((1, 3), 1.0 / (0.4 * 0.25)))
+    s.set_velocities(np.array([v0], float))
+    s.set_gravity(0, 0, -10.0)
+    s.set_material_params(0.0, 0.0, mu)
+    s.set_solver_iterations(12, 8)
+    for _ in range(steps):
+        s.step(dt)
+    return s
+
+
+def test_wall_floor_matches_plane_floor():
+    """A linear SDF floor samples exactly like the plane z = 0 (normal (0,0,1), gap z - r)."""
+    a = sphere_on("plane", v0=(4.0, 0, 0))
+    b = sphere_on("wall", v0=(4.0, 0, 0))
+    np.testing.assert_allclose(b.get_positions(), a.get_positions(), rtol=0, atol=1e-9)
+    np.testing.assert_allclose(b.get_velocities(), a.get_velocities(), rtol=0, atol=1e-9)
+    assert b.get_velocities()[0, 0] / 4.0 == pytest.approx(5 / 7, abs=0.01)
+
+
+def test_moving_wall_belt_two_sevenths():
+    """A sphere dropped on a belt moving at V ends rolling at 2/7 V (angular momentum about the
+    contact point is conserved during the slip)."""
+    s = sphere_on("wall", belt=2.0, steps=1200)
+    v = s.get_velocities()[0]
+    w = s.get_angular_velocities()[0]
+    assert v[0] / 2.0 == pytest.approx(2 / 7, abs=0.01)
+    assert (v[0] - w[1] * 0.5) == pytest.approx(2.0, abs=0.02)  # contact point moves with the belt
