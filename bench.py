@@ -69,6 +69,10 @@ def kernel_bytes(name, units, st):
         return 36 * st["num_particles"] + 4 * N + 8 * st["num_pairs"]
     if name == "k_np_pairs":
         return 156 * st["num_pairs"] + 104 * st["num_contacts"]
+    if name == "k_vsolve_block":  # every sweep of the substep's velocity loops
+        return 601 * st["active_manifolds"] * (st["sweeps_vel"] + st["sweeps_stab"])
+    if name == "k_psolve_block":
+        return 388 * st["num_contacts"] * st["sweeps_pos"]
     if name == "k_predict":
         return 240 * units
     if name == "k_reintegrate":

@@ -175,6 +175,10 @@ int pd_batch_step(pd_batch* b, double dt, int nsteps); /* nsteps substeps of eve
  * position sweep, broadphase query, narrowphase, predict, re-integrate).  Values accumulate
  * until pd_set_timing is called again. */
 int pd_set_timing(pd_sim* s, int level);
+/* Launch-shape tuning (no reference counterpart): "block_solver_rows" = the largest velocity /
+ * position sweep loop (in rows) solved inside one workgroup instead of one launch per colour
+ * (default 16384; 0 disables).  Results do not depend on it. */
+int pd_set_tuning(pd_sim* s, const char* name, long value);
 int pd_get_stage_times(pd_sim* s, double* out_ms, int cap);
 const char* pd_stage_name(int stage);
 /* per timed kernel k: summed ms, launch count and units processed (manifold rows, contact rows,

@@ -1622,6 +1622,72 @@ __global__ void k_vsweep_serial(Dev D, Params P, int numManifolds, int colour, i
   }
 }
 
+// ---- small problems: a whole adaptive sweep loop in one workgroup ---------------------------
+// Rows of one colour are body-disjoint, so the colours run in order separated by workgroup
+// barriers and each sweep's residual is a block reduction; the stop rule is the loop's own.  The
+// result is the same as one launch per colour, without ~(colours x sweeps) launches.
+constexpr int BLOCK_SOLVER_THREADS = 1024;
+constexpr int BLOCK_SOLVER_ROWS = 16384;
+struct ClassStarts {
+  int s[65];
+};
+__device__ __forceinline__ unsigned long long blockMaxAll(unsigned long long v,
+                                                          unsigned long long* sh) {
+  v = waveMaxU64(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  unsigned long long m = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = sh[w] > m ? sh[w] : m;
+  return m;
+}
+// MODE 0 = warm start (one pass), 1 = PGS, 2 = one-shot GS.  `run` (optional) gates the whole
+// loop; `stop` / `count` / `bits` are the loop's stop flag, sweep counter and residual.
+template <int MODE>
+__global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_vsolve_block(
+    Dev D, Params P, ClassStarts cs, int numColors, int numManifolds, int iters, double thr,
+    double stopThr, int* stop, int* count, const int* run) {
+  __shared__ unsigned long long sh[BLOCK_SOLVER_THREADS / 64];
+  if ((run && !*run) || *stop) return;
+  for (int it = 0; it < iters; ++it) {
+    unsigned long long m = 0;
+    for (int c = 0; c < numColors; ++c) {
+      const int b = cs.s[c], e = cs.s[c + 1];
+      if (e <= b) continue;
+      if (c == 62) {
+        if (threadIdx.x == 0)
+          for (int q = 0; q < numManifolds; ++q) {
+            if (D.mColor[q] != c) continue;
+            const int k = D.vRowOf[q];
+            if (MODE == 0) warmApplyRow(D, k);
+            else {
+              const unsigned long long a = dbits(MODE == 1 ? pgsRow(D, P, k, thr) : gsRow(D, P, k, thr));
+              m = a > m ? a : m;
+            }
+          }
+      } else {
+        for (int k = b + (int)threadIdx.x; k < e; k += blockDim.x) {
+          if (MODE == 0) warmApplyRow(D, k);
+          else {
+            const unsigned long long a = dbits(MODE == 1 ? pgsRow(D, P, k, thr) : gsRow(D, P, k, thr));
+            m = a > m ? a : m;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (MODE == 0) return;
+    const unsigned long long M = blockMaxAll(m, sh);
+    if (threadIdx.x == 0) {
+      *D.maxApproachBits = M;
+      *count += 1;
+    }
+    if (bitsd(M) <= stopThr) {
+      if (threadIdx.x == 0) *stop = 1;
+      return;
+    }
+  }
+}
 // commitPairKeysLambdaKokkos, src/contact_preprocessing.hpp:160-192: bucket by the low real index
 // (key >> 32), then sort each bucket by (key, manifold index) — a stable key sort
 __global__ void k_ledger_count(Dev D, int numManifolds, int numReal) {
@@ -1785,6 +1851,44 @@ __global__ void k_psweep_serial(Dev D, int numContacts, int colour, const int* s
   if (gtid() != 0 || *stop) return;
   for (int c = 0; c < numContacts; ++c)
     if (D.cColor[c] == colour) atomicMaxD(D.maxOverlapBits, posRow(D, D.pRowOf[c]));
+}
+
+// position sweeps (XPBD), strict stop: residual < tol
+__global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psolve_block(
+    Dev D, ClassStarts cs, int numColors, int numContacts, int iters, double tol, int* stop,
+    int* count) {
+  __shared__ unsigned long long sh[BLOCK_SOLVER_THREADS / 64];
+  if (*stop) return;
+  for (int it = 0; it < iters; ++it) {
+    unsigned long long m = 0;
+    for (int c = 0; c < numColors; ++c) {
+      const int b = cs.s[c], e = cs.s[c + 1];
+      if (e <= b) continue;
+      if (c == 62) {
+        if (threadIdx.x == 0)
+          for (int q = 0; q < numContacts; ++q)
+            if (D.cColor[q] == c) {
+              const unsigned long long a = dbits(posRow(D, D.pRowOf[q]));
+              m = a > m ? a : m;
+            }
+      } else {
+        for (int k = b + (int)threadIdx.x; k < e; k += blockDim.x) {
+          const unsigned long long a = dbits(posRow(D, k));
+          m = a > m ? a : m;
+        }
+      }
+      __syncthreads();
+    }
+    const unsigned long long M = blockMaxAll(m, sh);
+    if (threadIdx.x == 0) {
+      *D.maxOverlapBits = M;
+      *count += 1;
+    }
+    if (bitsd(M) < tol) {
+      if (threadIdx.x == 0) *stop = 1;
+      return;
+    }
+  }
 }
 
 // commitPosImpulseKokkos, src/contact_preprocessing.hpp:501-538 (sum in contact order; among

@@ -135,12 +135,15 @@ struct pd_sim {
   DBuf<unsigned long long> hPrevKeys;
   DBuf<HzTab> hTab;
   HzDev Z{};
+  // sweep loops over at most this many rows run in one workgroup (k_vsolve_block/k_psolve_block)
+  int blockRows = BLOCK_SOLVER_ROWS;
   // timing: stage events (level >= 1) and per-launch events of the sweep kernels (level >= 2)
   int timing = 0;
   double stageMs[ST_COUNT] = {};
   hipEvent_t ev[ST_COUNT + 1] = {};
   // kernels timed per launch at level 2; rows = the units each launch processed
-  enum { K_VSWEEP, K_PSWEEP, K_BPQUERY, K_NPPAIRS, K_PREDICT, K_REINTEGRATE, K_COUNT };
+  enum { K_VSWEEP, K_PSWEEP, K_BPQUERY, K_NPPAIRS, K_PREDICT, K_REINTEGRATE, K_VBLOCK, K_PBLOCK,
+         K_COUNT };
   double kernMs[K_COUNT] = {};
   long kernLaunches[K_COUNT] = {};
   long kernRows[K_COUNT] = {};
@@ -778,6 +781,11 @@ struct pd_sim {
       batch *= 2;
     }
   }
+  static ClassStarts classStarts(const std::vector<int>& s) {
+    ClassStarts c{};
+    for (int k = 0; k < 65 && k < (int)s.size(); ++k) c.s[k] = s[k];
+    return c;
+  }
   double readMaxApproach() {
     return bitsd(read1(bits.p + 1));
   }
@@ -907,7 +915,12 @@ struct pd_sim {
         std::swap(ga, gb);
       }
       // 8 swaps: the result is back in `ground`
-      velocitySweep(vs, numColors, 0, 0.0, ctl.p + CTL_NONE);
+      if (nRowsV <= blockRows)
+        k_vsolve_block<0><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(D, P, classStarts(vs), numColors, nm,
+                                                             1, 0.0, 0.0, ctl.p + CTL_NONE,
+                                                             ctl.p + CTL_NONE, nullptr);
+      else
+        velocitySweep(vs, numColors, 0, 0.0, ctl.p + CTL_NONE);
     }
     HIPCHECK(hipGetLastError());
     mark(ST_VSWEEP);
@@ -918,7 +931,19 @@ struct pd_sim {
       // device-controlled adaptive stops: no host round trip per sweep
       zero(ctl.p, CTL_COUNT);
       const double stopThr = usePGS ? 0.02 * vRest : vRest;
-      sweepLoop(velIters, ctl.p + CTL_STOP_V, [&] {
+      const bool smallV = nRowsV <= blockRows;
+      if (smallV && velIters > 0) {
+        kernBegin();
+        if (usePGS)
+          k_vsolve_block<1><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(
+              D, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
+              ctl.p + CTL_SWEEPS_V, nullptr);
+        else
+          k_vsolve_block<2><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(
+              D, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
+              ctl.p + CTL_SWEEPS_V, nullptr);
+        kernEnd(K_VBLOCK, nRowsV);
+      } else if (!smallV) sweepLoop(velIters, ctl.p + CTL_STOP_V, [&] {
         k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V);
         velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest, ctl.p + CTL_STOP_V);
         k_sweep_check<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V, ctl.p + CTL_SWEEPS_V,
@@ -927,7 +952,19 @@ struct pd_sim {
       mark(ST_STAB);
       if (usePGS) {
         k_stab_gate<<<1, 1, 0, st>>>(bits.p + 1, ctl.p, stabMode, vRestS);
-        if (stabMode == 1 && velIters > 0 && read1(ctl.p + CTL_STAB_RUN)) {
+        if (smallV) {
+          // the gate stays on the device: the flags kernel and the loop read it
+          if (stabMode == 1 && velIters > 0) {
+            if (nRowsV > 0)
+              k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
+                                                         8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
+            kernBegin();
+            k_vsolve_block<1><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(
+                D, P, classStarts(vs), numColors, nm, 2 * velIters, vRestS, vRestS,
+                ctl.p + CTL_STOP_S, ctl.p + CTL_SWEEPS_S, ctl.p + CTL_STAB_RUN);
+            kernEnd(K_VBLOCK, nRowsV);
+          }
+        } else if (stabMode == 1 && velIters > 0 && read1(ctl.p + CTL_STAB_RUN)) {
           if (nRowsV > 0)
             k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
                                                        8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
@@ -1059,6 +1096,15 @@ struct pd_sim {
         k_pos_jacobi_accum<<<1, 1, 0, st>>>(D, nc, 0);
         k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
         stats.sweeps_pos++;
+      }
+      HIPCHECK(hipGetLastError());
+    } else if (posLeftover == 0 && nRowsP <= blockRows) {
+      if (posIters > 0) {
+        kernBegin();
+        k_psolve_block<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(D, classStarts(ps), numPosColors, nc,
+                                                           posIters, posTol, ctl.p + CTL_STOP_P,
+                                                           ctl.p + CTL_SWEEPS_P);
+        kernEnd(K_PBLOCK, nRowsP);
       }
       HIPCHECK(hipGetLastError());
     } else if (posLeftover == 0) {
@@ -2009,6 +2055,17 @@ long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, in
   return used;
 }
 
+int pd_set_tuning(pd_sim* s, const char* name, long value) {
+  return guard([&] {
+    const std::string k = name ? name : "";
+    if (k == "block_solver_rows") {
+      if (value < 0) throw ArgError("block_solver_rows must be >= 0");
+      s->blockRows = (int)std::min<long>(value, 1L << 30);
+    } else {
+      throw ArgError("unknown tuning parameter: " + k);
+    }
+  });
+}
 int pd_set_timing(pd_sim* s, int level) {
   s->timing = level;
   for (double& t : s->stageMs) t = 0.0;
@@ -2032,8 +2089,9 @@ int pd_get_stage_times(pd_sim* s, double* out, int cap) {
   return ST_COUNT;
 }
 const char* pd_kernel_name(int k) {
-  static const char* names[pd_sim::K_COUNT] = {"k_vsweep",  "k_psweep",  "k_bp_query1",
-                                               "k_np_pairs", "k_predict", "k_reintegrate"};
+  static const char* names[pd_sim::K_COUNT] = {
+      "k_vsweep",  "k_psweep",      "k_bp_query1",    "k_np_pairs",
+      "k_predict", "k_reintegrate", "k_vsolve_block", "k_psolve_block"};
   return (k >= 0 && k < pd_sim::K_COUNT) ? names[k] : "";
 }
 const char* pd_stage_name(int stage) {

@@ -471,6 +471,10 @@ class Simulation:
                            _ptr(lamT), _ptr(pimp), n)
         return keys, lam, lamT, pimp
 
+    def set_tuning(self, name, value):
+        """Launch-shape knob of the HIP library (extension); results do not depend on it."""
+        self._chk(self._b.set_tuning(self._h, name.encode(), int(value)))
+
     def debug_hertz_pairs(self):
         n = self._b.get_hertz_pairs(self._h, None, None, 0)
         pairs = np.empty((n, 2), np.int32)

@@ -125,6 +125,7 @@ PRODUCT_SIGNATURES = {
     "stage_name": (C.c_char_p, [_i]),
     "get_kernel_times": (_i, [_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long), _i]),
     "kernel_name": (C.c_char_p, [_i]),
+    "set_tuning": (_i, [_p, C.c_char_p, C.c_long]),
     "batch_create": (_p, [_i, _ip]),
     "batch_destroy": (None, [_p]),
     "batch_size": (_i, [_p]),

@@ -76,10 +76,12 @@ def assert_ledger(h, r, tag):
     assert rel_err(hp[head], rp[head]) <= RTOL, f"{tag}: ledger position impulse"
 
 
-def run_pair(kind, n, steps, seed=0, check_topology_every=1, **over):
+def run_pair(kind, n, steps, seed=0, check_topology_every=1, block_rows=None, **over):
     scene = make_scene(kind, n, seed)
     scene.update(over)
     h = apply_scene(hip_sim(n), scene)
+    if block_rows is not None:
+        h.set_tuning("block_solver_rows", block_rows)
     r = apply_scene(oracle_sim(n), scene)
     for k in range(steps):
         tag = f"{kind} n={n} seed={seed} substep {k}"
@@ -101,13 +103,16 @@ def test_bed_first_substep(hip_lib):
 
 
 @pytest.mark.gpu
-def test_bed_multistep(hip_lib):
-    run_pair("bed", 300, 25)
+@pytest.mark.parametrize("block_rows", [None, 0])
+def test_bed_multistep(hip_lib, block_rows):
+    # None: sweep loops inside one workgroup; 0: one launch per colour and sweep
+    run_pair("bed", 300, 25, block_rows=block_rows)
 
 
 @pytest.mark.gpu
-def test_bed_periodic_multistep(hip_lib):
-    h, r = run_pair("bed_periodic", 400, 25)
+@pytest.mark.parametrize("block_rows", [None, 0])
+def test_bed_periodic_multistep(hip_lib, block_rows):
+    h, r = run_pair("bed_periodic", 400, 25, block_rows=block_rows)
     assert h.get_stats()["num_particles"] > h.num_particles()  # ghosts were generated
 
 
@@ -118,8 +123,9 @@ def test_bed_small_periodic_box_twins(hip_lib):
 
 
 @pytest.mark.gpu
-def test_gas_one_shot_gs(hip_lib):
-    run_pair("gas", 300, 20)
+@pytest.mark.parametrize("block_rows", [None, 0])
+def test_gas_one_shot_gs(hip_lib, block_rows):
+    run_pair("gas", 300, 20, block_rows=block_rows)
 
 
 @pytest.mark.gpu
@@ -130,6 +136,11 @@ def test_hollow_cylinders(hip_lib):
 @pytest.mark.gpu
 def test_boxes(hip_lib):
     run_pair("boxes", 27, 8)
+
+
+@pytest.mark.gpu
+def test_cylinders_one_launch_per_colour(hip_lib):
+    run_pair("cylinders", 27, 8, block_rows=0)
 
 
 @pytest.mark.gpu
