@@ -55,6 +55,12 @@ int pd_synchronize(pd_sim* s);        /* Kokkos::fence equivalent: wait for the 
 /* initializeShape(shape_type, radius, height, thickness), src/sim.hpp:949-1021 */
 int pd_initialize_shape(pd_sim* s, int shape_type, double radius, double height,
                         double thickness);
+/* setSdfShape(grid, nx, ny, nz, origin, spacing, shell, invInertia, boundingRadius),
+ * src/sim.hpp:1034-1084: shape 0 becomes an imported grid SDF (x-fastest samples, negative
+ * inside) probed by the (nPts,3) point shell; boundingRadius is the canonical bound */
+int pd_set_sdf_shape(pd_sim* s, const double* grid, int nx, int ny, int nz, double ox, double oy,
+                     double oz, double sx, double sy, double sz, const double* shell, int nPts,
+                     double iix, double iiy, double iiz, double boundingRadius);
 /* Extension (no reference counterpart): replace the surface point shell of shape 0 (N,3). */
 int pd_set_shell_points(pd_sim* s, const double* pts, int n);
 int pd_get_shell_points(pd_sim* s, double* out, int cap);

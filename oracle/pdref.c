@@ -197,6 +197,10 @@ typedef struct Sim {
   real maxOverlap, maxApproach;
   int numColorsVel, numColorsPos;
   pdref_stats stats;
+  /* grid-SDF particle shape (src/sim.hpp:1034-1084) */
+  real* sdfGrid;
+  int gnx, gny, gnz;
+  V3 gOrigin, gInvSpacing;
   /* SDF walls (src/sim.hpp:1242-1287) */
   Wall walls[MAXWALLS];
   int numWalls;
@@ -478,7 +482,7 @@ void pdref_destroy(void* s) {
   free(S->sideFlags); free(S->lambdaAcc); free(S->posImpulse); free(S->vn0);
   free(S->posLambda); free(S->lambdaT); free(S->vt0); free(S->prevKeys);
   free(S->prevLambda); free(S->prevPosImpulse); free(S->prevLambdaT);
-  free(S->wallGrid); free(S->hPairs); free(S->hKeys); free(S->hPrevKeys); free(S->hXi);
+  free(S->wallGrid); free(S->sdfGrid); free(S->hPairs); free(S->hKeys); free(S->hPrevKeys); free(S->hXi);
   free(S->hPrevXi); free(S->hSn); free(S->hRefPos); free(S->hForce); free(S->hTorque);
   free(S->hXiWall); free(S->hSnWall); free(S->hWallCand);
   free(S);
@@ -1035,7 +1039,41 @@ static void pushContact(Sim* S, const Contact* c) {
   ensureContacts(S, S->nc + 1);
   S->contacts[S->nc++] = *c;
 }
-static real sdfShape(const Sim* S, V3 p) { return sdfEval(p, S->shapeType, S->shapeParams); }
+/* trilinear grid SDF with the off-grid residual added (object convention),
+ * src/narrowphase.hpp:97-135 */
+static real sampleGridSdf(const Sim* S, V3 p) {
+  const real fx = (p.x - S->gOrigin.x) * S->gInvSpacing.x;
+  const real fy = (p.y - S->gOrigin.y) * S->gInvSpacing.y;
+  const real fz = (p.z - S->gOrigin.z) * S->gInvSpacing.z;
+  const real cx = fmin(fmax(fx, 0.0), (real)(S->gnx - 1));
+  const real cy = fmin(fmax(fy, 0.0), (real)(S->gny - 1));
+  const real cz = fmin(fmax(fz, 0.0), (real)(S->gnz - 1));
+  const int ix = (int)cx, iy = (int)cy, iz = (int)cz;
+  const int ix1 = ix < S->gnx - 1 ? ix + 1 : ix;
+  const int iy1 = iy < S->gny - 1 ? iy + 1 : iy;
+  const int iz1 = iz < S->gnz - 1 ? iz + 1 : iz;
+  const real tx = cx - ix, ty = cy - iy, tz = cz - iz;
+  const long nxny = (long)S->gnx * S->gny;
+  const real* g = S->sdfGrid;
+#define AT(x, y, z) g[(long)(z) * nxny + (long)(y) * S->gnx + (x)]
+  const real c00 = AT(ix, iy, iz) * (1.0 - tx) + AT(ix1, iy, iz) * tx;
+  const real c10 = AT(ix, iy1, iz) * (1.0 - tx) + AT(ix1, iy1, iz) * tx;
+  const real c01 = AT(ix, iy, iz1) * (1.0 - tx) + AT(ix1, iy, iz1) * tx;
+  const real c11 = AT(ix, iy1, iz1) * (1.0 - tx) + AT(ix1, iy1, iz1) * tx;
+#undef AT
+  const real c0 = c00 * (1.0 - ty) + c10 * ty;
+  const real c1 = c01 * (1.0 - ty) + c11 * ty;
+  const real val = c0 * (1.0 - tz) + c1 * tz;
+  const real rx = (S->gInvSpacing.x > 0.0) ? (fx - cx) / S->gInvSpacing.x : 0.0;
+  const real ry = (S->gInvSpacing.y > 0.0) ? (fy - cy) / S->gInvSpacing.y : 0.0;
+  const real rz = (S->gInvSpacing.z > 0.0) ? (fz - cz) / S->gInvSpacing.z : 0.0;
+  return val + sqrt(rx * rx + ry * ry + rz * rz);
+}
+/* sdfEvalShape, src/narrowphase.hpp:169-173 */
+static real sdfShape(const Sim* S, V3 p) {
+  if (S->shapeType == SHAPE_GRID_SDF) return sampleGridSdf(S, p);
+  return sdfEval(p, S->shapeType, S->shapeParams);
+}
 
 static void detectContacts(Sim* S, real margin) {
   const real gs = S->globalScale;
@@ -3043,4 +3081,31 @@ static void detectWallSdf(Sim* S, real margin) {
       }
     }
   }
+}
+
+/* setSdfShape(grid, nx, ny, nz, origin, spacing, shell, invInertia, boundingRadius),
+ * src/sim.hpp:1034-1084: shape 0 becomes a grid SDF probed by the given point shell */
+int pdref_set_sdf_shape(void* s, const double* grid, int nx, int ny, int nz, double ox,
+                        double oy, double oz, double sx, double sy, double sz,
+                        const double* shell, int nPts, double iix, double iiy, double iiz,
+                        double boundingRadius) {
+  Sim* S = asSim(s);
+  if (nx < 2 || ny < 2 || nz < 2) return fail("setSdfShape: grid dims must be >= 2 on each axis");
+  if (nPts <= 0) return fail("setSdfShape: empty surface point shell");
+  const long cells = (long)nx * ny * nz;
+  S->sdfGrid = (real*)xrealloc(S->sdfGrid, sizeof(real) * (size_t)cells);
+  memcpy(S->sdfGrid, grid, sizeof(real) * (size_t)cells);
+  S->gnx = nx; S->gny = ny; S->gnz = nz;
+  S->gOrigin = mk3(ox, oy, oz);
+  S->gInvSpacing = mk3(sx > 0 ? 1.0 / sx : 0.0, sy > 0 ? 1.0 / sy : 0.0, sz > 0 ? 1.0 / sz : 0.0);
+  free(S->shell);
+  S->shell = (V3*)xrealloc(NULL, sizeof(V3) * (size_t)nPts);
+  for (int i = 0; i < nPts; ++i) S->shell[i] = mk3(shell[3 * i], shell[3 * i + 1], shell[3 * i + 2]);
+  S->numShell = nPts;
+  S->shapeType = SHAPE_GRID_SDF;
+  S->shapeParams[0] = boundingRadius;
+  S->shapeParams[1] = S->shapeParams[2] = S->shapeParams[3] = 0.0;
+  S->baseRadius = boundingRadius;
+  S->defaultInvI = mk3(iix, iiy, iiz);
+  return 0;
 }

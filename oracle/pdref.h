@@ -88,6 +88,10 @@ int pdref_get_inv_mass(void* s, double* out, int n);
 int pdref_get_scales(void* s, double* out, int n);
 
 int pdref_step(void* s, double dt);
+int pdref_set_sdf_shape(void* s, const double* grid, int nx, int ny, int nz, double ox,
+                        double oy, double oz, double sx, double sy, double sz,
+                        const double* shell, int nPts, double iix, double iiy, double iiz,
+                        double boundingRadius);
 /* SDF walls and the Hertz–Mindlin engine (src/sim.hpp:1153-1287, :628-750) */
 int pdref_add_sdf_wall(void* s, const double* grid, int nx, int ny, int nz, double ox, double oy,
                        double oz, double sx, double sy, double sz, double restitution,

@@ -78,12 +78,48 @@ struct ShapeP {
   int type;
   int numShell;
   double p0, p1, p2, p3;
+  // imported grid SDF (type SHAPE_GRID_SDF): samples x fastest at origin + (x,y,z) / invSpacing
+  const double* grid;
+  int nx, ny, nz;
+  d3 origin, invSpacing;
 };
 
+// trilinear grid SDF with the off-grid residual added (object convention),
+// src/narrowphase.hpp:97-135
+PD_HD double sampleGridSdf(const ShapeP& s, d3 p) {
+  const double fx = (p.x - s.origin.x) * s.invSpacing.x;
+  const double fy = (p.y - s.origin.y) * s.invSpacing.y;
+  const double fz = (p.z - s.origin.z) * s.invSpacing.z;
+  const double cx = fmin(fmax(fx, 0.0), (double)(s.nx - 1));
+  const double cy = fmin(fmax(fy, 0.0), (double)(s.ny - 1));
+  const double cz = fmin(fmax(fz, 0.0), (double)(s.nz - 1));
+  const int ix = (int)cx, iy = (int)cy, iz = (int)cz;
+  const int ix1 = ix < s.nx - 1 ? ix + 1 : ix;
+  const int iy1 = iy < s.ny - 1 ? iy + 1 : iy;
+  const int iz1 = iz < s.nz - 1 ? iz + 1 : iz;
+  const double tx = cx - ix, ty = cy - iy, tz = cz - iz;
+  const long nxny = (long)s.nx * s.ny;
+  const double* g = s.grid;
+  auto at = [&](int x, int y, int z) { return g[(long)z * nxny + (long)y * s.nx + x]; };
+  const double c00 = at(ix, iy, iz) * (1.0 - tx) + at(ix1, iy, iz) * tx;
+  const double c10 = at(ix, iy1, iz) * (1.0 - tx) + at(ix1, iy1, iz) * tx;
+  const double c01 = at(ix, iy, iz1) * (1.0 - tx) + at(ix1, iy, iz1) * tx;
+  const double c11 = at(ix, iy1, iz1) * (1.0 - tx) + at(ix1, iy1, iz1) * tx;
+  const double c0 = c00 * (1.0 - ty) + c10 * ty;
+  const double c1 = c01 * (1.0 - ty) + c11 * ty;
+  const double val = c0 * (1.0 - tz) + c1 * tz;
+  const double rx = (s.invSpacing.x > 0.0) ? (fx - cx) / s.invSpacing.x : 0.0;
+  const double ry = (s.invSpacing.y > 0.0) ? (fy - cy) / s.invSpacing.y : 0.0;
+  const double rz = (s.invSpacing.z > 0.0) ? (fz - cz) / s.invSpacing.z : 0.0;
+  return val + sqrt(rx * rx + ry * ry + rz * rz);
+}
+
+// sdfEvalShape, src/narrowphase.hpp:169-173
 PD_HD double sdfShape(const ShapeP& s, d3 p) {
   if (s.type == SHAPE_SPHERE) return sdfSphere(p, s.p0);
   if (s.type == SHAPE_HOLLOW_CYLINDER) return sdfHollowCylinder(p, s.p0, s.p1, s.p2);
   if (s.type == SHAPE_BOX) return sdfBox(p, s.p0, s.p1, s.p2);
+  if (s.type == SHAPE_GRID_SDF && s.grid) return sampleGridSdf(s, p);
   return 1e9;
 }
 

@@ -121,6 +121,10 @@ struct pd_sim {
   int ledgerBuckets = 0;  // real-index buckets of the sorted ledger (bucketStart has +1 entries)
   double maxOverlap = 0.0, maxApproach = 0.0;
   pd_stats stats{};
+  // imported grid-SDF particle shape (src/sim.hpp:1034-1084)
+  DBuf<double> sdfGridD;
+  int gridDims[3] = {0, 0, 0};
+  double gridOrigin[3] = {0, 0, 0}, gridInvSpacing[3] = {0, 0, 0};
   // SDF walls (src/sim.hpp:1242-1287) and the Hertz–Mindlin engine (src/particles.hpp:113-130)
   std::vector<WallP> wallsHost;
   std::vector<double> wallGridHost;
@@ -252,7 +256,7 @@ struct pd_sim {
     scanTmp.release(); ghostCnt.release(); ghostOff.release();
     dVel.release(); dAng.release(); dPos.release(); dCnt.release(); cLamN.release();
     frCnt.release();
-    wallGridD.release(); hPairs.release(); hIStart.release(); hJStart.release();
+    wallGridD.release(); sdfGridD.release(); hPairs.release(); hIStart.release(); hJStart.release();
     hJList.release(); hJCnt.release(); hWStart.release(); hWCand.release(); hWCnt.release();
     hXi.release(); hF.release(); hTA.release(); hTB.release(); hPrevXi.release();
     hRefPos.release(); hXiWall.release(); hWF.release(); hWT.release(); hPrevKeys.release();
@@ -394,6 +398,14 @@ struct pd_sim {
     P.shape.p1 = shapeParams[1];
     P.shape.p2 = shapeParams[2];
     P.shape.p3 = shapeParams[3];
+    if (shapeType == SHAPE_GRID_SDF) {
+      P.shape.grid = sdfGridD.p;
+      P.shape.nx = gridDims[0];
+      P.shape.ny = gridDims[1];
+      P.shape.nz = gridDims[2];
+      P.shape.origin = d3{gridOrigin[0], gridOrigin[1], gridOrigin[2]};
+      P.shape.invSpacing = d3{gridInvSpacing[0], gridInvSpacing[1], gridInvSpacing[2]};
+    }
     for (int k = 0; k < numPlanes; ++k) {
       P.planeP[k] = make_double4(planeP[k][0], planeP[k][1], planeP[k][2], 0.0);
       P.planeN[k] = make_double4(planeN[k][0], planeN[k][1], planeN[k][2], 0.0);
@@ -1722,6 +1734,36 @@ int pd_step(pd_sim* s, double dt) {
   return guard([&] {
     s->dt = dt;
     s->demStep();
+  });
+}
+int pd_set_sdf_shape(pd_sim* s, const double* grid, int nx, int ny, int nz, double ox, double oy,
+                     double oz, double sx, double sy, double sz, const double* shell, int nPts,
+                     double iix, double iiy, double iiz, double boundingRadius) {
+  return guard([&] {
+    if (nx < 2 || ny < 2 || nz < 2)
+      throw std::runtime_error("setSdfShape: grid dims must be >= 2 on each axis");
+    if (nPts <= 0) throw std::runtime_error("setSdfShape: empty surface point shell");
+    const size_t cells = (size_t)nx * ny * nz;
+    s->sdfGridD.ensure(cells, s->st);
+    s->writeN(s->sdfGridD.p, grid, cells);
+    s->gridDims[0] = nx;
+    s->gridDims[1] = ny;
+    s->gridDims[2] = nz;
+    s->gridOrigin[0] = ox;
+    s->gridOrigin[1] = oy;
+    s->gridOrigin[2] = oz;
+    s->gridInvSpacing[0] = sx > 0 ? 1.0 / sx : 0.0;
+    s->gridInvSpacing[1] = sy > 0 ? 1.0 / sy : 0.0;
+    s->gridInvSpacing[2] = sz > 0 ? 1.0 / sz : 0.0;
+    s->shellHost.assign(shell, shell + 3 * (size_t)nPts);
+    s->uploadShell();
+    s->shapeType = SHAPE_GRID_SDF;
+    s->shapeParams[0] = boundingRadius;
+    s->shapeParams[1] = s->shapeParams[2] = s->shapeParams[3] = 0.0;
+    s->baseRadius = boundingRadius;
+    s->defaultInvI[0] = iix;
+    s->defaultInvI[1] = iiy;
+    s->defaultInvI[2] = iiz;
   });
 }
 int pd_set_hertz_material(pd_sim* s, int mat, double youngs, double poisson) {

@@ -194,7 +194,16 @@ class Simulation:
 
     def set_sdf_shape(self, grid, nx, ny, nz, origin, spacing, shell, inv_inertia,
                       bounding_radius):
-        raise NotImplementedError("grid-SDF particle shapes are not implemented in this build")
+        """Shape 0 becomes an imported grid SDF (canonical space, x-fastest samples, negative
+        inside) probed by the (N,3) surface point shell."""
+        g = np.ascontiguousarray(np.asarray(grid, dtype=np.float64).ravel())
+        if g.size != int(nx) * int(ny) * int(nz):
+            raise RuntimeError("setSdfShape: grid.size() must equal nx*ny*nz")
+        sh = _f64(shell, 3)
+        self._chk(self._b.set_sdf_shape(self._h, _ptr(g), int(nx), int(ny), int(nz),
+                                        *map(float, origin), *map(float, spacing), _ptr(sh),
+                                        int(sh.shape[0]), *map(float, inv_inertia),
+                                        float(bounding_radius)))
 
     # ------------------------------------------------------------------ domain
     def set_domain(self, *args, **kw):

@@ -103,6 +103,7 @@ SIGNATURES = {
     "get_ledger": (_i, [_p, _u64p, _dp, _dp, _dp, _i]),
     "get_grounded": (_i, [_p, _ip, _i]),
     "add_sdf_wall": (_i, [_p, _dp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d]),
+    "set_sdf_shape": (_i, [_p, _dp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _dp, _i, _d, _d, _d, _d]),
     "set_wall_velocity": (_i, [_p, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d]),
     "set_wall_material_id": (_i, [_p, _i, _i]),
     "num_walls": (_i, [_p]),

@@ -33,7 +33,9 @@ def lattice_bed(n, r, seed, spacing=1.95, jitter=0.03, box_xy=None, layers=None,
 def apply_scene(sim, scene):
     """scene: dict produced by make_scene; calls the reference API on `sim`."""
     s = scene
-    if s["shape"][0] == SPHERE:
+    if "sdf_shape" in s:
+        sim.set_sdf_shape(*s["sdf_shape"])
+    elif s["shape"][0] == SPHERE:
         sim.set_sphere_shape(s["shape"][1])
     else:
         sim.initialize_shape(*s["shape"])
@@ -118,6 +120,21 @@ def make_scene(kind, n=256, seed=0):
                             bounds=(lo, hi), res=(24, 24, 40), e=0.2, mu=0.6,
                             vel=((1e-3, 0.0, 0.0), (0.0, 0.0, 2.0), (L / 2, L / 2, 0.0)))]
         sc["pos"][:, :2] = np.clip(sc["pos"][:, :2], 1.05e-3, L - 1.05e-3)
+        return sc
+    if kind == "gridboxes":
+        # the box scene with shape 0 imported as a grid SDF of the box and the box's shell
+        sc = make_scene("boxes", n, seed)
+        axes = [np.linspace(-0.8, 0.8, 33)] * 3
+        X, Y, Z = np.meshgrid(*axes, indexing="ij")
+        p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+        d = np.abs(p) - 0.5
+        f = np.linalg.norm(np.maximum(d, 0.0), axis=1) + np.minimum(np.max(d, axis=1), 0.0)
+        g = f.reshape(33, 33, 33).ravel(order="F")
+        k = np.linspace(-0.5, 0.5, 5)
+        shell = np.array([(x, y, z) for x in k for y in k for z in k
+                          if max(abs(x), abs(y), abs(z)) == 0.5])
+        sc["sdf_shape"] = (g, 33, 33, 33, (-0.8, -0.8, -0.8), (0.05, 0.05, 0.05), shell,
+                           (6.0, 6.0, 6.0), 0.5)
         return sc
     if kind == "gas":
         # granular gas in a closed box, g = 0: one-shot coloured GS path

@@ -24,10 +24,13 @@ def box_wall(lo, hi, res):
          p[:, 2]]), (lo, hi), resolution=res)
 
 
-def pile_scene(sim, n, seed, shape=None, mats=False, sp=1.12, row=5):
+def pile_scene(sim, n, seed, shape=None, mats=False, sp=1.12, row=5, sdf=None):
     rng = np.random.default_rng(seed)
     lo, hi = (0.0, 0.0, -1.0), (8.0, 8.0, 20.0)
-    if shape is None:
+    if sdf is not None:
+        sim.set_sdf_shape(*sdf)
+        shape = "grid"
+    elif shape is None:
         sim.set_sphere_shape(0.5)
     else:
         sim.initialize(**shape)
@@ -131,3 +134,11 @@ def test_hertz_rejects_planes(hip_lib):
     s.add_plane(0, 0, 0, 0, 0, 1)
     with pytest.raises(RuntimeError):
         s.step_hertz(1e-4)
+
+
+@pytest.mark.gpu
+def test_hertz_grid_sdf_boxes(hip_lib):
+    from scenes import make_scene
+
+    sdf = make_scene("gridboxes", 8, 0)["sdf_shape"]
+    run(27, 13, 2, 300, sdf=sdf, sp=1.8, row=3)

@@ -238,3 +238,8 @@ def test_frictional_wall_drives_legacy_friction(hip_lib):
     sc_walls = [dict(f=lambda p: 5.4 - np.abs(p[:, 0] - 6.0), bounds=((-1, -1, -1), (13, 13, 13)),
                      res=(20, 20, 20), e=0.9, mu=0.5)]
     run_pair("gas", 300, 12, walls=sc_walls)
+
+
+@pytest.mark.gpu
+def test_grid_sdf_boxes(hip_lib):
+    run_pair("gridboxes", 27, 20)

@@ -510,3 +510,87 @@ def test_moving_wall_belt_two_sevenths():
     w = s.get_angular_velocities()[0]
     assert v[0] / 2.0 == pytest.approx(2 / 7, abs=0.01)
     assert (v[0] - w[1] * 0.5) == pytest.approx(2.0, abs=0.02)  # contact point moves with the belt
+
+
+# --------------------------------------------------------------------------- grid-SDF shapes
+def box_sdf(p, h=1.0):
+    d = np.abs(p) - h
+    outside = np.linalg.norm(np.maximum(d, 0.0), axis=1)
+    inside = np.minimum(np.max(d, axis=1), 0.0)
+    return outside + inside
+
+
+def grid_of(f, lo, hi, n):
+    axes = [np.linspace(lo, hi, n)] * 3
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    g = f(pts).reshape(n, n, n).ravel(order="F")  # x fastest
+    return g, (lo, lo, lo), ((hi - lo) / (n - 1),) * 3
+
+
+def two_bodies(shape_setup, sep, n=2):
+    s = oracle_sim(n)
+    shape_setup(s)
+    s.set_domain((-10, -10, -10), (10, 10, 10))
+    s.enable_periodicity(False, False, False)
+    s.set_positions(np.array([[-sep / 2, 0.1, -0.05], [sep / 2, 0.1, -0.05]]))
+    s.set_solver_iterations(0, 0)
+    s.step(1e-9)
+    return s
+
+
+def test_grid_box_matches_analytic_box():
+    """Face-to-face boxes: the trilinear grid of the (piecewise linear) box SDF reproduces the
+    analytic box's contacts away from edges."""
+    ref_shell = oracle_sim(1)
+    ref_shell.initialize(shape_type=3, radius=1.0)
+    shell = ref_shell.get_shell_points()
+    g, origin, spacing = grid_of(box_sdf, -1.6, 1.6, 65)
+
+    def grid_box(s):
+        s.set_sdf_shape(g, 65, 65, 65, origin, spacing, shell, (1.5, 1.5, 1.5), 1.0)
+
+    def analytic_box(s):
+        s.initialize(shape_type=3, radius=1.0)
+
+    a = two_bodies(grid_box, 1.96)
+    b = two_bodies(analytic_box, 1.96)
+    ia, da = a.debug_contacts()
+    ib, db = b.debug_contacts()
+    assert len(ia) == len(ib) > 8
+    np.testing.assert_array_equal(ia, ib)
+    face = np.abs(np.abs(db[:, 0]) - 1.0) < 1e-9  # contacts whose normal is the face normal
+    assert face.sum() > 4
+    np.testing.assert_allclose(da[face, 9], db[face, 9], atol=1e-9)
+    np.testing.assert_allclose(da[face, 0:3], db[face, 0:3], atol=1e-6)
+
+
+def test_grid_sdf_off_grid_residual():
+    """A probe beyond the stored lattice reads the clamped sample plus its distance to the
+    clamp point, so a unit-sphere grid ending at 1.05 still measures a 0.08 gap exactly."""
+    sph = lambda p: np.linalg.norm(p, axis=1) - 1.0  # noqa: E731
+    g, origin, spacing = grid_of(sph, -1.05, 1.05, 43)
+    shell = np.array([[1.0, 0.0, 0.0], [-1.0, 0.0, 0.0]])
+
+    def grid_sphere(s):
+        s.set_sdf_shape(g, 43, 43, 43, origin, spacing, shell, (2.5, 2.5, 2.5), 1.0)
+
+    s = oracle_sim(2)
+    grid_sphere(s)
+    s.set_domain((-10, -10, -10), (10, 10, 10))
+    s.enable_periodicity(False, False, False)
+    s.set_positions(np.array([[0.0, 0.0, 0.0], [2.08, 0.0, 0.0]]))
+    s.set_solver_iterations(0, 0)
+    s.step(1e-9)
+    ids, d = s.debug_contacts()
+    k = [q for q, (a, b) in enumerate(ids.tolist()) if a == 0 and b == 1]
+    assert len(k) == 1
+    assert d[k[0], 9] == pytest.approx(0.08, abs=1e-12)
+
+
+def test_sdf_shape_errors():
+    s = oracle_sim(1)
+    with pytest.raises(RuntimeError):
+        s.set_sdf_shape(np.zeros(8), 2, 2, 1, (0, 0, 0), (1, 1, 1), np.zeros((1, 3)), (1, 1, 1), 1.0)
+    with pytest.raises(RuntimeError):
+        s.set_sdf_shape(np.zeros(8), 2, 2, 2, (0, 0, 0), (1, 1, 1), np.zeros((0, 3)), (1, 1, 1), 1.0)
