@@ -2138,11 +2138,11 @@ static void fillRad(Sim* S, int n) {
 
 static void detectWallSdf(Sim* S, real margin);
 static real wallFrictionMax(const Sim* S);
+static void orderedStabilization(Sim* S, V3 gHat, real vRestS);
 
 static int demStep(Sim* S) {
   memset(&S->stats, 0, sizeof S->stats);
-  if (S->stabMode == 2 || S->stabMode == 4)
-    return fail("stabilization modes 'multilevel' and 'ordered' are not implemented");
+  if (S->stabMode == 2) return fail("stabilization mode 'multilevel' is not implemented");
   /* growth ramp, src/sim.hpp:100-108 */
   if (S->growthFactor != -1.0 && S->growthRate != 0.0) {
     S->growthFactor *= exp(S->growthRate * S->dt);
@@ -2248,6 +2248,8 @@ static int demStep(Sim* S) {
           S->stats.sweeps_stab++;
           if (S->maxApproach <= vRestS) break;
         }
+      } else if (smode == 4) {
+        orderedStabilization(S, gHat, vRestS);
       }
     }
   }
@@ -3108,4 +3110,84 @@ int pdref_set_sdf_shape(void* s, const double* grid, int nx, int ny, int nz, dou
   S->baseRadius = boundingRadius;
   S->defaultInvI = mk3(iix, iiy, iiz);
   return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* stabilization mode 4 ("ordered"), src/sim.hpp:356-397: fresh height-from-floor levels, the  */
+/* coloured manifolds bucketed by (level, colour), then bottom-up + top-down PGS passes         */
+/* ------------------------------------------------------------------------------------------ */
+#define LEVEL_INF (1 << 28)
+/* src/contact_preprocessing.hpp:255-297: the fixed point of the min-propagation is the exact
+ * support distance, whatever order the relaxation visits the manifolds in */
+static void computeHeightLevels(const Sim* S, V3 gHat, int* h) {
+  for (int i = 0; i < S->numReal; ++i) h[i] = LEVEL_INF;
+  for (int s = 0; s < 1024; ++s) {
+    int changed = 0;
+    for (int idx = 0; idx < S->nm; ++idx) {
+      const Manifold* m = &S->manifolds[idx];
+      if (m->numPoints <= 0) continue;
+      const int realA = S->realIdx[m->bodyA];
+      if (m->bodyB < 0) {
+        if (h[realA] > 0) { h[realA] = 0; changed++; }
+        continue;
+      }
+      const int realB = S->realIdx[m->bodyB];
+      const V3 dx = sub3(S->posPred[m->bodyA], S->posPred[m->bodyB]);
+      const real up = -(dx.x * gHat.x + dx.y * gHat.y + dx.z * gHat.z);
+      const real thr = 0.3 * sqrt(dot3(dx, dx));
+      if (up > thr) {
+        const int cand = h[realB];
+        if (cand < LEVEL_INF && h[realA] > cand + 1) { h[realA] = cand + 1; changed++; }
+      } else if (up < -thr) {
+        const int cand = h[realA];
+        if (cand < LEVEL_INF && h[realB] > cand + 1) { h[realB] = cand + 1; changed++; }
+      }
+    }
+    if (!changed) break;
+  }
+}
+typedef struct { int key, idx; } LvlKey;
+static int cmpLvl(const void* a, const void* b) {
+  const LvlKey* x = (const LvlKey*)a;
+  const LvlKey* y = (const LvlKey*)b;
+  if (x->key != y->key) return x->key < y->key ? -1 : 1;
+  return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+static void orderedStabilization(Sim* S, V3 gHat, real vRestS) {
+  int* h = (int*)xrealloc(NULL, sizeof(int) * (size_t)(S->numReal + 1));
+  computeHeightLevels(S, gHat, h);
+  /* src/solver_velocity.hpp:914-965: key = min level (clamped 1023) * 64 + colour */
+  LvlKey* v = (LvlKey*)xrealloc(NULL, sizeof(LvlKey) * (size_t)(S->nm + 1));
+  int n = 0;
+  for (int idx = 0; idx < S->nm; ++idx) {
+    const Manifold* m = &S->manifolds[idx];
+    const int c = S->mColor[idx];
+    if (m->numPoints <= 0 || c < 0) continue;
+    int lv = h[S->realIdx[m->bodyA]];
+    if (m->bodyB >= 0) {
+      const int hb = h[S->realIdx[m->bodyB]];
+      if (hb < lv) lv = hb;
+    }
+    if (lv > 1023) lv = 1023;
+    v[n].key = lv * 64 + c;
+    v[n].idx = idx;
+    n++;
+  }
+  qsort(v, (size_t)n, sizeof(LvlKey), cmpLvl);
+  /* buckets of equal key; within one the manifolds share a colour, so order is free */
+  for (int it = 0; it < 2 * S->velIters; ++it) {
+    S->maxApproach = 0.0;
+    for (int k = 0; k < n; ++k) pgsSolveOne(S, v[k].idx, vRestS);
+    int b = n;
+    while (b > 0) {  /* top-down: buckets in descending key order */
+      int a = b - 1;
+      while (a > 0 && v[a - 1].key == v[b - 1].key) --a;
+      for (int k = a; k < b; ++k) pgsSolveOne(S, v[k].idx, vRestS);
+      b = a;
+    }
+    S->stats.sweeps_stab++;
+    if (S->maxApproach <= vRestS) break;
+  }
+  free(v);
+  free(h);
 }

@@ -1688,6 +1688,110 @@ __global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_vsolve_block(
     }
   }
 }
+// ---- stabilization mode 4 ("ordered"), src/sim.hpp:356-397 ---------------------------------
+constexpr int LEVEL_INF = 1 << 28;
+// one relaxation sweep of the height-from-floor levels (src/contact_preprocessing.hpp:255-297);
+// repeated until nothing changes, it reaches the exact support distance in any visiting order
+__global__ __launch_bounds__(TPB) void k_height_sweep(Dev D, int numManifolds, d3 gHat,
+                                                      int* h, int* changed) {
+  __shared__ int sh[WAVES];
+  const int idx = gtid();
+  int ch = 0;
+  if (idx < numManifolds && D.mNp[idx] > 0) {
+    const int2 ab = D.mAB[idx];
+    const int2 r = D.mReal[idx];
+    if (ab.y < 0) {
+      ch = atomicMin(&h[r.x], 0) > 0;
+    } else {
+      const d3 dx = sub3(xyz(D.posPred[ab.x]), xyz(D.posPred[ab.y]));
+      const double up = -(dx.x * gHat.x + dx.y * gHat.y + dx.z * gHat.z);
+      const double thr = 0.3 * sqrt(dot3(dx, dx));
+      if (up > thr) {
+        const int cand = atomicAdd(&h[r.y], 0);
+        if (cand < LEVEL_INF) ch = atomicMin(&h[r.x], cand + 1) > cand + 1;
+      } else if (up < -thr) {
+        const int cand = atomicAdd(&h[r.x], 0);
+        if (cand < LEVEL_INF) ch = atomicMin(&h[r.y], cand + 1) > cand + 1;
+      }
+    }
+  }
+  blockAtomicAddI(changed, ch, sh);
+}
+// key = min level of the two bodies (clamped 1023) * 64 + colour, per velocity row
+__global__ void k_level_keys(Dev D, int numRows, const int* h, int* key, int* cnt) {
+  const int k = gtid();
+  if (k >= numRows) return;
+  const int2 r = D.vAB[k];
+  const int m = D.vPerm[k];
+  int lv = h[r.x];
+  if (r.y >= 0) {
+    const int hb = h[r.y];
+    if (hb < lv) lv = hb;
+  }
+  if (lv > 1023) lv = 1023;
+  const int kk = lv * 64 + D.mColor[m];
+  key[k] = kk;
+  atomicAdd(&cnt[kk], 1);
+}
+// rows placed bucket by bucket (order inside a bucket is free: one colour is body-disjoint)
+__global__ void k_level_place(int numRows, const int* key, const int* start, int* cursor,
+                              int* perm) {
+  const int k = gtid();
+  if (k >= numRows) return;
+  const int kk = key[k];
+  perm[start[kk] + atomicAdd(&cursor[kk], 1)] = k;
+}
+// bucket sweep over perm[b, e) (one colour), PGS
+__global__ __launch_bounds__(TPB) void k_vsweep_perm(Dev D, Params P, const int* perm, int b,
+                                                     int e, double thr, int serial) {
+  __shared__ unsigned long long sh[WAVES];
+  double a = 0.0;
+  if (serial) {
+    if (gtid() == 0)
+      for (int q = b; q < e; ++q) {
+        const double x = pgsRow(D, P, perm[q], thr);
+        a = x > a ? x : a;
+      }
+  } else {
+    const int q = b + gtid();
+    if (q < e) a = pgsRow(D, P, perm[q], thr);
+  }
+  blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
+}
+// the whole ordered loop in one workgroup: buckets ascending then descending per iteration;
+// bucket = (begin, end, colour) into perm
+__global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_ordered_block(
+    Dev D, Params P, const int* perm, const int4* buckets, int numBuckets, int iters,
+    double thr, int* count) {
+  __shared__ unsigned long long sh[BLOCK_SOLVER_THREADS / 64];
+  for (int it = 0; it < iters; ++it) {
+    unsigned long long m = 0;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = 0; q < numBuckets; ++q) {
+        const int4 bk = buckets[pass == 0 ? q : numBuckets - 1 - q];
+        if (bk.z == 62) {  // colour 62 may share bodies: one lane, row order
+          if (threadIdx.x == 0)
+            for (int k = bk.x; k < bk.y; ++k) {
+              const unsigned long long a = dbits(pgsRow(D, P, perm[k], thr));
+              m = a > m ? a : m;
+            }
+        } else {
+          for (int k = bk.x + (int)threadIdx.x; k < bk.y; k += blockDim.x) {
+            const unsigned long long a = dbits(pgsRow(D, P, perm[k], thr));
+            m = a > m ? a : m;
+          }
+        }
+        __syncthreads();
+      }
+    const unsigned long long M = blockMaxAll(m, sh);
+    if (threadIdx.x == 0) {
+      *D.maxApproachBits = M;
+      *count += 1;
+    }
+    if (bitsd(M) <= thr) return;
+  }
+}
+
 // commitPairKeysLambdaKokkos, src/contact_preprocessing.hpp:160-192: bucket by the low real index
 // (key >> 32), then sort each bucket by (key, manifold index) — a stable key sort
 __global__ void k_ledger_count(Dev D, int numManifolds, int numReal) {

@@ -262,7 +262,8 @@ struct pd_sim {
     hRefPos.release(); hXiWall.release(); hWF.release(); hWT.release(); hPrevKeys.release();
     hTab.release();
     jpWork[0].release(); jpWork[1].release(); jpCnt.release(); redPart.release();
-    blockCnt.release(); ctl.release();
+    blockCnt.release(); ctl.release(); hLevel.release(); lvlKey.release(); lvlCnt.release();
+    lvlStart.release(); lvlPerm.release(); lvlChanged.release(); lvlBuckets.release();
   }
 
   // ---------------------------------------------------------------- small helpers
@@ -793,6 +794,66 @@ struct pd_sim {
       batch *= 2;
     }
   }
+  // stabilization mode 4, src/sim.hpp:356-397: levels, (level, colour) buckets, ordered passes
+  DBuf<int> hLevel, lvlKey, lvlCnt, lvlStart, lvlPerm, lvlChanged;
+  DBuf<int4> lvlBuckets;
+  void orderedStabilization(d3 gHat, double vRestS) {
+    const int n = numReal, rows = nRowsV;
+    hLevel.ensure(n + 1, st);
+    lvlChanged.ensure(8, st);
+    k_fill_i32<<<nblk(n), TPB, 0, st>>>(hLevel.p, LEVEL_INF, n);
+    for (int s = 0; s < 1024; s += 8) {
+      zero(lvlChanged.p, 8);
+      for (int j = 0; j < 8; ++j)
+        k_height_sweep<<<nblk(nm), TPB, 0, st>>>(D, nm, gHat, hLevel.p, lvlChanged.p + j);
+      int ch[8];
+      readN(ch, lvlChanged.p, 8);
+      bool settled = false;
+      for (int j = 0; j < 8; ++j) settled |= (ch[j] == 0);
+      if (settled) break;
+    }
+    const int NK = 1024 * 64;
+    lvlKey.ensure(rows + 1, st);
+    lvlPerm.ensure(rows + 1, st);
+    lvlCnt.ensure(NK + 1, st);
+    lvlStart.ensure(NK + 1, st);
+    zero(lvlCnt.p, NK + 1);
+    if (rows > 0) k_level_keys<<<nblk(rows), TPB, 0, st>>>(D, rows, hLevel.p, lvlKey.p, lvlCnt.p);
+    scanQuiet(lvlCnt.p, lvlStart.p, NK);
+    zero(lvlCnt.p, NK + 1);
+    if (rows > 0)
+      k_level_place<<<nblk(rows), TPB, 0, st>>>(rows, lvlKey.p, lvlStart.p, lvlCnt.p, lvlPerm.p);
+    std::vector<int> starts(NK + 1);
+    readN(starts.data(), lvlStart.p, NK + 1);
+    std::vector<int4> bk;
+    for (int k = 0; k < NK; ++k)
+      if (starts[k + 1] > starts[k]) bk.push_back(make_int4(starts[k], starts[k + 1], k & 63, 0));
+    const int nb = (int)bk.size();
+    Params P = params(0.0, 0.0);
+    if (rows <= blockRows) {
+      lvlBuckets.ensure(nb + 1, st);
+      if (nb) writeN(lvlBuckets.p, bk.data(), nb);
+      zero(ctl.p + CTL_SWEEPS_S, 1);
+      k_ordered_block<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(D, P, lvlPerm.p, lvlBuckets.p, nb,
+                                                          2 * velIters, vRestS,
+                                                          ctl.p + CTL_SWEEPS_S);
+      stats.sweeps_stab = read1(ctl.p + CTL_SWEEPS_S);
+      maxApproach = readMaxApproach();
+      return;
+    }
+    for (int it = 0; it < 2 * velIters; ++it) {
+      zero(bits.p + 1, 1);
+      for (int pass = 0; pass < 2; ++pass)
+        for (int q = 0; q < nb; ++q) {
+          const int4 b = bk[pass == 0 ? q : nb - 1 - q];
+          k_vsweep_perm<<<nblk(b.y - b.x), TPB, 0, st>>>(D, P, lvlPerm.p, b.x, b.y, vRestS,
+                                                          b.z == 62 ? 1 : 0);
+        }
+      stats.sweeps_stab++;
+      maxApproach = readMaxApproach();
+      if (maxApproach <= vRestS) break;
+    }
+  }
   static ClassStarts classStarts(const std::vector<int>& s) {
     ClassStarts c{};
     for (int k = 0; k < 65 && k < (int)s.size(); ++k) c.s[k] = s[k];
@@ -826,8 +887,7 @@ struct pd_sim {
 
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
-    if (stabMode == 2 || stabMode == 4)
-      throw std::runtime_error("stabilization modes 'multilevel' and 'ordered' are not implemented");
+    if (stabMode == 2) throw std::runtime_error("stabilization mode 'multilevel' is not implemented");
     // a frictional wall drives friction even for frictionless bodies (src/sim.hpp:169-170)
     double wallFrictionMax = 0.0;
     for (const WallP& w : wallsHost) wallFrictionMax = std::max(wallFrictionMax, w.friction);
@@ -938,7 +998,8 @@ struct pd_sim {
     mark(ST_VSWEEP);
     const double vRest = 2.0 * dt * std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
     const double vRestS = 2.0 * dt * gMagP;
-    const bool hostLoop = velLeftover > 0 || (usePGS && stabMode == 3) || !useGS || legacyFriction;
+    const bool hostLoop = velLeftover > 0 || (usePGS && (stabMode == 3 || stabMode == 4)) || !useGS ||
+                          legacyFriction;
     if (!hostLoop) {
       // device-controlled adaptive stops: no host round trip per sweep
       zero(ctl.p, CTL_COUNT);
@@ -1024,13 +1085,17 @@ struct pd_sim {
           if (stabMode == 1 && nRowsV > 0)
             k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
                                                        8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
-          const int cap = stabMode == 1 ? 2 * velIters : 256;
-          for (int it = 0; it < cap; ++it) {
-            zero(bits.p + 1, 1);
-            velocitySweep(vs, numColors, 1, vRestS, ctl.p + CTL_NONE);
-            stats.sweeps_stab++;
-            maxApproach = readMaxApproach();
-            if (maxApproach <= vRestS) break;
+          if (stabMode == 4) {
+            orderedStabilization(gHat, vRestS);
+          } else {
+            const int cap = stabMode == 1 ? 2 * velIters : 256;
+            for (int it = 0; it < cap; ++it) {
+              zero(bits.p + 1, 1);
+              velocitySweep(vs, numColors, 1, vRestS, ctl.p + CTL_NONE);
+              stats.sweeps_stab++;
+              maxApproach = readMaxApproach();
+              if (maxApproach <= vRestS) break;
+            }
           }
         }
       }

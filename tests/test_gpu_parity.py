@@ -149,6 +149,12 @@ def test_escalate_stabilization(hip_lib):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("block_rows", [None, 0])
+def test_ordered_stabilization(hip_lib, block_rows):
+    h, r = run_pair("bed", 200, 6, stab="ordered", block_rows=block_rows)
+
+
+@pytest.mark.gpu
 def test_pair_materials(hip_lib):
     n = 200
     ids = (np.arange(n) % 3).astype(np.int32)
