@@ -594,3 +594,23 @@ def test_sdf_shape_errors():
         s.set_sdf_shape(np.zeros(8), 2, 2, 1, (0, 0, 0), (1, 1, 1), np.zeros((1, 3)), (1, 1, 1), 1.0)
     with pytest.raises(RuntimeError):
         s.set_sdf_shape(np.zeros(8), 2, 2, 2, (0, 0, 0), (1, 1, 1), np.zeros((0, 3)), (1, 1, 1), 1.0)
+
+
+def test_legacy_friction_oblique_binary_collision():
+    """Zero gravity with friction: count-averaged Coulomb friction after the GS restitution.
+    Equal spheres (m = 1, r = 0.5, I^-1 = 10) meet head-on with approach 2 and tangential slip
+    2: the friction bound is mu times the impulse that stops the approach (2 / w_n, w_n = 2),
+    and since that is below the stick impulse 2 / 7 the contact slides: each body's tangential
+    velocity changes by mu * 1 and its spin by r * mu * I^-1."""
+    mu, e = 0.1, 0.5
+    s = two_spheres(-1e-6, e=e, mu=mu, v=((1, 1, 0), (-1, -1, 0)), iters=(1, 4))
+    s.set_inv_inertia(np.full((2, 3), 10.0))
+    s.set_inv_mass(np.ones(2))
+    s.step(1e-6)
+    v = s.get_velocities()
+    w = s.get_angular_velocities()
+    # (the predicted positions tilt the normal by ~2e-6 during the 1e-6 s step)
+    np.testing.assert_allclose(v[:, 0], [-e, e], atol=2e-5)           # GS restitution
+    np.testing.assert_allclose(v[:, 1], [1 - mu, -1 + mu], atol=2e-5)  # sliding friction
+    np.testing.assert_allclose(w[:, 2], [-0.5 * mu * 10, -0.5 * mu * 10], atol=2e-5)
+    np.testing.assert_allclose(v.sum(0), [0, 0, 0], atol=1e-12)
