@@ -146,6 +146,12 @@ int pd_write_lammps_dump(const char* filename, int step, const double* pos, cons
 long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, int start,
                   const double* lo, const double* hi, double* out, int* placed);
 
+/* getSdfGrid(rx, ry, rz), src/sim.hpp:1538-1542 -> src/output_sdf.hpp: packed-bed SDF on an
+ * (rx,ry,rz) voxel grid over the domain, x-fastest, negative inside (out: rx*ry*rz doubles) */
+int pd_get_sdf_grid(pd_sim* s, int rx, int ry, int rz, double* out);
+/* exportSdf(filename, rx, ry, rz), src/sim.hpp:1478-1483: the same grid as an ASCII VTI file */
+int pd_export_sdf(pd_sim* s, const char* filename, int rx, int ry, int rz);
+
 /* --- SDF walls and the Hertz–Mindlin engine --------------------------------------------- */
 /* addSdfWall(grid, nx, ny, nz, origin, spacing, restitution, friction), src/sim.hpp:1242-1277:
  * flat x-fastest samples, positive in the void; returns the wall index (>= 0) or a status */

@@ -3191,3 +3191,106 @@ static void orderedStabilization(Sim* S, V3 gHat, real vRestS) {
   free(v);
   free(h);
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* packed-bed SDF reconstruction (get_sdf_grid / export_sdf), src/output_sdf.hpp:25-197:        */
+/* splat each particle's exact analytic distance over its bounding band (minimum), then a       */
+/* Jacobi fast-iterative Eikonal solve for the far field, periodic-wrapped; x-fastest output    */
+/* ------------------------------------------------------------------------------------------ */
+int pdref_get_sdf_grid(void* s, int rx, int ry, int rz, double* out) {
+  Sim* S = asSim(s);
+  if (rx < 1 || ry < 1 || rz < 1) return badarg("resolution must be >= 1 on each axis");
+  const long total = (long)rx * ry * rz;
+  const V3 o = S->dom.min;
+  const V3 vox = mk3((S->dom.max.x - S->dom.min.x) / rx, (S->dom.max.y - S->dom.min.y) / ry,
+                     (S->dom.max.z - S->dom.min.z) / rz);
+  real* g = (real*)xrealloc(NULL, sizeof(real) * (size_t)total);
+  real* g2 = (real*)xrealloc(NULL, sizeof(real) * (size_t)total);
+  unsigned char* fixedCell = (unsigned char*)calloc((size_t)total, 1);
+  for (long i = 0; i < total; ++i) g[i] = 100.0;
+  const int px = S->dom.px, py = S->dom.py, pz = S->dom.pz;
+  for (int i = 0; i < S->numReal; ++i) {
+    const V3 pw = S->pos[i];
+    const Q4 q = S->quat[i];
+    const real sc = S->scale[i];
+    real rbound = 1.0;
+    if (S->shapeType == SHAPE_HOLLOW_CYLINDER) {
+      const real r = S->shapeParams[0], h = S->shapeParams[1];
+      rbound = sqrt(r * r + (h * 0.5) * (h * 0.5));
+    } else if (S->shapeType == SHAPE_SPHERE || S->shapeType == SHAPE_GRID_SDF) {
+      rbound = S->shapeParams[0];
+    }
+    rbound *= sc;
+    rbound *= 1.2;
+    int minx = (int)floor((pw.x - rbound - o.x) / vox.x), maxx = (int)ceil((pw.x + rbound - o.x) / vox.x);
+    int miny = (int)floor((pw.y - rbound - o.y) / vox.y), maxy = (int)ceil((pw.y + rbound - o.y) / vox.y);
+    int minz = (int)floor((pw.z - rbound - o.z) / vox.z), maxz = (int)ceil((pw.z + rbound - o.z) / vox.z);
+    if (!px) { if (minx < 0) minx = 0; if (maxx > rx - 1) maxx = rx - 1; }
+    if (!py) { if (miny < 0) miny = 0; if (maxy > ry - 1) maxy = ry - 1; }
+    if (!pz) { if (minz < 0) minz = 0; if (maxz > rz - 1) maxz = rz - 1; }
+    for (int z = minz; z <= maxz; ++z)
+      for (int y = miny; y <= maxy; ++y)
+        for (int x = minx; x <= maxx; ++x) {
+          const V3 vp = mk3(o.x + (x + 0.5) * vox.x, o.y + (y + 0.5) * vox.y, o.z + (z + 0.5) * vox.z);
+          const V3 plocal = scale3(invRotate(q, sub3(vp, pw)), 1.0 / sc);
+          real distc = 100.0;
+          if (S->shapeType == SHAPE_HOLLOW_CYLINDER || S->shapeType == SHAPE_SPHERE ||
+              S->shapeType == SHAPE_GRID_SDF)
+            distc = sdfShape(S, plocal);
+          const real dist = distc * sc;
+          const int wx = (x % rx + rx) % rx, wy = (y % ry + ry) % ry, wz = (z % rz + rz) % rz;
+          const long vi = (long)wz * rx * ry + (long)wy * rx + wx;
+          if (dist < g[vi]) g[vi] = dist;
+          if (dist < rbound) fixedCell[vi] = 1;
+        }
+  }
+  int iters = rx > ry ? rx : ry;
+  if (rz > iters) iters = rz;
+  iters *= 4;
+  const real h = vox.x;
+  real* in = g;
+  real* o2 = g2;
+  const long sy = rx, sz = (long)rx * ry;
+  for (int it = 0; it < iters; ++it) {
+    for (long idx = 0; idx < total; ++idx) {
+      if (fixedCell[idx]) { o2[idx] = in[idx]; continue; }
+      const int z = (int)(idx / sz);
+      const long rem = idx % sz;
+      const int y = (int)(rem / sy), x = (int)(rem % sy);
+      real xm = 100.0, xp = 100.0, ym = 100.0, yp = 100.0, zm = 100.0, zp = 100.0;
+      if (x > 0) xm = in[idx - 1]; else if (px) xm = in[idx - 1 + rx];
+      if (x < rx - 1) xp = in[idx + 1]; else if (px) xp = in[idx + 1 - rx];
+      if (y > 0) ym = in[idx - sy]; else if (py) ym = in[(long)z * sz + (long)(ry - 1) * sy + x];
+      if (y < ry - 1) yp = in[idx + sy]; else if (py) yp = in[(long)z * sz + x];
+      if (z > 0) zm = in[idx - sz]; else if (pz) zm = in[(long)(rz - 1) * sz + (long)y * sy + x];
+      if (z < rz - 1) zp = in[idx + sz]; else if (pz) zp = in[(long)y * sy + x];
+      real a = fmin(xm, xp), b = fmin(ym, yp), c = fmin(zm, zp), t;
+      if (a > b) { t = a; a = b; b = t; }
+      if (b > c) { t = b; b = c; c = t; }
+      if (a > b) { t = a; a = b; b = t; }
+      real u = 100.0;
+      const real u1 = a + h;
+      if (u1 <= b) {
+        u = u1;
+      } else {
+        const real d2 = 2.0 * h * h - (a - b) * (a - b);
+        const real u2 = (d2 < 0.0) ? 100.0 : (a + b + sqrt(d2)) * 0.5;
+        if (u2 <= c) {
+          u = u2;
+        } else {
+          const real s3 = a + b + c, qd = s3 * s3 - 3.0 * (a * a + b * b + c * c - h * h);
+          u = (qd < 0.0) ? 100.0 : (s3 + sqrt(qd)) / 3.0;
+        }
+      }
+      o2[idx] = u;
+    }
+    real* tmp = in;
+    in = o2;
+    o2 = tmp;
+  }
+  if (out) memcpy(out, in, sizeof(real) * (size_t)total);
+  free(g);
+  free(g2);
+  free(fixedCell);
+  return 0;
+}

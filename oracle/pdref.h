@@ -92,6 +92,8 @@ int pdref_set_sdf_shape(void* s, const double* grid, int nx, int ny, int nz, dou
                         double oy, double oz, double sx, double sy, double sz,
                         const double* shell, int nPts, double iix, double iiy, double iiz,
                         double boundingRadius);
+/* packed-bed SDF on an (rx, ry, rz) voxel grid over the domain, x-fastest (src/output_sdf.hpp) */
+int pdref_get_sdf_grid(void* s, int rx, int ry, int rz, double* out);
 /* SDF walls and the Hertz–Mindlin engine (src/sim.hpp:1153-1287, :628-750) */
 int pdref_add_sdf_wall(void* s, const double* grid, int nx, int ny, int nz, double ox, double oy,
                        double oz, double sx, double sy, double sz, double restitution,

@@ -2356,4 +2356,100 @@ __global__ void k_fr_apply(Dev D, int n) {
   D.dAng[i] = make_double4(0, 0, 0, 0);
 }
 
+// ============================================================================================
+// packed-bed SDF reconstruction, src/output_sdf.hpp:25-197: splat (minimum over particles of the
+// exact analytic distance in each particle's band; ordered keys make the minimum order-free),
+// then Jacobi fast-iterative Eikonal sweeps, periodic-wrapped
+// ============================================================================================
+struct SdfGridP {
+  int rx, ry, rz, px, py, pz;
+  d3 origin, vox;
+};
+__global__ void k_sdf_splat(Dev D, Params P, SdfGridP G, unsigned long long* key,
+                            unsigned char* fixedCell) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  const d3 pw = xyz(D.pos[i]);
+  const q4 q = toq(D.bc[i].q);
+  const double sc = D.scale[i];
+  double rbound = 1.0;
+  if (P.shape.type == SHAPE_HOLLOW_CYLINDER) {
+    const double r = P.shape.p0, h = P.shape.p1;
+    rbound = sqrt(r * r + (h * 0.5) * (h * 0.5));
+  } else if (P.shape.type == SHAPE_SPHERE || P.shape.type == SHAPE_GRID_SDF) {
+    rbound = P.shape.p0;
+  }
+  rbound *= sc;
+  rbound *= 1.2;
+  int minx = (int)floor((pw.x - rbound - G.origin.x) / G.vox.x);
+  int maxx = (int)ceil((pw.x + rbound - G.origin.x) / G.vox.x);
+  int miny = (int)floor((pw.y - rbound - G.origin.y) / G.vox.y);
+  int maxy = (int)ceil((pw.y + rbound - G.origin.y) / G.vox.y);
+  int minz = (int)floor((pw.z - rbound - G.origin.z) / G.vox.z);
+  int maxz = (int)ceil((pw.z + rbound - G.origin.z) / G.vox.z);
+  if (!G.px) { minx = minx < 0 ? 0 : minx; maxx = maxx > G.rx - 1 ? G.rx - 1 : maxx; }
+  if (!G.py) { miny = miny < 0 ? 0 : miny; maxy = maxy > G.ry - 1 ? G.ry - 1 : maxy; }
+  if (!G.pz) { minz = minz < 0 ? 0 : minz; maxz = maxz > G.rz - 1 ? G.rz - 1 : maxz; }
+  const bool analytic = P.shape.type == SHAPE_HOLLOW_CYLINDER || P.shape.type == SHAPE_SPHERE ||
+                        P.shape.type == SHAPE_GRID_SDF;
+  for (int z = minz; z <= maxz; ++z)
+    for (int y = miny; y <= maxy; ++y)
+      for (int x = minx; x <= maxx; ++x) {
+        const d3 vp{G.origin.x + (x + 0.5) * G.vox.x, G.origin.y + (y + 0.5) * G.vox.y,
+                    G.origin.z + (z + 0.5) * G.vox.z};
+        const d3 plocal = scale3(invRotate(q, sub3(vp, pw)), 1.0 / sc);
+        const double distc = analytic ? sdfShape(P.shape, plocal) : 100.0;
+        const double dist = distc * sc;
+        const int wx = (x % G.rx + G.rx) % G.rx, wy = (y % G.ry + G.ry) % G.ry;
+        const int wz = (z % G.rz + G.rz) % G.rz;
+        const long vi = (long)wz * G.rx * G.ry + (long)wy * G.rx + wx;
+        atomicMin(&key[vi], ordKey(dist));
+        if (dist < rbound) fixedCell[vi] = 1;
+      }
+}
+__global__ void k_sdf_from_keys(const unsigned long long* key, double* g, long total) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) g[i] = ordVal(key[i]);
+}
+__global__ void k_sdf_eikonal(SdfGridP G, const double* din, double* dout,
+                              const unsigned char* fixedCell, double h) {
+  const long total = (long)G.rx * G.ry * G.rz;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  if (fixedCell[idx]) {
+    dout[idx] = din[idx];
+    return;
+  }
+  const long sy = G.rx, sz = (long)G.rx * G.ry;
+  const int z = (int)(idx / sz);
+  const long rem = idx % sz;
+  const int y = (int)(rem / sy), x = (int)(rem % sy);
+  double xm = 100.0, xp = 100.0, ym = 100.0, yp = 100.0, zm = 100.0, zp = 100.0;
+  if (x > 0) xm = din[idx - 1]; else if (G.px) xm = din[idx - 1 + G.rx];
+  if (x < G.rx - 1) xp = din[idx + 1]; else if (G.px) xp = din[idx + 1 - G.rx];
+  if (y > 0) ym = din[idx - sy]; else if (G.py) ym = din[(long)z * sz + (long)(G.ry - 1) * sy + x];
+  if (y < G.ry - 1) yp = din[idx + sy]; else if (G.py) yp = din[(long)z * sz + x];
+  if (z > 0) zm = din[idx - sz]; else if (G.pz) zm = din[(long)(G.rz - 1) * sz + (long)y * sy + x];
+  if (z < G.rz - 1) zp = din[idx + sz]; else if (G.pz) zp = din[(long)y * sy + x];
+  double a = fmin(xm, xp), b = fmin(ym, yp), c = fmin(zm, zp), t;
+  if (a > b) { t = a; a = b; b = t; }
+  if (b > c) { t = b; b = c; c = t; }
+  if (a > b) { t = a; a = b; b = t; }
+  double u = 100.0;
+  const double u1 = a + h;
+  if (u1 <= b) {
+    u = u1;
+  } else {
+    const double d2 = 2.0 * h * h - (a - b) * (a - b);
+    const double u2 = (d2 < 0.0) ? 100.0 : (a + b + sqrt(d2)) * 0.5;
+    if (u2 <= c) {
+      u = u2;
+    } else {
+      const double s3 = a + b + c, qd = s3 * s3 - 3.0 * (a * a + b * b + c * c - h * h);
+      u = (qd < 0.0) ? 100.0 : (s3 + sqrt(qd)) / 3.0;
+    }
+  }
+  dout[idx] = u;
+}
+
 }  // namespace pd

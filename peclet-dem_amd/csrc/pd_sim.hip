@@ -1831,6 +1831,86 @@ int pd_set_sdf_shape(pd_sim* s, const double* grid, int nx, int ny, int nz, doub
     s->defaultInvI[2] = iiz;
   });
 }
+// getSdfGrid / exportSdf, src/sim.hpp:1478-1483,1538-1542 (x-fastest, over the domain box)
+static void sdfGrid(pd_sim* s, int rx, int ry, int rz, std::vector<double>& out) {
+  if (rx < 1 || ry < 1 || rz < 1) throw ArgError("resolution must be >= 1 on each axis");
+  const long total = (long)rx * ry * rz;
+  SdfGridP G{};
+  G.rx = rx; G.ry = ry; G.rz = rz;
+  G.px = s->per[0]; G.py = s->per[1]; G.pz = s->per[2];
+  G.origin = d3{s->dmin[0], s->dmin[1], s->dmin[2]};
+  G.vox = d3{(s->dmax[0] - s->dmin[0]) / rx, (s->dmax[1] - s->dmin[1]) / ry,
+             (s->dmax[2] - s->dmin[2]) / rz};
+  unsigned long long* key = nullptr;
+  double *g1 = nullptr, *g2 = nullptr;
+  unsigned char* fixedCell = nullptr;
+  HIPCHECK(hipMalloc(&key, total * sizeof(unsigned long long)));
+  HIPCHECK(hipMalloc(&g1, total * sizeof(double)));
+  HIPCHECK(hipMalloc(&g2, total * sizeof(double)));
+  HIPCHECK(hipMalloc(&fixedCell, total));
+  try {
+    const hipStream_t st = s->st;
+    const int nb = (int)((total + TPB - 1) / TPB);
+    k_fill_u64<<<nb, TPB, 0, st>>>(key, ordKey(100.0), total);
+    HIPCHECK(hipMemsetAsync(fixedCell, 0, total, st));
+    s->refreshDev();
+    Params P = s->params(0.0, 0.0);
+    if (s->numReal > 0) k_sdf_splat<<<nblk(s->numReal), TPB, 0, st>>>(s->D, P, G, key, fixedCell);
+    k_sdf_from_keys<<<nb, TPB, 0, st>>>(key, g1, total);
+    const int iters = std::max(std::max(rx, ry), rz) * 4;
+    double* in = g1;
+    double* o2 = g2;
+    for (int it = 0; it < iters; ++it) {
+      k_sdf_eikonal<<<nb, TPB, 0, st>>>(G, in, o2, fixedCell, G.vox.x);
+      std::swap(in, o2);
+    }
+    HIPCHECK(hipGetLastError());
+    out.resize(total);
+    s->readN(out.data(), in, total);
+  } catch (...) {
+    (void)hipFree(key); (void)hipFree(g1); (void)hipFree(g2); (void)hipFree(fixedCell);
+    throw;
+  }
+  HIPCHECK(hipFree(key));
+  HIPCHECK(hipFree(g1));
+  HIPCHECK(hipFree(g2));
+  HIPCHECK(hipFree(fixedCell));
+}
+int pd_get_sdf_grid(pd_sim* s, int rx, int ry, int rz, double* out) {
+  return guard([&] {
+    std::vector<double> g;
+    sdfGrid(s, rx, ry, rz, g);
+    if (out) std::memcpy(out, g.data(), g.size() * sizeof(double));
+  });
+}
+// writeSdfVti, src/io.hpp:86-110 (ASCII ImageData, Float32 samples)
+int pd_export_sdf(pd_sim* s, const char* filename, int rx, int ry, int rz) {
+  return guard([&] {
+    std::vector<double> g;
+    sdfGrid(s, rx, ry, rz, g);
+    const float mn[3] = {(float)s->dmin[0], (float)s->dmin[1], (float)s->dmin[2]};
+    const float mx[3] = {(float)s->dmax[0], (float)s->dmax[1], (float)s->dmax[2]};
+    const double sx = rx > 1 ? (double(mx[0]) - mn[0]) / (rx - 1) : 1.0;
+    const double sy = ry > 1 ? (double(mx[1]) - mn[1]) / (ry - 1) : 1.0;
+    const double sz = rz > 1 ? (double(mx[2]) - mn[2]) / (rz - 1) : 1.0;
+    std::ofstream f(filename);
+    if (!f) throw std::runtime_error(std::string("Could not open file for writing: ") + filename);
+    f << "<?xml version=\"1.0\"?>\n";
+    f << "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+    f << "  <ImageData WholeExtent=\"0 " << (rx - 1) << " 0 " << (ry - 1) << " 0 " << (rz - 1)
+      << "\" Origin=\"" << mn[0] << " " << mn[1] << " " << mn[2] << "\" Spacing=\"" << sx << " "
+      << sy << " " << sz << "\">\n";
+    f << "    <Piece Extent=\"0 " << (rx - 1) << " 0 " << (ry - 1) << " 0 " << (rz - 1) << "\">\n";
+    f << "      <PointData Scalars=\"sdf\">\n";
+    f << "        <DataArray type=\"Float32\" Name=\"sdf\" format=\"ascii\">\n";
+    for (double v : g) f << (float)v << " ";
+    f << "\n        </DataArray>\n";
+    f << "      </PointData>\n";
+    f << "    </Piece>\n";
+    f << "  </ImageData>\n";
+    f << "</VTKFile>\n";
+  });
+}
 int pd_set_hertz_material(pd_sim* s, int mat, double youngs, double poisson) {
   return guard([&] {
     if (mat < 0 || mat >= 8) throw ArgError("material id out of range");

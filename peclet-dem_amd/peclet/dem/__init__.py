@@ -505,10 +505,17 @@ class Simulation:
         self._chk(self._b.write_vtp(self._h, os.fsencode(filename)))
 
     def get_sdf_grid(self, resolution):
-        raise NotImplementedError("SDF-field export is not implemented in this build")
+        """Packed-bed SDF on a (rx, ry, rz) voxel grid over the domain (negative inside).  As in
+        the reference binding, the x-fastest samples are returned as a C-ordered (rx, ry, rz)
+        array."""
+        rx, ry, rz = (int(v) for v in resolution)
+        out = np.empty(rx * ry * rz, np.float64)
+        self._chk(self._b.get_sdf_grid(self._h, rx, ry, rz, _ptr(out)))
+        return out.reshape(rx, ry, rz)
 
     def export_sdf(self, filename, resolution):
-        raise NotImplementedError("SDF-field export is not implemented in this build")
+        rx, ry, rz = (int(v) for v in resolution)
+        self._chk(self._b.export_sdf(self._h, os.fsencode(filename), rx, ry, rz))
 
 
 def export_lammps(filename, step, pos, vel, quats, radii, box_min=None, box_max=None,

@@ -103,13 +103,14 @@ SIGNATURES = {
     "get_ledger": (_i, [_p, _u64p, _dp, _dp, _dp, _i]),
     "get_grounded": (_i, [_p, _ip, _i]),
     "add_sdf_wall": (_i, [_p, _dp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d]),
-    "set_sdf_shape": (_i, [_p, _dp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _dp, _i, _d, _d, _d, _d]),
     "set_wall_velocity": (_i, [_p, _i, _d, _d, _d, _d, _d, _d, _d, _d, _d]),
     "set_wall_material_id": (_i, [_p, _i, _i]),
     "num_walls": (_i, [_p]),
     "set_hertz_material": (_i, [_p, _i, _d, _d]),
     "step_hertz": (_i, [_p, _d, _i, _d]),
     "get_hertz_pairs": (_i, [_p, _ip, _dp, _i]),
+    "set_sdf_shape": (_i, [_p, _dp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _dp, _i, _d, _d, _d, _d]),
+    "get_sdf_grid": (_i, [_p, _i, _i, _i, _dp]),
 }
 
 # Product-only entry points (no counterpart in a restatement library).
@@ -127,6 +128,7 @@ PRODUCT_SIGNATURES = {
     "get_kernel_times": (_i, [_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long), _i]),
     "kernel_name": (C.c_char_p, [_i]),
     "set_tuning": (_i, [_p, C.c_char_p, C.c_long]),
+    "export_sdf": (_i, [_p, C.c_char_p, _i, _i, _i]),
     "batch_create": (_p, [_i, _ip]),
     "batch_destroy": (None, [_p]),
     "batch_size": (_i, [_p]),

@@ -249,3 +249,25 @@ def test_frictional_wall_drives_legacy_friction(hip_lib):
 @pytest.mark.gpu
 def test_grid_sdf_boxes(hip_lib):
     run_pair("gridboxes", 27, 20)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,n", [("bed_periodic", 300), ("cylinders", 27), ("gridboxes", 27)])
+def test_sdf_grid_export(hip_lib, kind, n, tmp_path):
+    scene = make_scene(kind, n, 2)
+    h = apply_scene(hip_sim(n), scene)
+    r = apply_scene(oracle_sim(n), scene)
+    for _ in range(2):
+        h.step(scene["dt"])
+        r.step(scene["dt"])
+    res = (20, 18, 24)
+    a, b = h.get_sdf_grid(res), r.get_sdf_grid(res)
+    assert a.shape == res
+    assert np.array_equal(a, b)
+    f = tmp_path / "bed.vti"
+    h.export_sdf(str(f), res)
+    text = f.read_text()
+    vals = text.split('format="ascii">')[1].split("</DataArray>")[0].split()
+    assert len(vals) == res[0] * res[1] * res[2]
+    np.testing.assert_allclose(np.array(vals, float), b.ravel().astype(np.float32), rtol=1e-5,
+                               atol=1e-6)

@@ -614,3 +614,43 @@ def test_legacy_friction_oblique_binary_collision():
     np.testing.assert_allclose(v[:, 1], [1 - mu, -1 + mu], atol=2e-5)  # sliding friction
     np.testing.assert_allclose(w[:, 2], [-0.5 * mu * 10, -0.5 * mu * 10], atol=2e-5)
     np.testing.assert_allclose(v.sum(0), [0, 0, 0], atol=1e-12)
+
+
+# --------------------------------------------------------------------------- SDF export
+def test_sdf_grid_single_sphere_is_exact_distance():
+    """One sphere: inside its splat band every voxel holds the exact distance; the Eikonal
+    far field grows with slope <= 1 (a distance function) and stays close to the exact value."""
+    s = oracle_sim(1)
+    s.set_sphere_shape(1.0)
+    s.set_domain((0, 0, 0), (8, 8, 8))
+    s.enable_periodicity(False, False, False)
+    s.set_positions(np.array([[4.0, 4.0, 4.0]]))
+    n = 32
+    g = s.get_sdf_grid((n, n, n)).ravel()  # x-fastest samples
+    idx = np.arange(n ** 3)
+    x, y, z = idx % n, (idx // n) % n, idx // (n * n)
+    c = (np.stack([x, y, z], 1) + 0.5) * (8.0 / n)
+    exact = np.linalg.norm(c - 4.0, axis=1) - 1.0
+    # splatted cells (inside the particle's 1.2 r bounding box) nearer than 1.2 r stay fixed
+    lo, hi = np.floor((4.0 - 1.2) / (8.0 / n)), np.ceil((4.0 + 1.2) / (8.0 / n))
+    ijk = np.stack([x, y, z], 1)
+    band = (exact < 1.2 - 1e-9) & np.all((ijk >= lo) & (ijk <= hi), axis=1)
+    np.testing.assert_allclose(g[band], exact[band], atol=1e-12)
+    far = ~band
+    assert np.all(g[far] >= exact[far] - 1e-9)          # Eikonal never undershoots the distance
+    assert np.max(g[far] - exact[far]) < 0.35             # first-order scheme error
+    assert np.all(g[far] < 100.0)                         # everything reached
+
+
+def test_sdf_grid_periodic_wrap():
+    s = oracle_sim(1)
+    s.set_sphere_shape(1.0)
+    s.set_domain((0, 0, 0), (8, 8, 8))
+    s.enable_periodicity(True, False, False)
+    s.set_positions(np.array([[0.2, 4.0, 4.0]]))
+    n = 16
+    g = s.get_sdf_grid((n, n, n)).ravel()
+    # the voxel just left of x = 8 sees the sphere through the periodic boundary
+    x_hi = (n - 1) + n * (n // 2) + n * n * (n // 2)
+    c = np.array([(n - 0.5) * 0.5, (n // 2 + 0.5) * 0.5, (n // 2 + 0.5) * 0.5])
+    assert g[x_hi] == pytest.approx(np.linalg.norm(c - [8.2, 4.0, 4.0]) - 1.0, abs=1e-12)
