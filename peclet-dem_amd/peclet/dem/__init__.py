@@ -57,6 +57,16 @@ def __getattr__(name):
     raise AttributeError(name)
 
 
+class _DeviceArray:
+    """__cuda_array_interface__ holder for a strided FP64 device array owned by a Simulation."""
+
+    def __init__(self, ptr, shape, strides, owner):
+        self._owner = owner
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "strides": tuple(strides),
+                                         "typestr": "<f8", "data": (int(ptr), False),
+                                         "version": 2}
+
+
 class Batch:
     """Independent systems advanced together (no reference counterpart).
 
@@ -503,6 +513,34 @@ class Simulation:
 
     def write_vtp(self, filename):
         self._chk(self._b.write_vtp(self._h, os.fsencode(filename)))
+
+    # ------------------------------------------------------------------ zero-copy device views
+    # (src/dem_bindings.cpp:302-335).  torch tensors on the HIP device that alias the FP64 state
+    # (the reference's views are float32); they stay valid until the particle count grows.
+    def _view(self, name, cols, stride_doubles):
+        import torch
+
+        ptr = self._b.get_device_pointer(self._h, name)
+        if not ptr:
+            raise RuntimeError(self._b.error())
+        self._b.synchronize(self._h)
+        n = self.num_particles()
+        shape = (n, cols) if cols > 1 else (n,)
+        strides = (8 * stride_doubles, 8) if cols > 1 else (8 * stride_doubles,)
+        holder = _DeviceArray(ptr, shape, strides, self)
+        return torch.as_tensor(holder, device="cuda")
+
+    def get_positions_view(self):
+        return self._view(b"pos", 3, 4)
+
+    def get_velocities_view(self):
+        return self._view(b"vel", 3, 8)
+
+    def get_external_forces_view(self):
+        return self._view(b"ext_force", 3, 4)
+
+    def get_inv_mass_view(self):
+        return self._view(b"inv_mass", 1, 8)
 
     def get_sdf_grid(self, resolution):
         """Packed-bed SDF on a (rx, ry, rz) voxel grid over the domain (negative inside).  As in
