@@ -141,6 +141,9 @@ struct pd_sim {
   HzDev Z{};
   // sweep loops over at most this many rows run in one workgroup (k_vsolve_block/k_psolve_block)
   int blockRows = BLOCK_SOLVER_ROWS;
+  // the largest scale, cached while no scale is written (every writer clears scaleKnown)
+  bool scaleKnown = false;
+  unsigned long long maxScaleKey = 0;
   // timing: stage events (level >= 1) and per-launch events of the sweep kernels (level >= 2)
   int timing = 0;
   double stageMs[ST_COUNT] = {};
@@ -588,6 +591,7 @@ struct pd_sim {
     writeN(bc.p, B.data(), n);
     writeN(vw0.p, V.data(), n);
     writeN(scale.p, one.data(), n);
+    scaleKnown = false;
     writeN(targetScale.p, one.data(), n);
   }
 
@@ -733,7 +737,7 @@ struct pd_sim {
     int prevRemaining = -1;
     int cur = 0;
     const long maxRounds = (long)numBodies + 2;
-    int batch = 2;
+    int batch = 4;
     int hostCnt[17];
     while (remaining > 0 && rounds < maxRounds) {
       const int B = (int)std::min<long>(batch, maxRounds - rounds);
@@ -902,16 +906,22 @@ struct pd_sim {
       growthFactor *= std::exp(growthRate * dt);
       if (growthFactor > 1.0) growthFactor = 1.0;
     }
-    if (growthFactor > 0.0 && numReal > 0)
+    if (growthFactor > 0.0 && numReal > 0) {
       k_growth<<<nblk(numReal), TPB, 0, st>>>(scale.p, targetScale.p, growthFactor, numReal);
-    // maxOwnedRadius, src/sim.hpp:61-70
+      scaleKnown = false;
+    }
+    // maxOwnedRadius, src/sim.hpp:61-70 (read back only when a scale changed)
     double maxRad = globalScale * baseRadius;
     if (numReal > 0) {
-      const int nbs = std::min(nblk(numReal), RED_BLOCKS);
-      redPart.ensure(8 * RED_BLOCKS, st);
-      k_max_scale<<<nbs, TPB, 0, st>>>(scale.p, numReal, redPart.p);
-      k_max_final<<<1, TPB, 0, st>>>(redPart.p, nbs, red.p);
-      const unsigned long long k = read1(red.p);
+      if (!scaleKnown) {
+        const int nbs = std::min(nblk(numReal), RED_BLOCKS);
+        redPart.ensure(8 * RED_BLOCKS, st);
+        k_max_scale<<<nbs, TPB, 0, st>>>(scale.p, numReal, redPart.p);
+        k_max_final<<<1, TPB, 0, st>>>(redPart.p, nbs, red.p);
+        maxScaleKey = read1(red.p);
+        scaleKnown = true;
+      }
+      const unsigned long long k = maxScaleKey;
       const double mx = k ? ordVal(k) : -std::numeric_limits<double>::max();
       maxRad = mx * globalScale * baseRadius;
     }
@@ -1224,9 +1234,10 @@ struct pd_sim {
     {
       // one read-back of the loop counters and the residuals
       int c[CTL_COUNT];
-      readN(c, ctl.p, CTL_COUNT);
       unsigned long long b2[2];
-      readN(b2, bits.p, 2);
+      HIPCHECK(hipMemcpyAsync(c, ctl.p, sizeof c, hipMemcpyDeviceToHost, st));
+      HIPCHECK(hipMemcpyAsync(b2, bits.p, sizeof b2, hipMemcpyDeviceToHost, st));
+      sync();
       if (!hostLoop) {
         stats.sweeps_vel = c[CTL_SWEEPS_V];
         stats.sweeps_stab = c[CTL_SWEEPS_S];
@@ -1729,6 +1740,7 @@ int pd_set_scales(pd_sim* s, const double* sc, int n) {
     HIPCHECK(hipMemcpyAsync(s->scale.p, s->targetScale.p, s->scale.cap * sizeof(double),
                             hipMemcpyDeviceToDevice, s->st));
     s->sync();
+    s->scaleKnown = false;
   });
 }
 int pd_set_scales_uniform(pd_sim* s, double v) {
@@ -1739,6 +1751,7 @@ int pd_set_scales_uniform(pd_sim* s, double v) {
     HIPCHECK(hipMemcpyAsync(s->targetScale.p, s->scale.p, s->scale.cap * sizeof(double),
                             hipMemcpyDeviceToDevice, s->st));
     s->sync();
+    s->scaleKnown = false;
   });
 }
 int pd_set_growth_params(pd_sim* s, double rate, double nf) {
@@ -1751,6 +1764,7 @@ int pd_set_growth_params(pd_sim* s, double rate, double nf) {
       k_growth<<<nblk(s->numReal), TPB, 0, s->st>>>(s->scale.p, s->targetScale.p,
                                                     s->growthFactor, s->numReal);
       s->sync();
+      s->scaleKnown = false;
     }
   });
 }
