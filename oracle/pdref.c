@@ -17,6 +17,13 @@
  *   - position-impulse reduction: contacts ascending within a manifold
  *   - sorted ledger: stable (key, manifold index)
  * Build with -ffp-contract=off so no multiply-add is fused.
+ *
+ * Pinning (tests/test_oracle.py, tests/test_oracle_hertz.py): reference test inputs, brute force,
+ * closed forms and independent algorithms.  Parity unpinned: the Jacobi leftover solves
+ * (solveVelocityJacobi / positionSolveJacobi with onlyUncoloured) beyond momentum conservation,
+ * the escalate and ordered stabilization loops beyond the PGS row they share, and the Hertz
+ * point-spring shape paths (hertzShapePairForces, the shape branch of hertzWallForces) beyond
+ * determinism.
  */
 #include "pdref.h"
 
