@@ -1095,25 +1095,31 @@ __device__ __forceinline__ unsigned long long realKey(int realA, int realB) {
 
 // per manifold: ledger key (markPersistentManifoldsKokkos semantics, duplicates keep their key)
 // and the support relation of updateGroundedLevelsKokkos, src/contact_preprocessing.hpp:78-253
-__global__ void k_mkeys(Dev D, int numManifolds, d3 gHat) {
+// keys, support relation (0 none, 1 B supports A, 2 A supports B, 3 boundary) and the compacted
+// list of support manifolds (the only ones the grounded-level sweeps visit)
+__global__ void k_mkeys(Dev D, int numManifolds, d3 gHat, int* supList, int* supCount) {
+  __shared__ int shCnt, shBase;
   const int m = gtid();
-  if (m >= numManifolds) return;
-  if (D.mNp[m] <= 0) {
-    D.mKey[m] = ~0ull;
-    D.mCode[m] = 0;
-    return;
+  unsigned char code = 0;
+  if (m < numManifolds) {
+    if (D.mNp[m] <= 0) {
+      D.mKey[m] = ~0ull;
+    } else {
+      const int2 ab = D.mAB[m];
+      const int2 r = D.mReal[m];
+      D.mKey[m] = realKey(r.x, r.y);
+      if (ab.y < 0) {
+        code = 3;
+      } else {
+        const d3 dx = sub3(xyz(D.posPred[ab.x]), xyz(D.posPred[ab.y]));
+        const double up = -(dx.x * gHat.x + dx.y * gHat.y + dx.z * gHat.z);
+        const double thr = 0.3 * sqrt(dot3(dx, dx));
+        code = (up > thr) ? 1 : ((up < -thr) ? 2 : 0);
+      }
+    }
+    D.mCode[m] = code;
   }
-  const int2 ab = D.mAB[m];
-  const int2 r = D.mReal[m];
-  D.mKey[m] = realKey(r.x, r.y);
-  if (ab.y < 0) {
-    D.mCode[m] = 3;
-    return;
-  }
-  const d3 dx = sub3(xyz(D.posPred[ab.x]), xyz(D.posPred[ab.y]));
-  const double up = -(dx.x * gHat.x + dx.y * gHat.y + dx.z * gHat.z);
-  const double thr = 0.3 * sqrt(dot3(dx, dx));
-  D.mCode[m] = (up > thr) ? 1 : ((up < -thr) ? 2 : 0);
+  blockAppend(code != 0, m, supList, supCount, &shCnt, &shBase);
 }
 
 // velocity rows: gather the manifold (colour-sorted), warm start, persistence, vn0 / vt0
@@ -1208,6 +1214,39 @@ __global__ void k_ground_decay(int* g, int n, int decay) {
 __global__ void k_copy_i32(int* dst, const int* src, int n) {
   const int i = gtid();
   if (i < n) dst[i] = src[i];
+}
+// Grounded sweeps over the support list only.  Levels never decrease across the 8 sweeps, so the
+// buffer written by sweep s (it holds sweep s-2's levels, which are <= the current ones) is
+// brought up to date for the listed bodies by a max with the levels read, then updated; bodies
+// outside the list keep their decayed level in the final buffer.  Same result as a full copy per
+// sweep (src/contact_preprocessing.hpp:213-253, double-buffered).
+__global__ void k_ground_seed(Dev D, const int* list, const int* count, const int* src, int* dst) {
+  const int n = *count;
+  for (int q = gtid(); q < n; q += gridDim.x * blockDim.x) {
+    const int2 r = D.mReal[list[q]];
+    dst[r.x] = src[r.x];
+    if (r.y >= 0) dst[r.y] = src[r.y];
+  }
+}
+__global__ void k_ground_sweep_list(Dev D, const int* list, const int* count, const int* gOld,
+                                    int* gNew) {
+  const int n = *count;
+  for (int q = gtid(); q < n; q += gridDim.x * blockDim.x) {
+    const int m = list[q];
+    const int2 r = D.mReal[m];
+    atomicMax(&gNew[r.x], gOld[r.x]);
+    if (r.y >= 0) atomicMax(&gNew[r.y], gOld[r.y]);
+    const unsigned char code = D.mCode[m];
+    if (code == 3) {
+      atomicMax(&gNew[r.x], 255);
+    } else if (code == 1) {
+      const int lvl = gOld[r.y] - 1;
+      if (lvl > 0) atomicMax(&gNew[r.x], lvl);
+    } else {
+      const int lvl = gOld[r.x] - 1;
+      if (lvl > 0) atomicMax(&gNew[r.y], lvl);
+    }
+  }
 }
 __global__ void k_ground_sweep(Dev D, int numManifolds, const int* gOld, int* gNew) {
   const int m = gtid();

@@ -206,6 +206,7 @@ struct pd_sim {
   DBuf<double> vVn0, vLam, pLam, lLam;
   DBuf<unsigned long long> mKey, winner, mask, lKey, uKey, red, bits, redPart;
   DBuf<int> blockCnt;
+  DBuf<int> supList, supCnt;  // support manifolds of the grounded-level sweeps
   DBuf<int> ctl;  // device loop control (CTL_* in pd_kernels.hpp)
   DBuf<int> lIdx, bucketCnt, bucketStart, counters, colCnt, colFill, colStart, scanTmp, ghostCnt,
       ghostOff;
@@ -265,7 +266,7 @@ struct pd_sim {
     hRefPos.release(); hXiWall.release(); hWF.release(); hWT.release(); hPrevKeys.release();
     hTab.release();
     jpWork[0].release(); jpWork[1].release(); jpCnt.release(); redPart.release();
-    blockCnt.release(); ctl.release(); hLevel.release(); lvlKey.release(); lvlCnt.release();
+    blockCnt.release(); ctl.release(); supList.release(); supCnt.release(); hLevel.release(); lvlKey.release(); lvlCnt.release();
     lvlStart.release(); lvlPerm.release(); lvlChanged.release(); lvlBuckets.release();
   }
 
@@ -982,7 +983,10 @@ struct pd_sim {
     vRB.ensure(R, st); vVt0.ensure(R, st); vLamT.ensure(R, st); vVn0.ensure(R, st);
     vLam.ensure(R, st); vFlag.ensure(R, st); vWV.ensure(R, st);
     refreshDev();
-    if (nm > 0) k_mkeys<<<nblk(nm), TPB, 0, st>>>(D, nm, gHat);
+    supList.ensure(nm + 1, st);
+    supCnt.ensure(1, st);
+    zero(supCnt.p, 1);
+    if (nm > 0) k_mkeys<<<nblk(nm), TPB, 0, st>>>(D, nm, gHat, supList.p, supCnt.p);
     if (nRowsV > 0)
       k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(D, P, nRowsV, prevCount > 0 ? ledgerBuckets : 0,
                                             usePGS ? 1 : 0);
@@ -991,10 +995,13 @@ struct pd_sim {
       k_ground_decay<<<nblk(numReal), TPB, 0, st>>>(ground.p, numReal, 8);
       int* ga = ground.p;
       int* gb = ground2.p;
-      for (int s = 0; s < 8; ++s) {
-        k_copy_i32<<<nblk(numReal), TPB, 0, st>>>(gb, ga, numReal);
-        if (nm > 0) k_ground_sweep<<<nblk(nm), TPB, 0, st>>>(D, nm, ga, gb);
-        std::swap(ga, gb);
+      if (nm > 0) {
+        const int gg = std::min(nblk(nm), 1024);
+        k_ground_seed<<<gg, TPB, 0, st>>>(D, supList.p, supCnt.p, ga, gb);
+        for (int s = 0; s < 8; ++s) {
+          k_ground_sweep_list<<<gg, TPB, 0, st>>>(D, supList.p, supCnt.p, ga, gb);
+          std::swap(ga, gb);
+        }
       }
       // 8 swaps: the result is back in `ground`
       if (nRowsV <= blockRows)
