@@ -809,6 +809,85 @@ __global__ __launch_bounds__(TPB) void k_np_pairs(Dev D, Params P, int numPairs)
   if (WRITE) blockAtomicMaxU64(D.maxOverlapBits, dbits(ovl), sh);
 }
 
+// Shell shapes: one wavefront per candidate pair.  The block stages the point shell in LDS; the
+// 64 lanes take points k = lane, lane + 64, ...; a ballot prefix gives every hit its place in
+// shell order, so the contacts (and their count) are the serial loop's.
+constexpr int NP_WAVES = TPB / 64;
+constexpr int NP_LDS_POINTS = 1024;  // larger shells read the shell from global memory
+template <bool WRITE>
+__global__ __launch_bounds__(TPB) void k_np_pairs_wave(Dev D, Params P, int numPairs) {
+  __shared__ double4 shShell[NP_LDS_POINTS];
+  __shared__ unsigned long long shMax[NP_WAVES];
+  const int countA = P.shape.numShell;
+  const bool inLds = countA <= NP_LDS_POINTS;
+  if (inLds)
+    for (int k = threadIdx.x; k < countA; k += blockDim.x) shShell[k] = D.shell[k];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int p = blockIdx.x * NP_WAVES + wave;
+  double ovl = 0.0;
+  if (p < numPairs) {
+    const int2 ab = D.pairs[p];
+    const int idA = ab.x, idB = ab.y;
+    const d3 posA = xyz(D.posPred[idA]), posB = xyz(D.posPred[idB]);
+    const q4 qA = toq(D.quatPred[idA]), qB = toq(D.quatPred[idB]);
+    const double effScaleA = D.scale[idA] * P.gs, effScaleB = D.scale[idB] * P.gs;
+    double eb = -1.0, mub = -1.0;
+    if (WRITE && P.hasPairTable) {
+      const int tt = ((int)D.matId[idA] * 8 + (int)D.matId[idB]) * 2;
+      eb = D.pairTable[tt];
+      mub = D.pairTable[tt + 1];
+    }
+    const int base = WRITE ? D.noff[p] : 0;
+    int n = 0;
+    for (int k0 = 0; k0 < countA; k0 += 64) {
+      const int k = k0 + lane;
+      bool hit = false;
+      d3 pWorld{0, 0, 0}, pCanB{0, 0, 0};
+      double effDist = 0.0;
+      if (k < countA) {
+        const d3 pLocalA = xyz(inLds ? shShell[k] : D.shell[k]);
+        pWorld = add3(posA, rotate(qA, scale3(pLocalA, effScaleA)));
+        const d3 pLocalB = invRotate(qB, sub3(pWorld, posB));
+        pCanB = scale3(pLocalB, 1.0 / effScaleB);
+        effDist = sdfShape(P.shape, pCanB) * effScaleB;  // point radius 0 for shell points
+        hit = effDist < P.margin;
+      }
+      const unsigned long long mask = __ballot(hit);
+      if (WRITE && hit) {
+        if (effDist < 0.0 && -effDist > ovl) ovl = -effDist;
+        const double eps = 1e-4;
+        d3 nLoc{sdfShape(P.shape, d3{pCanB.x + eps, pCanB.y, pCanB.z}) -
+                    sdfShape(P.shape, d3{pCanB.x - eps, pCanB.y, pCanB.z}),
+                sdfShape(P.shape, d3{pCanB.x, pCanB.y + eps, pCanB.z}) -
+                    sdfShape(P.shape, d3{pCanB.x, pCanB.y - eps, pCanB.z}),
+                sdfShape(P.shape, d3{pCanB.x, pCanB.y, pCanB.z + eps}) -
+                    sdfShape(P.shape, d3{pCanB.x, pCanB.y, pCanB.z - eps})};
+        const double len = len3(nLoc);
+        nLoc = (len > 1e-9) ? scale3(nLoc, 1.0 / len) : d3{0.0, 1.0, 0.0};
+        const d3 nWorld = rotate(qB, nLoc);
+        const d3 pSurfA = sub3(pWorld, scale3(nWorld, 0.0));
+        const d3 rA = sub3(pSurfA, posA);
+        const d3 rB = sub3(sub3(pSurfA, scale3(nWorld, effDist)), posB);
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        writeContact(D, base + n + rank, idA, idB, nWorld, rA, rB, effDist, eb, mub);
+      }
+      n += __popcll(mask);
+    }
+    if (!WRITE && lane == 0) D.ncnt[p] = n;
+  }
+  if (WRITE) {
+    unsigned long long m = waveMaxU64(dbits(ovl));
+    if (lane == 0) shMax[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long b = shMax[0];
+      for (int w = 1; w < NP_WAVES; ++w) b = shMax[w] > b ? shMax[w] : b;
+      if (b) atomicMax(D.maxOverlapBits, b);
+    }
+  }
+}
+
 template <bool WRITE>
 __device__ __forceinline__ double npPlanes(Dev& D, const Params& P, int numPairs, int i) {
   const d3 posA = xyz(D.posPred[i]);

@@ -679,7 +679,10 @@ struct pd_sim {
     ncnt.ensure(nslots + 1, st);
     noff.ensure(nslots + 1, st);
     refreshDev();
-    k_np_pairs<false><<<nblk(np), TPB, 0, st>>>(D, P, np);
+    const bool waveNp = P.shape.numShell > 0;  // shell shapes: one wavefront per pair
+    const int npBlocks = (int)std::max<long>(1, ((long)np + NP_WAVES - 1) / NP_WAVES);
+    if (waveNp) k_np_pairs_wave<false><<<npBlocks, TPB, 0, st>>>(D, P, np);
+    else k_np_pairs<false><<<nblk(np), TPB, 0, st>>>(D, P, np);
     k_np_planes<false><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
     k_np_walls<false><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
     nc = scan(ncnt.p, noff.p, nslots);
@@ -692,7 +695,8 @@ struct pd_sim {
     refreshDev();
     zero(bits.p, 1);
     kernBegin();
-    k_np_pairs<true><<<nblk(np), TPB, 0, st>>>(D, P, np);
+    if (waveNp) k_np_pairs_wave<true><<<npBlocks, TPB, 0, st>>>(D, P, np);
+    else k_np_pairs<true><<<nblk(np), TPB, 0, st>>>(D, P, np);
     kernEnd(K_NPPAIRS, np);
     k_np_planes<true><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
     if (W.n > 0) k_np_walls<true><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
