@@ -81,19 +81,28 @@ def kernel_bytes(name, units, st):
 
 
 def make_workload(kind, n, seed, column):
-    from peclet.dem.workloads import lattice_bed, sphere_bed
+    from peclet.dem.workloads import cylinder_bed, lattice_bed, sphere_bed
 
     if kind == "lattice":
         return lattice_bed(n, 0.16, seed)
+    if kind == "cylinders":
+        return cylinder_bed(n, 0.2, seed)
     return sphere_bed(n, 0.8e-3, 1.2e-3, (0.16, 0.16, 0.4), seed, column=column)
 
 
-def run_cpu_baseline(bed, dt, substeps):
+def setup(sim, kind, bed, dt):
+    from peclet.dem.workloads import setup_cylinder_bed, setup_gravity_bed
+
+    if kind == "cylinders":
+        return setup_cylinder_bed(sim, bed, dt, 0.4, 0.2)
+    return setup_gravity_bed(sim, bed, dt, 0.5, 0.3)
+
+
+def run_cpu_baseline(kind, bed, dt, substeps):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_binding import oracle_sim
-    from peclet.dem.workloads import setup_gravity_bed
 
-    sim = setup_gravity_bed(oracle_sim(len(bed["pos"])), bed, dt, 0.5, 0.3)
+    sim = setup(oracle_sim(len(bed["pos"])), kind, bed, dt)
     t0 = time.perf_counter()
     for _ in range(substeps):
         sim.step(dt)
@@ -180,7 +189,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000, help="timed substeps")
     ap.add_argument("--warmup", type=int, default=8, help="untimed substeps")
-    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--n", type=int, default=None,
+                    help="particles (default 1,048,576 spheres; 200,000 cylinders)")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--column", type=float, default=0.7, help="seeding column height (m)")
     ap.add_argument("--cpu-substeps", type=int, default=16)
@@ -191,10 +201,13 @@ def main():
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=24)
     ap.add_argument("--batch-workers", type=int, default=16)
-    ap.add_argument("--workload", choices=["pour", "lattice"], default="pour",
+    ap.add_argument("--workload", choices=["pour", "lattice", "cylinders"], default="pour",
                     help="pour: the 1M-sphere gravity pour (default); lattice: a contact-rich "
-                         "bed of spheres on a slightly overlapping cubic lattice")
+                         "bed of spheres on a slightly overlapping cubic lattice; cylinders: "
+                         "200,000 hollow cylinders with a 256-point shell")
     args = ap.parse_args()
+    if args.n is None:
+        args.n = 200000 if args.workload == "cylinders" else 1048576
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in the design notes, not implemented",
@@ -206,13 +219,12 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     from peclet.dem import Simulation
-    from peclet.dem.workloads import setup_gravity_bed
 
     dt = 5e-6
     t_gen = time.perf_counter()
     bed = make_workload(args.workload, args.n, args.seed, args.column)
     t_gen = time.perf_counter() - t_gen
-    sim = setup_gravity_bed(Simulation(args.n), bed, dt, 0.5, 0.3)
+    sim = setup(Simulation(args.n), args.workload, bed, dt)
     print(f"# workload ready: n={args.n} (generation {t_gen:.1f}s)", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         sim.step(dt)
@@ -266,7 +278,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cps, tc = run_cpu_baseline(bed, dt, args.cpu_substeps)
+        cps, tc = run_cpu_baseline(args.workload, bed, dt, args.cpu_substeps)
         cpu = {"value": cps, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
                "sample": f"oracle (serial C restatement) on the same seeded {args.n}-sphere input, "
                          f"first {args.cpu_substeps} substeps, {tc:.1f} s"}
@@ -292,7 +304,11 @@ def main():
                          if args.workload == "pour" else
                          f"lattice bed, {args.n} spheres r~U[0.99,1.01] mm on a 1.996 r cubic "
                          "lattice, 160x160 mm periodic xy + floor, dt=5e-6 s, mu=0.5, e=0.3, "
-                         "iters pos 12 / vel 8"),
+                         "iters pos 12 / vel 8"
+                         if args.workload == "lattice" else
+                         f"{args.n} hollow cylinders r_out 2 mm, r_in 1.2 mm, L 4 mm, 256-point "
+                         "seeded shell, random orientations, 200x200 mm periodic xy + floor, RSA "
+                         "start, dt=5e-6 s, mu=0.4, e=0.2, iters pos 12 / vel 8"),
             "form": "plain",
             "substeps_timed": args.steps,
         },

@@ -202,3 +202,76 @@ def ensemble(num, seed, n_lo=500, n_hi=20000, r_lo=0.8e-3, r_hi=1.2e-3, mu=(0.2,
         bed["e"] = m_e
         out.append(bed)
     return out
+
+
+def cylinder_shell(n, r_out, r_in, length, seed=3):
+    """``n`` surface points of a hollow cylinder (axis y, canonical frame) on a seeded lattice:
+    points are split over the outer wall, inner wall and the two annular caps in proportion to
+    their areas and laid on golden-angle lattices with a seeded phase."""
+    rng = np.random.default_rng(seed)
+    a_out = 2 * math.pi * r_out * length
+    a_in = 2 * math.pi * r_in * length
+    a_cap = math.pi * (r_out ** 2 - r_in ** 2)
+    areas = np.array([a_out, a_in, a_cap, a_cap])
+    counts = np.floor(n * areas / areas.sum()).astype(int)
+    counts[0] += n - counts.sum()
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    pts = []
+    for part, m in enumerate(counts):
+        phase = rng.uniform(0, 2 * math.pi)
+        k = np.arange(m) + 0.5
+        th = phase + golden * np.arange(m)
+        if part < 2:
+            r = r_out if part == 0 else r_in
+            y = -0.5 * length + length * k / m
+            pts.append(np.stack([r * np.cos(th), y, r * np.sin(th)], 1))
+        else:
+            r = np.sqrt(r_in ** 2 + (r_out ** 2 - r_in ** 2) * k / m)
+            y = np.full(m, 0.5 * length if part == 2 else -0.5 * length)
+            pts.append(np.stack([r * np.cos(th), y, r * np.sin(th)], 1))
+    return np.concatenate(pts, 0)
+
+
+def cylinder_bed(n, box_xy, seed, r_out=2e-3, r_in=1.2e-3, length=4e-3, shell_points=256,
+                 rho=2500.0, column=None):
+    """Hollow cylinders seeded (RSA on the bounding sphere) in a ``box_xy`` x ``box_xy`` column
+    with uniform random orientations; returns the bed dict plus ``shell`` and the shape tuple."""
+    rb = math.sqrt(r_out ** 2 + (0.5 * length) ** 2)
+    radii = np.full(n, rb)
+    col = column if column is not None else n * (4.0 / 3.0 * math.pi * rb ** 3) / (0.28 * box_xy ** 2)
+    pos, _ = rsa_positions(n, radii, (0.0, 0.0, 0.0), (box_xy, box_xy, col), seed, native=True)
+    rng = np.random.default_rng(seed + 1)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    vol = math.pi * (r_out ** 2 - r_in ** 2) * length
+    m = rho * vol
+    # principal moments of a hollow cylinder about its axis (y) and a diameter
+    i_axis = 0.5 * m * (r_out ** 2 + r_in ** 2)
+    i_diam = m * (3.0 * (r_out ** 2 + r_in ** 2) + length ** 2) / 12.0
+    return {
+        "pos": pos,
+        "quat": q,
+        "inv_mass": np.full(n, 1.0 / m),
+        "inv_inertia": np.tile([1.0 / i_diam, 1.0 / i_axis, 1.0 / i_diam], (n, 1)),
+        "shell": cylinder_shell(shell_points, r_out, r_in, length),
+        "shape": (2, r_out, length, r_out - r_in),
+        "domain": (float(box_xy), float(box_xy), float(col)),
+        "base_radius": r_out,
+    }
+
+
+def setup_cylinder_bed(sim, bed, dt, mu, e, iters=(12, 8)):
+    sim.initialize_shape(*bed["shape"])
+    sim.set_shell_points(bed["shell"])
+    lx, ly, lz = bed["domain"]
+    sim.set_domain(lx, ly, lz, True, True, False)
+    sim.add_plane(0.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+    sim.set_positions(bed["pos"])
+    sim.set_quaternions(bed["quat"])
+    sim.set_inv_mass(bed["inv_mass"])
+    sim.set_inv_inertia(bed["inv_inertia"])
+    sim.set_gravity(0.0, 0.0, -9.81)
+    sim.set_material_params(e, 0.0, mu)
+    sim.set_solver_iterations(*iters)
+    sim.set_dt(dt)
+    return sim
