@@ -13,6 +13,7 @@
 #ifndef PECLET_DEM_H
 #define PECLET_DEM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -111,6 +112,10 @@ int pd_get_scales(pd_sim* s, double* out, int n);             /* getScales, :145
  * src/dem_bindings.cpp:302-335.  Row strides in doubles: "pos" 4 (x, y, z, pad), "ext_force" 4,
  * "vel" 8 (the {v, w} record: v at 0, angular velocity at 4), "inv_mass" 8 (packed record). */
 void* pd_get_device_pointer(pd_sim* s, const char* name);
+/* Copies between caller host memory and device memory of this library's runtime, ordered on the
+ * handle's stream (for consumers of pd_get_device_pointer that run no HIP code of their own). */
+int pd_read_device(pd_sim* s, const void* src, void* host, size_t bytes);
+int pd_write_device(pd_sim* s, void* dst, const void* host, size_t bytes);
 
 /* --- the substep ------------------------------------------------------------------------ */
 int pd_step(pd_sim* s, double dt);        /* Simulation::step -> demStep, src/sim.hpp:1456, :97 */

@@ -1822,6 +1822,18 @@ void* pd_get_device_pointer(pd_sim* s, const char* name) {
   return nullptr;
 }
 
+int pd_read_device(pd_sim* s, const void* src, void* host, size_t bytes) {
+  return guard([&] {
+    HIPCHECK(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, s->st));
+    s->sync();
+  });
+}
+int pd_write_device(pd_sim* s, void* dst, const void* host, size_t bytes) {
+  return guard([&] {
+    HIPCHECK(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, s->st));
+    s->sync();
+  });
+}
 int pd_step(pd_sim* s, double dt) {
   return guard([&] {
     s->dt = dt;

@@ -58,13 +58,48 @@ def __getattr__(name):
 
 
 class _DeviceArray:
-    """__cuda_array_interface__ holder for a strided FP64 device array owned by a Simulation."""
+    """A strided FP64 device array owned by a Simulation.
+
+    It carries ``__cuda_array_interface__`` (pointer, shape, byte strides) for consumers that
+    share this library's HIP runtime, and copies to / from host NumPy arrays through the
+    library itself (``numpy()``, ``assign()``), ordered on the simulation's stream.
+    """
 
     def __init__(self, ptr, shape, strides, owner):
         self._owner = owner
-        self.__cuda_array_interface__ = {"shape": tuple(shape), "strides": tuple(strides),
-                                         "typestr": "<f8", "data": (int(ptr), False),
+        self.ptr = int(ptr)
+        self.shape = tuple(shape)
+        self.strides = tuple(strides)
+        self.__cuda_array_interface__ = {"shape": self.shape, "strides": self.strides,
+                                         "typestr": "<f8", "data": (self.ptr, False),
                                          "version": 2}
+
+    def _block(self):
+        n = self.shape[0]
+        row = self.strides[0] // 8
+        return n, row, np.empty(n * row, np.float64)
+
+    def numpy(self):
+        n, row, buf = self._block()
+        if n:
+            self._owner._chk(self._owner._b.read_device(
+                self._owner._h, C.c_void_p(self.ptr), buf.ctypes.data_as(C.c_void_p), buf.nbytes))
+        cols = self.shape[1] if len(self.shape) > 1 else 1
+        out = buf.reshape(n, row)[:, :cols]
+        return out.copy() if len(self.shape) > 1 else out[:, 0].copy()
+
+    def assign(self, values):
+        """Write ``values`` (same shape) into the aliased state."""
+        n, row, buf = self._block()
+        if not n:
+            return
+        self._owner._chk(self._owner._b.read_device(
+            self._owner._h, C.c_void_p(self.ptr), buf.ctypes.data_as(C.c_void_p), buf.nbytes))
+        cols = self.shape[1] if len(self.shape) > 1 else 1
+        blk = buf.reshape(n, row)
+        blk[:, :cols] = np.asarray(values, np.float64).reshape(n, cols)
+        self._owner._chk(self._owner._b.write_device(
+            self._owner._h, C.c_void_p(self.ptr), buf.ctypes.data_as(C.c_void_p), buf.nbytes))
 
 
 class Batch:
@@ -515,11 +550,9 @@ class Simulation:
         self._chk(self._b.write_vtp(self._h, os.fsencode(filename)))
 
     # ------------------------------------------------------------------ zero-copy device views
-    # (src/dem_bindings.cpp:302-335).  torch tensors on the HIP device that alias the FP64 state
-    # (the reference's views are float32); they stay valid until the particle count grows.
+    # (src/dem_bindings.cpp:302-335).  Strided FP64 device arrays that alias the state (the
+    # reference's are float32); they stay valid until the particle count grows.
     def _view(self, name, cols, stride_doubles):
-        import torch
-
         ptr = self._b.get_device_pointer(self._h, name)
         if not ptr:
             raise RuntimeError(self._b.error())
@@ -527,8 +560,7 @@ class Simulation:
         n = self.num_particles()
         shape = (n, cols) if cols > 1 else (n,)
         strides = (8 * stride_doubles, 8) if cols > 1 else (8 * stride_doubles,)
-        holder = _DeviceArray(ptr, shape, strides, self)
-        return torch.as_tensor(holder, device="cuda")
+        return _DeviceArray(ptr, shape, strides, self)
 
     def get_positions_view(self):
         return self._view(b"pos", 3, 4)

@@ -129,6 +129,8 @@ PRODUCT_SIGNATURES = {
     "kernel_name": (C.c_char_p, [_i]),
     "set_tuning": (_i, [_p, C.c_char_p, C.c_long]),
     "export_sdf": (_i, [_p, C.c_char_p, _i, _i, _i]),
+    "read_device": (_i, [_p, _p, _p, C.c_size_t]),
+    "write_device": (_i, [_p, _p, _p, C.c_size_t]),
     "batch_create": (_p, [_i, _ip]),
     "batch_destroy": (None, [_p]),
     "batch_size": (_i, [_p]),
