@@ -75,7 +75,6 @@ struct Dev {
   int* cellStart;
   int* order;
   double4* sbox;
-  float4* sboxF;  // the same boxes in FP32, widened so the FP32 test never rejects a true pair
   int* pairCnt;
   int* pairStart;
   int2* pairs;
@@ -555,21 +554,6 @@ __device__ __forceinline__ double4 saneBox(double4 pp, double margin, double box
   }
   return make_double4(x, y, z, b);
 }
-// FP32 copy of a box for the candidate scan: the half-width grows by 2^-20 of itself and of the
-// largest coordinate magnitude (16x the FP32 rounding of centres, difference and sum) and is
-// rounded up, so |xi - xj| <= bi + bj in FP64 implies the FP32 test passes too; the FP64 test
-// then decides
-__device__ __forceinline__ float4 widenedBoxF(const double4& B) {
-  const double m = fmax(fmax(fabs(B.x), fabs(B.y)), fabs(B.z));
-  const double w = B.w * (1.0 + 9.5367431640625e-07) + m * 9.5367431640625e-07 + 1e-30;
-  return make_float4((float)B.x, (float)B.y, (float)B.z, __double2float_ru(w));
-}
-__device__ __forceinline__ bool boxOverlapF(const float4& a, const float4& c) {
-  if (fabsf(a.x - c.x) > a.w + c.w) return false;
-  if (fabsf(a.y - c.y) > a.w + c.w) return false;
-  if (fabsf(a.z - c.z) > a.w + c.w) return false;
-  return true;
-}
 __device__ __forceinline__ bool boxOverlap(const double4& a, const double4& c) {
   if (a.x - a.w > c.x + c.w || a.x + a.w < c.x - c.w) return false;
   if (a.y - a.w > c.y + c.w || a.y + a.w < c.y - c.w) return false;
@@ -665,9 +649,7 @@ __global__ void k_cell_fill(Dev D, Params P) {
   const int c = D.cellOf[s];
   const int k = D.cellStart[c] + atomicAdd(&D.cellCnt[c], 1);
   D.order[k] = s;
-  const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
-  D.sbox[k] = B;
-  D.sboxF[k] = widenedBoxF(B);
+  D.sbox[k] = saneBox(D.posPred[s], P.margin, P.boxCap);
 }
 // Single-pass query.  Threads walk the cell-sorted order, so a wave's particles share their
 // 27-cell neighbourhood (cache-resident reads, little divergence).  Each real stages up to
@@ -678,7 +660,6 @@ constexpr int BP_CAP = 32;
 template <bool STAGE>
 __device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, const double4 Bi,
                                           int2* out) {
-  const float4 Fi = widenedBoxF(Bi);
   const int cx = cellCoord(Bi.x, G.lo.x, G.h, G.dim[0]);
   const int cy = cellCoord(Bi.y, G.lo.y, G.h, G.dim[1]);
   const int cz = cellCoord(Bi.z, G.lo.z, G.h, G.dim[2]);
@@ -693,7 +674,7 @@ __device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, co
       const int k0 = D.cellStart[row + x0], k1 = D.cellStart[row + x1 + 1];
       for (int k = k0; k < k1; ++k) {
         const int j = D.order[k];
-        if (j > i && boxOverlapF(Fi, D.sboxF[k]) && boxOverlap(Bi, D.sbox[k])) {
+        if (j > i && boxOverlap(Bi, D.sbox[k])) {
           if (!STAGE || n < BP_CAP) out[n] = make_int2(i, j);
           ++n;
         }

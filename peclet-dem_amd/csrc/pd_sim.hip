@@ -205,7 +205,6 @@ struct pd_sim {
       pRA, pRB, lLamT;
   DBuf<double> vVn0, vLam, pLam, lLam;
   DBuf<unsigned long long> mKey, winner, mask, lKey, uKey, red, bits, redPart;
-  DBuf<float4> sboxF;
   DBuf<int> blockCnt;
   DBuf<int> supList, supCnt;  // support manifolds of the grounded-level sweeps
   DBuf<int> ctl;  // device loop control (CTL_* in pd_kernels.hpp)
@@ -267,7 +266,7 @@ struct pd_sim {
     hRefPos.release(); hXiWall.release(); hWF.release(); hWT.release(); hPrevKeys.release();
     hTab.release();
     jpWork[0].release(); jpWork[1].release(); jpCnt.release(); redPart.release();
-    blockCnt.release(); sboxF.release(); ctl.release(); supList.release(); supCnt.release(); hLevel.release(); lvlKey.release(); lvlCnt.release();
+    blockCnt.release(); ctl.release(); supList.release(); supCnt.release(); hLevel.release(); lvlKey.release(); lvlCnt.release();
     lvlStart.release(); lvlPerm.release(); lvlChanged.release(); lvlBuckets.release();
   }
 
@@ -344,7 +343,7 @@ struct pd_sim {
     d.realIdx = realIdx.p; d.matId = matId.p; d.ground = ground.p; d.ground2 = ground2.p;
     d.shell = shell.p; d.pairTable = pairTableD.p;
     d.cellOf = cellOf.p; d.cellCnt = cellCnt.p; d.cellStart = cellStart.p; d.order = order.p;
-    d.sbox = sbox.p; d.sboxF = sboxF.p; d.pairCnt = pairCnt.p; d.pairStart = pairStart.p; d.pairs = pairs.p;
+    d.sbox = sbox.p; d.pairCnt = pairCnt.p; d.pairStart = pairStart.p; d.pairs = pairs.p;
     d.ncnt = ncnt.p; d.noff = noff.p; d.cAB = cAB.p; d.cN = cN.p; d.cRA = cRA.p; d.cRB = cRB.p;
     d.cSlot = cSlot.p; d.mCnt = mCnt.p; d.mStart = mStart.p; d.mAB = mAB.p; d.mReal = mReal.p;
     d.mRange = mRange.p; d.mRange2 = mRange2.p; d.cBV = cBV.p; d.mWV = mWV.p; d.vWV = vWV.p;
@@ -644,7 +643,6 @@ struct pd_sim {
     cellOf.ensure(numParticles, st);
     order.ensure(numParticles, st);
     sbox.ensure(numParticles, st);
-    sboxF.ensure(numParticles, st);
     pairCnt.ensure(numReal + 1, st);
     pairStart.ensure(numReal + 1, st);
     refreshDev();
