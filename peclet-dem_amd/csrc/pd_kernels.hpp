@@ -657,9 +657,11 @@ __global__ void k_cell_fill(Dev D, Params P) {
 // moves every segment to its scanned offset in ascending partner order.  A real with more than
 // BP_CAP partners is queried again by k_bp_compact, writing straight into the final list.
 constexpr int BP_CAP = 32;
+// STAGE: partner j of staged slot n goes to stage[n * stride] (slot-major, so the compaction
+// reads coalesce); otherwise the pair (i, j) goes to out[n]
 template <bool STAGE>
 __device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, const double4 Bi,
-                                          int2* out) {
+                                          int* stage, int stride, int2* out) {
   const int cx = cellCoord(Bi.x, G.lo.x, G.h, G.dim[0]);
   const int cy = cellCoord(Bi.y, G.lo.y, G.h, G.dim[1]);
   const int cz = cellCoord(Bi.z, G.lo.z, G.h, G.dim[2]);
@@ -675,7 +677,11 @@ __device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, co
       for (int k = k0; k < k1; ++k) {
         const int j = D.order[k];
         if (j > i && boxOverlap(Bi, D.sbox[k])) {
-          if (!STAGE || n < BP_CAP) out[n] = make_int2(i, j);
+          if (STAGE) {
+            if (n < BP_CAP) stage[(size_t)n * stride] = j;
+          } else {
+            out[n] = make_int2(i, j);
+          }
           ++n;
         }
       }
@@ -683,25 +689,23 @@ __device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, co
   }
   return n;
 }
-__global__ __launch_bounds__(TPB) void k_bp_query1(Dev D, Params P, Grid G, int2* pairTmp) {
+__global__ __launch_bounds__(TPB) void k_bp_query1(Dev D, Params P, Grid G, int* stage) {
   const int k = gtid();
   if (k >= P.numParticles) return;
   const int i = D.order[k];
   if (i >= P.numReal) return;
-  D.pairCnt[i] = bpQueryOne<true>(D, G, i, D.sbox[k], pairTmp + (size_t)i * BP_CAP);
+  D.pairCnt[i] = bpQueryOne<true>(D, G, i, D.sbox[k], stage + i, P.numReal, nullptr);
 }
-__global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, Grid G,
-                                                    const int2* pairTmp) {
+__global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, Grid G, const int* stage) {
   const int i = gtid();
   if (i >= P.numReal) return;
   const int n = D.pairCnt[i];
   if (n == 0) return;
   int2* seg = D.pairs + D.pairStart[i];
   if (n <= BP_CAP) {
-    const int2* src = pairTmp + (size_t)i * BP_CAP;
-    for (int a = 0; a < n; ++a) seg[a] = src[a];
+    for (int a = 0; a < n; ++a) seg[a] = make_int2(i, stage[(size_t)a * P.numReal + i]);
   } else {
-    bpQueryOne<false>(D, G, i, saneBox(D.posPred[i], P.margin, P.boxCap), seg);
+    bpQueryOne<false>(D, G, i, saneBox(D.posPred[i], P.margin, P.boxCap), nullptr, 0, seg);
   }
   // ascending partner order (segments are short)
   for (int a = 1; a < n; ++a) {

@@ -198,7 +198,8 @@ struct pd_sim {
   DBuf<int> frCnt;
   DBuf<int> realIdx, ground, ground2, cellOf, cellCnt, cellStart, order, pairCnt, pairStart;
   DBuf<unsigned char> matId, mCode, vFlag;
-  DBuf<int2> pairs, pairTmp, cAB, mAB, mReal, mRange, mRange2, vAB, pAB;
+  DBuf<int> pairStage;  // broadphase partners staged slot-major
+  DBuf<int2> pairs, cAB, mAB, mReal, mRange, mRange2, vAB, pAB;
   DBuf<int> ncnt, noff, cSlot, mCnt, mStart, mNp, mColor, cColor, vPerm, vRowOf, pPerm, pRowOf;
   DBuf<double4> cN, cRA, cRB, cBV, mNs, mTA, mTB, mRA, mRB, mWV, vN, vTA, vTB, vRA, vRB, vVt0,
       vLamT, vWV, pN,
@@ -244,7 +245,7 @@ struct pd_sim {
     sbox.release(); vw0.release(); vw.release(); bc.release(); scale.release();
     targetScale.release(); pairTableD.release(); thermo.release(); realIdx.release();
     ground.release(); ground2.release(); cellOf.release(); cellCnt.release(); cellStart.release();
-    order.release(); pairCnt.release(); pairStart.release(); pairTmp.release(); matId.release(); mCode.release();
+    order.release(); pairCnt.release(); pairStart.release(); pairStage.release(); matId.release(); mCode.release();
     vFlag.release(); pairs.release(); cAB.release(); mAB.release(); mReal.release();
     mRange.release(); mRange2.release(); cBV.release(); mWV.release(); vWV.release();
     vAB.release(); pAB.release(); ncnt.release(); noff.release();
@@ -651,14 +652,14 @@ struct pd_sim {
     scanQuiet(cellCnt.p, cellStart.p, ncell);
     zero(cellCnt.p, ncell);
     k_cell_fill<<<nblk(numParticles), TPB, 0, st>>>(D, P);
-    pairTmp.ensure((size_t)numReal * BP_CAP + 1, st);
+    pairStage.ensure((size_t)numReal * BP_CAP + 1, st);
     kernBegin();
-    k_bp_query1<<<nblk(numParticles), TPB, 0, st>>>(D, P, G, pairTmp.p);
+    k_bp_query1<<<nblk(numParticles), TPB, 0, st>>>(D, P, G, pairStage.p);
     kernEnd(K_BPQUERY, numParticles);
     np = scan(pairCnt.p, pairStart.p, numReal);
     pairs.ensure(np + 1, st);
     refreshDev();
-    k_bp_compact<<<nblk(numReal), TPB, 0, st>>>(D, P, G, pairTmp.p);
+    k_bp_compact<<<nblk(numReal), TPB, 0, st>>>(D, P, G, pairStage.p);
     HIPCHECK(hipGetLastError());
   }
 
