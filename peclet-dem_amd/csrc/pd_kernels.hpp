@@ -263,23 +263,21 @@ __global__ void k_scan_reduce(const int* in, int n, int* blockSums) {
   }
   if (threadIdx.x == 0) blockSums[blockIdx.x] = sh[0];
 }
-// single block: exclusive scan of the block sums in place, total -> *total
-__global__ void k_scan_top(int* blockSums, int nb, int* total) {
+// scan of one tile; the tile's offset is the block-reduced sum of the earlier tiles' totals
+// (written by k_scan_reduce), so no separate top-level pass is needed. The last tile writes
+// the grand total to out[n].
+__global__ void k_scan_down2(const int* in, int n, const int* blockSums, int* out) {
   __shared__ int sh[TPB];
-  int carry = 0;
-  for (int base = 0; base < nb; base += TPB) {
-    const int i = base + threadIdx.x;
-    const int v = (i < nb) ? blockSums[i] : 0;
-    int tot;
-    const int ex = blockExclusiveScan(v, sh, &tot);
-    if (i < nb) blockSums[i] = ex + carry;
-    carry += tot;
+  int pre = 0;
+  for (int i = threadIdx.x; i < (int)blockIdx.x; i += TPB) pre += blockSums[i];
+  sh[threadIdx.x] = pre;
+  __syncthreads();
+  for (int off = TPB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = carry;
-}
-__global__ void k_scan_down(const int* in, int n, const int* blockSums, int* out) {
-  __shared__ int sh[TPB];
+  const int prefix = sh[0];
+  __syncthreads();
   const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int vals[SCAN_ITEMS];
   int s = 0;
@@ -288,11 +286,12 @@ __global__ void k_scan_down(const int* in, int n, const int* blockSums, int* out
     s += vals[k];
   }
   int tot;
-  int ex = blockExclusiveScan(s, sh, &tot) + blockSums[blockIdx.x];
+  int ex = blockExclusiveScan(s, sh, &tot) + prefix;
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (base + k < n) out[base + k] = ex;
     ex += vals[k];
   }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = prefix + tot;
 }
 
 // ============================================================================================

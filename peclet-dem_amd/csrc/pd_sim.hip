@@ -314,9 +314,8 @@ struct pd_sim {
     }
     const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     scanTmp.ensure(nb + 1, st);
-    k_scan_reduce<<<nb, TPB, 0, st>>>(in, n, scanTmp.p);
-    k_scan_top<<<1, TPB, 0, st>>>(scanTmp.p, nb, out + n);
-    k_scan_down<<<nb, TPB, 0, st>>>(in, n, scanTmp.p, out);
+    if (nb > 1) k_scan_reduce<<<nb, TPB, 0, st>>>(in, n, scanTmp.p);
+    k_scan_down2<<<nb, TPB, 0, st>>>(in, n, scanTmp.p, out);
     HIPCHECK(hipGetLastError());
   }
 
