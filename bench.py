@@ -139,16 +139,22 @@ def load_traffic(workload, kernel):
 
 
 def copy_probe(mib):
-    """A plain device-to-device copy of known size, for scaling the HBM byte counters."""
-    import torch
-
-    x = torch.empty(mib << 18, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    x.fill_(1.0)
-    torch.cuda.synchronize()
+    """A plain device-to-device copy of known size (hipMemcpy, run as the runtime's blit kernel),
+    for scaling the HBM byte counters. Uses the HIP runtime the library itself links."""
+    hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    nbytes = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), nbytes) != 0:
+            raise RuntimeError("copy probe: hipMalloc failed")
+    hip.hipMemset(a, 1, nbytes)
+    hip.hipDeviceSynchronize()
     for _ in range(3):
-        y.copy_(x)
-    torch.cuda.synchronize()
+        if hip.hipMemcpy(b, a, nbytes, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("copy probe: hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
     print(f"# copy probe: 3 copies of {mib} MiB", file=sys.stderr, flush=True)
 
 
