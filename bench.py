@@ -20,6 +20,10 @@ The JSON line carries:
                   k_np_pairs          156 per pair (pair, offset, both bodies' pose and scale)
                                       + 104 per contact written
                   k_predict           240 per real;  k_reintegrate  264 per slot
+                  k_vsolve_block      601 per manifold row per sweep (one-workgroup loop)
+                  k_psolve_block      388 per contact row per sweep
+                The timed loop is only sim.step(dt): the library resolves its events lazily and
+                logs each substep's counters, both read once after the timed region.
                 traffic = measured HBM bytes per launch of that kernel (rocprofv3 --pmc FETCH_SIZE
                 and WRITE_SIZE, scaled by the same counters on a plain copy of known size), read
                 from profiles/traffic_<workload>.json when present, else null.
@@ -58,21 +62,17 @@ def substep_bytes(st):
     return b
 
 
-def kernel_bytes(name, units, st):
-    """Algorithmic bytes of all launches of one kernel in one substep (units = rows processed)."""
-    N = st["num_real"]
-    if name == "k_vsweep":
+def kernel_bytes(name, units, log):
+    """Algorithmic bytes of all timed launches of one kernel: units = the library's summed units
+    for it (rows, row-sweeps, slots or pairs), log = the per-substep counters of the same run."""
+    if name in ("k_vsweep", "k_vsolve_block"):  # manifold rows (x sweeps for the block loop)
         return 601 * units
-    if name == "k_psweep":
+    if name in ("k_psweep", "k_psolve_block"):  # contact rows (x sweeps)
         return 388 * units
-    if name == "k_bp_query1":
-        return 36 * st["num_particles"] + 4 * N + 8 * st["num_pairs"]
+    if name == "k_bp_query1":  # slots queried, plus each real's count and pair partners
+        return 36 * units + sum(4 * s["num_real"] + 8 * s["num_pairs"] for s in log)
     if name == "k_np_pairs":
-        return 156 * st["num_pairs"] + 104 * st["num_contacts"]
-    if name == "k_vsolve_block":  # every sweep of the substep's velocity loops
-        return 601 * st["active_manifolds"] * (st["sweeps_vel"] + st["sweeps_stab"])
-    if name == "k_psolve_block":
-        return 388 * st["num_contacts"] * st["sweeps_pos"]
+        return 156 * units + 104 * sum(s["num_contacts"] for s in log)
     if name == "k_predict":
         return 240 * units
     if name == "k_reintegrate":
@@ -110,22 +110,28 @@ def run_cpu_baseline(kind, bed, dt, substeps):
     return len(bed["pos"]) * substeps / t, t
 
 
-class KernelClock:
-    """Per-kernel HIP-event times of the library, differenced per substep."""
+def kernel_times(sim):
+    """Per-kernel HIP-event totals of the library: names, ms, launches, units."""
+    b, h = sim._b, sim._h
+    count = b.get_kernel_times(h, None, None, None, 0)
+    names = [b.kernel_name(k).decode() for k in range(count)]
+    ms = np.zeros(count)
+    launches = np.zeros(count, np.int64)
+    units = np.zeros(count, np.int64)
+    b.get_kernel_times(h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                       launches.ctypes.data_as(C.POINTER(C.c_long)),
+                       units.ctypes.data_as(C.POINTER(C.c_long)), count)
+    return names, ms, launches, units
 
-    def __init__(self, sim):
-        self.sim = sim
-        self.count = sim._b.get_kernel_times(sim._h, None, None, None, 0)
-        self.names = [sim._b.kernel_name(k).decode() for k in range(self.count)]
 
-    def read(self):
-        ms = np.zeros(self.count)
-        launches = np.zeros(self.count, np.int64)
-        rows = np.zeros(self.count, np.int64)
-        self.sim._b.get_kernel_times(self.sim._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
-                                     launches.ctypes.data_as(C.POINTER(C.c_long)),
-                                     rows.ctypes.data_as(C.POINTER(C.c_long)), self.count)
-        return ms, launches, rows
+def stats_log(sim):
+    """The per-substep counters the library logged while timing at level 2."""
+    from peclet.dem._binding import Stats
+
+    n = sim._b.get_stats_log(sim._h, None, 0)
+    arr = (Stats * max(n, 1))()
+    sim._b.get_stats_log(sim._h, arr, n)
+    return [arr[k].as_dict() for k in range(n)]
 
 
 def load_traffic(workload, kernel):
@@ -236,30 +242,22 @@ def main():
         sim.step(dt)
     sim._b.synchronize(sim._h)
     sim._b.set_timing(sim._h, 2)
-    clock = KernelClock(sim)
-    names = clock.names
-    kbytes = np.zeros(len(names))
-    prev_ms, prev_l, prev_r = clock.read()
-    stats = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     report = max(1, args.steps // 8)
     for k in range(args.steps):
         sim.step(dt)
-        st = sim.get_stats()
-        stats.append(st)
-        ms, launches, rows = clock.read()
-        for q, name in enumerate(names):
-            if launches[q] > prev_l[q]:
-                kbytes[q] += kernel_bytes(name, rows[q] - prev_r[q], st)
-        prev_ms, prev_l, prev_r = ms, launches, rows
         if (k + 1) % report == 0:
             print(f"# substep {k + 1}/{args.steps}", file=sys.stderr, flush=True)
     sim._b.synchronize(sim._h)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    ms, launches, rows = clock.read()
     value = args.n * args.steps / t
+    names, ms, launches, units = kernel_times(sim)
+    stats = stats_log(sim)
+    assert len(stats) == args.steps, (len(stats), args.steps)
+    kbytes = np.array([kernel_bytes(nm, int(units[q]), stats) for q, nm in enumerate(names)],
+                      dtype=float)
     dom = int(np.argmax(ms))
     nl = max(int(launches[dom]), 1)
     bytes_per_launch = kbytes[dom] / nl

@@ -189,8 +189,10 @@ int pd_batch_step(pd_batch* b, double dt, int nsteps); /* nsteps substeps of eve
 /* Timing with HIP events on the handle's stream (no reference counterpart; the reference's
  * get_profiling_info returns an empty dict, src/dem_bindings.cpp:385): level 0 off, 1 per-stage
  * times, 2 also one event pair around every launch of the main kernels (PGS velocity sweep,
- * position sweep, broadphase query, narrowphase, predict, re-integrate).  Values accumulate
- * until pd_set_timing is called again. */
+ * position sweep, broadphase query, narrowphase, predict, re-integrate, and the one-workgroup
+ * sweep loops).  Events are resolved lazily, so timing adds no host synchronisation to a
+ * substep.  Values accumulate until pd_set_timing is called again; at level 2 each substep's
+ * counters are also logged (pd_get_stats_log). */
 int pd_set_timing(pd_sim* s, int level);
 /* Launch-shape tuning (no reference counterpart): "block_solver_rows" = the largest velocity /
  * position sweep loop (in rows) solved inside one workgroup instead of one launch per colour
@@ -198,9 +200,14 @@ int pd_set_timing(pd_sim* s, int level);
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 int pd_get_stage_times(pd_sim* s, double* out_ms, int cap);
 const char* pd_stage_name(int stage);
-/* per timed kernel k: summed ms, launch count and units processed (manifold rows, contact rows,
- * particles, pairs, particles, slots); returns the number of timed kernels */
+/* per timed kernel k: summed ms, launch count and units processed (k_vsweep: manifold rows,
+ * k_psweep: contact rows, k_bp_query1: slots queried, k_np_pairs: pairs, k_predict: particles,
+ * k_reintegrate: slots, k_vsolve_block: manifold rows x sweeps, k_psolve_block: contact rows x
+ * sweeps); returns the number of timed kernels */
 int pd_get_kernel_times(pd_sim* s, double* ms, long* launches, long* rows, int cap);
+/* the counters of every substep since pd_set_timing(s, 2), oldest first: copies at most cap
+ * records into out (out may be NULL) and returns how many are logged */
+int pd_get_stats_log(pd_sim* s, pd_stats* out, int cap);
 const char* pd_kernel_name(int k);
 
 #ifdef __cplusplus

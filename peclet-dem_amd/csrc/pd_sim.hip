@@ -144,48 +144,87 @@ struct pd_sim {
   // the largest scale, cached while no scale is written (every writer clears scaleKnown)
   bool scaleKnown = false;
   unsigned long long maxScaleKey = 0;
-  // timing: stage events (level >= 1) and per-launch events of the sweep kernels (level >= 2)
+  // timing: stage events (level >= 1) and per-launch events of the main kernels (level >= 2).
+  // Events come from a pool and are resolved lazily (at a query, or every few hundred substeps),
+  // so timing adds no host synchronisation to a substep.
   int timing = 0;
   double stageMs[ST_COUNT] = {};
-  hipEvent_t ev[ST_COUNT + 1] = {};
-  // kernels timed per launch at level 2; rows = the units each launch processed
+  // kernels timed per launch at level 2; units = what each launch processed
   enum { K_VSWEEP, K_PSWEEP, K_BPQUERY, K_NPPAIRS, K_PREDICT, K_REINTEGRATE, K_VBLOCK, K_PBLOCK,
          K_COUNT };
   double kernMs[K_COUNT] = {};
   long kernLaunches[K_COUNT] = {};
   long kernRows[K_COUNT] = {};
-  std::vector<hipEvent_t> kev;
-  std::vector<std::pair<int, int>> kevUse;  // (kernel id, rows) per recorded pair
-  size_t kevNext = 0;
+  struct KernRec {
+    hipEvent_t a, b;
+    int kid;
+    long units;
+  };
+  std::vector<hipEvent_t> evFree;
+  std::vector<KernRec> kernPending;
+  std::vector<hipEvent_t> stagePending;  // ST_COUNT + 1 events per substep
+  hipEvent_t stageCur[ST_COUNT + 1] = {};
+  int stageMarked = -1;  // last stage marked in the current substep
+  hipEvent_t kernOpen = nullptr;
+  // per-substep counters, logged while timing >= 2
+  std::vector<pd_stats> statsLog;
+  hipEvent_t takeEvent() {
+    if (evFree.empty()) {
+      hipEvent_t e;
+      HIPCHECK(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = evFree.back();
+    evFree.pop_back();
+    return e;
+  }
   void kernBegin() {
     if (timing < 2) return;
-    if (kevNext + 2 > kev.size()) {
-      for (int k = 0; k < 64; ++k) {
-        hipEvent_t e;
-        HIPCHECK(hipEventCreate(&e));
-        kev.push_back(e);
-      }
-    }
-    HIPCHECK(hipEventRecord(kev[kevNext], st));
+    kernOpen = takeEvent();
+    HIPCHECK(hipEventRecord(kernOpen, st));
   }
-  void kernEnd(int kid, int rows) {
-    if (timing < 2) return;
-    HIPCHECK(hipEventRecord(kev[kevNext + 1], st));
-    kevUse.push_back({kid, rows});
-    kevNext += 2;
+  // returns the record's index (valid until the next resolve) or -1
+  long kernEnd(int kid, long units) {
+    if (timing < 2) return -1;
+    hipEvent_t b = takeEvent();
+    HIPCHECK(hipEventRecord(b, st));
+    kernPending.push_back({kernOpen, b, kid, units});
+    return (long)kernPending.size() - 1;
   }
-  void kernCollect() {
-    if (timing < 2) return;
+  // wait for every recorded event and fold the elapsed times into the totals
+  void resolveTiming() {
+    if (kernPending.empty() && stagePending.empty()) return;
     sync();
-    for (size_t u = 0; u < kevUse.size(); ++u) {
+    for (const KernRec& r : kernPending) {
       float ms = 0.f;
-      HIPCHECK(hipEventElapsedTime(&ms, kev[2 * u], kev[2 * u + 1]));
-      kernMs[kevUse[u].first] += ms;
-      kernLaunches[kevUse[u].first] += 1;
-      kernRows[kevUse[u].first] += kevUse[u].second;
+      HIPCHECK(hipEventElapsedTime(&ms, r.a, r.b));
+      kernMs[r.kid] += ms;
+      kernLaunches[r.kid] += 1;
+      kernRows[r.kid] += r.units;
+      evFree.push_back(r.a);
+      evFree.push_back(r.b);
     }
-    kevUse.clear();
-    kevNext = 0;
+    kernPending.clear();
+    for (size_t base = 0; base + ST_COUNT < stagePending.size(); base += ST_COUNT + 1) {
+      for (int k = 0; k < ST_COUNT; ++k) {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, stagePending[base + k], stagePending[base + k + 1]));
+        stageMs[k] += ms;
+      }
+      for (int k = 0; k <= ST_COUNT; ++k) evFree.push_back(stagePending[base + k]);
+    }
+    stagePending.clear();
+  }
+  void resetTiming(int level) {
+    resolveTiming();
+    timing = level;
+    for (double& x : stageMs) x = 0.0;
+    for (int k = 0; k < K_COUNT; ++k) {
+      kernMs[k] = 0.0;
+      kernLaunches[k] = 0;
+      kernRows[k] = 0;
+    }
+    statsLog.clear();
   }
   // device buffers
   DBuf<double4> pos, posPred, quatPred, extForce, shell, sbox;
@@ -219,7 +258,6 @@ struct pd_sim {
   pd_sim(int cap) : capacity(cap) {
     HIPCHECK(hipStreamCreate(&st));
     HIPCHECK(hipHostMalloc(&hostScratch, 4096, hipHostMallocDefault));
-    for (int k = 0; k <= ST_COUNT; ++k) HIPCHECK(hipEventCreate(&ev[k]));
     ensureSlots(cap > 0 ? cap : 1, 0);
     red.ensure(8, st);
     bits.ensure(4, st);
@@ -235,8 +273,14 @@ struct pd_sim {
   ~pd_sim() {
     (void)hipStreamSynchronize(st);
     DBufReleaseAll();
-    for (int k = 0; k <= ST_COUNT; ++k) (void)hipEventDestroy(ev[k]);
-    for (hipEvent_t e : kev) (void)hipEventDestroy(e);
+    for (const KernRec& r : kernPending) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : stagePending) (void)hipEventDestroy(e);
+    if (stageMarked >= 0)
+      for (int k = 0; k <= stageMarked; ++k) (void)hipEventDestroy(stageCur[k]);
+    for (hipEvent_t e : evFree) (void)hipEventDestroy(e);
     if (hostScratch) (void)hipHostFree(hostScratch);
     (void)hipStreamDestroy(st);
   }
@@ -298,7 +342,20 @@ struct pd_sim {
     if (n) HIPCHECK(hipMemsetAsync(dptr, 0, n * sizeof(T), st));
   }
   void mark(int stage) {
-    if (timing) HIPCHECK(hipEventRecord(ev[stage], st));
+    if (!timing) return;
+    if (stage == ST_PREDICT && stageMarked >= 0) {
+      // a substep that ended early (an error) left its events open: recycle them
+      for (int k = 0; k <= stageMarked; ++k) evFree.push_back(stageCur[k]);
+      stageMarked = -1;
+    }
+    if (stage != stageMarked + 1) return;
+    stageCur[stage] = takeEvent();
+    HIPCHECK(hipEventRecord(stageCur[stage], st));
+    stageMarked = stage;
+    if (stage == ST_COUNT) {
+      stagePending.insert(stagePending.end(), stageCur, stageCur + ST_COUNT + 1);
+      stageMarked = -1;
+    }
   }
 
   // exclusive scan of n ints into out[0..n), total into out[n]; returns the total
@@ -894,8 +951,10 @@ struct pd_sim {
     HIPCHECK(hipGetLastError());
   }
 
+  long vBlockRec[2] = {-1, -1}, pBlockRec = -1;
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
+    vBlockRec[0] = vBlockRec[1] = pBlockRec = -1;
     if (stabMode == 2) throw std::runtime_error("stabilization mode 'multilevel' is not implemented");
     // a frictional wall drives friction even for frictionless bodies (src/sim.hpp:169-170)
     double wallFrictionMax = 0.0;
@@ -1036,7 +1095,7 @@ struct pd_sim {
           k_vsolve_block<2><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(
               D, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
               ctl.p + CTL_SWEEPS_V, nullptr);
-        kernEnd(K_VBLOCK, nRowsV);
+        vBlockRec[0] = kernEnd(K_VBLOCK, nRowsV);
       } else if (!smallV) sweepLoop(velIters, ctl.p + CTL_STOP_V, [&] {
         k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V);
         velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest, ctl.p + CTL_STOP_V);
@@ -1056,7 +1115,7 @@ struct pd_sim {
             k_vsolve_block<1><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(
                 D, P, classStarts(vs), numColors, nm, 2 * velIters, vRestS, vRestS,
                 ctl.p + CTL_STOP_S, ctl.p + CTL_SWEEPS_S, ctl.p + CTL_STAB_RUN);
-            kernEnd(K_VBLOCK, nRowsV);
+            vBlockRec[1] = kernEnd(K_VBLOCK, nRowsV);
           }
         } else if (stabMode == 1 && velIters > 0 && read1(ctl.p + CTL_STAB_RUN)) {
           if (nRowsV > 0)
@@ -1202,7 +1261,7 @@ struct pd_sim {
         k_psolve_block<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(D, classStarts(ps), numPosColors, nc,
                                                            posIters, posTol, ctl.p + CTL_STOP_P,
                                                            ctl.p + CTL_SWEEPS_P);
-        kernEnd(K_PBLOCK, nRowsP);
+        pBlockRec = kernEnd(K_PBLOCK, nRowsP);
       }
       HIPCHECK(hipGetLastError());
     } else if (posLeftover == 0) {
@@ -1264,15 +1323,6 @@ struct pd_sim {
     if (thermoTau > 0.0 && dt > 0.0) applyThermostat();
     HIPCHECK(hipGetLastError());
     mark(ST_COUNT);
-    kernCollect();
-    if (timing) {
-      sync();
-      for (int k = 0; k < ST_COUNT; ++k) {
-        float ms = 0.f;
-        HIPCHECK(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-        stageMs[k] += ms;
-      }
-    }
     stats.num_real = numReal;
     stats.num_particles = numParticles;
     stats.num_pairs = np;
@@ -1283,6 +1333,14 @@ struct pd_sim {
     stats.num_colors_pos = numPosColors;
     stats.max_overlap = maxOverlap;
     stats.max_approach = maxApproach;
+    if (timing >= 2) {
+      // the one-workgroup loops processed rows x sweeps, known only now
+      if (vBlockRec[0] >= 0) kernPending[vBlockRec[0]].units = (long)nRowsV * stats.sweeps_vel;
+      if (vBlockRec[1] >= 0) kernPending[vBlockRec[1]].units = (long)nRowsV * stats.sweeps_stab;
+      if (pBlockRec >= 0) kernPending[pBlockRec].units = (long)nRowsP * stats.sweeps_pos;
+      statsLog.push_back(stats);
+    }
+    if (kernPending.size() > 8192 || stagePending.size() > 256 * (ST_COUNT + 1)) resolveTiming();
   }
 
   // applyThermostatKokkos, src/integration.hpp:229-271 (block partials summed in block order)
@@ -2291,16 +2349,11 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
   });
 }
 int pd_set_timing(pd_sim* s, int level) {
-  s->timing = level;
-  for (double& t : s->stageMs) t = 0.0;
-  for (int k = 0; k < pd_sim::K_COUNT; ++k) {
-    s->kernMs[k] = 0.0;
-    s->kernLaunches[k] = 0;
-    s->kernRows[k] = 0;
-  }
-  return 0;
+  return guard([&] { s->resetTiming(level); });
 }
 int pd_get_kernel_times(pd_sim* s, double* ms, long* launches, long* rows, int cap) {
+  const int rc = guard([&] { s->resolveTiming(); });
+  if (rc) return rc;
   for (int k = 0; k < pd_sim::K_COUNT && k < cap; ++k) {
     ms[k] = s->kernMs[k];
     launches[k] = s->kernLaunches[k];
@@ -2309,8 +2362,16 @@ int pd_get_kernel_times(pd_sim* s, double* ms, long* launches, long* rows, int c
   return pd_sim::K_COUNT;
 }
 int pd_get_stage_times(pd_sim* s, double* out, int cap) {
+  const int rc = guard([&] { s->resolveTiming(); });
+  if (rc) return rc;
   for (int k = 0; k < ST_COUNT && k < cap; ++k) out[k] = s->stageMs[k];
   return ST_COUNT;
+}
+int pd_get_stats_log(pd_sim* s, pd_stats* out, int cap) {
+  const int n = (int)s->statsLog.size();
+  if (out)
+    for (int k = 0; k < n && k < cap; ++k) out[k] = s->statsLog[k];
+  return n;
 }
 const char* pd_kernel_name(int k) {
   static const char* names[pd_sim::K_COUNT] = {

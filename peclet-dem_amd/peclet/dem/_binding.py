@@ -126,6 +126,7 @@ PRODUCT_SIGNATURES = {
     "get_stage_times": (_i, [_p, _dp, _i]),
     "stage_name": (C.c_char_p, [_i]),
     "get_kernel_times": (_i, [_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long), _i]),
+    "get_stats_log": (_i, [_p, C.POINTER(Stats), _i]),
     "kernel_name": (C.c_char_p, [_i]),
     "set_tuning": (_i, [_p, C.c_char_p, C.c_long]),
     "export_sdf": (_i, [_p, C.c_char_p, _i, _i, _i]),

@@ -296,3 +296,45 @@ def test_zero_copy_views(hip_lib):
     assert np.array_equal(h.get_positions(), r.get_positions())
     h.step(scene["dt"])
     np.testing.assert_array_equal(p.numpy(), h.get_positions())  # still aliases the state
+
+
+@pytest.mark.gpu
+def test_timing_is_passive_and_logs_counters(hip_lib):
+    # level-2 timing records events and logs counters; it must not change a single bit of state
+    import ctypes as C
+
+    from peclet.dem._binding import Stats
+
+    scene = make_scene("bed_periodic", 400, 3)
+    a = apply_scene(hip_sim(400), scene)
+    b = apply_scene(hip_sim(400), scene)
+    b._b.set_timing(b._h, 2)
+    plain = []
+    for _ in range(12):
+        a.step(scene["dt"])
+        b.step(scene["dt"])
+        plain.append(a.get_stats())
+    for name in ["get_positions", "get_velocities", "get_quaternions", "get_angular_velocities"]:
+        assert np.array_equal(getattr(a, name)(), getattr(b, name)()), name
+    n = b._b.get_stats_log(b._h, None, 0)
+    assert n == 12
+    arr = (Stats * n)()
+    assert b._b.get_stats_log(b._h, arr, n) == n
+    for k in range(n):
+        assert arr[k].as_dict() == plain[k], k
+    count = b._b.get_kernel_times(b._h, None, None, None, 0)
+    ms = np.zeros(count)
+    launches = np.zeros(count, np.int64)
+    units = np.zeros(count, np.int64)
+    b._b.get_kernel_times(b._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                          launches.ctypes.data_as(C.POINTER(C.c_long)),
+                          units.ctypes.data_as(C.POINTER(C.c_long)), count)
+    names = [b._b.kernel_name(k).decode() for k in range(count)]
+    q = names.index("k_predict")
+    assert launches[q] == 12 and units[q] == 12 * 400 and ms[q] > 0
+    v = names.index("k_vsolve_block")
+    want = sum(s["active_manifolds"] * (s["sweeps_vel"] + s["sweeps_stab"]) for s in plain)
+    assert units[v] == want
+    stage = np.zeros(32)
+    nst = b._b.get_stage_times(b._h, stage.ctypes.data_as(C.POINTER(C.c_double)), 32)
+    assert nst > 0 and np.all(stage[:nst] >= 0) and stage[:nst].sum() > 0
