@@ -71,6 +71,7 @@ struct Dev {
   double* pairTable;
   // broadphase
   int* cellOf;
+  int* cellRank;
   int* cellCnt;
   int* cellStart;
   int* order;
@@ -221,10 +222,6 @@ __global__ void k_fill_i32(int* a, int v, int n) {
   const int i = gtid();
   if (i < n) a[i] = v;
 }
-__global__ void k_set_counts(int* a, int first, int n) {
-  const int i = gtid();
-  if (i < n) a[i] = i == 0 ? first : 0;
-}
 __global__ void k_fill_u64(unsigned long long* a, unsigned long long v, int n) {
   const int i = gtid();
   if (i < n) a[i] = v;
@@ -249,8 +246,10 @@ __device__ int blockExclusiveScan(int v, int* sh, int* total) {
   return incl - v;
 }
 
-__global__ void k_scan_reduce(const int* in, int n, int* blockSums) {
+// nDev (optional): the element count on the device; n is then only the launch bound
+__global__ void k_scan_reduce(const int* in, int n, int* blockSums, const int* nDev) {
   __shared__ int sh[TPB];
+  if (nDev) n = *nDev;
   const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int s = 0;
   for (int k = 0; k < SCAN_ITEMS; ++k)
@@ -266,8 +265,12 @@ __global__ void k_scan_reduce(const int* in, int n, int* blockSums) {
 // scan of one tile; the tile's offset is the block-reduced sum of the earlier tiles' totals
 // (written by k_scan_reduce), so no separate top-level pass is needed. The last tile writes
 // the grand total to out[n].
-__global__ void k_scan_down2(const int* in, int n, const int* blockSums, int* out) {
+__global__ void k_scan_down2(const int* in, int n, const int* blockSums, int* out,
+                             const int* nDev) {
   __shared__ int sh[TPB];
+  if (nDev) n = *nDev;
+  const int last = n > 0 ? (n - 1) / SCAN_TILE : 0;
+  if ((int)blockIdx.x > last) return;
   int pre = 0;
   for (int i = threadIdx.x; i < (int)blockIdx.x; i += TPB) pre += blockSums[i];
   sh[threadIdx.x] = pre;
@@ -291,7 +294,7 @@ __global__ void k_scan_down2(const int* in, int n, const int* blockSums, int* ou
     if (base + k < n) out[base + k] = ex;
     ex += vals[k];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = prefix + tot;
+  if ((int)blockIdx.x == last && threadIdx.x == 0) out[n] = prefix + tot;
 }
 
 // ============================================================================================
@@ -603,8 +606,13 @@ __global__ __launch_bounds__(TPB) void k_bp_bounds(Dev D, Params P, unsigned lon
     part[blockIdx.x * 8 + d] = r;
   }
 }
-__global__ void k_bp_bounds_final(const unsigned long long* part, int nb, unsigned long long* red) {
+// Final bounds reduction, then the grid on the device (thread 0): cell edge 2 x the largest
+// half-width (x 1.0001), grown by 1.5x until there are at most 4 x slots + 64 cells (the cell
+// buffers' capacity).  Any grid with cells at least that wide finds every overlapping pair.
+__global__ void k_bp_bounds_final(const unsigned long long* part, int nb, unsigned long long* red,
+                                  int numSlots, Grid* grid, int* numCells) {
   __shared__ unsigned long long sh[7][WAVES];
+  __shared__ unsigned long long fin[7];
   unsigned long long v[7] = {~0ull, ~0ull, ~0ull, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
     for (int d = 0; d < 7; ++d) {
@@ -621,7 +629,34 @@ __global__ void k_bp_bounds_final(const unsigned long long* part, int nb, unsign
     unsigned long long r = sh[d][0];
     for (int k = 1; k < WAVES; ++k) r = (d < 3) ? (sh[d][k] < r ? sh[d][k] : r) : (sh[d][k] > r ? sh[d][k] : r);
     red[d] = r;
+    fin[d] = r;
   }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = fin[d] == ~0ull ? 0.0 : ordVal(fin[d]);
+    hi[d] = fin[3 + d] == 0 ? 0.0 : ordVal(fin[3 + d]);
+    if (lo[d] > hi[d]) lo[d] = hi[d] = 0.0;
+  }
+  const double maxB = fin[6] ? ordVal(fin[6]) : 0.0;
+  Grid G;
+  G.lo = d3{lo[0], lo[1], lo[2]};
+  G.h = 2.0 * maxB * 1.0001 + 1e-300;
+  for (;;) {
+    double cells = 1.0;
+    for (int d = 0; d < 3; ++d) {
+      double nd = floor((hi[d] - lo[d]) / G.h) + 1.0;
+      if (!(nd >= 1.0)) nd = 1.0;
+      if (nd > 1e9) nd = 1e9;
+      G.dim[d] = (int)nd;
+      cells *= nd;
+    }
+    if (cells <= 4.0 * numSlots + 64.0) break;
+    G.h *= 1.5;
+  }
+  *grid = G;
+  *numCells = G.dim[0] * G.dim[1] * G.dim[2];
 }
 
 __device__ __forceinline__ int cellCoord(double c, double lo, double h, int dim) {
@@ -630,25 +665,28 @@ __device__ __forceinline__ int cellCoord(double c, double lo, double h, int dim)
   if (f >= (double)(dim - 1)) return dim - 1;
   return (int)f;
 }
-__global__ void k_cell_count(Dev D, Params P, Grid G) {
+// cell of every slot and its rank inside the cell (cellCnt is all zero on entry)
+__global__ void k_cell_count(Dev D, Params P, const Grid* grid) {
   const int s = gtid();
   if (s >= P.numParticles) return;
+  const Grid G = *grid;
   const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
   const int cx = cellCoord(B.x, G.lo.x, G.h, G.dim[0]);
   const int cy = cellCoord(B.y, G.lo.y, G.h, G.dim[1]);
   const int cz = cellCoord(B.z, G.lo.z, G.h, G.dim[2]);
   const int c = (cz * G.dim[1] + cy) * G.dim[0] + cx;
   D.cellOf[s] = c;
-  atomicAdd(&D.cellCnt[c], 1);
+  D.cellRank[s] = atomicAdd(&D.cellCnt[c], 1);
 }
-// cellCnt is reused as the fill cursor (zeroed by the host before this kernel)
+// slots in cell order; every occupied cell's count is cleared again for the next broadphase
 __global__ void k_cell_fill(Dev D, Params P) {
   const int s = gtid();
   if (s >= P.numParticles) return;
   const int c = D.cellOf[s];
-  const int k = D.cellStart[c] + atomicAdd(&D.cellCnt[c], 1);
+  const int k = D.cellStart[c] + D.cellRank[s];
   D.order[k] = s;
   D.sbox[k] = saneBox(D.posPred[s], P.margin, P.boxCap);
+  D.cellCnt[c] = 0;
 }
 // Single-pass query.  Threads walk the cell-sorted order, so a wave's particles share their
 // 27-cell neighbourhood (cache-resident reads, little divergence).  Each real stages up to
@@ -688,16 +726,19 @@ __device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, co
   }
   return n;
 }
-__global__ __launch_bounds__(TPB) void k_bp_query1(Dev D, Params P, Grid G, int* stage) {
+__global__ __launch_bounds__(TPB) void k_bp_query1(Dev D, Params P, const Grid* grid, int* stage) {
   const int k = gtid();
   if (k >= P.numParticles) return;
+  const Grid G = *grid;
   const int i = D.order[k];
   if (i >= P.numReal) return;
   D.pairCnt[i] = bpQueryOne<true>(D, G, i, D.sbox[k], stage + i, P.numReal, nullptr);
 }
-__global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, Grid G, const int* stage) {
+__global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, const Grid* grid,
+                                                    const int* stage) {
   const int i = gtid();
   if (i >= P.numReal) return;
+  const Grid G = *grid;
   const int n = D.pairCnt[i];
   if (n == 0) return;
   int2* seg = D.pairs + D.pairStart[i];
@@ -1030,47 +1071,58 @@ __global__ void k_jp_init_manifolds(Dev D, int numManifolds, int* work, int* cou
 // One JP round over the compacted work list.  winner[] holds, per body, 1 + the largest priority
 // among its uncoloured edges; a committing edge clears the slots of its two bodies, a losing edge
 // keeps its key there (it re-contends with the same key), so no per-round reset is needed.
+// Both round kernels walk their list in grid-stride chunks, so any grid covers it (the host sizes
+// grids from the previous colouring's list sizes).
 __global__ void k_jp_contend(const int2* ends, const int* work, const int* count,
                              unsigned long long* winner) {
-  const int t = gtid();
-  if (t >= *count) return;
-  const int e = work[t];
-  const int2 ab = ends[e];
-  const unsigned long long key = colorKey1(e);
-  atomicMax(&winner[ab.x], key);
-  if (ab.y >= 0) atomicMax(&winner[ab.y], key);
+  const int n = *count;
+  for (int t = gtid(); t < n; t += gridDim.x * blockDim.x) {
+    const int e = work[t];
+    const int2 ab = ends[e];
+    const unsigned long long key = colorKey1(e);
+    atomicMax(&winner[ab.x], key);
+    if (ab.y >= 0) atomicMax(&winner[ab.y], key);
+  }
 }
 __global__ void k_jp_commit(const int2* ends, int* colour, const int* work, const int* count,
                             unsigned long long* winner, unsigned long long* mask, int* next,
                             int* nextCount) {
   __shared__ int shCnt, shBase;
-  const int t = gtid();
   const int n = *count;
-  int e = -1;
-  bool lose = false;
-  if (t < n) {
-    e = work[t];
-    const int2 ab = ends[e];
-    const unsigned long long key = colorKey1(e);
-    const unsigned long long wa = winner[ab.x];
-    const unsigned long long wb = ab.y >= 0 ? winner[ab.y] : key;
-    if (wa != key || wb != key) {
-      lose = true;
-    } else {
-      unsigned long long forbidden = mask[ab.x];
-      if (ab.y >= 0) forbidden |= mask[ab.y];
-      int c = 0;
-      while (c < 62 && (forbidden & (1ull << c))) ++c;
-      colour[e] = c;
-      mask[ab.x] |= 1ull << c;
-      winner[ab.x] = 0;
-      if (ab.y >= 0) {
-        mask[ab.y] |= 1ull << c;
-        winner[ab.y] = 0;
+  // block-uniform trip count: every thread reaches each blockAppend
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int t = base + threadIdx.x;
+    int e = -1;
+    bool lose = false;
+    if (t < n) {
+      e = work[t];
+      const int2 ab = ends[e];
+      const unsigned long long key = colorKey1(e);
+      const unsigned long long wa = winner[ab.x];
+      const unsigned long long wb = ab.y >= 0 ? winner[ab.y] : key;
+      if (wa != key || wb != key) {
+        lose = true;
+      } else {
+        unsigned long long forbidden = mask[ab.x];
+        if (ab.y >= 0) forbidden |= mask[ab.y];
+        int c = 0;
+        while (c < 62 && (forbidden & (1ull << c))) ++c;
+        colour[e] = c;
+        mask[ab.x] |= 1ull << c;
+        winner[ab.x] = 0;
+        if (ab.y >= 0) {
+          mask[ab.y] |= 1ull << c;
+          winner[ab.y] = 0;
+        }
       }
     }
+    blockAppend(lose, e, next, nextCount, &shCnt, &shBase);
   }
-  blockAppend(lose, e, next, nextCount, &shCnt, &shBase);
+}
+// round counters of a batch: cnt[0] = the entering list size (from src when given), rest 0
+__global__ void k_jp_counts(int* cnt, const int* src, int first, int n) {
+  const int i = threadIdx.x;
+  if (i < n) cnt[i] = i == 0 ? (src ? *src : first) : 0;
 }
 // Colour classes: pass 1 counts per (block, colour) with LDS atomics; k_color_offsets turns the
 // counts into per-block bases; pass 2 places every edge at base + its rank inside the block.  Both
@@ -1110,14 +1162,18 @@ __global__ void k_color_offsets(int* blockCnt, int nb, int* colTot) {
   }
   if (threadIdx.x == 0) colTot[blockIdx.x] = tot;
 }
+// one wave: exclusive scan of the 64 colour totals
 __global__ void k_color_starts(const int* colTot, int* colStart) {
-  if (threadIdx.x != 0) return;
-  int run = 0;
-  for (int c = 0; c < 64; ++c) {
-    colStart[c] = run;
-    run += colTot[c];
+  const int c = threadIdx.x;
+  if (c >= 64) return;
+  const int v = colTot[c];
+  int incl = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(incl, off, 64);
+    if (c >= off) incl += o;
   }
-  colStart[64] = run;
+  colStart[c] = incl - v;
+  if (c == 63) colStart[64] = incl;
 }
 __global__ void k_color_place(const int* colour, int numEdges, const int* blockBase,
                               const int* colStart, int* perm, int* rowOf) {
@@ -1731,8 +1787,9 @@ __global__ __launch_bounds__(TPB) void k_vsweep(Dev D, Params P, int rowBegin, i
   if (MODE != 0) blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
 }
 // colour 62 may hold manifolds sharing a body (mask saturation): one thread, manifold order
-__global__ void k_vsweep_serial(Dev D, Params P, int numManifolds, int colour, int mode,
-                                double thr, const int* stop) {
+__global__ __launch_bounds__(64) void k_vsweep_serial(Dev D, Params P, int numManifolds,
+                                                      int colour, int mode, double thr,
+                                                      const int* stop) {
   if (gtid() != 0 || *stop) return;
   for (int m = 0; m < numManifolds; ++m) {
     if (D.mColor[m] != colour) continue;
@@ -1748,6 +1805,8 @@ __global__ void k_vsweep_serial(Dev D, Params P, int numManifolds, int colour, i
 // barriers and each sweep's residual is a block reduction; the stop rule is the loop's own.  The
 // result is the same as one launch per colour, without ~(colours x sweeps) launches.
 constexpr int BLOCK_SOLVER_THREADS = 1024;
+// the velocity rows need ~150 VGPRs: 512 threads (2 waves per SIMD) keep them out of scratch
+constexpr int VBLOCK_THREADS = 512;
 constexpr int BLOCK_SOLVER_ROWS = 16384;
 struct ClassStarts {
   int s[65];
@@ -1765,10 +1824,10 @@ __device__ __forceinline__ unsigned long long blockMaxAll(unsigned long long v,
 // MODE 0 = warm start (one pass), 1 = PGS, 2 = one-shot GS.  `run` (optional) gates the whole
 // loop; `stop` / `count` / `bits` are the loop's stop flag, sweep counter and residual.
 template <int MODE>
-__global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_vsolve_block(
+__global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
     Dev D, Params P, ClassStarts cs, int numColors, int numManifolds, int iters, double thr,
     double stopThr, int* stop, int* count, const int* run) {
-  __shared__ unsigned long long sh[BLOCK_SOLVER_THREADS / 64];
+  __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
   if ((run && !*run) || *stop) return;
   for (int it = 0; it < iters; ++it) {
     unsigned long long m = 0;
@@ -1881,10 +1940,10 @@ __global__ __launch_bounds__(TPB) void k_vsweep_perm(Dev D, Params P, const int*
 }
 // the whole ordered loop in one workgroup: buckets ascending then descending per iteration;
 // bucket = (begin, end, colour) into perm
-__global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_ordered_block(
+__global__ __launch_bounds__(VBLOCK_THREADS) void k_ordered_block(
     Dev D, Params P, const int* perm, const int4* buckets, int numBuckets, int iters,
     double thr, int* count) {
-  __shared__ unsigned long long sh[BLOCK_SOLVER_THREADS / 64];
+  __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
   for (int it = 0; it < iters; ++it) {
     unsigned long long m = 0;
     for (int pass = 0; pass < 2; ++pass)

@@ -235,7 +235,8 @@ struct pd_sim {
   DBuf<int> dCnt;
   DBuf<double> cLamN;
   DBuf<int> frCnt;
-  DBuf<int> realIdx, ground, ground2, cellOf, cellCnt, cellStart, order, pairCnt, pairStart;
+  DBuf<int> realIdx, ground, ground2, cellOf, cellRank, cellCnt, cellStart, order, pairCnt,
+      pairStart;
   DBuf<unsigned char> matId, mCode, vFlag;
   DBuf<int> pairStage;  // broadphase partners staged slot-major
   DBuf<int2> pairs, cAB, mAB, mReal, mRange, mRange2, vAB, pAB;
@@ -251,13 +252,14 @@ struct pd_sim {
   DBuf<int> lIdx, bucketCnt, bucketStart, counters, colCnt, colFill, colStart, scanTmp, ghostCnt,
       ghostOff;
   // pinned host scratch
+  static constexpr size_t HOST_SCRATCH_BYTES = 4096;
   void* hostScratch = nullptr;
 
   Dev D{};
 
   pd_sim(int cap) : capacity(cap) {
     HIPCHECK(hipStreamCreate(&st));
-    HIPCHECK(hipHostMalloc(&hostScratch, 4096, hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc(&hostScratch, HOST_SCRATCH_BYTES, hipHostMallocDefault));
     ensureSlots(cap > 0 ? cap : 1, 0);
     red.ensure(8, st);
     bits.ensure(4, st);
@@ -288,7 +290,8 @@ struct pd_sim {
     pos.release(); posPred.release(); quatPred.release(); extForce.release(); shell.release();
     sbox.release(); vw0.release(); vw.release(); bc.release(); scale.release();
     targetScale.release(); pairTableD.release(); thermo.release(); realIdx.release();
-    ground.release(); ground2.release(); cellOf.release(); cellCnt.release(); cellStart.release();
+    ground.release(); ground2.release(); cellOf.release(); cellRank.release(); cellCnt.release();
+    cellStart.release(); gridD.release(); ncellD.release(); cellCntDirty = true;
     order.release(); pairCnt.release(); pairStart.release(); pairStage.release(); matId.release(); mCode.release();
     vFlag.release(); pairs.release(); cAB.release(); mAB.release(); mReal.release();
     mRange.release(); mRange2.release(); cBV.release(); mWV.release(); vWV.release();
@@ -328,8 +331,26 @@ struct pd_sim {
   template <class T>
   void readN(T* host, const T* dptr, size_t n) {
     if (!n) return;
+    if (n * sizeof(T) <= HOST_SCRATCH_BYTES) {
+      // small reads go through the pinned scratch (a direct copy, no staging)
+      HIPCHECK(hipMemcpyAsync(hostScratch, dptr, n * sizeof(T), hipMemcpyDeviceToHost, st));
+      sync();
+      std::memcpy(host, hostScratch, n * sizeof(T));
+      return;
+    }
     HIPCHECK(hipMemcpyAsync(host, dptr, n * sizeof(T), hipMemcpyDeviceToHost, st));
     sync();
+  }
+  // several small reads with one synchronisation: enqueue each into the pinned scratch at
+  // `off` (bytes, advanced), sync() once, then read the returned host pointers
+  template <class T>
+  const T* enqueueRead(const T* dptr, size_t n, size_t& off) {
+    off = (off + 15) & ~(size_t)15;
+    if (off + n * sizeof(T) > HOST_SCRATCH_BYTES) throw std::runtime_error("read-back batch too large");
+    char* h = static_cast<char*>(hostScratch) + off;
+    if (n) HIPCHECK(hipMemcpyAsync(h, dptr, n * sizeof(T), hipMemcpyDeviceToHost, st));
+    off += n * sizeof(T);
+    return reinterpret_cast<const T*>(h);
   }
   template <class T>
   void writeN(T* dptr, const T* host, size_t n) {
@@ -371,8 +392,16 @@ struct pd_sim {
     }
     const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     scanTmp.ensure(nb + 1, st);
-    if (nb > 1) k_scan_reduce<<<nb, TPB, 0, st>>>(in, n, scanTmp.p);
-    k_scan_down2<<<nb, TPB, 0, st>>>(in, n, scanTmp.p, out);
+    if (nb > 1) k_scan_reduce<<<nb, TPB, 0, st>>>(in, n, scanTmp.p, nullptr);
+    k_scan_down2<<<nb, TPB, 0, st>>>(in, n, scanTmp.p, out, nullptr);
+    HIPCHECK(hipGetLastError());
+  }
+  // the same scan over a count that lives on the device (*nDev <= nMax)
+  void scanDev(const int* in, int* out, const int* nDev, int nMax) {
+    const int nb = std::max(1, (nMax + SCAN_TILE - 1) / SCAN_TILE);
+    scanTmp.ensure(nb + 1, st);
+    if (nb > 1) k_scan_reduce<<<nb, TPB, 0, st>>>(in, nMax, scanTmp.p, nDev);
+    k_scan_down2<<<nb, TPB, 0, st>>>(in, nMax, scanTmp.p, out, nDev);
     HIPCHECK(hipGetLastError());
   }
 
@@ -399,7 +428,7 @@ struct pd_sim {
     d.vw = vw.p; d.scale = scale.p; d.targetScale = targetScale.p; d.extForce = extForce.p;
     d.realIdx = realIdx.p; d.matId = matId.p; d.ground = ground.p; d.ground2 = ground2.p;
     d.shell = shell.p; d.pairTable = pairTableD.p;
-    d.cellOf = cellOf.p; d.cellCnt = cellCnt.p; d.cellStart = cellStart.p; d.order = order.p;
+    d.cellOf = cellOf.p; d.cellRank = cellRank.p; d.cellCnt = cellCnt.p; d.cellStart = cellStart.p; d.order = order.p;
     d.sbox = sbox.p; d.pairCnt = pairCnt.p; d.pairStart = pairStart.p; d.pairs = pairs.p;
     d.ncnt = ncnt.p; d.noff = noff.p; d.cAB = cAB.p; d.cN = cN.p; d.cRA = cRA.p; d.cRB = cRB.p;
     d.cSlot = cSlot.p; d.mCnt = mCnt.p; d.mStart = mStart.p; d.mAB = mAB.p; d.mReal = mReal.p;
@@ -664,60 +693,45 @@ struct pd_sim {
   // uniform-grid broadphase over posPred (.w = radius) of numParticles slots with P.margin;
   // fills pairs / pairStart / np
   void broadphase(const Params& P) {
-    // bounds of the box centres -> uniform grid with cell edge >= 2 * max half-width
+    // bounds of the box centres -> uniform grid (computed on the device, no read-back)
     const int nbb = std::min(nblk(numParticles), RED_BLOCKS);
+    const long cellCap = 4L * numParticles + 64 + 1;  // the grid keeps cells <= 4 x slots + 64
     redPart.ensure(8 * RED_BLOCKS, st);
-    k_bp_bounds<<<nbb, TPB, 0, st>>>(D, P, redPart.p);
-    k_bp_bounds_final<<<1, TPB, 0, st>>>(redPart.p, nbb, red.p);
-    unsigned long long rk[7];
-    readN(rk, red.p, 7);
-    double lo[3], hi[3];
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = rk[d] == ~0ull ? 0.0 : ordVal(rk[d]);
-      hi[d] = rk[3 + d] == 0 ? 0.0 : ordVal(rk[3 + d]);
-      if (lo[d] > hi[d]) lo[d] = hi[d] = 0.0;
-    }
-    const double maxB = rk[6] ? ordVal(rk[6]) : 0.0;
-    Grid G{};
-    G.lo = d3{lo[0], lo[1], lo[2]};
-    G.h = 2.0 * maxB * 1.0001 + 1e-300;
-    double cells;
-    for (;;) {
-      cells = 1.0;
-      for (int d = 0; d < 3; ++d) {
-        double nd = std::floor((hi[d] - lo[d]) / G.h) + 1.0;
-        if (!(nd >= 1.0)) nd = 1.0;
-        if (nd > 1e9) nd = 1e9;
-        G.dim[d] = (int)nd;
-        cells *= nd;
-      }
-      if (cells <= 4.0 * numParticles + 64.0) break;
-      G.h *= 1.5;
-    }
-    const int ncell = G.dim[0] * G.dim[1] * G.dim[2];
-    cellCnt.ensure(ncell + 1, st);
-    cellStart.ensure(ncell + 1, st);
+    gridD.ensure(1, st);
+    ncellD.ensure(1, st);
+    cellCnt.ensure(cellCap, st);
+    cellStart.ensure(cellCap + 1, st);
     cellOf.ensure(numParticles, st);
+    cellRank.ensure(numParticles, st);
     order.ensure(numParticles, st);
     sbox.ensure(numParticles, st);
     pairCnt.ensure(numReal + 1, st);
     pairStart.ensure(numReal + 1, st);
     refreshDev();
-    zero(cellCnt.p, ncell);
-    k_cell_count<<<nblk(numParticles), TPB, 0, st>>>(D, P, G);
-    scanQuiet(cellCnt.p, cellStart.p, ncell);
-    zero(cellCnt.p, ncell);
+    // cellCnt is all zero between broadphases (k_cell_fill clears what k_cell_count set); a pass
+    // cut short by an error leaves it dirty
+    if (cellCntDirty) zero(cellCnt.p, cellCnt.cap);
+    cellCntDirty = true;
+    k_bp_bounds<<<nbb, TPB, 0, st>>>(D, P, redPart.p);
+    k_bp_bounds_final<<<1, TPB, 0, st>>>(redPart.p, nbb, red.p, numParticles, gridD.p, ncellD.p);
+    k_cell_count<<<nblk(numParticles), TPB, 0, st>>>(D, P, gridD.p);
+    scanDev(cellCnt.p, cellStart.p, ncellD.p, (int)cellCap);
     k_cell_fill<<<nblk(numParticles), TPB, 0, st>>>(D, P);
+    HIPCHECK(hipGetLastError());
+    cellCntDirty = false;
     pairStage.ensure((size_t)numReal * BP_CAP + 1, st);
     kernBegin();
-    k_bp_query1<<<nblk(numParticles), TPB, 0, st>>>(D, P, G, pairStage.p);
+    k_bp_query1<<<nblk(numParticles), TPB, 0, st>>>(D, P, gridD.p, pairStage.p);
     kernEnd(K_BPQUERY, numParticles);
     np = scan(pairCnt.p, pairStart.p, numReal);
     pairs.ensure(np + 1, st);
     refreshDev();
-    k_bp_compact<<<nblk(numReal), TPB, 0, st>>>(D, P, G, pairStage.p);
+    k_bp_compact<<<nblk(numReal), TPB, 0, st>>>(D, P, gridD.p, pairStage.p);
     HIPCHECK(hipGetLastError());
   }
+  DBuf<Grid> gridD;
+  DBuf<int> ncellD;
+  bool cellCntDirty = true;
 
   // narrowphase: counts, scan, write (pairs first, then planes per real)
   WallSet wallSet() const {
@@ -777,45 +791,71 @@ struct pd_sim {
     HIPCHECK(hipGetLastError());
   }
 
-  // Jones-Plassmann rounds over a shrinking work list (the uncoloured edges, already in jpWork[0]
-  // with `entering` entries).  Rounds run in batches: round j of a batch reads its list size from
-  // jpCnt[j] and appends the losers to the other list, counting into jpCnt[j + 1]; the host reads
-  // the counts once per batch and replays the reference loop (src/solver_velocity.hpp:305-355) on
-  // them.  Every round colours at least the globally highest-priority edge, so rounds after the
-  // list empties are no-ops.  Returns the uncoloured leftovers after the round cap.
+  // Jones-Plassmann colouring then the colour classes, with one host read per batch of rounds.
+  // The uncoloured edges are in jpWork[0], their count in *enteringDev (device) or enteringHost.
+  // Round j of a batch reads its list size from jpCnt[j] and appends the losers to the other list,
+  // counting into jpCnt[j + 1]; every round colours at least the globally highest-priority edge,
+  // so rounds after the list empties are no-ops.  The class kernels are enqueued behind every
+  // batch, so the batch that finishes the colouring also brings back the class starts.  The batch
+  // and grid sizes come from the previous colouring of the same kind (the round kernels loop over
+  // their lists, so any grid is correct); the host replays the reference loop
+  // (src/solver_velocity.hpp:305-355) on the counts.  Returns the uncoloured leftovers after the
+  // round cap.
   DBuf<int> jpWork[2];
   DBuf<int> jpCnt;
-  int jpColor(const int2* ends, int* colour, int numEdges, int numBodies, int entering,
-              int& rounds, long long& uncol) {
+  std::vector<int> jpHist[2];  // per kind: list size entering each round, last colouring
+  static constexpr int JP_MAX_BATCH = 32;
+  static constexpr long JP_GRID_CAP = 2048;
+  int colourEdges(int kind, const int2* ends, int* colour, int numEdges, int numBodies,
+                  const int* enteringDev, int enteringHost, int* perm, int* rowOf,
+                  std::vector<int>& starts, int& numColors, int& rounds, long long& uncol) {
     winner.ensure(numBodies + 1, st);
     mask.ensure(numBodies + 1, st);
-    jpCnt.ensure(17, st);
+    jpCnt.ensure(JP_MAX_BATCH + 1, st);
+    blockCnt.ensure(64 * CP_BLOCKS, st);
     refreshDev();
     zero(mask.p, numBodies);
     zero(winner.p, numBodies);
+    std::vector<int>& hist = jpHist[kind];
+    std::vector<int> seen;
     int remaining = 1;  // the reference starts with remaining = 1: one round always runs
     uncol = 0;
     rounds = 0;
     int prevRemaining = -1;
     int cur = 0;
     const long maxRounds = (long)numBodies + 2;
-    int batch = 4;
-    int hostCnt[17];
-    while (remaining > 0 && rounds < maxRounds) {
+    int batch = std::min<int>(JP_MAX_BATCH, std::max<int>(4, (int)hist.size() + 2));
+    long entering = enteringDev ? -1 : enteringHost;
+    const int nbc = std::min(nblk(numEdges), CP_BLOCKS);
+    for (bool first = true;; first = false) {
       const int B = (int)std::min<long>(batch, maxRounds - rounds);
-      k_set_counts<<<1, 64, 0, st>>>(jpCnt.p, entering, B + 1);
-      const int grid = nblk(entering);
+      k_jp_counts<<<1, 64, 0, st>>>(jpCnt.p, first ? enteringDev : nullptr, (int)entering, B + 1);
       for (int j = 0; j < B; ++j) {
+        const int r = rounds + j;
+        long est = numEdges;
+        if (r < (int)hist.size()) est = hist[r];
+        else if (j == 0 && entering >= 0) est = entering;
+        else if (!hist.empty()) est = hist.back();
+        const int grid = (int)std::min<long>(nblk(est + est / 4 + 64), JP_GRID_CAP);
         const int* work = jpWork[(cur + j) & 1].p;
         int* next = jpWork[(cur + j + 1) & 1].p;
         k_jp_contend<<<grid, TPB, 0, st>>>(ends, work, jpCnt.p + j, winner.p);
         k_jp_commit<<<grid, TPB, 0, st>>>(ends, colour, work, jpCnt.p + j, winner.p, mask.p, next,
                                           jpCnt.p + j + 1);
       }
+      // colour classes: counts, per-block bases, starts, placement
+      k_color_count<<<nbc, TPB, 0, st>>>(colour, numEdges, blockCnt.p);
+      k_color_offsets<<<64, TPB, 0, st>>>(blockCnt.p, nbc, colCnt.p);
+      k_color_starts<<<1, 64, 0, st>>>(colCnt.p, colStart.p);
+      k_color_place<<<nbc, TPB, 0, st>>>(colour, numEdges, blockCnt.p, colStart.p, perm, rowOf);
       HIPCHECK(hipGetLastError());
-      readN(hostCnt, jpCnt.p, B + 1);
+      size_t off = 0;
+      const int* hostCnt = enqueueRead(jpCnt.p, B + 1, off);
+      const int* hostStarts = enqueueRead(colStart.p, 65, off);
+      sync();
       for (int j = 0; j < B && remaining > 0; ++j) {
         uncol += hostCnt[j];
+        seen.push_back(hostCnt[j]);
         ++rounds;
         const int rem = hostCnt[j + 1];
         if (rem == prevRemaining && rem > 0)
@@ -825,26 +865,18 @@ struct pd_sim {
       }
       entering = hostCnt[B];
       cur = (cur + B) & 1;
-      batch = std::min(batch * 2, 16);
+      if (remaining <= 0 || rounds >= maxRounds) {
+        starts.assign(hostStarts, hostStarts + 65);
+        break;
+      }
+      batch = std::min(batch * 2, JP_MAX_BATCH);
     }
-    return remaining > 0 ? entering : 0;
-  }
-  // colour classes: counts, starts, rows; returns maxColour+1 and fills the class starts
-  int colourClasses(const int* colour, int numEdges, int* perm, int* rowOf,
-                    std::vector<int>& starts) {
-    const int nb = std::min(nblk(numEdges), CP_BLOCKS);
-    blockCnt.ensure(64 * CP_BLOCKS, st);
-    k_color_count<<<nb, TPB, 0, st>>>(colour, numEdges, blockCnt.p);
-    k_color_offsets<<<64, TPB, 0, st>>>(blockCnt.p, nb, colCnt.p);
-    k_color_starts<<<1, 64, 0, st>>>(colCnt.p, colStart.p);
-    k_color_place<<<nb, TPB, 0, st>>>(colour, numEdges, blockCnt.p, colStart.p, perm, rowOf);
-    HIPCHECK(hipGetLastError());
-    starts.assign(65, 0);
-    readN(starts.data(), colStart.p, 65);
+    hist.swap(seen);
     int maxc = -1;
     for (int c = 0; c < 64; ++c)
       if (starts[c + 1] > starts[c]) maxc = c;
-    return maxc + 1;
+    numColors = maxc + 1;
+    return remaining > 0 ? (int)entering : 0;
   }
 
   // Enqueue up to `iters` device-gated sweeps in geometric batches (1, 2, 4, ...), reading the
@@ -900,7 +932,7 @@ struct pd_sim {
       lvlBuckets.ensure(nb + 1, st);
       if (nb) writeN(lvlBuckets.p, bk.data(), nb);
       zero(ctl.p + CTL_SWEEPS_S, 1);
-      k_ordered_block<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(D, P, lvlPerm.p, lvlBuckets.p, nb,
+      k_ordered_block<<<1, VBLOCK_THREADS, 0, st>>>(D, P, lvlPerm.p, lvlBuckets.p, nb,
                                                           2 * velIters, vRestS,
                                                           ctl.p + CTL_SWEEPS_S);
       stats.sweeps_stab = read1(ctl.p + CTL_SWEEPS_S);
@@ -1028,10 +1060,8 @@ struct pd_sim {
     int numColors = 0;
     int velLeftover = 0;
     if (useGS && nm > 0 && numReal > 0) {
-      const int entering = read1(counters.p + 1);
-      velLeftover = jpColor(mReal.p, mColor.p, nm, numReal, entering, stats.rounds_vel,
-                            stats.uncolored_vel);
-      numColors = colourClasses(mColor.p, nm, vPerm.p, vRowOf.p, vs);
+      velLeftover = colourEdges(0, mReal.p, mColor.p, nm, numReal, counters.p + 1, 0, vPerm.p,
+                                vRowOf.p, vs, numColors, stats.rounds_vel, stats.uncolored_vel);
     } else {
       vs.assign(65, 0);
     }
@@ -1068,7 +1098,7 @@ struct pd_sim {
       }
       // 8 swaps: the result is back in `ground`
       if (nRowsV <= blockRows)
-        k_vsolve_block<0><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(D, P, classStarts(vs), numColors, nm,
+        k_vsolve_block<0><<<1, VBLOCK_THREADS, 0, st>>>(D, P, classStarts(vs), numColors, nm,
                                                              1, 0.0, 0.0, ctl.p + CTL_NONE,
                                                              ctl.p + CTL_NONE, nullptr);
       else
@@ -1088,11 +1118,11 @@ struct pd_sim {
       if (smallV && velIters > 0) {
         kernBegin();
         if (usePGS)
-          k_vsolve_block<1><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(
+          k_vsolve_block<1><<<1, VBLOCK_THREADS, 0, st>>>(
               D, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
               ctl.p + CTL_SWEEPS_V, nullptr);
         else
-          k_vsolve_block<2><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(
+          k_vsolve_block<2><<<1, VBLOCK_THREADS, 0, st>>>(
               D, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
               ctl.p + CTL_SWEEPS_V, nullptr);
         vBlockRec[0] = kernEnd(K_VBLOCK, nRowsV);
@@ -1112,7 +1142,7 @@ struct pd_sim {
               k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
                                                          8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
             kernBegin();
-            k_vsolve_block<1><<<1, BLOCK_SOLVER_THREADS, 0, st>>>(
+            k_vsolve_block<1><<<1, VBLOCK_THREADS, 0, st>>>(
                 D, P, classStarts(vs), numColors, nm, 2 * velIters, vRestS, vRestS,
                 ctl.p + CTL_STOP_S, ctl.p + CTL_SWEEPS_S, ctl.p + CTL_STAB_RUN);
             vBlockRec[1] = kernEnd(K_VBLOCK, nRowsV);
@@ -1230,9 +1260,9 @@ struct pd_sim {
     int posLeftover = 0;
     if (useGS && nc > 0 && numParticles > 0) {
       k_jp_init_contacts<<<nblk(nc), TPB, 0, st>>>(cColor.p, nc, jpWork[0].p);
-      posLeftover = jpColor(cAB.p, cColor.p, nc, numParticles, nc, stats.rounds_pos,
-                            stats.uncolored_pos);
-      numPosColors = colourClasses(cColor.p, nc, pPerm.p, pRowOf.p, ps);
+      posLeftover = colourEdges(1, cAB.p, cColor.p, nc, numParticles, nullptr, nc, pPerm.p,
+                                pRowOf.p, ps, numPosColors, stats.rounds_pos,
+                                stats.uncolored_pos);
     } else {
       ps.assign(65, 0);
     }
@@ -1303,10 +1333,9 @@ struct pd_sim {
     }
     {
       // one read-back of the loop counters and the residuals
-      int c[CTL_COUNT];
-      unsigned long long b2[2];
-      HIPCHECK(hipMemcpyAsync(c, ctl.p, sizeof c, hipMemcpyDeviceToHost, st));
-      HIPCHECK(hipMemcpyAsync(b2, bits.p, sizeof b2, hipMemcpyDeviceToHost, st));
+      size_t off = 0;
+      const int* c = enqueueRead(ctl.p, CTL_COUNT, off);
+      const unsigned long long* b2 = enqueueRead(bits.p, 2, off);
       sync();
       if (!hostLoop) {
         stats.sweeps_vel = c[CTL_SWEEPS_V];
