@@ -1119,6 +1119,86 @@ __global__ void k_jp_commit(const int2* ends, int* colour, const int* work, cons
     blockAppend(lose, e, next, nextCount, &shCnt, &shBase);
   }
 }
+// The small rounds of a colouring inside one workgroup: the contend and commit phases of each
+// round are separated by workgroup barriers instead of launches.  Round j reads its list from
+// `in` (rounds alternate between the two lists) with its size in cnt[j] and appends the losers
+// to the other list, counting into cnt[j + 1]; it stops at round j1 or once a list is empty.
+// Values other threads of the workgroup wrote or updated atomically in earlier phases are read
+// with device-scope atomic loads (from L2), so no stale L1 line is ever used.
+constexpr int JP_TAIL_THREADS = 1024;
+__device__ __forceinline__ int ldL2(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ldL2(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stL2(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// workgroup barrier after every earlier memory operation of the wave has completed (atomics
+// and stores included: the plain barrier does not wait for them)
+__device__ __forceinline__ void waitBarrier() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+}
+__global__ __launch_bounds__(JP_TAIL_THREADS) void k_jp_tail(const int2* ends, int* colour,
+                                                             int* in, int* out, int* cnt, int j0,
+                                                             int j1, unsigned long long* winner,
+                                                             unsigned long long* mask) {
+  __shared__ int shCnt, shBase;
+  for (int j = j0; j < j1; ++j) {
+    const int n = ldL2(cnt + j);
+    if (n == 0) break;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int e = ldL2(in + t);
+      const int2 ab = ends[e];
+      const unsigned long long key = colorKey1(e);
+      atomicMax(&winner[ab.x], key);
+      if (ab.y >= 0) atomicMax(&winner[ab.y], key);
+    }
+    waitBarrier();
+    for (int base = 0; base < n; base += blockDim.x) {
+      const int t = base + threadIdx.x;
+      int e = -1;
+      bool lose = false;
+      if (t < n) {
+        e = ldL2(in + t);
+        const int2 ab = ends[e];
+        const unsigned long long key = colorKey1(e);
+        const unsigned long long wa = ldL2(&winner[ab.x]);
+        const unsigned long long wb = ab.y >= 0 ? ldL2(&winner[ab.y]) : key;
+        if (wa != key || wb != key) {
+          lose = true;
+        } else {
+          unsigned long long forbidden = ldL2(&mask[ab.x]);
+          if (ab.y >= 0) forbidden |= ldL2(&mask[ab.y]);
+          int c = 0;
+          while (c < 62 && (forbidden & (1ull << c))) ++c;
+          colour[e] = c;
+          stL2(&mask[ab.x], ldL2(&mask[ab.x]) | (1ull << c));
+          stL2(&winner[ab.x], 0ull);
+          if (ab.y >= 0) {
+            stL2(&mask[ab.y], ldL2(&mask[ab.y]) | (1ull << c));
+            stL2(&winner[ab.y], 0ull);
+          }
+        }
+      }
+      // append the losers: block-aggregated slots, stores that go to L2
+      if (threadIdx.x == 0) shCnt = 0;
+      waitBarrier();
+      const int local = lose ? atomicAdd(&shCnt, 1) : 0;
+      waitBarrier();
+      if (threadIdx.x == 0 && shCnt) shBase = atomicAdd(cnt + j + 1, shCnt);
+      waitBarrier();
+      if (lose)
+        __hip_atomic_store(out + shBase + local, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    waitBarrier();
+    int* sw = in;
+    in = out;
+    out = sw;
+  }
+}
 // round counters of a batch: cnt[0] = the entering list size (from src when given), rest 0
 __global__ void k_jp_counts(int* cnt, const int* src, int first, int n) {
   const int i = threadIdx.x;
@@ -1805,8 +1885,9 @@ __global__ __launch_bounds__(64) void k_vsweep_serial(Dev D, Params P, int numMa
 // barriers and each sweep's residual is a block reduction; the stop rule is the loop's own.  The
 // result is the same as one launch per colour, without ~(colours x sweeps) launches.
 constexpr int BLOCK_SOLVER_THREADS = 1024;
-// the velocity rows need ~150 VGPRs: 512 threads (2 waves per SIMD) keep them out of scratch
-constexpr int VBLOCK_THREADS = 512;
+// the velocity rows need ~160 VGPRs: 768 threads (3 waves per SIMD, <= 168 VGPRs) keep them out
+// of scratch
+constexpr int VBLOCK_THREADS = 768;
 constexpr int BLOCK_SOLVER_ROWS = 16384;
 struct ClassStarts {
   int s[65];

@@ -753,7 +753,8 @@ struct pd_sim {
     if (waveNp) k_np_pairs_wave<false><<<npBlocks, TPB, 0, st>>>(D, P, np);
     else k_np_pairs<false><<<nblk(np), TPB, 0, st>>>(D, P, np);
     k_np_planes<false><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
-    k_np_walls<false><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
+    if (W.n > 0) k_np_walls<false><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
+    else zero(ncnt.p + np + numReal, numReal);  // no walls: every wall slot holds 0 contacts
     nc = scan(ncnt.p, noff.p, nslots);
     cAB.ensure(nc + 1, st);
     cBV.ensure(nc + 1, st);
@@ -806,6 +807,7 @@ struct pd_sim {
   std::vector<int> jpHist[2];  // per kind: list size entering each round, last colouring
   static constexpr int JP_MAX_BATCH = 32;
   static constexpr long JP_GRID_CAP = 2048;
+  static constexpr long JP_TAIL_ROWS = 4096;  // rounds expected this small run in one workgroup
   int colourEdges(int kind, const int2* ends, int* colour, int numEdges, int numBodies,
                   const int* enteringDev, int enteringHost, int* perm, int* rowOf,
                   std::vector<int>& starts, int& numColors, int& rounds, long long& uncol) {
@@ -836,9 +838,15 @@ struct pd_sim {
         if (r < (int)hist.size()) est = hist[r];
         else if (j == 0 && entering >= 0) est = entering;
         else if (!hist.empty()) est = hist.back();
-        const int grid = (int)std::min<long>(nblk(est + est / 4 + 64), JP_GRID_CAP);
-        const int* work = jpWork[(cur + j) & 1].p;
+        int* work = jpWork[(cur + j) & 1].p;
         int* next = jpWork[(cur + j + 1) & 1].p;
+        if (est <= JP_TAIL_ROWS) {
+          // the rest of the batch is expected to be small: one workgroup runs those rounds
+          k_jp_tail<<<1, JP_TAIL_THREADS, 0, st>>>(ends, colour, work, next, jpCnt.p, j, B,
+                                                   winner.p, mask.p);
+          break;
+        }
+        const int grid = (int)std::min<long>(nblk(est + est / 4 + 64), JP_GRID_CAP);
         k_jp_contend<<<grid, TPB, 0, st>>>(ends, work, jpCnt.p + j, winner.p);
         k_jp_commit<<<grid, TPB, 0, st>>>(ends, colour, work, jpCnt.p + j, winner.p, mask.p, next,
                                           jpCnt.p + j + 1);
