@@ -265,8 +265,10 @@ __global__ void k_scan_reduce(const int* in, int n, int* blockSums, const int* n
 // scan of one tile; the tile's offset is the block-reduced sum of the earlier tiles' totals
 // (written by k_scan_reduce), so no separate top-level pass is needed. The last tile writes
 // the grand total to out[n].
+// inv (optional): for the elements below invLimit, which hold 0 or 1, inv[out[k]] = k for every
+// element holding 1 (the compaction of those elements)
 __global__ void k_scan_down2(const int* in, int n, const int* blockSums, int* out,
-                             const int* nDev) {
+                             const int* nDev, int* inv = nullptr, int invLimit = 0) {
   __shared__ int sh[TPB];
   if (nDev) n = *nDev;
   const int last = n > 0 ? (n - 1) / SCAN_TILE : 0;
@@ -292,6 +294,7 @@ __global__ void k_scan_down2(const int* in, int n, const int* blockSums, int* ou
   int ex = blockExclusiveScan(s, sh, &tot) + prefix;
   for (int k = 0; k < SCAN_ITEMS; ++k) {
     if (base + k < n) out[base + k] = ex;
+    if (inv && base + k < invLimit && vals[k]) inv[ex] = base + k;
     ex += vals[k];
   }
   if ((int)blockIdx.x == last && threadIdx.x == 0) out[n] = prefix + tot;
@@ -832,6 +835,16 @@ __global__ __launch_bounds__(TPB) void k_np_pairs(Dev D, Params P, int numPairs)
   const int p = gtid();
   const double ovl = (p < numPairs) ? npPair<WRITE>(D, P, p) : 0.0;
   if (WRITE) blockAtomicMaxU64(D.maxOverlapBits, dbits(ovl), sh);
+}
+// Write pass of point shapes (at most one contact per pair) over the contacts only: contact c
+// comes from pair pairOf[c] (the compaction written by the slot scan) and lands at its offset
+// noff[pairOf[c]] = c, as in the full pass.  The pair contacts are the first noff[numPairs].
+__global__ __launch_bounds__(TPB) void k_np_pairs_sparse(Dev D, Params P, int numPairs,
+                                                         const int* pairOf) {
+  __shared__ unsigned long long sh[WAVES];
+  const int c = gtid();
+  const double ovl = (c < D.noff[numPairs]) ? npPair<true>(D, P, pairOf[c]) : 0.0;
+  blockAtomicMaxU64(D.maxOverlapBits, dbits(ovl), sh);
 }
 
 // Shell shapes: one wavefront per candidate pair.  The block stages the point shell in LDS; the

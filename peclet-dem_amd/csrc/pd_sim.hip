@@ -239,6 +239,7 @@ struct pd_sim {
       pairStart;
   DBuf<unsigned char> matId, mCode, vFlag;
   DBuf<int> pairStage;  // broadphase partners staged slot-major
+  DBuf<int> pairOfContact;  // point shapes: the pair of each pair contact
   DBuf<int2> pairs, cAB, mAB, mReal, mRange, mRange2, vAB, pAB;
   DBuf<int> ncnt, noff, cSlot, mCnt, mStart, mNp, mColor, cColor, vPerm, vRowOf, pPerm, pRowOf;
   DBuf<double4> cN, cRA, cRB, cBV, mNs, mTA, mTB, mRA, mRB, mWV, vN, vTA, vTB, vRA, vRB, vVt0,
@@ -292,7 +293,8 @@ struct pd_sim {
     targetScale.release(); pairTableD.release(); thermo.release(); realIdx.release();
     ground.release(); ground2.release(); cellOf.release(); cellRank.release(); cellCnt.release();
     cellStart.release(); gridD.release(); ncellD.release(); cellCntDirty = true;
-    order.release(); pairCnt.release(); pairStart.release(); pairStage.release(); matId.release(); mCode.release();
+    order.release(); pairCnt.release(); pairStart.release(); pairStage.release();
+    pairOfContact.release(); matId.release(); mCode.release();
     vFlag.release(); pairs.release(); cAB.release(); mAB.release(); mReal.release();
     mRange.release(); mRange2.release(); cBV.release(); mWV.release(); vWV.release();
     vAB.release(); pAB.release(); ncnt.release(); noff.release();
@@ -755,7 +757,19 @@ struct pd_sim {
     k_np_planes<false><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
     if (W.n > 0) k_np_walls<false><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
     else zero(ncnt.p + np + numReal, numReal);  // no walls: every wall slot holds 0 contacts
-    nc = scan(ncnt.p, noff.p, nslots);
+    if (waveNp) {
+      nc = scan(ncnt.p, noff.p, nslots);
+    } else {
+      // point shapes: the scan also lists the pairs in contact (their counts are 0 or 1)
+      pairOfContact.ensure(np + 1, st);
+      const int nb = (nslots + SCAN_TILE - 1) / SCAN_TILE;
+      scanTmp.ensure(nb + 1, st);
+      if (nb > 1) k_scan_reduce<<<nb, TPB, 0, st>>>(ncnt.p, nslots, scanTmp.p, nullptr);
+      k_scan_down2<<<nb, TPB, 0, st>>>(ncnt.p, nslots, scanTmp.p, noff.p, nullptr,
+                                       pairOfContact.p, np);
+      HIPCHECK(hipGetLastError());
+      nc = read1(noff.p + nslots);
+    }
     cAB.ensure(nc + 1, st);
     cBV.ensure(nc + 1, st);
     cN.ensure(nc + 1, st);
@@ -766,8 +780,8 @@ struct pd_sim {
     zero(bits.p, 1);
     kernBegin();
     if (waveNp) k_np_pairs_wave<true><<<npBlocks, TPB, 0, st>>>(D, P, np);
-    else k_np_pairs<true><<<nblk(np), TPB, 0, st>>>(D, P, np);
-    kernEnd(K_NPPAIRS, np);
+    else if (nc > 0) k_np_pairs_sparse<<<nblk(nc), TPB, 0, st>>>(D, P, np, pairOfContact.p);
+    kernEnd(K_NPPAIRS, waveNp ? np : nc);
     k_np_planes<true><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
     if (W.n > 0) k_np_walls<true><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
     HIPCHECK(hipGetLastError());
