@@ -17,9 +17,13 @@ The JSON line carries:
                   k_psweep            388 per contact row
                   k_bp_query1         36 per slot (order + box) + 4 per real (count)
                                       + 8 per candidate pair staged
-                  k_np_pairs          156 per pair (pair, offset, both bodies' pose and scale)
-                                      + 104 per contact written
-                  k_predict           240 per real;  k_reintegrate  264 per slot
+                  k_np_pairs          (write pass) 156 per pair processed (pair, offset, both
+                                      bodies' pose and scale) + 104 per contact written; for
+                                      point shapes it processes only the pairs in contact
+                  k_predict           332 per real (bc, vw0, force, pos, scale in; vw,
+                                      posPred, quatPred, real index out)
+                  k_reintegrate       344 per slot (vw0, vw, bc, pos in; vw0, posPred.xyz,
+                                      quatPred out)
                   k_vsolve_block      601 per manifold row per sweep (one-workgroup loop)
                   k_psolve_block      388 per contact row per sweep
                 The timed loop is only sim.step(dt): the library resolves its events lazily and
@@ -54,7 +58,7 @@ def substep_bytes(st):
     N = st["num_real"]
     G = st["num_particles"] - N
     P, Cn, M = st["num_pairs"], st["num_contacts"], st["num_manifolds"]
-    b = 684 * N + 460 * G + 152 * P + 236 * Cn + 1661 * M
+    b = 872 * N + 540 * G + 152 * P + 236 * Cn + 1661 * M
     b += 88 * st["uncolored_vel"] + 8 * N * st["rounds_vel"]
     b += 80 * st["uncolored_pos"] + 8 * (N + G) * st["rounds_pos"]
     b += 601 * (st["sweeps_vel"] + st["sweeps_stab"]) * st["active_manifolds"]
@@ -74,9 +78,9 @@ def kernel_bytes(name, units, log):
     if name == "k_np_pairs":
         return 156 * units + 104 * sum(s["num_contacts"] for s in log)
     if name == "k_predict":
-        return 240 * units
+        return 332 * units
     if name == "k_reintegrate":
-        return 264 * units
+        return 344 * units
     raise KeyError(name)
 
 
