@@ -15,7 +15,7 @@ The JSON line carries:
                 Algorithmic bytes per unit (reads + writes once, FP64, int32 indices):
                   k_vsweep (PGS)      601 per active manifold row
                   k_psweep            388 per contact row
-                  k_bp_query1         36 per slot (order + box) + 4 per real (count)
+                  k_bp_query         36 per slot (order + box) + 4 per real (count)
                                       + 8 per candidate pair staged
                   k_np_pairs          (write pass) 156 per pair processed (pair, offset, both
                                       bodies' pose and scale) + 104 per contact written; for
@@ -73,7 +73,7 @@ def kernel_bytes(name, units, log):
         return 601 * units
     if name in ("k_psweep", "k_psolve_block"):  # contact rows (x sweeps)
         return 388 * units
-    if name == "k_bp_query1":  # slots queried, plus each real's count and pair partners
+    if name == "k_bp_query":  # slots queried, plus each real's count and pair partners
         return 36 * units + sum(4 * s["num_real"] + 8 * s["num_pairs"] for s in log)
     if name == "k_np_pairs":
         return 156 * units + 104 * sum(s["num_contacts"] for s in log)

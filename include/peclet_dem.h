@@ -201,7 +201,7 @@ int pd_set_tuning(pd_sim* s, const char* name, long value);
 int pd_get_stage_times(pd_sim* s, double* out_ms, int cap);
 const char* pd_stage_name(int stage);
 /* per timed kernel k: summed ms, launch count and units processed (k_vsweep: manifold rows,
- * k_psweep: contact rows, k_bp_query1: slots queried, k_np_pairs: pairs, k_predict: particles,
+ * k_psweep: contact rows, k_bp_query: slots queried, k_np_pairs: pairs, k_predict: particles,
  * k_reintegrate: slots, k_vsolve_block: manifold rows x sweeps, k_psolve_block: contact rows x
  * sweeps); returns the number of timed kernels */
 int pd_get_kernel_times(pd_sim* s, double* ms, long* launches, long* rows, int cap);

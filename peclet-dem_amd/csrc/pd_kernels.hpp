@@ -691,17 +691,11 @@ __global__ void k_cell_fill(Dev D, Params P) {
   D.sbox[k] = saneBox(D.posPred[s], P.margin, P.boxCap);
   D.cellCnt[c] = 0;
 }
-// Single-pass query.  Threads walk the cell-sorted order, so a wave's particles share their
-// 27-cell neighbourhood (cache-resident reads, little divergence).  Each real stages up to
-// BP_CAP partners in pairTmp[i * BP_CAP ...] and records its full count; k_bp_compact then
-// moves every segment to its scanned offset in ascending partner order.  A real with more than
-// BP_CAP partners is queried again by k_bp_compact, writing straight into the final list.
-constexpr int BP_CAP = 32;
-// STAGE: partner j of staged slot n goes to stage[n * stride] (slot-major, so the compaction
-// reads coalesce); otherwise the pair (i, j) goes to out[n]
-template <bool STAGE>
+constexpr int BP_CAP = 32;  // staged partners per real; a real with more is queried again
+// Full-stencil query of one real i: every partner j > i in the 27 neighbouring cells, written as
+// pairs (i, j) to out[0..n) (cell order; k_bp_compact sorts the segment)
 __device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, const double4 Bi,
-                                          int* stage, int stride, int2* out) {
+                                          int2* out) {
   const int cx = cellCoord(Bi.x, G.lo.x, G.h, G.dim[0]);
   const int cy = cellCoord(Bi.y, G.lo.y, G.h, G.dim[1]);
   const int cz = cellCoord(Bi.z, G.lo.z, G.h, G.dim[2]);
@@ -716,26 +710,51 @@ __device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, co
       const int k0 = D.cellStart[row + x0], k1 = D.cellStart[row + x1 + 1];
       for (int k = k0; k < k1; ++k) {
         const int j = D.order[k];
-        if (j > i && boxOverlap(Bi, D.sbox[k])) {
-          if (STAGE) {
-            if (n < BP_CAP) stage[(size_t)n * stride] = j;
-          } else {
-            out[n] = make_int2(i, j);
-          }
-          ++n;
-        }
+        if (j > i && boxOverlap(Bi, D.sbox[k])) out[n++] = make_int2(i, j);
       }
     }
   }
   return n;
 }
-__global__ __launch_bounds__(TPB) void k_bp_query1(Dev D, Params P, const Grid* grid, int* stage) {
+// Half-stencil query: every slot (real or ghost) tests only its forward neighbourhood -- the rest
+// of its own cell and the next cell in x (one contiguous range in cell order), the row y+1 at the
+// same z, and the three rows at z+1 -- so each neighbouring pair of slots is tested once (box
+// overlap is symmetric).  The pair belongs to its lower slot, which must be real; the partner is
+// appended to that real's staged row with an atomic count.  pairCnt is zero on entry.  A real
+// with more than BP_CAP partners is queried again in full by k_bp_compact.
+__device__ __forceinline__ void bpStage(const Dev& D, int a, int b, int numReal, int* stage) {
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  if (lo >= numReal) return;  // two ghosts
+  const int n = atomicAdd(&D.pairCnt[lo], 1);
+  if (n < BP_CAP) stage[(size_t)n * numReal + lo] = hi;
+}
+__global__ __launch_bounds__(TPB) void k_bp_query(Dev D, Params P, const Grid* grid, int* stage) {
   const int k = gtid();
   if (k >= P.numParticles) return;
   const Grid G = *grid;
-  const int i = D.order[k];
-  if (i >= P.numReal) return;
-  D.pairCnt[i] = bpQueryOne<true>(D, G, i, D.sbox[k], stage + i, P.numReal, nullptr);
+  const int s = D.order[k];
+  const double4 Bs = D.sbox[k];
+  const int cx = cellCoord(Bs.x, G.lo.x, G.h, G.dim[0]);
+  const int cy = cellCoord(Bs.y, G.lo.y, G.h, G.dim[1]);
+  const int cz = cellCoord(Bs.z, G.lo.z, G.h, G.dim[2]);
+  const int x0 = cx > 0 ? cx - 1 : 0;
+  const int x1 = cx + 1 < G.dim[0] ? cx + 1 : G.dim[0] - 1;
+  const int row = (cz * G.dim[1] + cy) * G.dim[0];
+  // own cell after this slot, then cell x+1
+  {
+    const int e = D.cellStart[row + x1 + 1];
+    for (int q = k + 1; q < e; ++q)
+      if (boxOverlap(Bs, D.sbox[q])) bpStage(D, s, D.order[q], P.numReal, stage);
+  }
+  for (int w = 0; w < 4; ++w) {
+    const int y = w == 0 ? cy + 1 : cy + w - 2;
+    const int z = w == 0 ? cz : cz + 1;
+    if (y < 0 || y >= G.dim[1] || z >= G.dim[2]) continue;
+    const int r = (z * G.dim[1] + y) * G.dim[0];
+    const int b = D.cellStart[r + x0], e = D.cellStart[r + x1 + 1];
+    for (int q = b; q < e; ++q)
+      if (boxOverlap(Bs, D.sbox[q])) bpStage(D, s, D.order[q], P.numReal, stage);
+  }
 }
 __global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, const Grid* grid,
                                                     const int* stage) {
@@ -748,7 +767,7 @@ __global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, const Grid*
   if (n <= BP_CAP) {
     for (int a = 0; a < n; ++a) seg[a] = make_int2(i, stage[(size_t)a * P.numReal + i]);
   } else {
-    bpQueryOne<false>(D, G, i, saneBox(D.posPred[i], P.margin, P.boxCap), nullptr, 0, seg);
+    bpQueryOne(D, G, i, saneBox(D.posPred[i], P.margin, P.boxCap), seg);
   }
   // ascending partner order (segments are short)
   for (int a = 1; a < n; ++a) {

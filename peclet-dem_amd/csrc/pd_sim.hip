@@ -722,8 +722,9 @@ struct pd_sim {
     HIPCHECK(hipGetLastError());
     cellCntDirty = false;
     pairStage.ensure((size_t)numReal * BP_CAP + 1, st);
+    zero(pairCnt.p, numReal);
     kernBegin();
-    k_bp_query1<<<nblk(numParticles), TPB, 0, st>>>(D, P, gridD.p, pairStage.p);
+    k_bp_query<<<nblk(numParticles), TPB, 0, st>>>(D, P, gridD.p, pairStage.p);
     kernEnd(K_BPQUERY, numParticles);
     np = scan(pairCnt.p, pairStart.p, numReal);
     pairs.ensure(np + 1, st);
@@ -2426,7 +2427,7 @@ int pd_get_stats_log(pd_sim* s, pd_stats* out, int cap) {
 }
 const char* pd_kernel_name(int k) {
   static const char* names[pd_sim::K_COUNT] = {
-      "k_vsweep",  "k_psweep",      "k_bp_query1",    "k_np_pairs",
+      "k_vsweep",  "k_psweep",      "k_bp_query",     "k_np_pairs",
       "k_predict", "k_reintegrate", "k_vsolve_block", "k_psolve_block"};
   return (k >= 0 && k < pd_sim::K_COUNT) ? names[k] : "";
 }
