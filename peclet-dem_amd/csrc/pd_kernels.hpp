@@ -230,6 +230,19 @@ __global__ void k_fill_u64(unsigned long long* a, unsigned long long v, int n) {
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = TPB * SCAN_ITEMS;
 
+// the SCAN_ITEMS values at base (0 past n); in and out of every scan are 16-byte aligned buffer
+// starts, so full groups are two int4 accesses
+static_assert(SCAN_ITEMS == 8, "scanLoad reads two int4");
+__device__ __forceinline__ void scanLoad(const int* in, int n, int base, int* vals) {
+  if (base + SCAN_ITEMS <= n) {
+    const int4* p = reinterpret_cast<const int4*>(in + base);
+    const int4 a = p[0], b = p[1];
+    vals[0] = a.x; vals[1] = a.y; vals[2] = a.z; vals[3] = a.w;
+    vals[4] = b.x; vals[5] = b.y; vals[6] = b.z; vals[7] = b.w;
+  } else {
+    for (int k = 0; k < SCAN_ITEMS; ++k) vals[k] = (base + k < n) ? in[base + k] : 0;
+  }
+}
 __device__ int blockExclusiveScan(int v, int* sh, int* total) {
   const int t = threadIdx.x;
   sh[t] = v;
@@ -251,9 +264,10 @@ __global__ void k_scan_reduce(const int* in, int n, int* blockSums, const int* n
   __shared__ int sh[TPB];
   if (nDev) n = *nDev;
   const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int vals[SCAN_ITEMS];
+  scanLoad(in, n, base, vals);
   int s = 0;
-  for (int k = 0; k < SCAN_ITEMS; ++k)
-    if (base + k < n) s += in[base + k];
+  for (int k = 0; k < SCAN_ITEMS; ++k) s += vals[k];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int off = TPB / 2; off > 0; off >>= 1) {
@@ -285,17 +299,24 @@ __global__ void k_scan_down2(const int* in, int n, const int* blockSums, int* ou
   __syncthreads();
   const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int vals[SCAN_ITEMS];
+  scanLoad(in, n, base, vals);
   int s = 0;
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    vals[k] = (base + k < n) ? in[base + k] : 0;
-    s += vals[k];
-  }
+  for (int k = 0; k < SCAN_ITEMS; ++k) s += vals[k];
   int tot;
   int ex = blockExclusiveScan(s, sh, &tot) + prefix;
+  int res[SCAN_ITEMS];
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    if (base + k < n) out[base + k] = ex;
+    res[k] = ex;
     if (inv && base + k < invLimit && vals[k]) inv[ex] = base + k;
     ex += vals[k];
+  }
+  if (base + SCAN_ITEMS <= n) {
+    int4* o = reinterpret_cast<int4*>(out + base);
+    o[0] = make_int4(res[0], res[1], res[2], res[3]);
+    o[1] = make_int4(res[4], res[5], res[6], res[7]);
+  } else {
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+      if (base + k < n) out[base + k] = res[k];
   }
   if ((int)blockIdx.x == last && threadIdx.x == 0) out[n] = prefix + tot;
 }
