@@ -107,8 +107,6 @@ struct Dev {
   // colouring
   int* mColor;
   int* cColor;
-  unsigned long long* winner;
-  unsigned long long* mask;
   int* vPerm;
   int* vRowOf;
   int* pPerm;
@@ -143,9 +141,6 @@ struct Dev {
   // scalars
   unsigned long long* red;  // reduction scratch (ordered keys)
   int* counters;            // [0] remaining JP edges, [1] uncoloured entering, [2..] misc
-  int* colCnt;              // [64]
-  int* colFill;             // [64]
-  int* colStart;            // [65]
   unsigned long long* maxOverlapBits;
   unsigned long long* maxApproachBits;
   double* thermo;           // partial sums

@@ -246,11 +246,10 @@ struct pd_sim {
       vLamT, vWV, pN,
       pRA, pRB, lLamT;
   DBuf<double> vVn0, vLam, pLam, lLam;
-  DBuf<unsigned long long> mKey, winner, mask, lKey, uKey, red, bits, redPart;
-  DBuf<int> blockCnt;
+  DBuf<unsigned long long> mKey, lKey, uKey, red, bits, redPart;
   DBuf<int> supList, supCnt;  // support manifolds of the grounded-level sweeps
   DBuf<int> ctl;  // device loop control (CTL_* in pd_kernels.hpp)
-  DBuf<int> lIdx, bucketCnt, bucketStart, counters, colCnt, colFill, colStart, scanTmp, ghostCnt,
+  DBuf<int> lIdx, bucketCnt, bucketStart, counters, scanTmp, ghostCnt,
       ghostOff;
   // pinned host scratch
   static constexpr size_t HOST_SCRATCH_BYTES = 4096;
@@ -258,23 +257,30 @@ struct pd_sim {
 
   Dev D{};
 
+  hipStream_t sideStream = nullptr;  // default priority, like st
+  hipEvent_t evSide = nullptr;
+
   pd_sim(int cap) : capacity(cap) {
     HIPCHECK(hipStreamCreate(&st));
+    HIPCHECK(hipStreamCreate(&sideStream));
+    HIPCHECK(hipEventCreateWithFlags(&evSide, hipEventDisableTiming));
     HIPCHECK(hipHostMalloc(&hostScratch, HOST_SCRATCH_BYTES, hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc(&jobV.pinned, 1024, hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc(&jobP.pinned, 1024, hipHostMallocDefault));
+    jobV.stream = st;
+    jobP.stream = sideStream;
     ensureSlots(cap > 0 ? cap : 1, 0);
     red.ensure(8, st);
     bits.ensure(4, st);
     counters.ensure(16, st);
     ctl.ensure(CTL_COUNT + 1, st);
-    colCnt.ensure(64, st);
-    colFill.ensure(64, st);
-    colStart.ensure(65, st);
     pairTableD.ensure(128, st);
     shell.ensure(1, st);
     initializeShape(SHAPE_SPHERE, 1.0, 0.0, 0.0);
   }
   ~pd_sim() {
     (void)hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(sideStream);
     DBufReleaseAll();
     for (const KernRec& r : kernPending) {
       (void)hipEventDestroy(r.a);
@@ -285,6 +291,10 @@ struct pd_sim {
       for (int k = 0; k <= stageMarked; ++k) (void)hipEventDestroy(stageCur[k]);
     for (hipEvent_t e : evFree) (void)hipEventDestroy(e);
     if (hostScratch) (void)hipHostFree(hostScratch);
+    if (jobV.pinned) (void)hipHostFree(jobV.pinned);
+    if (jobP.pinned) (void)hipHostFree(jobP.pinned);
+    (void)hipEventDestroy(evSide);
+    (void)hipStreamDestroy(sideStream);
     (void)hipStreamDestroy(st);
   }
   void DBufReleaseAll() {
@@ -304,9 +314,9 @@ struct pd_sim {
     mRA.release(); mRB.release(); vN.release(); vTA.release(); vTB.release(); vRA.release();
     vRB.release(); vVt0.release(); vLamT.release(); pN.release(); pRA.release(); pRB.release();
     lLamT.release(); vVn0.release(); vLam.release(); pLam.release(); lLam.release();
-    mKey.release(); winner.release(); mask.release(); lKey.release(); uKey.release();
+    mKey.release(); lKey.release(); uKey.release();
     red.release(); bits.release(); lIdx.release(); bucketCnt.release(); bucketStart.release();
-    counters.release(); colCnt.release(); colFill.release(); colStart.release();
+    counters.release();
     scanTmp.release(); ghostCnt.release(); ghostOff.release();
     dVel.release(); dAng.release(); dPos.release(); dCnt.release(); cLamN.release();
     frCnt.release();
@@ -315,8 +325,8 @@ struct pd_sim {
     hXi.release(); hF.release(); hTA.release(); hTB.release(); hPrevXi.release();
     hRefPos.release(); hXiWall.release(); hWF.release(); hWT.release(); hPrevKeys.release();
     hTab.release();
-    jpWork[0].release(); jpWork[1].release(); jpCnt.release(); redPart.release();
-    blockCnt.release(); ctl.release(); supList.release(); supCnt.release(); hLevel.release(); lvlKey.release(); lvlCnt.release();
+    jobV.release(); jobP.release(); redPart.release();
+    ctl.release(); supList.release(); supCnt.release(); hLevel.release(); lvlKey.release(); lvlCnt.release();
     lvlStart.release(); lvlPerm.release(); lvlChanged.release(); lvlBuckets.release();
   }
 
@@ -437,14 +447,14 @@ struct pd_sim {
     d.mRange = mRange.p; d.mRange2 = mRange2.p; d.cBV = cBV.p; d.mWV = mWV.p; d.vWV = vWV.p;
     d.mNs = mNs.p; d.mTA = mTA.p; d.mTB = mTB.p; d.mRA = mRA.p;
     d.mRB = mRB.p; d.mNp = mNp.p; d.mKey = mKey.p; d.mCode = mCode.p; d.mColor = mColor.p;
-    d.cColor = cColor.p; d.winner = winner.p; d.mask = mask.p; d.vPerm = vPerm.p;
+    d.cColor = cColor.p; d.vPerm = vPerm.p;
     d.vRowOf = vRowOf.p; d.pPerm = pPerm.p; d.pRowOf = pRowOf.p; d.vAB = vAB.p; d.vN = vN.p;
     d.vTA = vTA.p; d.vTB = vTB.p; d.vRA = vRA.p; d.vRB = vRB.p; d.vVt0 = vVt0.p;
     d.vLamT = vLamT.p; d.vVn0 = vVn0.p; d.vLam = vLam.p; d.vFlag = vFlag.p; d.pAB = pAB.p;
     d.pN = pN.p; d.pRA = pRA.p; d.pRB = pRB.p; d.pLam = pLam.p; d.lKey = lKey.p;
     d.lLam = lLam.p; d.lLamT = lLamT.p; d.lIdx = lIdx.p; d.uKey = uKey.p;
     d.bucketCnt = bucketCnt.p; d.bucketStart = bucketStart.p; d.red = red.p;
-    d.counters = counters.p; d.colCnt = colCnt.p; d.colFill = colFill.p; d.colStart = colStart.p;
+    d.counters = counters.p;
     d.maxOverlapBits = bits.p; d.maxApproachBits = bits.p + 1; d.thermo = thermo.p;
     d.dVel = dVel.p; d.dAng = dAng.p; d.dPos = dPos.p; d.dCnt = dCnt.p;
     d.cLamN = cLamN.p; d.frCnt = frCnt.p;
@@ -807,99 +817,150 @@ struct pd_sim {
     HIPCHECK(hipGetLastError());
   }
 
-  // Jones-Plassmann colouring then the colour classes, with one host read per batch of rounds.
-  // The uncoloured edges are in jpWork[0], their count in *enteringDev (device) or enteringHost.
-  // Round j of a batch reads its list size from jpCnt[j] and appends the losers to the other list,
-  // counting into jpCnt[j + 1]; every round colours at least the globally highest-priority edge,
-  // so rounds after the list empties are no-ops.  The class kernels are enqueued behind every
-  // batch, so the batch that finishes the colouring also brings back the class starts.  The batch
-  // and grid sizes come from the previous colouring of the same kind (the round kernels loop over
-  // their lists, so any grid is correct); the host replays the reference loop
-  // (src/solver_velocity.hpp:305-355) on the counts.  Returns the uncoloured leftovers after the
-  // round cap.
-  DBuf<int> jpWork[2];
-  DBuf<int> jpCnt;
-  std::vector<int> jpHist[2];  // per kind: list size entering each round, last colouring
   static constexpr int JP_MAX_BATCH = 32;
   static constexpr long JP_GRID_CAP = 2048;
   static constexpr long JP_TAIL_ROWS = 4096;  // rounds expected this small run in one workgroup
-  int colourEdges(int kind, const int2* ends, int* colour, int numEdges, int numBodies,
-                  const int* enteringDev, int enteringHost, int* perm, int* rowOf,
-                  std::vector<int>& starts, int& numColors, int& rounds, long long& uncol) {
-    winner.ensure(numBodies + 1, st);
-    mask.ensure(numBodies + 1, st);
-    jpCnt.ensure(JP_MAX_BATCH + 1, st);
-    blockCnt.ensure(64 * CP_BLOCKS, st);
-    refreshDev();
-    zero(mask.p, numBodies);
-    zero(winner.p, numBodies);
-    std::vector<int>& hist = jpHist[kind];
+  // One colouring in progress: its stream, its own work lists, counters, per-body winner / mask
+  // slots, class buffers and pinned read-back area, so the manifold and contact colourings can
+  // run on different streams at once.
+  struct ColourJob {
+    hipStream_t stream = nullptr;
+    void* pinned = nullptr;  // read-back area (pinned host memory)
+    DBuf<int> work[2], cnt, blockCnt, colCnt, colStart;
+    DBuf<unsigned long long> winner, mask;
+    std::vector<int> hist;  // list size entering each round, last colouring of this job
+    // the colouring being run
+    const int2* ends = nullptr;
+    int* colour = nullptr;
+    int* perm = nullptr;
+    int* rowOf = nullptr;
+    int numEdges = 0;
     std::vector<int> seen;
-    int remaining = 1;  // the reference starts with remaining = 1: one round always runs
-    uncol = 0;
-    rounds = 0;
-    int prevRemaining = -1;
-    int cur = 0;
-    const long maxRounds = (long)numBodies + 2;
-    int batch = std::min<int>(JP_MAX_BATCH, std::max<int>(4, (int)hist.size() + 2));
-    long entering = enteringDev ? -1 : enteringHost;
-    const int nbc = std::min(nblk(numEdges), CP_BLOCKS);
-    for (bool first = true;; first = false) {
-      const int B = (int)std::min<long>(batch, maxRounds - rounds);
-      k_jp_counts<<<1, 64, 0, st>>>(jpCnt.p, first ? enteringDev : nullptr, (int)entering, B + 1);
-      for (int j = 0; j < B; ++j) {
-        const int r = rounds + j;
-        long est = numEdges;
-        if (r < (int)hist.size()) est = hist[r];
-        else if (j == 0 && entering >= 0) est = entering;
-        else if (!hist.empty()) est = hist.back();
-        int* work = jpWork[(cur + j) & 1].p;
-        int* next = jpWork[(cur + j + 1) & 1].p;
-        if (est <= JP_TAIL_ROWS) {
-          // the rest of the batch is expected to be small: one workgroup runs those rounds
-          k_jp_tail<<<1, JP_TAIL_THREADS, 0, st>>>(ends, colour, work, next, jpCnt.p, j, B,
-                                                   winner.p, mask.p);
-          break;
-        }
-        const int grid = (int)std::min<long>(nblk(est + est / 4 + 64), JP_GRID_CAP);
-        k_jp_contend<<<grid, TPB, 0, st>>>(ends, work, jpCnt.p + j, winner.p);
-        k_jp_commit<<<grid, TPB, 0, st>>>(ends, colour, work, jpCnt.p + j, winner.p, mask.p, next,
-                                          jpCnt.p + j + 1);
-      }
-      // colour classes: counts, per-block bases, starts, placement
-      k_color_count<<<nbc, TPB, 0, st>>>(colour, numEdges, blockCnt.p);
-      k_color_offsets<<<64, TPB, 0, st>>>(blockCnt.p, nbc, colCnt.p);
-      k_color_starts<<<1, 64, 0, st>>>(colCnt.p, colStart.p);
-      k_color_place<<<nbc, TPB, 0, st>>>(colour, numEdges, blockCnt.p, colStart.p, perm, rowOf);
-      HIPCHECK(hipGetLastError());
-      size_t off = 0;
-      const int* hostCnt = enqueueRead(jpCnt.p, B + 1, off);
-      const int* hostStarts = enqueueRead(colStart.p, 65, off);
-      sync();
-      for (int j = 0; j < B && remaining > 0; ++j) {
-        uncol += hostCnt[j];
-        seen.push_back(hostCnt[j]);
-        ++rounds;
-        const int rem = hostCnt[j + 1];
-        if (rem == prevRemaining && rem > 0)
-          throw std::runtime_error("colouring stalled: a round coloured no edge");
-        prevRemaining = rem;
-        remaining = rem;
-      }
-      entering = hostCnt[B];
-      cur = (cur + B) & 1;
-      if (remaining <= 0 || rounds >= maxRounds) {
-        starts.assign(hostStarts, hostStarts + 65);
+    int remaining = 0, rounds = 0, prevRemaining = -1, cur = 0, batch = 0, B = 0;
+    long long uncol = 0;
+    long maxRounds = 0, entering = 0;
+    const int* enteringDev = nullptr;
+    bool first = true;
+    const int* hostCnt = nullptr;
+    const int* hostStarts = nullptr;
+    void release() {
+      work[0].release(); work[1].release(); cnt.release(); blockCnt.release(); colCnt.release();
+      colStart.release(); winner.release(); mask.release();
+    }
+  };
+  ColourJob jobV, jobP;  // manifold colouring (main stream), contact colouring (side stream)
+
+  // Jones-Plassmann colouring then the colour classes, with one host read per batch of rounds.
+  // The uncoloured edges are in job.work[0], their count in *enteringDev (device) or
+  // enteringHost.  Round j of a batch reads its list size from cnt[j] and appends the losers to
+  // the other list, counting into cnt[j + 1]; every round colours at least the globally
+  // highest-priority edge, so rounds after the list empties are no-ops.  The class kernels are
+  // enqueued behind every batch, so the batch that finishes the colouring also brings back the
+  // class starts.  The batch and grid sizes come from the job's previous colouring (the round
+  // kernels loop over their lists, so any grid is correct); the host replays the reference loop
+  // (src/solver_velocity.hpp:305-355) on the counts.
+  void colourBegin(ColourJob& J, const int2* ends, int* colour, int numEdges, int numBodies,
+                   const int* enteringDev, int enteringHost, int* perm, int* rowOf) {
+    J.winner.ensure(numBodies + 1, st);
+    J.mask.ensure(numBodies + 1, st);
+    J.cnt.ensure(JP_MAX_BATCH + 1, st);
+    J.blockCnt.ensure(64 * CP_BLOCKS, st);
+    J.colCnt.ensure(64, st);
+    J.colStart.ensure(65, st);
+    HIPCHECK(hipMemsetAsync(J.mask.p, 0, numBodies * sizeof(unsigned long long), J.stream));
+    HIPCHECK(hipMemsetAsync(J.winner.p, 0, numBodies * sizeof(unsigned long long), J.stream));
+    J.ends = ends;
+    J.colour = colour;
+    J.perm = perm;
+    J.rowOf = rowOf;
+    J.numEdges = numEdges;
+    J.seen.clear();
+    J.remaining = 1;  // the reference starts with remaining = 1: one round always runs
+    J.rounds = 0;
+    J.uncol = 0;
+    J.prevRemaining = -1;
+    J.cur = 0;
+    J.maxRounds = (long)numBodies + 2;
+    J.batch = std::min<int>(JP_MAX_BATCH, std::max<int>(4, (int)J.hist.size() + 2));
+    J.entering = enteringDev ? -1 : enteringHost;
+    J.enteringDev = enteringDev;
+    J.first = true;
+    colourLaunch(J);
+  }
+  // enqueue one batch of rounds, the class kernels and the read-back of counts and class starts
+  void colourLaunch(ColourJob& J) {
+    hipStream_t s = J.stream;
+    const int B = (int)std::min<long>(J.batch, J.maxRounds - J.rounds);
+    J.B = B;
+    k_jp_counts<<<1, 64, 0, s>>>(J.cnt.p, J.first ? J.enteringDev : nullptr, (int)J.entering,
+                                 B + 1);
+    for (int j = 0; j < B; ++j) {
+      const int r = J.rounds + j;
+      long est = J.numEdges;
+      if (r < (int)J.hist.size()) est = J.hist[r];
+      else if (j == 0 && J.entering >= 0) est = J.entering;
+      else if (!J.hist.empty()) est = J.hist.back();
+      int* work = J.work[(J.cur + j) & 1].p;
+      int* next = J.work[(J.cur + j + 1) & 1].p;
+      if (est <= JP_TAIL_ROWS) {
+        // the rest of the batch is expected to be small: one workgroup runs those rounds
+        k_jp_tail<<<1, JP_TAIL_THREADS, 0, s>>>(J.ends, J.colour, work, next, J.cnt.p, j, B,
+                                                J.winner.p, J.mask.p);
         break;
       }
-      batch = std::min(batch * 2, JP_MAX_BATCH);
+      const int grid = (int)std::min<long>(nblk(est + est / 4 + 64), JP_GRID_CAP);
+      k_jp_contend<<<grid, TPB, 0, s>>>(J.ends, work, J.cnt.p + j, J.winner.p);
+      k_jp_commit<<<grid, TPB, 0, s>>>(J.ends, J.colour, work, J.cnt.p + j, J.winner.p, J.mask.p,
+                                       next, J.cnt.p + j + 1);
     }
-    hist.swap(seen);
+    const int nbc = std::min(nblk(J.numEdges), CP_BLOCKS);
+    k_color_count<<<nbc, TPB, 0, s>>>(J.colour, J.numEdges, J.blockCnt.p);
+    k_color_offsets<<<64, TPB, 0, s>>>(J.blockCnt.p, nbc, J.colCnt.p);
+    k_color_starts<<<1, 64, 0, s>>>(J.colCnt.p, J.colStart.p);
+    k_color_place<<<nbc, TPB, 0, s>>>(J.colour, J.numEdges, J.blockCnt.p, J.colStart.p, J.perm,
+                                      J.rowOf);
+    HIPCHECK(hipGetLastError());
+    char* h = static_cast<char*>(J.pinned);
+    HIPCHECK(hipMemcpyAsync(h, J.cnt.p, (B + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(h + 256, J.colStart.p, 65 * sizeof(int), hipMemcpyDeviceToHost, s));
+    J.hostCnt = reinterpret_cast<const int*>(h);
+    J.hostStarts = reinterpret_cast<const int*>(h + 256);
+    J.first = false;
+  }
+  // wait for the job's read-back and replay its rounds; enqueue further batches until done.
+  // Returns the uncoloured leftovers after the round cap.
+  int colourFinish(ColourJob& J, std::vector<int>& starts, int& numColors, int& rounds,
+                   long long& uncol) {
+    for (;;) {
+      HIPCHECK(hipStreamSynchronize(J.stream));
+      const int B = J.B;
+      for (int j = 0; j < B && J.remaining > 0; ++j) {
+        J.uncol += J.hostCnt[j];
+        J.seen.push_back(J.hostCnt[j]);
+        ++J.rounds;
+        const int rem = J.hostCnt[j + 1];
+        if (rem == J.prevRemaining && rem > 0)
+          throw std::runtime_error("colouring stalled: a round coloured no edge");
+        J.prevRemaining = rem;
+        J.remaining = rem;
+      }
+      J.entering = J.hostCnt[B];
+      J.cur = (J.cur + B) & 1;
+      if (J.remaining <= 0 || J.rounds >= J.maxRounds) {
+        starts.assign(J.hostStarts, J.hostStarts + 65);
+        break;
+      }
+      J.batch = std::min(J.batch * 2, JP_MAX_BATCH);
+      colourLaunch(J);
+    }
+    J.hist.swap(J.seen);
     int maxc = -1;
     for (int c = 0; c < 64; ++c)
       if (starts[c + 1] > starts[c]) maxc = c;
     numColors = maxc + 1;
-    return remaining > 0 ? (int)entering : 0;
+    rounds = J.rounds;
+    uncol = J.uncol;
+    return J.remaining > 0 ? (int)J.entering : 0;
   }
 
   // Enqueue up to `iters` device-gated sweeps in geometric batches (1, 2, 4, ...), reading the
@@ -1075,16 +1136,31 @@ struct pd_sim {
     // manifold colouring, src/solver_velocity.hpp:277-374
     zero(counters.p, 2);
     vPerm.ensure(nm + 1, st);
-    jpWork[0].ensure(std::max(nm, nc) + 1, st);
-    jpWork[1].ensure(std::max(nm, nc) + 1, st);
+    jobV.work[0].ensure(nm + 1, st);
+    jobV.work[1].ensure(nm + 1, st);
     refreshDev();
-    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(D, nm, jpWork[0].p, counters.p + 1);
+    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(D, nm, jobV.work[0].p, counters.p + 1);
+    // the contact colouring (src/solver_position.hpp:155-241) needs only the contact list: it
+    // starts now on the side stream and overlaps the velocity stages
+    const bool colourP = useGS && nc > 0 && numParticles > 0;
+    if (colourP) {
+      cColor.ensure(nc + 1, st);
+      pPerm.ensure(nc + 1, st);
+      pRowOf.ensure(nc + 1, st);
+      jobP.work[0].ensure(nc + 1, st);
+      jobP.work[1].ensure(nc + 1, st);
+      refreshDev();
+      HIPCHECK(hipEventRecord(evSide, st));
+      HIPCHECK(hipStreamWaitEvent(jobP.stream, evSide, 0));
+      k_jp_init_contacts<<<nblk(nc), TPB, 0, jobP.stream>>>(cColor.p, nc, jobP.work[0].p);
+      colourBegin(jobP, cAB.p, cColor.p, nc, numParticles, nullptr, nc, pPerm.p, pRowOf.p);
+    }
     std::vector<int> vs;
     int numColors = 0;
     int velLeftover = 0;
     if (useGS && nm > 0 && numReal > 0) {
-      velLeftover = colourEdges(0, mReal.p, mColor.p, nm, numReal, counters.p + 1, 0, vPerm.p,
-                                vRowOf.p, vs, numColors, stats.rounds_vel, stats.uncolored_vel);
+      colourBegin(jobV, mReal.p, mColor.p, nm, numReal, counters.p + 1, 0, vPerm.p, vRowOf.p);
+      velLeftover = colourFinish(jobV, vs, numColors, stats.rounds_vel, stats.uncolored_vel);
     } else {
       vs.assign(65, 0);
     }
@@ -1272,20 +1348,18 @@ struct pd_sim {
     kernEnd(K_REINTEGRATE, numParticles);
     mark(ST_COLOR_P);
 
-    // contact colouring over raw slots, src/solver_position.hpp:155-241
+    // contact colouring over raw slots (started on the side stream), src/solver_position.hpp:155-241
     cColor.ensure(nc + 1, st);
     pPerm.ensure(nc + 1, st);
     pRowOf.ensure(nc + 1, st);
     refreshDev();
-    zero(counters.p, 2);
     std::vector<int> ps;
     int numPosColors = 0;
     int posLeftover = 0;
-    if (useGS && nc > 0 && numParticles > 0) {
-      k_jp_init_contacts<<<nblk(nc), TPB, 0, st>>>(cColor.p, nc, jpWork[0].p);
-      posLeftover = colourEdges(1, cAB.p, cColor.p, nc, numParticles, nullptr, nc, pPerm.p,
-                                pRowOf.p, ps, numPosColors, stats.rounds_pos,
-                                stats.uncolored_pos);
+    if (colourP) {
+      posLeftover = colourFinish(jobP, ps, numPosColors, stats.rounds_pos, stats.uncolored_pos);
+      HIPCHECK(hipEventRecord(evSide, jobP.stream));
+      HIPCHECK(hipStreamWaitEvent(st, evSide, 0));
     } else {
       ps.assign(65, 0);
     }
