@@ -773,7 +773,7 @@ struct pd_sim {
     } else {
       // point shapes: the scan also lists the pairs in contact (their counts are 0 or 1)
       pairOfContact.ensure(np + 1, st);
-      const int nb = (nslots + SCAN_TILE - 1) / SCAN_TILE;
+      const int nb = std::max(1, (nslots + SCAN_TILE - 1) / SCAN_TILE);
       scanTmp.ensure(nb + 1, st);
       if (nb > 1) k_scan_reduce<<<nb, TPB, 0, st>>>(ncnt.p, nslots, scanTmp.p, nullptr);
       k_scan_down2<<<nb, TPB, 0, st>>>(ncnt.p, nslots, scanTmp.p, noff.p, nullptr,
