@@ -119,6 +119,7 @@ struct Dev {
   double4* vRA;
   double4* vRB;
   double4* vVt0;
+  double4* vPre;  // per row, constant in the solve: (wA_n, wB_n, growth term, 1 / lenN)
   double4* vLamT;
   double4* vWV;   // boundary rows: wall velocity sum (the body-B velocity is vWV / points)
   double* vVn0;
@@ -1460,6 +1461,24 @@ __global__ __launch_bounds__(TPB) void k_vrows(Dev D, Params P, int numRows, int
   double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
   vn += dot3(vGrowth, Nsum);
   D.vVn0[k] = vn;
+  {
+    // the row's constants of the PGS / GS updates, evaluated as they are there: body masses,
+    // inertia and orientation do not change during the velocity solve
+    const BodyC ca = D.bc[realA];
+    const double Nsq = dot3(Nsum, Nsum);
+    const double wAn = Nsq * ca.iim.w + genInvMass(TauA, xyz(ca.iim), toq(ca.q));
+    double invMassB = 0.0;
+    d3 invIB{0, 0, 0};
+    q4 qB{0, 0, 0, 1};
+    if (realB >= 0) {
+      const BodyC cb = D.bc[realB];
+      invMassB = cb.iim.w;
+      invIB = xyz(cb.iim);
+      qB = toq(cb.q);
+    }
+    const double wBn = Nsq * invMassB + genInvMass(TauB, invIB, qB);
+    D.vPre[k] = make_double4(wAn, wBn, dot3(vGrowth, Nsum), lenN > 0.0 ? 1.0 / lenN : 0.0);
+  }
   d3 vt{0, 0, 0};
   if (lenN > 1e-9) {
     const d3 nhat = scale3(Nsum, 1.0 / lenN);
@@ -1651,14 +1670,12 @@ __device__ double pgsRow(Dev& D, const Params& P, int k, double restVelThreshold
   const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
   const double lenN = ra4.w;
   if (lenN < 1e-9) return 0.0;
-  const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
-  const d3 vGrowth = scale3(diffCenters, P.growthRate);
+  const double4 pre = D.vPre[k];
   double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
-  vn += dot3(vGrowth, Nsum);
+  vn += pre.z;  // growth term dot3(diffCenters * growthRate, Nsum)
   const double sgn = rb4.w;
-  const double Nsq = dot3(Nsum, Nsum);
-  double wA_n = Nsq * invMassA + genInvMass(TauA, invIA, qA);
-  double wB_n = Nsq * invMassB + genInvMass(TauB, invIB, qB);
+  double wA_n = pre.x;  // Nsq * invMassA + genInvMass(TauA, invIA, qA)
+  double wB_n = pre.y;
   bool applyA = true, applyB = true;
   const unsigned char fl = D.vFlag[k];
   const int sf = (fl & F_SIDE1) ? 1 : ((fl & F_SIDE2) ? 2 : 0);
@@ -1716,7 +1733,7 @@ __device__ double pgsRow(Dev& D, const Params& P, int k, double restVelThreshold
   const d3 ltOld = xyz(lto4);
   const bool haveOld = nonzero3(ltOld);
   if (mu > 0.0 || haveOld) {
-    const d3 nhat = scale3(Nsum, 1.0 / lenN);
+    const d3 nhat = scale3(Nsum, pre.w);  // 1 / lenN
     const d3 vA2 = xyz(ba.v), wA2 = xyz(ba.w);
     d3 vB2{0, 0, 0}, wB2{0, 0, 0};
     if (realB >= 0) {
@@ -1821,18 +1838,16 @@ __device__ double gsRow(Dev& D, const Params& P, int k, double restVelThreshold)
   const double lenN = ra4.w;
   if (lenN < 1e-9) return 0.0;
   const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
-  const d3 vGrowth = scale3(diffCenters, P.growthRate);
+  const double4 pre = D.vPre[k];
   double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
-  vn += dot3(vGrowth, Nsum);
+  vn += pre.z;  // growth term
   const double alignment = dot3(Nsum, diffCenters);
   if (alignment > 0.0) {
     if (vn < 0.0) return 0.0;
   } else {
     if (vn > 0.0) return 0.0;
   }
-  const double Nsq = dot3(Nsum, Nsum);
-  const double wA_n = Nsq * invMassA + genInvMass(TauA, invIA, qA);
-  const double wB_n = Nsq * invMassB + genInvMass(TauB, invIB, qB);
+  const double wA_n = pre.x, wB_n = pre.y;
   const double wTotal = wA_n + wB_n;
   if (wTotal <= 0.0) return 0.0;
   const double amax = fabs(vn) / lenN;

@@ -242,7 +242,7 @@ struct pd_sim {
   DBuf<int> pairOfContact;  // point shapes: the pair of each pair contact
   DBuf<int2> pairs, cAB, mAB, mReal, mRange, mRange2, vAB, pAB;
   DBuf<int> ncnt, noff, cSlot, mCnt, mStart, mNp, mColor, cColor, vPerm, vRowOf, pPerm, pRowOf;
-  DBuf<double4> cN, cRA, cRB, cBV, mNs, mTA, mTB, mRA, mRB, mWV, vN, vTA, vTB, vRA, vRB, vVt0,
+  DBuf<double4> cN, cRA, cRB, cBV, mNs, mTA, mTB, mRA, mRB, mWV, vN, vTA, vTB, vRA, vRB, vVt0, vPre,
       vLamT, vWV, pN,
       pRA, pRB, lLamT;
   DBuf<double> vVn0, vLam, pLam, lLam;
@@ -267,8 +267,11 @@ struct pd_sim {
     HIPCHECK(hipHostMalloc(&hostScratch, HOST_SCRATCH_BYTES, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&jobV.pinned, 1024, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&jobP.pinned, 1024, hipHostMallocDefault));
+    // the single-workgroup rounds pay device-scope atomic latency on every phase: worth it only
+    // off the critical path, for the contact colouring that overlaps the velocity stages
     jobV.stream = st;
     jobP.stream = sideStream;
+    jobP.tailRows = 4096;
     ensureSlots(cap > 0 ? cap : 1, 0);
     red.ensure(8, st);
     bits.ensure(4, st);
@@ -312,7 +315,7 @@ struct pd_sim {
     cColor.release(); vPerm.release(); vRowOf.release(); pPerm.release(); pRowOf.release();
     cN.release(); cRA.release(); cRB.release(); mNs.release(); mTA.release(); mTB.release();
     mRA.release(); mRB.release(); vN.release(); vTA.release(); vTB.release(); vRA.release();
-    vRB.release(); vVt0.release(); vLamT.release(); pN.release(); pRA.release(); pRB.release();
+    vRB.release(); vVt0.release(); vPre.release(); vLamT.release(); pN.release(); pRA.release(); pRB.release();
     lLamT.release(); vVn0.release(); vLam.release(); pLam.release(); lLam.release();
     mKey.release(); lKey.release(); uKey.release();
     red.release(); bits.release(); lIdx.release(); bucketCnt.release(); bucketStart.release();
@@ -449,7 +452,7 @@ struct pd_sim {
     d.mRB = mRB.p; d.mNp = mNp.p; d.mKey = mKey.p; d.mCode = mCode.p; d.mColor = mColor.p;
     d.cColor = cColor.p; d.vPerm = vPerm.p;
     d.vRowOf = vRowOf.p; d.pPerm = pPerm.p; d.pRowOf = pRowOf.p; d.vAB = vAB.p; d.vN = vN.p;
-    d.vTA = vTA.p; d.vTB = vTB.p; d.vRA = vRA.p; d.vRB = vRB.p; d.vVt0 = vVt0.p;
+    d.vTA = vTA.p; d.vTB = vTB.p; d.vRA = vRA.p; d.vRB = vRB.p; d.vVt0 = vVt0.p; d.vPre = vPre.p;
     d.vLamT = vLamT.p; d.vVn0 = vVn0.p; d.vLam = vLam.p; d.vFlag = vFlag.p; d.pAB = pAB.p;
     d.pN = pN.p; d.pRA = pRA.p; d.pRB = pRB.p; d.pLam = pLam.p; d.lKey = lKey.p;
     d.lLam = lLam.p; d.lLamT = lLamT.p; d.lIdx = lIdx.p; d.uKey = uKey.p;
@@ -819,12 +822,12 @@ struct pd_sim {
 
   static constexpr int JP_MAX_BATCH = 32;
   static constexpr long JP_GRID_CAP = 2048;
-  static constexpr long JP_TAIL_ROWS = 4096;  // rounds expected this small run in one workgroup
   // One colouring in progress: its stream, its own work lists, counters, per-body winner / mask
   // slots, class buffers and pinned read-back area, so the manifold and contact colourings can
   // run on different streams at once.
   struct ColourJob {
     hipStream_t stream = nullptr;
+    long tailRows = 0;  // rounds expected to hold at most this many edges run in one workgroup
     void* pinned = nullptr;  // read-back area (pinned host memory)
     DBuf<int> work[2], cnt, blockCnt, colCnt, colStart;
     DBuf<unsigned long long> winner, mask;
@@ -902,7 +905,7 @@ struct pd_sim {
       else if (!J.hist.empty()) est = J.hist.back();
       int* work = J.work[(J.cur + j) & 1].p;
       int* next = J.work[(J.cur + j + 1) & 1].p;
-      if (est <= JP_TAIL_ROWS) {
+      if (est <= J.tailRows) {
         // the rest of the batch is expected to be small: one workgroup runs those rounds
         k_jp_tail<<<1, JP_TAIL_THREADS, 0, s>>>(J.ends, J.colour, work, next, J.cnt.p, j, B,
                                                 J.winner.p, J.mask.p);
@@ -1068,6 +1071,7 @@ struct pd_sim {
   }
 
   long vBlockRec[2] = {-1, -1}, pBlockRec = -1;
+  int posSweepsPrev = 1;  // sweeps of the last batched position loop (first batch size)
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
     vBlockRec[0] = vBlockRec[1] = pBlockRec = -1;
@@ -1172,7 +1176,7 @@ struct pd_sim {
     const d3 gHat = usePersist ? d3{g[0] / gMagP, g[1] / gMagP, g[2] / gMagP} : d3{0, 0, 0};
     const size_t R = nRowsV + 1;
     vAB.ensure(R, st); vN.ensure(R, st); vTA.ensure(R, st); vTB.ensure(R, st); vRA.ensure(R, st);
-    vRB.ensure(R, st); vVt0.ensure(R, st); vLamT.ensure(R, st); vVn0.ensure(R, st);
+    vRB.ensure(R, st); vVt0.ensure(R, st); vPre.ensure(R, st); vLamT.ensure(R, st); vVn0.ensure(R, st);
     vLam.ensure(R, st); vFlag.ensure(R, st); vWV.ensure(R, st);
     refreshDev();
     supList.ensure(nm + 1, st);
@@ -1373,6 +1377,8 @@ struct pd_sim {
     if (nRowsP > 0) k_prows<<<nblk(nRowsP), TPB, 0, st>>>(D, nRowsP);
     mark(ST_PSWEEP);
     const double posTol = 1e-4 * baseRadius * globalScale;
+    const int* finalCtl = nullptr;
+    const unsigned long long* finalBits = nullptr;
     if (!useGS) {
       // Jacobi XPBD over every contact, no adaptive stop (src/solver_position.hpp:34-134)
       for (int it = 0; it < posIters; ++it) {
@@ -1392,23 +1398,38 @@ struct pd_sim {
       }
       HIPCHECK(hipGetLastError());
     } else if (posLeftover == 0) {
-      sweepLoop(posIters, ctl.p + CTL_STOP_P, [&] {
-        k_sweep_begin<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P);
-        for (int c = 0; c < numPosColors; ++c) {
-          const int b = ps[c], e = ps[c + 1];
-          if (e <= b) continue;
-          if (c == 62) {
-            k_psweep_serial<<<1, 1, 0, st>>>(D, nc, c, ctl.p + CTL_STOP_P);
-          } else {
-            kernBegin();
-            k_psweep<<<nblk(e - b), TPB, 0, st>>>(D, b, e, ctl.p + CTL_STOP_P);
-            kernEnd(K_PSWEEP, e - b);
+      // batches of device-gated sweeps, the first as long as the previous substep's loop; each
+      // batch ends with one read of the loop counters and residuals, which is also the substep's
+      // final read-back once the loop has stopped
+      int done = 0, batch = std::max(1, std::min(posIters, posSweepsPrev));
+      while (done < posIters) {
+        const int b = std::min(batch, posIters - done);
+        for (int j = 0; j < b; ++j) {
+          k_sweep_begin<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P);
+          for (int c = 0; c < numPosColors; ++c) {
+            const int cb = ps[c], ce = ps[c + 1];
+            if (ce <= cb) continue;
+            if (c == 62) {
+              k_psweep_serial<<<1, 1, 0, st>>>(D, nc, c, ctl.p + CTL_STOP_P);
+            } else {
+              kernBegin();
+              k_psweep<<<nblk(ce - cb), TPB, 0, st>>>(D, cb, ce, ctl.p + CTL_STOP_P);
+              kernEnd(K_PSWEEP, ce - cb);
+            }
           }
+          k_sweep_check<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P, ctl.p + CTL_SWEEPS_P,
+                                         posTol, 1);
         }
-        k_sweep_check<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P, ctl.p + CTL_SWEEPS_P, posTol,
-                                       1);
-      });
-      HIPCHECK(hipGetLastError());
+        HIPCHECK(hipGetLastError());
+        done += b;
+        size_t off = 0;
+        finalCtl = enqueueRead(ctl.p, CTL_COUNT, off);
+        finalBits = enqueueRead(bits.p, 2, off);
+        sync();
+        if (finalCtl[CTL_STOP_P]) break;
+        batch *= 2;
+      }
+      posSweepsPrev = finalCtl[CTL_SWEEPS_P];
     } else {
       for (int it = 0; it < posIters; ++it) {
         zero(bits.p, 1);
@@ -1429,11 +1450,16 @@ struct pd_sim {
       }
     }
     {
-      // one read-back of the loop counters and the residuals
-      size_t off = 0;
-      const int* c = enqueueRead(ctl.p, CTL_COUNT, off);
-      const unsigned long long* b2 = enqueueRead(bits.p, 2, off);
-      sync();
+      // one read-back of the loop counters and the residuals (the position loop's last one
+      // when it ran in batches)
+      if (!finalCtl) {
+        size_t off = 0;
+        finalCtl = enqueueRead(ctl.p, CTL_COUNT, off);
+        finalBits = enqueueRead(bits.p, 2, off);
+        sync();
+      }
+      const int* c = finalCtl;
+      const unsigned long long* b2 = finalBits;
       if (!hostLoop) {
         stats.sweeps_vel = c[CTL_SWEEPS_V];
         stats.sweeps_stab = c[CTL_SWEEPS_S];
