@@ -781,12 +781,32 @@ __global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, const Grid*
   const int n = D.pairCnt[i];
   if (n == 0) return;
   int2* seg = D.pairs + D.pairStart[i];
+  constexpr int R = 8;  // short rows are sorted in registers
+  if (n <= R) {
+    int v[R];
+#pragma unroll
+    for (int a = 0; a < R; ++a) v[a] = a < n ? stage[(size_t)a * P.numReal + i] : 0x7fffffff;
+    // odd-even transposition sort of R values (fixed indices: stays in registers)
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int a = r & 1; a + 1 < R; a += 2) {
+        const int lo = v[a] < v[a + 1] ? v[a] : v[a + 1];
+        const int hi = v[a] < v[a + 1] ? v[a + 1] : v[a];
+        v[a] = lo;
+        v[a + 1] = hi;
+      }
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+      if (a < n) seg[a] = make_int2(i, v[a]);
+    return;
+  }
   if (n <= BP_CAP) {
     for (int a = 0; a < n; ++a) seg[a] = make_int2(i, stage[(size_t)a * P.numReal + i]);
   } else {
     bpQueryOne(D, G, i, saneBox(D.posPred[i], P.margin, P.boxCap), seg);
   }
-  // ascending partner order (segments are short)
+  // ascending partner order
   for (int a = 1; a < n; ++a) {
     const int2 x = seg[a];
     int b = a - 1;

@@ -1144,8 +1144,17 @@ struct pd_sim {
     jobV.work[1].ensure(nm + 1, st);
     refreshDev();
     k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(D, nm, jobV.work[0].p, counters.p + 1);
+    std::vector<int> vs;
+    int numColors = 0;
+    int velLeftover = 0;
+    if (useGS && nm > 0 && numReal > 0) {
+      colourBegin(jobV, mReal.p, mColor.p, nm, numReal, counters.p + 1, 0, vPerm.p, vRowOf.p);
+      velLeftover = colourFinish(jobV, vs, numColors, stats.rounds_vel, stats.uncolored_vel);
+    } else {
+      vs.assign(65, 0);
+    }
     // the contact colouring (src/solver_position.hpp:155-241) needs only the contact list: it
-    // starts now on the side stream and overlaps the velocity stages
+    // runs on the side stream, overlapping the velocity stages (whose sweep loops occupy one CU)
     const bool colourP = useGS && nc > 0 && numParticles > 0;
     if (colourP) {
       cColor.ensure(nc + 1, st);
@@ -1158,15 +1167,6 @@ struct pd_sim {
       HIPCHECK(hipStreamWaitEvent(jobP.stream, evSide, 0));
       k_jp_init_contacts<<<nblk(nc), TPB, 0, jobP.stream>>>(cColor.p, nc, jobP.work[0].p);
       colourBegin(jobP, cAB.p, cColor.p, nc, numParticles, nullptr, nc, pPerm.p, pRowOf.p);
-    }
-    std::vector<int> vs;
-    int numColors = 0;
-    int velLeftover = 0;
-    if (useGS && nm > 0 && numReal > 0) {
-      colourBegin(jobV, mReal.p, mColor.p, nm, numReal, counters.p + 1, 0, vPerm.p, vRowOf.p);
-      velLeftover = colourFinish(jobV, vs, numColors, stats.rounds_vel, stats.uncolored_vel);
-    } else {
-      vs.assign(65, 0);
     }
     nRowsV = vs[numColors];
     mark(ST_PREP);
