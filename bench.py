@@ -271,6 +271,10 @@ def main():
     stage = np.zeros(32)
     nst = sim._b.get_stage_times(sim._h, stage.ctypes.data_as(C.POINTER(C.c_double)), 32)
     stage_ms = {sim._b.stage_name(k).decode(): float(stage[k]) / args.steps for k in range(nst)}
+    hstage = np.zeros(32)
+    sim._b.get_stage_host_times(sim._h, hstage.ctypes.data_as(C.POINTER(C.c_double)), 32)
+    stage_host_ms = {sim._b.stage_name(k).decode(): float(hstage[k]) / args.steps
+                     for k in range(nst)}
     mean = {k: float(np.mean([s[k] for s in stats])) for k in stats[0]}
     sim._b.set_timing(sim._h, 0)
     if args.copy_probe:
@@ -344,6 +348,7 @@ def main():
                                "GBs": (kbytes[q] / (ms[q] / 1e3) / 1e9) if ms[q] > 0 else 0.0}
                     for q in range(len(names))},
         "stage_ms_per_substep": stage_ms,
+        "stage_host_ms_per_substep": stage_host_ms,
         "counters_mean": mean,
     }
     print(json.dumps(line))

@@ -199,6 +199,9 @@ int pd_set_timing(pd_sim* s, int level);
  * (default 16384; 0 disables).  Results do not depend on it. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 int pd_get_stage_times(pd_sim* s, double* out_ms, int cap);
+/* host wall time between the same stage marks (enqueueing plus waiting), summed like the stage
+ * times; a stage whose host time exceeds its device time is bound by launch overhead */
+int pd_get_stage_host_times(pd_sim* s, double* out_ms, int cap);
 const char* pd_stage_name(int stage);
 /* per timed kernel k: summed ms, launch count and units processed (k_vsweep: manifold rows,
  * k_psweep: contact rows, k_bp_query: slots queried, k_np_pairs: pairs, k_predict: particles,

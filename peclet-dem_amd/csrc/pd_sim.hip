@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -149,6 +150,9 @@ struct pd_sim {
   // so timing adds no host synchronisation to a substep.
   int timing = 0;
   double stageMs[ST_COUNT] = {};
+  // host wall time between the same marks (enqueue work plus waits), at timing >= 1
+  double stageHostMs[ST_COUNT] = {};
+  std::chrono::steady_clock::time_point stageHostT[ST_COUNT + 1];
   // kernels timed per launch at level 2; units = what each launch processed
   enum { K_VSWEEP, K_PSWEEP, K_BPQUERY, K_NPPAIRS, K_PREDICT, K_REINTEGRATE, K_VBLOCK, K_PBLOCK,
          K_COUNT };
@@ -219,6 +223,7 @@ struct pd_sim {
     resolveTiming();
     timing = level;
     for (double& x : stageMs) x = 0.0;
+    for (double& x : stageHostMs) x = 0.0;
     for (int k = 0; k < K_COUNT; ++k) {
       kernMs[k] = 0.0;
       kernLaunches[k] = 0;
@@ -387,6 +392,11 @@ struct pd_sim {
     if (stage != stageMarked + 1) return;
     stageCur[stage] = takeEvent();
     HIPCHECK(hipEventRecord(stageCur[stage], st));
+    stageHostT[stage] = std::chrono::steady_clock::now();
+    if (stage > 0)
+      stageHostMs[stage - 1] +=
+          std::chrono::duration<double, std::milli>(stageHostT[stage] - stageHostT[stage - 1])
+              .count();
     stageMarked = stage;
     if (stage == ST_COUNT) {
       stagePending.insert(stagePending.end(), stageCur, stageCur + ST_COUNT + 1);
@@ -2517,6 +2527,10 @@ int pd_get_stage_times(pd_sim* s, double* out, int cap) {
   const int rc = guard([&] { s->resolveTiming(); });
   if (rc) return rc;
   for (int k = 0; k < ST_COUNT && k < cap; ++k) out[k] = s->stageMs[k];
+  return ST_COUNT;
+}
+int pd_get_stage_host_times(pd_sim* s, double* out, int cap) {
+  for (int k = 0; k < ST_COUNT && k < cap; ++k) out[k] = s->stageHostMs[k];
   return ST_COUNT;
 }
 int pd_get_stats_log(pd_sim* s, pd_stats* out, int cap) {

@@ -124,6 +124,7 @@ PRODUCT_SIGNATURES = {
     "get_device_pointer": (_p, [_p, C.c_char_p]),
     "set_timing": (_i, [_p, _i]),
     "get_stage_times": (_i, [_p, _dp, _i]),
+    "get_stage_host_times": (_i, [_p, _dp, _i]),
     "stage_name": (C.c_char_p, [_i]),
     "get_kernel_times": (_i, [_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long), _i]),
     "get_stats_log": (_i, [_p, C.POINTER(Stats), _i]),
