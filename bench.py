@@ -214,6 +214,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (scales the HBM byte counters)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--timing-level", type=int, default=2, choices=[0, 1, 2],
+                    help="library timing in the timed region: 2 = per-kernel events and counter "
+                         "log (needed for roofline), 1 = stage events only, 0 = none")
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=24)
     ap.add_argument("--batch-workers", type=int, default=16)
@@ -245,7 +248,7 @@ def main():
     for _ in range(args.warmup):
         sim.step(dt)
     sim._b.synchronize(sim._h)
-    sim._b.set_timing(sim._h, 2)
+    sim._b.set_timing(sim._h, args.timing_level)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     report = max(1, args.steps // 8)
@@ -257,6 +260,12 @@ def main():
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     value = args.n * args.steps / t
+    if args.timing_level < 2:
+        # diagnostic run (how much the per-kernel events cost): throughput only
+        print(json.dumps({"metric": METRIC, "value": value, "unit": "particle-substeps/s",
+                          "ms_per_step": 1e3 * t / args.steps, "steps": args.steps,
+                          "timing_level": args.timing_level, "roofline": None}))
+        return
     names, ms, launches, units = kernel_times(sim)
     stats = stats_log(sim)
     assert len(stats) == args.steps, (len(stats), args.steps)

@@ -263,12 +263,14 @@ struct pd_sim {
   Dev D{};
 
   hipStream_t sideStream = nullptr;  // default priority, like st
-  hipEvent_t evSide = nullptr;
+  hipEvent_t evSide = nullptr;      // side stream done -> main stream
+  hipEvent_t evContacts = nullptr;  // contact list built -> side stream
 
   pd_sim(int cap) : capacity(cap) {
     HIPCHECK(hipStreamCreate(&st));
     HIPCHECK(hipStreamCreate(&sideStream));
     HIPCHECK(hipEventCreateWithFlags(&evSide, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&evContacts, hipEventDisableTiming));
     HIPCHECK(hipHostMalloc(&hostScratch, HOST_SCRATCH_BYTES, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&jobV.pinned, 1024, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&jobP.pinned, 1024, hipHostMallocDefault));
@@ -302,6 +304,7 @@ struct pd_sim {
     if (jobV.pinned) (void)hipHostFree(jobV.pinned);
     if (jobP.pinned) (void)hipHostFree(jobP.pinned);
     (void)hipEventDestroy(evSide);
+    (void)hipEventDestroy(evContacts);
     (void)hipStreamDestroy(sideStream);
     (void)hipStreamDestroy(st);
   }
@@ -1146,6 +1149,7 @@ struct pd_sim {
       if (nc > 0) k_fr_plane_load<<<nblk(nc), TPB, 0, st>>>(D, nc);
     }
     mark(ST_COLOR_V);
+    HIPCHECK(hipEventRecord(evContacts, st));
 
     // manifold colouring, src/solver_velocity.hpp:277-374
     zero(counters.p, 2);
@@ -1162,21 +1166,6 @@ struct pd_sim {
       velLeftover = colourFinish(jobV, vs, numColors, stats.rounds_vel, stats.uncolored_vel);
     } else {
       vs.assign(65, 0);
-    }
-    // the contact colouring (src/solver_position.hpp:155-241) needs only the contact list: it
-    // runs on the side stream, overlapping the velocity stages (whose sweep loops occupy one CU)
-    const bool colourP = useGS && nc > 0 && numParticles > 0;
-    if (colourP) {
-      cColor.ensure(nc + 1, st);
-      pPerm.ensure(nc + 1, st);
-      pRowOf.ensure(nc + 1, st);
-      jobP.work[0].ensure(nc + 1, st);
-      jobP.work[1].ensure(nc + 1, st);
-      refreshDev();
-      HIPCHECK(hipEventRecord(evSide, st));
-      HIPCHECK(hipStreamWaitEvent(jobP.stream, evSide, 0));
-      k_jp_init_contacts<<<nblk(nc), TPB, 0, jobP.stream>>>(cColor.p, nc, jobP.work[0].p);
-      colourBegin(jobP, cAB.p, cColor.p, nc, numParticles, nullptr, nc, pPerm.p, pRowOf.p);
     }
     nRowsV = vs[numColors];
     mark(ST_PREP);
@@ -1322,6 +1311,23 @@ struct pd_sim {
           }
         }
       }
+    }
+    // The contact colouring (src/solver_position.hpp:155-241) needs only the contact list: it
+    // runs on the side stream, overlapping the velocity stages, whose sweep loops occupy one CU.
+    // It is enqueued here, behind the velocity loops, so the host spends the time it takes to
+    // enqueue it while the main stream has work queued.
+    const bool colourP = useGS && nc > 0 && numParticles > 0;
+    if (colourP) {
+      cColor.ensure(nc + 1, st);
+      pPerm.ensure(nc + 1, st);
+      pRowOf.ensure(nc + 1, st);
+      jobP.work[0].ensure(nc + 1, st);
+      jobP.work[1].ensure(nc + 1, st);
+      refreshDev();
+      // ordered after the contact list only (evContacts was recorded behind the manifolds)
+      HIPCHECK(hipStreamWaitEvent(jobP.stream, evContacts, 0));
+      k_jp_init_contacts<<<nblk(nc), TPB, 0, jobP.stream>>>(cColor.p, nc, jobP.work[0].p);
+      colourBegin(jobP, cAB.p, cColor.p, nc, numParticles, nullptr, nc, pPerm.p, pRowOf.p);
     }
     mark(ST_LEDGER);
     if (usePGS && nm > 0) {
