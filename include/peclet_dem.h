@@ -194,9 +194,11 @@ int pd_batch_step(pd_batch* b, double dt, int nsteps); /* nsteps substeps of eve
  * substep.  Values accumulate until pd_set_timing is called again; at level 2 each substep's
  * counters are also logged (pd_get_stats_log). */
 int pd_set_timing(pd_sim* s, int level);
-/* Launch-shape tuning (no reference counterpart): "block_solver_rows" = the largest velocity /
- * position sweep loop (in rows) solved inside one workgroup instead of one launch per colour
- * (default 16384; 0 disables).  Results do not depend on it. */
+/* Tuning (no reference counterpart): "block_solver_rows" = the largest velocity / position
+ * sweep loop (in rows) solved inside one workgroup instead of one launch per colour (default
+ * 16384; 0 disables); "timing_every" = N records timing events only on substeps 0, N, 2N, ...
+ * after pd_set_timing (default 1), which bounds the events' host cost.  Results do not depend
+ * on either. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 int pd_get_stage_times(pd_sim* s, double* out_ms, int cap);
 /* host wall time between the same stage marks (enqueueing plus waiting), summed like the stage

@@ -149,6 +149,10 @@ struct pd_sim {
   // Events come from a pool and are resolved lazily (at a query, or every few hundred substeps),
   // so timing adds no host synchronisation to a substep.
   int timing = 0;
+  // events are recorded on every timingEvery-th substep after pd_set_timing (substeps 0, N, 2N..)
+  int timingEvery = 1;
+  long timingSubstep = 0;
+  bool timedNow = false;
   double stageMs[ST_COUNT] = {};
   // host wall time between the same marks (enqueue work plus waits), at timing >= 1
   double stageHostMs[ST_COUNT] = {};
@@ -183,13 +187,13 @@ struct pd_sim {
     return e;
   }
   void kernBegin() {
-    if (timing < 2) return;
+    if (timing < 2 || !timedNow) return;
     kernOpen = takeEvent();
     HIPCHECK(hipEventRecord(kernOpen, st));
   }
   // returns the record's index (valid until the next resolve) or -1
   long kernEnd(int kid, long units) {
-    if (timing < 2) return -1;
+    if (timing < 2 || !timedNow) return -1;
     hipEvent_t b = takeEvent();
     HIPCHECK(hipEventRecord(b, st));
     kernPending.push_back({kernOpen, b, kid, units});
@@ -230,6 +234,7 @@ struct pd_sim {
       kernRows[k] = 0;
     }
     statsLog.clear();
+    timingSubstep = 0;
   }
   // device buffers
   DBuf<double4> pos, posPred, quatPred, extForce, shell, sbox;
@@ -386,7 +391,7 @@ struct pd_sim {
     if (n) HIPCHECK(hipMemsetAsync(dptr, 0, n * sizeof(T), st));
   }
   void mark(int stage) {
-    if (!timing) return;
+    if (!timing || !timedNow) return;
     if (stage == ST_PREDICT && stageMarked >= 0) {
       // a substep that ended early (an error) left its events open: recycle them
       for (int k = 0; k <= stageMarked; ++k) evFree.push_back(stageCur[k]);
@@ -1088,6 +1093,8 @@ struct pd_sim {
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
     vBlockRec[0] = vBlockRec[1] = pBlockRec = -1;
+    timedNow = timing > 0 && timingSubstep % timingEvery == 0;
+    ++timingSubstep;
     if (stabMode == 2) throw std::runtime_error("stabilization mode 'multilevel' is not implemented");
     // a frictional wall drives friction even for frictionless bodies (src/sim.hpp:169-170)
     double wallFrictionMax = 0.0;
@@ -2511,6 +2518,9 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     if (k == "block_solver_rows") {
       if (value < 0) throw ArgError("block_solver_rows must be >= 0");
       s->blockRows = (int)std::min<long>(value, 1L << 30);
+    } else if (k == "timing_every") {
+      if (value < 1) throw ArgError("timing_every must be >= 1");
+      s->timingEvery = (int)std::min<long>(value, 1L << 30);
     } else {
       throw ArgError("unknown tuning parameter: " + k);
     }
