@@ -338,3 +338,33 @@ def test_timing_is_passive_and_logs_counters(hip_lib):
     stage = np.zeros(32)
     nst = b._b.get_stage_times(b._h, stage.ctypes.data_as(C.POINTER(C.c_double)), 32)
     assert nst > 0 and np.all(stage[:nst] >= 0) and stage[:nst].sum() > 0
+
+
+@pytest.mark.gpu
+def test_sampled_timing(hip_lib):
+    # timing_every = 3: events on substeps 0, 3, 6, 9 only; the counter log still has every substep
+    import ctypes as C
+
+    scene = make_scene("bed_periodic", 400, 5)
+    a = apply_scene(hip_sim(400), scene)
+    b = apply_scene(hip_sim(400), scene)
+    b.set_tuning("timing_every", 3)
+    b._b.set_timing(b._h, 2)
+    for _ in range(11):
+        a.step(scene["dt"])
+        b.step(scene["dt"])
+    for name in ["get_positions", "get_velocities"]:
+        assert np.array_equal(getattr(a, name)(), getattr(b, name)()), name
+    assert b._b.get_stats_log(b._h, None, 0) == 11
+    count = b._b.get_kernel_times(b._h, None, None, None, 0)
+    ms = np.zeros(count)
+    launches = np.zeros(count, np.int64)
+    units = np.zeros(count, np.int64)
+    b._b.get_kernel_times(b._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                          launches.ctypes.data_as(C.POINTER(C.c_long)),
+                          units.ctypes.data_as(C.POINTER(C.c_long)), count)
+    names = [b._b.kernel_name(k).decode() for k in range(count)]
+    q = names.index("k_predict")
+    assert launches[q] == 4 and units[q] == 4 * 400
+    with pytest.raises(ValueError):
+        b.set_tuning("timing_every", 0)
