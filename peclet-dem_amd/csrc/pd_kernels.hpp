@@ -1188,84 +1188,131 @@ __global__ void k_jp_commit(const int2* ends, int* colour, const int* work, cons
     blockAppend(lose, e, next, nextCount, &shCnt, &shBase);
   }
 }
-// The small rounds of a colouring inside one workgroup: the contend and commit phases of each
-// round are separated by workgroup barriers instead of launches.  Round j reads its list from
-// `in` (rounds alternate between the two lists) with its size in cnt[j] and appends the losers
-// to the other list, counting into cnt[j + 1]; it stops at round j1 or once a list is empty.
-// Values other threads of the workgroup wrote or updated atomically in earlier phases are read
-// with device-scope atomic loads (from L2), so no stale L1 line is ever used.
-constexpr int JP_TAIL_THREADS = 1024;
-__device__ __forceinline__ int ldL2(const int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long ldL2(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void stL2(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// workgroup barrier after every earlier memory operation of the wave has completed (atomics
-// and stores included: the plain barrier does not wait for them)
-__device__ __forceinline__ void waitBarrier() {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-}
-__global__ __launch_bounds__(JP_TAIL_THREADS) void k_jp_tail(const int2* ends, int* colour,
-                                                             int* in, int* out, int* cnt, int j0,
-                                                             int j1, unsigned long long* winner,
-                                                             unsigned long long* mask) {
-  __shared__ int shCnt, shBase;
-  for (int j = j0; j < j1; ++j) {
-    const int n = ldL2(cnt + j);
-    if (n == 0) break;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-      const int e = ldL2(in + t);
-      const int2 ab = ends[e];
-      const unsigned long long key = colorKey1(e);
-      atomicMax(&winner[ab.x], key);
-      if (ab.y >= 0) atomicMax(&winner[ab.y], key);
+// The small rounds of a colouring inside one workgroup, with the edges and their bodies' winner /
+// mask slots in LDS.  The bodies are hashed into JPS_HASH slots (linear probing) and loaded from
+// the global winner / mask arrays; each round is a contend phase (LDS atomic max) and a commit
+// phase separated by workgroup barriers, so a round costs a few microseconds instead of two
+// launches.  Round j reads the list of round j0 from `in` (then LDS), writes its remaining count
+// to cnt[j + 1] and stops at j1 or once the list is empty.  If the list still holds edges at j1,
+// it goes to the global list of round j1 (`in` if j1 - j0 is even, else `out`) and the bodies'
+// winner / mask slots go back to the global arrays, so launch-per-round batches can continue.
+// More than JPS_EDGES edges or JPS_HASH * 3 / 4 bodies: nothing is changed and cnt[j0 + 1] = -1.
+constexpr int JPS_THREADS = 1024;
+constexpr int JPS_EDGES = 4096;
+constexpr int JPS_HASH = 4096;
+__device__ __forceinline__ int jpsSlot(int* hKey, int* used, int body, bool insert) {
+  unsigned h = ((unsigned)body * 2654435761u) & (JPS_HASH - 1);
+  for (int probe = 0; probe < JPS_HASH; ++probe) {
+    const int k = hKey[h];
+    if (k == body) return (int)h;
+    if (k == -1) {
+      if (!insert) return -1;
+      if (atomicAdd(used, 1) >= JPS_HASH * 3 / 4) return -1;
+      const int old = atomicCAS(&hKey[h], -1, body);
+      if (old == -1 || old == body) return (int)h;
+      atomicSub(used, 1);
+      continue;  // lost the slot to another body: look at it again
     }
-    waitBarrier();
-    for (int base = 0; base < n; base += blockDim.x) {
-      const int t = base + threadIdx.x;
-      int e = -1;
-      bool lose = false;
-      if (t < n) {
-        e = ldL2(in + t);
-        const int2 ab = ends[e];
-        const unsigned long long key = colorKey1(e);
-        const unsigned long long wa = ldL2(&winner[ab.x]);
-        const unsigned long long wb = ab.y >= 0 ? ldL2(&winner[ab.y]) : key;
-        if (wa != key || wb != key) {
-          lose = true;
-        } else {
-          unsigned long long forbidden = ldL2(&mask[ab.x]);
-          if (ab.y >= 0) forbidden |= ldL2(&mask[ab.y]);
-          int c = 0;
-          while (c < 62 && (forbidden & (1ull << c))) ++c;
-          colour[e] = c;
-          stL2(&mask[ab.x], ldL2(&mask[ab.x]) | (1ull << c));
-          stL2(&winner[ab.x], 0ull);
-          if (ab.y >= 0) {
-            stL2(&mask[ab.y], ldL2(&mask[ab.y]) | (1ull << c));
-            stL2(&winner[ab.y], 0ull);
-          }
+    h = (h + 1) & (JPS_HASH - 1);
+  }
+  return -1;
+}
+__global__ __launch_bounds__(JPS_THREADS) void k_jp_small(const int2* ends, int* colour, int* in,
+                                                          int* out, int* cnt, int j0, int j1,
+                                                          unsigned long long* winnerG,
+                                                          unsigned long long* maskG) {
+  __shared__ int hKey[JPS_HASH];
+  __shared__ unsigned long long hWin[JPS_HASH], hMask[JPS_HASH];
+  __shared__ int eId[JPS_EDGES];
+  __shared__ unsigned eAB[JPS_EDGES];  // slot of a | slot of b << 16 (0xFFFF: boundary)
+  __shared__ unsigned short lst[2][JPS_EDGES];
+  __shared__ int sNext, sFail, sUsed;
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int n0 = cnt[j0];
+  if (n0 > JPS_EDGES) {
+    if (tid == 0) cnt[j0 + 1] = -1;
+    return;
+  }
+  for (int h = tid; h < JPS_HASH; h += T) hKey[h] = -1;
+  if (tid == 0) {
+    sFail = 0;
+    sUsed = 0;
+  }
+  __syncthreads();
+  for (int q = tid; q < n0; q += T) {
+    const int e = in[q];
+    const int2 ab = ends[e];
+    const int sa = jpsSlot(hKey, &sUsed, ab.x, true);
+    const int sb = ab.y >= 0 ? jpsSlot(hKey, &sUsed, ab.y, true) : 0xFFFF;
+    if (sa < 0 || sb < 0) sFail = 1;
+    eId[q] = e;
+    eAB[q] = (unsigned)(sa & 0xFFFF) | ((unsigned)sb << 16);
+    lst[0][q] = (unsigned short)q;
+  }
+  __syncthreads();
+  if (sFail) {
+    if (tid == 0) cnt[j0 + 1] = -1;
+    return;
+  }
+  for (int h = tid; h < JPS_HASH; h += T) {
+    const int b = hKey[h];
+    if (b >= 0) {
+      hWin[h] = winnerG[b];
+      hMask[h] = maskG[b];
+    }
+  }
+  __syncthreads();
+  int cur = 0, n = n0, j = j0;
+  for (; j < j1 && n > 0; ++j) {
+    for (int q = tid; q < n; q += T) {
+      const int k = lst[cur][q];
+      const unsigned long long key = colorKey1(eId[k]);
+      const unsigned ab = eAB[k];
+      atomicMax(&hWin[ab & 0xFFFFu], key);
+      if ((ab >> 16) != 0xFFFFu) atomicMax(&hWin[ab >> 16], key);
+    }
+    if (tid == 0) sNext = 0;
+    __syncthreads();
+    for (int q = tid; q < n; q += T) {
+      const int k = lst[cur][q];
+      const unsigned long long key = colorKey1(eId[k]);
+      const unsigned ab = eAB[k];
+      const int sa = (int)(ab & 0xFFFFu), sb = (int)(ab >> 16);
+      const bool hasB = sb != 0xFFFF;
+      const unsigned long long wa = hWin[sa];
+      const unsigned long long wb = hasB ? hWin[sb] : key;
+      if (wa != key || wb != key) {
+        lst[cur ^ 1][atomicAdd(&sNext, 1)] = (unsigned short)k;
+      } else {
+        unsigned long long forbidden = hMask[sa];
+        if (hasB) forbidden |= hMask[sb];
+        int c = 0;
+        while (c < 62 && (forbidden & (1ull << c))) ++c;
+        colour[eId[k]] = c;
+        hMask[sa] |= 1ull << c;
+        hWin[sa] = 0;
+        if (hasB) {
+          hMask[sb] |= 1ull << c;
+          hWin[sb] = 0;
         }
       }
-      // append the losers: block-aggregated slots, stores that go to L2
-      if (threadIdx.x == 0) shCnt = 0;
-      waitBarrier();
-      const int local = lose ? atomicAdd(&shCnt, 1) : 0;
-      waitBarrier();
-      if (threadIdx.x == 0 && shCnt) shBase = atomicAdd(cnt + j + 1, shCnt);
-      waitBarrier();
-      if (lose)
-        __hip_atomic_store(out + shBase + local, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    waitBarrier();
-    int* sw = in;
-    in = out;
-    out = sw;
+    __syncthreads();
+    n = sNext;
+    cur ^= 1;
+    if (tid == 0) cnt[j + 1] = n;
+    __syncthreads();  // sNext is reset by the next round
+  }
+  if (n > 0) {
+    int* dst = ((j - j0) & 1) ? out : in;
+    for (int q = tid; q < n; q += T) dst[q] = eId[lst[cur][q]];
+    for (int h = tid; h < JPS_HASH; h += T) {
+      const int b = hKey[h];
+      if (b >= 0) {
+        winnerG[b] = hWin[h];
+        maskG[b] = hMask[h];
+      }
+    }
   }
 }
 // round counters of a batch: cnt[0] = the entering list size (from src when given), rest 0

@@ -279,11 +279,8 @@ struct pd_sim {
     HIPCHECK(hipHostMalloc(&hostScratch, HOST_SCRATCH_BYTES, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&jobV.pinned, 1024, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&jobP.pinned, 1024, hipHostMallocDefault));
-    // the single-workgroup rounds pay device-scope atomic latency on every phase: worth it only
-    // off the critical path, for the contact colouring that overlaps the velocity stages
     jobV.stream = st;
     jobP.stream = sideStream;
-    jobP.tailRows = 4096;
     ensureSlots(cap > 0 ? cap : 1, 0);
     red.ensure(8, st);
     bits.ensure(4, st);
@@ -845,7 +842,10 @@ struct pd_sim {
   // run on different streams at once.
   struct ColourJob {
     hipStream_t stream = nullptr;
-    long tailRows = 0;  // rounds expected to hold at most this many edges run in one workgroup
+    // rounds expected to hold at most this many edges run in one workgroup out of LDS
+    long smallRows = JPS_EDGES * 3 / 4;
+    bool noSmall = false;  // this colouring's list did not fit: launch-per-round batches only
+    int smallAt = -1;      // round of the current batch where the one-workgroup kernel starts
     void* pinned = nullptr;  // read-back area (pinned host memory)
     DBuf<int> work[2], cnt, blockCnt, colCnt, colStart;
     DBuf<unsigned long long> winner, mask;
@@ -906,27 +906,42 @@ struct pd_sim {
     J.entering = enteringDev ? -1 : enteringHost;
     J.enteringDev = enteringDev;
     J.first = true;
+    J.noSmall = false;
     colourLaunch(J);
   }
   // enqueue one batch of rounds, the class kernels and the read-back of counts and class starts
   void colourLaunch(ColourJob& J) {
     hipStream_t s = J.stream;
-    const int B = (int)std::min<long>(J.batch, J.maxRounds - J.rounds);
-    J.B = B;
-    k_jp_counts<<<1, 64, 0, s>>>(J.cnt.p, J.first ? J.enteringDev : nullptr, (int)J.entering,
-                                 B + 1);
-    for (int j = 0; j < B; ++j) {
+    int B = (int)std::min<long>(J.batch, J.maxRounds - J.rounds);
+    // the list size expected at round j of this batch (from the last colouring of this job)
+    auto estimate = [&](int j) {
       const int r = J.rounds + j;
       long est = J.numEdges;
       if (r < (int)J.hist.size()) est = J.hist[r];
       else if (j == 0 && J.entering >= 0) est = J.entering;
       else if (!J.hist.empty()) est = J.hist.back();
+      return est;
+    };
+    J.smallAt = -1;
+    if (!J.noSmall)
+      for (int j = 0; j < B; ++j)
+        if (estimate(j) <= J.smallRows) {
+          // rounds from j on are expected to be small: one workgroup runs them out of LDS, so the
+          // batch may as well run to the cap
+          J.smallAt = j;
+          B = (int)std::min<long>(JP_MAX_BATCH, J.maxRounds - J.rounds);
+          break;
+        }
+    J.B = B;
+    k_jp_counts<<<1, 64, 0, s>>>(J.cnt.p, J.first ? J.enteringDev : nullptr, (int)J.entering,
+                                 B + 1);
+    for (int j = 0; j < B; ++j) {
+      const long est = estimate(j);
       int* work = J.work[(J.cur + j) & 1].p;
       int* next = J.work[(J.cur + j + 1) & 1].p;
-      if (est <= J.tailRows) {
-        // the rest of the batch is expected to be small: one workgroup runs those rounds
-        k_jp_tail<<<1, JP_TAIL_THREADS, 0, s>>>(J.ends, J.colour, work, next, J.cnt.p, j, B,
-                                                J.winner.p, J.mask.p);
+      if (j == J.smallAt) {
+        k_jp_small<<<1, JPS_THREADS, 0, s>>>(J.ends, J.colour, work, next, J.cnt.p, j, B,
+                                             J.winner.p, J.mask.p);
         break;
       }
       const int grid = (int)std::min<long>(nblk(est + est / 4 + 64), JP_GRID_CAP);
@@ -954,7 +969,14 @@ struct pd_sim {
                    long long& uncol) {
     for (;;) {
       HIPCHECK(hipStreamSynchronize(J.stream));
-      const int B = J.B;
+      // a small-colouring kernel that could not take its list changed nothing: replay the
+      // rounds before it and continue with launch-per-round batches
+      const bool failed = J.smallAt >= 0 && J.hostCnt[J.smallAt + 1] < 0;
+      const int B = failed ? J.smallAt : J.B;
+      if (failed) {
+        J.noSmall = true;
+        J.smallRows = std::max<long>(256, J.smallRows / 2);  // lists of this job are too wide
+      }
       for (int j = 0; j < B && J.remaining > 0; ++j) {
         J.uncol += J.hostCnt[j];
         J.seen.push_back(J.hostCnt[j]);
@@ -967,11 +989,11 @@ struct pd_sim {
       }
       J.entering = J.hostCnt[B];
       J.cur = (J.cur + B) & 1;
-      if (J.remaining <= 0 || J.rounds >= J.maxRounds) {
+      if (!failed && (J.remaining <= 0 || J.rounds >= J.maxRounds)) {
         starts.assign(J.hostStarts, J.hostStarts + 65);
         break;
       }
-      J.batch = std::min(J.batch * 2, JP_MAX_BATCH);
+      if (!failed) J.batch = std::min(J.batch * 2, JP_MAX_BATCH);
       colourLaunch(J);
     }
     J.hist.swap(J.seen);
