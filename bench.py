@@ -216,6 +216,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain copy of MIB MiB (scales the HBM byte counters)")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--block-rows", type=int, default=None, metavar="ROWS",
+                    help="largest sweep loop solved inside one workgroup (library default 16384; "
+                         "0 = one launch per colour)")
     ap.add_argument("--timing-every", type=int, default=8, metavar="N",
                     help="record timing events on every N-th timed substep (bounds their host "
                          "cost; averages come from the sampled substeps)")
@@ -249,6 +252,8 @@ def main():
     bed = make_workload(args.workload, args.n, args.seed, args.column)
     t_gen = time.perf_counter() - t_gen
     sim = setup(Simulation(args.n), args.workload, bed, dt)
+    if args.block_rows is not None:
+        sim.set_tuning("block_solver_rows", args.block_rows)
     print(f"# workload ready: n={args.n} (generation {t_gen:.1f}s)", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         sim.step(dt)
