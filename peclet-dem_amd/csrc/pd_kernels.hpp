@@ -428,7 +428,8 @@ __global__ void k_ghost_count(Dev D, Params P, int* cnt) {
   cnt[i] = (sx ? 2 : 1) * (sy ? 2 : 1) * (sz ? 2 : 1) - 1;
 }
 // pass 2: emit ghosts at numReal + off[i] in the (a,b,c) loop order
-__global__ void k_ghost_emit(Dev D, Params P, const int* off) {
+// slots at or past capSlots are not written (the host grows the buffers and emits again)
+__global__ void k_ghost_emit(Dev D, Params P, const int* off, int capSlots) {
   const int i = gtid();
   if (i >= P.numReal) return;
   const d3 p = xyz(D.pos[i]);
@@ -453,6 +454,7 @@ __global__ void k_ghost_emit(Dev D, Params P, const int* off) {
       for (int c = 0; c < nz; ++c) {
         const int ix = ax[a], iy = ay[bb], iz = az[c];
         if (ix == 0 && iy == 0 && iz == 0) continue;
+        if (slot >= capSlots) return;
         const d3 shift{ix * P.dsize.x, iy * P.dsize.y, iz * P.dsize.z};
         D.pos[slot] = tod4(add3(p, shift), 0.0);
         D.posPred[slot] = tod4(add3(pPred, shift), pp.w);
@@ -466,6 +468,15 @@ __global__ void k_ghost_emit(Dev D, Params P, const int* off) {
         ++slot;
       }
 }
+
+// slot count of the substep from the ghost total: info[0] = reals + the ghosts that fit in
+// capSlots, info[1] = 1 when some did not fit
+__global__ void k_slot_count(const int* ghostTotal, int numReal, int capSlots, int* info) {
+  const long n = (long)numReal + *ghostTotal;
+  info[0] = (int)(n < capSlots ? n : capSlots);
+  info[1] = n > capSlots ? 1 : 0;
+}
+__global__ void k_set_int(int* dst, int v) { *dst = v; }
 
 // applyVelocityAndPredictPositionKokkos, src/integration.hpp:112-148 (reals and ghosts)
 __global__ void k_reintegrate(Dev D, Params P) {
@@ -594,10 +605,13 @@ __device__ __forceinline__ unsigned long long waveMinU64(unsigned long long v) {
 }
 constexpr int RED_BLOCKS = 256;
 // part[b * 8 + 0..2] = ordered min, [3..5] = ordered max, [6] = ordered max half-width
-__global__ __launch_bounds__(TPB) void k_bp_bounds(Dev D, Params P, unsigned long long* part) {
+// nSlots: the slot count on the device (the host may not know it yet)
+__global__ __launch_bounds__(TPB) void k_bp_bounds(Dev D, Params P, const int* nSlots,
+                                                   unsigned long long* part) {
   __shared__ unsigned long long sh[7][WAVES];
   unsigned long long v[7] = {~0ull, ~0ull, ~0ull, 0, 0, 0, 0};
-  for (int s = gtid(); s < P.numParticles; s += gridDim.x * blockDim.x) {
+  const int n = *nSlots;
+  for (int s = gtid(); s < n; s += gridDim.x * blockDim.x) {
     const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
     if (isfinite(B.w)) {
       const unsigned long long k = ordKey(B.w);
@@ -630,7 +644,7 @@ __global__ __launch_bounds__(TPB) void k_bp_bounds(Dev D, Params P, unsigned lon
 // half-width (x 1.0001), grown by 1.5x until there are at most 4 x slots + 64 cells (the cell
 // buffers' capacity).  Any grid with cells at least that wide finds every overlapping pair.
 __global__ void k_bp_bounds_final(const unsigned long long* part, int nb, unsigned long long* red,
-                                  int numSlots, Grid* grid, int* numCells) {
+                                  const int* nSlots, Grid* grid, int* numCells) {
   __shared__ unsigned long long sh[7][WAVES];
   __shared__ unsigned long long fin[7];
   unsigned long long v[7] = {~0ull, ~0ull, ~0ull, 0, 0, 0, 0};
@@ -672,7 +686,7 @@ __global__ void k_bp_bounds_final(const unsigned long long* part, int nb, unsign
       G.dim[d] = (int)nd;
       cells *= nd;
     }
-    if (cells <= 4.0 * numSlots + 64.0) break;
+    if (cells <= 4.0 * (*nSlots) + 64.0) break;
     G.h *= 1.5;
   }
   *grid = G;
@@ -686,27 +700,29 @@ __device__ __forceinline__ int cellCoord(double c, double lo, double h, int dim)
   return (int)f;
 }
 // cell of every slot and its rank inside the cell (cellCnt is all zero on entry)
-__global__ void k_cell_count(Dev D, Params P, const Grid* grid) {
-  const int s = gtid();
-  if (s >= P.numParticles) return;
+__global__ void k_cell_count(Dev D, Params P, const int* nSlots, const Grid* grid) {
+  const int n = *nSlots;
   const Grid G = *grid;
-  const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
-  const int cx = cellCoord(B.x, G.lo.x, G.h, G.dim[0]);
-  const int cy = cellCoord(B.y, G.lo.y, G.h, G.dim[1]);
-  const int cz = cellCoord(B.z, G.lo.z, G.h, G.dim[2]);
-  const int c = (cz * G.dim[1] + cy) * G.dim[0] + cx;
-  D.cellOf[s] = c;
-  D.cellRank[s] = atomicAdd(&D.cellCnt[c], 1);
+  for (int s = gtid(); s < n; s += gridDim.x * blockDim.x) {
+    const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
+    const int cx = cellCoord(B.x, G.lo.x, G.h, G.dim[0]);
+    const int cy = cellCoord(B.y, G.lo.y, G.h, G.dim[1]);
+    const int cz = cellCoord(B.z, G.lo.z, G.h, G.dim[2]);
+    const int c = (cz * G.dim[1] + cy) * G.dim[0] + cx;
+    D.cellOf[s] = c;
+    D.cellRank[s] = atomicAdd(&D.cellCnt[c], 1);
+  }
 }
 // slots in cell order; every occupied cell's count is cleared again for the next broadphase
-__global__ void k_cell_fill(Dev D, Params P) {
-  const int s = gtid();
-  if (s >= P.numParticles) return;
-  const int c = D.cellOf[s];
-  const int k = D.cellStart[c] + D.cellRank[s];
-  D.order[k] = s;
-  D.sbox[k] = saneBox(D.posPred[s], P.margin, P.boxCap);
-  D.cellCnt[c] = 0;
+__global__ void k_cell_fill(Dev D, Params P, const int* nSlots) {
+  const int n = *nSlots;
+  for (int s = gtid(); s < n; s += gridDim.x * blockDim.x) {
+    const int c = D.cellOf[s];
+    const int k = D.cellStart[c] + D.cellRank[s];
+    D.order[k] = s;
+    D.sbox[k] = saneBox(D.posPred[s], P.margin, P.boxCap);
+    D.cellCnt[c] = 0;
+  }
 }
 constexpr int BP_CAP = 32;  // staged partners per real; a real with more is queried again
 // Full-stencil query of one real i: every partner j > i in the 27 neighbouring cells, written as
@@ -745,10 +761,8 @@ __device__ __forceinline__ void bpStage(const Dev& D, int a, int b, int numReal,
   const int n = atomicAdd(&D.pairCnt[lo], 1);
   if (n < BP_CAP) stage[(size_t)n * numReal + lo] = hi;
 }
-__global__ __launch_bounds__(TPB) void k_bp_query(Dev D, Params P, const Grid* grid, int* stage) {
-  const int k = gtid();
-  if (k >= P.numParticles) return;
-  const Grid G = *grid;
+__device__ __forceinline__ void bpQuerySlot(const Dev& D, const Params& P, const Grid& G, int k,
+                                            int* stage) {
   const int s = D.order[k];
   const double4 Bs = D.sbox[k];
   const int cx = cellCoord(Bs.x, G.lo.x, G.h, G.dim[0]);
@@ -772,6 +786,12 @@ __global__ __launch_bounds__(TPB) void k_bp_query(Dev D, Params P, const Grid* g
     for (int q = b; q < e; ++q)
       if (boxOverlap(Bs, D.sbox[q])) bpStage(D, s, D.order[q], P.numReal, stage);
   }
+}
+__global__ __launch_bounds__(TPB) void k_bp_query(Dev D, Params P, const int* nSlots,
+                                                  const Grid* grid, int* stage) {
+  const int n = *nSlots;
+  const Grid G = *grid;
+  for (int k = gtid(); k < n; k += gridDim.x * blockDim.x) bpQuerySlot(D, P, G, k, stage);
 }
 __global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, const Grid* grid,
                                                     const int* stage) {

@@ -142,6 +142,8 @@ struct pd_sim {
   HzDev Z{};
   // sweep loops over at most this many rows run in one workgroup (k_vsolve_block/k_psolve_block)
   int blockRows = BLOCK_SOLVER_ROWS;
+  // extra slots reserved by setPositions for periodic ghosts (more are added when needed)
+  long slotHeadroom = 4096;
   // the largest scale, cached while no scale is written (every writer clears scaleKnown)
   bool scaleKnown = false;
   unsigned long long maxScaleKey = 0;
@@ -315,7 +317,8 @@ struct pd_sim {
     sbox.release(); vw0.release(); vw.release(); bc.release(); scale.release();
     targetScale.release(); pairTableD.release(); thermo.release(); realIdx.release();
     ground.release(); ground2.release(); cellOf.release(); cellRank.release(); cellCnt.release();
-    cellStart.release(); gridD.release(); ncellD.release(); cellCntDirty = true;
+    cellStart.release(); gridD.release(); ncellD.release(); slotInfo.release();
+    cellCntDirty = true;
     order.release(); pairCnt.release(); pairStart.release(); pairStage.release();
     pairOfContact.release(); matId.release(); mCode.release();
     vFlag.release(); pairs.release(); cAB.release(); mAB.release(); mReal.release();
@@ -687,7 +690,7 @@ struct pd_sim {
   void setPositions(const double* xyz, int n) {
     if (n < 0) throw ArgError("negative particle count");
     // headroom for the periodic ghosts (src/periodicity.hpp:25-38); grown on demand
-    const long est = (long)n + (long)n / 4 + 4096;
+    const long est = (long)n + (long)n / 4 + slotHeadroom;
     ensureSlots((int)std::max<long>(est, capacity), numReal);
     numReal = n;
     numParticles = n;
@@ -722,19 +725,21 @@ struct pd_sim {
 
   // uniform-grid broadphase over posPred (.w = radius) of numParticles slots with P.margin;
   // fills pairs / pairStart / np
-  void broadphase(const Params& P) {
-    // bounds of the box centres -> uniform grid (computed on the device, no read-back)
-    const int nbb = std::min(nblk(numParticles), RED_BLOCKS);
-    const long cellCap = 4L * numParticles + 64 + 1;  // the grid keeps cells <= 4 x slots + 64
+  // Broadphase up to the pair scan, without a read-back: the slot count lives on the device
+  // (slotInfo[0]), gridSlots sizes the launches (the slot loops are grid-strided, so any grid is
+  // correct) and capSlots bounds the slot count.  The pair total is at pairStart[numReal].
+  void broadphaseQuery(const Params& P, const int* nSlotsDev, int gridSlots, int capSlots) {
+    const int nbb = std::min(nblk(gridSlots), RED_BLOCKS);
+    const long cellCap = 4L * capSlots + 64 + 1;  // the grid keeps cells <= 4 x slots + 64
     redPart.ensure(8 * RED_BLOCKS, st);
     gridD.ensure(1, st);
     ncellD.ensure(1, st);
     cellCnt.ensure(cellCap, st);
     cellStart.ensure(cellCap + 1, st);
-    cellOf.ensure(numParticles, st);
-    cellRank.ensure(numParticles, st);
-    order.ensure(numParticles, st);
-    sbox.ensure(numParticles, st);
+    cellOf.ensure(capSlots, st);
+    cellRank.ensure(capSlots, st);
+    order.ensure(capSlots, st);
+    sbox.ensure(capSlots, st);
     pairCnt.ensure(numReal + 1, st);
     pairStart.ensure(numReal + 1, st);
     refreshDev();
@@ -742,24 +747,45 @@ struct pd_sim {
     // cut short by an error leaves it dirty
     if (cellCntDirty) zero(cellCnt.p, cellCnt.cap);
     cellCntDirty = true;
-    k_bp_bounds<<<nbb, TPB, 0, st>>>(D, P, redPart.p);
-    k_bp_bounds_final<<<1, TPB, 0, st>>>(redPart.p, nbb, red.p, numParticles, gridD.p, ncellD.p);
-    k_cell_count<<<nblk(numParticles), TPB, 0, st>>>(D, P, gridD.p);
+    k_bp_bounds<<<nbb, TPB, 0, st>>>(D, P, nSlotsDev, redPart.p);
+    k_bp_bounds_final<<<1, TPB, 0, st>>>(redPart.p, nbb, red.p, nSlotsDev, gridD.p, ncellD.p);
+    k_cell_count<<<nblk(gridSlots), TPB, 0, st>>>(D, P, nSlotsDev, gridD.p);
     scanDev(cellCnt.p, cellStart.p, ncellD.p, (int)cellCap);
-    k_cell_fill<<<nblk(numParticles), TPB, 0, st>>>(D, P);
+    k_cell_fill<<<nblk(gridSlots), TPB, 0, st>>>(D, P, nSlotsDev);
     HIPCHECK(hipGetLastError());
     cellCntDirty = false;
     pairStage.ensure((size_t)numReal * BP_CAP + 1, st);
     zero(pairCnt.p, numReal);
     kernBegin();
-    k_bp_query<<<nblk(numParticles), TPB, 0, st>>>(D, P, gridD.p, pairStage.p);
-    kernEnd(K_BPQUERY, numParticles);
-    np = scan(pairCnt.p, pairStart.p, numReal);
+    k_bp_query<<<nblk(gridSlots), TPB, 0, st>>>(D, P, nSlotsDev, gridD.p, pairStage.p);
+    kernEnd(K_BPQUERY, gridSlots);
+    scanQuiet(pairCnt.p, pairStart.p, numReal);
+  }
+  // the pair list from the staged partners, once the pair total np is known
+  void broadphaseCompact(const Params& P) {
     pairs.ensure(np + 1, st);
     refreshDev();
     k_bp_compact<<<nblk(numReal), TPB, 0, st>>>(D, P, gridD.p, pairStage.p);
     HIPCHECK(hipGetLastError());
   }
+  // the whole broadphase over the numParticles slots the host already knows; fills pairs / np
+  void broadphase(const Params& P) {
+    slotInfo.ensure(4, st);
+    k_set_int<<<1, 1, 0, st>>>(slotInfo.p, numParticles);
+    broadphaseQuery(P, slotInfo.p, numParticles, numParticles);
+    np = numReal > 0 ? read1(pairStart.p + numReal) : 0;
+    broadphaseCompact(P);
+  }
+  // per-slot capacity: every array the ghost emitter writes
+  int slotCapacity() const {
+    size_t c = pos.cap;
+    for (size_t x : {posPred.cap, quatPred.cap, bc.cap, vw0.cap, vw.cap, scale.cap, matId.cap,
+                     realIdx.cap})
+      c = std::min(c, x);
+    return (int)std::min<size_t>(c, (size_t)INT32_MAX);
+  }
+  DBuf<int> slotInfo;  // [0] slot count of the substep, [1] ghosts did not fit
+  int lastGhosts = 0;
   DBuf<Grid> gridD;
   DBuf<int> ncellD;
   bool cellCntDirty = true;
@@ -1157,18 +1183,39 @@ struct pd_sim {
     kernBegin();
     k_predict<<<nblk(numReal), TPB, 0, st>>>(D, P);
     kernEnd(K_PREDICT, numReal);
-    // periodic ghosts, src/periodicity.hpp:43-94
+    // periodic ghosts, src/periodicity.hpp:43-94.  The ghost total is read back together with
+    // the pair total: the broadphase runs on the device-side slot count, and if the ghosts did
+    // not fit the slot buffers they grow and the ghosts and the broadphase run again.
     ghostCnt.ensure(numReal + 1, st);
     ghostOff.ensure(numReal + 1, st);
+    slotInfo.ensure(4, st);
     k_ghost_count<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostCnt.p);
-    const int G = scan(ghostCnt.p, ghostOff.p, numReal);
-    if ((size_t)(numReal + G) > pos.cap) ensureSlots(numReal + G, numReal);
-    refreshDev();
-    k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostOff.p);
+    scanQuiet(ghostCnt.p, ghostOff.p, numReal);
+    const Params PB = params(margin, 0.0);
+    int G = 0;
+    for (;;) {
+      const int capSlots = slotCapacity();
+      k_slot_count<<<1, 1, 0, st>>>(ghostOff.p + numReal, numReal, capSlots, slotInfo.p);
+      refreshDev();
+      k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostOff.p, capSlots);
+      HIPCHECK(hipGetLastError());
+      mark(ST_BROAD);
+      const long gridSlots = std::min<long>(capSlots, (long)numReal + lastGhosts + 1024);
+      broadphaseQuery(PB, slotInfo.p, (int)gridSlots, capSlots);
+      size_t off = 0;
+      const int* npH = enqueueRead(pairStart.p + numReal, 1, off);
+      const int* gH = enqueueRead(ghostOff.p + numReal, 1, off);
+      sync();
+      np = *npH;
+      G = *gH;
+      if ((long)numReal + G <= capSlots) break;
+      ensureSlots(numReal + G, numReal);
+    }
+    lastGhosts = G;
     numParticles = numReal + G;
-    HIPCHECK(hipGetLastError());
-    mark(ST_BROAD);
-    broadAndNarrow(margin);
+    broadphaseCompact(PB);
+    mark(ST_NARROW);
+    narrowphase(PB);
     mark(ST_MANIFOLD);
     buildManifolds();
     if (legacyFriction) {
@@ -1750,7 +1797,7 @@ struct pd_sim {
     const int G = scan(ghostCnt.p, ghostOff.p, numReal);
     if ((size_t)(numReal + G) > pos.cap) ensureSlots(numReal + G, numReal);
     refreshDev();
-    k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostOff.p);
+    k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostOff.p, slotCapacity());
     numParticles = numReal + G;
     broadAndNarrow(margin);
     numParticles = numReal;
@@ -2540,6 +2587,9 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     if (k == "block_solver_rows") {
       if (value < 0) throw ArgError("block_solver_rows must be >= 0");
       s->blockRows = (int)std::min<long>(value, 1L << 30);
+    } else if (k == "slot_headroom") {
+      if (value < 0) throw ArgError("slot_headroom must be >= 0");
+      s->slotHeadroom = std::min<long>(value, 1L << 30);
     } else if (k == "timing_every") {
       if (value < 1) throw ArgError("timing_every must be >= 1");
       s->timingEvery = (int)std::min<long>(value, 1L << 30);

@@ -108,6 +108,20 @@ def make_scene(kind, n=256, seed=0):
             sc["pos"][:, 0] %= L
             sc["pos"][:, 1] %= L
         return sc
+    if kind == "column_small_box":
+        # a 2 x 2 column of spheres in a periodic box two diameters wide: every sphere has three
+        # periodic images
+        r = 1e-3
+        pos, v, L = lattice_bed(n, r, seed, box_xy=2, vel=0.02)
+        scale = rng.uniform(0.97, 1.03, n)
+        rad = r * scale
+        m = 2500.0 * 4.0 / 3.0 * np.pi * rad ** 3
+        return dict(shape=(SPHERE, r), domain=(L, L, (n / 4 + 8) * 2 * r),
+                    periodic=(True, True, False), planes=[(0, 0, 0, 0, 0, 1)], pos=pos,
+                    scale=scale, inv_mass=1.0 / m,
+                    inv_inertia=np.repeat((1.0 / (0.4 * m * rad ** 2))[:, None], 3, 1), vel=v,
+                    angvel=rng.normal(0, 5.0, (n, 3)), gravity=(0, 0, -9.81),
+                    material=(0.3, 0.2, 0.5), iters=(12, 8), dt=1e-5)
     if kind == "bed_walls":
         # non-periodic bed in an SDF box whose floor moves like a belt
         sc = make_scene("bed", n, seed)

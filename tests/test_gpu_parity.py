@@ -368,3 +368,25 @@ def test_sampled_timing(hip_lib):
     assert launches[q] == 4 and units[q] == 4 * 400
     with pytest.raises(ValueError):
         b.set_tuning("timing_every", 0)
+
+
+@pytest.mark.gpu
+def test_ghosts_outgrow_the_slot_buffers(hip_lib):
+    # 200 spheres with three periodic images each (600 ghosts) in slot buffers reserved for
+    # ~310 slots: the first substep finds the ghosts do not fit, grows the buffers and runs the
+    # ghosts and the broadphase again; every substep must still equal the oracle's
+    scene = make_scene("column_small_box", 200, 4)
+    h = hip_sim(200)
+    h.set_tuning("slot_headroom", 0)
+    h = apply_scene(h, scene)
+    r = apply_scene(oracle_sim(200), scene)
+    for k in range(6):
+        tag = f"column_small_box substep {k}"
+        print(tag, flush=True)
+        h.step(scene["dt"])
+        r.step(scene["dt"])
+        assert_stats(h, r, tag)
+        assert_topology(h, r, tag)
+        assert_state(h, r, tag)
+        assert_ledger(h, r, tag)
+    assert h.get_stats()["num_particles"] - 200 >= 600
