@@ -755,12 +755,8 @@ __device__ __forceinline__ int bpQueryOne(const Dev& D, const Grid& G, int i, co
 // overlap is symmetric).  The pair belongs to its lower slot, which must be real; the partner is
 // appended to that real's staged row with an atomic count.  pairCnt is zero on entry.  A real
 // with more than BP_CAP partners is queried again in full by k_bp_compact.
-__device__ __forceinline__ void bpStage(const Dev& D, int a, int b, int numReal, int* stage) {
-  const int lo = a < b ? a : b, hi = a < b ? b : a;
-  if (lo >= numReal) return;  // two ghosts
-  const int n = atomicAdd(&D.pairCnt[lo], 1);
-  if (n < BP_CAP) stage[(size_t)n * numReal + lo] = hi;
-}
+// Hits are buffered four at a time so their count atomics are in flight together (each returns
+// the partner's place in its row, and the store waits for it).
 __device__ __forceinline__ void bpQuerySlot(const Dev& D, const Params& P, const Grid& G, int k,
                                             int* stage) {
   const int s = D.order[k];
@@ -771,11 +767,39 @@ __device__ __forceinline__ void bpQuerySlot(const Dev& D, const Params& P, const
   const int x0 = cx > 0 ? cx - 1 : 0;
   const int x1 = cx + 1 < G.dim[0] ? cx + 1 : G.dim[0] - 1;
   const int row = (cz * G.dim[1] + cy) * G.dim[0];
+  const int numReal = P.numReal;
+  int lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0, lo3 = 0, hi3 = 0, nh = 0;
+  auto flush = [&]() __attribute__((always_inline)) {
+    int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (nh > 0) s0 = atomicAdd(&D.pairCnt[lo0], 1);
+    if (nh > 1) s1 = atomicAdd(&D.pairCnt[lo1], 1);
+    if (nh > 2) s2 = atomicAdd(&D.pairCnt[lo2], 1);
+    if (nh > 3) s3 = atomicAdd(&D.pairCnt[lo3], 1);
+    if (nh > 0 && s0 < BP_CAP) stage[(size_t)s0 * numReal + lo0] = hi0;
+    if (nh > 1 && s1 < BP_CAP) stage[(size_t)s1 * numReal + lo1] = hi1;
+    if (nh > 2 && s2 < BP_CAP) stage[(size_t)s2 * numReal + lo2] = hi2;
+    if (nh > 3 && s3 < BP_CAP) stage[(size_t)s3 * numReal + lo3] = hi3;
+    nh = 0;
+  };
+  auto hit = [&](int j) __attribute__((always_inline)) {
+    const int lo = s < j ? s : j, hi = s < j ? j : s;
+    if (lo >= numReal) return;  // two ghosts
+    // shift in (a register queue: no indexed store, which would go to scratch memory)
+    lo3 = lo2;
+    hi3 = hi2;
+    lo2 = lo1;
+    hi2 = hi1;
+    lo1 = lo0;
+    hi1 = hi0;
+    lo0 = lo;
+    hi0 = hi;
+    if (++nh == 4) flush();
+  };
   // own cell after this slot, then cell x+1
   {
     const int e = D.cellStart[row + x1 + 1];
     for (int q = k + 1; q < e; ++q)
-      if (boxOverlap(Bs, D.sbox[q])) bpStage(D, s, D.order[q], P.numReal, stage);
+      if (boxOverlap(Bs, D.sbox[q])) hit(D.order[q]);
   }
   for (int w = 0; w < 4; ++w) {
     const int y = w == 0 ? cy + 1 : cy + w - 2;
@@ -784,8 +808,9 @@ __device__ __forceinline__ void bpQuerySlot(const Dev& D, const Params& P, const
     const int r = (z * G.dim[1] + y) * G.dim[0];
     const int b = D.cellStart[r + x0], e = D.cellStart[r + x1 + 1];
     for (int q = b; q < e; ++q)
-      if (boxOverlap(Bs, D.sbox[q])) bpStage(D, s, D.order[q], P.numReal, stage);
+      if (boxOverlap(Bs, D.sbox[q])) hit(D.order[q]);
   }
+  flush();
 }
 __global__ __launch_bounds__(TPB) void k_bp_query(Dev D, Params P, const int* nSlots,
                                                   const Grid* grid, int* stage) {
