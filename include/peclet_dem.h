@@ -196,7 +196,9 @@ int pd_batch_step(pd_batch* b, double dt, int nsteps); /* nsteps substeps of eve
 int pd_set_timing(pd_sim* s, int level);
 /* Tuning (no reference counterpart): "block_solver_rows" = the largest velocity / position
  * sweep loop (in rows) solved inside one workgroup instead of one launch per colour (default
- * 16384; 0 disables); "slot_headroom" = slots pd_set_positions reserves beyond n + n/4 for
+ * 16384; 0 disables); "candidate_skin_permille" = the skin of the candidate pair list kept
+ * across substeps, in thousandths of the largest radius (default 100; 0 runs the broadphase
+ * every substep); "slot_headroom" = slots pd_set_positions reserves beyond n + n/4 for
  * periodic ghosts (default 4096; more are allocated when a substep needs them);
  * "timing_every" = N records timing events only on substeps 0, N, 2N, ...
  * after pd_set_timing (default 1), which bounds the events' host cost.  Results do not depend

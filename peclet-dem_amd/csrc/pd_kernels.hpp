@@ -33,6 +33,7 @@ struct BodyC {
 
 struct Params {
   double dt, gs, baseRadius, margin, band, boxCap, growthRate;
+  double skin;  // broadphase boxes grown by this much (candidate-list rebuilds), else 0
   double restN, restT, mu;
   d3 g;
   d3 dmin, dmax, dsize;
@@ -587,6 +588,12 @@ __device__ __forceinline__ double4 saneBox(double4 pp, double margin, double box
   }
   return make_double4(x, y, z, b);
 }
+// broadphase box: the sane box grown by P.skin unless parked (parking is the plain box's)
+__device__ __forceinline__ double4 bpBox(double4 pp, const Params& P) {
+  double4 B = saneBox(pp, P.margin, P.boxCap);
+  if (P.skin > 0.0 && B.x != -1.0e30) B.w += P.skin;
+  return B;
+}
 __device__ __forceinline__ bool boxOverlap(const double4& a, const double4& c) {
   if (a.x - a.w > c.x + c.w || a.x + a.w < c.x - c.w) return false;
   if (a.y - a.w > c.y + c.w || a.y + a.w < c.y - c.w) return false;
@@ -612,7 +619,7 @@ __global__ __launch_bounds__(TPB) void k_bp_bounds(Dev D, Params P, const int* n
   unsigned long long v[7] = {~0ull, ~0ull, ~0ull, 0, 0, 0, 0};
   const int n = *nSlots;
   for (int s = gtid(); s < n; s += gridDim.x * blockDim.x) {
-    const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
+    const double4 B = bpBox(D.posPred[s], P);
     if (isfinite(B.w)) {
       const unsigned long long k = ordKey(B.w);
       v[6] = k > v[6] ? k : v[6];
@@ -704,7 +711,7 @@ __global__ void k_cell_count(Dev D, Params P, const int* nSlots, const Grid* gri
   const int n = *nSlots;
   const Grid G = *grid;
   for (int s = gtid(); s < n; s += gridDim.x * blockDim.x) {
-    const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
+    const double4 B = bpBox(D.posPred[s], P);
     const int cx = cellCoord(B.x, G.lo.x, G.h, G.dim[0]);
     const int cy = cellCoord(B.y, G.lo.y, G.h, G.dim[1]);
     const int cz = cellCoord(B.z, G.lo.z, G.h, G.dim[2]);
@@ -720,7 +727,7 @@ __global__ void k_cell_fill(Dev D, Params P, const int* nSlots) {
     const int c = D.cellOf[s];
     const int k = D.cellStart[c] + D.cellRank[s];
     D.order[k] = s;
-    D.sbox[k] = saneBox(D.posPred[s], P.margin, P.boxCap);
+    D.sbox[k] = bpBox(D.posPred[s], P);
     D.cellCnt[c] = 0;
   }
 }
@@ -849,7 +856,7 @@ __global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, const Grid*
   if (n <= BP_CAP) {
     for (int a = 0; a < n; ++a) seg[a] = make_int2(i, stage[(size_t)a * P.numReal + i]);
   } else {
-    bpQueryOne(D, G, i, saneBox(D.posPred[i], P.margin, P.boxCap), seg);
+    bpQueryOne(D, G, i, bpBox(D.posPred[i], P), seg);
   }
   // ascending partner order
   for (int a = 1; a < n; ++a) {
@@ -861,6 +868,72 @@ __global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, const Grid*
     }
     seg[b + 1] = x;
   }
+}
+
+// ---- candidate pairs kept across substeps -------------------------------------------------
+// A rebuild runs the broadphase with every box grown by the skin; its pair list (sorted, per-real
+// offsets) is the candidate list, and each slot's plain box and each real's ghost images are
+// recorded.  While no box centre has moved by more than skin / 4 on any axis, no box has grown
+// by more than skin / 4 and every real has the same images, any two boxes that overlap now
+// overlapped grown by the skin at the rebuild (|c0i - c0j| <= |ci - cj| + skin / 2 <= wi + wj +
+// skin / 2 <= w0i + w0j + skin), so filtering the candidates with the exact test gives the exact
+// pair list, in the same order.
+__device__ __forceinline__ int ghostCode(const Params& P, d3 p) {
+  int sx, sy, sz;
+  ghostShifts(P, p, sx, sy, sz);
+  return (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+}
+__global__ void k_cand_ref(Dev D, Params P, int numSlots, double4* ref, unsigned char* code) {
+  const int s = gtid();
+  if (s >= numSlots) return;
+  ref[s] = saneBox(D.posPred[s], P.margin, P.boxCap);
+  if (s < P.numReal) code[s] = (unsigned char)ghostCode(P, xyz(D.pos[s]));
+}
+// flag = 1 when the candidates may miss a pair (or the slot count changed)
+__global__ __launch_bounds__(TPB) void k_cand_check(Dev D, Params P, const int* nSlots,
+                                                    int refSlots, const double4* ref,
+                                                    const unsigned char* code, double lim,
+                                                    int* flag) {
+  const int n = *nSlots;
+  bool bad = (n != refSlots);
+  for (int s = gtid(); s < n && !bad; s += gridDim.x * blockDim.x) {
+    const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
+    const double4 R = ref[s];
+    // written so that a NaN fails the test
+    if (!(fabs(B.x - R.x) <= lim && fabs(B.y - R.y) <= lim && fabs(B.z - R.z) <= lim &&
+          B.w <= R.w + lim))
+      bad = true;
+    if (s < P.numReal && code[s] != (unsigned char)ghostCode(P, xyz(D.pos[s]))) bad = true;
+  }
+  if (bad) atomicOr(flag, 1);
+}
+// the exact test on every candidate of real i (the broadphase's own box test); keep[c] marks
+// the pairs, pairCnt[i] counts them
+__global__ __launch_bounds__(TPB) void k_cand_count(Dev D, Params P, const int2* cand,
+                                                    const int* candStart, unsigned char* keep) {
+  const int i = gtid();
+  if (i >= P.numReal) return;
+  const int b = candStart[i], e = candStart[i + 1];
+  int n = 0;
+  if (e > b) {
+    const double4 Bi = saneBox(D.posPred[i], P.margin, P.boxCap);
+    for (int c = b; c < e; ++c) {
+      const bool k = boxOverlap(Bi, saneBox(D.posPred[cand[c].y], P.margin, P.boxCap));
+      keep[c] = k ? 1 : 0;
+      n += k;
+    }
+  }
+  D.pairCnt[i] = n;
+}
+__global__ __launch_bounds__(TPB) void k_cand_write(Dev D, int numReal, const int2* cand,
+                                                    const int* candStart,
+                                                    const unsigned char* keep) {
+  const int i = gtid();
+  if (i >= numReal) return;
+  int2* out = D.pairs + D.pairStart[i];
+  int n = 0;
+  for (int c = candStart[i], e = candStart[i + 1]; c < e; ++c)
+    if (keep[c]) out[n++] = cand[c];
 }
 
 // ============================================================================================

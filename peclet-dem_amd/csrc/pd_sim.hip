@@ -319,6 +319,8 @@ struct pd_sim {
     ground.release(); ground2.release(); cellOf.release(); cellRank.release(); cellCnt.release();
     cellStart.release(); gridD.release(); ncellD.release(); slotInfo.release();
     cellCntDirty = true;
+    cand.release(); candStart.release(); candFlag.release(); candKeep.release();
+    candCode.release(); candRef.release(); candValid = false;
     order.release(); pairCnt.release(); pairStart.release(); pairStage.release();
     pairOfContact.release(); matId.release(); mCode.release();
     vFlag.release(); pairs.release(); cAB.release(); mAB.release(); mReal.release();
@@ -695,6 +697,7 @@ struct pd_sim {
     numReal = n;
     numParticles = n;
     hNumPairs = -1;  // particle indices changed: the Hertz pair cache is stale, :1294-1295
+    candValid = false;
     hPrevCount = 0;
     std::vector<double4> P((size_t)n);
     std::vector<BodyC> B((size_t)n);
@@ -785,6 +788,17 @@ struct pd_sim {
     return (int)std::min<size_t>(c, (size_t)INT32_MAX);
   }
   DBuf<int> slotInfo;  // [0] slot count of the substep, [1] ghosts did not fit
+  // candidate pairs of the last rebuild (boxes grown by candSkin), sorted with per-real offsets,
+  // the slots' plain boxes and the reals' ghost images at that rebuild
+  DBuf<int2> cand;
+  DBuf<int> candStart, candFlag;
+  DBuf<unsigned char> candKeep, candCode;
+  DBuf<double4> candRef;
+  bool candValid = false;
+  int candSlots = 0, candNumReal = -1;
+  double candSkin = 0.0, candMargin = 0.0;
+  double candSkinFrac = 0.1;  // skin = this x the largest radius (0: rebuild every substep)
+  long candRebuilds = 0, candReuses = 0;
   int lastGhosts = 0;
   DBuf<Grid> gridD;
   DBuf<int> ncellD;
@@ -1192,6 +1206,11 @@ struct pd_sim {
     k_ghost_count<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostCnt.p);
     scanQuiet(ghostCnt.p, ghostOff.p, numReal);
     const Params PB = params(margin, 0.0);
+    // candidate pairs kept across substeps (grown boxes): used while they provably hold every
+    // pair, rebuilt otherwise; the pair list is the same either way
+    const double skin = candSkinFrac * maxRad;
+    bool useCand = candValid && skin > 0.0 && candNumReal == numReal && candSkin == skin &&
+                   candMargin == margin;
     int G = 0;
     for (;;) {
       const int capSlots = slotCapacity();
@@ -1201,19 +1220,77 @@ struct pd_sim {
       HIPCHECK(hipGetLastError());
       mark(ST_BROAD);
       const long gridSlots = std::min<long>(capSlots, (long)numReal + lastGhosts + 1024);
-      broadphaseQuery(PB, slotInfo.p, (int)gridSlots, capSlots);
       size_t off = 0;
+      if (useCand) {
+        // speculative: filter the candidates and check, in the same read-back, that they hold
+        zero(candFlag.p, 1);
+        pairCnt.ensure(numReal + 1, st);
+        pairStart.ensure(numReal + 1, st);
+        refreshDev();
+        k_cand_check<<<std::min(nblk(gridSlots), 2048), TPB, 0, st>>>(
+            D, P, slotInfo.p, candSlots, candRef.p, candCode.p, 0.25 * skin, candFlag.p);
+        k_cand_count<<<nblk(numReal), TPB, 0, st>>>(D, PB, cand.p, candStart.p, candKeep.p);
+        scanQuiet(pairCnt.p, pairStart.p, numReal);
+      } else {
+        Params PS = PB;
+        PS.skin = skin;
+        broadphaseQuery(PS, slotInfo.p, (int)gridSlots, capSlots);
+      }
       const int* npH = enqueueRead(pairStart.p + numReal, 1, off);
       const int* gH = enqueueRead(ghostOff.p + numReal, 1, off);
+      const int* fH = useCand ? enqueueRead(candFlag.p, 1, off) : nullptr;
       sync();
       np = *npH;
       G = *gH;
-      if ((long)numReal + G <= capSlots) break;
-      ensureSlots(numReal + G, numReal);
+      const bool stale = fH && *fH;
+      if ((long)numReal + G > capSlots) {
+        ensureSlots(numReal + G, numReal);
+        continue;
+      }
+      if (stale) {
+        useCand = false;  // a box moved too far or the images changed: rebuild
+        continue;
+      }
+      break;
     }
     lastGhosts = G;
     numParticles = numReal + G;
-    broadphaseCompact(PB);
+    if (useCand) {
+      ++candReuses;
+      pairs.ensure(np + 1, st);
+      refreshDev();
+      k_cand_write<<<nblk(numReal), TPB, 0, st>>>(D, numReal, cand.p, candStart.p, candKeep.p);
+      HIPCHECK(hipGetLastError());
+    } else if (skin > 0.0) {
+      // rebuild: the grown broadphase's pairs become the candidates, then the exact filter
+      Params PS = PB;
+      PS.skin = skin;
+      broadphaseCompact(PS);
+      std::swap(cand, pairs);
+      std::swap(candStart, pairStart);
+      candKeep.ensure(np + 1, st);
+      candRef.ensure(numParticles + 1, st);
+      candCode.ensure(numReal + 1, st);
+      candFlag.ensure(1, st);
+      pairCnt.ensure(numReal + 1, st);
+      pairStart.ensure(numReal + 1, st);
+      refreshDev();
+      k_cand_ref<<<nblk(numParticles), TPB, 0, st>>>(D, P, numParticles, candRef.p, candCode.p);
+      k_cand_count<<<nblk(numReal), TPB, 0, st>>>(D, PB, cand.p, candStart.p, candKeep.p);
+      np = scan(pairCnt.p, pairStart.p, numReal);
+      pairs.ensure(np + 1, st);
+      refreshDev();
+      k_cand_write<<<nblk(numReal), TPB, 0, st>>>(D, numReal, cand.p, candStart.p, candKeep.p);
+      HIPCHECK(hipGetLastError());
+      candValid = true;
+      candSlots = numParticles;
+      candNumReal = numReal;
+      candSkin = skin;
+      candMargin = margin;
+      ++candRebuilds;
+    } else {
+      broadphaseCompact(PB);
+    }
     mark(ST_NARROW);
     narrowphase(PB);
     mark(ST_MANIFOLD);
@@ -2587,6 +2664,10 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     if (k == "block_solver_rows") {
       if (value < 0) throw ArgError("block_solver_rows must be >= 0");
       s->blockRows = (int)std::min<long>(value, 1L << 30);
+    } else if (k == "candidate_skin_permille") {
+      if (value < 0) throw ArgError("candidate_skin_permille must be >= 0");
+      s->candSkinFrac = std::min<long>(value, 100000) / 1000.0;
+      s->candValid = false;
     } else if (k == "slot_headroom") {
       if (value < 0) throw ArgError("slot_headroom must be >= 0");
       s->slotHeadroom = std::min<long>(value, 1L << 30);

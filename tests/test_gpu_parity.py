@@ -391,3 +391,21 @@ def test_ghosts_outgrow_the_slot_buffers(hip_lib):
         assert_topology(h, r, tag)
         assert_state(h, r, tag)
         assert_ledger(h, r, tag)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["bed_periodic", "gas"])
+def test_candidate_pairs_equal_a_fresh_broadphase(hip_lib, kind):
+    # the candidate list kept across substeps (default) against a broadphase every substep
+    # (skin 0): identical pair lists and identical state, substep after substep
+    scene = make_scene(kind, 400, 7)
+    a = apply_scene(hip_sim(400), scene)
+    b = hip_sim(400)
+    b.set_tuning("candidate_skin_permille", 0)
+    b = apply_scene(b, scene)
+    for k in range(30):
+        a.step(scene["dt"])
+        b.step(scene["dt"])
+        assert np.array_equal(a.debug_pairs(), b.debug_pairs()), (kind, k)
+    for name in ["get_positions", "get_velocities", "get_angular_velocities"]:
+        assert np.array_equal(getattr(a, name)(), getattr(b, name)()), name
