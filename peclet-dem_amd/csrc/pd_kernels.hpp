@@ -889,34 +889,38 @@ __global__ void k_cand_ref(Dev D, Params P, int numSlots, double4* ref, unsigned
   ref[s] = saneBox(D.posPred[s], P.margin, P.boxCap);
   if (s < P.numReal) code[s] = (unsigned char)ghostCode(P, xyz(D.pos[s]));
 }
-// flag = 1 when the candidates may miss a pair (or the slot count changed)
+// whether slot s's plain box B has left its rebuild box R by more than lim (written so that a
+// NaN counts as moved)
+__device__ __forceinline__ bool candMoved(const double4& B, const double4& R, double lim) {
+  return !(fabs(B.x - R.x) <= lim && fabs(B.y - R.y) <= lim && fabs(B.z - R.z) <= lim &&
+           B.w <= R.w + lim);
+}
+// flag = 1 when a ghost slot moved or the slot count changed (the reals are checked by
+// k_cand_count)
 __global__ __launch_bounds__(TPB) void k_cand_check(Dev D, Params P, const int* nSlots,
-                                                    int refSlots, const double4* ref,
-                                                    const unsigned char* code, double lim,
+                                                    int refSlots, const double4* ref, double lim,
                                                     int* flag) {
   const int n = *nSlots;
   bool bad = (n != refSlots);
-  for (int s = gtid(); s < n && !bad; s += gridDim.x * blockDim.x) {
-    const double4 B = saneBox(D.posPred[s], P.margin, P.boxCap);
-    const double4 R = ref[s];
-    // written so that a NaN fails the test
-    if (!(fabs(B.x - R.x) <= lim && fabs(B.y - R.y) <= lim && fabs(B.z - R.z) <= lim &&
-          B.w <= R.w + lim))
-      bad = true;
-    if (s < P.numReal && code[s] != (unsigned char)ghostCode(P, xyz(D.pos[s]))) bad = true;
-  }
+  for (int s = P.numReal + gtid(); s < n && !bad; s += gridDim.x * blockDim.x)
+    bad = candMoved(saneBox(D.posPred[s], P.margin, P.boxCap), ref[s], lim);
   if (bad) atomicOr(flag, 1);
 }
 // the exact test on every candidate of real i (the broadphase's own box test); keep[c] marks
-// the pairs, pairCnt[i] counts them
+// the pairs, pairCnt[i] counts them.  With a flag, real i's own box and images are checked too.
 __global__ __launch_bounds__(TPB) void k_cand_count(Dev D, Params P, const int2* cand,
-                                                    const int* candStart, unsigned char* keep) {
+                                                    const int* candStart, unsigned char* keep,
+                                                    const double4* ref, const unsigned char* code,
+                                                    double lim, int* flag) {
   const int i = gtid();
   if (i >= P.numReal) return;
   const int b = candStart[i], e = candStart[i + 1];
   int n = 0;
+  const double4 Bi = saneBox(D.posPred[i], P.margin, P.boxCap);
+  if (flag && (candMoved(Bi, ref[i], lim) ||
+               code[i] != (unsigned char)ghostCode(P, xyz(D.pos[i]))))
+    atomicOr(flag, 1);
   if (e > b) {
-    const double4 Bi = saneBox(D.posPred[i], P.margin, P.boxCap);
     for (int c = b; c < e; ++c) {
       const bool k = boxOverlap(Bi, saneBox(D.posPred[cand[c].y], P.margin, P.boxCap));
       keep[c] = k ? 1 : 0;
@@ -2501,6 +2505,46 @@ __global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psolve_block(
       if (threadIdx.x == 0) *stop = 1;
       return;
     }
+  }
+}
+
+// The small colours of one position sweep in one workgroup, then the sweep's stop check: the
+// colours from cFrom on run in order separated by barriers, the residual is the max of theirs
+// and what the launches before them accumulated in *maxOverlapBits; the accumulator is cleared
+// for the next sweep unless the loop ends here (stop, or the cap of sweeps reached).
+__global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psweep_tail(
+    Dev D, ClassStarts cs, int cFrom, int numColors, int numContacts, double tol, int* stop,
+    int* count, int cap) {
+  __shared__ unsigned long long sh[BLOCK_SOLVER_THREADS / 64];
+  if (*stop) return;
+  unsigned long long m = 0;
+  for (int c = cFrom; c < numColors; ++c) {
+    const int b = cs.s[c], e = cs.s[c + 1];
+    if (e <= b) continue;
+    if (c == 62) {
+      if (threadIdx.x == 0)
+        for (int q = 0; q < numContacts; ++q)
+          if (D.cColor[q] == c) {
+            const unsigned long long a = dbits(posRow(D, D.pRowOf[q]));
+            m = a > m ? a : m;
+          }
+    } else {
+      for (int k = b + (int)threadIdx.x; k < e; k += blockDim.x) {
+        const unsigned long long a = dbits(posRow(D, k));
+        m = a > m ? a : m;
+      }
+    }
+    __syncthreads();
+  }
+  unsigned long long M = blockMaxAll(m, sh);
+  if (threadIdx.x == 0) {
+    const unsigned long long before = *D.maxOverlapBits;
+    M = before > M ? before : M;
+    const int n = *count + 1;
+    *count = n;
+    const bool done = bitsd(M) < tol;
+    if (done) *stop = 1;
+    *D.maxOverlapBits = (done || n >= cap) ? M : 0ull;
   }
 }
 

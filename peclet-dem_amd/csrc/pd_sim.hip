@@ -1152,6 +1152,7 @@ struct pd_sim {
 
   long vBlockRec[2] = {-1, -1}, pBlockRec = -1;
   int posSweepsPrev = 1;  // sweeps of the last batched position loop (first batch size)
+  static constexpr int POS_TAIL_ROWS = 8192;
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
     vBlockRec[0] = vBlockRec[1] = pBlockRec = -1;
@@ -1227,9 +1228,11 @@ struct pd_sim {
         pairCnt.ensure(numReal + 1, st);
         pairStart.ensure(numReal + 1, st);
         refreshDev();
-        k_cand_check<<<std::min(nblk(gridSlots), 2048), TPB, 0, st>>>(
-            D, P, slotInfo.p, candSlots, candRef.p, candCode.p, 0.25 * skin, candFlag.p);
-        k_cand_count<<<nblk(numReal), TPB, 0, st>>>(D, PB, cand.p, candStart.p, candKeep.p);
+        k_cand_check<<<std::min(nblk(lastGhosts + 1024), 1024), TPB, 0, st>>>(
+            D, P, slotInfo.p, candSlots, candRef.p, 0.25 * skin, candFlag.p);
+        k_cand_count<<<nblk(numReal), TPB, 0, st>>>(D, P, cand.p, candStart.p, candKeep.p,
+                                                    candRef.p, candCode.p, 0.25 * skin,
+                                                    candFlag.p);
         scanQuiet(pairCnt.p, pairStart.p, numReal);
       } else {
         Params PS = PB;
@@ -1276,7 +1279,8 @@ struct pd_sim {
       pairStart.ensure(numReal + 1, st);
       refreshDev();
       k_cand_ref<<<nblk(numParticles), TPB, 0, st>>>(D, P, numParticles, candRef.p, candCode.p);
-      k_cand_count<<<nblk(numReal), TPB, 0, st>>>(D, PB, cand.p, candStart.p, candKeep.p);
+      k_cand_count<<<nblk(numReal), TPB, 0, st>>>(D, P, cand.p, candStart.p, candKeep.p, nullptr,
+                                                  nullptr, 0.0, nullptr);
       np = scan(pairCnt.p, pairStart.p, numReal);
       pairs.ensure(np + 1, st);
       refreshDev();
@@ -1570,12 +1574,17 @@ struct pd_sim {
       // batches of device-gated sweeps, the first as long as the previous substep's loop; each
       // batch ends with one read of the loop counters and residuals, which is also the substep's
       // final read-back once the loop has stopped
+      // the trailing colours holding at most POS_TAIL_ROWS rows together run in one workgroup
+      // launch that also ends the sweep (stop check, accumulator cleared for the next sweep)
+      int cFrom = numPosColors;
+      while (cFrom > 0 && ps[numPosColors] - ps[cFrom - 1] <= POS_TAIL_ROWS) --cFrom;
+      const ClassStarts pcs = classStarts(ps);
       int done = 0, batch = std::max(1, std::min(posIters, posSweepsPrev));
+      k_sweep_begin<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P);
       while (done < posIters) {
         const int b = std::min(batch, posIters - done);
         for (int j = 0; j < b; ++j) {
-          k_sweep_begin<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P);
-          for (int c = 0; c < numPosColors; ++c) {
+          for (int c = 0; c < cFrom; ++c) {
             const int cb = ps[c], ce = ps[c + 1];
             if (ce <= cb) continue;
             if (c == 62) {
@@ -1586,8 +1595,9 @@ struct pd_sim {
               kernEnd(K_PSWEEP, ce - cb);
             }
           }
-          k_sweep_check<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P, ctl.p + CTL_SWEEPS_P,
-                                         posTol, 1);
+          k_psweep_tail<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(D, pcs, cFrom, numPosColors, nc, posTol,
+                                                            ctl.p + CTL_STOP_P,
+                                                            ctl.p + CTL_SWEEPS_P, posIters);
         }
         HIPCHECK(hipGetLastError());
         done += b;
