@@ -140,7 +140,8 @@ __device__ __forceinline__ d3 hertzForce(double delta, d3 nhat, d3 vrel, double 
 }
 
 // radii into posPred.w for the broadphase; minimum radius as an ordered key
-__global__ __launch_bounds__(TPB) void k_hz_prep(Dev D, HzParams H, unsigned long long* minKey) {
+__global__ __launch_bounds__(TPB) void k_hz_prep(const Dev* __restrict__ Dp, HzParams H, unsigned long long* minKey) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[WAVES];
   const int i = gtid();
   unsigned long long k = ~0ull;
@@ -171,7 +172,8 @@ __global__ void k_hz_commit(HzDev Z, int n) {
   Z.prevXi[q] = Z.xi[p];
 }
 // new list: copy, carry history by key, clear the patch stiffness (src/sim.hpp:559-590)
-__global__ void k_hz_carry(Dev D, HzDev Z, int np, int prevCount) {
+__global__ void k_hz_carry(const Dev* __restrict__ Dp, HzDev Z, int np, int prevCount) {
+  const Dev& D = *Dp;
   const int p = gtid();
   if (p >= np) return;
   const int2 ab = D.pairs[p];
@@ -214,14 +216,16 @@ __global__ void k_hz_jsort(HzDev Z, HzParams H) {
     seg[c + 1] = x;
   }
 }
-__global__ void k_hz_ref(Dev D, HzDev Z, HzParams H) {
+__global__ void k_hz_ref(const Dev* __restrict__ Dp, HzDev Z, HzParams H) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= H.numReal) return;
   Z.refPos[i] = D.pos[i];
 }
 // wall candidates: bodies within radius + skin of a wall's zero level (src/solver_hertz.hpp:343-370)
 template <bool WRITE>
-__global__ void k_hz_wallcand(Dev D, HzDev Z, HzParams H, double skin, int* cnt) {
+__global__ void k_hz_wallcand(const Dev* __restrict__ Dp, HzDev Z, HzParams H, double skin, int* cnt) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= H.numReal) return;
   const double4 pp = D.posPred[i];
@@ -239,7 +243,8 @@ __global__ void k_hz_wallcand(Dev D, HzDev Z, HzParams H, double skin, int* cnt)
 }
 
 // spheres, src/solver_hertz.hpp:88-144
-__global__ __launch_bounds__(TPB) void k_hz_pairs(Dev D, HzDev Z, HzParams H) {
+__global__ __launch_bounds__(TPB) void k_hz_pairs(const Dev* __restrict__ Dp, HzDev Z, HzParams H) {
+  const Dev& D = *Dp;
   const int idx = gtid();
   if (idx >= H.numPairs) return;
   const int2 ab = Z.pairs[idx];
@@ -275,7 +280,8 @@ __global__ __launch_bounds__(TPB) void k_hz_pairs(Dev D, HzDev Z, HzParams H) {
 }
 
 // point-shell shapes, src/solver_hertz.hpp:454-606
-__global__ __launch_bounds__(TPB) void k_hz_shape_pairs(Dev D, HzDev Z, HzParams H) {
+__global__ __launch_bounds__(TPB) void k_hz_shape_pairs(const Dev* __restrict__ Dp, HzDev Z, HzParams H) {
+  const Dev& D = *Dp;
   const int idx = gtid();
   if (idx >= H.numPairs) return;
   const int2 ab = Z.pairs[idx];
@@ -377,7 +383,8 @@ __global__ __launch_bounds__(TPB) void k_hz_shape_pairs(Dev D, HzDev Z, HzParams
 }
 
 // wall forces per candidate, src/solver_hertz.hpp:148-338
-__global__ __launch_bounds__(TPB) void k_hz_walls(Dev D, HzDev Z, HzParams H, int numCand) {
+__global__ __launch_bounds__(TPB) void k_hz_walls(const Dev* __restrict__ Dp, HzDev Z, HzParams H, int numCand) {
+  const Dev& D = *Dp;
   const int ci = gtid();
   if (ci >= numCand) return;
   const int enc = Z.wCand[ci];
@@ -486,7 +493,8 @@ __global__ __launch_bounds__(TPB) void k_hz_walls(Dev D, HzDev Z, HzParams H, in
 }
 
 // gather in serial order, then symplectic Euler (src/solver_hertz.hpp:374-422)
-__global__ __launch_bounds__(TPB) void k_hz_integrate(Dev D, HzDev Z, HzParams H) {
+__global__ __launch_bounds__(TPB) void k_hz_integrate(const Dev* __restrict__ Dp, HzDev Z, HzParams H) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= H.numReal) return;
   d3 F{0.0, 0.0, 0.0}, T{0.0, 0.0, 0.0};
@@ -549,8 +557,9 @@ __global__ __launch_bounds__(TPB) void k_hz_integrate(Dev D, HzDev Z, HzParams H
 }
 
 // max |pos - ref|^2 since the last build (ordered bits of a non-negative double)
-__global__ __launch_bounds__(TPB) void k_hz_disp(Dev D, HzDev Z, HzParams H,
+__global__ __launch_bounds__(TPB) void k_hz_disp(const Dev* __restrict__ Dp, HzDev Z, HzParams H,
                                                  unsigned long long* out) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[WAVES];
   unsigned long long m = 0;
   for (int i = gtid(); i < H.numReal; i += gridDim.x * blockDim.x) {
@@ -571,7 +580,8 @@ struct WallSet {
   const double* grid;
 };
 template <bool WRITE>
-__global__ __launch_bounds__(TPB) void k_np_walls(Dev D, Params P, WallSet W, int slotBase) {
+__global__ __launch_bounds__(TPB) void k_np_walls(const Dev* __restrict__ Dp, Params P, WallSet W, int slotBase) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[WAVES];
   const int i = gtid();
   double ovl = 0.0;

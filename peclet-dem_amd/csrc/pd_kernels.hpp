@@ -364,7 +364,8 @@ __global__ void k_growth(double* scale, const double* target, double factor, int
 }
 
 // predictVelocityKokkos, src/integration.hpp:44-93 (+ realIndices and broadphase radius fill)
-__global__ void k_predict(Dev D, Params P) {
+__global__ void k_predict(const Dev* __restrict__ Dp, Params P) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   const BodyC b = D.bc[i];
@@ -404,7 +405,8 @@ __global__ void k_predict(Dev D, Params P) {
 }
 
 // computeOverlapsKokkos preparation, src/sim.hpp:494-510: predicted := committed
-__global__ void k_overlap_prep(Dev D, Params P) {
+__global__ void k_overlap_prep(const Dev* __restrict__ Dp, Params P) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   const double rad = D.scale[i] * P.gs * P.baseRadius;
@@ -421,7 +423,8 @@ __device__ __forceinline__ void ghostShifts(const Params& P, d3 p, int& sx, int&
   sz = P.pz ? ((p.z < P.dmin.z + skin) ? 1 : ((p.z > P.dmax.z - skin) ? -1 : 0)) : 0;
 }
 // generateGhostsKokkos, src/periodicity.hpp:43-94 — pass 1: ghost count per real
-__global__ void k_ghost_count(Dev D, Params P, int* cnt) {
+__global__ void k_ghost_count(const Dev* __restrict__ Dp, Params P, int* cnt) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   int sx, sy, sz;
@@ -430,7 +433,8 @@ __global__ void k_ghost_count(Dev D, Params P, int* cnt) {
 }
 // pass 2: emit ghosts at numReal + off[i] in the (a,b,c) loop order
 // slots at or past capSlots are not written (the host grows the buffers and emits again)
-__global__ void k_ghost_emit(Dev D, Params P, const int* off, int capSlots) {
+__global__ void k_ghost_emit(const Dev* __restrict__ Dp, Params P, const int* off, int capSlots) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   const d3 p = xyz(D.pos[i]);
@@ -480,7 +484,8 @@ __global__ void k_slot_count(const int* ghostTotal, int numReal, int capSlots, i
 __global__ void k_set_int(int* dst, int v) { *dst = v; }
 
 // applyVelocityAndPredictPositionKokkos, src/integration.hpp:112-148 (reals and ghosts)
-__global__ void k_reintegrate(Dev D, Params P) {
+__global__ void k_reintegrate(const Dev* __restrict__ Dp, Params P) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numParticles) return;
   const double dt = P.dt;
@@ -514,7 +519,8 @@ __global__ void k_reintegrate(Dev D, Params P) {
 }
 
 // finalCommitKokkos, src/integration.hpp:187-215
-__global__ void k_final_commit(Dev D, Params P) {
+__global__ void k_final_commit(const Dev* __restrict__ Dp, Params P) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   d3 p = xyz(D.posPred[i]);
@@ -535,7 +541,8 @@ __global__ void k_final_commit(Dev D, Params P) {
 }
 
 // applyThermostatKokkos, src/integration.hpp:229-271 — per-block partial energies (fixed order)
-__global__ void k_thermo_partial(Dev D, Params P, double* part) {
+__global__ void k_thermo_partial(const Dev* __restrict__ Dp, Params P, double* part) {
+  const Dev& D = *Dp;
   __shared__ double st[TPB], sr[TPB];
   const int i = gtid();
   double kt = 0.0, kr = 0.0;
@@ -567,7 +574,8 @@ __global__ void k_thermo_partial(Dev D, Params P, double* part) {
     part[2 * blockIdx.x + 1] = c;
   }
 }
-__global__ void k_thermo_scale(Dev D, Params P, double lt, double lr) {
+__global__ void k_thermo_scale(const Dev* __restrict__ Dp, Params P, double lt, double lr) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   Body64 v = D.vw0[i];
@@ -613,8 +621,9 @@ __device__ __forceinline__ unsigned long long waveMinU64(unsigned long long v) {
 constexpr int RED_BLOCKS = 256;
 // part[b * 8 + 0..2] = ordered min, [3..5] = ordered max, [6] = ordered max half-width
 // nSlots: the slot count on the device (the host may not know it yet)
-__global__ __launch_bounds__(TPB) void k_bp_bounds(Dev D, Params P, const int* nSlots,
+__global__ __launch_bounds__(TPB) void k_bp_bounds(const Dev* __restrict__ Dp, Params P, const int* nSlots,
                                                    unsigned long long* part) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[7][WAVES];
   unsigned long long v[7] = {~0ull, ~0ull, ~0ull, 0, 0, 0, 0};
   const int n = *nSlots;
@@ -707,7 +716,8 @@ __device__ __forceinline__ int cellCoord(double c, double lo, double h, int dim)
   return (int)f;
 }
 // cell of every slot and its rank inside the cell (cellCnt is all zero on entry)
-__global__ void k_cell_count(Dev D, Params P, const int* nSlots, const Grid* grid) {
+__global__ void k_cell_count(const Dev* __restrict__ Dp, Params P, const int* nSlots, const Grid* grid) {
+  const Dev& D = *Dp;
   const int n = *nSlots;
   const Grid G = *grid;
   for (int s = gtid(); s < n; s += gridDim.x * blockDim.x) {
@@ -721,7 +731,8 @@ __global__ void k_cell_count(Dev D, Params P, const int* nSlots, const Grid* gri
   }
 }
 // slots in cell order; every occupied cell's count is cleared again for the next broadphase
-__global__ void k_cell_fill(Dev D, Params P, const int* nSlots) {
+__global__ void k_cell_fill(const Dev* __restrict__ Dp, Params P, const int* nSlots) {
+  const Dev& D = *Dp;
   const int n = *nSlots;
   for (int s = gtid(); s < n; s += gridDim.x * blockDim.x) {
     const int c = D.cellOf[s];
@@ -819,14 +830,16 @@ __device__ __forceinline__ void bpQuerySlot(const Dev& D, const Params& P, const
   }
   flush();
 }
-__global__ __launch_bounds__(TPB) void k_bp_query(Dev D, Params P, const int* nSlots,
+__global__ __launch_bounds__(TPB) void k_bp_query(const Dev* __restrict__ Dp, Params P, const int* nSlots,
                                                   const Grid* grid, int* stage) {
+  const Dev& D = *Dp;
   const int n = *nSlots;
   const Grid G = *grid;
   for (int k = gtid(); k < n; k += gridDim.x * blockDim.x) bpQuerySlot(D, P, G, k, stage);
 }
-__global__ __launch_bounds__(TPB) void k_bp_compact(Dev D, Params P, const Grid* grid,
+__global__ __launch_bounds__(TPB) void k_bp_compact(const Dev* __restrict__ Dp, Params P, const Grid* grid,
                                                     const int* stage) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   const Grid G = *grid;
@@ -883,7 +896,8 @@ __device__ __forceinline__ int ghostCode(const Params& P, d3 p) {
   ghostShifts(P, p, sx, sy, sz);
   return (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
 }
-__global__ void k_cand_ref(Dev D, Params P, int numSlots, double4* ref, unsigned char* code) {
+__global__ void k_cand_ref(const Dev* __restrict__ Dp, Params P, int numSlots, double4* ref, unsigned char* code) {
+  const Dev& D = *Dp;
   const int s = gtid();
   if (s >= numSlots) return;
   ref[s] = saneBox(D.posPred[s], P.margin, P.boxCap);
@@ -897,9 +911,10 @@ __device__ __forceinline__ bool candMoved(const double4& B, const double4& R, do
 }
 // flag = 1 when a ghost slot moved or the slot count changed (the reals are checked by
 // k_cand_count)
-__global__ __launch_bounds__(TPB) void k_cand_check(Dev D, Params P, const int* nSlots,
+__global__ __launch_bounds__(TPB) void k_cand_check(const Dev* __restrict__ Dp, Params P, const int* nSlots,
                                                     int refSlots, const double4* ref, double lim,
                                                     int* flag) {
+  const Dev& D = *Dp;
   const int n = *nSlots;
   bool bad = (n != refSlots);
   for (int s = P.numReal + gtid(); s < n && !bad; s += gridDim.x * blockDim.x)
@@ -908,10 +923,11 @@ __global__ __launch_bounds__(TPB) void k_cand_check(Dev D, Params P, const int* 
 }
 // the exact test on every candidate of real i (the broadphase's own box test); keep[c] marks
 // the pairs, pairCnt[i] counts them.  With a flag, real i's own box and images are checked too.
-__global__ __launch_bounds__(TPB) void k_cand_count(Dev D, Params P, const int2* cand,
+__global__ __launch_bounds__(TPB) void k_cand_count(const Dev* __restrict__ Dp, Params P, const int2* cand,
                                                     const int* candStart, unsigned char* keep,
                                                     const double4* ref, const unsigned char* code,
                                                     double lim, int* flag) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   const int b = candStart[i], e = candStart[i + 1];
@@ -929,9 +945,10 @@ __global__ __launch_bounds__(TPB) void k_cand_count(Dev D, Params P, const int2*
   }
   D.pairCnt[i] = n;
 }
-__global__ __launch_bounds__(TPB) void k_cand_write(Dev D, int numReal, const int2* cand,
+__global__ __launch_bounds__(TPB) void k_cand_write(const Dev* __restrict__ Dp, int numReal, const int2* cand,
                                                     const int* candStart,
                                                     const unsigned char* keep) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= numReal) return;
   int2* out = D.pairs + D.pairStart[i];
@@ -943,7 +960,7 @@ __global__ __launch_bounds__(TPB) void k_cand_write(Dev D, int numReal, const in
 // ============================================================================================
 // narrowphase — src/narrowphase.hpp:179-268 (pairs) and :371-449 (planes)
 // ============================================================================================
-__device__ __forceinline__ void writeContact(Dev& D, int slot, int a, int b, d3 n, d3 rA, d3 rB,
+__device__ __forceinline__ void writeContact(const Dev& D, int slot, int a, int b, d3 n, d3 rA, d3 rB,
                                              double dist, double eb, double mub,
                                              d3 bv = d3{0.0, 0.0, 0.0}) {
   D.cAB[slot] = make_int2(a, b);
@@ -954,7 +971,7 @@ __device__ __forceinline__ void writeContact(Dev& D, int slot, int a, int b, d3 
 }
 
 template <bool WRITE>
-__device__ __forceinline__ double npPair(Dev& D, const Params& P, int p) {
+__device__ __forceinline__ double npPair(const Dev& D, const Params& P, int p) {
   const int2 ab = D.pairs[p];
   const int idA = ab.x, idB = ab.y;
   const d3 posA = xyz(D.posPred[idA]), posB = xyz(D.posPred[idB]);
@@ -1008,7 +1025,8 @@ __device__ __forceinline__ double npPair(Dev& D, const Params& P, int p) {
   return ovl;
 }
 template <bool WRITE>
-__global__ __launch_bounds__(TPB) void k_np_pairs(Dev D, Params P, int numPairs) {
+__global__ __launch_bounds__(TPB) void k_np_pairs(const Dev* __restrict__ Dp, Params P, int numPairs) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[WAVES];
   const int p = gtid();
   const double ovl = (p < numPairs) ? npPair<WRITE>(D, P, p) : 0.0;
@@ -1017,8 +1035,9 @@ __global__ __launch_bounds__(TPB) void k_np_pairs(Dev D, Params P, int numPairs)
 // Write pass of point shapes (at most one contact per pair) over the contacts only: contact c
 // comes from pair pairOf[c] (the compaction written by the slot scan) and lands at its offset
 // noff[pairOf[c]] = c, as in the full pass.  The pair contacts are the first noff[numPairs].
-__global__ __launch_bounds__(TPB) void k_np_pairs_sparse(Dev D, Params P, int numPairs,
+__global__ __launch_bounds__(TPB) void k_np_pairs_sparse(const Dev* __restrict__ Dp, Params P, int numPairs,
                                                          const int* pairOf) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[WAVES];
   const int c = gtid();
   const double ovl = (c < D.noff[numPairs]) ? npPair<true>(D, P, pairOf[c]) : 0.0;
@@ -1031,7 +1050,8 @@ __global__ __launch_bounds__(TPB) void k_np_pairs_sparse(Dev D, Params P, int nu
 constexpr int NP_WAVES = TPB / 64;
 constexpr int NP_LDS_POINTS = 1024;  // larger shells read the shell from global memory
 template <bool WRITE>
-__global__ __launch_bounds__(TPB) void k_np_pairs_wave(Dev D, Params P, int numPairs) {
+__global__ __launch_bounds__(TPB) void k_np_pairs_wave(const Dev* __restrict__ Dp, Params P, int numPairs) {
+  const Dev& D = *Dp;
   __shared__ double4 shShell[NP_LDS_POINTS];
   __shared__ unsigned long long shMax[NP_WAVES];
   const int countA = P.shape.numShell;
@@ -1105,7 +1125,7 @@ __global__ __launch_bounds__(TPB) void k_np_pairs_wave(Dev D, Params P, int numP
 }
 
 template <bool WRITE>
-__device__ __forceinline__ double npPlanes(Dev& D, const Params& P, int numPairs, int i) {
+__device__ __forceinline__ double npPlanes(const Dev& D, const Params& P, int numPairs, int i) {
   const d3 posA = xyz(D.posPred[i]);
   const double s = D.scale[i] * P.gs;
   double baseR = P.shape.p0;
@@ -1144,7 +1164,8 @@ __device__ __forceinline__ double npPlanes(Dev& D, const Params& P, int numPairs
   return ovl;
 }
 template <bool WRITE>
-__global__ void k_np_planes(Dev D, Params P, int numPairs) {
+__global__ void k_np_planes(const Dev* __restrict__ Dp, Params P, int numPairs) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[WAVES];
   const int i = gtid();
   const double ovl = (i < P.numReal) ? npPlanes<WRITE>(D, P, numPairs, i) : 0.0;
@@ -1155,7 +1176,8 @@ __global__ void k_np_planes(Dev D, Params P, int numPairs) {
 // contact -> manifold reduction — src/contact_preprocessing.hpp:320-471
 // ============================================================================================
 // manifolds per real body: its pairs with >= 1 contact, then one boundary manifold
-__global__ void k_mcount(Dev D, Params P, int numPairs) {
+__global__ void k_mcount(const Dev* __restrict__ Dp, Params P, int numPairs) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   int m = 0;
@@ -1163,7 +1185,8 @@ __global__ void k_mcount(Dev D, Params P, int numPairs) {
   m += (D.ncnt[numPairs + i] + D.ncnt[numPairs + P.numReal + i]) > 0;
   D.mCnt[i] = m;
 }
-__global__ void k_mbuild(Dev D, Params P, int numPairs) {
+__global__ void k_mbuild(const Dev* __restrict__ Dp, Params P, int numPairs) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   int idx = D.mStart[i];
@@ -1185,7 +1208,8 @@ __global__ void k_mbuild(Dev D, Params P, int numPairs) {
   }
 }
 // transformContact + ordered segmented sum, src/contact_preprocessing.hpp:339-466
-__global__ __launch_bounds__(TPB) void k_mreduce(Dev D, int numManifolds) {
+__global__ __launch_bounds__(TPB) void k_mreduce(const Dev* __restrict__ Dp, int numManifolds) {
+  const Dev& D = *Dp;
   const int m = gtid();
   if (m >= numManifolds) return;
   const int2 ab = D.mAB[m];
@@ -1246,7 +1270,8 @@ __global__ __launch_bounds__(TPB) void k_mreduce(Dev D, int numManifolds) {
 // Jones-Plassmann colouring — src/solver_velocity.hpp:277-374, src/solver_position.hpp:155-241
 // ============================================================================================
 // uncoloured manifolds (-1) are appended to the first work list; inactive / duplicate ones get -2
-__global__ void k_jp_init_manifolds(Dev D, int numManifolds, int* work, int* count) {
+__global__ void k_jp_init_manifolds(const Dev* __restrict__ Dp, int numManifolds, int* work, int* count) {
+  const Dev& D = *Dp;
   __shared__ int shCnt, shBase;
   const int m = gtid();
   int c = -2;
@@ -1553,7 +1578,8 @@ __device__ __forceinline__ unsigned long long realKey(int realA, int realB) {
 // and the support relation of updateGroundedLevelsKokkos, src/contact_preprocessing.hpp:78-253
 // keys, support relation (0 none, 1 B supports A, 2 A supports B, 3 boundary) and the compacted
 // list of support manifolds (the only ones the grounded-level sweeps visit)
-__global__ void k_mkeys(Dev D, int numManifolds, d3 gHat, int* supList, int* supCount) {
+__global__ void k_mkeys(const Dev* __restrict__ Dp, int numManifolds, d3 gHat, int* supList, int* supCount) {
+  const Dev& D = *Dp;
   __shared__ int shCnt, shBase;
   const int m = gtid();
   unsigned char code = 0;
@@ -1580,8 +1606,9 @@ __global__ void k_mkeys(Dev D, int numManifolds, d3 gHat, int* supList, int* sup
 
 // velocity rows: gather the manifold (colour-sorted), warm start, persistence, vn0 / vt0
 // (gatherWarmLambdaKokkos :113, markPersistentManifoldsKokkos :78, computeVn0Kokkos :390)
-__global__ __launch_bounds__(TPB) void k_vrows(Dev D, Params P, int numRows, int ledgerBuckets,
+__global__ __launch_bounds__(TPB) void k_vrows(const Dev* __restrict__ Dp, Params P, int numRows, int ledgerBuckets,
                                                int usePGS) {
+  const Dev& D = *Dp;
   const int k = gtid();
   if (k >= numRows) return;
   const int m = D.vPerm[k];
@@ -1694,7 +1721,8 @@ __global__ void k_copy_i32(int* dst, const int* src, int n) {
 // brought up to date for the listed bodies by a max with the levels read, then updated; bodies
 // outside the list keep their decayed level in the final buffer.  Same result as a full copy per
 // sweep (src/contact_preprocessing.hpp:213-253, double-buffered).
-__global__ void k_ground_seed(Dev D, const int* list, const int* count, const int* src, int* dst) {
+__global__ void k_ground_seed(const Dev* __restrict__ Dp, const int* list, const int* count, const int* src, int* dst) {
+  const Dev& D = *Dp;
   const int n = *count;
   for (int q = gtid(); q < n; q += gridDim.x * blockDim.x) {
     const int2 r = D.mReal[list[q]];
@@ -1702,8 +1730,9 @@ __global__ void k_ground_seed(Dev D, const int* list, const int* count, const in
     if (r.y >= 0) dst[r.y] = src[r.y];
   }
 }
-__global__ void k_ground_sweep_list(Dev D, const int* list, const int* count, const int* gOld,
+__global__ void k_ground_sweep_list(const Dev* __restrict__ Dp, const int* list, const int* count, const int* gOld,
                                     int* gNew) {
+  const Dev& D = *Dp;
   const int n = *count;
   for (int q = gtid(); q < n; q += gridDim.x * blockDim.x) {
     const int m = list[q];
@@ -1722,7 +1751,8 @@ __global__ void k_ground_sweep_list(Dev D, const int* list, const int* count, co
     }
   }
 }
-__global__ void k_ground_sweep(Dev D, int numManifolds, const int* gOld, int* gNew) {
+__global__ void k_ground_sweep(const Dev* __restrict__ Dp, int numManifolds, const int* gOld, int* gNew) {
+  const Dev& D = *Dp;
   const int m = gtid();
   if (m >= numManifolds) return;
   const unsigned char code = D.mCode[m];
@@ -1749,7 +1779,7 @@ __device__ __forceinline__ void addVW(Body64& b, d3 dv, d3 dw) {
 }
 
 // warmStartApplyKokkos, src/solver_velocity.hpp:498-605 (one row)
-__device__ void warmApplyRow(Dev& D, int k) {
+__device__ void warmApplyRow(const Dev& D, int k) {
   const double p = D.vLam[k];
   const double4 lt4 = D.vLamT[k];
   d3 lt = xyz(lt4);
@@ -1824,7 +1854,7 @@ __device__ void warmApplyRow(Dev& D, int k) {
 }
 
 // PGSManifoldSweep::solveOne, src/solver_velocity.hpp:641-858 (one row)
-__device__ double pgsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
+__device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
   double amax = 0.0;
   const int2 r = D.vAB[k];
   const int realA = r.x, realB = r.y;
@@ -1992,7 +2022,7 @@ __device__ double pgsRow(Dev& D, const Params& P, int k, double restVelThreshold
 }
 
 // solveVelocityColoredGSKokkos one-shot restitution (g = 0), src/solver_velocity.hpp:989-1133
-__device__ double gsRow(Dev& D, const Params& P, int k, double restVelThreshold) {
+__device__ double gsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
   const int2 r = D.vAB[k];
   const int realA = r.x, realB = r.y;
   const BodyC ca = D.bc[realA];
@@ -2067,8 +2097,9 @@ __device__ double gsRow(Dev& D, const Params& P, int k, double restVelThreshold)
 }
 
 // computeSideFlagsKokkos, src/solver_velocity.hpp:456-495 (rows are active, non-duplicate)
-__global__ void k_side_flags(Dev D, int numRows, d3 gHat, double riseThr, double approachThr,
+__global__ void k_side_flags(const Dev* __restrict__ Dp, int numRows, d3 gHat, double riseThr, double approachThr,
                              const int* run) {
+  const Dev& D = *Dp;
   const int k = gtid();
   if (k >= numRows || !*run) return;
   unsigned char fl = D.vFlag[k] & (unsigned char)~(F_SIDE1 | F_SIDE2);
@@ -2105,8 +2136,9 @@ __global__ void k_stab_gate(const unsigned long long* bits, int* ctl, int mode, 
 }
 
 template <int MODE>
-__global__ __launch_bounds__(TPB) void k_vsweep(Dev D, Params P, int rowBegin, int rowEnd,
+__global__ __launch_bounds__(TPB) void k_vsweep(const Dev* __restrict__ Dp, Params P, int rowBegin, int rowEnd,
                                                 double thr, const int* stop) {
+  const Dev& D = *Dp;
   if (*stop) return;
   __shared__ unsigned long long sh[WAVES];
   const int k = rowBegin + gtid();
@@ -2119,9 +2151,10 @@ __global__ __launch_bounds__(TPB) void k_vsweep(Dev D, Params P, int rowBegin, i
   if (MODE != 0) blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
 }
 // colour 62 may hold manifolds sharing a body (mask saturation): one thread, manifold order
-__global__ __launch_bounds__(64) void k_vsweep_serial(Dev D, Params P, int numManifolds,
+__global__ __launch_bounds__(64) void k_vsweep_serial(const Dev* __restrict__ Dp, Params P, int numManifolds,
                                                       int colour, int mode, double thr,
                                                       const int* stop) {
+  const Dev& D = *Dp;
   if (gtid() != 0 || *stop) return;
   for (int m = 0; m < numManifolds; ++m) {
     if (D.mColor[m] != colour) continue;
@@ -2158,8 +2191,9 @@ __device__ __forceinline__ unsigned long long blockMaxAll(unsigned long long v,
 // loop; `stop` / `count` / `bits` are the loop's stop flag, sweep counter and residual.
 template <int MODE>
 __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
-    Dev D, Params P, ClassStarts cs, int numColors, int numManifolds, int iters, double thr,
+    const Dev* __restrict__ Dp, Params P, ClassStarts cs, int numColors, int numManifolds, int iters, double thr,
     double stopThr, int* stop, int* count, const int* run) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
   if ((run && !*run) || *stop) return;
   for (int it = 0; it < iters; ++it) {
@@ -2205,8 +2239,9 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
 constexpr int LEVEL_INF = 1 << 28;
 // one relaxation sweep of the height-from-floor levels (src/contact_preprocessing.hpp:255-297);
 // repeated until nothing changes, it reaches the exact support distance in any visiting order
-__global__ __launch_bounds__(TPB) void k_height_sweep(Dev D, int numManifolds, d3 gHat,
+__global__ __launch_bounds__(TPB) void k_height_sweep(const Dev* __restrict__ Dp, int numManifolds, d3 gHat,
                                                       int* h, int* changed) {
+  const Dev& D = *Dp;
   __shared__ int sh[WAVES];
   const int idx = gtid();
   int ch = 0;
@@ -2231,7 +2266,8 @@ __global__ __launch_bounds__(TPB) void k_height_sweep(Dev D, int numManifolds, d
   blockAtomicAddI(changed, ch, sh);
 }
 // key = min level of the two bodies (clamped 1023) * 64 + colour, per velocity row
-__global__ void k_level_keys(Dev D, int numRows, const int* h, int* key, int* cnt) {
+__global__ void k_level_keys(const Dev* __restrict__ Dp, int numRows, const int* h, int* key, int* cnt) {
+  const Dev& D = *Dp;
   const int k = gtid();
   if (k >= numRows) return;
   const int2 r = D.vAB[k];
@@ -2255,8 +2291,9 @@ __global__ void k_level_place(int numRows, const int* key, const int* start, int
   perm[start[kk] + atomicAdd(&cursor[kk], 1)] = k;
 }
 // bucket sweep over perm[b, e) (one colour), PGS
-__global__ __launch_bounds__(TPB) void k_vsweep_perm(Dev D, Params P, const int* perm, int b,
+__global__ __launch_bounds__(TPB) void k_vsweep_perm(const Dev* __restrict__ Dp, Params P, const int* perm, int b,
                                                      int e, double thr, int serial) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[WAVES];
   double a = 0.0;
   if (serial) {
@@ -2274,8 +2311,9 @@ __global__ __launch_bounds__(TPB) void k_vsweep_perm(Dev D, Params P, const int*
 // the whole ordered loop in one workgroup: buckets ascending then descending per iteration;
 // bucket = (begin, end, colour) into perm
 __global__ __launch_bounds__(VBLOCK_THREADS) void k_ordered_block(
-    Dev D, Params P, const int* perm, const int4* buckets, int numBuckets, int iters,
+    const Dev* __restrict__ Dp, Params P, const int* perm, const int4* buckets, int numBuckets, int iters,
     double thr, int* count) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
   for (int it = 0; it < iters; ++it) {
     unsigned long long m = 0;
@@ -2307,7 +2345,8 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_ordered_block(
 
 // commitPairKeysLambdaKokkos, src/contact_preprocessing.hpp:160-192: bucket by the low real index
 // (key >> 32), then sort each bucket by (key, manifold index) — a stable key sort
-__global__ void k_ledger_count(Dev D, int numManifolds, int numReal) {
+__global__ void k_ledger_count(const Dev* __restrict__ Dp, int numManifolds, int numReal) {
+  const Dev& D = *Dp;
   __shared__ int sh[WAVES];
   const int m = gtid();
   int dead = 0;
@@ -2318,7 +2357,8 @@ __global__ void k_ledger_count(Dev D, int numManifolds, int numReal) {
   }
   blockAtomicAddI(&D.bucketCnt[numReal], dead, sh);
 }
-__global__ void k_ledger_scatter(Dev D, int numManifolds, int numReal) {
+__global__ void k_ledger_scatter(const Dev* __restrict__ Dp, int numManifolds, int numReal) {
+  const Dev& D = *Dp;
   __shared__ int ndead, baseDead;
   if (threadIdx.x == 0) ndead = 0;
   __syncthreads();
@@ -2345,7 +2385,8 @@ __global__ void k_ledger_scatter(Dev D, int numManifolds, int numReal) {
     D.lIdx[baseDead + local] = m;
   }
 }
-__global__ void k_ledger_sort(Dev D, int numReal) {
+__global__ void k_ledger_sort(const Dev* __restrict__ Dp, int numReal) {
+  const Dev& D = *Dp;
   const int b = gtid();
   if (b >= numReal) return;
   const int s = D.bucketStart[b], e = D.bucketStart[b + 1];
@@ -2362,7 +2403,8 @@ __global__ void k_ledger_sort(Dev D, int numReal) {
     D.lIdx[c + 1] = ix;
   }
 }
-__global__ void k_ledger_values(Dev D, int numManifolds) {
+__global__ void k_ledger_values(const Dev* __restrict__ Dp, int numManifolds) {
+  const Dev& D = *Dp;
   const int pos = gtid();
   if (pos >= numManifolds) return;
   const int m = D.lIdx[pos];
@@ -2391,7 +2433,8 @@ __global__ void k_jp_init_contacts(int* colour, int n, int* work) {
 // sweeps never change, so they are evaluated once here (same expressions as per sweep in
 // src/solver_position.hpp:276-312).  Row: pN = (n, wTotal), pRA = (rA, invMassA),
 // pRB = (rB or plane point, invMassB).
-__global__ __launch_bounds__(TPB) void k_prows(Dev D, int numRows) {
+__global__ __launch_bounds__(TPB) void k_prows(const Dev* __restrict__ Dp, int numRows) {
+  const Dev& D = *Dp;
   const int k = gtid();
   if (k >= numRows) return;
   const int c = D.pPerm[k];
@@ -2426,7 +2469,7 @@ __global__ __launch_bounds__(TPB) void k_prows(Dev D, int numRows) {
   D.pLam[k] = 0.0;
 }
 // solvePositionColoredGSKokkos, one contact (src/solver_position.hpp:259-326)
-__device__ double posRow(Dev& D, int k) {
+__device__ double posRow(const Dev& D, int k) {
   const int2 ab = D.pAB[k];
   const int idA = ab.x, idB = ab.y;
   const double4 n4 = D.pN[k], ra4 = D.pRA[k], rb4 = D.pRB[k];
@@ -2457,14 +2500,16 @@ __device__ double posRow(Dev& D, int k) {
   }
   return -C;
 }
-__global__ __launch_bounds__(TPB) void k_psweep(Dev D, int rowBegin, int rowEnd, const int* stop) {
+__global__ __launch_bounds__(TPB) void k_psweep(const Dev* __restrict__ Dp, int rowBegin, int rowEnd, const int* stop) {
+  const Dev& D = *Dp;
   if (*stop) return;
   __shared__ unsigned long long sh[WAVES];
   const int k = rowBegin + gtid();
   const double o = (k < rowEnd) ? posRow(D, k) : 0.0;
   blockAtomicMaxU64(D.maxOverlapBits, dbits(o), sh);
 }
-__global__ void k_psweep_serial(Dev D, int numContacts, int colour, const int* stop) {
+__global__ void k_psweep_serial(const Dev* __restrict__ Dp, int numContacts, int colour, const int* stop) {
+  const Dev& D = *Dp;
   if (gtid() != 0 || *stop) return;
   for (int c = 0; c < numContacts; ++c)
     if (D.cColor[c] == colour) atomicMaxD(D.maxOverlapBits, posRow(D, D.pRowOf[c]));
@@ -2472,8 +2517,9 @@ __global__ void k_psweep_serial(Dev D, int numContacts, int colour, const int* s
 
 // position sweeps (XPBD), strict stop: residual < tol
 __global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psolve_block(
-    Dev D, ClassStarts cs, int numColors, int numContacts, int iters, double tol, int* stop,
+    const Dev* __restrict__ Dp, ClassStarts cs, int numColors, int numContacts, int iters, double tol, int* stop,
     int* count) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[BLOCK_SOLVER_THREADS / 64];
   if (*stop) return;
   for (int it = 0; it < iters; ++it) {
@@ -2513,8 +2559,9 @@ __global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psolve_block(
 // and what the launches before them accumulated in *maxOverlapBits; the accumulator is cleared
 // for the next sweep unless the loop ends here (stop, or the cap of sweeps reached).
 __global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psweep_tail(
-    Dev D, ClassStarts cs, int cFrom, int numColors, int numContacts, double tol, int* stop,
+    const Dev* __restrict__ Dp, ClassStarts cs, int cFrom, int numColors, int numContacts, double tol, int* stop,
     int* count, int cap) {
+  const Dev& D = *Dp;
   __shared__ unsigned long long sh[BLOCK_SOLVER_THREADS / 64];
   if (*stop) return;
   unsigned long long m = 0;
@@ -2550,7 +2597,8 @@ __global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psweep_tail(
 
 // commitPosImpulseKokkos, src/contact_preprocessing.hpp:501-538 (sum in contact order; among
 // manifolds sharing a key the highest manifold index writes, as a serial loop would)
-__global__ void k_commit_pos_impulse(Dev D, int numManifolds, int numReal, double dt) {
+__global__ void k_commit_pos_impulse(const Dev* __restrict__ Dp, int numManifolds, int numReal, double dt) {
+  const Dev& D = *Dp;
   const int m = gtid();
   if (m >= numManifolds) return;
   const int2 rg = D.mRange[m], rg2 = D.mRange2[m];
@@ -2576,8 +2624,9 @@ __global__ void k_commit_pos_impulse(Dev D, int numManifolds, int numReal, doubl
 // averaged apply is parallel (one body per thread).
 // ============================================================================================
 // solveVelocityKokkos restricted to uncoloured manifolds, src/solver_velocity.hpp:38-224
-__global__ void k_vel_jacobi_accum(Dev D, Params P, int numManifolds, double thr,
+__global__ void k_vel_jacobi_accum(const Dev* __restrict__ Dp, Params P, int numManifolds, double thr,
                                    int onlyUncoloured) {
+  const Dev& D = *Dp;
   if (gtid() != 0) return;
   for (int m = 0; m < numManifolds; ++m) {
     if ((onlyUncoloured && D.mColor[m] != -1) || D.mNp[m] <= 0) continue;
@@ -2660,7 +2709,8 @@ __global__ void k_vel_jacobi_accum(Dev D, Params P, int numManifolds, double thr
   }
 }
 // applyVelocityDeltasAveragedKokkos, src/solver_velocity.hpp:233-255
-__global__ void k_apply_vel_avg(Dev D, int n) {
+__global__ void k_apply_vel_avg(const Dev* __restrict__ Dp, int n) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= n) return;
   const int count = D.dCnt[i];
@@ -2680,7 +2730,8 @@ __global__ void k_apply_vel_avg(Dev D, int n) {
   D.dCnt[i] = 0;
 }
 // solvePositionKokkos restricted to uncoloured contacts, src/solver_position.hpp:34-132
-__global__ void k_pos_jacobi_accum(Dev D, int numContacts, int onlyUncoloured) {
+__global__ void k_pos_jacobi_accum(const Dev* __restrict__ Dp, int numContacts, int onlyUncoloured) {
+  const Dev& D = *Dp;
   if (gtid() != 0) return;
   for (int c = 0; c < numContacts; ++c) {
     if (onlyUncoloured && D.cColor[c] != -1) continue;
@@ -2739,7 +2790,8 @@ __global__ void k_pos_jacobi_accum(Dev D, int numContacts, int onlyUncoloured) {
   }
 }
 // applyUpdatesKokkos, src/integration.hpp:168-184
-__global__ void k_apply_pos_avg(Dev D, int n) {
+__global__ void k_apply_pos_avg(const Dev* __restrict__ Dp, int n) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= n) return;
   const int count = D.dCnt[i];
@@ -2762,7 +2814,8 @@ __global__ void k_apply_pos_avg(Dev D, int n) {
 // src/sim.hpp:173-176,224-226,423-429 and src/solver_friction.hpp:32-220
 // ============================================================================================
 // plane contacts: initial normal impulse from the approach speed (src/sim.hpp:173-176)
-__global__ void k_fr_plane_load(Dev D, int nc) {
+__global__ void k_fr_plane_load(const Dev* __restrict__ Dp, int nc) {
+  const Dev& D = *Dp;
   const int c = gtid();
   if (c >= nc) return;
   const int2 ab = D.cAB[c];
@@ -2785,7 +2838,8 @@ __global__ void k_fr_plane_load(Dev D, int nc) {
   D.cLamN[c] = lam;
 }
 // src/solver_friction.hpp:32-85, once per velocity iteration (each contact its own field)
-__global__ void k_fr_accum_normal(Dev D, int nc, double growthRate) {
+__global__ void k_fr_accum_normal(const Dev* __restrict__ Dp, int nc, double growthRate) {
+  const Dev& D = *Dp;
   const int c = gtid();
   if (c >= nc) return;
   const int2 ab = D.cAB[c];
@@ -2817,7 +2871,8 @@ __global__ void k_fr_accum_normal(Dev D, int nc, double growthRate) {
   if (w_n > 1e-6) D.cLamN[c] += approach / w_n;
 }
 // src/solver_friction.hpp:123-140 (integer counts: atomics are exact)
-__global__ void k_fr_count(Dev D, int nc) {
+__global__ void k_fr_count(const Dev* __restrict__ Dp, int nc) {
+  const Dev& D = *Dp;
   const int c = gtid();
   if (c >= nc) return;
   if (!(D.cLamN[c] > 0.0)) return;
@@ -2827,7 +2882,8 @@ __global__ void k_fr_count(Dev D, int nc) {
 }
 // src/solver_friction.hpp:142-220: impulses accumulated in contact order (one lane), then
 // applied unaveraged by k_fr_apply (src/integration.hpp:97-109)
-__global__ void k_fr_solve_serial(Dev D, int nc, double frictionDynamic) {
+__global__ void k_fr_solve_serial(const Dev* __restrict__ Dp, int nc, double frictionDynamic) {
+  const Dev& D = *Dp;
   if (gtid() != 0) return;
   for (int c = 0; c < nc; ++c) {
     const double lambda_n = D.cLamN[c];
@@ -2893,7 +2949,8 @@ __global__ void k_fr_solve_serial(Dev D, int nc, double frictionDynamic) {
     }
   }
 }
-__global__ void k_fr_apply(Dev D, int n) {
+__global__ void k_fr_apply(const Dev* __restrict__ Dp, int n) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= n) return;
   Body64 b = D.vw[i];
@@ -2918,8 +2975,9 @@ struct SdfGridP {
   int rx, ry, rz, px, py, pz;
   d3 origin, vox;
 };
-__global__ void k_sdf_splat(Dev D, Params P, SdfGridP G, unsigned long long* key,
+__global__ void k_sdf_splat(const Dev* __restrict__ Dp, Params P, SdfGridP G, unsigned long long* key,
                             unsigned char* fixedCell) {
+  const Dev& D = *Dp;
   const int i = gtid();
   if (i >= P.numReal) return;
   const d3 pw = xyz(D.pos[i]);

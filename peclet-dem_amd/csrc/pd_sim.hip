@@ -279,6 +279,13 @@ struct pd_sim {
     HIPCHECK(hipEventCreateWithFlags(&evSide, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&evContacts, hipEventDisableTiming));
     HIPCHECK(hipHostMalloc(&hostScratch, HOST_SCRATCH_BYTES, hipHostMallocDefault));
+    HIPCHECK(hipMalloc((void**)&devD, sizeof(Dev)));
+    dD = devD;
+    HIPCHECK(hipHostMalloc((void**)&devRing, DEV_RING * sizeof(Dev), hipHostMallocDefault));
+    for (int k = 0; k < DEV_RING; ++k) {
+      HIPCHECK(hipEventCreateWithFlags(&devRingEv[k], hipEventDisableTiming));
+      HIPCHECK(hipEventRecord(devRingEv[k], st));
+    }
     HIPCHECK(hipHostMalloc(&jobV.pinned, 1024, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&jobP.pinned, 1024, hipHostMallocDefault));
     jobV.stream = st;
@@ -291,6 +298,7 @@ struct pd_sim {
     pairTableD.ensure(128, st);
     shell.ensure(1, st);
     initializeShape(SHAPE_SPHERE, 1.0, 0.0, 0.0);
+    refreshDev();
   }
   ~pd_sim() {
     (void)hipStreamSynchronize(st);
@@ -305,6 +313,9 @@ struct pd_sim {
       for (int k = 0; k <= stageMarked; ++k) (void)hipEventDestroy(stageCur[k]);
     for (hipEvent_t e : evFree) (void)hipEventDestroy(e);
     if (hostScratch) (void)hipHostFree(hostScratch);
+    for (int k = 0; k < DEV_RING; ++k) (void)hipEventDestroy(devRingEv[k]);
+    if (devRing) (void)hipHostFree(devRing);
+    if (devD) (void)hipFree(devD);
     if (jobV.pinned) (void)hipHostFree(jobV.pinned);
     if (jobP.pinned) (void)hipHostFree(jobP.pinned);
     (void)hipEventDestroy(evSide);
@@ -481,6 +492,28 @@ struct pd_sim {
     d.maxOverlapBits = bits.p; d.maxApproachBits = bits.p + 1; d.thermo = thermo.p;
     d.dVel = dVel.p; d.dAng = dAng.p; d.dPos = dPos.p; d.dCnt = dCnt.p;
     d.cLamN = cLamN.p; d.frCnt = frCnt.p;
+    uploadDev();
+  }
+  // The kernels read the pointer set from device memory (a few bytes of launch arguments instead
+  // of ~800): a changed set is copied, in stream order, from a ring of pinned host slots.
+  static constexpr int DEV_RING = 16;
+  Dev* devD = nullptr;       // device copy read by the kernels
+  const Dev* dD = nullptr;   // = devD, the kernels' argument
+  Dev* devRing = nullptr;    // pinned host slots
+  hipEvent_t devRingEv[DEV_RING] = {};
+  int devRingNext = 0;
+  Dev devLast{};
+  bool devUploaded = false;
+  void uploadDev() {
+    if (devUploaded && std::memcmp(&devLast, &D, sizeof(Dev)) == 0) return;
+    const int k = devRingNext;
+    devRingNext = (devRingNext + 1) % DEV_RING;
+    HIPCHECK(hipEventSynchronize(devRingEv[k]));  // the slot's previous copy has run
+    devRing[k] = D;
+    HIPCHECK(hipMemcpyAsync(devD, &devRing[k], sizeof(Dev), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipEventRecord(devRingEv[k], st));
+    devLast = D;
+    devUploaded = true;
   }
   // zeroed Jacobi accumulators for the colour-saturation fallback
   void prepareFallback() {
@@ -750,17 +783,17 @@ struct pd_sim {
     // cut short by an error leaves it dirty
     if (cellCntDirty) zero(cellCnt.p, cellCnt.cap);
     cellCntDirty = true;
-    k_bp_bounds<<<nbb, TPB, 0, st>>>(D, P, nSlotsDev, redPart.p);
+    k_bp_bounds<<<nbb, TPB, 0, st>>>(dD, P, nSlotsDev, redPart.p);
     k_bp_bounds_final<<<1, TPB, 0, st>>>(redPart.p, nbb, red.p, nSlotsDev, gridD.p, ncellD.p);
-    k_cell_count<<<nblk(gridSlots), TPB, 0, st>>>(D, P, nSlotsDev, gridD.p);
+    k_cell_count<<<nblk(gridSlots), TPB, 0, st>>>(dD, P, nSlotsDev, gridD.p);
     scanDev(cellCnt.p, cellStart.p, ncellD.p, (int)cellCap);
-    k_cell_fill<<<nblk(gridSlots), TPB, 0, st>>>(D, P, nSlotsDev);
+    k_cell_fill<<<nblk(gridSlots), TPB, 0, st>>>(dD, P, nSlotsDev);
     HIPCHECK(hipGetLastError());
     cellCntDirty = false;
     pairStage.ensure((size_t)numReal * BP_CAP + 1, st);
     zero(pairCnt.p, numReal);
     kernBegin();
-    k_bp_query<<<nblk(gridSlots), TPB, 0, st>>>(D, P, nSlotsDev, gridD.p, pairStage.p);
+    k_bp_query<<<nblk(gridSlots), TPB, 0, st>>>(dD, P, nSlotsDev, gridD.p, pairStage.p);
     kernEnd(K_BPQUERY, gridSlots);
     scanQuiet(pairCnt.p, pairStart.p, numReal);
   }
@@ -768,7 +801,7 @@ struct pd_sim {
   void broadphaseCompact(const Params& P) {
     pairs.ensure(np + 1, st);
     refreshDev();
-    k_bp_compact<<<nblk(numReal), TPB, 0, st>>>(D, P, gridD.p, pairStage.p);
+    k_bp_compact<<<nblk(numReal), TPB, 0, st>>>(dD, P, gridD.p, pairStage.p);
     HIPCHECK(hipGetLastError());
   }
   // the whole broadphase over the numParticles slots the host already knows; fills pairs / np
@@ -821,10 +854,10 @@ struct pd_sim {
     refreshDev();
     const bool waveNp = P.shape.numShell > 0;  // shell shapes: one wavefront per pair
     const int npBlocks = (int)std::max<long>(1, ((long)np + NP_WAVES - 1) / NP_WAVES);
-    if (waveNp) k_np_pairs_wave<false><<<npBlocks, TPB, 0, st>>>(D, P, np);
-    else k_np_pairs<false><<<nblk(np), TPB, 0, st>>>(D, P, np);
-    k_np_planes<false><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
-    if (W.n > 0) k_np_walls<false><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
+    if (waveNp) k_np_pairs_wave<false><<<npBlocks, TPB, 0, st>>>(dD, P, np);
+    else k_np_pairs<false><<<nblk(np), TPB, 0, st>>>(dD, P, np);
+    k_np_planes<false><<<nblk(numReal), TPB, 0, st>>>(dD, P, np);
+    if (W.n > 0) k_np_walls<false><<<nblk(numReal), TPB, 0, st>>>(dD, P, W, np + numReal);
     else zero(ncnt.p + np + numReal, numReal);  // no walls: every wall slot holds 0 contacts
     if (waveNp) {
       nc = scan(ncnt.p, noff.p, nslots);
@@ -848,11 +881,11 @@ struct pd_sim {
     refreshDev();
     zero(bits.p, 1);
     kernBegin();
-    if (waveNp) k_np_pairs_wave<true><<<npBlocks, TPB, 0, st>>>(D, P, np);
-    else if (nc > 0) k_np_pairs_sparse<<<nblk(nc), TPB, 0, st>>>(D, P, np, pairOfContact.p);
+    if (waveNp) k_np_pairs_wave<true><<<npBlocks, TPB, 0, st>>>(dD, P, np);
+    else if (nc > 0) k_np_pairs_sparse<<<nblk(nc), TPB, 0, st>>>(dD, P, np, pairOfContact.p);
     kernEnd(K_NPPAIRS, waveNp ? np : nc);
-    k_np_planes<true><<<nblk(numReal), TPB, 0, st>>>(D, P, np);
-    if (W.n > 0) k_np_walls<true><<<nblk(numReal), TPB, 0, st>>>(D, P, W, np + numReal);
+    k_np_planes<true><<<nblk(numReal), TPB, 0, st>>>(dD, P, np);
+    if (W.n > 0) k_np_walls<true><<<nblk(numReal), TPB, 0, st>>>(dD, P, W, np + numReal);
     HIPCHECK(hipGetLastError());
   }
 
@@ -861,7 +894,7 @@ struct pd_sim {
     mCnt.ensure(numReal + 1, st);
     mStart.ensure(numReal + 1, st);
     refreshDev();
-    k_mcount<<<nblk(numReal), TPB, 0, st>>>(D, P, np);
+    k_mcount<<<nblk(numReal), TPB, 0, st>>>(dD, P, np);
     nm = scan(mCnt.p, mStart.p, numReal);
     const size_t M = nm + 1;
     mAB.ensure(M, st); mReal.ensure(M, st); mRange.ensure(M, st); mRange2.ensure(M, st);
@@ -870,8 +903,8 @@ struct pd_sim {
     mNp.ensure(M, st); mKey.ensure(M, st); mCode.ensure(M, st); mColor.ensure(M, st);
     vRowOf.ensure(M, st);
     refreshDev();
-    k_mbuild<<<nblk(numReal), TPB, 0, st>>>(D, P, np);
-    k_mreduce<<<nblk(nm), TPB, 0, st>>>(D, nm);
+    k_mbuild<<<nblk(numReal), TPB, 0, st>>>(dD, P, np);
+    k_mreduce<<<nblk(nm), TPB, 0, st>>>(dD, nm);
     HIPCHECK(hipGetLastError());
   }
 
@@ -1070,7 +1103,7 @@ struct pd_sim {
     for (int s = 0; s < 1024; s += 8) {
       zero(lvlChanged.p, 8);
       for (int j = 0; j < 8; ++j)
-        k_height_sweep<<<nblk(nm), TPB, 0, st>>>(D, nm, gHat, hLevel.p, lvlChanged.p + j);
+        k_height_sweep<<<nblk(nm), TPB, 0, st>>>(dD, nm, gHat, hLevel.p, lvlChanged.p + j);
       int ch[8];
       readN(ch, lvlChanged.p, 8);
       bool settled = false;
@@ -1083,7 +1116,7 @@ struct pd_sim {
     lvlCnt.ensure(NK + 1, st);
     lvlStart.ensure(NK + 1, st);
     zero(lvlCnt.p, NK + 1);
-    if (rows > 0) k_level_keys<<<nblk(rows), TPB, 0, st>>>(D, rows, hLevel.p, lvlKey.p, lvlCnt.p);
+    if (rows > 0) k_level_keys<<<nblk(rows), TPB, 0, st>>>(dD, rows, hLevel.p, lvlKey.p, lvlCnt.p);
     scanQuiet(lvlCnt.p, lvlStart.p, NK);
     zero(lvlCnt.p, NK + 1);
     if (rows > 0)
@@ -1099,7 +1132,7 @@ struct pd_sim {
       lvlBuckets.ensure(nb + 1, st);
       if (nb) writeN(lvlBuckets.p, bk.data(), nb);
       zero(ctl.p + CTL_SWEEPS_S, 1);
-      k_ordered_block<<<1, VBLOCK_THREADS, 0, st>>>(D, P, lvlPerm.p, lvlBuckets.p, nb,
+      k_ordered_block<<<1, VBLOCK_THREADS, 0, st>>>(dD, P, lvlPerm.p, lvlBuckets.p, nb,
                                                           2 * velIters, vRestS,
                                                           ctl.p + CTL_SWEEPS_S);
       stats.sweeps_stab = read1(ctl.p + CTL_SWEEPS_S);
@@ -1111,7 +1144,7 @@ struct pd_sim {
       for (int pass = 0; pass < 2; ++pass)
         for (int q = 0; q < nb; ++q) {
           const int4 b = bk[pass == 0 ? q : nb - 1 - q];
-          k_vsweep_perm<<<nblk(b.y - b.x), TPB, 0, st>>>(D, P, lvlPerm.p, b.x, b.y, vRestS,
+          k_vsweep_perm<<<nblk(b.y - b.x), TPB, 0, st>>>(dD, P, lvlPerm.p, b.x, b.y, vRestS,
                                                           b.z == 62 ? 1 : 0);
         }
       stats.sweeps_stab++;
@@ -1134,16 +1167,16 @@ struct pd_sim {
       const int b = vs[c], e = vs[c + 1];
       if (e <= b) continue;
       if (c == 62) {
-        k_vsweep_serial<<<1, 1, 0, st>>>(D, P, nm, c, mode, thr, stop);
+        k_vsweep_serial<<<1, 1, 0, st>>>(dD, P, nm, c, mode, thr, stop);
       } else {
         if (mode == 0) {
-          k_vsweep<0><<<nblk(e - b), TPB, 0, st>>>(D, P, b, e, thr, stop);
+          k_vsweep<0><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
         } else if (mode == 1) {
           kernBegin();
-          k_vsweep<1><<<nblk(e - b), TPB, 0, st>>>(D, P, b, e, thr, stop);
+          k_vsweep<1><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
           kernEnd(K_VSWEEP, e - b);
         } else {
-          k_vsweep<2><<<nblk(e - b), TPB, 0, st>>>(D, P, b, e, thr, stop);
+          k_vsweep<2><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
         }
       }
     }
@@ -1196,7 +1229,7 @@ struct pd_sim {
     numParticles = numReal;
     Params P = params(margin, maxRad);
     kernBegin();
-    k_predict<<<nblk(numReal), TPB, 0, st>>>(D, P);
+    k_predict<<<nblk(numReal), TPB, 0, st>>>(dD, P);
     kernEnd(K_PREDICT, numReal);
     // periodic ghosts, src/periodicity.hpp:43-94.  The ghost total is read back together with
     // the pair total: the broadphase runs on the device-side slot count, and if the ghosts did
@@ -1204,7 +1237,7 @@ struct pd_sim {
     ghostCnt.ensure(numReal + 1, st);
     ghostOff.ensure(numReal + 1, st);
     slotInfo.ensure(4, st);
-    k_ghost_count<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostCnt.p);
+    k_ghost_count<<<nblk(numReal), TPB, 0, st>>>(dD, P, ghostCnt.p);
     scanQuiet(ghostCnt.p, ghostOff.p, numReal);
     const Params PB = params(margin, 0.0);
     // candidate pairs kept across substeps (grown boxes): used while they provably hold every
@@ -1217,7 +1250,7 @@ struct pd_sim {
       const int capSlots = slotCapacity();
       k_slot_count<<<1, 1, 0, st>>>(ghostOff.p + numReal, numReal, capSlots, slotInfo.p);
       refreshDev();
-      k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostOff.p, capSlots);
+      k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(dD, P, ghostOff.p, capSlots);
       HIPCHECK(hipGetLastError());
       mark(ST_BROAD);
       const long gridSlots = std::min<long>(capSlots, (long)numReal + lastGhosts + 1024);
@@ -1229,8 +1262,8 @@ struct pd_sim {
         pairStart.ensure(numReal + 1, st);
         refreshDev();
         k_cand_check<<<std::min(nblk(lastGhosts + 1024), 1024), TPB, 0, st>>>(
-            D, P, slotInfo.p, candSlots, candRef.p, 0.25 * skin, candFlag.p);
-        k_cand_count<<<nblk(numReal), TPB, 0, st>>>(D, P, cand.p, candStart.p, candKeep.p,
+            dD, P, slotInfo.p, candSlots, candRef.p, 0.25 * skin, candFlag.p);
+        k_cand_count<<<nblk(numReal), TPB, 0, st>>>(dD, P, cand.p, candStart.p, candKeep.p,
                                                     candRef.p, candCode.p, 0.25 * skin,
                                                     candFlag.p);
         scanQuiet(pairCnt.p, pairStart.p, numReal);
@@ -1262,7 +1295,7 @@ struct pd_sim {
       ++candReuses;
       pairs.ensure(np + 1, st);
       refreshDev();
-      k_cand_write<<<nblk(numReal), TPB, 0, st>>>(D, numReal, cand.p, candStart.p, candKeep.p);
+      k_cand_write<<<nblk(numReal), TPB, 0, st>>>(dD, numReal, cand.p, candStart.p, candKeep.p);
       HIPCHECK(hipGetLastError());
     } else if (skin > 0.0) {
       // rebuild: the grown broadphase's pairs become the candidates, then the exact filter
@@ -1278,13 +1311,13 @@ struct pd_sim {
       pairCnt.ensure(numReal + 1, st);
       pairStart.ensure(numReal + 1, st);
       refreshDev();
-      k_cand_ref<<<nblk(numParticles), TPB, 0, st>>>(D, P, numParticles, candRef.p, candCode.p);
-      k_cand_count<<<nblk(numReal), TPB, 0, st>>>(D, P, cand.p, candStart.p, candKeep.p, nullptr,
+      k_cand_ref<<<nblk(numParticles), TPB, 0, st>>>(dD, P, numParticles, candRef.p, candCode.p);
+      k_cand_count<<<nblk(numReal), TPB, 0, st>>>(dD, P, cand.p, candStart.p, candKeep.p, nullptr,
                                                   nullptr, 0.0, nullptr);
       np = scan(pairCnt.p, pairStart.p, numReal);
       pairs.ensure(np + 1, st);
       refreshDev();
-      k_cand_write<<<nblk(numReal), TPB, 0, st>>>(D, numReal, cand.p, candStart.p, candKeep.p);
+      k_cand_write<<<nblk(numReal), TPB, 0, st>>>(dD, numReal, cand.p, candStart.p, candKeep.p);
       HIPCHECK(hipGetLastError());
       candValid = true;
       candSlots = numParticles;
@@ -1303,7 +1336,7 @@ struct pd_sim {
       cLamN.ensure(nc + 1, st);
       frCnt.ensure(numParticles + 1, st);
       refreshDev();
-      if (nc > 0) k_fr_plane_load<<<nblk(nc), TPB, 0, st>>>(D, nc);
+      if (nc > 0) k_fr_plane_load<<<nblk(nc), TPB, 0, st>>>(dD, nc);
     }
     mark(ST_COLOR_V);
     HIPCHECK(hipEventRecord(evContacts, st));
@@ -1314,7 +1347,7 @@ struct pd_sim {
     jobV.work[0].ensure(nm + 1, st);
     jobV.work[1].ensure(nm + 1, st);
     refreshDev();
-    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(D, nm, jobV.work[0].p, counters.p + 1);
+    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(dD, nm, jobV.work[0].p, counters.p + 1);
     std::vector<int> vs;
     int numColors = 0;
     int velLeftover = 0;
@@ -1338,9 +1371,9 @@ struct pd_sim {
     supList.ensure(nm + 1, st);
     supCnt.ensure(1, st);
     zero(supCnt.p, 1);
-    if (nm > 0) k_mkeys<<<nblk(nm), TPB, 0, st>>>(D, nm, gHat, supList.p, supCnt.p);
+    if (nm > 0) k_mkeys<<<nblk(nm), TPB, 0, st>>>(dD, nm, gHat, supList.p, supCnt.p);
     if (nRowsV > 0)
-      k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(D, P, nRowsV, prevCount > 0 ? ledgerBuckets : 0,
+      k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(dD, P, nRowsV, prevCount > 0 ? ledgerBuckets : 0,
                                             usePGS ? 1 : 0);
     if (usePGS) {
       // grounded levels, double-buffered (src/contact_preprocessing.hpp:213-253)
@@ -1349,15 +1382,15 @@ struct pd_sim {
       int* gb = ground2.p;
       if (nm > 0) {
         const int gg = std::min(nblk(nm), 1024);
-        k_ground_seed<<<gg, TPB, 0, st>>>(D, supList.p, supCnt.p, ga, gb);
+        k_ground_seed<<<gg, TPB, 0, st>>>(dD, supList.p, supCnt.p, ga, gb);
         for (int s = 0; s < 8; ++s) {
-          k_ground_sweep_list<<<gg, TPB, 0, st>>>(D, supList.p, supCnt.p, ga, gb);
+          k_ground_sweep_list<<<gg, TPB, 0, st>>>(dD, supList.p, supCnt.p, ga, gb);
           std::swap(ga, gb);
         }
       }
       // 8 swaps: the result is back in `ground`
       if (nRowsV <= blockRows)
-        k_vsolve_block<0><<<1, VBLOCK_THREADS, 0, st>>>(D, P, classStarts(vs), numColors, nm,
+        k_vsolve_block<0><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, classStarts(vs), numColors, nm,
                                                              1, 0.0, 0.0, ctl.p + CTL_NONE,
                                                              ctl.p + CTL_NONE, nullptr);
       else
@@ -1378,11 +1411,11 @@ struct pd_sim {
         kernBegin();
         if (usePGS)
           k_vsolve_block<1><<<1, VBLOCK_THREADS, 0, st>>>(
-              D, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
+              dD, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
               ctl.p + CTL_SWEEPS_V, nullptr);
         else
           k_vsolve_block<2><<<1, VBLOCK_THREADS, 0, st>>>(
-              D, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
+              dD, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
               ctl.p + CTL_SWEEPS_V, nullptr);
         vBlockRec[0] = kernEnd(K_VBLOCK, nRowsV);
       } else if (!smallV) sweepLoop(velIters, ctl.p + CTL_STOP_V, [&] {
@@ -1398,17 +1431,17 @@ struct pd_sim {
           // the gate stays on the device: the flags kernel and the loop read it
           if (stabMode == 1 && velIters > 0) {
             if (nRowsV > 0)
-              k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
+              k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, 8.0 * dt * gMagP,
                                                          8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
             kernBegin();
             k_vsolve_block<1><<<1, VBLOCK_THREADS, 0, st>>>(
-                D, P, classStarts(vs), numColors, nm, 2 * velIters, vRestS, vRestS,
+                dD, P, classStarts(vs), numColors, nm, 2 * velIters, vRestS, vRestS,
                 ctl.p + CTL_STOP_S, ctl.p + CTL_SWEEPS_S, ctl.p + CTL_STAB_RUN);
             vBlockRec[1] = kernEnd(K_VBLOCK, nRowsV);
           }
         } else if (stabMode == 1 && velIters > 0 && read1(ctl.p + CTL_STAB_RUN)) {
           if (nRowsV > 0)
-            k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
+            k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, 8.0 * dt * gMagP,
                                                        8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
           sweepLoop(2 * velIters, ctl.p + CTL_STOP_S, [&] {
             k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S);
@@ -1424,12 +1457,12 @@ struct pd_sim {
       Params PF = params(margin, maxRad);
       for (int it = 0; it < velIters; ++it) {
         if (legacyFriction && nc > 0)
-          k_fr_accum_normal<<<nblk(nc), TPB, 0, st>>>(D, nc, growthRate);
+          k_fr_accum_normal<<<nblk(nc), TPB, 0, st>>>(dD, nc, growthRate);
         if (!useGS) {
           // count-averaged Jacobi over every manifold (src/solver_velocity.hpp:38-255)
           prepareFallback();
-          k_vel_jacobi_accum<<<1, 1, 0, st>>>(D, PF, nm, vRest, 0);
-          k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+          k_vel_jacobi_accum<<<1, 1, 0, st>>>(dD, PF, nm, vRest, 0);
+          k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
           stats.sweeps_vel++;
           continue;
         }
@@ -1437,8 +1470,8 @@ struct pd_sim {
         velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest, ctl.p + CTL_NONE);
         if (velLeftover > 0) {
           prepareFallback();
-          k_vel_jacobi_accum<<<1, 1, 0, st>>>(D, PF, nm, vRest, 1);
-          k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+          k_vel_jacobi_accum<<<1, 1, 0, st>>>(dD, PF, nm, vRest, 1);
+          k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
         }
         stats.sweeps_vel++;
         maxApproach = readMaxApproach();
@@ -1452,7 +1485,7 @@ struct pd_sim {
           const int one = 1;
           writeN(ctl.p + CTL_STAB_RUN, &one, 1);
           if (stabMode == 1 && nRowsV > 0)
-            k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(D, nRowsV, gHat, 8.0 * dt * gMagP,
+            k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, 8.0 * dt * gMagP,
                                                        8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
           if (stabMode == 4) {
             orderedStabilization(gHat, vRestS);
@@ -1497,12 +1530,12 @@ struct pd_sim {
       lLamT.ensure(nm + 1, st);
       refreshDev();
       zero(bucketCnt.p, numReal + 1);
-      k_ledger_count<<<nblk(nm), TPB, 0, st>>>(D, nm, numReal);
+      k_ledger_count<<<nblk(nm), TPB, 0, st>>>(dD, nm, numReal);
       scanQuiet(bucketCnt.p, bucketStart.p, numReal + 1);
       zero(bucketCnt.p, numReal + 1);
-      k_ledger_scatter<<<nblk(nm), TPB, 0, st>>>(D, nm, numReal);
-      k_ledger_sort<<<nblk(numReal), TPB, 0, st>>>(D, numReal);
-      k_ledger_values<<<nblk(nm), TPB, 0, st>>>(D, nm);
+      k_ledger_scatter<<<nblk(nm), TPB, 0, st>>>(dD, nm, numReal);
+      k_ledger_sort<<<nblk(numReal), TPB, 0, st>>>(dD, numReal);
+      k_ledger_values<<<nblk(nm), TPB, 0, st>>>(dD, nm);
       HIPCHECK(hipGetLastError());
       prevCount = nm;
       ledgerBuckets = numReal;
@@ -1512,16 +1545,16 @@ struct pd_sim {
     if (legacyFriction && nc > 0) {
       // src/solver_friction.hpp:123-220 then applyVelocityDeltas (src/integration.hpp:97-109)
       zero(frCnt.p, numParticles);
-      k_fr_count<<<nblk(nc), TPB, 0, st>>>(D, nc);
+      k_fr_count<<<nblk(nc), TPB, 0, st>>>(dD, nc);
       prepareFallback();
-      k_fr_solve_serial<<<1, 1, 0, st>>>(D, nc, frictionDynamic);
-      k_fr_apply<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+      k_fr_solve_serial<<<1, 1, 0, st>>>(dD, nc, frictionDynamic);
+      k_fr_apply<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
       HIPCHECK(hipGetLastError());
     }
     mark(ST_REINT);
     P = params(margin, maxRad);
     kernBegin();
-    k_reintegrate<<<nblk(numParticles), TPB, 0, st>>>(D, P);
+    k_reintegrate<<<nblk(numParticles), TPB, 0, st>>>(dD, P);
     kernEnd(K_REINTEGRATE, numParticles);
     mark(ST_COLOR_P);
 
@@ -1547,7 +1580,7 @@ struct pd_sim {
     pRB.ensure(nRowsP + 1, st);
     pLam.ensure(nRowsP + 1, st);
     refreshDev();
-    if (nRowsP > 0) k_prows<<<nblk(nRowsP), TPB, 0, st>>>(D, nRowsP);
+    if (nRowsP > 0) k_prows<<<nblk(nRowsP), TPB, 0, st>>>(dD, nRowsP);
     mark(ST_PSWEEP);
     const double posTol = 1e-4 * baseRadius * globalScale;
     const int* finalCtl = nullptr;
@@ -1556,15 +1589,15 @@ struct pd_sim {
       // Jacobi XPBD over every contact, no adaptive stop (src/solver_position.hpp:34-134)
       for (int it = 0; it < posIters; ++it) {
         prepareFallback();
-        k_pos_jacobi_accum<<<1, 1, 0, st>>>(D, nc, 0);
-        k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+        k_pos_jacobi_accum<<<1, 1, 0, st>>>(dD, nc, 0);
+        k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
         stats.sweeps_pos++;
       }
       HIPCHECK(hipGetLastError());
     } else if (posLeftover == 0 && nRowsP <= blockRows) {
       if (posIters > 0) {
         kernBegin();
-        k_psolve_block<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(D, classStarts(ps), numPosColors, nc,
+        k_psolve_block<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(dD, classStarts(ps), numPosColors, nc,
                                                            posIters, posTol, ctl.p + CTL_STOP_P,
                                                            ctl.p + CTL_SWEEPS_P);
         pBlockRec = kernEnd(K_PBLOCK, nRowsP);
@@ -1588,14 +1621,14 @@ struct pd_sim {
             const int cb = ps[c], ce = ps[c + 1];
             if (ce <= cb) continue;
             if (c == 62) {
-              k_psweep_serial<<<1, 1, 0, st>>>(D, nc, c, ctl.p + CTL_STOP_P);
+              k_psweep_serial<<<1, 1, 0, st>>>(dD, nc, c, ctl.p + CTL_STOP_P);
             } else {
               kernBegin();
-              k_psweep<<<nblk(ce - cb), TPB, 0, st>>>(D, cb, ce, ctl.p + CTL_STOP_P);
+              k_psweep<<<nblk(ce - cb), TPB, 0, st>>>(dD, cb, ce, ctl.p + CTL_STOP_P);
               kernEnd(K_PSWEEP, ce - cb);
             }
           }
-          k_psweep_tail<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(D, pcs, cFrom, numPosColors, nc, posTol,
+          k_psweep_tail<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(dD, pcs, cFrom, numPosColors, nc, posTol,
                                                             ctl.p + CTL_STOP_P,
                                                             ctl.p + CTL_SWEEPS_P, posIters);
         }
@@ -1616,13 +1649,13 @@ struct pd_sim {
           const int b = ps[c], e = ps[c + 1];
           if (e <= b) continue;
           if (c == 62)
-            k_psweep_serial<<<1, 1, 0, st>>>(D, nc, c, ctl.p + CTL_NONE);
+            k_psweep_serial<<<1, 1, 0, st>>>(dD, nc, c, ctl.p + CTL_NONE);
           else
-            k_psweep<<<nblk(e - b), TPB, 0, st>>>(D, b, e, ctl.p + CTL_NONE);
+            k_psweep<<<nblk(e - b), TPB, 0, st>>>(dD, b, e, ctl.p + CTL_NONE);
         }
         prepareFallback();
-        k_pos_jacobi_accum<<<1, 1, 0, st>>>(D, nc, 1);
-        k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(D, numParticles);
+        k_pos_jacobi_accum<<<1, 1, 0, st>>>(dD, nc, 1);
+        k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
         HIPCHECK(hipGetLastError());
         stats.sweeps_pos++;
         if (bitsd(read1(bits.p)) < posTol) break;
@@ -1649,8 +1682,8 @@ struct pd_sim {
       maxApproach = bitsd(b2[1]);
     }
     mark(ST_COMMIT);
-    if (usePGS && nm > 0) k_commit_pos_impulse<<<nblk(nm), TPB, 0, st>>>(D, nm, numReal, dt);
-    k_final_commit<<<nblk(numReal), TPB, 0, st>>>(D, P);
+    if (usePGS && nm > 0) k_commit_pos_impulse<<<nblk(nm), TPB, 0, st>>>(dD, nm, numReal, dt);
+    k_final_commit<<<nblk(numReal), TPB, 0, st>>>(dD, P);
     if (thermoTau > 0.0 && dt > 0.0) applyThermostat();
     HIPCHECK(hipGetLastError());
     mark(ST_COUNT);
@@ -1745,7 +1778,7 @@ struct pd_sim {
     numParticles = n;
     {  // the broadphase reads posPred: current positions, radius in .w
       HzParams H0 = hzParams(dt);
-      k_hz_prep<<<nblk(n), TPB, 0, st>>>(D, H0, red.p + 1);
+      k_hz_prep<<<nblk(n), TPB, 0, st>>>(dD, H0, red.p + 1);
     }
     Params P = params(skin, 0.0);
     broadphase(P);
@@ -1760,7 +1793,7 @@ struct pd_sim {
     hJCnt.ensure(n + 1, st);
     refreshDev();
     refreshHz();
-    k_hz_carry<<<nblk(np), TPB, 0, st>>>(D, Z, np, hPrevCount);
+    k_hz_carry<<<nblk(np), TPB, 0, st>>>(dD, Z, np, hPrevCount);
     HIPCHECK(hipMemcpyAsync(hIStart.p, pairStart.p, (n + 1) * sizeof(int),
                             hipMemcpyDeviceToDevice, st));
     zero(hJCnt.p, n + 1);
@@ -1771,18 +1804,18 @@ struct pd_sim {
     hNumPairs = np;
     HzParams H = hzParams(dt);
     k_hz_jsort<<<nblk(n), TPB, 0, st>>>(Z, H);
-    k_hz_ref<<<nblk(n), TPB, 0, st>>>(D, Z, H);
+    k_hz_ref<<<nblk(n), TPB, 0, st>>>(dD, Z, H);
     if (!wallsHost.empty()) {
       hWStart.ensure(n + 1, st);
       hWCnt.ensure(n + 1, st);
       refreshHz();
-      k_hz_wallcand<false><<<nblk(n), TPB, 0, st>>>(D, Z, H, skin, hWCnt.p);
+      k_hz_wallcand<false><<<nblk(n), TPB, 0, st>>>(dD, Z, H, skin, hWCnt.p);
       hNumWallCand = scan(hWCnt.p, hWStart.p, n);
       hWCand.ensure(hNumWallCand + 1, st);
       hWF.ensure(hNumWallCand + 1, st);
       hWT.ensure(hNumWallCand + 1, st);
       refreshHz();
-      k_hz_wallcand<true><<<nblk(n), TPB, 0, st>>>(D, Z, H, skin, hWCnt.p);
+      k_hz_wallcand<true><<<nblk(n), TPB, 0, st>>>(dD, Z, H, skin, hWCnt.p);
     } else {
       hNumWallCand = 0;
     }
@@ -1805,7 +1838,7 @@ struct pd_sim {
     refreshHz();
     HzParams H = hzParams(dtH);
     HIPCHECK(hipMemsetAsync(red.p, 0xff, sizeof(unsigned long long), st));
-    k_hz_prep<<<nblk(n), TPB, 0, st>>>(D, H, red.p);
+    k_hz_prep<<<nblk(n), TPB, 0, st>>>(dD, H, red.p);
     const unsigned long long mk = read1(red.p);
     const double minRad = (mk == ~0ull) ? 3.4e38 : ordVal(mk);
     const double skin = skinFrac * minRad;
@@ -1818,16 +1851,16 @@ struct pd_sim {
       }
       H = hzParams(dtH);
       if (H.numPairs > 0) {
-        if (H.hasShapes) k_hz_shape_pairs<<<nblk(H.numPairs), TPB, 0, st>>>(D, Z, H);
-        else k_hz_pairs<<<nblk(H.numPairs), TPB, 0, st>>>(D, Z, H);
+        if (H.hasShapes) k_hz_shape_pairs<<<nblk(H.numPairs), TPB, 0, st>>>(dD, Z, H);
+        else k_hz_pairs<<<nblk(H.numPairs), TPB, 0, st>>>(dD, Z, H);
       }
       if (H.numWalls > 0 && hNumWallCand > 0)
-        k_hz_walls<<<nblk(hNumWallCand), TPB, 0, st>>>(D, Z, H, hNumWallCand);
+        k_hz_walls<<<nblk(hNumWallCand), TPB, 0, st>>>(dD, Z, H, hNumWallCand);
       if (H.numWalls == 0 || hNumWallCand == 0) H.numWalls = 0;
-      k_hz_integrate<<<nblk(n), TPB, 0, st>>>(D, Z, H);
+      k_hz_integrate<<<nblk(n), TPB, 0, st>>>(dD, Z, H);
       if ((k & 15) == 15 || k == nsteps - 1) {
         zero(bits.p + 2, 1);
-        k_hz_disp<<<std::min(nblk(n), RED_BLOCKS), TPB, 0, st>>>(D, Z, H, bits.p + 2);
+        k_hz_disp<<<std::min(nblk(n), RED_BLOCKS), TPB, 0, st>>>(dD, Z, H, bits.p + 2);
         if (bitsd(read1(bits.p + 2)) > rebuildAt) rebuild = true;
       }
     }
@@ -1840,7 +1873,7 @@ struct pd_sim {
     thermo.ensure(2 * nb + 2, st);
     refreshDev();
     Params P = params(0.0, 0.0);
-    k_thermo_partial<<<nb, TPB, 0, st>>>(D, P, thermo.p);
+    k_thermo_partial<<<nb, TPB, 0, st>>>(dD, P, thermo.p);
     std::vector<double> part(2 * nb);
     readN(part.data(), thermo.p, part.size());
     double ke[2] = {0.0, 0.0};
@@ -1859,7 +1892,7 @@ struct pd_sim {
       if (f < 0.0) f = 0.0;
       lam[k] = std::sqrt(f);
     }
-    k_thermo_scale<<<nb, TPB, 0, st>>>(D, P, lam[0], lam[1]);
+    k_thermo_scale<<<nb, TPB, 0, st>>>(dD, P, lam[0], lam[1]);
   }
 
   // computeOverlapsKokkos, src/sim.hpp:494-541
@@ -1877,14 +1910,14 @@ struct pd_sim {
     const double margin = 0.1 * maxRad;
     numParticles = numReal;
     Params P = params(margin, maxRad);
-    k_overlap_prep<<<nblk(numReal), TPB, 0, st>>>(D, P);
+    k_overlap_prep<<<nblk(numReal), TPB, 0, st>>>(dD, P);
     ghostCnt.ensure(numReal + 1, st);
     ghostOff.ensure(numReal + 1, st);
-    k_ghost_count<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostCnt.p);
+    k_ghost_count<<<nblk(numReal), TPB, 0, st>>>(dD, P, ghostCnt.p);
     const int G = scan(ghostCnt.p, ghostOff.p, numReal);
     if ((size_t)(numReal + G) > pos.cap) ensureSlots(numReal + G, numReal);
     refreshDev();
-    k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(D, P, ghostOff.p, slotCapacity());
+    k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(dD, P, ghostOff.p, slotCapacity());
     numParticles = numReal + G;
     broadAndNarrow(margin);
     numParticles = numReal;
@@ -2281,7 +2314,7 @@ static void sdfGrid(pd_sim* s, int rx, int ry, int rz, std::vector<double>& out)
     HIPCHECK(hipMemsetAsync(fixedCell, 0, total, st));
     s->refreshDev();
     Params P = s->params(0.0, 0.0);
-    if (s->numReal > 0) k_sdf_splat<<<nblk(s->numReal), TPB, 0, st>>>(s->D, P, G, key, fixedCell);
+    if (s->numReal > 0) k_sdf_splat<<<nblk(s->numReal), TPB, 0, st>>>(s->dD, P, G, key, fixedCell);
     k_sdf_from_keys<<<nb, TPB, 0, st>>>(key, g1, total);
     const int iters = std::max(std::max(rx, ry), rz) * 4;
     double* in = g1;
