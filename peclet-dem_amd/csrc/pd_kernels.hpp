@@ -474,6 +474,35 @@ __global__ void k_ghost_emit(const Dev* __restrict__ Dp, Params P, const int* of
       }
 }
 
+// the per-substep control words start at zero: the candidate-check flag, the colouring counters,
+// the support-list count, the sweep-loop flags and counters, and the overlap accumulator
+__global__ void k_substep_reset(int* candFlag, int* counters, int* supCnt, int* ctl, int nCtl,
+                                unsigned long long* overlapBits) {
+  const int t = threadIdx.x;
+  if (t < nCtl) ctl[t] = 0;
+  if (t == 0) {
+    *candFlag = 0;
+    counters[0] = 0;
+    counters[1] = 0;
+    *supCnt = 0;
+    *overlapBits = 0ull;
+  }
+}
+// a few device int ranges copied in one launch to host-visible pinned memory (the host reads
+// them after synchronising the stream): one small kernel instead of a runtime copy per range
+struct PostSpec {
+  const int* src[4];
+  int len[4];
+  int n;
+};
+__global__ void k_post(PostSpec S, int* dst) {
+  int off = 0;
+  for (int r = 0; r < S.n; ++r) {
+    for (int i = threadIdx.x; i < S.len[r]; i += blockDim.x) dst[off + i] = S.src[r][i];
+    off += S.len[r];
+  }
+  __threadfence_system();
+}
 // slot count of the substep from the ghost total: info[0] = reals + the ghosts that fit in
 // capSlots, info[1] = 1 when some did not fit
 __global__ void k_slot_count(const int* ghostTotal, int numReal, int capSlots, int* info) {
@@ -1163,11 +1192,14 @@ __device__ __forceinline__ double npPlanes(const Dev& D, const Params& P, int nu
   if (!WRITE) D.ncnt[numPairs + i] = n;
   return ovl;
 }
+// zeroSlots >= 0: also write a zero count at slot zeroSlots + i (the wall slots, without walls)
 template <bool WRITE>
-__global__ void k_np_planes(const Dev* __restrict__ Dp, Params P, int numPairs) {
+__global__ void k_np_planes(const Dev* __restrict__ Dp, Params P, int numPairs,
+                            int zeroSlots = -1) {
   const Dev& D = *Dp;
   __shared__ unsigned long long sh[WAVES];
   const int i = gtid();
+  if (!WRITE && zeroSlots >= 0 && i < P.numReal) D.ncnt[zeroSlots + i] = 0;
   const double ovl = (i < P.numReal) ? npPlanes<WRITE>(D, P, numPairs, i) : 0.0;
   if (WRITE) blockAtomicMaxU64(D.maxOverlapBits, dbits(ovl), sh);
 }
@@ -1364,10 +1396,13 @@ __device__ __forceinline__ int jpsSlot(int* hKey, int* used, int body, bool inse
   }
   return -1;
 }
+// loadGlobal = 0: the global winner / mask arrays are stale (a colouring starting here), the
+// bodies start from zero
 __global__ __launch_bounds__(JPS_THREADS) void k_jp_small(const int2* ends, int* colour, int* in,
                                                           int* out, int* cnt, int j0, int j1,
                                                           unsigned long long* winnerG,
-                                                          unsigned long long* maskG) {
+                                                          unsigned long long* maskG,
+                                                          int loadGlobal) {
   __shared__ int hKey[JPS_HASH];
   __shared__ unsigned long long hWin[JPS_HASH], hMask[JPS_HASH];
   __shared__ int eId[JPS_EDGES];
@@ -1404,8 +1439,8 @@ __global__ __launch_bounds__(JPS_THREADS) void k_jp_small(const int2* ends, int*
   for (int h = tid; h < JPS_HASH; h += T) {
     const int b = hKey[h];
     if (b >= 0) {
-      hWin[h] = winnerG[b];
-      hMask[h] = maskG[b];
+      hWin[h] = loadGlobal ? winnerG[b] : 0ull;
+      hMask[h] = loadGlobal ? maskG[b] : 0ull;
     }
   }
   __syncthreads();
@@ -2345,45 +2380,39 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_ordered_block(
 
 // commitPairKeysLambdaKokkos, src/contact_preprocessing.hpp:160-192: bucket by the low real index
 // (key >> 32), then sort each bucket by (key, manifold index) — a stable key sort
-__global__ void k_ledger_count(const Dev* __restrict__ Dp, int numManifolds, int numReal) {
+// bucketCnt is all zero on entry (k_ledger_scatter clears what this pass counts).  Each live key
+// takes its place in its bucket from the count's atomic; the dead keys (~0) share one region,
+// their places handed out per block (one atomic per block, then the block's own order).
+__global__ void k_ledger_count(const Dev* __restrict__ Dp, int numManifolds, int numReal,
+                               int* rank) {
   const Dev& D = *Dp;
-  __shared__ int sh[WAVES];
+  __shared__ int shCnt, shBase;
   const int m = gtid();
-  int dead = 0;
+  bool dead = false;
   if (m < numManifolds) {
     const unsigned long long key = D.mKey[m];
-    if (key == ~0ull) dead = 1;
-    else atomicAdd(&D.bucketCnt[(int)(key >> 32)], 1);
+    if (key == ~0ull) dead = true;
+    else rank[m] = atomicAdd(&D.bucketCnt[(int)(key >> 32)], 1);
   }
-  blockAtomicAddI(&D.bucketCnt[numReal], dead, sh);
+  if (threadIdx.x == 0) shCnt = 0;
+  __syncthreads();
+  const int local = dead ? atomicAdd(&shCnt, 1) : 0;
+  __syncthreads();
+  if (threadIdx.x == 0 && shCnt) shBase = atomicAdd(&D.bucketCnt[numReal], shCnt);
+  __syncthreads();
+  if (dead) rank[m] = shBase + local;
 }
-__global__ void k_ledger_scatter(const Dev* __restrict__ Dp, int numManifolds, int numReal) {
+__global__ void k_ledger_scatter(const Dev* __restrict__ Dp, int numManifolds, int numReal,
+                                 const int* rank) {
   const Dev& D = *Dp;
-  __shared__ int ndead, baseDead;
-  if (threadIdx.x == 0) ndead = 0;
-  __syncthreads();
   const int m = gtid();
-  unsigned long long key = ~0ull;
-  int local = -1;
-  if (m < numManifolds) {
-    key = D.mKey[m];
-    if (key == ~0ull) {
-      local = atomicAdd(&ndead, 1);
-    } else {
-      const int b = (int)(key >> 32);
-      const int pos = D.bucketStart[b] + atomicAdd(&D.bucketCnt[b], 1);
-      D.lKey[pos] = key;
-      D.lIdx[pos] = m;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && ndead)
-    baseDead = D.bucketStart[numReal] + atomicAdd(&D.bucketCnt[numReal], ndead);
-  __syncthreads();
-  if (local >= 0) {
-    D.lKey[baseDead + local] = key;
-    D.lIdx[baseDead + local] = m;
-  }
+  if (m >= numManifolds) return;
+  const unsigned long long key = D.mKey[m];
+  const int b = key == ~0ull ? numReal : (int)(key >> 32);
+  const int pos = D.bucketStart[b] + rank[m];
+  D.lKey[pos] = key;
+  D.lIdx[pos] = m;
+  D.bucketCnt[b] = 0;  // back to all zero for the next commit
 }
 __global__ void k_ledger_sort(const Dev* __restrict__ Dp, int numReal) {
   const Dev& D = *Dp;

@@ -266,6 +266,7 @@ struct pd_sim {
   // pinned host scratch
   static constexpr size_t HOST_SCRATCH_BYTES = 4096;
   void* hostScratch = nullptr;
+  void* hostScratchDev = nullptr;  // device view of the pinned scratch
 
   Dev D{};
 
@@ -288,6 +289,9 @@ struct pd_sim {
     }
     HIPCHECK(hipHostMalloc(&jobV.pinned, 1024, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&jobP.pinned, 1024, hipHostMallocDefault));
+    HIPCHECK(hipHostGetDevicePointer(&hostScratchDev, hostScratch, 0));
+    HIPCHECK(hipHostGetDevicePointer(&jobV.pinnedDev, jobV.pinned, 0));
+    HIPCHECK(hipHostGetDevicePointer(&jobP.pinnedDev, jobP.pinned, 0));
     jobV.stream = st;
     jobP.stream = sideStream;
     ensureSlots(cap > 0 ? cap : 1, 0);
@@ -295,6 +299,8 @@ struct pd_sim {
     bits.ensure(4, st);
     counters.ensure(16, st);
     ctl.ensure(CTL_COUNT + 1, st);
+    candFlag.ensure(1, st);
+    supCnt.ensure(1, st);
     pairTableD.ensure(128, st);
     shell.ensure(1, st);
     initializeShape(SHAPE_SPHERE, 1.0, 0.0, 0.0);
@@ -331,6 +337,7 @@ struct pd_sim {
     cellStart.release(); gridD.release(); ncellD.release(); slotInfo.release();
     cellCntDirty = true;
     cand.release(); candStart.release(); candFlag.release(); candKeep.release();
+    lRank.release(); ledgerCntDirty = true;
     candCode.release(); candRef.release(); candValid = false;
     order.release(); pairCnt.release(); pairStart.release(); pairStage.release();
     pairOfContact.release(); matId.release(); mCode.release();
@@ -381,6 +388,35 @@ struct pd_sim {
     }
     HIPCHECK(hipMemcpyAsync(host, dptr, n * sizeof(T), hipMemcpyDeviceToHost, st));
     sync();
+  }
+  // Ranges read back together through one k_post launch into pinned memory `area` (host view;
+  // areaDev is the device view): add() returns where each range will be, post() launches.
+  struct PostBatch {
+    PostSpec spec{};
+    int used = 0;
+    int* area = nullptr;
+    int* areaDev = nullptr;
+    int cap = 0;
+    const int* add(const int* src, int len) {
+      if (spec.n >= 4 || used + len > cap) throw std::runtime_error("read-back batch too large");
+      spec.src[spec.n] = src;
+      spec.len[spec.n] = len;
+      ++spec.n;
+      const int* h = area + used;
+      used += len;
+      return h;
+    }
+    void post(hipStream_t s) {
+      k_post<<<1, 128, 0, s>>>(spec, areaDev);
+      HIPCHECK(hipGetLastError());
+    }
+  };
+  PostBatch postBatch(void* pinnedArea, void* pinnedAreaDev, int capInts) {
+    PostBatch b;
+    b.area = static_cast<int*>(pinnedArea);
+    b.areaDev = static_cast<int*>(pinnedAreaDev);
+    b.cap = capInts;
+    return b;
   }
   // several small reads with one synchronisation: enqueue each into the pinned scratch at
   // `off` (bytes, advanced), sync() once, then read the returned host pointers
@@ -821,6 +857,8 @@ struct pd_sim {
     return (int)std::min<size_t>(c, (size_t)INT32_MAX);
   }
   DBuf<int> slotInfo;  // [0] slot count of the substep, [1] ghosts did not fit
+  DBuf<int> lRank;     // ledger: each manifold's place in its bucket
+  bool ledgerCntDirty = true;
   // candidate pairs of the last rebuild (boxes grown by candSkin), sorted with per-real offsets,
   // the slots' plain boxes and the reals' ghost images at that rebuild
   DBuf<int2> cand;
@@ -846,7 +884,7 @@ struct pd_sim {
     return W;
   }
   // contact slots: pairs, then planes per real, then walls per real (the reference's order)
-  void narrowphase(const Params& P) {
+  void narrowphase(const Params& P, bool zeroOverlap = true) {
     const int nslots = np + 2 * numReal;
     const WallSet W = wallSet();
     ncnt.ensure(nslots + 1, st);
@@ -856,9 +894,9 @@ struct pd_sim {
     const int npBlocks = (int)std::max<long>(1, ((long)np + NP_WAVES - 1) / NP_WAVES);
     if (waveNp) k_np_pairs_wave<false><<<npBlocks, TPB, 0, st>>>(dD, P, np);
     else k_np_pairs<false><<<nblk(np), TPB, 0, st>>>(dD, P, np);
-    k_np_planes<false><<<nblk(numReal), TPB, 0, st>>>(dD, P, np);
+    // without walls every wall slot holds 0 contacts (written by the planes pass)
+    k_np_planes<false><<<nblk(numReal), TPB, 0, st>>>(dD, P, np, W.n > 0 ? -1 : np + numReal);
     if (W.n > 0) k_np_walls<false><<<nblk(numReal), TPB, 0, st>>>(dD, P, W, np + numReal);
-    else zero(ncnt.p + np + numReal, numReal);  // no walls: every wall slot holds 0 contacts
     if (waveNp) {
       nc = scan(ncnt.p, noff.p, nslots);
     } else {
@@ -879,7 +917,7 @@ struct pd_sim {
     cRB.ensure(nc + 1, st);
     cSlot.ensure(nc + 1, st);
     refreshDev();
-    zero(bits.p, 1);
+    if (zeroOverlap) zero(bits.p, 1);  // a substep clears it in k_substep_reset
     kernBegin();
     if (waveNp) k_np_pairs_wave<true><<<npBlocks, TPB, 0, st>>>(dD, P, np);
     else if (nc > 0) k_np_pairs_sparse<<<nblk(nc), TPB, 0, st>>>(dD, P, np, pairOfContact.p);
@@ -918,8 +956,12 @@ struct pd_sim {
     // rounds expected to hold at most this many edges run in one workgroup out of LDS
     long smallRows = JPS_EDGES * 3 / 4;
     bool noSmall = false;  // this colouring's list did not fit: launch-per-round batches only
+    bool globalDirty = true;  // winner / mask hold another colouring's values
+    int smallLoad = 1;        // whether the batch's LDS kernel loaded the global slots
+    int numBodies = 0;
     int smallAt = -1;      // round of the current batch where the one-workgroup kernel starts
-    void* pinned = nullptr;  // read-back area (pinned host memory)
+    void* pinned = nullptr;     // read-back area (pinned host memory)
+    void* pinnedDev = nullptr;  // its device view
     DBuf<int> work[2], cnt, blockCnt, colCnt, colStart;
     DBuf<unsigned long long> winner, mask;
     std::vector<int> hist;  // list size entering each round, last colouring of this job
@@ -961,8 +1003,8 @@ struct pd_sim {
     J.blockCnt.ensure(64 * CP_BLOCKS, st);
     J.colCnt.ensure(64, st);
     J.colStart.ensure(65, st);
-    HIPCHECK(hipMemsetAsync(J.mask.p, 0, numBodies * sizeof(unsigned long long), J.stream));
-    HIPCHECK(hipMemsetAsync(J.winner.p, 0, numBodies * sizeof(unsigned long long), J.stream));
+    J.numBodies = numBodies;
+    J.globalDirty = true;  // cleared lazily: a colouring run wholly in LDS never reads them
     J.ends = ends;
     J.colour = colour;
     J.perm = perm;
@@ -1006,6 +1048,15 @@ struct pd_sim {
           break;
         }
     J.B = B;
+    const bool launchRounds = J.smallAt != 0;  // some rounds of this batch are launched one by one
+    const int loadGlobal = J.globalDirty ? 0 : 1;
+    if (launchRounds && J.globalDirty) {
+      HIPCHECK(hipMemsetAsync(J.mask.p, 0, J.numBodies * sizeof(unsigned long long), s));
+      HIPCHECK(hipMemsetAsync(J.winner.p, 0, J.numBodies * sizeof(unsigned long long), s));
+    }
+    // after this batch the global slots are valid for every body a later round can touch: zeroed
+    // above, or written back by the LDS kernel for the bodies of the edges it leaves
+    J.globalDirty = false;
     k_jp_counts<<<1, 64, 0, s>>>(J.cnt.p, J.first ? J.enteringDev : nullptr, (int)J.entering,
                                  B + 1);
     for (int j = 0; j < B; ++j) {
@@ -1013,8 +1064,9 @@ struct pd_sim {
       int* work = J.work[(J.cur + j) & 1].p;
       int* next = J.work[(J.cur + j + 1) & 1].p;
       if (j == J.smallAt) {
+        J.smallLoad = j == 0 ? loadGlobal : 1;
         k_jp_small<<<1, JPS_THREADS, 0, s>>>(J.ends, J.colour, work, next, J.cnt.p, j, B,
-                                             J.winner.p, J.mask.p);
+                                             J.winner.p, J.mask.p, J.smallLoad);
         break;
       }
       const int grid = (int)std::min<long>(nblk(est + est / 4 + 64), JP_GRID_CAP);
@@ -1029,11 +1081,10 @@ struct pd_sim {
     k_color_place<<<nbc, TPB, 0, s>>>(J.colour, J.numEdges, J.blockCnt.p, J.colStart.p, J.perm,
                                       J.rowOf);
     HIPCHECK(hipGetLastError());
-    char* h = static_cast<char*>(J.pinned);
-    HIPCHECK(hipMemcpyAsync(h, J.cnt.p, (B + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(h + 256, J.colStart.p, 65 * sizeof(int), hipMemcpyDeviceToHost, s));
-    J.hostCnt = reinterpret_cast<const int*>(h);
-    J.hostStarts = reinterpret_cast<const int*>(h + 256);
+    PostBatch rb = postBatch(J.pinned, J.pinnedDev, 256);
+    J.hostCnt = rb.add(J.cnt.p, B + 1);
+    J.hostStarts = rb.add(J.colStart.p, 65);
+    rb.post(s);
     J.first = false;
   }
   // wait for the job's read-back and replay its rounds; enqueue further batches until done.
@@ -1047,6 +1098,8 @@ struct pd_sim {
       const bool failed = J.smallAt >= 0 && J.hostCnt[J.smallAt + 1] < 0;
       const int B = failed ? J.smallAt : J.B;
       if (failed) {
+        // it changed nothing: global slots it would have started from zero are still stale
+        if (J.smallAt == 0 && J.smallLoad == 0) J.globalDirty = true;
         J.noSmall = true;
         J.smallRows = std::max<long>(256, J.smallRows / 2);  // lists of this job are too wide
       }
@@ -1228,6 +1281,7 @@ struct pd_sim {
     const double margin = 0.1 * maxRad;
     numParticles = numReal;
     Params P = params(margin, maxRad);
+    k_substep_reset<<<1, 64, 0, st>>>(candFlag.p, counters.p, supCnt.p, ctl.p, CTL_COUNT, bits.p);
     kernBegin();
     k_predict<<<nblk(numReal), TPB, 0, st>>>(dD, P);
     kernEnd(K_PREDICT, numReal);
@@ -1254,10 +1308,8 @@ struct pd_sim {
       HIPCHECK(hipGetLastError());
       mark(ST_BROAD);
       const long gridSlots = std::min<long>(capSlots, (long)numReal + lastGhosts + 1024);
-      size_t off = 0;
       if (useCand) {
         // speculative: filter the candidates and check, in the same read-back, that they hold
-        zero(candFlag.p, 1);
         pairCnt.ensure(numReal + 1, st);
         pairStart.ensure(numReal + 1, st);
         refreshDev();
@@ -1272,9 +1324,11 @@ struct pd_sim {
         PS.skin = skin;
         broadphaseQuery(PS, slotInfo.p, (int)gridSlots, capSlots);
       }
-      const int* npH = enqueueRead(pairStart.p + numReal, 1, off);
-      const int* gH = enqueueRead(ghostOff.p + numReal, 1, off);
-      const int* fH = useCand ? enqueueRead(candFlag.p, 1, off) : nullptr;
+      PostBatch rb = postBatch(hostScratch, hostScratchDev, 64);
+      const int* npH = rb.add(pairStart.p + numReal, 1);
+      const int* gH = rb.add(ghostOff.p + numReal, 1);
+      const int* fH = useCand ? rb.add(candFlag.p, 1) : nullptr;
+      rb.post(st);
       sync();
       np = *npH;
       G = *gH;
@@ -1329,7 +1383,7 @@ struct pd_sim {
       broadphaseCompact(PB);
     }
     mark(ST_NARROW);
-    narrowphase(PB);
+    narrowphase(PB, false);
     mark(ST_MANIFOLD);
     buildManifolds();
     if (legacyFriction) {
@@ -1342,7 +1396,6 @@ struct pd_sim {
     HIPCHECK(hipEventRecord(evContacts, st));
 
     // manifold colouring, src/solver_velocity.hpp:277-374
-    zero(counters.p, 2);
     vPerm.ensure(nm + 1, st);
     jobV.work[0].ensure(nm + 1, st);
     jobV.work[1].ensure(nm + 1, st);
@@ -1370,7 +1423,6 @@ struct pd_sim {
     refreshDev();
     supList.ensure(nm + 1, st);
     supCnt.ensure(1, st);
-    zero(supCnt.p, 1);
     if (nm > 0) k_mkeys<<<nblk(nm), TPB, 0, st>>>(dD, nm, gHat, supList.p, supCnt.p);
     if (nRowsV > 0)
       k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(dD, P, nRowsV, prevCount > 0 ? ledgerBuckets : 0,
@@ -1404,7 +1456,6 @@ struct pd_sim {
                           legacyFriction;
     if (!hostLoop) {
       // device-controlled adaptive stops: no host round trip per sweep
-      zero(ctl.p, CTL_COUNT);
       const double stopThr = usePGS ? 0.02 * vRest : vRest;
       const bool smallV = nRowsV <= blockRows;
       if (smallV && velIters > 0) {
@@ -1453,7 +1504,6 @@ struct pd_sim {
       }
       HIPCHECK(hipGetLastError());
     } else {
-      zero(ctl.p, CTL_COUNT);
       Params PF = params(margin, maxRad);
       for (int it = 0; it < velIters; ++it) {
         if (legacyFriction && nc > 0)
@@ -1529,11 +1579,15 @@ struct pd_sim {
       lLam.ensure(nm + 1, st);
       lLamT.ensure(nm + 1, st);
       refreshDev();
-      zero(bucketCnt.p, numReal + 1);
-      k_ledger_count<<<nblk(nm), TPB, 0, st>>>(dD, nm, numReal);
+      lRank.ensure(nm + 1, st);
+      // bucketCnt is all zero between commits (the scatter clears what the count set); a commit
+      // cut short by an error leaves it dirty
+      if (ledgerCntDirty) zero(bucketCnt.p, bucketCnt.cap);
+      ledgerCntDirty = true;
+      k_ledger_count<<<nblk(nm), TPB, 0, st>>>(dD, nm, numReal, lRank.p);
       scanQuiet(bucketCnt.p, bucketStart.p, numReal + 1);
-      zero(bucketCnt.p, numReal + 1);
-      k_ledger_scatter<<<nblk(nm), TPB, 0, st>>>(dD, nm, numReal);
+      k_ledger_scatter<<<nblk(nm), TPB, 0, st>>>(dD, nm, numReal, lRank.p);
+      ledgerCntDirty = false;
       k_ledger_sort<<<nblk(numReal), TPB, 0, st>>>(dD, numReal);
       k_ledger_values<<<nblk(nm), TPB, 0, st>>>(dD, nm);
       HIPCHECK(hipGetLastError());
@@ -1634,9 +1688,11 @@ struct pd_sim {
         }
         HIPCHECK(hipGetLastError());
         done += b;
-        size_t off = 0;
-        finalCtl = enqueueRead(ctl.p, CTL_COUNT, off);
-        finalBits = enqueueRead(bits.p, 2, off);
+        PostBatch rb = postBatch(hostScratch, hostScratchDev, 64);
+        finalCtl = rb.add(ctl.p, CTL_COUNT);
+        finalBits = reinterpret_cast<const unsigned long long*>(
+            rb.add(reinterpret_cast<const int*>(bits.p), 4));
+        rb.post(st);
         sync();
         if (finalCtl[CTL_STOP_P]) break;
         batch *= 2;
@@ -1665,9 +1721,11 @@ struct pd_sim {
       // one read-back of the loop counters and the residuals (the position loop's last one
       // when it ran in batches)
       if (!finalCtl) {
-        size_t off = 0;
-        finalCtl = enqueueRead(ctl.p, CTL_COUNT, off);
-        finalBits = enqueueRead(bits.p, 2, off);
+        PostBatch rb = postBatch(hostScratch, hostScratchDev, 64);
+        finalCtl = rb.add(ctl.p, CTL_COUNT);
+        finalBits = reinterpret_cast<const unsigned long long*>(
+            rb.add(reinterpret_cast<const int*>(bits.p), 4));
+        rb.post(st);
         sync();
       }
       const int* c = finalCtl;
