@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--block-rows", type=int, default=None, metavar="ROWS",
                     help="largest sweep loop solved inside one workgroup (library default 16384; "
                          "0 = one launch per colour)")
+    ap.add_argument("--tuning", action="append", default=[], metavar="NAME=VALUE",
+                    help="any library tuning knob (pd_set_tuning), e.g. vloop_workgroups=1")
     ap.add_argument("--timing-every", type=int, default=8, metavar="N",
                     help="record timing events on every N-th timed substep (bounds their host "
                          "cost; averages come from the sampled substeps)")
@@ -254,6 +256,9 @@ def main():
     sim = setup(Simulation(args.n), args.workload, bed, dt)
     if args.block_rows is not None:
         sim.set_tuning("block_solver_rows", args.block_rows)
+    for kv in args.tuning:
+        name, value = kv.split("=", 1)
+        sim.set_tuning(name, int(value))
     print(f"# workload ready: n={args.n} (generation {t_gen:.1f}s)", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         sim.step(dt)

@@ -52,108 +52,119 @@ struct Grid {
   int dim[3];
 };
 
+// The device pointer table below lives in device memory and kernels read it through a pointer,
+// so a pointer member would otherwise be a generic (flat) address: flat accesses also count
+// against the LDS / scalar-load wait counter and take a 64-bit VGPR address.  In the device pass
+// the members are declared in the global address space, so accesses through them compile to
+// global_* instructions; the host pass sees plain pointers of the same size and layout.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PD_G __attribute__((address_space(1)))
+#else
+#define PD_G
+#endif
+
 // device pointers of one simulation
 struct Dev {
   // per slot
-  double4* pos;
-  Body64* vw0;
-  BodyC* bc;
-  double4* posPred;
-  double4* quatPred;
-  Body64* vw;
-  double* scale;
-  double* targetScale;
-  double4* extForce;
-  int* realIdx;
-  unsigned char* matId;
-  int* ground;
-  int* ground2;
-  double4* shell;
-  double* pairTable;
+  PD_G double4* pos;
+  PD_G Body64* vw0;
+  PD_G BodyC* bc;
+  PD_G double4* posPred;
+  PD_G double4* quatPred;
+  PD_G Body64* vw;
+  PD_G double* scale;
+  PD_G double* targetScale;
+  PD_G double4* extForce;
+  PD_G int* realIdx;
+  PD_G unsigned char* matId;
+  PD_G int* ground;
+  PD_G int* ground2;
+  PD_G double4* shell;
+  PD_G double* pairTable;
   // broadphase
-  int* cellOf;
-  int* cellRank;
-  int* cellCnt;
-  int* cellStart;
-  int* order;
-  double4* sbox;
-  int* pairCnt;
-  int* pairStart;
-  int2* pairs;
+  PD_G int* cellOf;
+  PD_G int* cellRank;
+  PD_G int* cellCnt;
+  PD_G int* cellStart;
+  PD_G int* order;
+  PD_G double4* sbox;
+  PD_G int* pairCnt;
+  PD_G int* pairStart;
+  PD_G int2* pairs;
   // narrowphase
-  int* ncnt;  // [numPairs + numReal] contact counts (pairs, then planes per real)
-  int* noff;  // exclusive scan of ncnt
-  int2* cAB;
-  double4* cN;
-  double4* cRA;
-  double4* cRB;
-  double4* cBV;   // boundary (wall) surface velocity at the contact, zero otherwise
-  int* cSlot;
+  PD_G int* ncnt;  // [numPairs + numReal] contact counts (pairs, then planes per real)
+  PD_G int* noff;  // exclusive scan of ncnt
+  PD_G int2* cAB;
+  PD_G double4* cN;
+  PD_G double4* cRA;
+  PD_G double4* cRB;
+  PD_G double4* cBV;   // boundary (wall) surface velocity at the contact, zero otherwise
+  PD_G int* cSlot;
   // manifolds
-  int* mCnt;
-  int* mStart;
-  int2* mAB;
-  int2* mReal;
-  int2* mRange;
-  int2* mRange2;  // boundary manifolds: the wall-contact range (planes are mRange)
-  double4* mNs;
-  double4* mTA;
-  double4* mTB;
-  double4* mRA;
-  double4* mRB;
-  double4* mWV;   // sum of the boundary velocities of the active contacts
-  int* mNp;
-  unsigned long long* mKey;
-  unsigned char* mCode;
+  PD_G int* mCnt;
+  PD_G int* mStart;
+  PD_G int2* mAB;
+  PD_G int2* mReal;
+  PD_G int2* mRange;
+  PD_G int2* mRange2;  // boundary manifolds: the wall-contact range (planes are mRange)
+  PD_G double4* mNs;
+  PD_G double4* mTA;
+  PD_G double4* mTB;
+  PD_G double4* mRA;
+  PD_G double4* mRB;
+  PD_G double4* mWV;   // sum of the boundary velocities of the active contacts
+  PD_G int* mNp;
+  PD_G unsigned long long* mKey;
+  PD_G unsigned char* mCode;
   // colouring
-  int* mColor;
-  int* cColor;
-  int* vPerm;
-  int* vRowOf;
-  int* pPerm;
-  int* pRowOf;
+  PD_G int* mColor;
+  PD_G int* cColor;
+  PD_G int* vPerm;
+  PD_G int* vRowOf;
+  PD_G int* pPerm;
+  PD_G int* pRowOf;
   // velocity rows
-  int2* vAB;
-  double4* vN;
-  double4* vTA;
-  double4* vTB;
-  double4* vRA;
-  double4* vRB;
-  double4* vVt0;
-  double4* vPre;  // per row, constant in the solve: (wA_n, wB_n, growth term, 1 / lenN)
-  double4* vLamT;
-  double4* vWV;   // boundary rows: wall velocity sum (the body-B velocity is vWV / points)
-  double* vVn0;
-  double* vLam;
-  unsigned char* vFlag;
+  PD_G int2* vAB;
+  PD_G double4* vN;
+  PD_G double4* vTA;
+  PD_G double4* vTB;
+  PD_G double4* vRA;
+  PD_G double4* vRB;
+  PD_G double4* vVt0;
+  PD_G double4* vPre;  // per row, constant in the solve: (wA_n, wB_n, growth term, 1 / lenN)
+  PD_G double4* vLamT;
+  PD_G double4* vWV;   // boundary rows: wall velocity sum (the body-B velocity is vWV / points)
+  PD_G double* vVn0;
+  PD_G double* vLam;
+  PD_G unsigned char* vFlag;
   // position rows
-  int2* pAB;
-  double4* pN;
-  double4* pRA;
-  double4* pRB;
-  double* pLam;
+  PD_G int2* pAB;
+  PD_G double4* pN;
+  PD_G double4* pRA;
+  PD_G double4* pRB;
+  PD_G double* pLam;
   // warm-start ledger (sorted by key)
-  unsigned long long* lKey;
-  double* lLam;
-  double4* lLamT;  // (lambdaT.xyz, position-channel impulse)
-  int* lIdx;
-  unsigned long long* uKey;
-  int* bucketCnt;
-  int* bucketStart;
+  PD_G unsigned long long* lKey;
+  PD_G double* lLam;
+  PD_G double4* lLamT;  // (lambdaT.xyz, position-channel impulse)
+  PD_G int* lIdx;
+  PD_G unsigned long long* uKey;
+  PD_G int* bucketCnt;
+  PD_G int* bucketStart;
   // scalars
-  unsigned long long* red;  // reduction scratch (ordered keys)
-  int* counters;            // [0] remaining JP edges, [1] uncoloured entering, [2..] misc
-  unsigned long long* maxOverlapBits;
-  unsigned long long* maxApproachBits;
-  double* thermo;           // partial sums
+  PD_G unsigned long long* red;  // reduction scratch (ordered keys)
+  PD_G int* counters;            // [0] remaining JP edges, [1] uncoloured entering, [2..] misc
+  PD_G unsigned long long* maxOverlapBits;
+  PD_G unsigned long long* maxApproachBits;
+  PD_G double* thermo;           // partial sums
   // Jacobi fallback accumulators (per slot)
-  double4* dVel;
-  double4* dAng;
-  double4* dPos;
-  int* dCnt;
+  PD_G double4* dVel;
+  PD_G double4* dAng;
+  PD_G double4* dPos;
+  PD_G int* dCnt;
   // legacy count-averaged friction (per contact normal impulse, per body contact count)
-  double* cLamN;
-  int* frCnt;
+  PD_G double* cLamN;
+  PD_G int* frCnt;
 };
 
 // velocity-row flag bits
@@ -1888,10 +1899,51 @@ __device__ void warmApplyRow(const Dev& D, int k) {
   if (realB >= 0) D.vw[realB] = bb;
 }
 
+// Where a velocity row finds its bodies' velocities: the global array (GBodies), or a copy of
+// the rows' bodies staged in LDS by a one-workgroup sweep loop (LBodies: 6 doubles per body,
+// rows hold the two LDS slots).  The arithmetic is the same either way.
+struct GBodies {
+  static constexpr bool kEarly = false;  // row state read where it is used
+  Body64* vw;
+  __device__ void row(const Dev& D, int k, int2& r, int2& sl) const {
+    r = D.vAB[k];
+    sl = make_int2(0, 0);
+  }
+  __device__ Body64 load(int real, int) const { return vw[real]; }
+  __device__ void store(int real, int, const Body64& b) const { vw[real] = b; }
+};
+struct LBodies {
+  static constexpr bool kEarly = true;  // row state read up front (more registers)
+  double2* bv;             // body s: bv[3s] = (v.x, v.y), bv[3s+1] = (v.z, w.x), bv[3s+2] = (w.y, w.z)
+  const unsigned* slots;   // row k - r0: slot of a | slot of b << 16 (0xFFFF: no body b)
+  const int* bodyOf;       // slot -> real body
+  int r0;
+  __device__ void row(const Dev&, int k, int2& r, int2& sl) const {
+    const unsigned p = slots[k - r0];
+    sl.x = (int)(p & 0xFFFFu);
+    sl.y = (int)(p >> 16);
+    r.x = bodyOf[sl.x];
+    r.y = sl.y != 0xFFFF ? bodyOf[sl.y] : -1;
+  }
+  __device__ Body64 load(int, int s) const {
+    const double2 a = bv[3 * s], b = bv[3 * s + 1], c = bv[3 * s + 2];
+    Body64 o;
+    o.v = make_double4(a.x, a.y, b.x, 0.0);
+    o.w = make_double4(b.y, c.x, c.y, 0.0);
+    return o;
+  }
+  __device__ void store(int, int s, const Body64& o) const {
+    bv[3 * s] = make_double2(o.v.x, o.v.y);
+    bv[3 * s + 1] = make_double2(o.v.z, o.w.x);
+    bv[3 * s + 2] = make_double2(o.w.y, o.w.z);
+  }
+};
+
 // PGSManifoldSweep::solveOne, src/solver_velocity.hpp:641-858 (one row)
-__device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
+template <class BS>
+__device__ double pgsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
+                          double restVelThreshold, const BS& bs) {
   double amax = 0.0;
-  const int2 r = D.vAB[k];
   const int realA = r.x, realB = r.y;
   const BodyC ca = D.bc[realA];
   const double invMassA = ca.iim.w;
@@ -1901,17 +1953,24 @@ __device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThr
   double invMassB = 0.0;
   d3 invIB{0, 0, 0};
   q4 qB{0, 0, 0, 1};
-  Body64 ba = D.vw[realA];
+  Body64 ba = bs.load(realA, sl.x);
   Body64 bb;
   d3 vB{0, 0, 0}, wB{0, 0, 0};
   const double4 tb4 = D.vTB[k];
   const double invN = tb4.w;
+  // with kEarly the row's friction state is read up front: it does not depend on the normal
+  // update, so its load latency overlaps the body loads
+  double4 lto4, vt04;
+  if constexpr (BS::kEarly) {
+    lto4 = D.vLamT[k];
+    vt04 = D.vVt0[k];
+  }
   if (realB >= 0) {
     cb = D.bc[realB];
     invMassB = cb.iim.w;
     invIB = xyz(cb.iim);
     qB = toq(cb.q);
-    bb = D.vw[realB];
+    bb = bs.load(realB, sl.y);
     vB = xyz(bb.v);
     wB = xyz(bb.w);
   } else {
@@ -1954,6 +2013,7 @@ __device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThr
   double pNew = pOld + dp;
   if (pNew < 0.0) pNew = 0.0;
   const double dApplied = pNew - pOld;
+  const double lamNow = dApplied != 0.0 ? pNew : pOld;  // the row's impulse after the update
   bool dirtyA = false, dirtyB = false;
   if (dApplied != 0.0) {
     D.vLam[k] = pNew;
@@ -1983,7 +2043,7 @@ __device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThr
   }
   // friction cone, src/solver_velocity.hpp:760-855
   double mu = ta4.w;
-  const double4 lto4 = D.vLamT[k];
+  if constexpr (!BS::kEarly) lto4 = D.vLamT[k];
   const d3 ltOld = xyz(lto4);
   const bool haveOld = nonzero3(ltOld);
   if (mu > 0.0 || haveOld) {
@@ -2001,7 +2061,7 @@ __device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThr
     double beta = P.restT;
     if (beta != 0.0 && (sf != 0 || fabs(vn0) < restVelThreshold * lenN)) beta = 0.0;
     d3 vtErr = vt;
-    const double4 vt04 = D.vVt0[k];
+    if constexpr (!BS::kEarly) vt04 = D.vVt0[k];
     if (beta != 0.0) {
       d3 v0 = xyz(vt04);
       v0 = sub3(v0, scale3(nhat, dot3(v0, nhat)));
@@ -2019,7 +2079,7 @@ __device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThr
       if (wT > 1e-9) ltNew = add3(ltOld, scale3(that, -vtLen / wT));
     }
     ltNew = sub3(ltNew, scale3(nhat, dot3(ltNew, nhat)));
-    double nTot = fmax(D.vLam[k], 0.0) * lenN;
+    double nTot = fmax(lamNow, 0.0) * lenN;
     if (fabs(vn0) < restVelThreshold * lenN) nTot += fmax(vt04.w, 0.0);
     const double bound = mu * nTot;
     const double ltLen = sqrt(dot3(ltNew, ltNew));
@@ -2051,14 +2111,21 @@ __device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThr
       }
     }
   }
-  if (dirtyA) D.vw[realA] = ba;
-  if (dirtyB) D.vw[realB] = bb;
+  if (dirtyA) bs.store(realA, sl.x, ba);
+  if (dirtyB) bs.store(realB, sl.y, bb);
   return amax;
+}
+__device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
+  const GBodies g{D.vw};
+  int2 r, sl;
+  g.row(D, k, r, sl);
+  return pgsRowB(D, P, k, r, sl, restVelThreshold, g);
 }
 
 // solveVelocityColoredGSKokkos one-shot restitution (g = 0), src/solver_velocity.hpp:989-1133
-__device__ double gsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
-  const int2 r = D.vAB[k];
+template <class BS>
+__device__ double gsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
+                         double restVelThreshold, const BS& bs) {
   const int realA = r.x, realB = r.y;
   const BodyC ca = D.bc[realA];
   const double invMassA = ca.iim.w;
@@ -2068,7 +2135,7 @@ __device__ double gsRow(const Dev& D, const Params& P, int k, double restVelThre
   double invMassB = 0.0;
   d3 invIB{0, 0, 0};
   q4 qB{0, 0, 0, 1};
-  Body64 ba = D.vw[realA];
+  Body64 ba = bs.load(realA, sl.x);
   Body64 bb;
   d3 vB{0, 0, 0}, wB{0, 0, 0};
   const double4 tb4 = D.vTB[k];
@@ -2078,7 +2145,7 @@ __device__ double gsRow(const Dev& D, const Params& P, int k, double restVelThre
     invMassB = cb.iim.w;
     invIB = xyz(cb.iim);
     qB = toq(cb.q);
-    bb = D.vw[realB];
+    bb = bs.load(realB, sl.y);
     vB = xyz(bb.v);
     wB = xyz(bb.w);
   } else {
@@ -2117,7 +2184,7 @@ __device__ double gsRow(const Dev& D, const Params& P, int k, double restVelThre
     ba.w.y += dw.y;
     ba.w.z += dw.z;
   }
-  D.vw[realA] = ba;
+  bs.store(realA, sl.x, ba);
   if (realB >= 0) {
     bb.v.x += -Jlin.x * invMassB;
     bb.v.y += -Jlin.y * invMassB;
@@ -2126,9 +2193,15 @@ __device__ double gsRow(const Dev& D, const Params& P, int k, double restVelThre
     bb.w.x += dw.x;
     bb.w.y += dw.y;
     bb.w.z += dw.z;
-    D.vw[realB] = bb;
+    bs.store(realB, sl.y, bb);
   }
   return amax;
+}
+__device__ double gsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
+  const GBodies g{D.vw};
+  int2 r, sl;
+  g.row(D, k, r, sl);
+  return gsRowB(D, P, k, r, sl, restVelThreshold, g);
 }
 
 // computeSideFlagsKokkos, src/solver_velocity.hpp:456-495 (rows are active, non-duplicate)
@@ -2151,7 +2224,7 @@ __global__ void k_side_flags(const Dev* __restrict__ Dp, int numRows, d3 gHat, d
 // mode: 0 = warm start, 1 = PGS sweep, 2 = one-shot GS sweep
 // ---- device-side loop control: ctl[0..] ints (see pd_sim.hip), sweeps after a stop are no-ops
 enum { CTL_STOP_V = 0, CTL_SWEEPS_V, CTL_STAB_RUN, CTL_STOP_S, CTL_SWEEPS_S, CTL_STOP_P,
-       CTL_SWEEPS_P, CTL_NONE, CTL_COUNT };
+       CTL_SWEEPS_P, CTL_NONE, CTL_VFALLBACK, CTL_COUNT };
 __global__ void k_sweep_begin(unsigned long long* bits, const int* stop) {
   if (!*stop) *bits = 0ull;
 }
@@ -2222,15 +2295,13 @@ __device__ __forceinline__ unsigned long long blockMaxAll(unsigned long long v,
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = sh[w] > m ? sh[w] : m;
   return m;
 }
-// MODE 0 = warm start (one pass), 1 = PGS, 2 = one-shot GS.  `run` (optional) gates the whole
-// loop; `stop` / `count` / `bits` are the loop's stop flag, sweep counter and residual.
-template <int MODE>
-__global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
-    const Dev* __restrict__ Dp, Params P, ClassStarts cs, int numColors, int numManifolds, int iters, double thr,
-    double stopThr, int* stop, int* count, const int* run) {
-  const Dev& D = *Dp;
-  __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
-  if ((run && !*run) || *stop) return;
+// The sweep loop proper (MODE 1 = PGS, 2 = one-shot GS) over rows whose bodies live in `bs`;
+// returns after the sweep that meets the stop rule or after `iters` sweeps.
+template <int MODE, class BS>
+__device__ __forceinline__ void vSweepLoop(const Dev& D, const Params& P, const ClassStarts& cs,
+                                           int numColors, int numManifolds, int iters, double thr,
+                                           double stopThr, int* stop, int* count, const BS& bs,
+                                           unsigned long long* sh) {
   for (int it = 0; it < iters; ++it) {
     unsigned long long m = 0;
     for (int c = 0; c < numColors; ++c) {
@@ -2241,24 +2312,23 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
           for (int q = 0; q < numManifolds; ++q) {
             if (D.mColor[q] != c) continue;
             const int k = D.vRowOf[q];
-            if (MODE == 0) warmApplyRow(D, k);
-            else {
-              const unsigned long long a = dbits(MODE == 1 ? pgsRow(D, P, k, thr) : gsRow(D, P, k, thr));
-              m = a > m ? a : m;
-            }
+            int2 r, sl;
+            bs.row(D, k, r, sl);
+            const unsigned long long a = dbits(MODE == 1 ? pgsRowB(D, P, k, r, sl, thr, bs)
+                                                         : gsRowB(D, P, k, r, sl, thr, bs));
+            m = a > m ? a : m;
           }
       } else {
         for (int k = b + (int)threadIdx.x; k < e; k += blockDim.x) {
-          if (MODE == 0) warmApplyRow(D, k);
-          else {
-            const unsigned long long a = dbits(MODE == 1 ? pgsRow(D, P, k, thr) : gsRow(D, P, k, thr));
-            m = a > m ? a : m;
-          }
+          int2 r, sl;
+          bs.row(D, k, r, sl);
+          const unsigned long long a = dbits(MODE == 1 ? pgsRowB(D, P, k, r, sl, thr, bs)
+                                                       : gsRowB(D, P, k, r, sl, thr, bs));
+          m = a > m ? a : m;
         }
       }
       __syncthreads();
     }
-    if (MODE == 0) return;
     const unsigned long long M = blockMaxAll(m, sh);
     if (threadIdx.x == 0) {
       *D.maxApproachBits = M;
@@ -2268,6 +2338,118 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
       if (threadIdx.x == 0) *stop = 1;
       return;
     }
+  }
+}
+// MODE 0 = warm start (one pass), 1 = PGS, 2 = one-shot GS.  `run` and `need` (optional) gate
+// the whole loop; `stop` / `count` / `bits` are the loop's stop flag, sweep counter and residual.
+template <int MODE>
+__global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
+    const Dev* __restrict__ Dp, Params P, ClassStarts cs, int numColors, int numManifolds, int iters, double thr,
+    double stopThr, int* stop, int* count, const int* run, const int* need) {
+  const Dev& D = *Dp;
+  __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
+  if ((run && !*run) || (need && !*need) || *stop) return;
+  if (MODE == 0) {
+    for (int c = 0; c < numColors; ++c) {
+      const int b = cs.s[c], e = cs.s[c + 1];
+      if (e <= b) continue;
+      if (c == 62) {
+        if (threadIdx.x == 0)
+          for (int q = 0; q < numManifolds; ++q)
+            if (D.mColor[q] == c) warmApplyRow(D, D.vRowOf[q]);
+      } else {
+        for (int k = b + (int)threadIdx.x; k < e; k += blockDim.x) warmApplyRow(D, k);
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  vSweepLoop<MODE>(D, P, cs, numColors, numManifolds, iters, thr, stopThr, stop, count,
+                   GBodies{D.vw}, sh);
+}
+// The same loop with its bodies staged in LDS: up to VL_ROWS rows touching up to VL_BODIES
+// bodies.  A colour pass then reads and writes its bodies' velocities in LDS (the row constants
+// stay in global memory, read-only in the loop) and the velocities go back to global memory
+// once.  Rows over VL_ROWS or bodies over bodyCap (<= VL_BODIES) change nothing and set
+// *fallback = 1 (then k_vsolve_block, gated on it, runs the loop); otherwise *fallback = 0.
+constexpr int VL_ROWS = 4096;
+constexpr int VL_BODIES = 2048;
+static_assert(VL_BODIES <= JPS_HASH * 3 / 4 && VL_BODIES < 0xFFFF, "LDS body slots");
+template <int MODE>
+__global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_lds(
+    const Dev* __restrict__ Dp, Params P, ClassStarts cs, int numColors, int numManifolds,
+    int iters, double thr, double stopThr, int* stop, int* count, const int* run, int* fallback,
+    int bodyCap) {
+  const Dev& D = *Dp;
+  __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
+  __shared__ int hKey[JPS_HASH];
+  __shared__ unsigned short hIdx[JPS_HASH];
+  __shared__ int bodyOf[VL_BODIES];
+  __shared__ double2 bv[3 * VL_BODIES];
+  __shared__ unsigned slots[VL_ROWS];
+  __shared__ int sUsed, sFailHash, sFailDense, sDense;
+  if ((run && !*run) || *stop) return;  // k_vsolve_block returns on the same test
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int r0 = cs.s[0], nRows = cs.s[numColors] - r0;
+  if (nRows > VL_ROWS) {
+    if (tid == 0) *fallback = 1;
+    return;
+  }
+  for (int h = tid; h < JPS_HASH; h += T) hKey[h] = -1;
+  if (tid == 0) {
+    sUsed = 0;
+    sFailHash = 0;
+    sFailDense = 0;
+    sDense = 0;
+  }
+  __syncthreads();
+  for (int q = tid; q < nRows; q += T) {
+    const int2 r = D.vAB[r0 + q];
+    if (jpsSlot(hKey, &sUsed, r.x, true) < 0) sFailHash = 1;
+    if (r.y >= 0 && jpsSlot(hKey, &sUsed, r.y, true) < 0) sFailHash = 1;
+  }
+  __syncthreads();
+  if (!sFailHash) {
+    for (int h = tid; h < JPS_HASH; h += T) {
+      const int body = hKey[h];
+      if (body < 0) continue;
+      const int s = atomicAdd(&sDense, 1);
+      if (s >= bodyCap) {
+        sFailDense = 1;
+        continue;
+      }
+      hIdx[h] = (unsigned short)s;
+      bodyOf[s] = body;
+      const Body64 g = D.vw[body];
+      bv[3 * s] = make_double2(g.v.x, g.v.y);
+      bv[3 * s + 1] = make_double2(g.v.z, g.w.x);
+      bv[3 * s + 2] = make_double2(g.w.y, g.w.z);
+    }
+  }
+  __syncthreads();
+  const bool staged = !sFailHash && !sFailDense;
+  if (tid == 0) *fallback = staged ? 0 : 1;
+  if (!staged) return;
+  for (int q = tid; q < nRows; q += T) {
+    const int2 r = D.vAB[r0 + q];
+    const unsigned sa = hIdx[jpsSlot(hKey, &sUsed, r.x, false)];
+    const unsigned sb = r.y >= 0 ? hIdx[jpsSlot(hKey, &sUsed, r.y, false)] : 0xFFFFu;
+    slots[q] = sa | (sb << 16);
+  }
+  __syncthreads();
+  vSweepLoop<MODE>(D, P, cs, numColors, numManifolds, iters, thr, stopThr, stop, count,
+                   LBodies{bv, slots, bodyOf, r0}, sh);
+  __syncthreads();
+  const int nb = sDense;
+  for (int s = tid; s < nb; s += T) {
+    Body64* g = D.vw + bodyOf[s];
+    const double2 a = bv[3 * s], b = bv[3 * s + 1], c = bv[3 * s + 2];
+    g->v.x = a.x;
+    g->v.y = a.y;
+    g->v.z = b.x;
+    g->w.x = b.y;
+    g->w.y = c.x;
+    g->w.z = c.y;
   }
 }
 // ---- stabilization mode 4 ("ordered"), src/sim.hpp:356-397 ---------------------------------

@@ -142,6 +142,8 @@ struct pd_sim {
   HzDev Z{};
   // sweep loops over at most this many rows run in one workgroup (k_vsolve_block/k_psolve_block)
   int blockRows = BLOCK_SOLVER_ROWS;
+  bool ldsLoop = true;  // one-workgroup velocity loops stage their bodies in LDS when they fit
+  int ldsBodies = VL_BODIES;  // ... up to this many bodies
   // extra slots reserved by setPositions for periodic ghosts (more are added when needed)
   long slotHeadroom = 4096;
   // the largest scale, cached while no scale is written (every writer clears scaleKnown)
@@ -504,30 +506,35 @@ struct pd_sim {
     ground2.ensure(n, st, keep);
   }
 
+  // a table member is a global-address-space pointer in the device pass (same bits)
+  template <class T>
+  static void gset(PD_G T*& dst, T* src) {
+    dst = (PD_G T*)src;
+  }
   void refreshDev() {
     Dev& d = D;
-    d.pos = pos.p; d.vw0 = vw0.p; d.bc = bc.p; d.posPred = posPred.p; d.quatPred = quatPred.p;
-    d.vw = vw.p; d.scale = scale.p; d.targetScale = targetScale.p; d.extForce = extForce.p;
-    d.realIdx = realIdx.p; d.matId = matId.p; d.ground = ground.p; d.ground2 = ground2.p;
-    d.shell = shell.p; d.pairTable = pairTableD.p;
-    d.cellOf = cellOf.p; d.cellRank = cellRank.p; d.cellCnt = cellCnt.p; d.cellStart = cellStart.p; d.order = order.p;
-    d.sbox = sbox.p; d.pairCnt = pairCnt.p; d.pairStart = pairStart.p; d.pairs = pairs.p;
-    d.ncnt = ncnt.p; d.noff = noff.p; d.cAB = cAB.p; d.cN = cN.p; d.cRA = cRA.p; d.cRB = cRB.p;
-    d.cSlot = cSlot.p; d.mCnt = mCnt.p; d.mStart = mStart.p; d.mAB = mAB.p; d.mReal = mReal.p;
-    d.mRange = mRange.p; d.mRange2 = mRange2.p; d.cBV = cBV.p; d.mWV = mWV.p; d.vWV = vWV.p;
-    d.mNs = mNs.p; d.mTA = mTA.p; d.mTB = mTB.p; d.mRA = mRA.p;
-    d.mRB = mRB.p; d.mNp = mNp.p; d.mKey = mKey.p; d.mCode = mCode.p; d.mColor = mColor.p;
-    d.cColor = cColor.p; d.vPerm = vPerm.p;
-    d.vRowOf = vRowOf.p; d.pPerm = pPerm.p; d.pRowOf = pRowOf.p; d.vAB = vAB.p; d.vN = vN.p;
-    d.vTA = vTA.p; d.vTB = vTB.p; d.vRA = vRA.p; d.vRB = vRB.p; d.vVt0 = vVt0.p; d.vPre = vPre.p;
-    d.vLamT = vLamT.p; d.vVn0 = vVn0.p; d.vLam = vLam.p; d.vFlag = vFlag.p; d.pAB = pAB.p;
-    d.pN = pN.p; d.pRA = pRA.p; d.pRB = pRB.p; d.pLam = pLam.p; d.lKey = lKey.p;
-    d.lLam = lLam.p; d.lLamT = lLamT.p; d.lIdx = lIdx.p; d.uKey = uKey.p;
-    d.bucketCnt = bucketCnt.p; d.bucketStart = bucketStart.p; d.red = red.p;
-    d.counters = counters.p;
-    d.maxOverlapBits = bits.p; d.maxApproachBits = bits.p + 1; d.thermo = thermo.p;
-    d.dVel = dVel.p; d.dAng = dAng.p; d.dPos = dPos.p; d.dCnt = dCnt.p;
-    d.cLamN = cLamN.p; d.frCnt = frCnt.p;
+    gset(d.pos, pos.p); gset(d.vw0, vw0.p); gset(d.bc, bc.p); gset(d.posPred, posPred.p); gset(d.quatPred, quatPred.p);
+    gset(d.vw, vw.p); gset(d.scale, scale.p); gset(d.targetScale, targetScale.p); gset(d.extForce, extForce.p);
+    gset(d.realIdx, realIdx.p); gset(d.matId, matId.p); gset(d.ground, ground.p); gset(d.ground2, ground2.p);
+    gset(d.shell, shell.p); gset(d.pairTable, pairTableD.p);
+    gset(d.cellOf, cellOf.p); gset(d.cellRank, cellRank.p); gset(d.cellCnt, cellCnt.p); gset(d.cellStart, cellStart.p); gset(d.order, order.p);
+    gset(d.sbox, sbox.p); gset(d.pairCnt, pairCnt.p); gset(d.pairStart, pairStart.p); gset(d.pairs, pairs.p);
+    gset(d.ncnt, ncnt.p); gset(d.noff, noff.p); gset(d.cAB, cAB.p); gset(d.cN, cN.p); gset(d.cRA, cRA.p); gset(d.cRB, cRB.p);
+    gset(d.cSlot, cSlot.p); gset(d.mCnt, mCnt.p); gset(d.mStart, mStart.p); gset(d.mAB, mAB.p); gset(d.mReal, mReal.p);
+    gset(d.mRange, mRange.p); gset(d.mRange2, mRange2.p); gset(d.cBV, cBV.p); gset(d.mWV, mWV.p); gset(d.vWV, vWV.p);
+    gset(d.mNs, mNs.p); gset(d.mTA, mTA.p); gset(d.mTB, mTB.p); gset(d.mRA, mRA.p);
+    gset(d.mRB, mRB.p); gset(d.mNp, mNp.p); gset(d.mKey, mKey.p); gset(d.mCode, mCode.p); gset(d.mColor, mColor.p);
+    gset(d.cColor, cColor.p); gset(d.vPerm, vPerm.p);
+    gset(d.vRowOf, vRowOf.p); gset(d.pPerm, pPerm.p); gset(d.pRowOf, pRowOf.p); gset(d.vAB, vAB.p); gset(d.vN, vN.p);
+    gset(d.vTA, vTA.p); gset(d.vTB, vTB.p); gset(d.vRA, vRA.p); gset(d.vRB, vRB.p); gset(d.vVt0, vVt0.p); gset(d.vPre, vPre.p);
+    gset(d.vLamT, vLamT.p); gset(d.vVn0, vVn0.p); gset(d.vLam, vLam.p); gset(d.vFlag, vFlag.p); gset(d.pAB, pAB.p);
+    gset(d.pN, pN.p); gset(d.pRA, pRA.p); gset(d.pRB, pRB.p); gset(d.pLam, pLam.p); gset(d.lKey, lKey.p);
+    gset(d.lLam, lLam.p); gset(d.lLamT, lLamT.p); gset(d.lIdx, lIdx.p); gset(d.uKey, uKey.p);
+    gset(d.bucketCnt, bucketCnt.p); gset(d.bucketStart, bucketStart.p); gset(d.red, red.p);
+    gset(d.counters, counters.p);
+    gset(d.maxOverlapBits, bits.p); gset(d.maxApproachBits, bits.p + 1); gset(d.thermo, thermo.p);
+    gset(d.dVel, dVel.p); gset(d.dAng, dAng.p); gset(d.dPos, dPos.p); gset(d.dCnt, dCnt.p);
+    gset(d.cLamN, cLamN.p); gset(d.frCnt, frCnt.p);
     uploadDev();
   }
   // The kernels read the pointer set from device memory (a few bytes of launch arguments instead
@@ -1236,6 +1243,35 @@ struct pd_sim {
     HIPCHECK(hipGetLastError());
   }
 
+  // One-workgroup velocity sweep loop (mode 1 PGS, 2 one-shot GS, or 0 the warm-start pass)
+  // over the colour classes vs.  With the bodies staged in LDS when the rows allow it; up to
+  // ldsBodies / 2 rows they always fit, above that the global-memory loop follows, gated on the
+  // staged kernel's fallback flag.
+  void vLoop(int mode, const Params& P, const std::vector<int>& vs, int numColors, int nRows,
+             int iters, double thr, double stopThr, int* stop, int* count, const int* run) {
+    const ClassStarts cs = classStarts(vs);
+    const bool lds = mode != 0 && ldsLoop && nRows <= VL_ROWS;
+    int* fb = ctl.p + CTL_VFALLBACK;
+    if (lds) {
+      if (mode == 1)
+        k_vsolve_lds<1><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
+                                                       stopThr, stop, count, run, fb, ldsBodies);
+      else
+        k_vsolve_lds<2><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
+                                                       stopThr, stop, count, run, fb, ldsBodies);
+      if (2 * (long)nRows <= ldsBodies) return;
+    }
+    const int* need = lds ? fb : nullptr;
+    if (mode == 0)
+      k_vsolve_block<0><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
+                                                       stopThr, stop, count, run, need);
+    else if (mode == 1)
+      k_vsolve_block<1><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
+                                                       stopThr, stop, count, run, need);
+    else
+      k_vsolve_block<2><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
+                                                       stopThr, stop, count, run, need);
+  }
   long vBlockRec[2] = {-1, -1}, pBlockRec = -1;
   int posSweepsPrev = 1;  // sweeps of the last batched position loop (first batch size)
   static constexpr int POS_TAIL_ROWS = 8192;
@@ -1442,9 +1478,8 @@ struct pd_sim {
       }
       // 8 swaps: the result is back in `ground`
       if (nRowsV <= blockRows)
-        k_vsolve_block<0><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, classStarts(vs), numColors, nm,
-                                                             1, 0.0, 0.0, ctl.p + CTL_NONE,
-                                                             ctl.p + CTL_NONE, nullptr);
+        vLoop(0, P, vs, numColors, nRowsV, 1, 0.0, 0.0, ctl.p + CTL_NONE, ctl.p + CTL_NONE,
+              nullptr);
       else
         velocitySweep(vs, numColors, 0, 0.0, ctl.p + CTL_NONE);
     }
@@ -1460,14 +1495,8 @@ struct pd_sim {
       const bool smallV = nRowsV <= blockRows;
       if (smallV && velIters > 0) {
         kernBegin();
-        if (usePGS)
-          k_vsolve_block<1><<<1, VBLOCK_THREADS, 0, st>>>(
-              dD, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
-              ctl.p + CTL_SWEEPS_V, nullptr);
-        else
-          k_vsolve_block<2><<<1, VBLOCK_THREADS, 0, st>>>(
-              dD, P, classStarts(vs), numColors, nm, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
-              ctl.p + CTL_SWEEPS_V, nullptr);
+        vLoop(usePGS ? 1 : 2, P, vs, numColors, nRowsV, velIters, vRest, stopThr,
+              ctl.p + CTL_STOP_V, ctl.p + CTL_SWEEPS_V, nullptr);
         vBlockRec[0] = kernEnd(K_VBLOCK, nRowsV);
       } else if (!smallV) sweepLoop(velIters, ctl.p + CTL_STOP_V, [&] {
         k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V);
@@ -1485,9 +1514,8 @@ struct pd_sim {
               k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, 8.0 * dt * gMagP,
                                                          8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
             kernBegin();
-            k_vsolve_block<1><<<1, VBLOCK_THREADS, 0, st>>>(
-                dD, P, classStarts(vs), numColors, nm, 2 * velIters, vRestS, vRestS,
-                ctl.p + CTL_STOP_S, ctl.p + CTL_SWEEPS_S, ctl.p + CTL_STAB_RUN);
+            vLoop(1, P, vs, numColors, nRowsV, 2 * velIters, vRestS, vRestS, ctl.p + CTL_STOP_S,
+                  ctl.p + CTL_SWEEPS_S, ctl.p + CTL_STAB_RUN);
             vBlockRec[1] = kernEnd(K_VBLOCK, nRowsV);
           }
         } else if (stabMode == 1 && velIters > 0 && read1(ctl.p + CTL_STAB_RUN)) {
@@ -2765,6 +2793,13 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     if (k == "block_solver_rows") {
       if (value < 0) throw ArgError("block_solver_rows must be >= 0");
       s->blockRows = (int)std::min<long>(value, 1L << 30);
+    } else if (k == "velocity_loop_lds") {
+      if (value != 0 && value != 1) throw ArgError("velocity_loop_lds must be 0 or 1");
+      s->ldsLoop = value != 0;
+    } else if (k == "velocity_loop_lds_bodies") {
+      if (value < 1 || value > VL_BODIES)
+        throw ArgError("velocity_loop_lds_bodies must be in [1, " + std::to_string(VL_BODIES) + "]");
+      s->ldsBodies = (int)value;
     } else if (k == "candidate_skin_permille") {
       if (value < 0) throw ArgError("candidate_skin_permille must be >= 0");
       s->candSkinFrac = std::min<long>(value, 100000) / 1000.0;

@@ -76,12 +76,15 @@ def assert_ledger(h, r, tag):
     assert rel_err(hp[head], rp[head]) <= RTOL, f"{tag}: ledger position impulse"
 
 
-def run_pair(kind, n, steps, seed=0, check_topology_every=1, block_rows=None, **over):
+def run_pair(kind, n, steps, seed=0, check_topology_every=1, block_rows=None, tuning=None,
+             **over):
     scene = make_scene(kind, n, seed)
     scene.update(over)
     h = apply_scene(hip_sim(n), scene)
     if block_rows is not None:
         h.set_tuning("block_solver_rows", block_rows)
+    for key, value in (tuning or {}).items():
+        h.set_tuning(key, value)
     r = apply_scene(oracle_sim(n), scene)
     for k in range(steps):
         tag = f"{kind} n={n} seed={seed} substep {k}"
@@ -114,6 +117,17 @@ def test_bed_multistep(hip_lib, block_rows):
 def test_bed_periodic_multistep(hip_lib, block_rows):
     h, r = run_pair("bed_periodic", 400, 25, block_rows=block_rows)
     assert h.get_stats()["num_particles"] > h.num_particles()  # ghosts were generated
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["bed", "bed_periodic", "gas"])
+@pytest.mark.parametrize("tuning", [{"velocity_loop_lds": 0}, {"velocity_loop_lds_bodies": 24}],
+                         ids=["global", "lds-fallback"])
+def test_velocity_loop_body_storage(hip_lib, kind, tuning):
+    # the one-workgroup velocity loops default to bodies staged in LDS; here they keep them in
+    # global memory, or stage only up to 24 bodies, so most substeps take the staged kernel's
+    # fallback to the global-memory loop: every variant is the oracle's result
+    run_pair(kind, 300, 15, tuning=tuning)
 
 
 @pytest.mark.gpu
