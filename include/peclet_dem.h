@@ -202,9 +202,11 @@ int pd_set_timing(pd_sim* s, int level);
  * periodic ghosts (default 4096; more are allocated when a substep needs them);
  * "timing_every" = N records timing events only on substeps 0, N, 2N, ...
  * after pd_set_timing (default 1), which bounds the events' host cost; "velocity_loop_lds" = 1
- * (default) lets a one-workgroup velocity loop keep its bodies' velocities in LDS when its rows
- * touch at most "velocity_loop_lds_bodies" bodies (default and maximum 2048), 0 keeps them in
- * global memory.  Results do not depend on any of them. */
+ * (default) lets a one-workgroup velocity loop of at most 8192 rows keep the velocities of up
+ * to "velocity_loop_lds_bodies" of its bodies in LDS (default and maximum 2400; the others stay
+ * in global memory) when its rows touch at most "velocity_loop_lds_keys" distinct bodies
+ * (default and maximum 12288), 0 keeps them all in global memory.  Results do not depend on any
+ * of them. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 int pd_get_stage_times(pd_sim* s, double* out_ms, int cap);
 /* host wall time between the same stage marks (enqueueing plus waiting), summed like the stage

@@ -1903,7 +1903,6 @@ __device__ void warmApplyRow(const Dev& D, int k) {
 // the rows' bodies staged in LDS by a one-workgroup sweep loop (LBodies: 6 doubles per body,
 // rows hold the two LDS slots).  The arithmetic is the same either way.
 struct GBodies {
-  static constexpr bool kEarly = false;  // row state read where it is used
   Body64* vw;
   __device__ void row(const Dev& D, int k, int2& r, int2& sl) const {
     r = D.vAB[k];
@@ -1912,27 +1911,38 @@ struct GBodies {
   __device__ Body64 load(int real, int) const { return vw[real]; }
   __device__ void store(int real, int, const Body64& b) const { vw[real] = b; }
 };
+// LDS slot codes of a row's bodies: a body without an LDS slot stays in global memory
+constexpr int VL_GLOBAL = 0xFFFE, VL_NONE = 0xFFFF;
 struct LBodies {
-  static constexpr bool kEarly = true;  // row state read up front (more registers)
   double2* bv;             // body s: bv[3s] = (v.x, v.y), bv[3s+1] = (v.z, w.x), bv[3s+2] = (w.y, w.z)
-  const unsigned* slots;   // row k - r0: slot of a | slot of b << 16 (0xFFFF: no body b)
+  const unsigned* slots;   // row k - r0: slot of a | slot of b << 16
   const int* bodyOf;       // slot -> real body
+  PD_G Body64* vw;         // bodies without a slot
   int r0;
-  __device__ void row(const Dev&, int k, int2& r, int2& sl) const {
+  __device__ void row(const Dev& D, int k, int2& r, int2& sl) const {
     const unsigned p = slots[k - r0];
     sl.x = (int)(p & 0xFFFFu);
     sl.y = (int)(p >> 16);
-    r.x = bodyOf[sl.x];
-    r.y = sl.y != 0xFFFF ? bodyOf[sl.y] : -1;
+    if (sl.x == VL_GLOBAL || sl.y == VL_GLOBAL) {
+      r = D.vAB[k];
+    } else {
+      r.x = bodyOf[sl.x];
+      r.y = sl.y != VL_NONE ? bodyOf[sl.y] : -1;
+    }
   }
-  __device__ Body64 load(int, int s) const {
+  __device__ Body64 load(int real, int s) const {
+    if (s == VL_GLOBAL) return vw[real];
     const double2 a = bv[3 * s], b = bv[3 * s + 1], c = bv[3 * s + 2];
     Body64 o;
     o.v = make_double4(a.x, a.y, b.x, 0.0);
     o.w = make_double4(b.y, c.x, c.y, 0.0);
     return o;
   }
-  __device__ void store(int, int s, const Body64& o) const {
+  __device__ void store(int real, int s, const Body64& o) const {
+    if (s == VL_GLOBAL) {
+      vw[real] = o;
+      return;
+    }
     bv[3 * s] = make_double2(o.v.x, o.v.y);
     bv[3 * s + 1] = make_double2(o.v.z, o.w.x);
     bv[3 * s + 2] = make_double2(o.w.y, o.w.z);
@@ -1958,13 +1968,6 @@ __device__ double pgsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
   d3 vB{0, 0, 0}, wB{0, 0, 0};
   const double4 tb4 = D.vTB[k];
   const double invN = tb4.w;
-  // with kEarly the row's friction state is read up front: it does not depend on the normal
-  // update, so its load latency overlaps the body loads
-  double4 lto4, vt04;
-  if constexpr (BS::kEarly) {
-    lto4 = D.vLamT[k];
-    vt04 = D.vVt0[k];
-  }
   if (realB >= 0) {
     cb = D.bc[realB];
     invMassB = cb.iim.w;
@@ -2043,7 +2046,7 @@ __device__ double pgsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
   }
   // friction cone, src/solver_velocity.hpp:760-855
   double mu = ta4.w;
-  if constexpr (!BS::kEarly) lto4 = D.vLamT[k];
+  const double4 lto4 = D.vLamT[k];
   const d3 ltOld = xyz(lto4);
   const bool haveOld = nonzero3(ltOld);
   if (mu > 0.0 || haveOld) {
@@ -2061,7 +2064,7 @@ __device__ double pgsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
     double beta = P.restT;
     if (beta != 0.0 && (sf != 0 || fabs(vn0) < restVelThreshold * lenN)) beta = 0.0;
     d3 vtErr = vt;
-    if constexpr (!BS::kEarly) vt04 = D.vVt0[k];
+    const double4 vt04 = D.vVt0[k];
     if (beta != 0.0) {
       d3 v0 = xyz(vt04);
       v0 = sub3(v0, scale3(nhat, dot3(v0, nhat)));
@@ -2367,27 +2370,51 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
   vSweepLoop<MODE>(D, P, cs, numColors, numManifolds, iters, thr, stopThr, stop, count,
                    GBodies{D.vw}, sh);
 }
-// The same loop with its bodies staged in LDS: up to VL_ROWS rows touching up to VL_BODIES
-// bodies.  A colour pass then reads and writes its bodies' velocities in LDS (the row constants
-// stay in global memory, read-only in the loop) and the velocities go back to global memory
-// once.  Rows over VL_ROWS or bodies over bodyCap (<= VL_BODIES) change nothing and set
-// *fallback = 1 (then k_vsolve_block, gated on it, runs the loop); otherwise *fallback = 0.
-constexpr int VL_ROWS = 4096;
-constexpr int VL_BODIES = 2048;
-static_assert(VL_BODIES <= JPS_HASH * 3 / 4 && VL_BODIES < 0xFFFF, "LDS body slots");
+// The same loop with its bodies staged in LDS: a colour pass reads and writes the velocities
+// of up to bodyCap (<= VL_BODIES) of the rows' bodies in LDS, the rest in global memory (the
+// row constants stay in global memory, read-only in the loop), and the staged velocities go back
+// to global memory once.  Staging hashes the rows' distinct bodies into LDS (the hash shares the
+// velocity area, which is filled after it); more than VL_ROWS rows or keyCap distinct bodies
+// change nothing and set *fallback = 1 (then k_vsolve_block, gated on it, runs the loop);
+// otherwise *fallback = 0.
+constexpr int VL_ROWS = 8192;
+constexpr int VL_BODIES = 2400;
+constexpr int VL_HASH = 16384;
+static_assert(VL_HASH * (sizeof(int) + sizeof(unsigned short)) <= 3 * VL_BODIES * sizeof(double2),
+              "the staging hash fits in the velocity area");
+static_assert(VL_BODIES < VL_GLOBAL, "LDS body slots");
+constexpr int VL_KEYS = VL_HASH * 3 / 4;  // distinct bodies the hash takes at most
+// open-addressing slot of `body` in an LDS hash of VL_HASH keys (-1: full or, on lookup, absent)
+__device__ __forceinline__ int vlSlot(int* hKey, int* used, int body, bool insert, int keyCap) {
+  unsigned h = ((unsigned)body * 2654435761u) & (VL_HASH - 1);
+  for (int probe = 0; probe < VL_HASH; ++probe) {
+    const int k = hKey[h];
+    if (k == body) return (int)h;
+    if (k == -1) {
+      if (!insert) return -1;
+      if (atomicAdd(used, 1) >= keyCap) return -1;
+      const int old = atomicCAS(&hKey[h], -1, body);
+      if (old == -1 || old == body) return (int)h;
+      atomicSub(used, 1);
+      continue;  // lost the slot to another body: look at it again
+    }
+    h = (h + 1) & (VL_HASH - 1);
+  }
+  return -1;
+}
 template <int MODE>
 __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_lds(
     const Dev* __restrict__ Dp, Params P, ClassStarts cs, int numColors, int numManifolds,
     int iters, double thr, double stopThr, int* stop, int* count, const int* run, int* fallback,
-    int bodyCap) {
+    int bodyCap, int keyCap) {
   const Dev& D = *Dp;
   __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
-  __shared__ int hKey[JPS_HASH];
-  __shared__ unsigned short hIdx[JPS_HASH];
-  __shared__ int bodyOf[VL_BODIES];
   __shared__ double2 bv[3 * VL_BODIES];
+  __shared__ int bodyOf[VL_BODIES];
   __shared__ unsigned slots[VL_ROWS];
-  __shared__ int sUsed, sFailHash, sFailDense, sDense;
+  __shared__ int sUsed, sFail, sDense;
+  int* hKey = reinterpret_cast<int*>(bv);
+  unsigned short* hIdx = reinterpret_cast<unsigned short*>(hKey + VL_HASH);
   if ((run && !*run) || *stop) return;  // k_vsolve_block returns on the same test
   const int T = blockDim.x, tid = threadIdx.x;
   const int r0 = cs.s[0], nRows = cs.s[numColors] - r0;
@@ -2395,54 +2422,55 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_lds(
     if (tid == 0) *fallback = 1;
     return;
   }
-  for (int h = tid; h < JPS_HASH; h += T) hKey[h] = -1;
+  for (int h = tid; h < VL_HASH; h += T) hKey[h] = -1;
   if (tid == 0) {
     sUsed = 0;
-    sFailHash = 0;
-    sFailDense = 0;
+    sFail = 0;
     sDense = 0;
   }
   __syncthreads();
   for (int q = tid; q < nRows; q += T) {
     const int2 r = D.vAB[r0 + q];
-    if (jpsSlot(hKey, &sUsed, r.x, true) < 0) sFailHash = 1;
-    if (r.y >= 0 && jpsSlot(hKey, &sUsed, r.y, true) < 0) sFailHash = 1;
+    if (vlSlot(hKey, &sUsed, r.x, true, keyCap) < 0) sFail = 1;
+    if (r.y >= 0 && vlSlot(hKey, &sUsed, r.y, true, keyCap) < 0) sFail = 1;
   }
   __syncthreads();
-  if (!sFailHash) {
-    for (int h = tid; h < JPS_HASH; h += T) {
-      const int body = hKey[h];
-      if (body < 0) continue;
-      const int s = atomicAdd(&sDense, 1);
-      if (s >= bodyCap) {
-        sFailDense = 1;
-        continue;
-      }
+  const bool staged = !sFail;
+  if (tid == 0) *fallback = staged ? 0 : 1;
+  if (!staged) return;
+  // dense LDS slots for the first bodyCap bodies met in hash order, the rest stay global
+  for (int h = tid; h < VL_HASH; h += T) {
+    const int body = hKey[h];
+    if (body < 0) continue;
+    const int s = atomicAdd(&sDense, 1);
+    if (s < bodyCap) {
       hIdx[h] = (unsigned short)s;
       bodyOf[s] = body;
-      const Body64 g = D.vw[body];
-      bv[3 * s] = make_double2(g.v.x, g.v.y);
-      bv[3 * s + 1] = make_double2(g.v.z, g.w.x);
-      bv[3 * s + 2] = make_double2(g.w.y, g.w.z);
+    } else {
+      hIdx[h] = (unsigned short)VL_GLOBAL;
     }
   }
   __syncthreads();
-  const bool staged = !sFailHash && !sFailDense;
-  if (tid == 0) *fallback = staged ? 0 : 1;
-  if (!staged) return;
   for (int q = tid; q < nRows; q += T) {
     const int2 r = D.vAB[r0 + q];
-    const unsigned sa = hIdx[jpsSlot(hKey, &sUsed, r.x, false)];
-    const unsigned sb = r.y >= 0 ? hIdx[jpsSlot(hKey, &sUsed, r.y, false)] : 0xFFFFu;
+    const unsigned sa = hIdx[vlSlot(hKey, &sUsed, r.x, false, keyCap)];
+    const unsigned sb = r.y >= 0 ? hIdx[vlSlot(hKey, &sUsed, r.y, false, keyCap)] : VL_NONE;
     slots[q] = sa | (sb << 16);
+  }
+  __syncthreads();  // the hash is dead: its area now takes the velocities
+  const int nb = sDense < bodyCap ? sDense : bodyCap;
+  for (int s = tid; s < nb; s += T) {
+    const Body64 g = D.vw[bodyOf[s]];
+    bv[3 * s] = make_double2(g.v.x, g.v.y);
+    bv[3 * s + 1] = make_double2(g.v.z, g.w.x);
+    bv[3 * s + 2] = make_double2(g.w.y, g.w.z);
   }
   __syncthreads();
   vSweepLoop<MODE>(D, P, cs, numColors, numManifolds, iters, thr, stopThr, stop, count,
-                   LBodies{bv, slots, bodyOf, r0}, sh);
+                   LBodies{bv, slots, bodyOf, D.vw, r0}, sh);
   __syncthreads();
-  const int nb = sDense;
   for (int s = tid; s < nb; s += T) {
-    Body64* g = D.vw + bodyOf[s];
+    PD_G Body64* g = D.vw + bodyOf[s];
     const double2 a = bv[3 * s], b = bv[3 * s + 1], c = bv[3 * s + 2];
     g->v.x = a.x;
     g->v.y = a.y;

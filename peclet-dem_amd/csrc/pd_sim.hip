@@ -143,7 +143,8 @@ struct pd_sim {
   // sweep loops over at most this many rows run in one workgroup (k_vsolve_block/k_psolve_block)
   int blockRows = BLOCK_SOLVER_ROWS;
   bool ldsLoop = true;  // one-workgroup velocity loops stage their bodies in LDS when they fit
-  int ldsBodies = VL_BODIES;  // ... up to this many bodies
+  int ldsBodies = VL_BODIES;  // ... up to this many bodies (the rest stay in global memory)
+  int ldsKeys = VL_KEYS;      // more distinct bodies than this: the global-memory loop
   // extra slots reserved by setPositions for periodic ghosts (more are added when needed)
   long slotHeadroom = 4096;
   // the largest scale, cached while no scale is written (every writer clears scaleKnown)
@@ -1245,7 +1246,7 @@ struct pd_sim {
 
   // One-workgroup velocity sweep loop (mode 1 PGS, 2 one-shot GS, or 0 the warm-start pass)
   // over the colour classes vs.  With the bodies staged in LDS when the rows allow it; up to
-  // ldsBodies / 2 rows they always fit, above that the global-memory loop follows, gated on the
+  // ldsKeys / 2 rows they always do, above that the global-memory loop follows, gated on the
   // staged kernel's fallback flag.
   void vLoop(int mode, const Params& P, const std::vector<int>& vs, int numColors, int nRows,
              int iters, double thr, double stopThr, int* stop, int* count, const int* run) {
@@ -1255,11 +1256,13 @@ struct pd_sim {
     if (lds) {
       if (mode == 1)
         k_vsolve_lds<1><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
-                                                       stopThr, stop, count, run, fb, ldsBodies);
+                                                       stopThr, stop, count, run, fb, ldsBodies,
+                                                       ldsKeys);
       else
         k_vsolve_lds<2><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
-                                                       stopThr, stop, count, run, fb, ldsBodies);
-      if (2 * (long)nRows <= ldsBodies) return;
+                                                       stopThr, stop, count, run, fb, ldsBodies,
+                                                       ldsKeys);
+      if (2 * (long)nRows <= ldsKeys) return;
     }
     const int* need = lds ? fb : nullptr;
     if (mode == 0)
@@ -2800,6 +2803,10 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
       if (value < 1 || value > VL_BODIES)
         throw ArgError("velocity_loop_lds_bodies must be in [1, " + std::to_string(VL_BODIES) + "]");
       s->ldsBodies = (int)value;
+    } else if (k == "velocity_loop_lds_keys") {
+      if (value < 1 || value > VL_KEYS)
+        throw ArgError("velocity_loop_lds_keys must be in [1, " + std::to_string(VL_KEYS) + "]");
+      s->ldsKeys = (int)value;
     } else if (k == "candidate_skin_permille") {
       if (value < 0) throw ArgError("candidate_skin_permille must be >= 0");
       s->candSkinFrac = std::min<long>(value, 100000) / 1000.0;

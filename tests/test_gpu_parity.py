@@ -121,12 +121,14 @@ def test_bed_periodic_multistep(hip_lib, block_rows):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["bed", "bed_periodic", "gas"])
-@pytest.mark.parametrize("tuning", [{"velocity_loop_lds": 0}, {"velocity_loop_lds_bodies": 24}],
-                         ids=["global", "lds-fallback"])
+@pytest.mark.parametrize("tuning", [{"velocity_loop_lds": 0}, {"velocity_loop_lds_bodies": 24},
+                                    {"velocity_loop_lds_keys": 16}],
+                         ids=["global", "lds-and-global", "lds-fallback"])
 def test_velocity_loop_body_storage(hip_lib, kind, tuning):
     # the one-workgroup velocity loops default to bodies staged in LDS; here they keep them in
-    # global memory, or stage only up to 24 bodies, so most substeps take the staged kernel's
-    # fallback to the global-memory loop: every variant is the oracle's result
+    # global memory, or stage only 24 bodies and leave the rest in global memory, or give up
+    # staging above 16 distinct bodies (the gated global-memory loop then runs): every variant
+    # is the oracle's result
     run_pair(kind, 300, 15, tuning=tuning)
 
 
