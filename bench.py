@@ -24,7 +24,7 @@ The JSON line carries:
                                       posPred, quatPred, real index out)
                   k_reintegrate       344 per slot (vw0, vw, bc, pos in; vw0, posPred.xyz,
                                       quatPred out)
-                  k_vsolve_block      601 per manifold row per sweep (one-workgroup loop)
+                  k_vsolve_lds        601 per manifold row per sweep (one-workgroup loop)
                   k_psolve_block      388 per contact row per sweep
                 The timed loop is only sim.step(dt): the library records its events on every
                 8th substep (--timing-every; the per-event host cost would otherwise slow the
@@ -71,7 +71,7 @@ def substep_bytes(st):
 def kernel_bytes(name, units, log):
     """Algorithmic bytes of all timed launches of one kernel: units = the library's summed units
     for it (rows, row-sweeps, slots or pairs), log = the per-substep counters of the same run."""
-    if name in ("k_vsweep", "k_vsolve_block"):  # manifold rows (x sweeps for the block loop)
+    if name in ("k_vsweep", "k_vsolve_lds"):  # manifold rows (x sweeps for the block loop)
         return 601 * units
     if name in ("k_psweep", "k_psolve_block"):  # contact rows (x sweeps)
         return 388 * units

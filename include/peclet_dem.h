@@ -215,8 +215,9 @@ int pd_get_stage_host_times(pd_sim* s, double* out_ms, int cap);
 const char* pd_stage_name(int stage);
 /* per timed kernel k: summed ms, launch count and units processed (k_vsweep: manifold rows,
  * k_psweep: contact rows, k_bp_query: slots queried, k_np_pairs: pairs, k_predict: particles,
- * k_reintegrate: slots, k_vsolve_block: manifold rows x sweeps, k_psolve_block: contact rows x
- * sweeps); returns the number of timed kernels */
+ * k_reintegrate: slots, k_vsolve_lds: manifold rows x sweeps of the one-workgroup velocity
+ * loops (staged in LDS, with the global-memory k_vsolve_block when they fall back),
+ * k_psolve_block: contact rows x sweeps); returns the number of timed kernels */
 int pd_get_kernel_times(pd_sim* s, double* ms, long* launches, long* rows, int cap);
 /* the counters of every substep since pd_set_timing(s, 2), oldest first: copies at most cap
  * records into out (out may be NULL) and returns how many are logged */

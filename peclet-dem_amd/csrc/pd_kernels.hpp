@@ -1824,81 +1824,6 @@ __device__ __forceinline__ void addVW(Body64& b, d3 dv, d3 dw) {
   b.w.z += dw.z;
 }
 
-// warmStartApplyKokkos, src/solver_velocity.hpp:498-605 (one row)
-__device__ void warmApplyRow(const Dev& D, int k) {
-  const double p = D.vLam[k];
-  const double4 lt4 = D.vLamT[k];
-  d3 lt = xyz(lt4);
-  const bool hasT = nonzero3(lt);
-  if (p == 0.0 && !hasT) return;
-  const int2 r = D.vAB[k];
-  const int realA = r.x, realB = r.y;
-  const d3 Nsum = xyz(D.vN[k]), TauA = xyz(D.vTA[k]), TauB = xyz(D.vTB[k]);
-  const d3 rAavg = xyz(D.vRA[k]);
-  const double4 rb4 = D.vRB[k];
-  const d3 rBavg = xyz(rb4);
-  const double sgn = rb4.w;
-  const double lambda = -sgn * p;
-  const d3 Jlin = scale3(Nsum, lambda);
-  const BodyC ca = D.bc[realA];
-  const double invMassA = ca.iim.w;
-  Body64 ba = D.vw[realA];
-  const q4 qA = toq(ca.q);
-  const d3 invIA = xyz(ca.iim);
-  ba.v.x += Jlin.x * invMassA;
-  ba.v.y += Jlin.y * invMassA;
-  ba.v.z += Jlin.z * invMassA;
-  {
-    const d3 dw = angDelta(qA, scale3(TauA, lambda), invIA);
-    ba.w.x += dw.x;
-    ba.w.y += dw.y;
-    ba.w.z += dw.z;
-  }
-  Body64 bb;
-  BodyC cb;
-  double invMassB = 0.0;
-  if (realB >= 0) {
-    cb = D.bc[realB];
-    invMassB = cb.iim.w;
-    bb = D.vw[realB];
-    bb.v.x += -Jlin.x * invMassB;
-    bb.v.y += -Jlin.y * invMassB;
-    bb.v.z += -Jlin.z * invMassB;
-    const d3 dw = angDelta(toq(cb.q), scale3(TauB, lambda), xyz(cb.iim));
-    bb.w.x += dw.x;
-    bb.w.y += dw.y;
-    bb.w.z += dw.z;
-  }
-  if (hasT) {
-    const double lenN2 = sqrt(dot3(Nsum, Nsum));
-    if (lenN2 > 1e-9) {
-      const d3 nhat = scale3(Nsum, 1.0 / lenN2);
-      lt = sub3(lt, scale3(nhat, dot3(lt, nhat)));
-    }
-    D.vLamT[k] = tod4(lt, 0.0);
-    ba.v.x += lt.x * invMassA;
-    ba.v.y += lt.y * invMassA;
-    ba.v.z += lt.z * invMassA;
-    {
-      const d3 dw = angDelta(qA, cross3(rAavg, lt), invIA);
-      ba.w.x += dw.x;
-      ba.w.y += dw.y;
-      ba.w.z += dw.z;
-    }
-    if (realB >= 0) {
-      bb.v.x += -lt.x * invMassB;
-      bb.v.y += -lt.y * invMassB;
-      bb.v.z += -lt.z * invMassB;
-      const d3 dw = angDelta(toq(cb.q), cross3(rBavg, scale3(lt, -1.0)), xyz(cb.iim));
-      bb.w.x += dw.x;
-      bb.w.y += dw.y;
-      bb.w.z += dw.z;
-    }
-  }
-  D.vw[realA] = ba;
-  if (realB >= 0) D.vw[realB] = bb;
-}
-
 // Where a velocity row finds its bodies' velocities: the global array (GBodies), or a copy of
 // the rows' bodies staged in LDS by a one-workgroup sweep loop (LBodies: 6 doubles per body,
 // rows hold the two LDS slots).  The arithmetic is the same either way.
@@ -1948,6 +1873,87 @@ struct LBodies {
     bv[3 * s + 2] = make_double2(o.w.y, o.w.z);
   }
 };
+
+// warmStartApplyKokkos, src/solver_velocity.hpp:498-605 (one row)
+template <class BS>
+__device__ void warmApplyRowB(const Dev& D, int k, int2 r, int2 sl, const BS& bs) {
+  const double p = D.vLam[k];
+  const double4 lt4 = D.vLamT[k];
+  d3 lt = xyz(lt4);
+  const bool hasT = nonzero3(lt);
+  if (p == 0.0 && !hasT) return;
+  const int realA = r.x, realB = r.y;
+  const d3 Nsum = xyz(D.vN[k]), TauA = xyz(D.vTA[k]), TauB = xyz(D.vTB[k]);
+  const d3 rAavg = xyz(D.vRA[k]);
+  const double4 rb4 = D.vRB[k];
+  const d3 rBavg = xyz(rb4);
+  const double sgn = rb4.w;
+  const double lambda = -sgn * p;
+  const d3 Jlin = scale3(Nsum, lambda);
+  const BodyC ca = D.bc[realA];
+  const double invMassA = ca.iim.w;
+  Body64 ba = bs.load(realA, sl.x);
+  const q4 qA = toq(ca.q);
+  const d3 invIA = xyz(ca.iim);
+  ba.v.x += Jlin.x * invMassA;
+  ba.v.y += Jlin.y * invMassA;
+  ba.v.z += Jlin.z * invMassA;
+  {
+    const d3 dw = angDelta(qA, scale3(TauA, lambda), invIA);
+    ba.w.x += dw.x;
+    ba.w.y += dw.y;
+    ba.w.z += dw.z;
+  }
+  Body64 bb;
+  BodyC cb;
+  double invMassB = 0.0;
+  if (realB >= 0) {
+    cb = D.bc[realB];
+    invMassB = cb.iim.w;
+    bb = bs.load(realB, sl.y);
+    bb.v.x += -Jlin.x * invMassB;
+    bb.v.y += -Jlin.y * invMassB;
+    bb.v.z += -Jlin.z * invMassB;
+    const d3 dw = angDelta(toq(cb.q), scale3(TauB, lambda), xyz(cb.iim));
+    bb.w.x += dw.x;
+    bb.w.y += dw.y;
+    bb.w.z += dw.z;
+  }
+  if (hasT) {
+    const double lenN2 = sqrt(dot3(Nsum, Nsum));
+    if (lenN2 > 1e-9) {
+      const d3 nhat = scale3(Nsum, 1.0 / lenN2);
+      lt = sub3(lt, scale3(nhat, dot3(lt, nhat)));
+    }
+    D.vLamT[k] = tod4(lt, 0.0);
+    ba.v.x += lt.x * invMassA;
+    ba.v.y += lt.y * invMassA;
+    ba.v.z += lt.z * invMassA;
+    {
+      const d3 dw = angDelta(qA, cross3(rAavg, lt), invIA);
+      ba.w.x += dw.x;
+      ba.w.y += dw.y;
+      ba.w.z += dw.z;
+    }
+    if (realB >= 0) {
+      bb.v.x += -lt.x * invMassB;
+      bb.v.y += -lt.y * invMassB;
+      bb.v.z += -lt.z * invMassB;
+      const d3 dw = angDelta(toq(cb.q), cross3(rBavg, scale3(lt, -1.0)), xyz(cb.iim));
+      bb.w.x += dw.x;
+      bb.w.y += dw.y;
+      bb.w.z += dw.z;
+    }
+  }
+  bs.store(realA, sl.x, ba);
+  if (realB >= 0) bs.store(realB, sl.y, bb);
+}
+__device__ void warmApplyRow(const Dev& D, int k) {
+  const GBodies g{D.vw};
+  int2 r, sl;
+  g.row(D, k, r, sl);
+  warmApplyRowB(D, k, r, sl, g);
+}
 
 // PGSManifoldSweep::solveOne, src/solver_velocity.hpp:641-858 (one row)
 template <class BS>
@@ -2208,20 +2214,28 @@ __device__ double gsRow(const Dev& D, const Params& P, int k, double restVelThre
 }
 
 // computeSideFlagsKokkos, src/solver_velocity.hpp:456-495 (rows are active, non-duplicate)
-__global__ void k_side_flags(const Dev* __restrict__ Dp, int numRows, d3 gHat, double riseThr, double approachThr,
-                             const int* run) {
-  const Dev& D = *Dp;
-  const int k = gtid();
-  if (k >= numRows || !*run) return;
+template <class BS>
+__device__ __forceinline__ void sideFlagsRow(const Dev& D, int k, d3 gHat, double riseThr,
+                                             double approachThr, const BS& bs) {
   unsigned char fl = D.vFlag[k] & (unsigned char)~(F_SIDE1 | F_SIDE2);
-  const int2 r = D.vAB[k];
+  int2 r, sl;
+  bs.row(D, k, r, sl);
   if (r.y >= 0 && (fl & F_PERSIST) && !(fabs(D.vVn0[k]) > approachThr)) {
-    if ((fl & F_UP) && -dot3(xyz(D.vw[r.y].v), gHat) <= riseThr && D.ground[r.y] > 0)
+    if ((fl & F_UP) && -dot3(xyz(bs.load(r.y, sl.y).v), gHat) <= riseThr && D.ground[r.y] > 0)
       fl |= F_SIDE1;
-    else if ((fl & F_DOWN) && -dot3(xyz(D.vw[r.x].v), gHat) <= riseThr && D.ground[r.x] > 0)
+    else if ((fl & F_DOWN) && -dot3(xyz(bs.load(r.x, sl.x).v), gHat) <= riseThr &&
+             D.ground[r.x] > 0)
       fl |= F_SIDE2;
   }
   D.vFlag[k] = fl;
+}
+// `need` (optional) gates the launch like `run`
+__global__ void k_side_flags(const Dev* __restrict__ Dp, int numRows, d3 gHat, double riseThr, double approachThr,
+                             const int* run, const int* need) {
+  const Dev& D = *Dp;
+  const int k = gtid();
+  if (k >= numRows || !*run || (need && !*need)) return;
+  sideFlagsRow(D, k, gHat, riseThr, approachThr, GBodies{D.vw});
 }
 
 // mode: 0 = warm start, 1 = PGS sweep, 2 = one-shot GS sweep
@@ -2240,7 +2254,9 @@ __global__ void k_sweep_check(const unsigned long long* bits, int* stop, int* co
   if (strict ? (m < thr) : (m <= thr)) *stop = 1;
 }
 // stabilization gate (src/sim.hpp:279-283): run iff mode != 0 and the residual exceeds vRestS
-__global__ void k_stab_gate(const unsigned long long* bits, int* ctl, int mode, double vRestS) {
+__global__ void k_stab_gate(const unsigned long long* bits, int* ctl, int mode, double vRestS,
+                            const int* need) {
+  if (need && !*need) return;
   const int run = (mode != 0 && bitsd(*bits) > vRestS) ? 1 : 0;
   ctl[CTL_STAB_RUN] = run;
   ctl[CTL_STOP_S] = run ? 0 : 1;
@@ -2370,7 +2386,7 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
   vSweepLoop<MODE>(D, P, cs, numColors, numManifolds, iters, thr, stopThr, stop, count,
                    GBodies{D.vw}, sh);
 }
-// The same loop with its bodies staged in LDS: a colour pass reads and writes the velocities
+// The velocity loops with their bodies staged in LDS: a colour pass reads and writes the velocities
 // of up to bodyCap (<= VL_BODIES) of the rows' bodies in LDS, the rest in global memory (the
 // row constants stay in global memory, read-only in the loop), and the staged velocities go back
 // to global memory once.  Staging hashes the rows' distinct bodies into LDS (the hash shares the
@@ -2402,12 +2418,28 @@ __device__ __forceinline__ int vlSlot(int* hKey, int* used, int body, bool inser
   }
   return -1;
 }
+// The velocity stage one launch runs on the staged bodies: optionally the warm-start pass, the
+// sweep loop, and optionally the stabilization gate (src/sim.hpp:279-283) followed, for
+// stabilization mode 1, by the side flags and the stabilization loop.
+struct VStage {
+  int warm;  // 1: warm-start pass first
+  int iters;
+  double thr, stopThr;
+  int *stop, *count;
+  int gate;      // 1: stabilization gate after the loop (writes stabRun and stabStop)
+  int stabMode;  // the gate's mode; mode 1 also runs the side flags and the loop
+  int stabIters;
+  double stabThr;
+  int *stabRun, *stabStop, *stabCount;
+  d3 gHat;
+  double riseThr, approachThr;
+};
 template <int MODE>
 __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_lds(
     const Dev* __restrict__ Dp, Params P, ClassStarts cs, int numColors, int numManifolds,
-    int iters, double thr, double stopThr, int* stop, int* count, const int* run, int* fallback,
-    int bodyCap, int keyCap) {
+    VStage S, int* fallback, int bodyCap, int keyCap) {
   const Dev& D = *Dp;
+
   __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
   __shared__ double2 bv[3 * VL_BODIES];
   __shared__ int bodyOf[VL_BODIES];
@@ -2415,7 +2447,7 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_lds(
   __shared__ int sUsed, sFail, sDense;
   int* hKey = reinterpret_cast<int*>(bv);
   unsigned short* hIdx = reinterpret_cast<unsigned short*>(hKey + VL_HASH);
-  if ((run && !*run) || *stop) return;  // k_vsolve_block returns on the same test
+  if (*S.stop) return;  // (zero at the start of a substep)
   const int T = blockDim.x, tid = threadIdx.x;
   const int r0 = cs.s[0], nRows = cs.s[numColors] - r0;
   if (nRows > VL_ROWS) {
@@ -2466,8 +2498,47 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_lds(
     bv[3 * s + 2] = make_double2(g.w.y, g.w.z);
   }
   __syncthreads();
-  vSweepLoop<MODE>(D, P, cs, numColors, numManifolds, iters, thr, stopThr, stop, count,
-                   LBodies{bv, slots, bodyOf, D.vw, r0}, sh);
+  const LBodies bs{bv, slots, bodyOf, D.vw, r0};
+  if (S.warm) {
+    for (int c = 0; c < numColors; ++c) {
+      const int b = cs.s[c], e = cs.s[c + 1];
+      if (e <= b) continue;
+      if (c == 62) {
+        if (tid == 0)
+          for (int q = 0; q < numManifolds; ++q) {
+            if (D.mColor[q] != c) continue;
+            const int k = D.vRowOf[q];
+            int2 r, sl;
+            bs.row(D, k, r, sl);
+            warmApplyRowB(D, k, r, sl, bs);
+          }
+      } else {
+        for (int k = b + tid; k < e; k += T) {
+          int2 r, sl;
+          bs.row(D, k, r, sl);
+          warmApplyRowB(D, k, r, sl, bs);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  vSweepLoop<MODE>(D, P, cs, numColors, numManifolds, S.iters, S.thr, S.stopThr, S.stop, S.count,
+                   bs, sh);
+  if (S.gate) {
+    __syncthreads();  // the loop's last residual is in *D.maxApproachBits
+    const int runS = (S.stabMode != 0 && bitsd(*D.maxApproachBits) > S.stabThr) ? 1 : 0;
+    if (tid == 0) {
+      *S.stabRun = runS;
+      *S.stabStop = runS ? 0 : 1;
+    }
+    if (runS && S.stabMode == 1) {
+      for (int k = r0 + tid; k < r0 + nRows; k += T)
+        sideFlagsRow(D, k, S.gHat, S.riseThr, S.approachThr, bs);
+      __syncthreads();
+      vSweepLoop<1>(D, P, cs, numColors, numManifolds, S.stabIters, S.stabThr, S.stabThr,
+                    S.stabStop, S.stabCount, bs, sh);
+    }
+  }
   __syncthreads();
   for (int s = tid; s < nb; s += T) {
     PD_G Body64* g = D.vw + bodyOf[s];

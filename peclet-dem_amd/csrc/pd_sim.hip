@@ -1244,27 +1244,12 @@ struct pd_sim {
     HIPCHECK(hipGetLastError());
   }
 
-  // One-workgroup velocity sweep loop (mode 1 PGS, 2 one-shot GS, or 0 the warm-start pass)
-  // over the colour classes vs.  With the bodies staged in LDS when the rows allow it; up to
-  // ldsKeys / 2 rows they always do, above that the global-memory loop follows, gated on the
-  // staged kernel's fallback flag.
-  void vLoop(int mode, const Params& P, const std::vector<int>& vs, int numColors, int nRows,
-             int iters, double thr, double stopThr, int* stop, int* count, const int* run) {
+  // One-workgroup velocity sweep loop in global memory (mode 1 PGS, 2 one-shot GS, or 0 the
+  // warm-start pass) over the colour classes vs; `need` (optional) gates it on a device flag.
+  void vLoop(int mode, const Params& P, const std::vector<int>& vs, int numColors, int iters,
+             double thr, double stopThr, int* stop, int* count, const int* run,
+             const int* need = nullptr) {
     const ClassStarts cs = classStarts(vs);
-    const bool lds = mode != 0 && ldsLoop && nRows <= VL_ROWS;
-    int* fb = ctl.p + CTL_VFALLBACK;
-    if (lds) {
-      if (mode == 1)
-        k_vsolve_lds<1><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
-                                                       stopThr, stop, count, run, fb, ldsBodies,
-                                                       ldsKeys);
-      else
-        k_vsolve_lds<2><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
-                                                       stopThr, stop, count, run, fb, ldsBodies,
-                                                       ldsKeys);
-      if (2 * (long)nRows <= ldsKeys) return;
-    }
-    const int* need = lds ? fb : nullptr;
     if (mode == 0)
       k_vsolve_block<0><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
                                                        stopThr, stop, count, run, need);
@@ -1275,12 +1260,14 @@ struct pd_sim {
       k_vsolve_block<2><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, cs, numColors, nm, iters, thr,
                                                        stopThr, stop, count, run, need);
   }
+  bool vFusedRec = false;  // the timed K_VBLOCK record covers both loops of the stage
   long vBlockRec[2] = {-1, -1}, pBlockRec = -1;
   int posSweepsPrev = 1;  // sweeps of the last batched position loop (first batch size)
   static constexpr int POS_TAIL_ROWS = 8192;
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
     vBlockRec[0] = vBlockRec[1] = pBlockRec = -1;
+    vFusedRec = false;
     timedNow = timing > 0 && timingSubstep % timingEvery == 0;
     ++timingSubstep;
     if (stabMode == 2) throw std::runtime_error("stabilization mode 'multilevel' is not implemented");
@@ -1455,6 +1442,13 @@ struct pd_sim {
     const bool usePGS = usePersist && useGS;
     const double gMagP = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
     const d3 gHat = usePersist ? d3{g[0] / gMagP, g[1] / gMagP, g[2] / gMagP} : d3{0, 0, 0};
+    const double vRest = 2.0 * dt * std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    const double vRestS = 2.0 * dt * gMagP;
+    const bool hostLoop = velLeftover > 0 || (usePGS && (stabMode == 3 || stabMode == 4)) || !useGS ||
+                          legacyFriction;
+    const bool smallV = nRowsV <= blockRows;
+    // the whole velocity stage in one launch on LDS-staged bodies
+    const bool fusedV = !hostLoop && smallV && velIters > 0 && ldsLoop && nRowsV <= VL_ROWS;
     const size_t R = nRowsV + 1;
     vAB.ensure(R, st); vN.ensure(R, st); vTA.ensure(R, st); vTB.ensure(R, st); vRA.ensure(R, st);
     vRB.ensure(R, st); vVt0.ensure(R, st); vPre.ensure(R, st); vLamT.ensure(R, st); vVn0.ensure(R, st);
@@ -1480,26 +1474,72 @@ struct pd_sim {
         }
       }
       // 8 swaps: the result is back in `ground`
-      if (nRowsV <= blockRows)
-        vLoop(0, P, vs, numColors, nRowsV, 1, 0.0, 0.0, ctl.p + CTL_NONE, ctl.p + CTL_NONE,
-              nullptr);
-      else
-        velocitySweep(vs, numColors, 0, 0.0, ctl.p + CTL_NONE);
+      if (!fusedV) {
+        if (nRowsV <= blockRows)
+          vLoop(0, P, vs, numColors, 1, 0.0, 0.0, ctl.p + CTL_NONE, ctl.p + CTL_NONE, nullptr);
+        else
+          velocitySweep(vs, numColors, 0, 0.0, ctl.p + CTL_NONE);
+      }
     }
     HIPCHECK(hipGetLastError());
     mark(ST_VSWEEP);
-    const double vRest = 2.0 * dt * std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-    const double vRestS = 2.0 * dt * gMagP;
-    const bool hostLoop = velLeftover > 0 || (usePGS && (stabMode == 3 || stabMode == 4)) || !useGS ||
-                          legacyFriction;
-    if (!hostLoop) {
+    if (fusedV) {
+      // warm start, sweep loop, stabilization gate, side flags and stabilization loop in one
+      // launch on bodies staged in LDS; above ldsKeys / 2 rows staging may give up, and the
+      // same sequence of global-memory launches follows, gated on its flag
+      const double stopThr = usePGS ? 0.02 * vRest : vRest;
+      VStage S{};
+      S.warm = usePGS ? 1 : 0;
+      S.iters = velIters;
+      S.thr = vRest;
+      S.stopThr = stopThr;
+      S.stop = ctl.p + CTL_STOP_V;
+      S.count = ctl.p + CTL_SWEEPS_V;
+      S.gate = usePGS ? 1 : 0;
+      S.stabMode = stabMode;
+      S.stabIters = 2 * velIters;
+      S.stabThr = vRestS;
+      S.stabRun = ctl.p + CTL_STAB_RUN;
+      S.stabStop = ctl.p + CTL_STOP_S;
+      S.stabCount = ctl.p + CTL_SWEEPS_S;
+      S.gHat = gHat;
+      S.riseThr = 8.0 * dt * gMagP;
+      S.approachThr = 8.0 * dt * gMagP;
+      int* fb = ctl.p + CTL_VFALLBACK;
+      kernBegin();
+      if (usePGS)
+        k_vsolve_lds<1><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, classStarts(vs), numColors, nm, S, fb,
+                                                       ldsBodies, ldsKeys);
+      else
+        k_vsolve_lds<2><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, classStarts(vs), numColors, nm, S, fb,
+                                                       ldsBodies, ldsKeys);
+      if (2 * (long)nRowsV > ldsKeys) {
+        if (usePGS)
+          vLoop(0, P, vs, numColors, 1, 0.0, 0.0, ctl.p + CTL_NONE, ctl.p + CTL_NONE, nullptr, fb);
+        vLoop(usePGS ? 1 : 2, P, vs, numColors, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
+              ctl.p + CTL_SWEEPS_V, nullptr, fb);
+        if (usePGS) {
+          k_stab_gate<<<1, 1, 0, st>>>(bits.p + 1, ctl.p, stabMode, vRestS, fb);
+          if (stabMode == 1) {
+            if (nRowsV > 0)
+              k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, S.riseThr,
+                                                         S.approachThr, ctl.p + CTL_STAB_RUN, fb);
+            vLoop(1, P, vs, numColors, 2 * velIters, vRestS, vRestS, ctl.p + CTL_STOP_S,
+                  ctl.p + CTL_SWEEPS_S, ctl.p + CTL_STAB_RUN, fb);
+          }
+        }
+      }
+      vBlockRec[0] = kernEnd(K_VBLOCK, nRowsV);
+      vFusedRec = true;
+      HIPCHECK(hipGetLastError());
+      mark(ST_STAB);
+    } else if (!hostLoop) {
       // device-controlled adaptive stops: no host round trip per sweep
       const double stopThr = usePGS ? 0.02 * vRest : vRest;
-      const bool smallV = nRowsV <= blockRows;
       if (smallV && velIters > 0) {
         kernBegin();
-        vLoop(usePGS ? 1 : 2, P, vs, numColors, nRowsV, velIters, vRest, stopThr,
-              ctl.p + CTL_STOP_V, ctl.p + CTL_SWEEPS_V, nullptr);
+        vLoop(usePGS ? 1 : 2, P, vs, numColors, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
+              ctl.p + CTL_SWEEPS_V, nullptr);
         vBlockRec[0] = kernEnd(K_VBLOCK, nRowsV);
       } else if (!smallV) sweepLoop(velIters, ctl.p + CTL_STOP_V, [&] {
         k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V);
@@ -1509,22 +1549,24 @@ struct pd_sim {
       });
       mark(ST_STAB);
       if (usePGS) {
-        k_stab_gate<<<1, 1, 0, st>>>(bits.p + 1, ctl.p, stabMode, vRestS);
+        k_stab_gate<<<1, 1, 0, st>>>(bits.p + 1, ctl.p, stabMode, vRestS, nullptr);
         if (smallV) {
           // the gate stays on the device: the flags kernel and the loop read it
           if (stabMode == 1 && velIters > 0) {
             if (nRowsV > 0)
               k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, 8.0 * dt * gMagP,
-                                                         8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
+                                                         8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN,
+                                                         nullptr);
             kernBegin();
-            vLoop(1, P, vs, numColors, nRowsV, 2 * velIters, vRestS, vRestS, ctl.p + CTL_STOP_S,
+            vLoop(1, P, vs, numColors, 2 * velIters, vRestS, vRestS, ctl.p + CTL_STOP_S,
                   ctl.p + CTL_SWEEPS_S, ctl.p + CTL_STAB_RUN);
             vBlockRec[1] = kernEnd(K_VBLOCK, nRowsV);
           }
         } else if (stabMode == 1 && velIters > 0 && read1(ctl.p + CTL_STAB_RUN)) {
           if (nRowsV > 0)
             k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, 8.0 * dt * gMagP,
-                                                       8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
+                                                       8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN,
+                                                       nullptr);
           sweepLoop(2 * velIters, ctl.p + CTL_STOP_S, [&] {
             k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S);
             velocitySweep(vs, numColors, 1, vRestS, ctl.p + CTL_STOP_S);
@@ -1567,7 +1609,8 @@ struct pd_sim {
           writeN(ctl.p + CTL_STAB_RUN, &one, 1);
           if (stabMode == 1 && nRowsV > 0)
             k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, 8.0 * dt * gMagP,
-                                                       8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN);
+                                                       8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN,
+                                                       nullptr);
           if (stabMode == 4) {
             orderedStabilization(gHat, vRestS);
           } else {
@@ -1788,7 +1831,9 @@ struct pd_sim {
     stats.max_approach = maxApproach;
     if (timing >= 2) {
       // the one-workgroup loops processed rows x sweeps, known only now
-      if (vBlockRec[0] >= 0) kernPending[vBlockRec[0]].units = (long)nRowsV * stats.sweeps_vel;
+      if (vBlockRec[0] >= 0)
+        kernPending[vBlockRec[0]].units =
+            (long)nRowsV * (stats.sweeps_vel + (vFusedRec ? stats.sweeps_stab : 0));
       if (vBlockRec[1] >= 0) kernPending[vBlockRec[1]].units = (long)nRowsV * stats.sweeps_stab;
       if (pBlockRec >= 0) kernPending[pBlockRec].units = (long)nRowsP * stats.sweeps_pos;
       statsLog.push_back(stats);
@@ -2854,7 +2899,7 @@ int pd_get_stats_log(pd_sim* s, pd_stats* out, int cap) {
 const char* pd_kernel_name(int k) {
   static const char* names[pd_sim::K_COUNT] = {
       "k_vsweep",  "k_psweep",      "k_bp_query",     "k_np_pairs",
-      "k_predict", "k_reintegrate", "k_vsolve_block", "k_psolve_block"};
+      "k_predict", "k_reintegrate", "k_vsolve_lds", "k_psolve_block"};
   return (k >= 0 && k < pd_sim::K_COUNT) ? names[k] : "";
 }
 const char* pd_stage_name(int stage) {

@@ -348,7 +348,7 @@ def test_timing_is_passive_and_logs_counters(hip_lib):
     names = [b._b.kernel_name(k).decode() for k in range(count)]
     q = names.index("k_predict")
     assert launches[q] == 12 and units[q] == 12 * 400 and ms[q] > 0
-    v = names.index("k_vsolve_block")
+    v = names.index("k_vsolve_lds")
     want = sum(s["active_manifolds"] * (s["sweeps_vel"] + s["sweeps_stab"]) for s in plain)
     assert units[v] == want
     stage = np.zeros(32)
