@@ -133,6 +133,16 @@ def test_velocity_loop_body_storage(hip_lib, kind, tuning):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,steps", [(2500, 4), (4000, 3)])
+def test_velocity_stage_large_beds(hip_lib, n, steps):
+    # 2500 spheres: ~4500-6700 rows over ~2900 bodies, so the staged stage keeps 2400 bodies in
+    # LDS and the rest in global memory, with several rounds per colour and the stabilization
+    # loop running; 4000 spheres: more than 8192 rows, so staging gives up and the gated
+    # global-memory launches run the stage
+    run_pair("bed_periodic", n, steps)
+
+
+@pytest.mark.gpu
 def test_bed_small_periodic_box_twins(hip_lib):
     # box of ~2 diameters: both periodic twins of a contact exist (duplicate ledger keys)
     run_pair("bed_small_box", 24, 20)
