@@ -184,7 +184,8 @@ struct pd_sim {
   hipEvent_t takeEvent() {
     if (evFree.empty()) {
       hipEvent_t e;
-      HIPCHECK(hipEventCreate(&e));
+      // timestamps only: no system-scope fence when recorded (it would cost the timed work)
+      HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
       return e;
     }
     hipEvent_t e = evFree.back();
@@ -280,8 +281,11 @@ struct pd_sim {
   pd_sim(int cap) : capacity(cap) {
     HIPCHECK(hipStreamCreate(&st));
     HIPCHECK(hipStreamCreate(&sideStream));
-    HIPCHECK(hipEventCreateWithFlags(&evSide, hipEventDisableTiming));
-    HIPCHECK(hipEventCreateWithFlags(&evContacts, hipEventDisableTiming));
+    // the two streams exchange device memory only: no system-scope fence (cache writeback and
+    // invalidation) when these are recorded
+    HIPCHECK(hipEventCreateWithFlags(&evSide, hipEventDisableTiming | hipEventDisableSystemFence));
+    HIPCHECK(hipEventCreateWithFlags(&evContacts,
+                                     hipEventDisableTiming | hipEventDisableSystemFence));
     HIPCHECK(hipHostMalloc(&hostScratch, HOST_SCRATCH_BYTES, hipHostMallocDefault));
     HIPCHECK(hipMalloc((void**)&devD, sizeof(Dev)));
     dD = devD;
