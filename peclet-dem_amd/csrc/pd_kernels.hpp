@@ -1075,12 +1075,17 @@ __global__ __launch_bounds__(TPB) void k_np_pairs(const Dev* __restrict__ Dp, Pa
 // Write pass of point shapes (at most one contact per pair) over the contacts only: contact c
 // comes from pair pairOf[c] (the compaction written by the slot scan) and lands at its offset
 // noff[pairOf[c]] = c, as in the full pass.  The pair contacts are the first noff[numPairs].
+// (grid-stride over the pair contacts, whose count noff[numPairs] is read on the device)
 __global__ __launch_bounds__(TPB) void k_np_pairs_sparse(const Dev* __restrict__ Dp, Params P, int numPairs,
                                                          const int* pairOf) {
   const Dev& D = *Dp;
   __shared__ unsigned long long sh[WAVES];
-  const int c = gtid();
-  const double ovl = (c < D.noff[numPairs]) ? npPair<true>(D, P, pairOf[c]) : 0.0;
+  const int n = D.noff[numPairs];
+  double ovl = 0.0;
+  for (int c = gtid(); c < n; c += gridDim.x * blockDim.x) {
+    const double o = npPair<true>(D, P, pairOf[c]);
+    ovl = o > ovl ? o : ovl;
+  }
   blockAtomicMaxU64(D.maxOverlapBits, dbits(ovl), sh);
 }
 

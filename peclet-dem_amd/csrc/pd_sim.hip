@@ -896,7 +896,13 @@ struct pd_sim {
     return W;
   }
   // contact slots: pairs, then planes per real, then walls per real (the reference's order)
-  void narrowphase(const Params& P, bool zeroOverlap = true) {
+  // deferNc (spheres without walls): the contact total is not read here but with the manifold
+  // total in buildManifolds (one host synchronisation for both); the contact buffers are sized
+  // for the bound np + numReal x planes (a sphere touches each plane once, each pair at most
+  // once), so the write pass never needs the exact count
+  bool ncDeferred = false;
+  long npRecNc = -1;  // timing record of the write pass, its units patched once nc is known
+  void narrowphase(const Params& P, bool zeroOverlap = true, bool deferNc = false) {
     const int nslots = np + 2 * numReal;
     const WallSet W = wallSet();
     ncnt.ensure(nslots + 1, st);
@@ -920,20 +926,27 @@ struct pd_sim {
       k_scan_down2<<<nb, TPB, 0, st>>>(ncnt.p, nslots, scanTmp.p, noff.p, nullptr,
                                        pairOfContact.p, np);
       HIPCHECK(hipGetLastError());
-      nc = read1(noff.p + nslots);
+      ncDeferred = deferNc && W.n == 0;
+      if (!ncDeferred) nc = read1(noff.p + nslots);
     }
-    cAB.ensure(nc + 1, st);
-    cBV.ensure(nc + 1, st);
-    cN.ensure(nc + 1, st);
-    cRA.ensure(nc + 1, st);
-    cRB.ensure(nc + 1, st);
-    cSlot.ensure(nc + 1, st);
+    const size_t ncCap =
+        ncDeferred ? (size_t)np + (size_t)numReal * (size_t)std::max(P.numPlanes, 0) : (size_t)nc;
+    cAB.ensure(ncCap + 1, st);
+    cBV.ensure(ncCap + 1, st);
+    cN.ensure(ncCap + 1, st);
+    cRA.ensure(ncCap + 1, st);
+    cRB.ensure(ncCap + 1, st);
+    cSlot.ensure(ncCap + 1, st);
     refreshDev();
     if (zeroOverlap) zero(bits.p, 1);  // a substep clears it in k_substep_reset
     kernBegin();
-    if (waveNp) k_np_pairs_wave<true><<<npBlocks, TPB, 0, st>>>(dD, P, np);
-    else if (nc > 0) k_np_pairs_sparse<<<nblk(nc), TPB, 0, st>>>(dD, P, np, pairOfContact.p);
-    kernEnd(K_NPPAIRS, waveNp ? np : nc);
+    if (waveNp) {
+      k_np_pairs_wave<true><<<npBlocks, TPB, 0, st>>>(dD, P, np);
+    } else if (ncDeferred || nc > 0) {
+      const int grid = std::max(1, std::min(nblk(ncDeferred ? np : nc), 4096));
+      k_np_pairs_sparse<<<grid, TPB, 0, st>>>(dD, P, np, pairOfContact.p);
+    }
+    npRecNc = kernEnd(K_NPPAIRS, waveNp ? np : nc);
     k_np_planes<true><<<nblk(numReal), TPB, 0, st>>>(dD, P, np);
     if (W.n > 0) k_np_walls<true><<<nblk(numReal), TPB, 0, st>>>(dD, P, W, np + numReal);
     HIPCHECK(hipGetLastError());
@@ -945,7 +958,21 @@ struct pd_sim {
     mStart.ensure(numReal + 1, st);
     refreshDev();
     k_mcount<<<nblk(numReal), TPB, 0, st>>>(dD, P, np);
-    nm = scan(mCnt.p, mStart.p, numReal);
+    if (ncDeferred) {
+      // the manifold total and the deferred contact total in one read
+      scanQuiet(mCnt.p, mStart.p, numReal);
+      PostBatch rb = postBatch(hostScratch, hostScratchDev, 64);
+      const int* hm = rb.add(mStart.p + numReal, 1);
+      const int* hc = rb.add(noff.p + (np + 2 * numReal), 1);
+      rb.post(st);
+      sync();
+      nm = numReal > 0 ? *hm : 0;
+      nc = *hc;
+      ncDeferred = false;
+      if (npRecNc >= 0 && npRecNc < (long)kernPending.size()) kernPending[npRecNc].units = nc;
+    } else {
+      nm = scan(mCnt.p, mStart.p, numReal);
+    }
     const size_t M = nm + 1;
     mAB.ensure(M, st); mReal.ensure(M, st); mRange.ensure(M, st); mRange2.ensure(M, st);
     mWV.ensure(M, st); mNs.ensure(M, st);
@@ -1413,7 +1440,7 @@ struct pd_sim {
       broadphaseCompact(PB);
     }
     mark(ST_NARROW);
-    narrowphase(PB, false);
+    narrowphase(PB, false, true);
     mark(ST_MANIFOLD);
     buildManifolds();
     if (legacyFriction) {
