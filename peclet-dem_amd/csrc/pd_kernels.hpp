@@ -506,13 +506,17 @@ struct PostSpec {
   int len[4];
   int n;
 };
-__global__ void k_post(PostSpec S, int* dst) {
+// (after the values, one thread writes `seq` to *seqDst: the host polls that word instead of
+// synchronising the stream)
+__global__ void k_post(PostSpec S, int* dst, int* seqDst, int seq) {
   int off = 0;
   for (int r = 0; r < S.n; ++r) {
     for (int i = threadIdx.x; i < S.len[r]; i += blockDim.x) dst[off + i] = S.src[r][i];
     off += S.len[r];
   }
-  __threadfence_system();
+  __threadfence_system();  // this thread's values reach host memory
+  __syncthreads();
+  if (threadIdx.x == 0) *(volatile int*)seqDst = seq;
 }
 // slot count of the substep from the ghost total: info[0] = reals + the ghosts that fit in
 // capSlots, info[1] = 1 when some did not fit

@@ -270,7 +270,8 @@ struct pd_sim {
   // pinned host scratch
   static constexpr size_t HOST_SCRATCH_BYTES = 4096;
   void* hostScratch = nullptr;
-  void* hostScratchDev = nullptr;  // device view of the pinned scratch
+  void* hostPost = nullptr;     // pinned area of the main stream's batched read-backs
+  void* hostPostDev = nullptr;  // its device view
 
   Dev D{};
 
@@ -296,7 +297,12 @@ struct pd_sim {
     }
     HIPCHECK(hipHostMalloc(&jobV.pinned, 1024, hipHostMallocDefault));
     HIPCHECK(hipHostMalloc(&jobP.pinned, 1024, hipHostMallocDefault));
-    HIPCHECK(hipHostGetDevicePointer(&hostScratchDev, hostScratch, 0));
+    // the read-back areas' sequence words start at 0, a number never posted
+    std::memset(jobV.pinned, 0, 1024);
+    std::memset(jobP.pinned, 0, 1024);
+    HIPCHECK(hipHostMalloc(&hostPost, 1024, hipHostMallocDefault));
+    std::memset(hostPost, 0, 1024);
+    HIPCHECK(hipHostGetDevicePointer(&hostPostDev, hostPost, 0));
     HIPCHECK(hipHostGetDevicePointer(&jobV.pinnedDev, jobV.pinned, 0));
     HIPCHECK(hipHostGetDevicePointer(&jobP.pinnedDev, jobP.pinned, 0));
     jobV.stream = st;
@@ -326,6 +332,7 @@ struct pd_sim {
       for (int k = 0; k <= stageMarked; ++k) (void)hipEventDestroy(stageCur[k]);
     for (hipEvent_t e : evFree) (void)hipEventDestroy(e);
     if (hostScratch) (void)hipHostFree(hostScratch);
+    if (hostPost) (void)hipHostFree(hostPost);
     for (int k = 0; k < DEV_RING; ++k) (void)hipEventDestroy(devRingEv[k]);
     if (devRing) (void)hipHostFree(devRing);
     if (devD) (void)hipFree(devD);
@@ -398,12 +405,33 @@ struct pd_sim {
   }
   // Ranges read back together through one k_post launch into pinned memory `area` (host view;
   // areaDev is the device view): add() returns where each range will be, post() launches.
+  // Wait until the posted word *seqp equals seq: the k_post that writes it wrote its values
+  // first.  Polling the pinned word wakes the host as soon as the values land; the stream is
+  // queried now and then so that a failed stream raises instead of spinning.
+  static void waitPosted(const volatile int* seqp, int seq, hipStream_t s) {
+    for (long spin = 1;; ++spin) {
+      if (*seqp == seq) break;
+      if ((spin & 4095) == 0) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) {
+          if (*seqp == seq) break;
+          throw std::runtime_error("read-back: the stream finished without posting");
+        }
+        if (e != hipErrorNotReady) HIPCHECK(e);
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+  }
+  int postSeq = 0;  // the last read-back number handed out (0 is never posted)
   struct PostBatch {
     PostSpec spec{};
     int used = 0;
     int* area = nullptr;
     int* areaDev = nullptr;
     int cap = 0;
+    int seq = 0;
+    int* seqHost = nullptr;
+    int* seqDev = nullptr;
     const int* add(const int* src, int len) {
       if (spec.n >= 4 || used + len > cap) throw std::runtime_error("read-back batch too large");
       spec.src[spec.n] = src;
@@ -414,15 +442,22 @@ struct pd_sim {
       return h;
     }
     void post(hipStream_t s) {
-      k_post<<<1, 128, 0, s>>>(spec, areaDev);
+      k_post<<<1, 128, 0, s>>>(spec, areaDev, seqDev, seq);
       HIPCHECK(hipGetLastError());
     }
+    // the values are in host memory after it
+    void wait(hipStream_t s) const { waitPosted(seqHost, seq, s); }
   };
+  // the area's last int is the posted-sequence word
   PostBatch postBatch(void* pinnedArea, void* pinnedAreaDev, int capInts) {
     PostBatch b;
     b.area = static_cast<int*>(pinnedArea);
     b.areaDev = static_cast<int*>(pinnedAreaDev);
-    b.cap = capInts;
+    b.cap = capInts - 1;
+    b.seqHost = b.area + capInts - 1;
+    b.seqDev = b.areaDev + capInts - 1;
+    postSeq = postSeq == 0x7fffffff ? 1 : postSeq + 1;
+    b.seq = postSeq;
     return b;
   }
   // several small reads with one synchronisation: enqueue each into the pinned scratch at
@@ -961,11 +996,11 @@ struct pd_sim {
     if (ncDeferred) {
       // the manifold total and the deferred contact total in one read
       scanQuiet(mCnt.p, mStart.p, numReal);
-      PostBatch rb = postBatch(hostScratch, hostScratchDev, 64);
+      PostBatch rb = postBatch(hostPost, hostPostDev, 64);
       const int* hm = rb.add(mStart.p + numReal, 1);
       const int* hc = rb.add(noff.p + (np + 2 * numReal), 1);
       rb.post(st);
-      sync();
+      rb.wait(st);
       nm = numReal > 0 ? *hm : 0;
       nc = *hc;
       ncDeferred = false;
@@ -1018,6 +1053,8 @@ struct pd_sim {
     bool first = true;
     const int* hostCnt = nullptr;
     const int* hostStarts = nullptr;
+    const int* postSeqHost = nullptr;  // the read-back's sequence word and number
+    int postSeq = 0;
     void release() {
       work[0].release(); work[1].release(); cnt.release(); blockCnt.release(); colCnt.release();
       colStart.release(); winner.release(); mask.release();
@@ -1124,6 +1161,8 @@ struct pd_sim {
     J.hostCnt = rb.add(J.cnt.p, B + 1);
     J.hostStarts = rb.add(J.colStart.p, 65);
     rb.post(s);
+    J.postSeqHost = rb.seqHost;
+    J.postSeq = rb.seq;
     J.first = false;
   }
   // wait for the job's read-back and replay its rounds; enqueue further batches until done.
@@ -1131,7 +1170,7 @@ struct pd_sim {
   int colourFinish(ColourJob& J, std::vector<int>& starts, int& numColors, int& rounds,
                    long long& uncol) {
     for (;;) {
-      HIPCHECK(hipStreamSynchronize(J.stream));
+      waitPosted(J.postSeqHost, J.postSeq, J.stream);
       // a small-colouring kernel that could not take its list changed nothing: replay the
       // rounds before it and continue with launch-per-round batches
       const bool failed = J.smallAt >= 0 && J.hostCnt[J.smallAt + 1] < 0;
@@ -1381,12 +1420,12 @@ struct pd_sim {
         PS.skin = skin;
         broadphaseQuery(PS, slotInfo.p, (int)gridSlots, capSlots);
       }
-      PostBatch rb = postBatch(hostScratch, hostScratchDev, 64);
+      PostBatch rb = postBatch(hostPost, hostPostDev, 64);
       const int* npH = rb.add(pairStart.p + numReal, 1);
       const int* gH = rb.add(ghostOff.p + numReal, 1);
       const int* fH = useCand ? rb.add(candFlag.p, 1) : nullptr;
       rb.post(st);
-      sync();
+      rb.wait(st);
       np = *npH;
       G = *gH;
       const bool stale = fH && *fH;
@@ -1793,12 +1832,12 @@ struct pd_sim {
         }
         HIPCHECK(hipGetLastError());
         done += b;
-        PostBatch rb = postBatch(hostScratch, hostScratchDev, 64);
+        PostBatch rb = postBatch(hostPost, hostPostDev, 64);
         finalCtl = rb.add(ctl.p, CTL_COUNT);
         finalBits = reinterpret_cast<const unsigned long long*>(
             rb.add(reinterpret_cast<const int*>(bits.p), 4));
         rb.post(st);
-        sync();
+        rb.wait(st);
         if (finalCtl[CTL_STOP_P]) break;
         batch *= 2;
       }
@@ -1826,12 +1865,12 @@ struct pd_sim {
       // one read-back of the loop counters and the residuals (the position loop's last one
       // when it ran in batches)
       if (!finalCtl) {
-        PostBatch rb = postBatch(hostScratch, hostScratchDev, 64);
+        PostBatch rb = postBatch(hostPost, hostPostDev, 64);
         finalCtl = rb.add(ctl.p, CTL_COUNT);
         finalBits = reinterpret_cast<const unsigned long long*>(
             rb.add(reinterpret_cast<const int*>(bits.p), 4));
         rb.post(st);
-        sync();
+        rb.wait(st);
       }
       const int* c = finalCtl;
       const unsigned long long* b2 = finalBits;
