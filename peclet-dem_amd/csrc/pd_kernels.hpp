@@ -374,11 +374,40 @@ __global__ void k_growth(double* scale, const double* target, double factor, int
   if (i < n) scale[i] = target[i] * factor;
 }
 
+// The per-substep control words the predict launch clears first (one launch fewer): the
+// candidate-check flag, the colouring counters, the support-list count, the sweep-loop flags and
+// counters, the overlap accumulator; and the grounded-level decay of the substep
+// (src/contact_preprocessing.hpp:213-253, first step), when `ground` is given.
+struct SubstepStart {
+  int* candFlag;
+  int* counters;
+  int* supCnt;
+  int* ctl;
+  int nCtl;
+  unsigned long long* overlapBits;
+  int* ground;  // nullptr: no grounded levels this substep
+  int decay;
+};
 // predictVelocityKokkos, src/integration.hpp:44-93 (+ realIndices and broadphase radius fill)
-__global__ void k_predict(const Dev* __restrict__ Dp, Params P) {
+__global__ void k_predict(const Dev* __restrict__ Dp, Params P, SubstepStart S) {
   const Dev& D = *Dp;
+  if (blockIdx.x == 0) {
+    const int t = threadIdx.x;
+    if (t < S.nCtl) S.ctl[t] = 0;
+    if (t == 0) {
+      *S.candFlag = 0;
+      S.counters[0] = 0;
+      S.counters[1] = 0;
+      *S.supCnt = 0;
+      *S.overlapBits = 0ull;
+    }
+  }
   const int i = gtid();
   if (i >= P.numReal) return;
+  if (S.ground) {
+    const int v = S.ground[i] - S.decay;
+    S.ground[i] = v > 0 ? v : 0;
+  }
   const BodyC b = D.bc[i];
   const Body64 v0 = D.vw0[i];
   const double invM = b.iim.w;
@@ -444,9 +473,17 @@ __global__ void k_ghost_count(const Dev* __restrict__ Dp, Params P, int* cnt) {
 }
 // pass 2: emit ghosts at numReal + off[i] in the (a,b,c) loop order
 // slots at or past capSlots are not written (the host grows the buffers and emits again)
-__global__ void k_ghost_emit(const Dev* __restrict__ Dp, Params P, const int* off, int capSlots) {
+// slotInfo (optional): info[0] = reals + the ghosts that fit in capSlots, info[1] = 1 when some
+// did not fit (from the ghost total off[numReal])
+__global__ void k_ghost_emit(const Dev* __restrict__ Dp, Params P, const int* off, int capSlots,
+                             int* slotInfo = nullptr) {
   const Dev& D = *Dp;
   const int i = gtid();
+  if (slotInfo && i == 0) {
+    const long n = (long)P.numReal + off[P.numReal];
+    slotInfo[0] = (int)(n < capSlots ? n : capSlots);
+    slotInfo[1] = n > capSlots ? 1 : 0;
+  }
   if (i >= P.numReal) return;
   const d3 p = xyz(D.pos[i]);
   int sx, sy, sz;
@@ -485,20 +522,6 @@ __global__ void k_ghost_emit(const Dev* __restrict__ Dp, Params P, const int* of
       }
 }
 
-// the per-substep control words start at zero: the candidate-check flag, the colouring counters,
-// the support-list count, the sweep-loop flags and counters, and the overlap accumulator
-__global__ void k_substep_reset(int* candFlag, int* counters, int* supCnt, int* ctl, int nCtl,
-                                unsigned long long* overlapBits) {
-  const int t = threadIdx.x;
-  if (t < nCtl) ctl[t] = 0;
-  if (t == 0) {
-    *candFlag = 0;
-    counters[0] = 0;
-    counters[1] = 0;
-    *supCnt = 0;
-    *overlapBits = 0ull;
-  }
-}
 // a few device int ranges copied in one launch to host-visible pinned memory (the host reads
 // them after synchronising the stream): one small kernel instead of a runtime copy per range
 struct PostSpec {
@@ -517,13 +540,6 @@ __global__ void k_post(PostSpec S, int* dst, int* seqDst, int seq) {
   __threadfence_system();  // this thread's values reach host memory
   __syncthreads();
   if (threadIdx.x == 0) *(volatile int*)seqDst = seq;
-}
-// slot count of the substep from the ghost total: info[0] = reals + the ghosts that fit in
-// capSlots, info[1] = 1 when some did not fit
-__global__ void k_slot_count(const int* ghostTotal, int numReal, int capSlots, int* info) {
-  const long n = (long)numReal + *ghostTotal;
-  info[0] = (int)(n < capSlots ? n : capSlots);
-  info[1] = n > capSlots ? 1 : 0;
 }
 __global__ void k_set_int(int* dst, int v) { *dst = v; }
 
@@ -1761,12 +1777,6 @@ __global__ __launch_bounds__(TPB) void k_vrows(const Dev* __restrict__ Dp, Param
 }
 
 // Guendelman support levels, double-buffered sweeps (src/contact_preprocessing.hpp:213-253)
-__global__ void k_ground_decay(int* g, int n, int decay) {
-  const int i = gtid();
-  if (i >= n) return;
-  const int v = g[i] - decay;
-  g[i] = v > 0 ? v : 0;
-}
 __global__ void k_copy_i32(int* dst, const int* src, int n) {
   const int i = gtid();
   if (i < n) dst[i] = src[i];
@@ -2752,9 +2762,12 @@ __global__ void k_jp_init_contacts(int* colour, int n, int* work) {
 // sweeps never change, so they are evaluated once here (same expressions as per sweep in
 // src/solver_position.hpp:276-312).  Row: pN = (n, wTotal), pRA = (rA, invMassA),
 // pRB = (rB or plane point, invMassB).
-__global__ __launch_bounds__(TPB) void k_prows(const Dev* __restrict__ Dp, int numRows) {
+// zeroBits (optional): the position loop's residual, cleared before the loop's first sweep
+__global__ __launch_bounds__(TPB) void k_prows(const Dev* __restrict__ Dp, int numRows,
+                                               unsigned long long* zeroBits = nullptr) {
   const Dev& D = *Dp;
   const int k = gtid();
+  if (zeroBits && k == 0) *zeroBits = 0ull;
   if (k >= numRows) return;
   const int c = D.pPerm[k];
   const int2 ab = D.cAB[c];

@@ -973,7 +973,7 @@ struct pd_sim {
     cRB.ensure(ncCap + 1, st);
     cSlot.ensure(ncCap + 1, st);
     refreshDev();
-    if (zeroOverlap) zero(bits.p, 1);  // a substep clears it in k_substep_reset
+    if (zeroOverlap) zero(bits.p, 1);  // a substep clears it in its predict launch
     kernBegin();
     if (waveNp) {
       k_np_pairs_wave<true><<<npBlocks, TPB, 0, st>>>(dD, P, np);
@@ -1377,9 +1377,10 @@ struct pd_sim {
     const double margin = 0.1 * maxRad;
     numParticles = numReal;
     Params P = params(margin, maxRad);
-    k_substep_reset<<<1, 64, 0, st>>>(candFlag.p, counters.p, supCnt.p, ctl.p, CTL_COUNT, bits.p);
+    SubstepStart S0{candFlag.p, counters.p, supCnt.p, ctl.p, CTL_COUNT, bits.p, nullptr, 8};
+    if (usePersist && useGS) S0.ground = ground.p;  // the grounded levels are updated this substep
     kernBegin();
-    k_predict<<<nblk(numReal), TPB, 0, st>>>(dD, P);
+    k_predict<<<std::max(1, nblk(numReal)), TPB, 0, st>>>(dD, P, S0);
     kernEnd(K_PREDICT, numReal);
     // periodic ghosts, src/periodicity.hpp:43-94.  The ghost total is read back together with
     // the pair total: the broadphase runs on the device-side slot count, and if the ghosts did
@@ -1398,9 +1399,9 @@ struct pd_sim {
     int G = 0;
     for (;;) {
       const int capSlots = slotCapacity();
-      k_slot_count<<<1, 1, 0, st>>>(ghostOff.p + numReal, numReal, capSlots, slotInfo.p);
       refreshDev();
-      k_ghost_emit<<<nblk(numReal), TPB, 0, st>>>(dD, P, ghostOff.p, capSlots);
+      k_ghost_emit<<<std::max(1, nblk(numReal)), TPB, 0, st>>>(dD, P, ghostOff.p, capSlots,
+                                                               slotInfo.p);
       HIPCHECK(hipGetLastError());
       mark(ST_BROAD);
       const long gridSlots = std::min<long>(capSlots, (long)numReal + lastGhosts + 1024);
@@ -1531,8 +1532,8 @@ struct pd_sim {
       k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(dD, P, nRowsV, prevCount > 0 ? ledgerBuckets : 0,
                                             usePGS ? 1 : 0);
     if (usePGS) {
-      // grounded levels, double-buffered (src/contact_preprocessing.hpp:213-253)
-      k_ground_decay<<<nblk(numReal), TPB, 0, st>>>(ground.p, numReal, 8);
+      // grounded levels, double-buffered (src/contact_preprocessing.hpp:213-253); the decay ran
+      // in the predict launch
       int* ga = ground.p;
       int* gb = ground2.p;
       if (nm > 0) {
@@ -1778,7 +1779,10 @@ struct pd_sim {
     pRB.ensure(nRowsP + 1, st);
     pLam.ensure(nRowsP + 1, st);
     refreshDev();
-    if (nRowsP > 0) k_prows<<<nblk(nRowsP), TPB, 0, st>>>(dD, nRowsP);
+    // with the batched position loop ahead, the rows launch also clears its residual
+    const bool posBatches = useGS && posLeftover == 0 && nRowsP > blockRows;
+    if (nRowsP > 0)
+      k_prows<<<nblk(nRowsP), TPB, 0, st>>>(dD, nRowsP, posBatches ? bits.p : nullptr);
     mark(ST_PSWEEP);
     const double posTol = 1e-4 * baseRadius * globalScale;
     const int* finalCtl = nullptr;
@@ -1811,7 +1815,7 @@ struct pd_sim {
       while (cFrom > 0 && ps[numPosColors] - ps[cFrom - 1] <= POS_TAIL_ROWS) --cFrom;
       const ClassStarts pcs = classStarts(ps);
       int done = 0, batch = std::max(1, std::min(posIters, posSweepsPrev));
-      k_sweep_begin<<<1, 1, 0, st>>>(bits.p, ctl.p + CTL_STOP_P);
+      // (the residual was cleared by the rows launch)
       while (done < posIters) {
         const int b = std::min(batch, posIters - done);
         for (int j = 0; j < b; ++j) {
