@@ -1338,7 +1338,9 @@ __global__ __launch_bounds__(TPB) void k_mreduce(const Dev* __restrict__ Dp, int
 // Jones-Plassmann colouring — src/solver_velocity.hpp:277-374, src/solver_position.hpp:155-241
 // ============================================================================================
 // uncoloured manifolds (-1) are appended to the first work list; inactive / duplicate ones get -2
-__global__ void k_jp_init_manifolds(const Dev* __restrict__ Dp, int numManifolds, int* work, int* count) {
+// (also rowOf[m] = -1: rows are placed for the coloured manifolds only)
+__global__ void k_jp_init_manifolds(const Dev* __restrict__ Dp, int numManifolds, int* work, int* count,
+                                    int* rowOf) {
   const Dev& D = *Dp;
   __shared__ int shCnt, shBase;
   const int m = gtid();
@@ -1349,6 +1351,7 @@ __global__ void k_jp_init_manifolds(const Dev* __restrict__ Dp, int numManifolds
     if (D.mNp[m] <= 0) c = -2;
     else if (D.mAB[m].y >= 0 && r.x > r.y) c = -2;
     D.mColor[m] = c;
+    rowOf[m] = -1;
   }
   blockAppend(c == -1, m, work, count, &shCnt, &shBase);
 }
@@ -1434,16 +1437,22 @@ __device__ __forceinline__ int jpsSlot(int* hKey, int* used, int body, bool inse
 }
 // loadGlobal = 0: the global winner / mask arrays are stale (a colouring starting here), the
 // bodies start from zero
+// place (a colouring that starts in this launch): when it completes, the launch also sorts its
+// edges into colour classes (class starts in colStart[0..64], perm / rowOf as k_color_place),
+// instead of the class kernels
 __global__ __launch_bounds__(JPS_THREADS) void k_jp_small(const int2* ends, int* colour, int* in,
                                                           int* out, int* cnt, int j0, int j1,
                                                           unsigned long long* winnerG,
                                                           unsigned long long* maskG,
-                                                          int loadGlobal) {
+                                                          int loadGlobal, int place, int* perm,
+                                                          int* rowOf, int* colStart) {
   __shared__ int hKey[JPS_HASH];
   __shared__ unsigned long long hWin[JPS_HASH], hMask[JPS_HASH];
   __shared__ int eId[JPS_EDGES];
   __shared__ unsigned eAB[JPS_EDGES];  // slot of a | slot of b << 16 (0xFFFF: boundary)
   __shared__ unsigned short lst[2][JPS_EDGES];
+  __shared__ unsigned char eCol[JPS_EDGES];  // colour given to entering edge q
+  __shared__ int cCnt[64], cBase[65];
   __shared__ int sNext, sFail, sUsed;
   const int T = blockDim.x, tid = threadIdx.x;
   const int n0 = cnt[j0];
@@ -1507,6 +1516,7 @@ __global__ __launch_bounds__(JPS_THREADS) void k_jp_small(const int2* ends, int*
         int c = 0;
         while (c < 62 && (forbidden & (1ull << c))) ++c;
         colour[eId[k]] = c;
+        eCol[k] = (unsigned char)c;
         hMask[sa] |= 1ull << c;
         hWin[sa] = 0;
         if (hasB) {
@@ -1530,6 +1540,31 @@ __global__ __launch_bounds__(JPS_THREADS) void k_jp_small(const int2* ends, int*
         winnerG[b] = hWin[h];
         maskG[b] = hMask[h];
       }
+    }
+  } else if (place) {
+    // every entering edge is coloured: class counts, starts, then each edge's place in its class
+    if (tid < 64) cCnt[tid] = 0;
+    __syncthreads();
+    for (int q = tid; q < n0; q += T) atomicAdd(&cCnt[eCol[q]], 1);
+    __syncthreads();
+    if (tid < 64) {
+      const int v = cCnt[tid];
+      int incl = v;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (tid >= off) incl += o;
+      }
+      cBase[tid] = incl - v;
+      colStart[tid] = incl - v;
+      if (tid == 63) colStart[64] = incl;
+      cCnt[tid] = 0;
+    }
+    __syncthreads();
+    for (int q = tid; q < n0; q += T) {
+      const int c = eCol[q];
+      const int k = cBase[c] + atomicAdd(&cCnt[c], 1);
+      perm[k] = eId[q];
+      rowOf[eId[q]] = k;
     }
   }
 }

@@ -1125,6 +1125,9 @@ struct pd_sim {
         }
     J.B = B;
     const bool launchRounds = J.smallAt != 0;  // some rounds of this batch are launched one by one
+    // a colouring run whole by the one-workgroup kernel also sorts its classes there (if it does
+    // not finish, the next batch runs the class kernels over every edge)
+    const bool place = J.first && J.smallAt == 0;
     const int loadGlobal = J.globalDirty ? 0 : 1;
     if (launchRounds && J.globalDirty) {
       HIPCHECK(hipMemsetAsync(J.mask.p, 0, J.numBodies * sizeof(unsigned long long), s));
@@ -1142,7 +1145,8 @@ struct pd_sim {
       if (j == J.smallAt) {
         J.smallLoad = j == 0 ? loadGlobal : 1;
         k_jp_small<<<1, JPS_THREADS, 0, s>>>(J.ends, J.colour, work, next, J.cnt.p, j, B,
-                                             J.winner.p, J.mask.p, J.smallLoad);
+                                             J.winner.p, J.mask.p, J.smallLoad, place ? 1 : 0,
+                                             J.perm, J.rowOf, J.colStart.p);
         break;
       }
       const int grid = (int)std::min<long>(nblk(est + est / 4 + 64), JP_GRID_CAP);
@@ -1150,12 +1154,14 @@ struct pd_sim {
       k_jp_commit<<<grid, TPB, 0, s>>>(J.ends, J.colour, work, J.cnt.p + j, J.winner.p, J.mask.p,
                                        next, J.cnt.p + j + 1);
     }
-    const int nbc = std::min(nblk(J.numEdges), CP_BLOCKS);
-    k_color_count<<<nbc, TPB, 0, s>>>(J.colour, J.numEdges, J.blockCnt.p);
-    k_color_offsets<<<64, TPB, 0, s>>>(J.blockCnt.p, nbc, J.colCnt.p);
-    k_color_starts<<<1, 64, 0, s>>>(J.colCnt.p, J.colStart.p);
-    k_color_place<<<nbc, TPB, 0, s>>>(J.colour, J.numEdges, J.blockCnt.p, J.colStart.p, J.perm,
-                                      J.rowOf);
+    if (!place) {
+      const int nbc = std::min(nblk(J.numEdges), CP_BLOCKS);
+      k_color_count<<<nbc, TPB, 0, s>>>(J.colour, J.numEdges, J.blockCnt.p);
+      k_color_offsets<<<64, TPB, 0, s>>>(J.blockCnt.p, nbc, J.colCnt.p);
+      k_color_starts<<<1, 64, 0, s>>>(J.colCnt.p, J.colStart.p);
+      k_color_place<<<nbc, TPB, 0, s>>>(J.colour, J.numEdges, J.blockCnt.p, J.colStart.p, J.perm,
+                                        J.rowOf);
+    }
     HIPCHECK(hipGetLastError());
     PostBatch rb = postBatch(J.pinned, J.pinnedDev, 256);
     J.hostCnt = rb.add(J.cnt.p, B + 1);
@@ -1497,7 +1503,8 @@ struct pd_sim {
     jobV.work[0].ensure(nm + 1, st);
     jobV.work[1].ensure(nm + 1, st);
     refreshDev();
-    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(dD, nm, jobV.work[0].p, counters.p + 1);
+    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(dD, nm, jobV.work[0].p, counters.p + 1,
+                                                  vRowOf.p);
     std::vector<int> vs;
     int numColors = 0;
     int velLeftover = 0;
