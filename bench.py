@@ -20,9 +20,9 @@ The JSON line carries:
                   k_np_pairs          (write pass) 156 per pair processed (pair, offset, both
                                       bodies' pose and scale) + 104 per contact written; for
                                       point shapes it processes only the pairs in contact
-                  k_predict           340 per real (bc, vw0, force, pos, scale in; vw,
-                                      posPred, quatPred, real index out; grounded level
-                                      read and written)
+                  k_predict           344 per real (bc, vw0, force, pos, scale in; vw,
+                                      posPred, quatPred, real index, ghost count out;
+                                      grounded level read and written)
                   k_reintegrate       344 per slot (vw0, vw, bc, pos in; vw0, posPred.xyz,
                                       quatPred out)
                   k_vsolve_lds        601 per manifold row per sweep (one-workgroup loop)
@@ -81,7 +81,7 @@ def kernel_bytes(name, units, log):
     if name == "k_np_pairs":
         return 156 * units + 104 * sum(s["num_contacts"] for s in log)
     if name == "k_predict":
-        return 340 * units  # (the pour steps with gravity: the grounded-level decay is in it)
+        return 344 * units  # (the pour steps with gravity: the grounded-level decay is in it)
     if name == "k_reintegrate":
         return 344 * units
     raise KeyError(name)

@@ -374,10 +374,17 @@ __global__ void k_growth(double* scale, const double* target, double factor, int
   if (i < n) scale[i] = target[i] * factor;
 }
 
-// The per-substep control words the predict launch clears first (one launch fewer): the
+__device__ __forceinline__ void ghostShifts(const Params& P, d3 p, int& sx, int& sy, int& sz) {
+  const double skin = P.band;
+  sx = P.px ? ((p.x < P.dmin.x + skin) ? 1 : ((p.x > P.dmax.x - skin) ? -1 : 0)) : 0;
+  sy = P.py ? ((p.y < P.dmin.y + skin) ? 1 : ((p.y > P.dmax.y - skin) ? -1 : 0)) : 0;
+  sz = P.pz ? ((p.z < P.dmin.z + skin) ? 1 : ((p.z > P.dmax.z - skin) ? -1 : 0)) : 0;
+}
+// The per-substep control words the predict launch clears first (launches fewer): the
 // candidate-check flag, the colouring counters, the support-list count, the sweep-loop flags and
-// counters, the overlap accumulator; and the grounded-level decay of the substep
-// (src/contact_preprocessing.hpp:213-253, first step), when `ground` is given.
+// counters, the overlap accumulator; the grounded-level decay of the substep
+// (src/contact_preprocessing.hpp:213-253, first step) when `ground` is given; and each real's
+// ghost count (as k_ghost_count).
 struct SubstepStart {
   int* candFlag;
   int* counters;
@@ -387,6 +394,7 @@ struct SubstepStart {
   unsigned long long* overlapBits;
   int* ground;  // nullptr: no grounded levels this substep
   int decay;
+  int* ghostCnt;  // ghost count per real (src/periodicity.hpp:43-94, pass 1), from pos
 };
 // predictVelocityKokkos, src/integration.hpp:44-93 (+ realIndices and broadphase radius fill)
 __global__ void k_predict(const Dev* __restrict__ Dp, Params P, SubstepStart S) {
@@ -407,6 +415,11 @@ __global__ void k_predict(const Dev* __restrict__ Dp, Params P, SubstepStart S) 
   if (S.ground) {
     const int v = S.ground[i] - S.decay;
     S.ground[i] = v > 0 ? v : 0;
+  }
+  {
+    int sx, sy, sz;
+    ghostShifts(P, xyz(D.pos[i]), sx, sy, sz);
+    S.ghostCnt[i] = (sx ? 2 : 1) * (sy ? 2 : 1) * (sz ? 2 : 1) - 1;
   }
   const BodyC b = D.bc[i];
   const Body64 v0 = D.vw0[i];
@@ -456,12 +469,6 @@ __global__ void k_overlap_prep(const Dev* __restrict__ Dp, Params P) {
   D.realIdx[i] = i;
 }
 
-__device__ __forceinline__ void ghostShifts(const Params& P, d3 p, int& sx, int& sy, int& sz) {
-  const double skin = P.band;
-  sx = P.px ? ((p.x < P.dmin.x + skin) ? 1 : ((p.x > P.dmax.x - skin) ? -1 : 0)) : 0;
-  sy = P.py ? ((p.y < P.dmin.y + skin) ? 1 : ((p.y > P.dmax.y - skin) ? -1 : 0)) : 0;
-  sz = P.pz ? ((p.z < P.dmin.z + skin) ? 1 : ((p.z > P.dmax.z - skin) ? -1 : 0)) : 0;
-}
 // generateGhostsKokkos, src/periodicity.hpp:43-94 — pass 1: ghost count per real
 __global__ void k_ghost_count(const Dev* __restrict__ Dp, Params P, int* cnt) {
   const Dev& D = *Dp;
@@ -1338,13 +1345,43 @@ __global__ __launch_bounds__(TPB) void k_mreduce(const Dev* __restrict__ Dp, int
 // Jones-Plassmann colouring — src/solver_velocity.hpp:277-374, src/solver_position.hpp:155-241
 // ============================================================================================
 // uncoloured manifolds (-1) are appended to the first work list; inactive / duplicate ones get -2
-// (also rowOf[m] = -1: rows are placed for the coloured manifolds only)
+__device__ __forceinline__ unsigned long long realKey(int realA, int realB) {
+  const unsigned a = (unsigned)realA;
+  const unsigned b = realB >= 0 ? (unsigned)realB : 0xFFFFFFFFu;
+  const unsigned hi = a < b ? a : b, lo = a < b ? b : a;
+  return ((unsigned long long)hi << 32) | lo;
+}
+// a manifold's ledger key (mKey) and support code (mCode: 3 wall, 1 / 2 body B below / above
+// A along gravity, 0 otherwise), src/contact_preprocessing.hpp:70-78,213-253; returns the code
+__device__ __forceinline__ unsigned char manifoldKeyCode(const Dev& D, int m, d3 gHat) {
+  unsigned char code = 0;
+  if (D.mNp[m] <= 0) {
+    D.mKey[m] = ~0ull;
+  } else {
+    const int2 ab = D.mAB[m];
+    const int2 r = D.mReal[m];
+    D.mKey[m] = realKey(r.x, r.y);
+    if (ab.y < 0) {
+      code = 3;
+    } else {
+      const d3 dx = sub3(xyz(D.posPred[ab.x]), xyz(D.posPred[ab.y]));
+      const double up = -(dx.x * gHat.x + dx.y * gHat.y + dx.z * gHat.z);
+      const double thr = 0.3 * sqrt(dot3(dx, dx));
+      code = (up > thr) ? 1 : ((up < -thr) ? 2 : 0);
+    }
+  }
+  D.mCode[m] = code;
+  return code;
+}
+// (also rowOf[m] = -1: rows are placed for the coloured manifolds only; and each manifold's
+// ledger key and support code with the support list, as k_mkeys, in the same pass)
 __global__ void k_jp_init_manifolds(const Dev* __restrict__ Dp, int numManifolds, int* work, int* count,
-                                    int* rowOf) {
+                                    int* rowOf, d3 gHat, int* supList, int* supCount) {
   const Dev& D = *Dp;
-  __shared__ int shCnt, shBase;
+  __shared__ int shCnt, shBase, shCnt2, shBase2;
   const int m = gtid();
   int c = -2;
+  unsigned char code = 0;
   if (m < numManifolds) {
     const int2 r = D.mReal[m];
     c = -1;
@@ -1352,8 +1389,10 @@ __global__ void k_jp_init_manifolds(const Dev* __restrict__ Dp, int numManifolds
     else if (D.mAB[m].y >= 0 && r.x > r.y) c = -2;
     D.mColor[m] = c;
     rowOf[m] = -1;
+    code = manifoldKeyCode(D, m, gHat);
   }
   blockAppend(c == -1, m, work, count, &shCnt, &shBase);
+  blockAppend(code != 0, m, supList, supCount, &shCnt2, &shBase2);
 }
 // One JP round over the compacted work list.  winner[] holds, per body, 1 + the largest priority
 // among its uncoloured edges; a committing edge clears the slots of its two bodies, a losing edge
@@ -1445,7 +1484,8 @@ __global__ __launch_bounds__(JPS_THREADS) void k_jp_small(const int2* ends, int*
                                                           unsigned long long* winnerG,
                                                           unsigned long long* maskG,
                                                           int loadGlobal, int place, int* perm,
-                                                          int* rowOf, int* colStart) {
+                                                          int* rowOf, int* colStart,
+                                                          const int* src) {
   __shared__ int hKey[JPS_HASH];
   __shared__ unsigned long long hWin[JPS_HASH], hMask[JPS_HASH];
   __shared__ int eId[JPS_EDGES];
@@ -1455,7 +1495,14 @@ __global__ __launch_bounds__(JPS_THREADS) void k_jp_small(const int2* ends, int*
   __shared__ int cCnt[64], cBase[65];
   __shared__ int sNext, sFail, sUsed;
   const int T = blockDim.x, tid = threadIdx.x;
-  const int n0 = cnt[j0];
+  // src (a colouring that starts here): the entering count, and the round counters this launch
+  // owns start from it (k_jp_counts is not launched then)
+  const int n0 = src ? *src : cnt[j0];
+  if (src) {
+    if (tid == 0) cnt[j0] = n0;
+    for (int q = j0 + 1 + tid; q <= j1; q += T) cnt[q] = 0;
+    __syncthreads();
+  }
   if (n0 > JPS_EDGES) {
     if (tid == 0) cnt[j0 + 1] = -1;
     return;
@@ -1673,42 +1720,11 @@ __device__ __forceinline__ int ledgerFind(const unsigned long long* keys, const 
   }
   return -1;
 }
-__device__ __forceinline__ unsigned long long realKey(int realA, int realB) {
-  const unsigned a = (unsigned)realA;
-  const unsigned b = realB >= 0 ? (unsigned)realB : 0xFFFFFFFFu;
-  const unsigned hi = a < b ? a : b, lo = a < b ? b : a;
-  return ((unsigned long long)hi << 32) | lo;
-}
 
 // per manifold: ledger key (markPersistentManifoldsKokkos semantics, duplicates keep their key)
 // and the support relation of updateGroundedLevelsKokkos, src/contact_preprocessing.hpp:78-253
 // keys, support relation (0 none, 1 B supports A, 2 A supports B, 3 boundary) and the compacted
 // list of support manifolds (the only ones the grounded-level sweeps visit)
-__global__ void k_mkeys(const Dev* __restrict__ Dp, int numManifolds, d3 gHat, int* supList, int* supCount) {
-  const Dev& D = *Dp;
-  __shared__ int shCnt, shBase;
-  const int m = gtid();
-  unsigned char code = 0;
-  if (m < numManifolds) {
-    if (D.mNp[m] <= 0) {
-      D.mKey[m] = ~0ull;
-    } else {
-      const int2 ab = D.mAB[m];
-      const int2 r = D.mReal[m];
-      D.mKey[m] = realKey(r.x, r.y);
-      if (ab.y < 0) {
-        code = 3;
-      } else {
-        const d3 dx = sub3(xyz(D.posPred[ab.x]), xyz(D.posPred[ab.y]));
-        const double up = -(dx.x * gHat.x + dx.y * gHat.y + dx.z * gHat.z);
-        const double thr = 0.3 * sqrt(dot3(dx, dx));
-        code = (up > thr) ? 1 : ((up < -thr) ? 2 : 0);
-      }
-    }
-    D.mCode[m] = code;
-  }
-  blockAppend(code != 0, m, supList, supCount, &shCnt, &shBase);
-}
 
 // velocity rows: gather the manifold (colour-sorted), warm start, persistence, vn0 / vt0
 // (gatherWarmLambdaKokkos :113, markPersistentManifoldsKokkos :78, computeVn0Kokkos :390)

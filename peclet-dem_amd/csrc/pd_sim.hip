@@ -1136,8 +1136,10 @@ struct pd_sim {
     // after this batch the global slots are valid for every body a later round can touch: zeroed
     // above, or written back by the LDS kernel for the bodies of the edges it leaves
     J.globalDirty = false;
-    k_jp_counts<<<1, 64, 0, s>>>(J.cnt.p, J.first ? J.enteringDev : nullptr, (int)J.entering,
-                                 B + 1);
+    // (a colouring that starts in the one-workgroup kernel sets its counters there)
+    if (!(place && J.enteringDev))
+      k_jp_counts<<<1, 64, 0, s>>>(J.cnt.p, J.first ? J.enteringDev : nullptr, (int)J.entering,
+                                   B + 1);
     for (int j = 0; j < B; ++j) {
       const long est = estimate(j);
       int* work = J.work[(J.cur + j) & 1].p;
@@ -1146,7 +1148,8 @@ struct pd_sim {
         J.smallLoad = j == 0 ? loadGlobal : 1;
         k_jp_small<<<1, JPS_THREADS, 0, s>>>(J.ends, J.colour, work, next, J.cnt.p, j, B,
                                              J.winner.p, J.mask.p, J.smallLoad, place ? 1 : 0,
-                                             J.perm, J.rowOf, J.colStart.p);
+                                             J.perm, J.rowOf, J.colStart.p,
+                                             place ? J.enteringDev : nullptr);
         break;
       }
       const int grid = (int)std::min<long>(nblk(est + est / 4 + 64), JP_GRID_CAP);
@@ -1383,18 +1386,19 @@ struct pd_sim {
     const double margin = 0.1 * maxRad;
     numParticles = numReal;
     Params P = params(margin, maxRad);
-    SubstepStart S0{candFlag.p, counters.p, supCnt.p, ctl.p, CTL_COUNT, bits.p, nullptr, 8};
+    // periodic ghosts, src/periodicity.hpp:43-94: counted in the predict launch.  The ghost total
+    // is read back together with the pair total: the broadphase runs on the device-side slot
+    // count, and if the ghosts did not fit the slot buffers they grow and the ghosts and the
+    // broadphase run again.
+    ghostCnt.ensure(numReal + 1, st);
+    ghostOff.ensure(numReal + 1, st);
+    slotInfo.ensure(4, st);
+    SubstepStart S0{candFlag.p, counters.p, supCnt.p, ctl.p, CTL_COUNT, bits.p, nullptr, 8,
+                    ghostCnt.p};
     if (usePersist && useGS) S0.ground = ground.p;  // the grounded levels are updated this substep
     kernBegin();
     k_predict<<<std::max(1, nblk(numReal)), TPB, 0, st>>>(dD, P, S0);
     kernEnd(K_PREDICT, numReal);
-    // periodic ghosts, src/periodicity.hpp:43-94.  The ghost total is read back together with
-    // the pair total: the broadphase runs on the device-side slot count, and if the ghosts did
-    // not fit the slot buffers they grow and the ghosts and the broadphase run again.
-    ghostCnt.ensure(numReal + 1, st);
-    ghostOff.ensure(numReal + 1, st);
-    slotInfo.ensure(4, st);
-    k_ghost_count<<<nblk(numReal), TPB, 0, st>>>(dD, P, ghostCnt.p);
     scanQuiet(ghostCnt.p, ghostOff.p, numReal);
     const Params PB = params(margin, 0.0);
     // candidate pairs kept across substeps (grown boxes): used while they provably hold every
@@ -1503,8 +1507,15 @@ struct pd_sim {
     jobV.work[0].ensure(nm + 1, st);
     jobV.work[1].ensure(nm + 1, st);
     refreshDev();
-    k_jp_init_manifolds<<<nblk(nm), TPB, 0, st>>>(dD, nm, jobV.work[0].p, counters.p + 1,
-                                                  vRowOf.p);
+    // (the same pass writes each manifold's ledger key and support code and the support list)
+    supList.ensure(nm + 1, st);
+    supCnt.ensure(1, st);
+    refreshDev();
+    const double gMag0 = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    const d3 gHat0 = (usePersist && gMag0 > 0.0) ? d3{g[0] / gMag0, g[1] / gMag0, g[2] / gMag0}
+                                                 : d3{0, 0, 0};
+    k_jp_init_manifolds<<<std::max(1, nblk(nm)), TPB, 0, st>>>(
+        dD, nm, jobV.work[0].p, counters.p + 1, vRowOf.p, gHat0, supList.p, supCnt.p);
     std::vector<int> vs;
     int numColors = 0;
     int velLeftover = 0;
@@ -1532,9 +1543,7 @@ struct pd_sim {
     vRB.ensure(R, st); vVt0.ensure(R, st); vPre.ensure(R, st); vLamT.ensure(R, st); vVn0.ensure(R, st);
     vLam.ensure(R, st); vFlag.ensure(R, st); vWV.ensure(R, st);
     refreshDev();
-    supList.ensure(nm + 1, st);
-    supCnt.ensure(1, st);
-    if (nm > 0) k_mkeys<<<nblk(nm), TPB, 0, st>>>(dD, nm, gHat, supList.p, supCnt.p);
+
     if (nRowsV > 0)
       k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(dD, P, nRowsV, prevCount > 0 ? ledgerBuckets : 0,
                                             usePGS ? 1 : 0);
