@@ -976,26 +976,24 @@ __device__ __forceinline__ bool candMoved(const double4& B, const double4& R, do
   return !(fabs(B.x - R.x) <= lim && fabs(B.y - R.y) <= lim && fabs(B.z - R.z) <= lim &&
            B.w <= R.w + lim);
 }
-// flag = 1 when a ghost slot moved or the slot count changed (the reals are checked by
-// k_cand_count)
-__global__ __launch_bounds__(TPB) void k_cand_check(const Dev* __restrict__ Dp, Params P, const int* nSlots,
-                                                    int refSlots, const double4* ref, double lim,
-                                                    int* flag) {
-  const Dev& D = *Dp;
-  const int n = *nSlots;
-  bool bad = (n != refSlots);
-  for (int s = P.numReal + gtid(); s < n && !bad; s += gridDim.x * blockDim.x)
-    bad = candMoved(saneBox(D.posPred[s], P.margin, P.boxCap), ref[s], lim);
-  if (bad) atomicOr(flag, 1);
-}
 // the exact test on every candidate of real i (the broadphase's own box test); keep[c] marks
-// the pairs, pairCnt[i] counts them.  With a flag, real i's own box and images are checked too.
+// the pairs, pairCnt[i] counts them.  With a flag, real i's own box and images are checked too,
+// and (nSlots given) flag = 1 also when a ghost slot moved or the slot count changed.
 __global__ __launch_bounds__(TPB) void k_cand_count(const Dev* __restrict__ Dp, Params P, const int2* cand,
                                                     const int* candStart, unsigned char* keep,
                                                     const double4* ref, const unsigned char* code,
-                                                    double lim, int* flag) {
+                                                    double lim, int* flag,
+                                                    const int* nSlots = nullptr,
+                                                    int refSlots = 0) {
   const Dev& D = *Dp;
   const int i = gtid();
+  if (nSlots) {
+    const int ns = *nSlots;
+    bool bad = (ns != refSlots);
+    for (int s = P.numReal + i; s < ns && !bad; s += gridDim.x * blockDim.x)
+      bad = candMoved(saneBox(D.posPred[s], P.margin, P.boxCap), ref[s], lim);
+    if (bad) atomicOr(flag, 1);
+  }
   if (i >= P.numReal) return;
   const int b = candStart[i], e = candStart[i + 1];
   int n = 0;

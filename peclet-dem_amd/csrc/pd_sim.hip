@@ -1420,11 +1420,10 @@ struct pd_sim {
         pairCnt.ensure(numReal + 1, st);
         pairStart.ensure(numReal + 1, st);
         refreshDev();
-        k_cand_check<<<std::min(nblk(lastGhosts + 1024), 1024), TPB, 0, st>>>(
-            dD, P, slotInfo.p, candSlots, candRef.p, 0.25 * skin, candFlag.p);
-        k_cand_count<<<nblk(numReal), TPB, 0, st>>>(dD, P, cand.p, candStart.p, candKeep.p,
-                                                    candRef.p, candCode.p, 0.25 * skin,
-                                                    candFlag.p);
+        // (the same launch checks the ghost slots)
+        k_cand_count<<<std::max(1, nblk(numReal)), TPB, 0, st>>>(
+            dD, P, cand.p, candStart.p, candKeep.p, candRef.p, candCode.p, 0.25 * skin,
+            candFlag.p, slotInfo.p, candSlots);
         scanQuiet(pairCnt.p, pairStart.p, numReal);
       } else {
         Params PS = PB;
