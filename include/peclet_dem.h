@@ -185,6 +185,33 @@ pd_sim* pd_batch_system(pd_batch* b, int k); /* owned by the batch */
 int pd_batch_set_workers(pd_batch* b, int workers);
 int pd_batch_step(pd_batch* b, double dt, int nsteps); /* nsteps substeps of every system */
 
+/* --- differentiation of the substep (extension: the reference has no adjoint) -------------
+ * State layout, per real body, 13 doubles: position (3), velocity (3), angular velocity (3),
+ * orientation quaternion (x, y, z, w).  Supported while recording: spheres (no shell, no grid
+ * SDF), planes, periodic axes, gravity, the coloured Gauss-Seidel velocity path with the
+ * Coulomb cone, stabilization modes off / one-sided; no SDF walls, pair-material table, growth
+ * or thermostat (pd_step returns an error naming the feature).  Masses, inertia, scales and
+ * external forces must not change between pd_tape_begin and the last derivative call.
+ *
+ * pd_tape_begin opens a recording (dropping any earlier one): every following substep stores
+ * its discrete structure (contacts, manifolds, colour orders, sweep counts, stabilization
+ * flags, warm-start map), and every `checkpoint_every` substeps the body state and the
+ * warm-start ledger.  pd_tape_end closes it. */
+int pd_tape_begin(pd_sim* s, int checkpoint_every);
+int pd_tape_end(pd_sim* s);
+int pd_tape_length(pd_sim* s); /* recorded substeps */
+/* replay of the recorded substeps from the first checkpoint (equals the stepped state) */
+int pd_tape_replay(pd_sim* s, double* state_out);
+/* forward mode: the end state and its directional derivative along (dstate0, dmu, drest),
+ * where mu and rest are the friction coefficient and normal restitution; dstate0 may be NULL */
+int pd_tape_tangent(pd_sim* s, const double* dstate0, double dmu, double drest, double* state_out,
+                    double* dstate_out);
+/* reverse mode (adjoint): given dL/d(end state), returns dL/d(start state) and
+ * dL/d(mu, rest) in grad_params[2]; checkpointed segments are replayed, then reversed
+ * substep by substep (sweeps in reverse colour order) */
+int pd_tape_adjoint(pd_sim* s, const double* grad_state_end, double* grad_state0,
+                    double* grad_params);
+
 /* --- measurement ------------------------------------------------------------------------ */
 /* Timing with HIP events on the handle's stream (no reference counterpart; the reference's
  * get_profiling_info returns an empty dict, src/dem_bindings.cpp:385): level 0 off, 1 per-stage

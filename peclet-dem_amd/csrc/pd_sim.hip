@@ -21,6 +21,7 @@
 #include <thread>
 #include <vector>
 
+#include "pd_adjoint.hpp"
 #include "pd_hertz.hpp"
 #include "pd_kernels.hpp"
 #include "peclet_dem.h"
@@ -92,6 +93,9 @@ const char* kStageNames[ST_COUNT] = {"predict_ghosts", "broadphase",   "narrowph
                                      "vel_sweeps",     "stabilization", "ledger",
                                      "reintegrate",    "color_pos",    "pos_sweeps",
                                      "commit"};
+
+struct AdjTape;
+void adjFree(AdjTape* t);
 
 }  // namespace
 
@@ -275,6 +279,15 @@ struct pd_sim {
 
   Dev D{};
 
+  // recording of substeps for the adjoint (pd_adjoint_host.hpp); null when never opened
+  AdjTape* tape = nullptr;
+  bool taping = false;  // a tape is open: every substep is recorded
+  void tapeCheck(bool usePGS, bool legacyFriction) const;
+  void tapeSubstepStart(bool usePGS, bool legacyFriction);
+  void tapeSubstepEnd(const Params& P, const std::vector<int>& vs, int numColors,
+                      const std::vector<int>& ps, int numPosColors, int nRowsP, int velLeftover,
+                      int posLeftover);
+
   hipStream_t sideStream = nullptr;  // default priority, like st
   hipEvent_t evSide = nullptr;      // side stream done -> main stream
   hipEvent_t evContacts = nullptr;  // contact list built -> side stream
@@ -322,6 +335,7 @@ struct pd_sim {
   ~pd_sim() {
     (void)hipStreamSynchronize(st);
     (void)hipStreamSynchronize(sideStream);
+    adjFree(tape);
     DBufReleaseAll();
     for (const KernRec& r : kernPending) {
       (void)hipEventDestroy(r.a);
@@ -1359,6 +1373,7 @@ struct pd_sim {
     const bool legacyFriction = frictionOn && !(usePersist && useGS);
     mark(ST_PREDICT);
     refreshDev();
+    if (taping) tapeSubstepStart(usePersist && useGS, legacyFriction);
     // growth ramp, src/sim.hpp:100-108
     if (growthFactor != -1.0 && growthRate != 0.0) {
       growthFactor *= std::exp(growthRate * dt);
@@ -1907,6 +1922,8 @@ struct pd_sim {
     k_final_commit<<<nblk(numReal), TPB, 0, st>>>(dD, P);
     if (thermoTau > 0.0 && dt > 0.0) applyThermostat();
     HIPCHECK(hipGetLastError());
+    if (taping)
+      tapeSubstepEnd(P, vs, numColors, ps, numPosColors, nRowsP, velLeftover, posLeftover);
     mark(ST_COUNT);
     stats.num_real = numReal;
     stats.num_particles = numParticles;
@@ -2152,6 +2169,9 @@ struct pd_sim {
 // ================================================================================================
 // C-ABI
 // ================================================================================================
+
+#include "pd_adjoint_host.hpp"
+
 namespace {
 template <class F>
 int guard(F&& f) {
@@ -2671,6 +2691,59 @@ int pd_step_n(pd_sim* s, double dt, int n) {
     for (int k = 0; k < n; ++k) {
       s->dt = dt;
       s->demStep();
+    }
+  });
+}
+int pd_tape_begin(pd_sim* s, int every) {
+  return guard([&] {
+    if (every < 1) throw ArgError("checkpoint_every must be >= 1");
+    s->sync();
+    if (!s->tape) s->tape = new AdjTape();
+    s->tape->clear();
+    s->tape->every = every;
+    s->tape->on = true;
+    s->taping = true;
+  });
+}
+int pd_tape_end(pd_sim* s) {
+  return guard([&] {
+    if (!s->tape || !s->tape->on) throw ArgError("no open tape (pd_tape_begin)");
+    s->sync();
+    s->tape->on = false;
+    s->taping = false;
+  });
+}
+int pd_tape_length(pd_sim* s) { return s->tape ? (int)s->tape->steps.size() : 0; }
+int pd_tape_replay(pd_sim* s, double* state_out) {
+  return guard([&] {
+    std::vector<double> out;
+    adjrun::replayAll(*s, s->tape, out);
+    if (state_out) std::memcpy(state_out, out.data(), out.size() * sizeof(double));
+  });
+}
+int pd_tape_tangent(pd_sim* s, const double* dstate0, double dmu, double drest, double* state_out,
+                    double* dstate_out) {
+  return guard([&] {
+    const size_t n = s->tape ? (size_t)13 * s->tape->numReal : 0;
+    std::vector<double> d0(n, 0.0), v, t;
+    if (dstate0) std::memcpy(d0.data(), dstate0, n * sizeof(double));
+    adjrun::tangentAll(*s, s->tape, d0, dmu, drest, v, t);
+    if (state_out) std::memcpy(state_out, v.data(), n * sizeof(double));
+    if (dstate_out) std::memcpy(dstate_out, t.data(), n * sizeof(double));
+  });
+}
+int pd_tape_adjoint(pd_sim* s, const double* grad_state_end, double* grad_state0,
+                    double* grad_params) {
+  return guard([&] {
+    if (!grad_state_end) throw ArgError("grad_state_end is required");
+    const size_t n = s->tape ? (size_t)13 * s->tape->numReal : 0;
+    std::vector<double> gE(grad_state_end, grad_state_end + n), g0;
+    double gp[2] = {0.0, 0.0};
+    adjrun::adjointAll(*s, s->tape, gE, g0, gp);
+    if (grad_state0) std::memcpy(grad_state0, g0.data(), n * sizeof(double));
+    if (grad_params) {
+      grad_params[0] = gp[0];
+      grad_params[1] = gp[1];
     }
   });
 }

@@ -451,6 +451,44 @@ class Simulation:
         """Extension: ``n`` substeps of size ``dt`` with no host round trip between them."""
         self._chk(self._b.step_n(self._h, float(dt), int(n)))
 
+    # ------------------------------------------------------------------ differentiation
+    # Extension (no counterpart in the reference): the state of a body is 13 numbers, position,
+    # velocity, angular velocity and orientation (x, y, z, w); see ``peclet.dem.adjoint``.
+    def tape_begin(self, checkpoint_every=16):
+        """Record the following substeps for ``tape_tangent`` / ``tape_adjoint``."""
+        self._chk(self._b.tape_begin(self._h, int(checkpoint_every)))
+
+    def tape_end(self):
+        self._chk(self._b.tape_end(self._h))
+
+    def tape_length(self):
+        return int(self._b.tape_length(self._h))
+
+    def tape_replay(self):
+        """End state (n, 13) of the recorded substeps replayed from the first checkpoint."""
+        out = np.empty((self.num_particles(), 13), np.float64)
+        self._chk(self._b.tape_replay(self._h, _ptr(out)))
+        return out
+
+    def tape_tangent(self, dstate0=None, dmu=0.0, drest=0.0):
+        """Forward mode: (end state, its derivative along dstate0 (n, 13), dmu, drest)."""
+        n = self.num_particles()
+        d0 = None if dstate0 is None else _f64(dstate0, 13)
+        val = np.empty((n, 13), np.float64)
+        tan = np.empty((n, 13), np.float64)
+        self._chk(self._b.tape_tangent(self._h, None if d0 is None else _ptr(d0), float(dmu),
+                                       float(drest), _ptr(val), _ptr(tan)))
+        return val, tan
+
+    def tape_adjoint(self, grad_state_end):
+        """Reverse mode: (dL/d start state (n, 13), dL/d (friction, restitution)) for
+        grad_state_end = dL/d end state (n, 13)."""
+        g = _f64(grad_state_end, 13)
+        g0 = np.empty_like(g)
+        gp = np.zeros(2, np.float64)
+        self._chk(self._b.tape_adjoint(self._h, _ptr(g), _ptr(g0), _ptr(gp)))
+        return g0, gp
+
     def compute_overlaps(self):
         out = C.c_double(0.0)
         self._chk(self._b.compute_overlaps(self._h, C.byref(out)))

@@ -140,6 +140,12 @@ PRODUCT_SIGNATURES = {
     "batch_set_workers": (_i, [_p, _i]),
     "batch_step": (_i, [_p, _d, _i]),
     "rsa_place": (C.c_long, [_dp, C.c_long, _dp, _i, _i, _dp, _dp, _dp, _ip]),
+    "tape_begin": (_i, [_p, _i]),
+    "tape_end": (_i, [_p]),
+    "tape_length": (_i, [_p]),
+    "tape_replay": (_i, [_p, _dp]),
+    "tape_tangent": (_i, [_p, _dp, _d, _d, _dp, _dp]),
+    "tape_adjoint": (_i, [_p, _dp, _dp, _dp]),
 }
 
 

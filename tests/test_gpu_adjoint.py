@@ -1,0 +1,185 @@
+"""Derivatives of recorded substeps: replay, forward mode, adjoint (GPU).
+
+* The replay of a recording reproduces the stepped state bit for bit (the replay re-evaluates
+  every continuous value of the substep with the recorded discrete structure).
+* The adjoint is the transpose of the forward-mode (dual-number) run of the same substeps:
+  <gE, J d0> = <J^T gE, d0> to 1e-9 relative to the sum of the terms' magnitudes.
+* The adjoint matches central finite differences (step 1e-6) of the CPU oracle stepped from the
+  same state: 1e-4 relative, for single-body and random initial-velocity directions, friction
+  and restitution, wherever the loss is smooth over the step.
+"""
+import numpy as np
+import pytest
+
+from oracle_binding import oracle_sim
+from scenes import apply_scene, make_scene
+
+
+def hip_sim(capacity):
+    from peclet.dem import Simulation
+
+    return Simulation(capacity)
+
+
+def state(sim):
+    return np.concatenate([sim.get_positions(), sim.get_velocities(),
+                           sim.get_angular_velocities(), sim.get_quaternions()], axis=1)
+
+
+def recorded(kind, n, pre, steps, every, seed=0):
+    scene = make_scene(kind, n, seed)
+    h = apply_scene(hip_sim(n), scene)
+    for _ in range(pre):
+        h.step(scene["dt"])
+    h.tape_begin(every)
+    for _ in range(steps):
+        h.step(scene["dt"])
+    h.tape_end()
+    assert h.tape_length() == steps
+    return scene, h
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,every", [("bed", 4), ("bed_periodic", 3), ("bed_periodic", 16)])
+def test_tape_replay_is_the_stepped_state(hip_lib, kind, every):
+    print(kind, every, flush=True)
+    scene, h = recorded(kind, 300, 4, 10, every)
+    assert h.get_stats()["num_contacts"] > 0
+    rep = h.tape_replay()
+    assert np.array_equal(rep, state(h))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["bed", "bed_periodic"])
+def test_adjoint_is_the_transpose_of_the_tangent(hip_lib, kind):
+    print(kind, flush=True)
+    scene, h = recorded(kind, 300, 4, 10, 4)
+    n = h.num_particles()
+    rng = np.random.default_rng(1)
+    d0 = np.zeros((n, 13))
+    d0[:, 0:3] = rng.normal(size=(n, 3)) * 1e-5
+    d0[:, 3:6] = rng.normal(size=(n, 3)) * 1e-2
+    d0[:, 6:9] = rng.normal(size=(n, 3))
+    dmu, de = 0.3, -0.2
+    val, tan = h.tape_tangent(d0, dmu, de)
+    assert np.array_equal(val, state(h))
+    gE = rng.normal(size=(n, 13))
+    g0, gp = h.tape_adjoint(gE)
+    lhs = np.sum(gE * tan)
+    terms = np.concatenate([(g0 * d0).ravel(), [gp[0] * dmu, gp[1] * de]])
+    rhs = terms.sum()
+    scale = np.abs(gE * tan).sum() + np.abs(terms).sum()
+    assert abs(lhs - rhs) <= 1e-9 * scale, (lhs, rhs, scale)
+    assert abs(gp[0]) > 0.0 and abs(gp[1]) > 0.0
+
+
+def fd_directions(n, rng):
+    """Single-particle velocity directions (both components of a few bodies) and two random
+    directions over every body."""
+    dirs = []
+    for i in (0, n // 4, n // 2, 3 * n // 4, n - 1):
+        for c in (0, 2):
+            d = np.zeros((n, 3))
+            d[i, c] = 1.0
+            dirs.append((f"v[{i},{c}]", d))
+    for k in range(2):
+        d = rng.normal(size=(n, 3))
+        dirs.append((f"random{k}", d / np.linalg.norm(d)))
+    return dirs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,friction", [("bed", 0.5), ("bed_periodic", 0.5), ("bed", 0.0)])
+def test_adjoint_matches_central_differences_of_the_oracle(hip_lib, kind, friction):
+    # Contact dynamics is piecewise smooth: jumps (the restitution threshold, contact switching)
+    # and kinks where a constraint sits on its bound at the base point (a resting contact left at
+    # C = 0 by the position projection, an impulse at 0) make a central difference the mean of
+    # two one-sided slopes there.  A difference is trusted only where the loss is smooth over
+    # the step: both one-sided slopes at 1e-7 and the central differences at 1e-6 and 1e-7 agree.
+    n, pre, steps = 200, 4, 8
+    print(kind, friction, n, pre, steps, flush=True)
+    scene = make_scene(kind, n, 0)
+    e0, et0, _ = scene["material"]
+    scene["material"] = (e0, et0, friction)
+    h = apply_scene(hip_sim(n), scene)
+    for _ in range(pre):
+        h.step(scene["dt"])
+    h.tape_begin(4)
+    for _ in range(steps):
+        h.step(scene["dt"])
+    h.tape_end()
+    rng = np.random.default_rng(2)
+    gE = np.zeros((n, 13))
+    gE[:, 0:3] = rng.normal(size=(n, 3))
+    gE[:, 3:6] = rng.normal(size=(n, 3)) * 1e-3
+    g0, gp = h.tape_adjoint(gE)
+
+    def loss(dv=None, dmu=0.0, de=0.0):
+        r = apply_scene(oracle_sim(n), scene)
+        for _ in range(pre):
+            r.step(scene["dt"])
+        if dv is not None:
+            r.set_velocities(r.get_velocities() + dv)
+        r.set_material_params(e0 + de, et0, friction + dmu)
+        for _ in range(steps):
+            r.step(scene["dt"])
+        return float(np.sum(gE * state(r)))
+
+    f0 = loss()
+
+    checks = [(name, (lambda s, d=d: loss(dv=s * d)), float(np.sum(g0[:, 3:6] * d)))
+              for name, d in fd_directions(n, rng)]
+    checks.append(("restitution", lambda s: loss(de=s), gp[1]))
+    if friction > 0.0:
+        checks.append(("friction", lambda s: loss(dmu=s), gp[0]))
+    smooth = 0
+    for name, f, adj in checks:
+        p6, m6, p7, m7 = f(1e-6), f(-1e-6), f(1e-7), f(-1e-7)
+        fd6, fd7 = (p6 - m6) / 2e-6, (p7 - m7) / 2e-7
+        right, left = (p7 - f0) / 1e-7, (f0 - m7) / 1e-7
+        size = max(abs(fd6), abs(fd7), 1e-12)
+        ok_fd = abs(fd6 - fd7) <= 1e-3 * size and abs(right - left) <= 2e-3 * size
+        print(f"  {name:12s} fd {fd6:.9e} {fd7:.9e} one-sided {right:.6e} {left:.6e} "
+              f"adjoint {adj:.9e} smooth {ok_fd}", flush=True)
+        if not ok_fd:
+            continue
+        smooth += 1
+        assert abs(fd6 - adj) <= 1e-4 * abs(adj) + 1e-10, (name, fd6, adj)
+    assert smooth >= len(checks) // 2, f"only {smooth} of {len(checks)} directions smooth"
+
+
+@pytest.mark.gpu
+def test_torch_rollout_gradients(hip_lib):
+    import torch
+
+    from peclet.dem.adjoint import bed_height, packing_fraction_below, rollout
+
+    n = 200
+    scene = make_scene("bed_periodic", n, 0)
+    h = apply_scene(hip_sim(n), scene)
+    for _ in range(4):
+        h.step(scene["dt"])
+    v0 = torch.tensor(h.get_velocities(), dtype=torch.float64, requires_grad=True)
+    mu = torch.tensor(scene["material"][2], dtype=torch.float64, requires_grad=True)
+    e = torch.tensor(scene["material"][0], dtype=torch.float64, requires_grad=True)
+    end = rollout(h, v0, mu, e, scene["dt"], 8, checkpoint_every=4,
+                  restitution_tangent=scene["material"][1])
+    rad = h.get_scales() * 1e-3
+    L = scene["domain"][0]
+    loss = bed_height(end[:, 0:3]) + packing_fraction_below(end[:, 0:3], rad, L * L)
+    loss.backward()
+    # the same gradient through the C-ABI directly
+    gE = np.zeros((n, 13))
+    gE[:, 2] = 1.0 / n
+    H = 2.0 * end[:, 2].detach().numpy().mean()
+    inside = end[:, 2].detach().numpy() < 0.6 * H
+    vol = (4.0 / 3.0) * np.pi * rad ** 3
+    phi = (inside * vol).sum() / (L * L * 0.6 * H)
+    gE[:, 2] += -phi / H * (2.0 / n)
+    g0, gp = h.tape_adjoint(gE)
+    # two adjoint runs: their atomic accumulations may be summed in different orders
+    scale = np.abs(g0[:, 3:6]).max()
+    assert np.abs(v0.grad.numpy() - g0[:, 3:6]).max() <= 1e-9 * scale
+    assert np.isclose(mu.grad.item(), gp[0], rtol=1e-9)
+    assert np.isclose(e.grad.item(), gp[1], rtol=1e-9)
+    assert np.abs(v0.grad.numpy()).max() > 0.0
