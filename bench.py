@@ -203,6 +203,60 @@ def run_batched(num, seed, steps, seq_steps, workers):
                         "r~U[0.8,1.2] mm, mu~U[0.2,0.8], e~U[0.1,0.6], phi0=0.1, dt=1e-5 s"}
 
 
+def run_adjoint(n, substeps, every, warm, dense):
+    """Forward + adjoint form: particle-substeps/s of a recorded forward run plus its adjoint
+    (gradient of the mean bed height w.r.t. initial velocities, friction and restitution),
+    against the plain forward run of the same substeps from the same state."""
+    from peclet.dem import Simulation
+    from peclet.dem.workloads import lattice_bed, setup_gravity_bed, sphere_bed
+
+    dt = 1e-5
+    if dense:
+        bed = lattice_bed(n, 0.06, 5)
+        what = (f"{n} spheres r~U[0.99,1.01] mm on a cubic lattice resting on the floor "
+                "(contacts everywhere), 60x60 mm periodic xy")
+    else:
+        bed = sphere_bed(n, 0.9e-3, 1.1e-3, (0.06, 0.06, 0.2), 5, column=0.2)
+        what = (f"{n} spheres r~U[0.9,1.1] mm, RSA start in 60x60x200 mm periodic xy + floor, "
+                "v~N(0,0.01 m/s)")
+
+    def fresh():
+        sim = setup_gravity_bed(Simulation(n), bed, dt, 0.5, 0.3)
+        if not dense:
+            sim.set_velocities(np.random.default_rng(5).normal(0.0, 0.01, (n, 3)))
+        sim.step_n(dt, warm)
+        sim.synchronize()
+        return sim
+
+    sim = fresh()
+    t0 = time.perf_counter()
+    sim.step_n(dt, substeps)
+    sim.synchronize()
+    t_plain = time.perf_counter() - t0
+    sim.close()
+    sim = fresh()
+    t0 = time.perf_counter()
+    sim.tape_begin(every)
+    sim.step_n(dt, substeps)
+    sim.tape_end()
+    t_fwd = time.perf_counter() - t0
+    grad = np.zeros((n, 13))
+    grad[:, 2] = 1.0 / n
+    t0 = time.perf_counter()
+    g0, gp = sim.tape_adjoint(grad)
+    t_adj = time.perf_counter() - t0
+    contacts = sim.num_contacts()
+    sim.close()
+    return {"value": n * substeps / (t_fwd + t_adj), "unit": "particle-substeps/s",
+            "workload": what + f", dt=1e-5 s, mu=0.5, e=0.3, {warm} substeps before the "
+                               f"recording, {substeps} recorded, checkpoint every {every}",
+            "contacts_at_end": contacts, "forward_ms": 1e3 * t_plain,
+            "recorded_forward_ms": 1e3 * t_fwd, "adjoint_ms": 1e3 * t_adj,
+            "cost_vs_forward": (t_fwd + t_adj) / t_plain,
+            "grad_norm_v0": float(np.linalg.norm(g0[:, 3:6])),
+            "grad_mu": float(gp[0]), "grad_restitution": float(gp[1])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +282,9 @@ def main():
     ap.add_argument("--timing-level", type=int, default=2, choices=[0, 1, 2],
                     help="library timing in the timed region: 2 = per-kernel events and counter "
                          "log (needed for roofline), 1 = stage events only, 0 = none")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-n", type=int, default=50000)
+    ap.add_argument("--adjoint-substeps", type=int, default=16)
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=24)
     ap.add_argument("--batch-workers", type=int, default=16)
@@ -316,6 +373,15 @@ def main():
         print(f"# batched: {batched['value']:.3e} particle-substeps/s "
               f"({batched['speedup_vs_sequential']:.2f}x sequential)", file=sys.stderr, flush=True)
 
+    fwd_adjoint = None
+    if not args.no_adjoint:
+        fwd_adjoint = {
+            "loose_start": run_adjoint(args.adjoint_n, args.adjoint_substeps, 8, 8, False),
+            "dense_bed": run_adjoint(args.adjoint_n, args.adjoint_substeps, 8, 8, True)}
+        for k, v in fwd_adjoint.items():
+            print(f"# fwd+adjoint {k}: {v['value']:.3e} particle-substeps/s, "
+                  f"{v['cost_vs_forward']:.1f}x the forward time", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cps, tc = run_cpu_baseline(args.workload, bed, dt, args.cpu_substeps)
@@ -367,6 +433,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "fwd_adjoint": fwd_adjoint,
         "substep_roofline": {
             "algorithmic_bytes_per_substep": total_bytes / args.steps,
             "achieved_GBs": total_bytes / t / 1e9,

@@ -816,7 +816,7 @@ __device__ __forceinline__ void ldT(T* dst, const T* src, int n) {
 
 // ---- slots: copy the start state, count ghosts (src/periodicity.hpp:25-38)
 template <class T>
-__global__ void a_start(RB<T> B, RK K) {
+__global__ __launch_bounds__(TPB) void a_start(RB<T> B, RK K) {
   const int i = gtid();
   if (i >= K.P.numReal) return;
   const T* X = B.X + 13 * i;
@@ -836,7 +836,7 @@ __global__ void a_start(RB<T> B, RK K) {
 
 // ---- predict + ghosts (src/integration.hpp:44-93, src/periodicity.hpp:43-94)
 template <class T>
-__global__ void a_predict(RB<T> B, RK K, int nSlots) {
+__global__ __launch_bounds__(TPB) void a_predict(RB<T> B, RK K, int nSlots) {
   const int i = gtid();
   if (i >= K.P.numReal) return;
   const BodyC b = K.bc[i];
@@ -890,7 +890,7 @@ __device__ __forceinline__ double planeRadius(const RK& K, double sc) {
 
 // ---- contacts
 template <class T>
-__global__ void a_contacts(RB<T> B, RK K, Topo t) {
+__global__ __launch_bounds__(TPB) void a_contacts(RB<T> B, RK K, Topo t) {
   const int c = gtid();
   if (c >= t.nc) return;
   B.cMan[c] = -1;
@@ -922,7 +922,7 @@ __global__ void a_contacts(RB<T> B, RK K, Topo t) {
 
 // ---- manifold sums, contact order within the manifold (src/contact_preprocessing.hpp:378-471)
 template <class T>
-__global__ void a_manifolds(RB<T> B, Topo t) {
+__global__ __launch_bounds__(TPB) void a_manifolds(RB<T> B, Topo t) {
   const int m = gtid();
   if (m >= t.nm) return;
   const int2 rg = t.mRange[m];
@@ -985,7 +985,7 @@ __device__ __forceinline__ int rowPoints(const RB<T>& B, const Topo& t, int m) {
 
 // ---- velocity rows: row constants, warm start from the previous ledger
 template <class T>
-__global__ void a_rows(RB<T> B, RK K, Topo t) {
+__global__ __launch_bounds__(TPB) void a_rows(RB<T> B, RK K, Topo t) {
   const int k = gtid();
   if (k >= t.nRows) return;
   const int m = t.vPerm[k];
@@ -1031,7 +1031,7 @@ __device__ __forceinline__ void vrowIn(const RB<T>& B, int k, int2 r, T (&in)[VR
 
 // one colour of one velocity pass: mode 0 warm start, 1 PGS, 2 PGS with the stabilization side
 template <class T>
-__global__ void a_vpass(RB<T> B, RK K, Topo t, int mode, int r0, int r1, double* tape) {
+__global__ __launch_bounds__(TPB) void a_vpass(RB<T> B, RK K, Topo t, int mode, int r0, int r1, double* tape) {
   const int k = r0 + gtid();
   if (k >= r1) return;
   const int2 r = t.vAB[k];
@@ -1063,7 +1063,7 @@ __global__ void a_vpass(RB<T> B, RK K, Topo t, int mode, int r0, int r1, double*
 
 // ---- ledger of this substep: impulses of the velocity rows (commitPairKeysLambdaKokkos)
 template <class T>
-__global__ void a_ledger(RB<T> B, Topo t) {
+__global__ __launch_bounds__(TPB) void a_ledger(RB<T> B, Topo t) {
   const int e = gtid();
   if (e >= t.nm) return;
   const int k = t.srcRowL[e];
@@ -1074,7 +1074,7 @@ __global__ void a_ledger(RB<T> B, Topo t) {
 
 // ---- re-integration (every slot; a ghost keeps its predicted velocity)
 template <class T>
-__global__ void a_reint(RB<T> B, RK K, int nSlots) {
+__global__ __launch_bounds__(TPB) void a_reint(RB<T> B, RK K, int nSlots) {
   const int i = gtid();
   if (i >= nSlots) return;
   ElReint f;
@@ -1093,7 +1093,7 @@ __global__ void a_reint(RB<T> B, RK K, int nSlots) {
 
 // ---- position rows
 template <class T>
-__global__ void a_prows(RB<T> B, Topo t) {
+__global__ __launch_bounds__(TPB) void a_prows(RB<T> B, Topo t) {
   const int k = gtid();
   if (k >= t.nPRows) return;
   const int c = t.pPerm[k];
@@ -1120,7 +1120,7 @@ __global__ void a_prows(RB<T> B, Topo t) {
 }
 
 template <class T>
-__global__ void a_ppass(RB<T> B, Topo t, int r0, int r1, double* tape) {
+__global__ __launch_bounds__(TPB) void a_ppass(RB<T> B, Topo t, int r0, int r1, double* tape) {
   const int k = r0 + gtid();
   if (k >= r1) return;
   const int c = t.pPerm[k];
@@ -1149,7 +1149,7 @@ __global__ void a_ppass(RB<T> B, Topo t, int r0, int r1, double* tape) {
 // src/contact_preprocessing.hpp:501-538): the last manifold of a key writes its sum / dt at the
 // key's first entry
 template <class T>
-__global__ void a_pimp(RB<T> B, RK K, Topo t) {
+__global__ __launch_bounds__(TPB) void a_pimp(RB<T> B, RK K, Topo t) {
   const int e = gtid();
   if (e >= t.nm) return;
   const int m = t.pimpMan[e];
@@ -1166,7 +1166,7 @@ __global__ void a_pimp(RB<T> B, RK K, Topo t) {
 
 // ---- final commit: periodic wrap of the position, orientation and velocities persist
 template <class T>
-__global__ void a_final(RB<T> B, RK K) {
+__global__ __launch_bounds__(TPB) void a_final(RB<T> B, RK K) {
   const int i = gtid();
   if (i >= K.P.numReal) return;
   T p[3] = {B.xr[3 * i], B.xr[3 * i + 1], B.xr[3 * i + 2]};
@@ -1211,7 +1211,7 @@ __device__ __forceinline__ void aadd(double* a, double v) {
 }
 
 // ledger adjoint -> the rows and position rows that produced it
-__global__ void b_ledger(RB<double> B, GB G, RK K, Topo t) {
+__global__ __launch_bounds__(TPB) void b_ledger(RB<double> B, GB G, RK K, Topo t) {
   const int e = gtid();
   if (e >= t.nm) return;
   const double* gl = G.L + 5 * e;
@@ -1229,7 +1229,7 @@ __global__ void b_ledger(RB<double> B, GB G, RK K, Topo t) {
 }
 
 // final commit: the wrap is a shift
-__global__ void b_final(GB G, RK K) {
+__global__ __launch_bounds__(TPB) void b_final(GB G, RK K) {
   const int i = gtid();
   if (i >= K.P.numReal) return;
   const double* gX = G.X + 13 * i;
@@ -1238,7 +1238,7 @@ __global__ void b_final(GB G, RK K) {
   for (int c = 0; c < 4; ++c) G.qr[4 * i + c] += gX[9 + c];
 }
 
-__global__ void b_ppass(RB<double> B, GB G, Topo t, int r0, int r1, const double* tape) {
+__global__ __launch_bounds__(TPB) void b_ppass(RB<double> B, GB G, Topo t, int r0, int r1, const double* tape) {
   const int k = r0 + gtid();
   if (k >= r1) return;
   const int c = t.pPerm[k];
@@ -1261,7 +1261,7 @@ __global__ void b_ppass(RB<double> B, GB G, Topo t, int r0, int r1, const double
   for (int q = 0; q < 10; ++q) G.pc[10 * k + q] += xb[7 + q];
 }
 
-__global__ void b_prows(RB<double> B, GB G, Topo t) {
+__global__ __launch_bounds__(TPB) void b_prows(RB<double> B, GB G, Topo t) {
   const int k = gtid();
   if (k >= t.nPRows) return;
   const int c = t.pPerm[k];
@@ -1293,7 +1293,7 @@ __global__ void b_prows(RB<double> B, GB G, Topo t) {
   }
 }
 
-__global__ void b_reint(RB<double> B, GB G, RK K, int nSlots) {
+__global__ __launch_bounds__(TPB) void b_reint(RB<double> B, GB G, RK K, int nSlots) {
   const int i = gtid();
   if (i >= nSlots) return;
   ElReint f;
@@ -1321,7 +1321,7 @@ __global__ void b_reint(RB<double> B, GB G, RK K, int nSlots) {
   for (int c = 0; c < 4; ++c) aadd(G.q0 + 4 * i + c, xb[12 + c]);
 }
 
-__global__ void b_vpass(RB<double> B, GB G, RK K, Topo t, int mode, int r0, int r1,
+__global__ __launch_bounds__(TPB) void b_vpass(RB<double> B, GB G, RK K, Topo t, int mode, int r0, int r1,
                         const double* tape) {
   const int k = r0 + gtid();
   if (k >= r1) return;
@@ -1358,7 +1358,7 @@ __global__ void b_vpass(RB<double> B, GB G, RK K, Topo t, int mode, int r0, int 
 }
 
 // warm-start values -> the previous ledger; the velocity-stage start -> the predicted velocities
-__global__ void b_warm(GB G, RK K, Topo t) {
+__global__ __launch_bounds__(TPB) void b_warm(GB G, RK K, Topo t) {
   const int k = gtid();
   if (k < t.nRows) {
     const int e = t.warmEntry[k];
@@ -1371,7 +1371,7 @@ __global__ void b_warm(GB G, RK K, Topo t) {
     for (int c = 0; c < 6; ++c) aadd(G.vh + 6 * k + c, G.vc[6 * k + c]);
 }
 
-__global__ void b_rows(RB<double> B, GB G, RK K, Topo t, double* gpar) {
+__global__ __launch_bounds__(TPB) void b_rows(RB<double> B, GB G, RK K, Topo t, double* gpar) {
   const int k = gtid();
   if (k >= t.nRows) return;
   const int m = t.vPerm[k];
@@ -1395,7 +1395,7 @@ __global__ void b_rows(RB<double> B, GB G, RK K, Topo t, double* gpar) {
   aadd(gpar + 1, G.rc[22 * k + 21]);
 }
 
-__global__ void b_manifolds(RB<double> B, GB G, Topo t) {
+__global__ __launch_bounds__(TPB) void b_manifolds(RB<double> B, GB G, Topo t) {
   const int c = gtid();
   if (c >= t.nc) return;
   const int m = B.cMan[c];
@@ -1410,7 +1410,7 @@ __global__ void b_manifolds(RB<double> B, GB G, Topo t) {
   for (int q = 0; q < 10; ++q) G.cg[10 * c + q] += xb[q];
 }
 
-__global__ void b_contacts(RB<double> B, GB G, RK K, Topo t) {
+__global__ __launch_bounds__(TPB) void b_contacts(RB<double> B, GB G, RK K, Topo t) {
   const int c = gtid();
   if (c >= t.nc) return;
   const int2 ab = t.cAB[c];
@@ -1453,7 +1453,7 @@ __global__ void b_contacts(RB<double> B, GB G, RK K, Topo t) {
 }
 
 // ghosts -> their reals (a shifted copy), then predict; writes the start-state adjoint
-__global__ void b_predict(RB<double> B, GB G, RK K, int nSlots) {
+__global__ __launch_bounds__(TPB) void b_predict(RB<double> B, GB G, RK K, int nSlots) {
   const int i = gtid();
   if (i >= K.P.numReal) return;
   double gxs[3], gxt[3], gv0[3], gvh[6], gq0[4];

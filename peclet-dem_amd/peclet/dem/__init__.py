@@ -489,6 +489,10 @@ class Simulation:
         self._chk(self._b.tape_adjoint(self._h, _ptr(g), _ptr(g0), _ptr(gp)))
         return g0, gp
 
+    def synchronize(self):
+        """Wait for the simulation's queued GPU work."""
+        self._chk(self._b.synchronize(self._h))
+
     def compute_overlaps(self):
         out = C.c_double(0.0)
         self._chk(self._b.compute_overlaps(self._h, C.byref(out)))
