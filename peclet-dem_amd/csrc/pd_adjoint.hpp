@@ -719,6 +719,9 @@ struct ElPos {
   }
 };
 
+// tangents per forward-mode evaluation in the velocity-row products (38 inputs)
+constexpr int VJP_K = 2;
+
 // vector-Jacobian product of one element: xb = J^T yb, J assembled from K forward-mode columns
 // per evaluation (the element's branches follow the primal input x)
 template <int K, class F>
@@ -726,6 +729,37 @@ __device__ void vjp(const F& f, const double (&x)[F::NI], const double (&yb)[F::
                     double (&xb)[F::NI]) {
 #pragma unroll 1
   for (int j0 = 0; j0 < F::NI; j0 += K) {
+    DN<K> in[F::NI];
+#pragma unroll
+    for (int i = 0; i < F::NI; ++i) {
+      in[i].v = x[i];
+#pragma unroll
+      for (int t = 0; t < K; ++t) in[i].d[t] = (i == j0 + t) ? 1.0 : 0.0;
+    }
+    DN<K> out[F::NO];
+    f(in, out);
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      double s = 0.0;
+#pragma unroll
+      for (int o = 0; o < F::NO; ++o) s += yb[o] * out[o].d[t];
+#pragma unroll
+      for (int i = 0; i < F::NI; ++i)
+        if (i == j0 + t) xb[i] = s;
+    }
+  }
+}
+
+// the columns of part `part` of `nparts`: evaluations j0 = part*K, (part+nparts)*K, ...
+template <int K>
+PD_HD bool ownsColumn(int i, int part, int nparts) {
+  return (i / K) % nparts == part;
+}
+template <int K, class F>
+__device__ void vjpPart(const F& f, const double (&x)[F::NI], const double (&yb)[F::NO],
+                        double (&xb)[F::NI], int part, int nparts) {
+#pragma unroll 1
+  for (int j0 = part * K; j0 < F::NI; j0 += nparts * K) {
     DN<K> in[F::NI];
 #pragma unroll
     for (int i = 0; i < F::NI; ++i) {
@@ -1321,25 +1355,36 @@ __global__ __launch_bounds__(TPB) void b_reint(RB<double> B, GB G, RK K, int nSl
   for (int c = 0; c < 4; ++c) aadd(G.q0 + 4 * i + c, xb[12 + c]);
 }
 
-__global__ __launch_bounds__(TPB) void b_vpass(RB<double> B, GB G, RK K, Topo t, int mode, int r0, int r1,
-                        const double* tape) {
-  const int k = r0 + gtid();
-  if (k >= r1) return;
-  const int2 r = t.vAB[k];
+// threads per velocity row in the reverse sweeps: each evaluates a disjoint set of the row's
+// forward-mode columns (a colour holds too few rows to fill the chip one row per thread)
+constexpr int VP_SPLIT = 4;
+__global__ __launch_bounds__(TPB) void b_vpass(RB<double> B, GB G, RK K, Topo t, int mode, int r0,
+                                               int r1, const double* tape) {
+  const int tid = gtid();
+  const int k = r0 + tid / VP_SPLIT;
+  const int part = tid % VP_SPLIT;
+  const bool live = k < r1;
+  int2 r = make_int2(0, -1);
   double x[VR_NI], yb[VR_NO], xb[VR_NI];
-  for (int c = 0; c < 16; ++c) x[c] = tape[16 * (size_t)k + c];
-  for (int c = 0; c < 20; ++c) x[16 + c] = B.rc[20 * k + c];
-  x[36] = B.par[0];
-  x[37] = B.par[1];
-  for (int c = 0; c < 6; ++c) yb[c] = G.vc[6 * r.x + c];
-  for (int c = 0; c < 6; ++c) yb[6 + c] = r.y >= 0 ? G.vc[6 * r.y + c] : 0.0;
-  for (int c = 0; c < 4; ++c) yb[12 + c] = G.lam[4 * k + c];
+  if (live) {
+    r = t.vAB[k];
+    for (int c = 0; c < 16; ++c) x[c] = tape[16 * (size_t)k + c];
+    for (int c = 0; c < 20; ++c) x[16 + c] = B.rc[20 * k + c];
+    x[36] = B.par[0];
+    x[37] = B.par[1];
+    for (int c = 0; c < 6; ++c) yb[c] = G.vc[6 * r.x + c];
+    for (int c = 0; c < 6; ++c) yb[6 + c] = r.y >= 0 ? G.vc[6 * r.y + c] : 0.0;
+    for (int c = 0; c < 4; ++c) yb[12 + c] = G.lam[4 * k + c];
+  }
+  // every thread of the row has read the adjoints before any overwrites them
+  __syncthreads();
+  if (!live) return;
   const RowBodies RBo = rowBodies(B, K, r);
   if (mode == 0) {
     ElWarm f;
     f.B = RBo;
     f.growthRate = K.P.growthRate;
-    vjp<1>(f, x, yb, xb);
+    vjpPart<VJP_K>(f, x, yb, xb, part, VP_SPLIT);
   } else {
     ElPgs f;
     f.B = RBo;
@@ -1348,13 +1393,17 @@ __global__ __launch_bounds__(TPB) void b_vpass(RB<double> B, GB G, RK K, Topo t,
     f.restT = K.P.restT;
     const unsigned char fl = t.vFlag[k];
     f.sf = mode == 2 ? ((fl & F_SIDE1) ? 1 : ((fl & F_SIDE2) ? 2 : 0)) : 0;
-    vjp<1>(f, x, yb, xb);
+    vjpPart<VJP_K>(f, x, yb, xb, part, VP_SPLIT);
   }
-  for (int c = 0; c < 6; ++c) G.vc[6 * r.x + c] = xb[c];
-  if (r.y >= 0)
-    for (int c = 0; c < 6; ++c) G.vc[6 * r.y + c] = xb[6 + c];
-  for (int c = 0; c < 4; ++c) G.lam[4 * k + c] = xb[12 + c];
-  for (int c = 0; c < 22; ++c) G.rc[22 * k + c] += xb[16 + c];
+#pragma unroll
+  for (int c = 0; c < VR_NI; ++c) {
+    if (!ownsColumn<VJP_K>(c, part, VP_SPLIT)) continue;
+    if (c < 6) G.vc[6 * r.x + c] = xb[c];
+    else if (c < 12) {
+      if (r.y >= 0) G.vc[6 * r.y + c - 6] = xb[c];
+    } else if (c < 16) G.lam[4 * k + c - 12] = xb[c];
+    else G.rc[22 * k + c - 16] += xb[c];
+  }
 }
 
 // warm-start values -> the previous ledger; the velocity-stage start -> the predicted velocities
