@@ -283,6 +283,8 @@ def main():
                     help="library timing in the timed region: 2 = per-kernel events and counter "
                          "log (needed for roofline), 1 = stage events only, 0 = none")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the forward+adjoint leg and print its JSON")
     ap.add_argument("--adjoint-n", type=int, default=50000)
     ap.add_argument("--adjoint-substeps", type=int, default=16)
     ap.add_argument("--batch-systems", type=int, default=256)
@@ -293,6 +295,10 @@ def main():
                          "bed of spheres on a slightly overlapping cubic lattice; cylinders: "
                          "200,000 hollow cylinders with a 256-point shell")
     args = ap.parse_args()
+    if args.adjoint_only:
+        print(json.dumps({k: run_adjoint(args.adjoint_n, args.adjoint_substeps, 8, 8, d)
+                          for k, d in (("loose_start", False), ("dense_bed", True))}))
+        return
     if args.n is None:
         args.n = 200000 if args.workload == "cylinders" else 1048576
     if args.gpus != 1:

@@ -1357,7 +1357,7 @@ __global__ __launch_bounds__(TPB) void b_reint(RB<double> B, GB G, RK K, int nSl
 
 // threads per velocity row in the reverse sweeps: each evaluates a disjoint set of the row's
 // forward-mode columns (a colour holds too few rows to fill the chip one row per thread)
-constexpr int VP_SPLIT = 4;
+constexpr int VP_SPLIT = 8;
 __global__ __launch_bounds__(TPB) void b_vpass(RB<double> B, GB G, RK K, Topo t, int mode, int r0,
                                                int r1, const double* tape) {
   const int tid = gtid();
