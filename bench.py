@@ -192,6 +192,8 @@ def run_batched(num, seed, steps, seq_steps, workers):
     t0 = time.perf_counter()
     batch.step(dt, steps)
     t_b = time.perf_counter() - t0
+    # algorithmic bytes of one batch substep from each system's last substep counters
+    bps = sum(substep_bytes(sim.get_stats()) for sim in batch.systems)
     batch.close()
     v_b = total * steps / t_b
     v_s = total * seq_steps / t_seq
@@ -199,6 +201,8 @@ def run_batched(num, seed, steps, seq_steps, workers):
             "substeps_timed": steps, "workers": workers, "ms_per_substep": 1e3 * t_b / steps,
             "sequential_value": v_s, "sequential_substeps_timed": seq_steps,
             "speedup_vs_sequential": v_b / v_s,
+            "algorithmic_bytes_per_substep": bps,
+            "hbm_frac": bps * steps / t_b / 1e9 / HBM_PEAK_GBS,
             "workload": f"{num} independent periodic-xy sphere beds, N~U[500,20000], "
                         "r~U[0.8,1.2] mm, mu~U[0.2,0.8], e~U[0.1,0.6], phi0=0.1, dt=1e-5 s"}
 
@@ -246,6 +250,7 @@ def run_adjoint(n, substeps, every, warm, dense):
     g0, gp = sim.tape_adjoint(grad)
     t_adj = time.perf_counter() - t0
     contacts = sim.num_contacts()
+    fwd_bytes = substep_bytes(sim.get_stats()) * substeps
     sim.close()
     return {"value": n * substeps / (t_fwd + t_adj), "unit": "particle-substeps/s",
             "workload": what + f", dt=1e-5 s, mu=0.5, e=0.3, {warm} substeps before the "
@@ -253,6 +258,9 @@ def run_adjoint(n, substeps, every, warm, dense):
             "contacts_at_end": contacts, "forward_ms": 1e3 * t_plain,
             "recorded_forward_ms": 1e3 * t_fwd, "adjoint_ms": 1e3 * t_adj,
             "cost_vs_forward": (t_fwd + t_adj) / t_plain,
+            # the forward substeps' algorithmic bytes (last substep's counters) over the whole
+            # forward+adjoint time; the adjoint's own traffic is not modelled
+            "hbm_frac_forward_bytes": fwd_bytes / (t_fwd + t_adj) / 1e9 / HBM_PEAK_GBS,
             "grad_norm_v0": float(np.linalg.norm(g0[:, 3:6])),
             "grad_mu": float(gp[0]), "grad_restitution": float(gp[1])}
 
