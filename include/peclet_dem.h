@@ -147,9 +147,12 @@ int pd_write_lammps_dump(const char* filename, int step, const double* pos, cons
 
 /* --- workload utility (no reference counterpart) ------------------------------------- */
 /* Random sequential addition of n spheres (radii[i]) from a caller-drawn candidate stream (3
- * doubles per candidate, centres already inside [lo+r, hi-r]); see pd_sim.hip for the contract. */
+ * doubles per candidate, inside the box); periodic[3] (may be null) marks axes on which [lo, hi)
+ * is one period and overlaps are tested against the nearest image; see pd_sim.hip for the
+ * contract. */
 long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, int start,
-                  const double* lo, const double* hi, double* out, int* placed);
+                  const double* lo, const double* hi, const int* periodic, double* out,
+                  int* placed);
 
 /* getSdfGrid(rx, ry, rz), src/sim.hpp:1538-1542 -> src/output_sdf.hpp: packed-bed SDF on an
  * (rx,ry,rz) voxel grid over the domain, x-fastest, negative inside (out: rx*ry*rz doubles) */

@@ -18,6 +18,11 @@
  *   - sorted ledger: stable (key, manifold index)
  * Build with -ffp-contract=off so no multiply-add is fused.
  *
+ * Threads (pdref_set_threads, default 1): element-wise loops and the rows of one colour class
+ * (which touch disjoint bodies; colour 62 stays serial) are split over OpenMP threads.  Each
+ * element is computed by the same statements either way and the running maxima are combined
+ * with max, so every result is bit-identical to the one-thread run.
+ *
  * Pinning (tests/test_oracle.py, tests/test_oracle_hertz.py): reference test inputs, brute force,
  * closed forms and independent algorithms.  Parity unpinned: the Jacobi leftover solves
  * (solveVelocityJacobi / positionSolveJacobi with onlyUncoloured) beyond momentum conservation,
@@ -32,6 +37,17 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static int g_threads = 1;
+int pdref_set_threads(int n) {
+  g_threads = n < 1 ? 1 : n;
+  return 0;
+}
+/* threads for a loop of n independent elements (small loops stay serial) */
+static int nthreads(long n) { return n >= 2048 ? g_threads : 1; }
 
 typedef double real;
 typedef struct { real x, y, z; } V3;
@@ -703,6 +719,7 @@ static real maxOwnedRadius(const Sim* S) {
 /* src/integration.hpp:44-93 */
 static void predictVelocity(Sim* S) {
   const real dt = S->dt;
+#pragma omp parallel for num_threads(nthreads(S->numReal)) schedule(static)
   for (int i = 0; i < S->numReal; ++i) {
     const real invM = S->invMass[i];
     V3 v = S->vel[i];
@@ -788,6 +805,7 @@ static void generateGhosts(Sim* S, real skin) {
 /* src/integration.hpp:112-148 */
 static void applyVelocityAndPredictPosition(Sim* S) {
   const real dt = S->dt;
+#pragma omp parallel for num_threads(nthreads(S->numParticles)) schedule(static)
   for (int i = 0; i < S->numParticles; ++i) {
     const V3 vStart = S->vel[i];
     const V3 vFinal = S->velPred[i];
@@ -819,6 +837,7 @@ static void applyVelocityAndPredictPosition(Sim* S) {
 /* src/integration.hpp:187-215 */
 static void finalCommit(Sim* S) {
   const Domain d = S->dom;
+#pragma omp parallel for num_threads(nthreads(S->numReal)) schedule(static)
   for (int i = 0; i < S->numReal; ++i) {
     V3 p = S->posPred[i];
     if (d.px) {
@@ -1399,6 +1418,7 @@ static int lowerBound(const uint64_t* keys, int n, uint64_t k) {
 
 /* src/contact_preprocessing.hpp:113-156 */
 static void gatherWarmLambda(Sim* S) {
+#pragma omp parallel for num_threads(nthreads(S->nm)) schedule(static)
   for (int idx = 0; idx < S->nm; ++idx) {
     const Manifold* m = &S->manifolds[idx];
     int dup = 0;
@@ -1496,6 +1516,7 @@ static void addNegScaled(V3* v, V3 J, real s) {
 
 /* src/solver_velocity.hpp:390-447 */
 static void computeVn0(Sim* S) {
+#pragma omp parallel for num_threads(nthreads(S->nm)) schedule(static)
   for (int idx = 0; idx < S->nm; ++idx) {
     const Manifold* m = &S->manifolds[idx];
     if (m->numPoints <= 0) continue;
@@ -2139,6 +2160,38 @@ static void commitPosImpulse(Sim* S) {
 /* ------------------------------------------------------------------------------------------ */
 /* the substep — src/sim.hpp:97-488                                                            */
 /* ------------------------------------------------------------------------------------------ */
+/* One colour class of rows: the rows of a class (below colour 62) touch disjoint bodies, so they
+ * are split over the threads; each thread keeps its own running maxima (a shallow copy of the
+ * Sim: the arrays are shared) and the maxima are combined with max afterwards. */
+typedef void (*RowFn)(Sim* S, int idx, real thr);
+static void runClass(Sim* S, const int* list, int n, int colour, RowFn fn, real thr) {
+  const int nt = colour == 62 ? 1 : nthreads(n);
+  if (nt <= 1) {
+    for (int k = 0; k < n; ++k) fn(S, list[k], thr);
+    return;
+  }
+  real ma = S->maxApproach, mo = S->maxOverlap;
+#pragma omp parallel num_threads(nt)
+  {
+    Sim L = *S;
+#pragma omp for schedule(static)
+    for (int k = 0; k < n; ++k) fn(&L, list[k], thr);
+#pragma omp critical(pdref_class_max)
+    {
+      if (L.maxApproach > ma) ma = L.maxApproach;
+      if (L.maxOverlap > mo) mo = L.maxOverlap;
+    }
+  }
+  S->maxApproach = ma;
+  S->maxOverlap = mo;
+}
+static void warmRow(Sim* S, int idx, real thr) { (void)thr; warmStartApplyOne(S, idx); }
+static void posRow(Sim* S, int idx, real thr) { (void)thr; positionSolveOne(S, idx); }
+static void runClasses(Sim* S, const ColourClasses* C, RowFn fn, real thr) {
+  for (int c = 0; c < C->ncol; ++c)
+    runClass(S, C->list + C->start[c], C->start[c + 1] - C->start[c], c, fn, thr);
+}
+
 static void fillRad(Sim* S, int n) {
   for (int i = 0; i < n; ++i) S->rad[i] = S->scale[i] * S->globalScale * S->baseRadius;
 }
@@ -2206,8 +2259,7 @@ static int demStep(Sim* S) {
     updateGroundedLevels(S, gHat, 8, 8);
     memset(S->sideFlags, 0, (size_t)nm);
     computeVn0(S);
-    for (int c = 0; c < VC.ncol; ++c)
-      for (int k = VC.start[c]; k < VC.start[c + 1]; ++k) warmStartApplyOne(S, VC.list[k]);
+    runClasses(S, &VC, warmRow, 0.0);
   }
   for (int it = 0; it < S->velIters; ++it) {
     if (legacyFriction) accumulateNormalImpulse(S);
@@ -2216,11 +2268,7 @@ static int demStep(Sim* S) {
                             S->gravity.z * S->gravity.z);
     if (S->useGS) {
       S->maxApproach = 0.0;
-      for (int c = 0; c < VC.ncol; ++c)
-        for (int k = VC.start[c]; k < VC.start[c + 1]; ++k) {
-          if (usePGS) pgsSolveOne(S, VC.list[k], vRest);
-          else gsSolveOne(S, VC.list[k], vRest);
-        }
+      runClasses(S, &VC, usePGS ? pgsSolveOne : gsSolveOne, vRest);
       if (velLeftover > 0) {
         solveVelocityJacobi(S, vRest, 1);
         applyVelocityDeltasAveraged(S);
@@ -2242,16 +2290,14 @@ static int demStep(Sim* S) {
         computeSideFlags(S, gHat, 8.0 * S->dt * gMagP, 8.0 * S->dt * gMagP);
         for (int it = 0; it < 2 * S->velIters; ++it) {
           S->maxApproach = 0.0;
-          for (int c = 0; c < VC.ncol; ++c)
-            for (int k = VC.start[c]; k < VC.start[c + 1]; ++k) pgsSolveOne(S, VC.list[k], vRestS);
+          runClasses(S, &VC, pgsSolveOne, vRestS);
           S->stats.sweeps_stab++;
           if (S->maxApproach <= vRestS) break;
         }
       } else if (smode == 3) {
         for (int it = 0; it < 256; ++it) {
           S->maxApproach = 0.0;
-          for (int c = 0; c < VC.ncol; ++c)
-            for (int k = VC.start[c]; k < VC.start[c + 1]; ++k) pgsSolveOne(S, VC.list[k], vRestS);
+          runClasses(S, &VC, pgsSolveOne, vRestS);
           S->stats.sweeps_stab++;
           if (S->maxApproach <= vRestS) break;
         }
@@ -2284,8 +2330,7 @@ static int demStep(Sim* S) {
   for (int it = 0; it < S->posIters; ++it) {
     if (S->useGS) {
       S->maxOverlap = 0.0;
-      for (int c = 0; c < PC.ncol; ++c)
-        for (int k = PC.start[c]; k < PC.start[c + 1]; ++k) positionSolveOne(S, PC.list[k]);
+      runClasses(S, &PC, posRow, 0.0);
       if (posLeftover > 0) positionSolveJacobi(S, 1);
       S->stats.sweeps_pos++;
       if (S->maxOverlap < posTol) break;

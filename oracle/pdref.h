@@ -42,6 +42,9 @@ typedef struct pdref_stats {
 
 void* pdref_create(int capacity);
 void pdref_destroy(void* s);
+/* host threads for the element-wise loops and the rows of one colour class (default 1); results
+ * are bit-identical for every thread count */
+int pdref_set_threads(int n);
 const char* pdref_last_error(void);
 
 int pdref_initialize_shape(void* s, int shape_type, double radius, double height,

@@ -47,13 +47,43 @@ struct ArgError : std::runtime_error {
 // every kernel bounds-checks its index, so an empty range still launches one block
 inline int nblk(long n) { return n > 0 ? (int)((n + TPB - 1) / TPB) : 1; }
 
+// Device allocation is serialised process-wide, and a buffer replaced by growth is not freed at
+// once: it is retired and freed at a point where no stream can still read it (after a device
+// synchronisation).  Systems of a batch grow their buffers from several host threads while the
+// other systems' streams run; freeing there would race their in-flight work (a system's second
+// stream, or kernels enqueued before its pointer table was refreshed) and interleave the
+// runtime's allocator calls across threads.
+std::mutex g_allocMutex;
+std::vector<void*> g_retired;       // replaced buffers waiting for a safe free (under the mutex)
+size_t g_retiredBytes = 0;
+std::atomic<int> g_batchDepth{0};   // > 0 while a batch step drives systems from several threads
+constexpr size_t RETIRED_LIMIT = size_t(4) << 30;
+
+// free every retired buffer; the caller holds g_allocMutex
+void drainRetiredLocked() {
+  if (g_retired.empty()) return;
+  HIPCHECK(hipDeviceSynchronize());
+  for (void* p : g_retired) (void)hipFree(p);
+  g_retired.clear();
+  g_retiredBytes = 0;
+}
+// free the retired buffers at a point where the calling thread is the only one driving systems
+void drainRetired() {
+  if (g_batchDepth.load() > 0) return;
+  std::lock_guard<std::mutex> lk(g_allocMutex);
+  drainRetiredLocked();
+}
+
 // grow-only device buffer; growth keeps the first `keep` elements
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t cap = 0;
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      std::lock_guard<std::mutex> lk(g_allocMutex);
+      (void)hipFree(p);
+    }
     p = nullptr;
     cap = 0;
   }
@@ -61,12 +91,18 @@ struct DBuf {
     if (n <= cap) return;
     size_t nc = n + n / 4 + 64;
     T* q = nullptr;
+    std::lock_guard<std::mutex> lk(g_allocMutex);
+    // many retired bytes: free them first (a device synchronisation; nothing still reads them)
+    if (g_retiredBytes > RETIRED_LIMIT) drainRetiredLocked();
     HIPCHECK(hipMalloc(&q, nc * sizeof(T)));
     HIPCHECK(hipMemsetAsync(q, 0, nc * sizeof(T), st));
     if (p && keep) HIPCHECK(hipMemcpyAsync(q, p, std::min(keep, cap) * sizeof(T),
                                            hipMemcpyDeviceToDevice, st));
     HIPCHECK(hipStreamSynchronize(st));
-    if (p) HIPCHECK(hipFree(p));
+    if (p) {
+      g_retired.push_back(p);
+      g_retiredBytes += cap * sizeof(T);
+    }
     p = q;
     cap = nc;
   }
@@ -1328,7 +1364,7 @@ struct pd_sim {
         } else if (mode == 1) {
           kernBegin();
           k_vsweep<1><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
-          kernEnd(K_VSWEEP, e - b);
+          tagSweepRec(kernEnd(K_VSWEEP, e - b));
         } else {
           k_vsweep<2><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
         }
@@ -1355,11 +1391,21 @@ struct pd_sim {
   }
   bool vFusedRec = false;  // the timed K_VBLOCK record covers both loops of the stage
   long vBlockRec[2] = {-1, -1}, pBlockRec = -1;
+  // per-colour sweep launches of the device-gated loops: (timing record, loop * 4096 + sweep);
+  // a launch after its loop's stop flag was set returns at once, so its units become 0 once the
+  // executed sweep counts are known
+  std::vector<std::pair<long, int>> sweepRecs;
+  int curSweepTag = -1;
+  void tagSweepRec(long rec) {
+    if (rec >= 0 && curSweepTag >= 0) sweepRecs.push_back({rec, curSweepTag});
+  }
   int posSweepsPrev = 1;  // sweeps of the last batched position loop (first batch size)
   static constexpr int POS_TAIL_ROWS = 8192;
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
     vBlockRec[0] = vBlockRec[1] = pBlockRec = -1;
+    sweepRecs.clear();
+    curSweepTag = -1;
     vFusedRec = false;
     timedNow = timing > 0 && timingSubstep % timingEvery == 0;
     ++timingSubstep;
@@ -1642,9 +1688,11 @@ struct pd_sim {
         vLoop(usePGS ? 1 : 2, P, vs, numColors, velIters, vRest, stopThr, ctl.p + CTL_STOP_V,
               ctl.p + CTL_SWEEPS_V, nullptr);
         vBlockRec[0] = kernEnd(K_VBLOCK, nRowsV);
-      } else if (!smallV) sweepLoop(velIters, ctl.p + CTL_STOP_V, [&] {
+      } else if (!smallV) sweepLoop(velIters, ctl.p + CTL_STOP_V, [&, sv = 0]() mutable {
         k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V);
+        curSweepTag = sv++;
         velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest, ctl.p + CTL_STOP_V);
+        curSweepTag = -1;
         k_sweep_check<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V, ctl.p + CTL_SWEEPS_V,
                                        stopThr, 0);
       });
@@ -1668,9 +1716,11 @@ struct pd_sim {
             k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, 8.0 * dt * gMagP,
                                                        8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN,
                                                        nullptr);
-          sweepLoop(2 * velIters, ctl.p + CTL_STOP_S, [&] {
+          sweepLoop(2 * velIters, ctl.p + CTL_STOP_S, [&, ss = 0]() mutable {
             k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S);
+            curSweepTag = 4096 + ss++;
             velocitySweep(vs, numColors, 1, vRestS, ctl.p + CTL_STOP_S);
+            curSweepTag = -1;
             k_sweep_check<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S, ctl.p + CTL_SWEEPS_S,
                                            vRestS, 0);
           });
@@ -1857,7 +1907,11 @@ struct pd_sim {
             } else {
               kernBegin();
               k_psweep<<<nblk(ce - cb), TPB, 0, st>>>(dD, cb, ce, ctl.p + CTL_STOP_P);
-              kernEnd(K_PSWEEP, ce - cb);
+              if (timing >= 2) {
+                curSweepTag = 2 * 4096 + done + j;
+                tagSweepRec(kernEnd(K_PSWEEP, ce - cb));
+                curSweepTag = -1;
+              }
             }
           }
           k_psweep_tail<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(dD, pcs, cFrom, numPosColors, nc, posTol,
@@ -1942,6 +1996,12 @@ struct pd_sim {
             (long)nRowsV * (stats.sweeps_vel + (vFusedRec ? stats.sweeps_stab : 0));
       if (vBlockRec[1] >= 0) kernPending[vBlockRec[1]].units = (long)nRowsV * stats.sweeps_stab;
       if (pBlockRec >= 0) kernPending[pBlockRec].units = (long)nRowsP * stats.sweeps_pos;
+      for (const auto& rt : sweepRecs) {
+        const int loop = rt.second / 4096, sweep = rt.second % 4096;
+        const long ran = loop == 0 ? stats.sweeps_vel : loop == 1 ? stats.sweeps_stab
+                                                                  : stats.sweeps_pos;
+        if (sweep >= ran && rt.first < (long)kernPending.size()) kernPending[rt.first].units = 0;
+      }
       statsLog.push_back(stats);
     }
     if (kernPending.size() > 8192 || stagePending.size() > 256 * (ST_COUNT + 1)) resolveTiming();
@@ -2272,7 +2332,10 @@ pd_sim* pd_create(int capacity) {
     return nullptr;
   }
 }
-void pd_destroy(pd_sim* s) { delete s; }
+void pd_destroy(pd_sim* s) {
+  delete s;
+  (void)guard([&] { drainRetired(); });
+}
 int pd_synchronize(pd_sim* s) {
   return guard([&] { s->sync(); });
 }
@@ -2501,6 +2564,7 @@ int pd_step(pd_sim* s, double dt) {
   return guard([&] {
     s->dt = dt;
     s->demStep();
+    drainRetired();
   });
 }
 int pd_set_sdf_shape(pd_sim* s, const double* grid, int nx, int ny, int nz, double ox, double oy,
@@ -2692,6 +2756,7 @@ int pd_step_n(pd_sim* s, double dt, int n) {
       s->dt = dt;
       s->demStep();
     }
+    drainRetired();
   });
 }
 int pd_tape_begin(pd_sim* s, int every) {
@@ -2934,15 +2999,23 @@ int pd_write_vtp(pd_sim* s, const char* filename) {
 // Spheres i = 0..n-1 (radius radii[i]) take candidates from `cand` (3 per candidate, in draw
 // order) until one overlaps no placed sphere; returns the number of candidates consumed, or -1 if
 // the stream ran out (then *placed tells how many were placed; call again with more candidates
-// and `start` = *placed to continue).
+// and `start` = *placed to continue).  On an axis with periodic[d] != 0 (periodic may be null)
+// the box [lo, hi) is one period: distances use the nearest periodic image, so spheres may
+// straddle that face.
 long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, int start,
-                  const double* lo, const double* hi, double* out, int* placed) {
+                  const double* lo, const double* hi, const int* periodic, double* out,
+                  int* placed) {
   static thread_local std::vector<int> head, next;
   double rmax = 0.0;
   for (int i = 0; i < n; ++i) rmax = std::max(rmax, radii[i]);
   const double h = 2.0 * rmax > 0.0 ? 2.0 * rmax : 1.0;
-  int dim[3];
-  for (int d = 0; d < 3; ++d) dim[d] = std::max(1, (int)((hi[d] - lo[d]) / h));
+  int dim[3], per[3];
+  double L[3];
+  for (int d = 0; d < 3; ++d) {
+    dim[d] = std::max(1, (int)((hi[d] - lo[d]) / h));
+    L[d] = hi[d] - lo[d];
+    per[d] = periodic && periodic[d] && L[d] > 0.0;
+  }
   const long ncell = (long)dim[0] * dim[1] * dim[2];
   head.assign(ncell, -1);
   next.assign(n, -1);
@@ -2956,6 +3029,19 @@ long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, in
     next[i] = head[k];
     head[k] = i;
   }
+  // neighbour cell offsets per axis: -1..1, wrapped on periodic axes (each cell visited once)
+  auto cells = [&](int d, int c, int* o) {
+    int m = 0;
+    for (int dc = -1; dc <= 1; ++dc) {
+      int x = c + dc;
+      if (per[d]) x = ((x % dim[d]) + dim[d]) % dim[d];
+      else if (x < 0 || x >= dim[d]) continue;
+      bool seen = false;
+      for (int q = 0; q < m; ++q) seen |= o[q] == x;
+      if (!seen) o[m++] = x;
+    }
+    return m;
+  };
   long used = 0;
   int i = start;
   while (i < n) {
@@ -2966,19 +3052,21 @@ long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, in
     const double* p = cand + 3 * used;
     ++used;
     const double r = radii[i];
-    int c[3];
+    int c[3], ox[3], oy[3], oz[3];
     cellOf(p, c);
+    const int mx = cells(0, c[0], ox), my = cells(1, c[1], oy), mz = cells(2, c[2], oz);
     bool ok = true;
-    for (int dz = -1; dz <= 1 && ok; ++dz)
-      for (int dy = -1; dy <= 1 && ok; ++dy)
-        for (int dx = -1; dx <= 1 && ok; ++dx) {
-          const int x = c[0] + dx, y = c[1] + dy, z = c[2] + dz;
-          if (x < 0 || y < 0 || z < 0 || x >= dim[0] || y >= dim[1] || z >= dim[2]) continue;
-          for (int j = head[((long)z * dim[1] + y) * dim[0] + x]; j >= 0; j = next[j]) {
-            const double ex = out[3 * j] - p[0], ey = out[3 * j + 1] - p[1],
-                         ez = out[3 * j + 2] - p[2];
+    for (int a = 0; a < mz && ok; ++a)
+      for (int b = 0; b < my && ok; ++b)
+        for (int e = 0; e < mx && ok; ++e) {
+          for (int j = head[((long)oz[a] * dim[1] + oy[b]) * dim[0] + ox[e]]; j >= 0; j = next[j]) {
+            double dv[3];
+            for (int d = 0; d < 3; ++d) {
+              dv[d] = out[3 * j + d] - p[d];
+              if (per[d]) dv[d] -= L[d] * std::nearbyint(dv[d] / L[d]);
+            }
             const double rr = r + radii[j];
-            if (ex * ex + ey * ey + ez * ez < rr * rr) {
+            if (dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2] < rr * rr) {
               ok = false;
               break;
             }
@@ -3140,10 +3228,20 @@ int pd_batch_step(pd_batch* b, double dt, int nsteps) {
     }
   };
   const int nt = std::max(1, std::min(b->workers, S));
+  // buffers the workers replace are freed after they have all finished (drainRetired)
+  ++g_batchDepth;
   std::vector<std::thread> pool;
   for (int w = 1; w < nt; ++w) pool.emplace_back(work);
   work();
   for (auto& th : pool) th.join();
+  --g_batchDepth;
+  if (err.empty()) {
+    try {
+      drainRetired();
+    } catch (const std::exception& e) {
+      err = e.what();
+    }
+  }
   if (!err.empty()) {
     g_error = err;
     return -1;

@@ -139,7 +139,7 @@ PRODUCT_SIGNATURES = {
     "batch_system": (_p, [_p, _i]),
     "batch_set_workers": (_i, [_p, _i]),
     "batch_step": (_i, [_p, _d, _i]),
-    "rsa_place": (C.c_long, [_dp, C.c_long, _dp, _i, _i, _dp, _dp, _dp, _ip]),
+    "rsa_place": (C.c_long, [_dp, C.c_long, _dp, _i, _i, _dp, _dp, _ip, _dp, _ip]),
     "tape_begin": (_i, [_p, _i]),
     "tape_end": (_i, [_p]),
     "tape_length": (_i, [_p]),

@@ -12,7 +12,16 @@ import math
 import numpy as np
 
 
-def _rsa_native(n, radii, lo, hi, rng, max_tries):
+def _draw(rng, m, lo, hi, rmax, per):
+    """m candidate centres: over the whole period on periodic axes, else inside [lo+rmax,
+    hi-rmax] (clear of the faces)."""
+    u = rng.random((m, 3))
+    a = np.where(per, lo, lo + rmax)
+    w = np.where(per, hi - lo, hi - lo - 2.0 * rmax)
+    return a + u * w
+
+
+def _rsa_native(n, radii, lo, hi, rng, max_tries, per):
     """Same acceptance rule as the Python loop below, run by the library's pd_rsa_place."""
     import ctypes as C
 
@@ -21,11 +30,13 @@ def _rsa_native(n, radii, lo, hi, rng, max_tries):
     lib = _product_binding().lib
     fn = lib.pd_rsa_place
     dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
     fn.restype = C.c_long
-    fn.argtypes = [dp, C.c_long, dp, C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int)]
+    fn.argtypes = [dp, C.c_long, dp, C.c_int, C.c_int, dp, dp, ip, dp, ip]
     out = np.zeros((n, 3), np.float64)
     lo3 = np.ascontiguousarray(lo, np.float64)
     hi3 = np.ascontiguousarray(hi, np.float64)
+    per3 = np.ascontiguousarray(per, np.int32)
     placed = C.c_int(0)
     rmax = float(radii.max())
     total = 0
@@ -34,23 +45,28 @@ def _rsa_native(n, radii, lo, hi, rng, max_tries):
         total += m
         if total > max_tries * max(n, 1):
             raise RuntimeError(f"rsa_positions: could only place {placed.value} of {n} spheres")
-        cand = lo3 + rmax + rng.random((m, 3)) * (hi3 - lo3 - 2.0 * rmax)
-        cand = np.ascontiguousarray(cand)
+        cand = np.ascontiguousarray(_draw(rng, m, lo3, hi3, rmax, per3.astype(bool)))
         fn(cand.ctypes.data_as(dp), m, radii.ctypes.data_as(dp), n, placed.value,
-           lo3.ctypes.data_as(dp), hi3.ctypes.data_as(dp), out.ctypes.data_as(dp),
-           C.byref(placed))
+           lo3.ctypes.data_as(dp), hi3.ctypes.data_as(dp), per3.ctypes.data_as(ip),
+           out.ctypes.data_as(dp), C.byref(placed))
     return out
 
 
-def rsa_positions(n, radii, lo, hi, seed, max_tries=200, order="cell", native=None):
+def rsa_positions(n, radii, lo, hi, seed, max_tries=200, order="cell", native=None,
+                  periodic=(False, False, False)):
     """Place ``n`` spheres with the given radii (array of n) uniformly in the box [lo, hi].
 
-    Candidates are drawn in the box shrunk by the largest radius.  ``native`` (default: when
-    n > 20000) runs the acceptance loop in the C library.
+    On a non-periodic axis candidates are drawn clear of both faces (inside [lo+rmax, hi-rmax],
+    rmax the largest radius).  On a periodic axis ([lo, hi) is one period) they are drawn over the
+    whole period and overlaps are tested against the nearest periodic image, so spheres straddle
+    the seam like anywhere else in the bed.  ``native`` (default: when n > 20000) runs the
+    acceptance loop in the C library.
     """
     radii = np.ascontiguousarray(radii, np.float64)
     lo = np.asarray(lo, np.float64)
     hi = np.asarray(hi, np.float64)
+    per = np.asarray(periodic, bool)
+    L = hi - lo
     rng = np.random.default_rng(seed)
     rmax = float(radii.max()) if n else 0.0
     h = 2.0 * rmax
@@ -58,7 +74,7 @@ def rsa_positions(n, radii, lo, hi, seed, max_tries=200, order="cell", native=No
     if native is None:
         native = n > 20000
     if native and n:
-        out = _rsa_native(n, radii, lo, hi, rng, max_tries)
+        out = _rsa_native(n, radii, lo, hi, rng, max_tries, per)
         if order == "cell":
             k = np.minimum(((out - lo) / h).astype(np.int64), dims - 1)
             key = (k[:, 2] * dims[1] + k[:, 1]) * dims[0] + k[:, 0]
@@ -73,20 +89,34 @@ def rsa_positions(n, radii, lo, hi, seed, max_tries=200, order="cell", native=No
     while placed < n:
         if tries > max_tries * max(n, 1):
             raise RuntimeError(f"rsa_positions: could only place {placed} of {n} spheres")
-        cand = lo + rmax + rng.random((batch, 3)) * (hi - lo - 2.0 * rmax)
+        cand = _draw(rng, batch, lo, hi, rmax, per)
         for c in cand:
             tries += 1
             r = radii[placed]
             key = tuple(np.minimum(((c - lo) / h).astype(np.int64), dims - 1))
+            # neighbour cells per axis, wrapped on periodic axes, each visited once
+            near = []
+            for ax in range(3):
+                lst = []
+                for dc in (-1, 0, 1):
+                    x = key[ax] + dc
+                    if per[ax]:
+                        x %= dims[ax]
+                    elif x < 0 or x >= dims[ax]:
+                        continue
+                    if x not in lst:
+                        lst.append(x)
+                near.append(lst)
             ok = True
-            for dz in (-1, 0, 1):
-                for dy in (-1, 0, 1):
-                    for dx in (-1, 0, 1):
-                        lst = cells.get((key[0] + dx, key[1] + dy, key[2] + dz))
+            for cz in near[2]:
+                for cy in near[1]:
+                    for cx in near[0]:
+                        lst = cells.get((cx, cy, cz))
                         if not lst:
                             continue
                         for j in lst:
                             d = out[j] - c
+                            d = np.where(per, d - L * np.round(d / L), d)
                             if d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < (r + radii[j]) ** 2:
                                 ok = False
                                 break
@@ -111,17 +141,18 @@ def rsa_positions(n, radii, lo, hi, seed, max_tries=200, order="cell", native=No
 
 
 def sphere_bed(n, r_lo, r_hi, box, seed, rho=2500.0, base_radius=1e-3, column=None,
-               order="cell", native=None):
+               order="cell", native=None, periodic=(True, True, False)):
     """Polydisperse sphere bed in SI units.
 
     Returns dict(pos, scale, inv_mass, inv_inertia, radius, domain) for a bed of ``n`` spheres
-    with radius ~ U[r_lo, r_hi] (metres) seeded in ``[0, box[0]] x [0, box[1]] x [0, column]``.
+    with radius ~ U[r_lo, r_hi] (metres) seeded in ``[0, box[0]] x [0, box[1]] x [0, column]``,
+    across the seams of the ``periodic`` axes (default x and y, the beds' footprint).
     """
     rng = np.random.default_rng(seed + 7919)
     radius = rng.uniform(r_lo, r_hi, n) if r_hi > r_lo else np.full(n, float(r_lo))
     col = box[2] if column is None else column
     pos, perm = rsa_positions(n, radius, (0.0, 0.0, 0.0), (box[0], box[1], col), seed,
-                              order=order, native=native)
+                              order=order, native=native, periodic=periodic)
     radius = radius[perm]
     mass = rho * 4.0 / 3.0 * math.pi * radius ** 3
     inv_mass = 1.0 / mass
@@ -239,7 +270,8 @@ def cylinder_bed(n, box_xy, seed, r_out=2e-3, r_in=1.2e-3, length=4e-3, shell_po
     rb = math.sqrt(r_out ** 2 + (0.5 * length) ** 2)
     radii = np.full(n, rb)
     col = column if column is not None else n * (4.0 / 3.0 * math.pi * rb ** 3) / (0.28 * box_xy ** 2)
-    pos, _ = rsa_positions(n, radii, (0.0, 0.0, 0.0), (box_xy, box_xy, col), seed, native=True)
+    pos, _ = rsa_positions(n, radii, (0.0, 0.0, 0.0), (box_xy, box_xy, col), seed, native=True,
+                           periodic=(True, True, False))
     rng = np.random.default_rng(seed + 1)
     q = rng.normal(size=(n, 4))
     q /= np.linalg.norm(q, axis=1, keepdims=True)
