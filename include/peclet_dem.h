@@ -194,26 +194,31 @@ int pd_batch_step(pd_batch* b, double dt, int nsteps); /* nsteps substeps of eve
  * SDF), planes, periodic axes, gravity, the coloured Gauss-Seidel velocity path with the
  * Coulomb cone, stabilization modes off / one-sided; no SDF walls, pair-material table, growth
  * or thermostat (pd_step returns an error naming the feature).  Masses, inertia, scales and
- * external forces must not change between pd_tape_begin and the last derivative call.
+ * external forces are captured when the recording opens; setting any of them while a recording
+ * is open makes the next substep fail, and the derivative calls use the captured values.
  *
  * pd_tape_begin opens a recording (dropping any earlier one): every following substep stores
  * its discrete structure (contacts, manifolds, colour orders, sweep counts, stabilization
  * flags, warm-start map), and every `checkpoint_every` substeps the body state and the
- * warm-start ledger.  pd_tape_end closes it. */
+ * warm-start ledger.  pd_tape_end closes it.  Every recording gets a new generation number
+ * (pd_tape_generation, 0 before the first), so a caller holding derivatives of an earlier
+ * recording can tell that it was replaced.  The state arrays of the calls below hold `count`
+ * doubles, which must be 13 x the recorded particle count (else -2, ArgError). */
 int pd_tape_begin(pd_sim* s, int checkpoint_every);
 int pd_tape_end(pd_sim* s);
 int pd_tape_length(pd_sim* s); /* recorded substeps */
+long pd_tape_generation(pd_sim* s);
 /* replay of the recorded substeps from the first checkpoint (equals the stepped state) */
-int pd_tape_replay(pd_sim* s, double* state_out);
+int pd_tape_replay(pd_sim* s, double* state_out, long count);
 /* forward mode: the end state and its directional derivative along (dstate0, dmu, drest),
  * where mu and rest are the friction coefficient and normal restitution; dstate0 may be NULL */
 int pd_tape_tangent(pd_sim* s, const double* dstate0, double dmu, double drest, double* state_out,
-                    double* dstate_out);
+                    double* dstate_out, long count);
 /* reverse mode (adjoint): given dL/d(end state), returns dL/d(start state) and
  * dL/d(mu, rest) in grad_params[2]; checkpointed segments are replayed, then reversed
  * substep by substep (sweeps in reverse colour order) */
 int pd_tape_adjoint(pd_sim* s, const double* grad_state_end, double* grad_state0,
-                    double* grad_params);
+                    double* grad_params, long count);
 
 /* --- measurement ------------------------------------------------------------------------ */
 /* Timing with HIP events on the handle's stream (no reference counterpart; the reference's

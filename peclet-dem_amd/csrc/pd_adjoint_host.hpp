@@ -60,6 +60,12 @@ struct AdjTape {
   bool on = false;
   int every = 16;
   int numReal = 0;
+  long generation = 0;     // number of the recording (one more at every pd_tape_begin)
+  long paramsVersion = 0;  // the simulation's parameter version when the recording opened
+  // masses / inertia (bc), scales and external forces captured when the recording opened
+  const BodyC* bc = nullptr;
+  const double* scale = nullptr;
+  const double4* extForce = nullptr;
   AdjArena arena;
   std::vector<TapeStep> steps;
   std::vector<TapeCkpt> ckpts;
@@ -67,6 +73,9 @@ struct AdjTape {
   int nKey0 = 0;
   void clear() {
     arena.release();
+    bc = nullptr;
+    scale = nullptr;
+    extForce = nullptr;
     steps.clear();
     ckpts.clear();
     key0 = nullptr;
@@ -97,10 +106,41 @@ void pd_sim::tapeCheck(bool usePGS, bool legacyFriction) const {
   if (stabMode != 0 && stabMode != 1) no("this stabilization mode");
 }
 
+// a new recording: its generation number and the captured per-body constants
+void pd_sim::tapeCapture() {
+  AdjTape& T = *tape;
+  ++T.generation;
+  T.paramsVersion = paramsVersion;
+  T.numReal = numReal;
+  const size_t n = (size_t)std::max(1, numReal);
+  BodyC* b = T.arena.takeN<BodyC>(n);
+  double* sc = T.arena.takeN<double>(n);
+  double4* ef = T.arena.takeN<double4>(n);
+  if (numReal > 0) {
+    HIPCHECK(hipMemcpyAsync(b, bc.p, numReal * sizeof(BodyC), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(sc, scale.p, numReal * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(ef, extForce.p, numReal * sizeof(double4), hipMemcpyDeviceToDevice,
+                            st));
+  }
+  T.bc = b;
+  T.scale = sc;
+  T.extForce = ef;
+}
+// a state array of `count` doubles must hold the recorded bodies' 13 numbers
+void pd_sim::tapeCount(long count) const {
+  const long want = tape ? 13L * tape->numReal : 0L;
+  if (count != want)
+    throw ArgError("adjoint tape: a state array of " + std::to_string(count) +
+                   " doubles was passed where the recording needs " + std::to_string(want) +
+                   " (13 per body)");
+}
+
 void pd_sim::tapeSubstepStart(bool usePGS, bool legacyFriction) {
   AdjTape& T = *tape;
   tapeCheck(usePGS, legacyFriction);
-  if (T.steps.empty()) T.numReal = numReal;
+  if (paramsVersion != T.paramsVersion)
+    throw ArgError("adjoint tape: the particle state, masses, inertia, scales or external forces "
+                   "were set while recording (open a new recording after setting them)");
   if (numReal != T.numReal) throw ArgError("adjoint tape: the particle count changed while recording");
   const int s = (int)T.steps.size();
   if (s % T.every != 0) return;
@@ -230,13 +270,14 @@ struct WS {  // replay workspace
   }
 };
 
-inline ad::RK constants(const pd_sim& s, const TapeStep& S) {
+// the recording's captured per-body constants (not the simulation's live arrays)
+inline ad::RK constants(const AdjTape& A, const TapeStep& S) {
   ad::RK K;
   K.P = S.P;
   K.vRest = S.vRest;
-  K.bc = s.bc.p;
-  K.scale = s.scale.p;
-  K.extForce = s.extForce.p;
+  K.bc = A.bc;
+  K.scale = A.scale;
+  K.extForce = A.extForce;
   return K;
 }
 
@@ -244,7 +285,7 @@ inline ad::RK constants(const pd_sim& s, const TapeStep& S) {
 template <class T>
 void replayStep(pd_sim& s, const TapeStep& S, ad::RB<T> B, double* vtape, double* ptape) {
   hipStream_t st = s.st;
-  const ad::RK K = constants(s, S);
+  const ad::RK K = constants(*s.tape, S);
   const ad::Topo& t = S.t;
   const int nR = K.P.numReal;
   ad::a_start<T><<<nb(nR), TPB, 0, st>>>(B, K);
@@ -326,7 +367,7 @@ struct GWS {  // adjoint workspace
 void reverseStep(pd_sim& s, const TapeStep& S, ad::RB<double> B, GWS& W, double* gL,
                  double* gLp, int nLp) {
   hipStream_t st = s.st;
-  const ad::RK K = constants(s, S);
+  const ad::RK K = constants(*s.tape, S);
   const ad::Topo& t = S.t;
   const int nR = K.P.numReal;
   const size_t sl = (size_t)t.nSlots;

@@ -1669,6 +1669,63 @@ __global__ void k_color_starts(const int* colTot, int* colStart) {
   colStart[c] = incl - v;
   if (c == 63) colStart[64] = incl;
 }
+// The last colour class (62) may hold rows that share a body; they run one after another, in
+// ascending edge order (the order of a serial scan over the edges).  These two launches place
+// that class's rows in edge order: contiguous chunks are counted, then each chunk writes its
+// members in order at its offset (ballot ranks within a wave, wave totals within the block).
+constexpr int CLS_BLOCKS = 256;
+__global__ __launch_bounds__(1024) void k_class_chunk_count(const int* colour, int numEdges,
+                                                            int cls, int* cnt) {
+  __shared__ int total;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  const int chunk = (numEdges + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * chunk, hi = min(numEdges, lo + chunk);
+  int c = 0;
+  for (int e = lo + (int)threadIdx.x; e < hi; e += blockDim.x) c += colour[e] == cls;
+  c = waveSumI(c);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(&total, c);
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[blockIdx.x] = total;
+}
+__global__ __launch_bounds__(1024) void k_class_chunk_place(const int* colour, int numEdges,
+                                                            int cls, int rowBegin,
+                                                            const int* cnt, int* perm,
+                                                            int* rowOf) {
+  __shared__ int waveCnt[16];
+  __shared__ int base;
+  if (threadIdx.x == 0) {
+    int b = rowBegin;
+    for (int q = 0; q < (int)blockIdx.x; ++q) b += cnt[q];
+    base = b;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int chunk = (numEdges + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * chunk, hi = min(numEdges, lo + chunk);
+  for (int e0 = lo; e0 < hi; e0 += 1024) {
+    const int e = e0 + (int)threadIdx.x;
+    const bool in = e < hi && colour[e] == cls;
+    const unsigned long long m = __ballot(in);
+    if (lane == 0) waveCnt[wave] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wave; ++w) off += waveCnt[w];
+    if (in) {
+      const int k = off + __popcll(m & ((1ull << lane) - 1ull));
+      perm[k] = e;
+      rowOf[e] = k;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < 16; ++w) t += waveCnt[w];
+      base += t;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_color_place(const int* colour, int numEdges, const int* blockBase,
                               const int* colStart, int* perm, int* rowOf) {
   __shared__ int cnt[64];
@@ -2345,15 +2402,14 @@ __global__ __launch_bounds__(TPB) void k_vsweep(const Dev* __restrict__ Dp, Para
   }
   if (MODE != 0) blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
 }
-// colour 62 may hold manifolds sharing a body (mask saturation): one thread, manifold order
-__global__ __launch_bounds__(64) void k_vsweep_serial(const Dev* __restrict__ Dp, Params P, int numManifolds,
-                                                      int colour, int mode, double thr,
+// colour 62 may hold manifolds sharing a body (mask saturation): one thread, in row order (the
+// class's rows are placed in manifold order)
+__global__ __launch_bounds__(64) void k_vsweep_serial(const Dev* __restrict__ Dp, Params P, int rowBegin,
+                                                      int rowEnd, int mode, double thr,
                                                       const int* stop) {
   const Dev& D = *Dp;
   if (gtid() != 0 || *stop) return;
-  for (int m = 0; m < numManifolds; ++m) {
-    if (D.mColor[m] != colour) continue;
-    const int k = D.vRowOf[m];
+  for (int k = rowBegin; k < rowEnd; ++k) {
     if (mode == 0) warmApplyRow(D, k);
     else if (mode == 1) atomicMaxD(D.maxApproachBits, pgsRow(D, P, k, thr));
     else atomicMaxD(D.maxApproachBits, gsRow(D, P, k, thr));
@@ -2396,9 +2452,7 @@ __device__ __forceinline__ void vSweepLoop(const Dev& D, const Params& P, const 
       if (e <= b) continue;
       if (c == 62) {
         if (threadIdx.x == 0)
-          for (int q = 0; q < numManifolds; ++q) {
-            if (D.mColor[q] != c) continue;
-            const int k = D.vRowOf[q];
+          for (int k = b; k < e; ++k) {
             int2 r, sl;
             bs.row(D, k, r, sl);
             const unsigned long long a = dbits(MODE == 1 ? pgsRowB(D, P, k, r, sl, thr, bs)
@@ -2442,8 +2496,7 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_block(
       if (e <= b) continue;
       if (c == 62) {
         if (threadIdx.x == 0)
-          for (int q = 0; q < numManifolds; ++q)
-            if (D.mColor[q] == c) warmApplyRow(D, D.vRowOf[q]);
+          for (int k = b; k < e; ++k) warmApplyRow(D, k);
       } else {
         for (int k = b + (int)threadIdx.x; k < e; k += blockDim.x) warmApplyRow(D, k);
       }
@@ -2573,9 +2626,7 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsolve_lds(
       if (e <= b) continue;
       if (c == 62) {
         if (tid == 0)
-          for (int q = 0; q < numManifolds; ++q) {
-            if (D.mColor[q] != c) continue;
-            const int k = D.vRowOf[q];
+          for (int k = b; k < e; ++k) {
             int2 r, sl;
             bs.row(D, k, r, sl);
             warmApplyRowB(D, k, r, sl, bs);
@@ -2889,11 +2940,11 @@ __global__ __launch_bounds__(TPB) void k_psweep(const Dev* __restrict__ Dp, int 
   const double o = (k < rowEnd) ? posRow(D, k) : 0.0;
   blockAtomicMaxU64(D.maxOverlapBits, dbits(o), sh);
 }
-__global__ void k_psweep_serial(const Dev* __restrict__ Dp, int numContacts, int colour, const int* stop) {
+// colour 62: one thread, in row order (the class's rows are placed in contact order)
+__global__ void k_psweep_serial(const Dev* __restrict__ Dp, int rowBegin, int rowEnd, const int* stop) {
   const Dev& D = *Dp;
   if (gtid() != 0 || *stop) return;
-  for (int c = 0; c < numContacts; ++c)
-    if (D.cColor[c] == colour) atomicMaxD(D.maxOverlapBits, posRow(D, D.pRowOf[c]));
+  for (int k = rowBegin; k < rowEnd; ++k) atomicMaxD(D.maxOverlapBits, posRow(D, k));
 }
 
 // position sweeps (XPBD), strict stop: residual < tol
@@ -2910,11 +2961,10 @@ __global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psolve_block(
       if (e <= b) continue;
       if (c == 62) {
         if (threadIdx.x == 0)
-          for (int q = 0; q < numContacts; ++q)
-            if (D.cColor[q] == c) {
-              const unsigned long long a = dbits(posRow(D, D.pRowOf[q]));
-              m = a > m ? a : m;
-            }
+          for (int k = b; k < e; ++k) {
+            const unsigned long long a = dbits(posRow(D, k));
+            m = a > m ? a : m;
+          }
       } else {
         for (int k = b + (int)threadIdx.x; k < e; k += blockDim.x) {
           const unsigned long long a = dbits(posRow(D, k));
@@ -2951,11 +3001,10 @@ __global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psweep_tail(
     if (e <= b) continue;
     if (c == 62) {
       if (threadIdx.x == 0)
-        for (int q = 0; q < numContacts; ++q)
-          if (D.cColor[q] == c) {
-            const unsigned long long a = dbits(posRow(D, D.pRowOf[q]));
-            m = a > m ? a : m;
-          }
+        for (int k = b; k < e; ++k) {
+          const unsigned long long a = dbits(posRow(D, k));
+          m = a > m ? a : m;
+        }
     } else {
       for (int k = b + (int)threadIdx.x; k < e; k += blockDim.x) {
         const unsigned long long a = dbits(posRow(D, k));

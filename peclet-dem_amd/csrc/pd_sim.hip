@@ -319,6 +319,11 @@ struct pd_sim {
   AdjTape* tape = nullptr;
   bool taping = false;  // a tape is open: every substep is recorded
   void tapeCheck(bool usePGS, bool legacyFriction) const;
+  void tapeCapture();
+  void tapeCount(long count) const;
+  // bumped by every setter of masses, inertia, scales or external forces (a recording fails if
+  // they change while it is open)
+  long paramsVersion = 0;
   void tapeSubstepStart(bool usePGS, bool legacyFriction);
   void tapeSubstepEnd(const Params& P, const std::vector<int>& vs, int numColors,
                       const std::vector<int>& ps, int numPosColors, int nRowsP, int velLeftover,
@@ -831,6 +836,7 @@ struct pd_sim {
     }
   }
   void setField(int f, const double* src, int n) {
+    ++paramsVersion;
     n = std::min(n, numReal);
     if (n <= 0) return;
     double* base;
@@ -856,6 +862,7 @@ struct pd_sim {
 
   // src/sim.hpp:1290-1330
   void setPositions(const double* xyz, int n) {
+    ++paramsVersion;
     if (n < 0) throw ArgError("negative particle count");
     // headroom for the periodic ghosts (src/periodicity.hpp:25-38); grown on demand
     const long est = (long)n + (long)n / 4 + slotHeadroom;
@@ -1086,7 +1093,7 @@ struct pd_sim {
     int smallAt = -1;      // round of the current batch where the one-workgroup kernel starts
     void* pinned = nullptr;     // read-back area (pinned host memory)
     void* pinnedDev = nullptr;  // its device view
-    DBuf<int> work[2], cnt, blockCnt, colCnt, colStart;
+    DBuf<int> work[2], cnt, blockCnt, colCnt, colStart, clsCnt;
     DBuf<unsigned long long> winner, mask;
     std::vector<int> hist;  // list size entering each round, last colouring of this job
     // the colouring being run
@@ -1107,7 +1114,7 @@ struct pd_sim {
     int postSeq = 0;
     void release() {
       work[0].release(); work[1].release(); cnt.release(); blockCnt.release(); colCnt.release();
-      colStart.release(); winner.release(); mask.release();
+      colStart.release(); winner.release(); mask.release(); clsCnt.release();
     }
   };
   ColourJob jobV, jobP;  // manifold colouring (main stream), contact colouring (side stream)
@@ -1260,6 +1267,16 @@ struct pd_sim {
       colourLaunch(J);
     }
     J.hist.swap(J.seen);
+    // the last class may hold rows sharing a body, which run one after another: its rows go in
+    // edge order
+    if (starts[63] > starts[62]) {
+      J.clsCnt.ensure(CLS_BLOCKS, J.stream);
+      const int nb = std::max(1, std::min(CLS_BLOCKS, nblk(J.numEdges)));
+      k_class_chunk_count<<<nb, 1024, 0, J.stream>>>(J.colour, J.numEdges, 62, J.clsCnt.p);
+      k_class_chunk_place<<<nb, 1024, 0, J.stream>>>(J.colour, J.numEdges, 62, starts[62],
+                                                     J.clsCnt.p, J.perm, J.rowOf);
+      HIPCHECK(hipGetLastError());
+    }
     int maxc = -1;
     for (int c = 0; c < 64; ++c)
       if (starts[c + 1] > starts[c]) maxc = c;
@@ -1357,7 +1374,7 @@ struct pd_sim {
       const int b = vs[c], e = vs[c + 1];
       if (e <= b) continue;
       if (c == 62) {
-        k_vsweep_serial<<<1, 1, 0, st>>>(dD, P, nm, c, mode, thr, stop);
+        k_vsweep_serial<<<1, 1, 0, st>>>(dD, P, b, e, mode, thr, stop);
       } else {
         if (mode == 0) {
           k_vsweep<0><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
@@ -1903,7 +1920,7 @@ struct pd_sim {
             const int cb = ps[c], ce = ps[c + 1];
             if (ce <= cb) continue;
             if (c == 62) {
-              k_psweep_serial<<<1, 1, 0, st>>>(dD, nc, c, ctl.p + CTL_STOP_P);
+              k_psweep_serial<<<1, 1, 0, st>>>(dD, cb, ce, ctl.p + CTL_STOP_P);
             } else {
               kernBegin();
               k_psweep<<<nblk(ce - cb), TPB, 0, st>>>(dD, cb, ce, ctl.p + CTL_STOP_P);
@@ -1937,7 +1954,7 @@ struct pd_sim {
           const int b = ps[c], e = ps[c + 1];
           if (e <= b) continue;
           if (c == 62)
-            k_psweep_serial<<<1, 1, 0, st>>>(dD, nc, c, ctl.p + CTL_NONE);
+            k_psweep_serial<<<1, 1, 0, st>>>(dD, b, e, ctl.p + CTL_NONE);
           else
             k_psweep<<<nblk(e - b), TPB, 0, st>>>(dD, b, e, ctl.p + CTL_NONE);
         }
@@ -2455,6 +2472,7 @@ int pd_set_external_forces(pd_sim* s, const double* f, int n) {
 }
 int pd_clear_external_forces(pd_sim* s) {
   return guard([&] {
+    ++s->paramsVersion;
     s->zero(s->extForce.p, s->extForce.cap);
     s->sync();
   });
@@ -2474,6 +2492,7 @@ int pd_set_inv_mass(pd_sim* s, const double* im, int n) {
 int pd_set_scales(pd_sim* s, const double* sc, int n) {
   // src/sim.hpp:1340-1346: target := given, scale := target
   return guard([&] {
+    ++s->paramsVersion;
     const int m = std::min(n, s->numReal);
     if (m > 0) s->writeN(s->targetScale.p, sc, m);
     HIPCHECK(hipMemcpyAsync(s->scale.p, s->targetScale.p, s->scale.cap * sizeof(double),
@@ -2485,6 +2504,7 @@ int pd_set_scales(pd_sim* s, const double* sc, int n) {
 int pd_set_scales_uniform(pd_sim* s, double v) {
   // src/sim.hpp:1332-1338
   return guard([&] {
+    ++s->paramsVersion;
     std::vector<double> h(s->numReal, v);
     if (s->numReal > 0) s->writeN(s->scale.p, h.data(), s->numReal);
     HIPCHECK(hipMemcpyAsync(s->targetScale.p, s->scale.p, s->scale.cap * sizeof(double),
@@ -2496,6 +2516,7 @@ int pd_set_scales_uniform(pd_sim* s, double v) {
 int pd_set_growth_params(pd_sim* s, double rate, double nf) {
   // src/sim.hpp:1417-1425
   return guard([&] {
+    ++s->paramsVersion;
     if (s->growthFactor == -1.0) s->growthFactor = (nf > 0.0) ? nf : 0.01;
     else if (nf > 0.0) s->growthFactor = nf;
     s->growthRate = rate;
@@ -2766,10 +2787,12 @@ int pd_tape_begin(pd_sim* s, int every) {
     if (!s->tape) s->tape = new AdjTape();
     s->tape->clear();
     s->tape->every = every;
+    s->tapeCapture();
     s->tape->on = true;
     s->taping = true;
   });
 }
+long pd_tape_generation(pd_sim* s) { return s->tape ? s->tape->generation : 0; }
 int pd_tape_end(pd_sim* s) {
   return guard([&] {
     if (!s->tape || !s->tape->on) throw ArgError("no open tape (pd_tape_begin)");
@@ -2779,16 +2802,18 @@ int pd_tape_end(pd_sim* s) {
   });
 }
 int pd_tape_length(pd_sim* s) { return s->tape ? (int)s->tape->steps.size() : 0; }
-int pd_tape_replay(pd_sim* s, double* state_out) {
+int pd_tape_replay(pd_sim* s, double* state_out, long count) {
   return guard([&] {
+    s->tapeCount(count);
     std::vector<double> out;
     adjrun::replayAll(*s, s->tape, out);
     if (state_out) std::memcpy(state_out, out.data(), out.size() * sizeof(double));
   });
 }
 int pd_tape_tangent(pd_sim* s, const double* dstate0, double dmu, double drest, double* state_out,
-                    double* dstate_out) {
+                    double* dstate_out, long count) {
   return guard([&] {
+    s->tapeCount(count);
     const size_t n = s->tape ? (size_t)13 * s->tape->numReal : 0;
     std::vector<double> d0(n, 0.0), v, t;
     if (dstate0) std::memcpy(d0.data(), dstate0, n * sizeof(double));
@@ -2798,8 +2823,9 @@ int pd_tape_tangent(pd_sim* s, const double* dstate0, double dmu, double drest, 
   });
 }
 int pd_tape_adjoint(pd_sim* s, const double* grad_state_end, double* grad_state0,
-                    double* grad_params) {
+                    double* grad_params, long count) {
   return guard([&] {
+    s->tapeCount(count);
     if (!grad_state_end) throw ArgError("grad_state_end is required");
     const size_t n = s->tape ? (size_t)13 * s->tape->numReal : 0;
     std::vector<double> gE(grad_state_end, grad_state_end + n), g0;

@@ -464,29 +464,40 @@ class Simulation:
     def tape_length(self):
         return int(self._b.tape_length(self._h))
 
+    def tape_generation(self):
+        """Number of the current recording (one more at every ``tape_begin``; 0 before)."""
+        return int(self._b.tape_generation(self._h))
+
+    def _state_array(self, a, what):
+        n = self.num_particles()
+        a = _f64(a)
+        if a.shape != (n, 13):
+            raise ValueError(f"{what} must have shape ({n}, 13), got {a.shape}")
+        return a
+
     def tape_replay(self):
         """End state (n, 13) of the recorded substeps replayed from the first checkpoint."""
         out = np.empty((self.num_particles(), 13), np.float64)
-        self._chk(self._b.tape_replay(self._h, _ptr(out)))
+        self._chk(self._b.tape_replay(self._h, _ptr(out), out.size))
         return out
 
     def tape_tangent(self, dstate0=None, dmu=0.0, drest=0.0):
         """Forward mode: (end state, its derivative along dstate0 (n, 13), dmu, drest)."""
         n = self.num_particles()
-        d0 = None if dstate0 is None else _f64(dstate0, 13)
+        d0 = None if dstate0 is None else self._state_array(dstate0, "dstate0")
         val = np.empty((n, 13), np.float64)
         tan = np.empty((n, 13), np.float64)
         self._chk(self._b.tape_tangent(self._h, None if d0 is None else _ptr(d0), float(dmu),
-                                       float(drest), _ptr(val), _ptr(tan)))
+                                       float(drest), _ptr(val), _ptr(tan), val.size))
         return val, tan
 
     def tape_adjoint(self, grad_state_end):
         """Reverse mode: (dL/d start state (n, 13), dL/d (friction, restitution)) for
         grad_state_end = dL/d end state (n, 13)."""
-        g = _f64(grad_state_end, 13)
+        g = self._state_array(grad_state_end, "grad_state_end")
         g0 = np.empty_like(g)
         gp = np.zeros(2, np.float64)
-        self._chk(self._b.tape_adjoint(self._h, _ptr(g), _ptr(g0), _ptr(gp)))
+        self._chk(self._b.tape_adjoint(self._h, _ptr(g), _ptr(g0), _ptr(gp), g.size))
         return g0, gp
 
     def synchronize(self):

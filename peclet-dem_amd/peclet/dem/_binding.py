@@ -143,9 +143,10 @@ PRODUCT_SIGNATURES = {
     "tape_begin": (_i, [_p, _i]),
     "tape_end": (_i, [_p]),
     "tape_length": (_i, [_p]),
-    "tape_replay": (_i, [_p, _dp]),
-    "tape_tangent": (_i, [_p, _dp, _d, _d, _dp, _dp]),
-    "tape_adjoint": (_i, [_p, _dp, _dp, _dp]),
+    "tape_generation": (C.c_long, [_p]),
+    "tape_replay": (_i, [_p, _dp, C.c_long]),
+    "tape_tangent": (_i, [_p, _dp, _d, _d, _dp, _dp, C.c_long]),
+    "tape_adjoint": (_i, [_p, _dp, _dp, _dp, C.c_long]),
 }
 
 

@@ -36,11 +36,16 @@ class _Rollout(torch.autograd.Function):
         end = np.concatenate([sim.get_positions(), sim.get_velocities(),
                               sim.get_angular_velocities(), sim.get_quaternions()], axis=1)
         ctx.sim = sim
+        ctx.generation = sim.tape_generation()
         ctx.device = vel0.device
         return torch.from_numpy(end).to(vel0.device)
 
     @staticmethod
     def backward(ctx, grad_end):
+        if ctx.sim.tape_generation() != ctx.generation:
+            raise RuntimeError("rollout backward: the simulation has recorded another rollout "
+                               "since this one (its recording was replaced); run backward before "
+                               "the next rollout or tape_begin on the same simulation")
         g = grad_end.detach().cpu().double().numpy()
         g0, gp = ctx.sim.tape_adjoint(g)
         dev = ctx.device

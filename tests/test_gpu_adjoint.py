@@ -132,7 +132,7 @@ def test_adjoint_matches_central_differences_of_the_oracle(hip_lib, kind, fricti
     checks.append(("restitution", lambda s: loss(de=s), gp[1]))
     if friction > 0.0:
         checks.append(("friction", lambda s: loss(dmu=s), gp[0]))
-    smooth = 0
+    skipped = []
     for name, f, adj in checks:
         p6, m6, p7, m7 = f(1e-6), f(-1e-6), f(1e-7), f(-1e-7)
         fd6, fd7 = (p6 - m6) / 2e-6, (p7 - m7) / 2e-7
@@ -142,10 +142,14 @@ def test_adjoint_matches_central_differences_of_the_oracle(hip_lib, kind, fricti
         print(f"  {name:12s} fd {fd6:.9e} {fd7:.9e} one-sided {right:.6e} {left:.6e} "
               f"adjoint {adj:.9e} smooth {ok_fd}", flush=True)
         if not ok_fd:
+            skipped.append(name)
             continue
-        smooth += 1
         assert abs(fd6 - adj) <= 1e-4 * abs(adj) + 1e-10, (name, fd6, adj)
-    assert smooth >= len(checks) // 2, f"only {smooth} of {len(checks)} directions smooth"
+    # directions where the loss is not differentiable over the step: few, and never the
+    # material parameters
+    print(f"  skipped (not differentiable over the step): {skipped}", flush=True)
+    assert len(skipped) <= 2, skipped
+    assert "restitution" not in skipped and "friction" not in skipped, skipped
 
 
 @pytest.mark.gpu
@@ -183,3 +187,144 @@ def test_torch_rollout_gradients(hip_lib):
     assert np.isclose(mu.grad.item(), gp[0], rtol=1e-9)
     assert np.isclose(e.grad.item(), gp[1], rtol=1e-9)
     assert np.abs(v0.grad.numpy()).max() > 0.0
+
+
+def settling_bed_64(seed=5):
+    """A reduced form of the 50,000-sphere differentiation case: 64 spheres r ~ U[0.9, 1.1] mm
+    seeded at solid fraction 0.294 in a 3:3:10 periodic-x/y box over a floor, v ~ N(0, 0.01 m/s),
+    mu 0.5, e 0.3, dt 1e-5 s."""
+    from peclet.dem.workloads import sphere_bed
+
+    n = 64
+    mean_r3 = (1.1e-3 ** 4 - 0.9e-3 ** 4) / (4.0 * 0.2e-3)
+    vol = n * 4.0 / 3.0 * np.pi * mean_r3 / 0.294
+    a = (vol * 0.3) ** (1.0 / 3.0)
+    bed = sphere_bed(n, 0.9e-3, 1.1e-3, (a, a, a / 0.3), seed)
+    bed["vel"] = np.random.default_rng(seed).normal(0.0, 0.01, (n, 3))
+    return bed
+
+
+def loss_gradient(end, radius, area):
+    """Loss = mean bed height + solid fraction below 0.6 H (H = 2 x the mean centre height) of an
+    end state (n, 13), and its gradient with respect to the end state."""
+    n = len(end)
+    z = end[:, 2]
+    H = 2.0 * z.mean()
+    inside = z < 0.6 * H
+    phi = (inside * 4.0 / 3.0 * np.pi * radius ** 3).sum() / (area * 0.6 * H)
+    g = np.zeros((n, 13))
+    g[:, 2] = 1.0 / n - phi / H * (2.0 / n)
+    return z.mean() + phi, g
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_settling_bed_gradients_every_direction(hip_lib):
+    """Every initial-velocity component, friction and restitution of the reduced differentiation
+    case against central differences (step 1e-6) of the oracle, over 16 recorded substeps (two
+    checkpoint segments) after 1500 substeps of settling.  A direction may be left out only where
+    the loss is not differentiable over the difference step (its two one-sided slopes differ):
+    at most 2 of the 194, each printed."""
+    from peclet.dem.workloads import setup_gravity_bed
+
+    bed = settling_bed_64()
+    n, dt, pre, steps = len(bed["pos"]), 1e-5, 1500, 16
+    area = bed["domain"][0] * bed["domain"][1]
+
+    def prepared(sim):
+        setup_gravity_bed(sim, bed, dt, 0.5, 0.3)
+        sim.set_velocities(bed["vel"])
+        for _ in range(pre):
+            sim.step(dt)
+        return sim
+
+    h = prepared(hip_sim(n))
+    assert h.get_stats()["active_manifolds"] > 0
+    h.tape_begin(8)
+    for _ in range(steps):
+        h.step(dt)
+    h.tape_end()
+    _, gE = loss_gradient(state(h), bed["radius"], area)
+    g0, gp = h.tape_adjoint(gE)
+
+    def loss(dv=None, dmu=0.0, de=0.0):
+        r = prepared(oracle_sim(n))
+        if dv is not None:
+            r.set_velocities(r.get_velocities() + dv)
+        r.set_material_params(0.3 + de, 0.0, 0.5 + dmu)
+        for _ in range(steps):
+            r.step(dt)
+        return loss_gradient(state(r), bed["radius"], area)[0]
+
+    f0 = loss()
+    checks = []
+    for i in range(n):
+        for c in range(3):
+            d = np.zeros((n, 3))
+            d[i, c] = 1.0
+            checks.append((f"v[{i},{c}]", (lambda s, d=d: loss(dv=s * d)), g0[i, 3 + c]))
+    checks.append(("restitution", lambda s: loss(de=s), gp[1]))
+    checks.append(("friction", lambda s: loss(dmu=s), gp[0]))
+    skipped = []
+    worst = 0.0
+    scale = max(abs(a) for _, _, a in checks)
+    for name, f, adj in checks:
+        p6, m6 = f(1e-6), f(-1e-6)
+        fd = (p6 - m6) / 2e-6
+        right, left = (p6 - f0) / 1e-6, (f0 - m6) / 1e-6
+        err = abs(fd - adj)
+        tol = 1e-4 * abs(adj) + 1e-8 * scale
+        if err > tol and abs(right - left) > 1e-3 * max(abs(right), abs(left), 1e-30):
+            skipped.append(name)
+            print(f"  {name}: not differentiable over the step (one-sided {right:.6e} "
+                  f"{left:.6e}), adjoint {adj:.6e}", flush=True)
+            continue
+        worst = max(worst, err / max(abs(adj), 1e-30))
+        assert err <= tol, (name, fd, adj)
+    print(f"  {len(checks) - len(skipped)} of {len(checks)} directions within 1e-4 "
+          f"(worst relative {worst:.2e}); skipped {skipped}", flush=True)
+    assert len(skipped) <= 2
+    assert "restitution" not in skipped and "friction" not in skipped
+    assert abs(gp[0]) > 0.0 and abs(gp[1]) > 0.0
+
+
+@pytest.mark.gpu
+def test_recording_guards(hip_lib):
+    """A rollout's backward refuses once a later recording replaced its tape; setting masses
+    while recording fails the next substep; state arrays of the wrong size are refused (Python
+    shape check and the C-ABI's element count)."""
+    import ctypes as C
+
+    import torch
+
+    from peclet.dem.adjoint import rollout
+
+    n = 100
+    scene = make_scene("bed_periodic", n, 0)
+    h = apply_scene(hip_sim(n), scene)
+    h.step(scene["dt"])
+    v0 = torch.tensor(h.get_velocities(), dtype=torch.float64, requires_grad=True)
+    g0 = h.tape_generation()
+    first = rollout(h, v0, 0.5, 0.3, scene["dt"], 4, checkpoint_every=2)
+    assert h.tape_generation() == g0 + 1
+    second = rollout(h, v0, 0.5, 0.3, scene["dt"], 4, checkpoint_every=2)
+    assert h.tape_generation() == g0 + 2
+    with pytest.raises(RuntimeError, match="replaced"):
+        first[:, 2].sum().backward()
+    second[:, 2].sum().backward()  # the current recording still differentiates
+    assert torch.all(torch.isfinite(v0.grad))
+    with pytest.raises(ValueError, match="shape"):
+        h.tape_adjoint(np.zeros((n, 6)))
+    g = np.zeros((n, 13))
+    out = np.zeros((n, 13))
+    gp = np.zeros(2)
+    dp = C.POINTER(C.c_double)
+    rc = h._b.tape_adjoint(h._h, g.ctypes.data_as(dp), out.ctypes.data_as(dp),
+                           gp.ctypes.data_as(dp), 13 * n - 1)
+    assert rc == -2
+    h.tape_begin(4)
+    h.step(scene["dt"])
+    h.set_inv_mass(np.full(n, 2.0))
+    with pytest.raises(ValueError, match="while recording"):
+        h.step(scene["dt"])
+    h.tape_end()

@@ -435,3 +435,33 @@ def test_candidate_pairs_equal_a_fresh_broadphase(hip_lib, kind):
         assert np.array_equal(a.debug_pairs(), b.debug_pairs()), (kind, k)
     for name in ["get_positions", "get_velocities", "get_angular_velocities"]:
         assert np.array_equal(getattr(a, name)(), getattr(b, name)()), name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["bed_periodic", "gas"])
+def test_growth_ramp(hip_lib, kind):
+    # particles grown from 0.9 of their target scales towards them (updateGrowthScalesKokkos,
+    # src/integration.hpp:275-283; the factor ramp of src/sim.hpp:100-108): the scales, the
+    # factor and every substep's contacts and state equal the oracle's while the ramp runs and
+    # after it has reached 1
+    n = 300
+    scene = make_scene(kind, n, 5)
+    h = apply_scene(hip_sim(n), scene)
+    r = apply_scene(oracle_sim(n), scene)
+    rate = np.log(1.0 / 0.9) / (15 * scene["dt"])  # the factor reaches 1 at substep 15
+    for s in (h, r):
+        s.set_growth_params(rate, 0.9)
+    grew = False
+    for k in range(30):
+        tag = f"growth {kind} substep {k}"
+        print(tag, flush=True)
+        h.step(scene["dt"])
+        r.step(scene["dt"])
+        assert h.get_growth_factor() == r.get_growth_factor(), tag
+        assert np.array_equal(h.get_scales(), r.get_scales()), tag
+        grew |= 0.9 < h.get_growth_factor() < 1.0
+        assert_stats(h, r, tag)
+        assert_topology(h, r, tag, colours=scene.get("use_gs", True))
+        assert_state(h, r, tag)
+        assert_ledger(h, r, tag)
+    assert grew and h.get_growth_factor() == 1.0
