@@ -701,3 +701,186 @@ def test_periodic_rsa_seeds_across_the_seams():
     straddle = ((p[:, 0] < r) | (p[:, 0] > 0.02 - r) | (p[:, 1] < r) | (p[:, 1] > 0.02 - r))
     assert straddle.sum() > 100
     assert (p[:, 2] - r).min() >= 0.0 and (p[:, 2] + r).max() <= 0.08
+
+
+# --- the Jacobi (count-averaged) solves: set_velocity_use_gs(False) runs them over every
+# manifold / contact (solveVelocityJacobi + applyVelocityDeltasAveraged,
+# src/solver_velocity.hpp:38-255; positionSolveJacobi, src/solver_position.hpp:34-134 with
+# applyUpdates, src/integration.hpp:168-184); the colouring leftovers run the same functions over
+# the uncoloured edges only.
+
+def _supported_ball(n_sup, angle, gap, vz, e, pos_iters=1):
+    """A unit ball (radius 0.5, mass 1) above n_sup fixed unit spheres placed symmetrically at
+    polar angle `angle` from the vertical, each at centre distance 1 + gap; zero gravity."""
+    s = oracle_sim(n_sup + 1)
+    s.set_sphere_shape(0.5)
+    s.set_domain((-5, -5, -5), (5, 5, 5))
+    s.enable_periodicity(False, False, False)
+    d = 1.0 + gap
+    pos = [[0.0, 0.0, 0.0]]
+    for k in range(n_sup):
+        phi = 2.0 * math.pi * k / n_sup
+        pos.append([-d * math.sin(angle) * math.cos(phi), -d * math.sin(angle) * math.sin(phi),
+                    -d * math.cos(angle)])
+    s.set_positions(np.array(pos))
+    s.set_inv_mass(np.array([1.0] + [0.0] * n_sup))
+    s.set_inv_inertia(np.array([[10.0] * 3] + [[0.0] * 3] * n_sup))
+    s.set_velocities(np.array([[0.0, 0.0, vz]] + [[0.0] * 3] * n_sup))
+    s.set_gravity(0, 0, 0)
+    s.set_material_params(e, 0.0, 0.0)
+    s.set_solver_iterations(pos_iters, 4)
+    s.set_velocity_use_gs(False)
+    return s
+
+
+@pytest.mark.parametrize("e", [0.0, 0.3, 0.9])
+def test_jacobi_isolated_contact_is_binary_restitution(e):
+    """One contact: count 1, weight min(1, 2/1) = 1, so the Jacobi impulse is the exact binary
+    collision: the approach velocity comes back times -e."""
+    s = _supported_ball(1, 0.0, -1e-3, -1.0, e)
+    s.step(1e-4)
+    v = s.get_velocities()
+    assert v[0, 2] == pytest.approx(e, abs=1e-12)
+    assert np.all(v[1] == 0.0) and abs(v[0, 0]) < 1e-12 and abs(v[0, 1]) < 1e-12
+
+
+@pytest.mark.parametrize("n_sup", [2, 3, 4])
+@pytest.mark.parametrize("e", [0.2, 0.6])
+def test_jacobi_symmetric_supports_count_averaged(n_sup, e):
+    """k symmetric fixed supports at 30 degrees: each contact alone would reverse its own normal
+    approach, (1+e) V cos(a) along n_i; the k deltas are summed and scaled by min(1, 2/k), so the
+    ball leaves with v_z = -V + min(1, 2/k) k (1+e) V cos^2(a) = -V + 2 (1+e) V cos^2(a) for every
+    k >= 2 (the first sweep; afterwards every contact separates and is skipped)."""
+    a, V, dt = math.radians(30.0), 1.0, 1e-4
+    s = _supported_ball(n_sup, a, -1e-3, -V, e)
+    s.step(dt)
+    v = s.get_velocities()[0]
+    # contact normals at the predicted positions (ball moved by -V dt); the narrowphase's
+    # central-difference normal matches the exact direction to O(eps^2) = 1e-8
+    d = 1.0 - 1e-3
+    nz = (d * math.cos(a) - V * dt) / math.hypot(d * math.sin(a), d * math.cos(a) - V * dt)
+    assert v[2] == pytest.approx(-V + 2.0 * (1.0 + e) * V * nz ** 2, abs=1e-7)
+    # the nominal-angle value differs by 1e-4: the check resolves the normals
+    assert abs(v[2] - (-V + 2.0 * (1.0 + e) * V * math.cos(a) ** 2)) > 1e-5
+    assert abs(v[0]) < 1e-8 and abs(v[1]) < 1e-8  # symmetric (up to the supports' rounding)
+
+
+def test_jacobi_position_averaging_converges_geometrically():
+    """A ball overlapping two fixed spheres at 30 degrees by delta each: the position rows keep
+    the narrowphase's normals and arms, so each constraint is linear in the ball's position.
+    Every Jacobi sweep moves the ball by the mean of the two single-contact corrections,
+    delta_k cos(a) upwards, and each overlap shrinks by the factor sin^2(a): after k sweeps the
+    ball has risen by delta cos(a) (1 - sin^(2k) a) / (1 - sin^2 a) = delta (1 - 4^-k) / cos(a)."""
+    a, delta = math.radians(30.0), 1e-3
+    for k in (1, 2, 12):
+        s = _supported_ball(2, a, -delta, 0.0, 0.0, pos_iters=k)
+        s.step(1e-4)
+        z = s.get_positions()[0, 2]
+        assert z == pytest.approx(delta * (1.0 - 0.25 ** k) / math.cos(a), rel=1e-9), k
+        assert s.get_stats()["sweeps_pos"] == k  # no early stop on the Jacobi path
+
+
+# --- stabilization modes "escalate" (3) and "ordered" (4), src/sim.hpp:356-415: a resting
+# column of spheres on the floor being compressed (zero restitution), against a one-dimensional
+# model of the same solve.
+# Every contact of the column is vertical, so the PGS row reduces to scalars: with approach
+# speed a (lower body's v_z minus the upper's), inverse-mass sum w and accumulated impulse p,
+# the row applies d = max(0, p + a / w) - p (restitution zero), the
+# upper body gains d / m, the lower loses it, and the sweep residual is max |d| w.  The model
+# orders the rows the way each mode specifies (colour classes; or height levels bottom-up, then
+# top-down) and stops where the mode stops.
+
+def _column(n, mode, dt=1e-3, g=10.0, iters=8):
+    s = oracle_sim(n)
+    s.set_sphere_shape(0.5)
+    s.set_domain((-5, -5, -1), (5, 5, 2 * n + 5))
+    s.enable_periodicity(False, False, False)
+    s.add_plane(0, 0, 0, 0, 0, 1)
+    ov = 1e-4
+    s.set_positions(np.array([[0.0, 0.0, 0.5 - ov + k * (1.0 - ov)] for k in range(n)]))
+    s.set_inv_mass(np.ones(n))
+    s.set_inv_inertia(np.full((n, 3), 10.0))
+    vel = np.zeros((n, 3))
+    vel[:, 2] = -0.05 * (np.arange(n) + 1)  # the column is being compressed
+    s.set_velocities(vel)
+    s.set_gravity(0, 0, -g)
+    s.set_material_params(0.0, 0.0, 0.0)
+    s.set_solver_iterations(4, iters)
+    s.set_stabilization_mode({0: "off", 3: "escalate", 4: "ordered"}[mode])
+    return s
+
+
+def _column_model(n, rows, colours, mode, dt, g, iters):
+    """rows: (lower, upper) body pairs (lower -1 = the floor); returns (v_z, main sweeps,
+    stabilization sweeps)."""
+    v = -0.05 * (np.arange(n) + 1) - g * dt  # predicted velocities (first substep, no warm start)
+    p = np.zeros(len(rows))
+    vrest = 2.0 * dt * g
+
+    def row(k):
+        lo, up = rows[k]
+        a = (v[lo] if lo >= 0 else 0.0) - v[up]
+        w = 1.0 + (1.0 if lo >= 0 else 0.0)
+        new = max(0.0, p[k] + a / w)
+        d = new - p[k]
+        p[k] = new
+        v[up] += d
+        if lo >= 0:
+            v[lo] -= d
+        return abs(d) * w
+
+    by_colour = [k for c in sorted(set(colours)) for k in range(len(rows)) if colours[k] == c]
+    sweeps = stab = 0
+    m = 0.0
+    for _ in range(iters):
+        m = max(row(k) for k in by_colour)
+        sweeps += 1
+        if m <= 0.02 * vrest:
+            break
+    if mode != 0 and m > vrest:
+        if mode == 3:
+            for _ in range(256):
+                m = max(row(k) for k in by_colour)
+                stab += 1
+                if m <= vrest:
+                    break
+        elif mode == 4:
+            # level of a row = the lower body's height level (floor row: 0); key = level, colour
+            level = [max(lo, 0) for lo, up in rows]
+            order = sorted(range(len(rows)), key=lambda k: (level[k], colours[k], k))
+            for _ in range(2 * iters):
+                m = max(row(k) for k in order)
+                m = max(m, max(row(k) for k in reversed(order)))
+                stab += 1
+                if m <= vrest:
+                    break
+    return v, sweeps, stab
+
+
+@pytest.mark.parametrize("mode", [0, 3, 4])
+@pytest.mark.parametrize("n", [8, 16])
+def test_column_stabilization_matches_the_1d_model(mode, n):
+    dt, g, iters = 1e-3, 10.0, 4
+    s = _column(n, mode, dt, g, iters)
+    s.step(dt)
+    ids, _ = s.debug_manifolds()
+    colours = s.debug_manifold_colors()
+    z = s.get_positions()[:, 2]
+    rows, cols = [], []
+    for (a, b), c in zip(ids[:, :2], colours):
+        if b < 0:
+            rows.append((-1, int(a)))
+        else:
+            lo, up = (a, b) if z[a] < z[b] else (b, a)
+            rows.append((int(lo), int(up)))
+        cols.append(int(c))
+    assert len(rows) == n  # the floor contact and n - 1 neighbour contacts
+    v, sweeps, stab = _column_model(n, rows, cols, mode, dt, g, iters)
+    st = s.get_stats()
+    assert st["sweeps_vel"] == sweeps and st["sweeps_stab"] == stab, (st, sweeps, stab)
+    vz = s.get_velocities()[:, 2]
+    np.testing.assert_allclose(vz, v, rtol=0, atol=1e-12 * g * dt)
+    if mode:
+        assert stab > 0  # the column did not converge in the main loop
+    if mode == 3:
+        assert np.abs(vz).max() <= n * 2.0 * dt * g  # drained to the rest threshold per contact

@@ -161,3 +161,108 @@ def test_hertz_rejects_periodic_and_planes():
         s.step_hertz(1e-4)
     with pytest.raises(ValueError):
         s.set_hertz_material(8, 1e7, 0.3)
+
+
+# --- grid-SDF shapes under the Hertz engine: per-point shell springs against another body or an
+# SDF wall (hertzShapePairForces, src/solver_hertz.hpp:454-606; the shape branch of
+# hertzWallForces, :148-338), pinned by the closed forms of the reference's test_hertz_shapes.py
+
+def _grid(f, lo, hi, res):
+    axes = [np.linspace(lo, hi, res)] * 3
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    return f(p).reshape(res, res, res).ravel(order="F"), (hi - lo) / (res - 1)
+
+
+def _fibonacci_sphere(n, r):
+    k = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * k / n
+    th = math.pi * (3.0 - math.sqrt(5.0)) * np.arange(n)
+    s = np.sqrt(1.0 - z * z)
+    return r * np.stack([s * np.cos(th), s * np.sin(th), z], 1)
+
+
+def _cube_shell(a, m):
+    """Points on the faces of the cube [-a, a]^3: an m x m lattice per face (edges and corners
+    included once)."""
+    t = np.linspace(-a, a, m)
+    pts = {(round(x, 12), round(y, 12), round(z, 12))
+           for x in t for y in t for z in t
+           if max(abs(x), abs(y), abs(z)) >= a - 1e-12}
+    return np.array(sorted(pts))
+
+
+def _sdf_sphere(s, r=0.5):
+    g, h = _grid(lambda p: np.linalg.norm(p, axis=1) - r, -0.7, 0.7, 56)
+    s.set_sdf_shape(g, 56, 56, 56, (-0.7,) * 3, (h,) * 3, _fibonacci_sphere(220, r),
+                    (1.0, 1.0, 1.0), r)
+
+
+def _sdf_cube(s, a=0.5):
+    def f(p):
+        q = np.abs(p) - a
+        return np.linalg.norm(np.maximum(q, 0.0), axis=1) + np.minimum(np.max(q, axis=1), 0.0)
+
+    g, h = _grid(f, -0.8, 0.8, 56)
+    s.set_sdf_shape(g, 56, 56, 56, (-0.8,) * 3, (h,) * 3, _cube_shell(a, 8), (6.0, 6.0, 6.0),
+                    a * math.sqrt(3.0))
+
+
+@pytest.mark.parametrize("e", [0.3, 0.8])
+def test_sdf_sphere_binary_restitution(e):
+    """Two grid-SDF spheres (220-point shells) colliding head-on at relative speed 2: the
+    effective restitution is the analytic path's e within the shell discretisation, 0.08
+    (test_hertz_shapes.py:34-51, :82-86)."""
+    s = oracle_sim(2)
+    _sdf_sphere(s)
+    s.set_domain((0, 0, 0), (20, 20, 20))
+    s.enable_periodicity(False, False, False)
+    s.set_positions(np.array([[9.45, 10.0, 10.0], [10.55, 10.0, 10.0]]))  # 0.1 apart
+    s.set_scales_uniform(1.0)
+    s.set_inv_mass(np.ones(2))
+    s.set_inv_inertia(np.ones((2, 3)))
+    s.set_velocities(np.array([[1.0, 0, 0], [-1.0, 0, 0]]))
+    s.set_gravity(0, 0, 0)
+    s.set_material_params(e, 0.0, 0.0)
+    s.set_hertz_material(0, E0, NU0)
+    s.step_hertz(2e-5, 6000)  # contact at t = 0.05 s, over well before 0.12 s
+    v = s.get_velocities()
+    e_eff = (v[1, 0] - v[0, 0]) / 2.0
+    assert abs(e_eff - e) < 0.08, e_eff
+    assert abs(v[0, 0] + v[1, 0]) < 1e-9  # momentum
+    assert v[0, 0] < 0.0 < v[1, 0]  # separating again
+    p = s.get_positions()
+    assert p[1, 0] - p[0, 0] > 1.0  # out of contact (centres more than a diameter apart)
+
+
+def _cube_on_incline(mu, steps):
+    s = oracle_sim(1)
+    _sdf_cube(s)
+    s.set_domain((0, 0, -1.0), (60, 12, 8))
+    s.enable_periodicity(False, False, False)
+    w = build_wall_sdf(lambda p: p[:, 2], ((0, 0, -1.0), (60, 12, 8)), resolution=(96, 24, 24))
+    w.add_to(s, restitution=0.1, friction=mu)
+    s.set_positions(np.array([[4.0, 4.0, 0.505]]))
+    s.set_scales_uniform(1.0)
+    s.set_inv_mass(np.ones(1))
+    s.set_inv_inertia(np.full((1, 3), 6.0))  # I = (2/3) m a^2 = 1/6 for a = 0.5
+    s.set_velocities(np.zeros((1, 3)))
+    s.set_gravity(5.0, 0.0, -10.0)  # a tan(theta) = 0.5 incline
+    s.set_material_params(0.1, 0.0, mu)
+    s.set_hertz_material(0, E0, NU0)
+    s.step_hertz(2e-5, steps)
+    p = s.get_positions()[0]
+    return float(p[0] - 4.0), float(p[2])
+
+
+def test_cube_sticks_and_slides_on_an_incline():
+    """A cube resting flat on a tan(theta) = 0.5 incline with free rotation sticks at mu = 0.9
+    (tan(theta) < mu: no motion beyond the spring settling) and slides at mu = 0.05 (kinetic
+    acceleration g (sin - mu cos) = 5 - 0.05 x 10 = 4.5, so ~9 units in 2 s), staying flat on the
+    floor (test_hertz_shapes.py:54-92)."""
+    d_hi, z_hi = _cube_on_incline(0.9, 50000)  # 1 s
+    d_lo, z_lo = _cube_on_incline(0.05, 100000)  # 2 s
+    assert abs(d_hi) < 1.0, d_hi
+    assert d_lo > 5.0, d_lo
+    assert d_lo < 0.5 * 4.5 * 2.0 ** 2 * 1.05  # no faster than the kinetic closed form
+    assert 0.3 < z_hi < 0.7 and 0.3 < z_lo < 0.7
