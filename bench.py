@@ -207,7 +207,7 @@ def copy_probe(mib):
     print(f"# copy probe: 3 copies of {mib} MiB", file=sys.stderr, flush=True)
 
 
-def run_batched(num, seed, steps, seq_steps, workers):
+def run_batched(num, seed, steps, seq_steps, workers, autotune=True):
     """Config-4 style ensemble: aggregate particle-substeps/s of one batch, against the same
     systems stepped one after another (a single worker)."""
     from peclet.dem import Batch
@@ -220,6 +220,8 @@ def run_batched(num, seed, steps, seq_steps, workers):
         setup_gravity_bed(sim, bed, dt, bed["mu"], bed["e"])
     total = sum(len(b["pos"]) for b in beds)
     batch.step(dt, 2)  # warm-up: buffers reach their working size
+    if autotune:
+        batch.autotune(dt, 2)  # the largest system of each size class is tuned
     batch.set_workers(1)
     t0 = time.perf_counter()
     batch.step(dt, seq_steps)
@@ -236,7 +238,7 @@ def run_batched(num, seed, steps, seq_steps, workers):
     return {"value": v_b, "unit": "particle-substeps/s", "systems": num, "particles": total,
             "substeps_timed": steps, "workers": workers, "ms_per_substep": 1e3 * t_b / steps,
             "sequential_value": v_s, "sequential_substeps_timed": seq_steps,
-            "speedup_vs_sequential": v_b / v_s,
+            "speedup_vs_sequential": v_b / v_s, "autotuned": autotune,
             "algorithmic_bytes_per_substep": bps,
             "hbm_frac": bps * steps / t_b / 1e9 / HBM_PEAK_GBS,
             "workload": f"{num} independent periodic-xy sphere beds, N~U[500,20000], "
@@ -374,8 +376,10 @@ def timed_run(sim, dt, substeps, timing_every, level, label):
     return out
 
 
-def plain_leg(kind, n, seed, column, dt, warmup, substeps, timing_every, level, tuning=()):
-    """Build one plain workload, run `warmup` untimed substeps and time `substeps`."""
+def plain_leg(kind, n, seed, column, dt, warmup, substeps, timing_every, level, tuning=(),
+              autotune=False):
+    """Build one plain workload, run `warmup` untimed substeps (and, with `autotune`, the
+    autotuner's trial substeps: its knobs change no result) and time `substeps`."""
     from peclet.dem import Simulation
 
     t_gen = time.perf_counter()
@@ -387,7 +391,13 @@ def plain_leg(kind, n, seed, column, dt, warmup, substeps, timing_every, level, 
     print(f"# {kind}: n={n} ready (generation {t_gen:.1f}s)", file=sys.stderr, flush=True)
     for _ in range(warmup):
         sim.step(dt)
+    chosen = None
+    if autotune:
+        chosen, best = sim.autotune(dt, substeps=2, use_cache=False)
+        print(f"# {kind}: autotuned {chosen} ({best:.3f} ms per substep in its trials)",
+              file=sys.stderr, flush=True)
     res = timed_run(sim, dt, substeps, timing_every, level, kind)
+    res["tuning"] = chosen
     res["workload"] = WORKLOAD_TEXT[kind].format(n=n, col=f"{column * 1e3:.0f}")
     if "roofline" in res:
         res["roofline"]["traffic"] = load_traffic(kind, res["roofline"]["kernel"])
@@ -462,6 +472,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--tuning", action="append", default=[], metavar="NAME=VALUE",
                     help="any library tuning knob (pd_set_tuning) for the headline")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the headline and the batch with the library's default knobs "
+                         "instead of autotuning them in the untimed warm-up")
     ap.add_argument("--timing-every", type=int, default=8, metavar="N",
                     help="record timing events on every N-th timed substep (bounds their host "
                          "cost; averages come from the sampled substeps)")
@@ -519,7 +532,8 @@ def main():
 
     sim, bed, head = plain_leg(args.workload, args.n, args.seed, args.column, dt,
                                args.warmup * SUBSTEPS_PER_STEP, args.steps * SUBSTEPS_PER_STEP,
-                               args.timing_every, args.timing_level, tuning)
+                               args.timing_every, args.timing_level, tuning,
+                               autotune=not args.no_autotune and not tuning)
     sim.close()
     del sim
     if args.copy_probe:
@@ -550,7 +564,8 @@ def main():
     batched = None
     if not args.no_batched:
         batched = run_batched(args.batch_systems, 4, args.batch_steps,
-                              max(2, args.batch_steps // 6), args.batch_workers)
+                              max(2, args.batch_steps // 6), args.batch_workers,
+                              not args.no_autotune)
         print(f"# batched: {batched['value']:.3e} particle-substeps/s "
               f"({batched['speedup_vs_sequential']:.2f}x sequential)", file=sys.stderr, flush=True)
 
@@ -590,6 +605,7 @@ def main():
         "config": {
             "workload": head["workload"],
             "form": "plain",
+            "tuning": head.get("tuning"),
             "substeps_per_step": SUBSTEPS_PER_STEP,
             "substeps_timed": head["substeps_timed"],
         },

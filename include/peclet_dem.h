@@ -122,9 +122,9 @@ int pd_step(pd_sim* s, double dt);        /* Simulation::step -> demStep, src/si
 /* Extension: n substeps of size dt without returning to the caller in between. */
 int pd_step_n(pd_sim* s, double dt, int n);
 int pd_compute_overlaps(pd_sim* s, double* out); /* computeOverlapsKokkos, src/sim.hpp:494 */
-int pd_num_contacts(pd_sim* s);                  /* numContacts, src/sim.hpp:1548 */
-int pd_num_manifolds(pd_sim* s);                 /* numManifolds, src/sim.hpp:1549 */
-double pd_max_overlap(pd_sim* s);                /* maxOverlap, src/sim.hpp:1550 */
+int pd_num_contacts(pd_sim* s);                  /* numContacts, src/sim.hpp:1550 */
+int pd_num_manifolds(pd_sim* s);                 /* numManifolds, src/sim.hpp:1551 */
+double pd_max_overlap(pd_sim* s);                /* maxOverlap, src/sim.hpp:1552-1556 */
 int pd_get_stats(pd_sim* s, pd_stats* out);
 
 /* --- diagnostics of the last substep (count returned; NULL output = count only) --------- */
@@ -186,6 +186,8 @@ void pd_batch_destroy(pd_batch* b);
 int pd_batch_size(pd_batch* b);
 pd_sim* pd_batch_system(pd_batch* b, int k); /* owned by the batch */
 int pd_batch_set_workers(pd_batch* b, int workers);
+/* pd_autotune for every system, largest first, sharing the choice per size class */
+int pd_batch_autotune(pd_batch* b, double dt, int substeps);
 int pd_batch_step(pd_batch* b, double dt, int nsteps); /* nsteps substeps of every system */
 
 /* --- differentiation of the substep (extension: the reference has no adjoint) -------------
@@ -240,9 +242,20 @@ int pd_set_timing(pd_sim* s, int level);
  * (default) lets a one-workgroup velocity loop of at most 8192 rows keep the velocities of up
  * to "velocity_loop_lds_bodies" of its bodies in LDS (default and maximum 2400; the others stay
  * in global memory) when its rows touch at most "velocity_loop_lds_keys" distinct bodies
- * (default and maximum 12288), 0 keeps them all in global memory.  Results do not depend on any
- * of them. */
+ * (default and maximum 12288), 0 keeps them all in global memory; "vsweep_waves" = 3 (default)
+ * or 4 selects the per-colour PGS sweep compiled for that many waves per SIMD (4: at most 128
+ * VGPRs, with some register spilling).  Results do not depend on any of them. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
+/* Autotuner (no reference counterpart): chooses candidate_skin_permille, block_solver_rows,
+ * velocity_loop_lds and vsweep_waves for this simulation by timing its own substeps (coordinate
+ * descent over {0, 50, 100, 200, 400}, {0, 2048, 16384, 65536}, {1, 0} and {3, 4}; each trial is
+ * one untimed and `substeps` timed substeps, so the call advances the simulation by up to
+ * 11 x (1 + substeps) substeps; results do not depend on the knobs).  The choice is cached per
+ * size class (shape kind, log2 particle count, log2 contact density); with use_cache != 0 a
+ * cached class is applied without stepping.  chosen[4] (may be NULL) receives (skin, rows, lds,
+ * waves); best_ms (may be NULL) the best trial's ms per substep (-1 when taken from the cache). */
+int pd_autotune(pd_sim* s, double dt, int substeps, int use_cache, long* chosen,
+                double* best_ms);
 int pd_get_stage_times(pd_sim* s, double* out_ms, int cap);
 /* host wall time between the same stage marks (enqueueing plus waiting), summed like the stage
  * times; a stage whose host time exceeds its device time is bound by launch overhead */

@@ -24,11 +24,11 @@
  * with max, so every result is bit-identical to the one-thread run.
  *
  * Pinning (tests/test_oracle.py, tests/test_oracle_hertz.py): reference test inputs, brute force,
- * closed forms and independent algorithms.  Parity unpinned: the Jacobi leftover solves
- * (solveVelocityJacobi / positionSolveJacobi with onlyUncoloured) beyond momentum conservation,
- * the escalate and ordered stabilization loops beyond the PGS row they share, and the Hertz
- * point-spring shape paths (hertzShapePairForces, the shape branch of hertzWallForces) beyond
- * determinism.
+ * closed forms and independent algorithms; the Jacobi solves by binary restitution, symmetric
+ * supports and the geometric convergence of the position average; the escalate and ordered
+ * loops, warm start, persistence, grounded levels and the one-sided side flags by a
+ * one-dimensional column model over two substeps; the Hertz shape paths by the SDF-sphere
+ * restitution and cube stick/slide closed forms.  No built function is left unpinned.
  */
 #include "pdref.h"
 

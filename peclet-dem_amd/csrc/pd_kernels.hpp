@@ -234,6 +234,10 @@ __global__ void k_fill_u64(unsigned long long* a, unsigned long long v, int n) {
   const int i = gtid();
   if (i < n) a[i] = v;
 }
+__global__ void k_fill_u8(unsigned char* a, unsigned char v, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
 
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = TPB * SCAN_ITEMS;
@@ -2402,6 +2406,20 @@ __global__ __launch_bounds__(TPB) void k_vsweep(const Dev* __restrict__ Dp, Para
   }
   if (MODE != 0) blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
 }
+// the PGS sweep compiled for at least 4 waves per SIMD (at most 128 VGPRs instead of ~153):
+// more rows in flight per CU to hide the body gathers' latency, against register pressure; the
+// autotuner picks between the two
+__global__ __launch_bounds__(TPB, 4) void k_vsweep_o4(const Dev* __restrict__ Dp, Params P,
+                                                      int rowBegin, int rowEnd, double thr,
+                                                      const int* stop) {
+  const Dev& D = *Dp;
+  if (*stop) return;
+  __shared__ unsigned long long sh[WAVES];
+  const int k = rowBegin + gtid();
+  double a = 0.0;
+  if (k < rowEnd) a = pgsRow(D, P, k, thr);
+  blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
+}
 // colour 62 may hold manifolds sharing a body (mask saturation): one thread, in row order (the
 // class's rows are placed in manifold order)
 __global__ __launch_bounds__(64) void k_vsweep_serial(const Dev* __restrict__ Dp, Params P, int rowBegin,
@@ -3049,95 +3067,10 @@ __global__ void k_commit_pos_impulse(const Dev* __restrict__ Dp, int numManifold
 }
 
 // ============================================================================================
-// colour-saturation fallbacks (colour == -1 after the round cap), src/sim.hpp:245-256,454-462
-// Accumulation is serial in manifold / contact order (the canonical order of the oracle); the
-// averaged apply is parallel (one body per thread).
+// colour-saturation fallbacks (colour == -1 after the round cap), src/sim.hpp:245-256,454-462,
+// and the Jacobi paths: contributions per element in parallel, summed per body in element order
+// (the canonical order of the oracle); the averaged apply is one body per thread.
 // ============================================================================================
-// solveVelocityKokkos restricted to uncoloured manifolds, src/solver_velocity.hpp:38-224
-__global__ void k_vel_jacobi_accum(const Dev* __restrict__ Dp, Params P, int numManifolds, double thr,
-                                   int onlyUncoloured) {
-  const Dev& D = *Dp;
-  if (gtid() != 0) return;
-  for (int m = 0; m < numManifolds; ++m) {
-    if ((onlyUncoloured && D.mColor[m] != -1) || D.mNp[m] <= 0) continue;
-    const int2 r = D.mReal[m];
-    const int realA = r.x, realB = r.y;
-    if (realB >= 0 && realA > realB) continue;
-    const BodyC ca = D.bc[realA];
-    const double invMassA = ca.iim.w;
-    const d3 invIA = xyz(ca.iim);
-    const q4 qA = toq(ca.q);
-    double invMassB = 0.0;
-    d3 invIB{0, 0, 0};
-    q4 qB{0, 0, 0, 1};
-    const Body64 ba = D.vw[realA];
-    const d3 vA = xyz(ba.v), wA = xyz(ba.w);
-    d3 vB{0, 0, 0}, wB{0, 0, 0};
-    if (realB >= 0) {
-      const BodyC cb = D.bc[realB];
-      invMassB = cb.iim.w;
-      invIB = xyz(cb.iim);
-      qB = toq(cb.q);
-      const Body64 bb = D.vw[realB];
-      vB = xyz(bb.v);
-      wB = xyz(bb.w);
-    }
-    const double4 ns4 = D.mNs[m];
-    const d3 Nsum = xyz(ns4), TauA = xyz(D.mTA[m]), TauB = xyz(D.mTB[m]);
-    const double invN = 1.0 / (double)D.mNp[m];
-    double restitution = P.restN;
-    if (realB < 0) vB = scale3(xyz(D.mWV[m]), invN);
-    {
-      const double ra = ns4.w * invN;
-      if (ra >= 0.0) restitution = ra;
-    }
-    const d3 rAavg = scale3(xyz(D.mRA[m]), invN);
-    const d3 rBavg = scale3(xyz(D.mRB[m]), invN);
-    const double lenN = sqrt(dot3(Nsum, Nsum));
-    if (lenN < 1e-9) continue;
-    const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
-    const d3 vGrowth = scale3(diffCenters, P.growthRate);
-    double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
-    vn += dot3(vGrowth, Nsum);
-    const double alignment = dot3(Nsum, diffCenters);
-    if (alignment > 0.0) {
-      if (vn < 0.0) continue;
-    } else {
-      if (vn > 0.0) continue;
-    }
-    const double Nsq = dot3(Nsum, Nsum);
-    const double wTotal = (Nsq * invMassA + genInvMass(TauA, invIA, qA)) +
-                          (Nsq * invMassB + genInvMass(TauB, invIB, qB));
-    if (wTotal <= 0.0) continue;
-    if (fabs(vn) < thr * lenN) restitution = 0.0;
-    const double lambda = (-restitution * vn - vn) / wTotal;
-    const d3 Jlin = scale3(Nsum, lambda);
-    double4& dva = D.dVel[realA];
-    dva.x += Jlin.x * invMassA;
-    dva.y += Jlin.y * invMassA;
-    dva.z += Jlin.z * invMassA;
-    {
-      const d3 dw = angDelta(qA, scale3(TauA, lambda), invIA);
-      double4& dwa = D.dAng[realA];
-      dwa.x += dw.x;
-      dwa.y += dw.y;
-      dwa.z += dw.z;
-    }
-    if (realB >= 0) {
-      double4& dvb = D.dVel[realB];
-      dvb.x += -Jlin.x * invMassB;
-      dvb.y += -Jlin.y * invMassB;
-      dvb.z += -Jlin.z * invMassB;
-      const d3 dw = angDelta(qB, scale3(TauB, lambda), invIB);
-      double4& dwb = D.dAng[realB];
-      dwb.x += dw.x;
-      dwb.y += dw.y;
-      dwb.z += dw.z;
-      D.dCnt[realB] += 1;
-    }
-    D.dCnt[realA] += 1;
-  }
-}
 // applyVelocityDeltasAveragedKokkos, src/solver_velocity.hpp:233-255
 __global__ void k_apply_vel_avg(const Dev* __restrict__ Dp, int n) {
   const Dev& D = *Dp;
@@ -3159,66 +3092,306 @@ __global__ void k_apply_vel_avg(const Dev* __restrict__ Dp, int n) {
   D.dAng[i] = make_double4(0, 0, 0, 0);
   D.dCnt[i] = 0;
 }
-// solvePositionKokkos restricted to uncoloured contacts, src/solver_position.hpp:34-132
-__global__ void k_pos_jacobi_accum(const Dev* __restrict__ Dp, int numContacts, int onlyUncoloured) {
+// ---- the same three accumulations in parallel, in the same order --------------------------
+// The serial loops above add each element's contribution (manifold or contact, ascending) to
+// its two bodies' accumulators, and no contribution reads an accumulator.  Here every element's
+// contribution is computed by its own thread (the same statements), and each body then sums the
+// contributions of its incident elements in ascending element order (A before B for one
+// element), which is the serial loop's order of additions into that body: the accumulators end
+// bit-identical.  Incidences per body: element * 2 + side (0 = body A, 1 = body B), sorted.
+struct JContrib {
+  double4 a0, a1, b0, b1;  // A's velocity (or position) and angular parts, then B's
+};
+// MODE 0: velocity Jacobi over manifolds (bodies real indices); 1: position Jacobi over contacts
+// (bodies slots); 2: count-averaged friction over contacts (bodies real indices)
+template <int MODE>
+__device__ __forceinline__ bool jElementBodies(const Dev& D, int e, int onlyUncoloured, int& a,
+                                               int& b) {
+  if (MODE == 0) {
+    if ((onlyUncoloured && D.mColor[e] != -1) || D.mNp[e] <= 0) return false;
+    const int2 r = D.mReal[e];
+    if (r.y >= 0 && r.x > r.y) return false;
+    a = r.x;
+    b = r.y;
+    return true;
+  }
+  if (MODE == 1) {
+    if (onlyUncoloured && D.cColor[e] != -1) return false;
+    const int2 ab = D.cAB[e];
+    a = ab.x;
+    b = ab.y;
+    return true;
+  }
+  if (!(D.cLamN[e] > 0.0)) return false;
+  const int2 ab = D.cAB[e];
+  a = D.realIdx[ab.x];
+  b = ab.y >= 0 ? D.realIdx[ab.y] : -1;
+  return true;
+}
+template <int MODE>
+__global__ void k_jinc_count(const Dev* __restrict__ Dp, int numElements, int onlyUncoloured,
+                             int* cnt) {
   const Dev& D = *Dp;
-  if (gtid() != 0) return;
-  for (int c = 0; c < numContacts; ++c) {
-    if (onlyUncoloured && D.cColor[c] != -1) continue;
-    const int2 ab = D.cAB[c];
-    const int idA = ab.x, idB = ab.y;
-    const BodyC ca = D.bc[idA];
-    const double invMassA = ca.iim.w;
-    const d3 pA = xyz(D.posPred[idA]);
-    const q4 qA = toq(D.quatPred[idA]);
-    d3 pB{0, 0, 0};
-    q4 qB{0, 0, 0, 1};
-    BodyC cb;
-    double invMassB = 0.0;
-    if (idB >= 0) {
-      cb = D.bc[idB];
-      invMassB = cb.iim.w;
-      pB = xyz(D.posPred[idB]);
-      qB = toq(D.quatPred[idB]);
+  const int e = gtid();
+  if (e >= numElements) return;
+  int a, b;
+  if (!jElementBodies<MODE>(D, e, onlyUncoloured, a, b)) return;
+  atomicAdd(&cnt[a], 1);
+  if (b >= 0) atomicAdd(&cnt[b], 1);
+}
+template <int MODE>
+__global__ void k_jinc_fill(const Dev* __restrict__ Dp, int numElements, int onlyUncoloured,
+                            const int* start, int* fill, int* list) {
+  const Dev& D = *Dp;
+  const int e = gtid();
+  if (e >= numElements) return;
+  int a, b;
+  if (!jElementBodies<MODE>(D, e, onlyUncoloured, a, b)) return;
+  list[start[a] + atomicAdd(&fill[a], 1)] = 2 * e;
+  if (b >= 0) list[start[b] + atomicAdd(&fill[b], 1)] = 2 * e + 1;
+}
+// each body's incidences in ascending order (insertion sort of the short segment)
+__global__ void k_jinc_sort(int numBodies, const int* start, int* list) {
+  const int i = gtid();
+  if (i >= numBodies) return;
+  const int b = start[i], e = start[i + 1];
+  for (int k = b + 1; k < e; ++k) {
+    const int v = list[k];
+    int j = k - 1;
+    while (j >= b && list[j] > v) {
+      list[j + 1] = list[j];
+      --j;
     }
-    const double4 n4 = D.cN[c], ra4 = D.cRA[c], rb4 = D.cRB[c];
-    const q4 qAdelta = quatMult(qA, quatInverse(toq(ca.q)));
-    const d3 rA = rotate(qAdelta, xyz(ra4));
-    d3 rB = xyz(rb4);
-    d3 n = xyz(n4);
-    if (idB >= 0) {
-      const q4 qBdelta = quatMult(qB, quatInverse(toq(cb.q)));
-      rB = rotate(qBdelta, rB);
-      n = rotate(qBdelta, n);
-    }
-    double C;
-    if (idB < 0) {
-      n = xyz(n4);
-      C = dot3(sub3(add3(pA, rA), xyz(rb4)), n);
-    } else {
-      C = dot3(sub3(add3(pA, rA), add3(pB, rB)), n);
-    }
-    if (C >= 0.0) continue;
-    const d3 invIA = xyz(ca.iim);
-    const d3 invIB = (idB >= 0) ? xyz(cb.iim) : d3{0, 0, 0};
-    const double wTotal = computeW(rA, n, invMassA, invIA) + computeW(rB, n, invMassB, invIB);
-    if (wTotal < 1e-6) continue;
-    const double dLambda = -C / wTotal;
-    double4& da = D.dPos[idA];
-    da.x += n.x * dLambda * invMassA;
-    da.y += n.y * dLambda * invMassA;
-    da.z += n.z * dLambda * invMassA;
-    if (idB >= 0) {
-      double4& db = D.dPos[idB];
-      db.x += -n.x * dLambda * invMassB;
-      db.y += -n.y * dLambda * invMassB;
-      db.z += -n.z * dLambda * invMassB;
-      D.dCnt[idB] += 1;
-    }
-    D.dCnt[idA] += 1;
-    atomicMaxD(D.maxOverlapBits, -C);
+    list[j + 1] = v;
   }
 }
+__device__ __forceinline__ double4 d4(double x, double y, double z) {
+  return make_double4(x, y, z, 0.0);
+}
+// solveVelocityKokkos, src/solver_velocity.hpp:38-224: one manifold's contribution
+__global__ void k_jvel_contrib(const Dev* __restrict__ Dp, Params P, int numManifolds, double thr,
+                               int onlyUncoloured, JContrib* jc, int* jf) {
+  const Dev& D = *Dp;
+  const int m = gtid();
+  if (m >= numManifolds) return;
+  jf[m] = 0;
+  int realA, realB;
+  if (!jElementBodies<0>(D, m, onlyUncoloured, realA, realB)) return;
+  const BodyC ca = D.bc[realA];
+  const double invMassA = ca.iim.w;
+  const d3 invIA = xyz(ca.iim);
+  const q4 qA = toq(ca.q);
+  double invMassB = 0.0;
+  d3 invIB{0, 0, 0};
+  q4 qB{0, 0, 0, 1};
+  const Body64 ba = D.vw[realA];
+  const d3 vA = xyz(ba.v), wA = xyz(ba.w);
+  d3 vB{0, 0, 0}, wB{0, 0, 0};
+  if (realB >= 0) {
+    const BodyC cb = D.bc[realB];
+    invMassB = cb.iim.w;
+    invIB = xyz(cb.iim);
+    qB = toq(cb.q);
+    const Body64 bb = D.vw[realB];
+    vB = xyz(bb.v);
+    wB = xyz(bb.w);
+  }
+  const double4 ns4 = D.mNs[m];
+  const d3 Nsum = xyz(ns4), TauA = xyz(D.mTA[m]), TauB = xyz(D.mTB[m]);
+  const double invN = 1.0 / (double)D.mNp[m];
+  double restitution = P.restN;
+  if (realB < 0) vB = scale3(xyz(D.mWV[m]), invN);
+  {
+    const double ra = ns4.w * invN;
+    if (ra >= 0.0) restitution = ra;
+  }
+  const d3 rAavg = scale3(xyz(D.mRA[m]), invN);
+  const d3 rBavg = scale3(xyz(D.mRB[m]), invN);
+  const double lenN = sqrt(dot3(Nsum, Nsum));
+  if (lenN < 1e-9) return;
+  const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
+  const d3 vGrowth = scale3(diffCenters, P.growthRate);
+  double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
+  vn += dot3(vGrowth, Nsum);
+  const double alignment = dot3(Nsum, diffCenters);
+  if (alignment > 0.0) {
+    if (vn < 0.0) return;
+  } else {
+    if (vn > 0.0) return;
+  }
+  const double Nsq = dot3(Nsum, Nsum);
+  const double wTotal = (Nsq * invMassA + genInvMass(TauA, invIA, qA)) +
+                        (Nsq * invMassB + genInvMass(TauB, invIB, qB));
+  if (wTotal <= 0.0) return;
+  if (fabs(vn) < thr * lenN) restitution = 0.0;
+  const double lambda = (-restitution * vn - vn) / wTotal;
+  const d3 Jlin = scale3(Nsum, lambda);
+  JContrib c;
+  c.a0 = d4(Jlin.x * invMassA, Jlin.y * invMassA, Jlin.z * invMassA);
+  const d3 dwA = angDelta(qA, scale3(TauA, lambda), invIA);
+  c.a1 = d4(dwA.x, dwA.y, dwA.z);
+  int f = 1;
+  if (realB >= 0) {
+    c.b0 = d4(-Jlin.x * invMassB, -Jlin.y * invMassB, -Jlin.z * invMassB);
+    const d3 dwB = angDelta(qB, scale3(TauB, lambda), invIB);
+    c.b1 = d4(dwB.x, dwB.y, dwB.z);
+    f |= 2;
+  }
+  jc[m] = c;
+  jf[m] = f;
+}
+// solvePositionKokkos, src/solver_position.hpp:34-132: one contact's contribution
+__global__ void k_jpos_contrib(const Dev* __restrict__ Dp, int numContacts, int onlyUncoloured,
+                               JContrib* jc, int* jf) {
+  const Dev& D = *Dp;
+  const int c = gtid();
+  if (c >= numContacts) return;
+  jf[c] = 0;
+  int idA, idB;
+  if (!jElementBodies<1>(D, c, onlyUncoloured, idA, idB)) return;
+  const BodyC ca = D.bc[idA];
+  const double invMassA = ca.iim.w;
+  const d3 pA = xyz(D.posPred[idA]);
+  const q4 qA = toq(D.quatPred[idA]);
+  d3 pB{0, 0, 0};
+  q4 qB{0, 0, 0, 1};
+  BodyC cb;
+  double invMassB = 0.0;
+  if (idB >= 0) {
+    cb = D.bc[idB];
+    invMassB = cb.iim.w;
+    pB = xyz(D.posPred[idB]);
+    qB = toq(D.quatPred[idB]);
+  }
+  const double4 n4 = D.cN[c], ra4 = D.cRA[c], rb4 = D.cRB[c];
+  const q4 qAdelta = quatMult(qA, quatInverse(toq(ca.q)));
+  const d3 rA = rotate(qAdelta, xyz(ra4));
+  d3 rB = xyz(rb4);
+  d3 n = xyz(n4);
+  if (idB >= 0) {
+    const q4 qBdelta = quatMult(qB, quatInverse(toq(cb.q)));
+    rB = rotate(qBdelta, rB);
+    n = rotate(qBdelta, n);
+  }
+  double C;
+  if (idB < 0) {
+    n = xyz(n4);
+    C = dot3(sub3(add3(pA, rA), xyz(rb4)), n);
+  } else {
+    C = dot3(sub3(add3(pA, rA), add3(pB, rB)), n);
+  }
+  if (C >= 0.0) return;
+  const d3 invIA = xyz(ca.iim);
+  const d3 invIB = (idB >= 0) ? xyz(cb.iim) : d3{0, 0, 0};
+  const double wTotal = computeW(rA, n, invMassA, invIA) + computeW(rB, n, invMassB, invIB);
+  if (wTotal < 1e-6) return;
+  const double dLambda = -C / wTotal;
+  JContrib q;
+  q.a0 = d4(n.x * dLambda * invMassA, n.y * dLambda * invMassA, n.z * dLambda * invMassA);
+  int f = 1;
+  if (idB >= 0) {
+    q.b0 = d4(-n.x * dLambda * invMassB, -n.y * dLambda * invMassB, -n.z * dLambda * invMassB);
+    f |= 2;
+  }
+  jc[c] = q;
+  jf[c] = f;
+  atomicMaxD(D.maxOverlapBits, -C);
+}
+// solveContactFrictionKokkos, src/solver_friction.hpp:142-220: one contact's contribution
+__global__ void k_jfr_contrib(const Dev* __restrict__ Dp, int nc, double frictionDynamic,
+                              JContrib* jc, int* jf) {
+  const Dev& D = *Dp;
+  const int c = gtid();
+  if (c >= nc) return;
+  jf[c] = 0;
+  int realA, realB;
+  if (!jElementBodies<2>(D, c, 0, realA, realB)) return;
+  const double lambda_n = D.cLamN[c];
+  const int idB = D.cAB[c].y;
+  const BodyC ca = D.bc[realA];
+  const double invMA = ca.iim.w;
+  const d3 invIA = xyz(ca.iim);
+  double invMB = 0.0;
+  d3 invIB{0.0, 0.0, 0.0};
+  const d3 rA = xyz(D.cRA[c]), n = xyz(D.cN[c]);
+  const double4 rb4 = D.cRB[c];
+  const d3 rB = xyz(rb4);
+  const Body64 ba = D.vw[realA];
+  const d3 vAc = add3(xyz(ba.v), cross3(xyz(ba.w), rA));
+  d3 vBc = xyz(D.cBV[c]);
+  if (idB >= 0) {
+    const BodyC cb = D.bc[realB];
+    invMB = cb.iim.w;
+    invIB = xyz(cb.iim);
+    const Body64 bb = D.vw[realB];
+    vBc = add3(xyz(bb.v), cross3(xyz(bb.w), rB));
+  }
+  const d3 vrel = sub3(vAc, vBc);
+  const double vn = dot3(vrel, n);
+  const d3 vt = sub3(vrel, scale3(n, vn));
+  const double vt_len = sqrt(dot3(vt, vt));
+  if (vt_len < 1e-7) return;
+  const d3 t = scale3(vt, 1.0 / vt_len);
+  const d3 rnA = cross3(rA, t), rnB = cross3(rB, t);
+  const double w_t = invMA + invMB + rnA.x * rnA.x * invIA.x + rnA.y * rnA.y * invIA.y +
+                     rnA.z * rnA.z * invIA.z + rnB.x * rnB.x * invIB.x +
+                     rnB.y * rnB.y * invIB.y + rnB.z * rnB.z * invIB.z;
+  if (w_t < 1e-6) return;
+  const double nA = (double)D.frCnt[realA];
+  const double nB = (idB >= 0) ? (double)D.frCnt[realB] : 0.0;
+  const double inv_n = 1.0 / fmax(fmax(nA, nB), 1.0);
+  double lt = -vt_len / w_t;
+  const double mu = (rb4.w >= 0.0) ? rb4.w : frictionDynamic;
+  const double maxf = mu * lambda_n;
+  if (lt < -maxf) lt = -maxf;
+  lt *= inv_n;
+  JContrib q;
+  q.a0 = d4(t.x * lt * invMA, t.y * lt * invMA, t.z * lt * invMA);
+  q.a1 = d4(rnA.x * invIA.x * lt, rnA.y * invIA.y * lt, rnA.z * invIA.z * lt);
+  int f = 1;
+  if (idB >= 0) {
+    q.b0 = d4(-t.x * lt * invMB, -t.y * lt * invMB, -t.z * lt * invMB);
+    q.b1 = d4(-rnB.x * invIB.x * lt, -rnB.y * invIB.y * lt, -rnB.z * invIB.z * lt);
+    f |= 2;
+  }
+  jc[c] = q;
+  jf[c] = f;
+}
+// each body sums its incident contributions in ascending element order; MODE 0 / 2 add to the
+// velocity accumulators (0 also counts), MODE 1 to the position accumulator (and counts)
+template <int MODE>
+__global__ void k_jgather(const Dev* __restrict__ Dp, int numBodies, const int* start,
+                          const int* list, const JContrib* jc, const int* jf) {
+  const Dev& D = *Dp;
+  const int i = gtid();
+  if (i >= numBodies) return;
+  const int b = start[i], e = start[i + 1];
+  if (b == e) return;
+  double4 s0 = MODE == 1 ? D.dPos[i] : D.dVel[i];
+  double4 s1 = D.dAng[i];
+  int cnt = D.dCnt[i];
+  for (int k = b; k < e; ++k) {
+    const int code = list[k];
+    const int el = code >> 1, side = code & 1;
+    if (!(jf[el] & (1 << side))) continue;
+    const double4 c0 = side ? jc[el].b0 : jc[el].a0;
+    s0.x += c0.x;
+    s0.y += c0.y;
+    s0.z += c0.z;
+    if (MODE != 1) {
+      const double4 c1 = side ? jc[el].b1 : jc[el].a1;
+      s1.x += c1.x;
+      s1.y += c1.y;
+      s1.z += c1.z;
+    }
+    cnt += 1;
+  }
+  if (MODE == 1) D.dPos[i] = s0;
+  else D.dVel[i] = s0;
+  if (MODE != 1) D.dAng[i] = s1;
+  if (MODE != 2) D.dCnt[i] = cnt;
+}
+
 // applyUpdatesKokkos, src/integration.hpp:168-184
 __global__ void k_apply_pos_avg(const Dev* __restrict__ Dp, int n) {
   const Dev& D = *Dp;
@@ -3310,75 +3483,8 @@ __global__ void k_fr_count(const Dev* __restrict__ Dp, int nc) {
   atomicAdd(&D.frCnt[D.realIdx[ab.x]], 1);
   if (ab.y >= 0) atomicAdd(&D.frCnt[D.realIdx[ab.y]], 1);
 }
-// src/solver_friction.hpp:142-220: impulses accumulated in contact order (one lane), then
-// applied unaveraged by k_fr_apply (src/integration.hpp:97-109)
-__global__ void k_fr_solve_serial(const Dev* __restrict__ Dp, int nc, double frictionDynamic) {
-  const Dev& D = *Dp;
-  if (gtid() != 0) return;
-  for (int c = 0; c < nc; ++c) {
-    const double lambda_n = D.cLamN[c];
-    if (lambda_n <= 0.0) continue;
-    const int2 ab = D.cAB[c];
-    const int idA = ab.x, idB = ab.y;
-    const int realA = D.realIdx[idA];
-    const int realB = (idB >= 0) ? D.realIdx[idB] : -1;
-    const BodyC ca = D.bc[realA];
-    const double invMA = ca.iim.w;
-    const d3 invIA = xyz(ca.iim);
-    double invMB = 0.0;
-    d3 invIB{0.0, 0.0, 0.0};
-    const d3 rA = xyz(D.cRA[c]), n = xyz(D.cN[c]);
-    const double4 rb4 = D.cRB[c];
-    const d3 rB = xyz(rb4);
-    const Body64 ba = D.vw[realA];
-    const d3 vAc = add3(xyz(ba.v), cross3(xyz(ba.w), rA));
-    d3 vBc = xyz(D.cBV[c]);
-    if (idB >= 0) {
-      const BodyC cb = D.bc[realB];
-      invMB = cb.iim.w;
-      invIB = xyz(cb.iim);
-      const Body64 bb = D.vw[realB];
-      vBc = add3(xyz(bb.v), cross3(xyz(bb.w), rB));
-    }
-    const d3 vrel = sub3(vAc, vBc);
-    const double vn = dot3(vrel, n);
-    const d3 vt = sub3(vrel, scale3(n, vn));
-    const double vt_len = sqrt(dot3(vt, vt));
-    if (vt_len < 1e-7) continue;
-    const d3 t = scale3(vt, 1.0 / vt_len);
-    const d3 rnA = cross3(rA, t), rnB = cross3(rB, t);
-    const double w_t = invMA + invMB + rnA.x * rnA.x * invIA.x + rnA.y * rnA.y * invIA.y +
-                       rnA.z * rnA.z * invIA.z + rnB.x * rnB.x * invIB.x +
-                       rnB.y * rnB.y * invIB.y + rnB.z * rnB.z * invIB.z;
-    if (w_t < 1e-6) continue;
-    const double nA = (double)D.frCnt[realA];
-    const double nB = (idB >= 0) ? (double)D.frCnt[realB] : 0.0;
-    const double inv_n = 1.0 / fmax(fmax(nA, nB), 1.0);
-    double lt = -vt_len / w_t;
-    const double mu = (rb4.w >= 0.0) ? rb4.w : frictionDynamic;
-    const double maxf = mu * lambda_n;
-    if (lt < -maxf) lt = -maxf;
-    lt *= inv_n;
-    double4& dva = D.dVel[realA];
-    dva.x += t.x * lt * invMA;
-    dva.y += t.y * lt * invMA;
-    dva.z += t.z * lt * invMA;
-    double4& dwa = D.dAng[realA];
-    dwa.x += rnA.x * invIA.x * lt;
-    dwa.y += rnA.y * invIA.y * lt;
-    dwa.z += rnA.z * invIA.z * lt;
-    if (idB >= 0) {
-      double4& dvb = D.dVel[realB];
-      dvb.x += -t.x * lt * invMB;
-      dvb.y += -t.y * lt * invMB;
-      dvb.z += -t.z * lt * invMB;
-      double4& dwb = D.dAng[realB];
-      dwb.x += -rnB.x * invIB.x * lt;
-      dwb.y += -rnB.y * invIB.y * lt;
-      dwb.z += -rnB.z * invIB.z * lt;
-    }
-  }
-}
+// src/solver_friction.hpp:142-220 runs as k_jfr_contrib + k_jgather<2> (contact order per
+// body), then the impulses are applied unaveraged (src/integration.hpp:97-109)
 __global__ void k_fr_apply(const Dev* __restrict__ Dp, int n) {
   const Dev& D = *Dp;
   const int i = gtid();

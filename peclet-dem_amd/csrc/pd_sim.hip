@@ -15,6 +15,7 @@
 #include <fstream>
 #include <iomanip>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -138,6 +139,7 @@ void adjFree(AdjTape* t);
 struct pd_sim {
   hipStream_t st = nullptr;
   int capacity = 0;
+  int device = 0;  // the device current when the handle was created (its streams' device)
   int numReal = 0, numParticles = 0;
   // shape 0 (src/sim.hpp:949-1021)
   int shapeType = SHAPE_SPHERE;
@@ -183,6 +185,7 @@ struct pd_sim {
   // sweep loops over at most this many rows run in one workgroup (k_vsolve_block/k_psolve_block)
   int blockRows = BLOCK_SOLVER_ROWS;
   bool ldsLoop = true;  // one-workgroup velocity loops stage their bodies in LDS when they fit
+  bool vsweepWaves4 = false;  // per-colour PGS sweeps compiled for 4 waves per SIMD
   int ldsBodies = VL_BODIES;  // ... up to this many bodies (the rest stay in global memory)
   int ldsKeys = VL_KEYS;      // more distinct bodies than this: the global-memory loop
   // extra slots reserved by setPositions for periodic ghosts (more are added when needed)
@@ -305,6 +308,10 @@ struct pd_sim {
   DBuf<unsigned long long> mKey, lKey, uKey, red, bits, redPart;
   DBuf<int> supList, supCnt;  // support manifolds of the grounded-level sweeps
   DBuf<int> ctl;  // device loop control (CTL_* in pd_kernels.hpp)
+  // order-exact parallel Jacobi / friction accumulation: per-element contributions and flags,
+  // per-body incidence lists (count, start, list)
+  DBuf<JContrib> jC;
+  DBuf<int> jF, jIncCnt, jIncStart, jIncList;
   DBuf<int> lIdx, bucketCnt, bucketStart, counters, scanTmp, ghostCnt,
       ghostOff;
   // pinned host scratch
@@ -334,6 +341,7 @@ struct pd_sim {
   hipEvent_t evContacts = nullptr;  // contact list built -> side stream
 
   pd_sim(int cap) : capacity(cap) {
+    HIPCHECK(hipGetDevice(&device));
     HIPCHECK(hipStreamCreate(&st));
     HIPCHECK(hipStreamCreate(&sideStream));
     // the two streams exchange device memory only: no system-scope fence (cache writeback and
@@ -425,6 +433,7 @@ struct pd_sim {
     scanTmp.release(); ghostCnt.release(); ghostOff.release();
     dVel.release(); dAng.release(); dPos.release(); dCnt.release(); cLamN.release();
     frCnt.release();
+    jC.release(); jF.release(); jIncCnt.release(); jIncStart.release(); jIncList.release();
     wallGridD.release(); sdfGridD.release(); hPairs.release(); hIStart.release(); hJStart.release();
     hJList.release(); hJCnt.release(); hWStart.release(); hWCand.release(); hWCnt.release();
     hXi.release(); hF.release(); hTA.release(); hTB.release(); hPrevXi.release();
@@ -664,6 +673,51 @@ struct pd_sim {
     zero(dAng.p, numParticles);
     zero(dPos.p, numParticles);
     zero(dCnt.p, numParticles);
+  }
+
+  // incidence lists of the elements (manifolds for MODE 0, contacts otherwise) that may add to
+  // a body's accumulators: ascending element order per body (slots 0..numParticles)
+  template <int MODE>
+  void buildIncidence(int numElements, int onlyUncoloured) {
+    const int nb = numParticles;
+    jIncCnt.ensure(nb + 1, st);
+    jIncStart.ensure(nb + 1, st);
+    jF.ensure(numElements + 1, st);
+    jC.ensure(numElements + 1, st);
+    refreshDev();
+    zero(jIncCnt.p, nb + 1);
+    if (numElements > 0)
+      k_jinc_count<MODE><<<nblk(numElements), TPB, 0, st>>>(dD, numElements, onlyUncoloured,
+                                                            jIncCnt.p);
+    const int total = scan(jIncCnt.p, jIncStart.p, nb);
+    jIncList.ensure(total + 1, st);
+    zero(jIncCnt.p, nb + 1);
+    if (numElements > 0)
+      k_jinc_fill<MODE><<<nblk(numElements), TPB, 0, st>>>(dD, numElements, onlyUncoloured,
+                                                           jIncStart.p, jIncCnt.p, jIncList.p);
+    k_jinc_sort<<<nblk(nb), TPB, 0, st>>>(nb, jIncStart.p, jIncList.p);
+    HIPCHECK(hipGetLastError());
+  }
+  // one Jacobi velocity pass (src/solver_velocity.hpp:38-255) over the incidence lists built
+  // for it: contributions, ordered per-body sums, averaged apply
+  void jacobiVelocityPass(const Params& PF, double thr, int onlyUncoloured) {
+    prepareFallback();
+    if (nm > 0)
+      k_jvel_contrib<<<nblk(nm), TPB, 0, st>>>(dD, PF, nm, thr, onlyUncoloured, jC.p, jF.p);
+    k_jgather<0><<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles, jIncStart.p, jIncList.p,
+                                                     jC.p, jF.p);
+    k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
+    HIPCHECK(hipGetLastError());
+  }
+  // one Jacobi position pass (src/solver_position.hpp:34-134, src/integration.hpp:168-184)
+  void jacobiPositionPass(int onlyUncoloured) {
+    prepareFallback();
+    if (nc > 0)
+      k_jpos_contrib<<<nblk(nc), TPB, 0, st>>>(dD, nc, onlyUncoloured, jC.p, jF.p);
+    k_jgather<1><<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles, jIncStart.p, jIncList.p,
+                                                     jC.p, jF.p);
+    k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
+    HIPCHECK(hipGetLastError());
   }
 
   Params params(double margin, double band) const {
@@ -1380,7 +1434,10 @@ struct pd_sim {
           k_vsweep<0><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
         } else if (mode == 1) {
           kernBegin();
-          k_vsweep<1><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
+          if (vsweepWaves4)
+            k_vsweep_o4<<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
+          else
+            k_vsweep<1><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
           tagSweepRec(kernEnd(K_VSWEEP, e - b));
         } else {
           k_vsweep<2><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
@@ -1751,18 +1808,16 @@ struct pd_sim {
           k_fr_accum_normal<<<nblk(nc), TPB, 0, st>>>(dD, nc, growthRate);
         if (!useGS) {
           // count-averaged Jacobi over every manifold (src/solver_velocity.hpp:38-255)
-          prepareFallback();
-          k_vel_jacobi_accum<<<1, 1, 0, st>>>(dD, PF, nm, vRest, 0);
-          k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
+          if (it == 0) buildIncidence<0>(nm, 0);
+          jacobiVelocityPass(PF, vRest, 0);
           stats.sweeps_vel++;
           continue;
         }
         zero(bits.p + 1, 1);
         velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest, ctl.p + CTL_NONE);
         if (velLeftover > 0) {
-          prepareFallback();
-          k_vel_jacobi_accum<<<1, 1, 0, st>>>(dD, PF, nm, vRest, 1);
-          k_apply_vel_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
+          if (it == 0) buildIncidence<0>(nm, 1);
+          jacobiVelocityPass(PF, vRest, 1);
         }
         stats.sweeps_vel++;
         maxApproach = readMaxApproach();
@@ -1842,8 +1897,11 @@ struct pd_sim {
       // src/solver_friction.hpp:123-220 then applyVelocityDeltas (src/integration.hpp:97-109)
       zero(frCnt.p, numParticles);
       k_fr_count<<<nblk(nc), TPB, 0, st>>>(dD, nc);
+      buildIncidence<2>(nc, 0);
       prepareFallback();
-      k_fr_solve_serial<<<1, 1, 0, st>>>(dD, nc, frictionDynamic);
+      k_jfr_contrib<<<nblk(nc), TPB, 0, st>>>(dD, nc, frictionDynamic, jC.p, jF.p);
+      k_jgather<2><<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles, jIncStart.p,
+                                                       jIncList.p, jC.p, jF.p);
       k_fr_apply<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
       HIPCHECK(hipGetLastError());
     }
@@ -1886,10 +1944,9 @@ struct pd_sim {
     const unsigned long long* finalBits = nullptr;
     if (!useGS) {
       // Jacobi XPBD over every contact, no adaptive stop (src/solver_position.hpp:34-134)
+      if (posIters > 0) buildIncidence<1>(nc, 0);
       for (int it = 0; it < posIters; ++it) {
-        prepareFallback();
-        k_pos_jacobi_accum<<<1, 1, 0, st>>>(dD, nc, 0);
-        k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
+        jacobiPositionPass(0);
         stats.sweeps_pos++;
       }
       HIPCHECK(hipGetLastError());
@@ -1948,6 +2005,8 @@ struct pd_sim {
       }
       posSweepsPrev = finalCtl[CTL_SWEEPS_P];
     } else {
+      // colouring leftovers: the coloured sweep, then Jacobi over the uncoloured contacts
+      if (posIters > 0) buildIncidence<1>(nc, 1);
       for (int it = 0; it < posIters; ++it) {
         zero(bits.p, 1);
         for (int c = 0; c < numPosColors; ++c) {
@@ -1958,10 +2017,7 @@ struct pd_sim {
           else
             k_psweep<<<nblk(e - b), TPB, 0, st>>>(dD, b, e, ctl.p + CTL_NONE);
         }
-        prepareFallback();
-        k_pos_jacobi_accum<<<1, 1, 0, st>>>(dD, nc, 1);
-        k_apply_pos_avg<<<nblk(numParticles), TPB, 0, st>>>(dD, numParticles);
-        HIPCHECK(hipGetLastError());
+        jacobiPositionPass(1);
         stats.sweeps_pos++;
         if (bitsd(read1(bits.p)) < posTol) break;
       }
@@ -2319,6 +2375,121 @@ void writeLammps(const std::string& filename, int step, const double* pos, const
 }
 }  // namespace
 
+// ---- autotuner (no reference counterpart) -------------------------------------------------
+// Chooses the performance knobs of a simulation by timing its own substeps: the candidate-list
+// skin (candidate_skin_permille), the largest sweep loop run inside one workgroup
+// (block_solver_rows), LDS staging of the one-workgroup velocity loop (velocity_loop_lds) and the
+// register budget of the per-colour PGS sweep (vsweep_waves: 3 or 4 waves per SIMD).
+// None of them changes a result, so the trials step the simulation itself.  Coordinate descent
+// from the current setting: each knob's values are tried in turn (one untimed substep after a
+// change, which may rebuild lists, then `substeps` timed ones) and a value is kept when it is
+// at least 2 % faster.  The choice is cached per size class (shape kind, log2 of the particle
+// count, log2 of 1 + 4 x contacts per particle) for later calls on systems of the same class.
+struct TuneCfg {
+  long skin, rows, lds, waves;
+};
+std::mutex g_tuneMutex;
+std::map<std::tuple<int, int, int>, TuneCfg> g_tuneCache;
+static std::tuple<int, int, int> tuneClass(const pd_sim* s) {
+  const int n = std::max(1, s->numReal);
+  int ln = 0, lc = 0;
+  while ((2L << ln) <= n) ++ln;
+  const long dens = 1 + (4L * s->nc) / n;
+  while ((2L << lc) <= dens) ++lc;
+  return {s->shapeType + (s->shellHost.empty() ? 0 : 16), ln, lc};
+}
+static TuneCfg tuneCurrent(const pd_sim* s) {
+  return {(long)std::lround(s->candSkinFrac * 1000.0), (long)s->blockRows, s->ldsLoop ? 1L : 0L,
+          s->vsweepWaves4 ? 4L : 3L};
+}
+static void tuneApply(pd_sim* s, const TuneCfg& c) {
+  if ((long)std::lround(s->candSkinFrac * 1000.0) != c.skin) {
+    s->candSkinFrac = c.skin / 1000.0;
+    s->candValid = false;
+  }
+  s->blockRows = (int)c.rows;
+  s->ldsLoop = c.lds != 0;
+  s->vsweepWaves4 = c.waves == 4;
+}
+static double tuneTrial(pd_sim* s, double dt, int substeps) {
+  s->dt = dt;
+  s->demStep();
+  s->sync();
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int k = 0; k < substeps; ++k) {
+    s->dt = dt;
+    s->demStep();
+  }
+  s->sync();
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+         std::max(1, substeps);
+}
+static TuneCfg tuneRun(pd_sim* s, double dt, int substeps, double* bestMs) {
+  TuneCfg cur = tuneCurrent(s);
+  double best = tuneTrial(s, dt, substeps);
+  const long skins[] = {0, 50, 100, 200, 400};
+  const long rows[] = {0, 2048, 16384, 65536};
+  const long lds[] = {1, 0};
+  const long waves[] = {3, 4};
+  auto sweep = [&](const long* vals, int nv, long TuneCfg::*field) {
+    const long start = cur.*field;
+    for (int q = 0; q < nv; ++q) {
+      if (vals[q] == start) continue;
+      TuneCfg c = cur;
+      c.*field = vals[q];
+      tuneApply(s, c);
+      const double t = tuneTrial(s, dt, substeps);
+      if (t < 0.98 * best) {
+        best = t;
+        cur = c;
+      }
+    }
+    tuneApply(s, cur);
+  };
+  sweep(skins, 5, &TuneCfg::skin);
+  sweep(rows, 4, &TuneCfg::rows);
+  sweep(lds, 2, &TuneCfg::lds);
+  sweep(waves, 2, &TuneCfg::waves);
+  if (bestMs) *bestMs = best;
+  return cur;
+}
+
+extern "C" {
+int pd_autotune(pd_sim* s, double dt, int substeps, int use_cache, long* chosen,
+                double* best_ms) {
+  return guard([&] {
+    if (substeps < 1) throw ArgError("autotune: substeps must be >= 1");
+    const auto key = tuneClass(s);
+    TuneCfg c{};
+    bool hit = false;
+    if (use_cache) {
+      std::lock_guard<std::mutex> lk(g_tuneMutex);
+      auto it = g_tuneCache.find(key);
+      if (it != g_tuneCache.end()) {
+        c = it->second;
+        hit = true;
+      }
+    }
+    double ms = -1.0;
+    if (hit) {
+      tuneApply(s, c);
+    } else {
+      c = tuneRun(s, dt, substeps, &ms);
+      std::lock_guard<std::mutex> lk(g_tuneMutex);
+      g_tuneCache[key] = c;
+    }
+    if (chosen) {
+      chosen[0] = c.skin;
+      chosen[1] = c.rows;
+      chosen[2] = c.lds;
+      chosen[3] = c.waves;
+    }
+    if (best_ms) *best_ms = ms;
+    drainRetired();
+  });
+}
+}  // extern "C"
+
 extern "C" {
 
 const char* pd_last_error(void) { return g_error.c_str(); }
@@ -2619,15 +2790,43 @@ int pd_set_sdf_shape(pd_sim* s, const double* grid, int nx, int ny, int nz, doub
   });
 }
 // getSdfGrid / exportSdf, src/sim.hpp:1478-1483,1538-1542 (x-fastest, over the domain box)
-static void sdfGrid(pd_sim* s, int rx, int ry, int rz, std::vector<double>& out) {
+static SdfGridP sdfGridParams(const pd_sim* s, int rx, int ry, int rz) {
   if (rx < 1 || ry < 1 || rz < 1) throw ArgError("resolution must be >= 1 on each axis");
-  const long total = (long)rx * ry * rz;
   SdfGridP G{};
   G.rx = rx; G.ry = ry; G.rz = rz;
   G.px = s->per[0]; G.py = s->per[1]; G.pz = s->per[2];
   G.origin = d3{s->dmin[0], s->dmin[1], s->dmin[2]};
   G.vox = d3{(s->dmax[0] - s->dmin[0]) / rx, (s->dmax[1] - s->dmin[1]) / ry,
              (s->dmax[2] - s->dmin[2]) / rz};
+  return G;
+}
+// The SDF grid on the simulation's stream, in caller-given working arrays of total voxels each:
+// key (u64), g1, g2 (double), fixedCell (u8).  key may share g2's memory: the keys are dead
+// once g1 holds the splatted distances.  Returns the array holding the result (g1 or g2).
+static double* sdfGridCompute(pd_sim* s, const SdfGridP& G, unsigned long long* key, double* g1,
+                              double* g2, unsigned char* fixedCell) {
+  const long total = (long)G.rx * G.ry * G.rz;
+  const hipStream_t st = s->st;
+  const int nb = (int)((total + TPB - 1) / TPB);
+  k_fill_u64<<<nb, TPB, 0, st>>>(key, ordKey(100.0), (int)total);
+  k_fill_u8<<<nb, TPB, 0, st>>>(fixedCell, 0, total);
+  s->refreshDev();
+  Params P = s->params(0.0, 0.0);
+  if (s->numReal > 0) k_sdf_splat<<<nblk(s->numReal), TPB, 0, st>>>(s->dD, P, G, key, fixedCell);
+  k_sdf_from_keys<<<nb, TPB, 0, st>>>(key, g1, total);
+  const int iters = std::max(std::max(G.rx, G.ry), G.rz) * 4;
+  double* in = g1;
+  double* o2 = g2;
+  for (int it = 0; it < iters; ++it) {
+    k_sdf_eikonal<<<nb, TPB, 0, st>>>(G, in, o2, fixedCell, G.vox.x);
+    std::swap(in, o2);
+  }
+  HIPCHECK(hipGetLastError());
+  return in;
+}
+static void sdfGrid(pd_sim* s, int rx, int ry, int rz, std::vector<double>& out) {
+  const SdfGridP G = sdfGridParams(s, rx, ry, rz);
+  const long total = (long)rx * ry * rz;
   unsigned long long* key = nullptr;
   double *g1 = nullptr, *g2 = nullptr;
   unsigned char* fixedCell = nullptr;
@@ -2636,24 +2835,9 @@ static void sdfGrid(pd_sim* s, int rx, int ry, int rz, std::vector<double>& out)
   HIPCHECK(hipMalloc(&g2, total * sizeof(double)));
   HIPCHECK(hipMalloc(&fixedCell, total));
   try {
-    const hipStream_t st = s->st;
-    const int nb = (int)((total + TPB - 1) / TPB);
-    k_fill_u64<<<nb, TPB, 0, st>>>(key, ordKey(100.0), total);
-    HIPCHECK(hipMemsetAsync(fixedCell, 0, total, st));
-    s->refreshDev();
-    Params P = s->params(0.0, 0.0);
-    if (s->numReal > 0) k_sdf_splat<<<nblk(s->numReal), TPB, 0, st>>>(s->dD, P, G, key, fixedCell);
-    k_sdf_from_keys<<<nb, TPB, 0, st>>>(key, g1, total);
-    const int iters = std::max(std::max(rx, ry), rz) * 4;
-    double* in = g1;
-    double* o2 = g2;
-    for (int it = 0; it < iters; ++it) {
-      k_sdf_eikonal<<<nb, TPB, 0, st>>>(G, in, o2, fixedCell, G.vox.x);
-      std::swap(in, o2);
-    }
-    HIPCHECK(hipGetLastError());
+    const double* res = sdfGridCompute(s, G, key, g1, g2, fixedCell);
     out.resize(total);
-    s->readN(out.data(), in, total);
+    s->readN(out.data(), res, total);
   } catch (...) {
     (void)hipFree(key); (void)hipFree(g1); (void)hipFree(g2); (void)hipFree(fixedCell);
     throw;
@@ -2663,6 +2847,7 @@ static void sdfGrid(pd_sim* s, int rx, int ry, int rz, std::vector<double>& out)
   HIPCHECK(hipFree(g2));
   HIPCHECK(hipFree(fixedCell));
 }
+
 int pd_get_sdf_grid(pd_sim* s, int rx, int ry, int rz, double* out) {
   return guard([&] {
     std::vector<double> g;
@@ -2670,6 +2855,7 @@ int pd_get_sdf_grid(pd_sim* s, int rx, int ry, int rz, double* out) {
     if (out) std::memcpy(out, g.data(), g.size() * sizeof(double));
   });
 }
+
 // writeSdfVti, src/io.hpp:86-110 (ASCII ImageData, Float32 samples)
 int pd_export_sdf(pd_sim* s, const char* filename, int rx, int ry, int rz) {
   return guard([&] {
@@ -3135,6 +3321,9 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     } else if (k == "slot_headroom") {
       if (value < 0) throw ArgError("slot_headroom must be >= 0");
       s->slotHeadroom = std::min<long>(value, 1L << 30);
+    } else if (k == "vsweep_waves") {
+      if (value != 3 && value != 4) throw ArgError("vsweep_waves must be 3 or 4");
+      s->vsweepWaves4 = value == 4;
     } else if (k == "timing_every") {
       if (value < 1) throw ArgError("timing_every must be >= 1");
       s->timingEvery = (int)std::min<long>(value, 1L << 30);
@@ -3223,6 +3412,22 @@ pd_sim* pd_batch_system(pd_batch* b, int k) {
     return nullptr;
   }
   return b->sims[k];
+}
+// every system takes the autotuned setting of its size class; the largest system of a class
+// is tuned (stepping it), the others reuse the cached choice
+int pd_batch_autotune(pd_batch* b, double dt, int substeps) {
+  std::vector<int> order(b->sims.size());
+  for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return b->sims[x]->numReal > b->sims[y]->numReal; });
+  for (int k : order) {
+    const int rc = pd_autotune(b->sims[k], dt, substeps, 1, nullptr, nullptr);
+    if (rc != 0) {
+      g_error = "system " + std::to_string(k) + ": " + g_error;
+      return rc;
+    }
+  }
+  return 0;
 }
 int pd_batch_set_workers(pd_batch* b, int workers) {
   if (workers < 1) {

@@ -139,6 +139,13 @@ class Batch:
         if self._b.batch_step(self._h, float(dt), int(substeps)) != 0:
             raise RuntimeError(self._b.error())
 
+    def autotune(self, dt, substeps: int = 4):
+        """Autotune every system (see :meth:`Simulation.autotune`): the largest system of each
+        size class is tuned by stepping it, the others take the cached choice."""
+        rc = self._b.batch_autotune(self._h, float(dt), int(substeps))
+        if rc != 0:
+            raise (ValueError if rc == -2 else RuntimeError)(self._b.error())
+
     def close(self):
         if getattr(self, "_h", None):
             for s in self.systems:
@@ -577,6 +584,19 @@ class Simulation:
         self._b.get_ledger(self._h, keys.ctypes.data_as(C.POINTER(C.c_uint64)), _ptr(lam),
                            _ptr(lamT), _ptr(pimp), n)
         return keys, lam, lamT, pimp
+
+    def autotune(self, dt, substeps=4, use_cache=True):
+        """Choose the performance knobs (candidate_skin_permille, block_solver_rows,
+        velocity_loop_lds, vsweep_waves) by timing this simulation's own substeps; the simulation advances by
+        the trial substeps (results do not depend on the knobs).  Returns the chosen setting and
+        the best trial's ms per substep (None when taken from the size-class cache)."""
+        chosen = (C.c_long * 4)()
+        ms = C.c_double(0.0)
+        self._chk(self._b.autotune(self._h, float(dt), int(substeps), 1 if use_cache else 0,
+                                   chosen, C.byref(ms)))
+        return ({"candidate_skin_permille": int(chosen[0]), "block_solver_rows": int(chosen[1]),
+                 "velocity_loop_lds": int(chosen[2]), "vsweep_waves": int(chosen[3])},
+                None if ms.value < 0 else ms.value)
 
     def set_tuning(self, name, value):
         """Launch-shape knob of the HIP library (extension); results do not depend on it."""

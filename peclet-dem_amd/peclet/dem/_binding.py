@@ -115,6 +115,8 @@ SIGNATURES = {
 
 # Product-only entry points (no counterpart in a restatement library).
 PRODUCT_SIGNATURES = {
+    "autotune": (_i, [_p, _d, _i, _i, C.POINTER(C.c_long), _dp]),
+    "batch_autotune": (_i, [_p, _d, _i]),
     "step_n": (_i, [_p, _d, _i]),
     "synchronize": (_i, [_p]),
     "execution_space": (C.c_char_p, []),

@@ -198,9 +198,15 @@ def test_hollow_cylinders_200k_invariants(hip_lib):
     n, dt, mu = 200000, 5e-6, 0.4
     sc = cylinder_scene(n, 3, 4.1e-3, 0.5, 49)  # 49 x 4.1 mm: the 200 mm footprint
     h = apply_cylinders(hip_sim(n), sc, dt)
+    import time
+
     for k in range(16):
+        t0 = time.perf_counter()
         h.step(dt)
-        print(f"cylinders 200k substep {k}: {h.get_stats()['num_contacts']} contacts", flush=True)
+        st = h.get_stats()
+        print(f"cylinders 200k substep {k}: {st['num_contacts']} contacts, "
+              f"{st['num_colors_pos']} contact colours, {st['rounds_pos']} rounds, "
+              f"{time.perf_counter() - t0:.2f} s", flush=True)
     st = h.get_stats()
     assert st["num_contacts"] > 1000 and st["num_particles"] > n
     for g in ("get_positions", "get_velocities", "get_angular_velocities", "get_quaternions"):

@@ -806,7 +806,7 @@ def _column(n, mode, dt=1e-3, g=10.0, iters=8):
     s.set_gravity(0, 0, -g)
     s.set_material_params(0.0, 0.0, 0.0)
     s.set_solver_iterations(4, iters)
-    s.set_stabilization_mode({0: "off", 3: "escalate", 4: "ordered"}[mode])
+    s.set_stabilization_mode({0: "off", 1: "onesided", 3: "escalate", 4: "ordered"}[mode])
     return s
 
 
@@ -884,3 +884,124 @@ def test_column_stabilization_matches_the_1d_model(mode, n):
         assert stab > 0  # the column did not converge in the main loop
     if mode == 3:
         assert np.abs(vz).max() <= n * 2.0 * dt * g  # drained to the rest threshold per contact
+
+
+def _column_substep(v, p_prev, rows, colours, persistent, grounded_lower, dt, g, iters, mode):
+    """One substep of the compressed column under the PGS path in one dimension: predicted
+    velocities, vn0, warm start from the previous impulses, the main loop, then the
+    stabilization (mode 1: side flags hold the lower body of a persistent, slowly approaching
+    row whose lower body is grounded and not rising; mode 3: plain sweeps to the threshold).
+    Returns (velocities, impulses, main sweeps, stabilization sweeps)."""
+    v = v - g * dt
+    n = len(rows)
+    vrest = 2.0 * dt * g
+
+    def approach(k):
+        lo, up = rows[k]
+        return (v[lo] if lo >= 0 else 0.0) - v[up]
+
+    vn0 = [approach(k) for k in range(n)]
+    by_colour = [k for c in sorted(set(colours)) for k in range(n) if colours[k] == c]
+    p = np.array(p_prev, float)
+    for k in by_colour:  # warm start: the previous impulse pushes the pair apart
+        lo, up = rows[k]
+        v[up] += p[k]
+        if lo >= 0:
+            v[lo] -= p[k]
+    held = [False] * n
+
+    def row(k):
+        lo, up = rows[k]
+        a = approach(k)
+        lower_moves = lo >= 0 and not held[k]
+        w = 1.0 + (1.0 if lower_moves else 0.0)
+        new = max(0.0, p[k] + a / w)
+        d = new - p[k]
+        p[k] = new
+        v[up] += d
+        if lower_moves:
+            v[lo] -= d
+        return abs(d) * w
+
+    sweeps = stab = 0
+    m = 0.0
+    for _ in range(iters):
+        m = max(row(k) for k in by_colour)
+        sweeps += 1
+        if m <= 0.02 * vrest:
+            break
+    if mode != 0 and m > vrest:
+        if mode == 1:
+            for k in range(n):
+                lo, up = rows[k]
+                # the lower body rises no faster than 8 dt g (its velocity along -g)
+                held[k] = (lo >= 0 and persistent[k] and abs(vn0[k]) <= 8.0 * dt * g
+                           and v[lo] <= 8.0 * dt * g and grounded_lower[k])
+        for _ in range(2 * iters if mode == 1 else 256):
+            m = max(row(k) for k in by_colour)
+            stab += 1
+            if m <= vrest:
+                break
+    return v, p, sweeps, stab
+
+
+def _column_levels(n, rows, level):
+    """Grounded levels after one substep's update: decay by 8, then 8 double-buffered sweeps
+    in which the floor sets its sphere to 255 and a sphere resting on a grounded one takes the
+    lower one's level minus 1."""
+    g = np.maximum(level - 8, 0)
+    for _ in range(8):
+        gn = g.copy()
+        for lo, up in rows:
+            if lo < 0:
+                gn[up] = 255
+            elif g[lo] - 1 > 0 and g[lo] - 1 > gn[up]:
+                gn[up] = g[lo] - 1
+        g = gn
+    return g
+
+
+def _column_rows(s):
+    ids, _ = s.debug_manifolds()
+    colours = s.debug_manifold_colors()
+    z = s.get_positions()[:, 2]
+    rows = []
+    for a, b in ids[:, :2]:
+        if b < 0:
+            rows.append((-1, int(a)))
+        else:
+            rows.append((int(a), int(b)) if z[a] < z[b] else (int(b), int(a)))
+    return rows, [int(c) for c in colours]
+
+
+@pytest.mark.parametrize("mode", [1, 3])
+def test_column_two_substeps_warm_start_and_side_flags(mode):
+    """Two substeps of the compressed column: the second is warm-started from the first's
+    ledger (every row persistent), and under the one-sided mode its stabilization holds the
+    lower body of each slowly approaching, grounded row (the grounded levels reach the 8 lowest
+    spheres in the first substep's 8 sweeps); the one-dimensional model with the same rules
+    gives the oracle's sweep counts and velocities."""
+    n, dt, g, iters = 12, 1e-3, 10.0, 4
+    s = _column(n, mode, dt, g, iters)
+    s.step(dt)
+    rows1, cols1 = _column_rows(s)
+    v1, p1, sw1, st1 = _column_substep(-0.05 * (np.arange(n) + 1.0), np.zeros(len(rows1)),
+                                       rows1, cols1, [False] * len(rows1), [False] * len(rows1),
+                                       dt, g, iters, mode)
+    st = s.get_stats()
+    assert (st["sweeps_vel"], st["sweeps_stab"]) == (sw1, st1)
+    np.testing.assert_allclose(s.get_velocities()[:, 2], v1, rtol=0, atol=1e-12 * g * dt)
+    key1 = {r: p1[k] for k, r in enumerate(rows1)}
+    s.step(dt)
+    rows2, cols2 = _column_rows(s)
+    prev = [key1.get(r, 0.0) for r in rows2]
+    persistent = [r in key1 for r in rows2]
+    level = _column_levels(n, rows2, _column_levels(n, rows1, np.zeros(n, int)))
+    grounded = [lo >= 0 and level[lo] > 0 for lo, up in rows2]
+    v2, p2, sw2, st2 = _column_substep(v1, prev, rows2, cols2, persistent, grounded, dt, g,
+                                       iters, mode)
+    st = s.get_stats()
+    assert (st["sweeps_vel"], st["sweeps_stab"]) == (sw2, st2), (st, sw2, st2)
+    np.testing.assert_allclose(s.get_velocities()[:, 2], v2, rtol=0, atol=1e-12 * g * dt)
+    assert st2 > 0 and all(persistent) and any(prev)
+    assert np.array_equal(s.debug_grounded(), level)
