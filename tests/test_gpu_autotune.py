@@ -20,7 +20,6 @@ def test_autotuned_run_equals_untuned(hip_lib, kind):
     scene = make_scene(kind, n, 7)
     tuned = apply_scene(Simulation(n), scene)
     plain = apply_scene(Simulation(n), scene)
-    before = tuned.get_stats()["num_real"]
     cfg, ms = tuned.autotune(scene["dt"], substeps=2, use_cache=False)
     print(kind, cfg, ms, flush=True)
     assert cfg["candidate_skin_permille"] in (0, 50, 100, 200, 400)
@@ -31,7 +30,7 @@ def test_autotuned_run_equals_untuned(hip_lib, kind):
     while steps < 64 and not np.array_equal(state(plain), state(tuned)):
         plain.step(scene["dt"])
         steps += 1
-    assert 3 <= steps <= 33 and before == n
+    assert 3 <= steps <= 33
     for _ in range(10):
         tuned.step(scene["dt"])
         plain.step(scene["dt"])

@@ -192,15 +192,47 @@ def test_hollow_cylinders_256_point_shell(hip_lib, oracle_threads):
     assert contacts > 200
 
 
+def ledger_cone_violations(sim, prev, mu):
+    """Rows of the ledger whose friction impulse leaves the Coulomb cone of
+    src/solver_velocity.hpp:760-855: |lambda_T| <= mu (max(p, 0) |N| + max(q, 0)), with |N| the
+    manifold's normal-sum length and q the position impulse the row was warm-started with (the
+    first entry of its key in the previous substep's ledger `prev`; it is added only for resting
+    contacts, so including it always gives an upper bound).  The ledger holds the manifolds
+    sorted by (key, manifold index), keys (min << 32) | max of the real indices (a wall's other
+    index 0xFFFFFFFF), ~0 for a manifold without points.  Returns (rows checked, violations)."""
+    keys, lam, lamT, _ = sim.debug_ledger()
+    ids, data = sim.debug_manifolds()
+    a = ids[:, 3].astype(np.uint64)
+    rb = ids[:, 4].astype(np.int64)
+    b = np.where(rb >= 0, rb, np.int64(0xFFFFFFFF)).astype(np.uint64)
+    mkey = (np.minimum(a, b) << np.uint64(32)) | np.maximum(a, b)
+    mkey[ids[:, 2] <= 0] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    order = np.lexsort((np.arange(len(mkey)), mkey))
+    assert np.array_equal(keys, mkey[order])  # the ledger is the manifolds in key order
+    lenN = np.linalg.norm(data[order, 0:3], axis=1)
+    q0 = {}
+    for k, q in zip(prev[0].tolist(), prev[3].tolist()):
+        q0.setdefault(k, q)
+    live = keys != np.uint64(0xFFFFFFFFFFFFFFFF)
+    bad = 0
+    for k, p, t, ln in zip(keys[live].tolist(), lam[live], lamT[live], lenN[live]):
+        bound = mu * (max(p, 0.0) * ln + max(q0.get(k, 0.0), 0.0))
+        bad += np.linalg.norm(t) > bound * (1.0 + 1e-9) + 1e-300
+    return int(live.sum()), bad
+
+
 @pytest.mark.gpu
-@pytest.mark.timeout(600)
+@pytest.mark.timeout(800)
 def test_hollow_cylinders_200k_invariants(hip_lib):
-    n, dt, mu = 200000, 5e-6, 0.4
-    sc = cylinder_scene(n, 3, 4.1e-3, 0.5, 49)  # 49 x 4.1 mm: the 200 mm footprint
-    h = apply_cylinders(hip_sim(n), sc, dt)
     import time
 
+    n, dt, mu = 200000, 5e-6, 0.4
+    sc = cylinder_scene(n, 3, 4.6e-3, 0.5, 43)  # 43 x 4.6 mm: the 200 mm footprint
+    h = apply_cylinders(hip_sim(n), sc, dt)
+    prev = None
     for k in range(16):
+        if k == 15:
+            prev = h.debug_ledger()
         t0 = time.perf_counter()
         h.step(dt)
         st = h.get_stats()
@@ -211,12 +243,12 @@ def test_hollow_cylinders_200k_invariants(hip_lib):
     assert st["num_contacts"] > 1000 and st["num_particles"] > n
     for g in ("get_positions", "get_velocities", "get_angular_velocities", "get_quaternions"):
         assert np.all(np.isfinite(getattr(h, g)())), g
-    keys, lam, lamT, _ = h.debug_ledger()
+    keys, lam, _, _ = h.debug_ledger()
     live = keys != np.uint64(0xFFFFFFFFFFFFFFFF)
-    assert live.sum() > 0
     assert np.all(lam[live] >= 0.0)
-    tn = np.linalg.norm(lamT[live], axis=1)
-    assert np.all(tn <= mu * lam[live] * (1.0 + 1e-9) + 1e-300)
+    rows, bad = ledger_cone_violations(h, prev, mu)
+    print(f"cone checked on {rows} ledger rows: {bad} outside", flush=True)
+    assert rows > 1000 and bad == 0
 
 
 @pytest.mark.gpu
