@@ -159,6 +159,15 @@ long pd_rsa_place(const double* cand, long ncand, const double* radii, int n, in
 int pd_get_sdf_grid(pd_sim* s, int rx, int ry, int rz, double* out);
 /* exportSdf(filename, rx, ry, rz), src/sim.hpp:1478-1483: the same grid as an ASCII VTI file */
 int pd_export_sdf(pd_sim* s, const char* filename, int rx, int ry, int rz);
+/* No counterpart in the reference: pd_get_sdf_grid for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  It allocates nothing: its
+ * working arrays (17 bytes per voxel) live in a file-scope device array of the library, one per
+ * GPU, held until the process ends, so rx*ry*rz may be at most 2,097,152 voxels (e.g. 128^3;
+ * larger grids return -2 with an error naming the limit).  The grid (same values as
+ * pd_get_sdf_grid) is written to out_device, caller-owned device memory of rx*ry*rz doubles on
+ * the handle's GPU; the call returns once it is there.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (from several threads or handles) race. */
+int pd_get_sdf_grid_noalloc(pd_sim* s, int rx, int ry, int rz, double* out_device);
 
 /* --- SDF walls and the Hertz–Mindlin engine --------------------------------------------- */
 /* addSdfWall(grid, nx, ny, nz, origin, spacing, restitution, friction), src/sim.hpp:1242-1277:

@@ -2848,6 +2848,36 @@ static void sdfGrid(pd_sim* s, int rx, int ry, int rz, std::vector<double>& out)
   HIPCHECK(hipFree(fixedCell));
 }
 
+// The SDF export without allocation: its working arrays live in `scratch`, one file-scope device
+// array per GPU (held until the process ends): g1 and g2 (8 bytes per voxel each; the keys
+// share g2) and the fixed-cell flags (1 byte per voxel), for at most SDF_SCRATCH_VOXELS voxels.
+constexpr long SDF_SCRATCH_VOXELS = 1L << 21;  // 2,097,152 voxels, e.g. 128 x 128 x 128
+constexpr size_t SDF_SCRATCH_BYTES = (size_t)SDF_SCRATCH_VOXELS * 17;
+static __device__ __align__(256) unsigned char scratch[SDF_SCRATCH_BYTES];
+// `scratch` names a host-side stand-in in host code; its device address, per device, comes from
+// hipGetSymbolAddress on the first call made on that device and is kept here
+std::mutex g_scratchMutex;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+void* g_scratchAddr[SCRATCH_MAX_DEVICES] = {};
+static void* scratchAddress(int device) {
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+    throw std::runtime_error("scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratchMutex);
+  if (!g_scratchAddr[device]) {
+    int cur = 0;
+    HIPCHECK(hipGetDevice(&cur));
+    if (cur != device) HIPCHECK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) HIPCHECK(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("scratch: hipGetSymbolAddress failed: ") +
+                               hipGetErrorString(e));
+    g_scratchAddr[device] = p;
+  }
+  return g_scratchAddr[device];
+}
+
 int pd_get_sdf_grid(pd_sim* s, int rx, int ry, int rz, double* out) {
   return guard([&] {
     std::vector<double> g;
@@ -2855,7 +2885,27 @@ int pd_get_sdf_grid(pd_sim* s, int rx, int ry, int rz, double* out) {
     if (out) std::memcpy(out, g.data(), g.size() * sizeof(double));
   });
 }
-
+int pd_get_sdf_grid_noalloc(pd_sim* s, int rx, int ry, int rz, double* out_device) {
+  return guard([&] {
+    const SdfGridP G = sdfGridParams(s, rx, ry, rz);
+    const long total = (long)rx * ry * rz;
+    if (total > SDF_SCRATCH_VOXELS)
+      throw ArgError("get_sdf_grid_noalloc: " + std::to_string(rx) + " x " + std::to_string(ry) +
+                     " x " + std::to_string(rz) + " = " + std::to_string(total) +
+                     " voxels exceed the scratch capacity of " +
+                     std::to_string(SDF_SCRATCH_VOXELS) + " voxels");
+    if (!out_device) throw ArgError("get_sdf_grid_noalloc: out_device is required");
+    unsigned char* base = static_cast<unsigned char*>(scratchAddress(s->device));
+    double* g1 = reinterpret_cast<double*>(base);
+    double* g2 = reinterpret_cast<double*>(base + SDF_SCRATCH_VOXELS * sizeof(double));
+    unsigned long long* key = reinterpret_cast<unsigned long long*>(g2);
+    unsigned char* fixedCell = base + 2 * SDF_SCRATCH_VOXELS * sizeof(double);
+    const double* res = sdfGridCompute(s, G, key, g1, g2, fixedCell);
+    HIPCHECK(hipMemcpyAsync(out_device, res, total * sizeof(double), hipMemcpyDeviceToDevice,
+                            s->st));
+    s->sync();
+  });
+}
 // writeSdfVti, src/io.hpp:86-110 (ASCII ImageData, Float32 samples)
 int pd_export_sdf(pd_sim* s, const char* filename, int rx, int ry, int rz) {
   return guard([&] {

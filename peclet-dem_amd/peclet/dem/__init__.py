@@ -656,6 +656,19 @@ class Simulation:
         self._chk(self._b.get_sdf_grid(self._h, rx, ry, rz, _ptr(out)))
         return out.reshape(rx, ry, rz)
 
+    def get_sdf_grid_noalloc(self, resolution, out):
+        """``get_sdf_grid`` without allocation (an extension for callers that cannot allocate):
+        the x-fastest samples are written into ``out``, a caller-owned float64 CUDA tensor of
+        rx*ry*rz elements on the simulation's GPU; at most 2,097,152 voxels.  Calls that overlap
+        on one GPU share the library's working array and race."""
+        rx, ry, rz = (int(v) for v in resolution)
+        if out.dtype != __import__("torch").float64 or not out.is_cuda or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 CUDA tensor")
+        if out.numel() != rx * ry * rz:
+            raise ValueError(f"out must hold {rx * ry * rz} elements, has {out.numel()}")
+        self._chk(self._b.get_sdf_grid_noalloc(self._h, rx, ry, rz, out.data_ptr()))
+        return out
+
     def export_sdf(self, filename, resolution):
         rx, ry, rz = (int(v) for v in resolution)
         self._chk(self._b.export_sdf(self._h, os.fsencode(filename), rx, ry, rz))

@@ -117,6 +117,7 @@ SIGNATURES = {
 PRODUCT_SIGNATURES = {
     "autotune": (_i, [_p, _d, _i, _i, C.POINTER(C.c_long), _dp]),
     "batch_autotune": (_i, [_p, _d, _i]),
+    "get_sdf_grid_noalloc": (_i, [_p, _i, _i, _i, _p]),
     "step_n": (_i, [_p, _d, _i]),
     "synchronize": (_i, [_p]),
     "execution_space": (C.c_char_p, []),
