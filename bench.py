@@ -512,9 +512,12 @@ def main():
     dt = 5e-6
     tuning = [(kv.split("=", 1)[0], int(kv.split("=", 1)[1])) for kv in args.tuning]
     if args.only == "pour":
-        _, _, r = plain_leg("pour", 1048576, args.seed, args.column, dt, 8,
-                            args.pour_steps * SUBSTEPS_PER_STEP, args.timing_every,
-                            args.timing_level)
+        sim, _, r = plain_leg("pour", 1048576, args.seed, args.column, dt, 8,
+                              args.pour_steps * SUBSTEPS_PER_STEP, args.timing_every,
+                              args.timing_level)
+        sim.close()
+        if args.copy_probe:
+            copy_probe(args.copy_probe)
         print(json.dumps(r))
         return
     if args.only == "cylinders":

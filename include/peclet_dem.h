@@ -43,6 +43,7 @@ typedef struct pd_stats {
   long long uncolored_pos; /* sum over colouring rounds of uncoloured contacts */
   double max_overlap;
   double max_approach;
+  int ml_levels;           /* multilevel stabilization: coarse levels built (0 otherwise) */
 } pd_stats;
 
 /* --- lifetime --------------------------------------------------------------------------- */

@@ -218,6 +218,8 @@ typedef struct Sim {
   V3* prevLambdaT;
   int prevCount, prevCap;
   real maxOverlap, maxApproach;
+  real maxApproachQS; /* the multilevel loop's quasi-static residual */
+  int mlGates;        /* multilevel eligibility gates (PECLET_DEM_ML_GATES, default slip) */
   int numColorsVel, numColorsPos;
   pdref_stats stats;
   /* grid-SDF particle shape (src/sim.hpp:1034-1084) */
@@ -482,6 +484,11 @@ void* pdref_create(int capacity) {
   S->velIters = 0;
   S->useGS = 1;
   S->stabMode = 1;
+  {
+    /* src/sim.hpp:318-321: the gate mask is read from the environment, default kGateSlip */
+    const char* e = getenv("PECLET_DEM_ML_GATES");
+    S->mlGates = e ? atoi(e) : 4;
+  }
   S->defaultInvI = mk3(2.5, 2.5, 2.5);
   for (int m = 0; m < KMAXMAT; ++m) { S->hertzE[m] = 1.0e9; S->hertzNu[m] = 0.25; }
   S->hertzContactRadiusFrac = 0.5;
@@ -2170,7 +2177,7 @@ static void runClass(Sim* S, const int* list, int n, int colour, RowFn fn, real 
     for (int k = 0; k < n; ++k) fn(S, list[k], thr);
     return;
   }
-  real ma = S->maxApproach, mo = S->maxOverlap;
+  real ma = S->maxApproach, mo = S->maxOverlap, mq = S->maxApproachQS;
 #pragma omp parallel num_threads(nt)
   {
     Sim L = *S;
@@ -2180,16 +2187,306 @@ static void runClass(Sim* S, const int* list, int n, int colour, RowFn fn, real 
     {
       if (L.maxApproach > ma) ma = L.maxApproach;
       if (L.maxOverlap > mo) mo = L.maxOverlap;
+      if (L.maxApproachQS > mq) mq = L.maxApproachQS;
     }
   }
   S->maxApproach = ma;
   S->maxOverlap = mo;
+  S->maxApproachQS = mq;
 }
 static void warmRow(Sim* S, int idx, real thr) { (void)thr; warmStartApplyOne(S, idx); }
 static void posRow(Sim* S, int idx, real thr) { (void)thr; positionSolveOne(S, idx); }
 static void runClasses(Sim* S, const ColourClasses* C, RowFn fn, real thr) {
   for (int c = 0; c < C->ncol; ++c)
     runClass(S, C->list + C->start[c], C->start[c + 1] - C->start[c], c, fn, thr);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* stabilization mode "multilevel", src/solver_multilevel.hpp (GraphMG-style aggregation) and  */
+/* its driver src/sim.hpp:303-355.                                                             */
+/* Canonical orders where the reference sums with atomics: a group's mass and momentum are     */
+/* summed over the aggregation tree (a level-l group = its leader's level-(l-1) sum plus its   */
+/* mate's), and the rows of the coarse colour 62, which may share a group, run in ascending    */
+/* manifold order.                                                                             */
+/* ------------------------------------------------------------------------------------------ */
+#define ML_MAX_LEVELS 10 /* kMlMaxLevels, src/solver_multilevel.hpp:44 */
+#define ML_SLOT_SKIP 63  /* kMlSlotSkip, :45 */
+#define ML_GATE_PERSISTENT 1 /* kGatePersistent, :82 */
+#define ML_GATE_CONE 2
+#define ML_GATE_SLIP 4
+typedef struct {
+  int numLevels;
+  int groupOff[ML_MAX_LEVELS], numGroups[ML_MAX_LEVELS], parentOff[ML_MAX_LEVELS];
+  int numColors[ML_MAX_LEVELS];
+  unsigned char* slot; /* [level][nm]: the manifold's colour at that level, ML_SLOT_SKIP = none */
+  int* parent;         /* parent maps (finer-level id -> group id), pooled by level */
+  int* child;          /* [2 per group]: its leader and its mate (-1 for a singleton) */
+  real *massG, *invMassG;
+  V3 *velG, *velG0;
+  V3* mom;             /* momentum tree: n bodies, then the pooled groups */
+  int* grp;            /* real body -> group of the current level */
+} MlHier;
+/* mldetail::effMass, src/solver_multilevel.hpp:68-70 */
+static real mlEffMass(real invMass) { return 1.0 / fmax(invMass, 1e-30); }
+/* mldetail::eligible, src/solver_multilevel.hpp:88-119 */
+static int mlEligible(const Sim* S, int idx, real qsThr, int gateMask, V3 gHat) {
+  const Manifold* m = &S->manifolds[idx];
+  if (m->numPoints <= 0 || S->mColor[idx] < 0) return 0;
+  if ((gateMask & ML_GATE_PERSISTENT) && S->persistent[idx] == 0) return 0;
+  if (fabs(S->vn0[idx]) > qsThr) return 0;
+  if (gateMask & ML_GATE_SLIP) {
+    const V3 vt = S->vt0[idx];
+    if (dot3(vt, vt) > qsThr * qsThr) return 0;
+  }
+  if (m->bodyB < 0) return 1;
+  if (gateMask & ML_GATE_CONE) {
+    const V3 dx = sub3(S->posPred[m->bodyA], S->posPred[m->bodyB]);
+    const real up = -(dx.x * gHat.x + dx.y * gHat.y + dx.z * gHat.z);
+    return fabs(up) > 0.3 * sqrt(dot3(dx, dx));
+  }
+  return 1;
+}
+static void mlFree(MlHier* H) {
+  free(H->slot); free(H->parent); free(H->child); free(H->massG); free(H->invMassG);
+  free(H->velG); free(H->velG0); free(H->mom); free(H->grp);
+  memset(H, 0, sizeof *H);
+}
+/* buildContactHierarchyKokkos, src/solver_multilevel.hpp:126-347: per level one round of
+ * random-priority matching (a manifold joins its two groups when its key is the largest at
+ * both), leader-ordered compaction, the group masses, and the colouring of the level's crossing
+ * manifolds by rounds of the same arbitration (at most 64 rounds; a round that colours nothing
+ * more ends it).  The pools are sized by the capacity as the reference's (5 and 4 x capacity,
+ * src/particles.hpp:244-248). */
+static void mlBuild(Sim* S, MlHier* H, real qsThr, int gateMask, V3 gHat) {
+  memset(H, 0, sizeof *H);
+  const int nm = S->nm, n = S->numReal;
+  if (nm <= 0 || n <= 0) return;
+  const int parentCap = 5 * S->capacity, groupCap = 4 * S->capacity;
+  H->slot = (unsigned char*)xrealloc(NULL, (size_t)nm * ML_MAX_LEVELS);
+  memset(H->slot, ML_SLOT_SKIP, (size_t)nm * ML_MAX_LEVELS);
+  H->parent = (int*)xrealloc(NULL, sizeof(int) * (size_t)(parentCap > 0 ? parentCap : 1));
+  H->child = (int*)xrealloc(NULL, sizeof(int) * 2 * (size_t)(groupCap > 0 ? groupCap : 1));
+  H->massG = (real*)xrealloc(NULL, sizeof(real) * (size_t)(groupCap > 0 ? groupCap : 1));
+  H->invMassG = (real*)xrealloc(NULL, sizeof(real) * (size_t)(groupCap > 0 ? groupCap : 1));
+  H->velG = (V3*)xrealloc(NULL, sizeof(V3) * (size_t)(groupCap > 0 ? groupCap : 1));
+  H->velG0 = (V3*)xrealloc(NULL, sizeof(V3) * (size_t)(groupCap > 0 ? groupCap : 1));
+  H->mom = (V3*)xrealloc(NULL, sizeof(V3) * ((size_t)n + (size_t)(groupCap > 0 ? groupCap : 1)));
+  H->grp = (int*)xrealloc(NULL, sizeof(int) * (size_t)n);
+  long long* winner = (long long*)xrealloc(NULL, sizeof(long long) * (size_t)n);
+  int* mate = (int*)xrealloc(NULL, sizeof(int) * (size_t)n);
+  uint64_t* mask = (uint64_t*)xrealloc(NULL, sizeof(uint64_t) * (size_t)n);
+  unsigned char* elig = (unsigned char*)xrealloc(NULL, (size_t)nm);
+  for (int idx = 0; idx < nm; ++idx)
+    elig[idx] = (unsigned char)mlEligible(S, idx, qsThr, gateMask, gHat);
+  for (int i = 0; i < n; ++i) H->grp[i] = i;
+  int ngPrev = n, parentOff = 0, groupOff = 0;
+  for (int lvl = 1; lvl <= ML_MAX_LEVELS; ++lvl) {
+    if (parentOff + ngPrev > parentCap) break;
+    /* matching, :169-207 */
+    for (int g = 0; g < ngPrev; ++g) winner[g] = -1;
+    for (int idx = 0; idx < nm; ++idx) {
+      const Manifold* m = &S->manifolds[idx];
+      if (m->bodyB < 0 || !elig[idx]) continue;
+      const int gA = H->grp[S->realIdx[m->bodyA]], gB = H->grp[S->realIdx[m->bodyB]];
+      if (gA == gB) continue;
+      const long long key = colorKey(idx);
+      if (key > winner[gA]) winner[gA] = key;
+      if (key > winner[gB]) winner[gB] = key;
+    }
+    for (int g = 0; g < ngPrev; ++g) mate[g] = g;
+    for (int idx = 0; idx < nm; ++idx) {
+      const Manifold* m = &S->manifolds[idx];
+      if (m->bodyB < 0 || !elig[idx]) continue;
+      const int gA = H->grp[S->realIdx[m->bodyA]], gB = H->grp[S->realIdx[m->bodyB]];
+      if (gA == gB) continue;
+      const long long key = colorKey(idx);
+      if (winner[gA] == key && winner[gB] == key) {
+        mate[gA] = gB;
+        mate[gB] = gA;
+      }
+    }
+    /* compaction, :209-232: a matched pair's leader (the smaller id) and singletons, in id order */
+    int ngNew = 0;
+    int* parent = H->parent + parentOff;
+    for (int g = 0; g < ngPrev; ++g)
+      if (mate[g] >= g) parent[g] = ngNew++;
+    for (int g = 0; g < ngPrev; ++g)
+      if (mate[g] < g) parent[g] = parent[mate[g]];
+    if (ngNew >= ngPrev || ngNew > (9 * ngPrev) / 10) break;
+    if (groupOff + ngNew > groupCap) break;
+    for (int g = 0; g < ngPrev; ++g)
+      if (mate[g] >= g) {
+        int* c = H->child + 2 * (size_t)(groupOff + parent[g]);
+        c[0] = g;
+        c[1] = mate[g] != g ? mate[g] : -1;
+      }
+    for (int i = 0; i < n; ++i) H->grp[i] = parent[H->grp[i]];
+    /* group masses, :243-256 (summed over the tree) */
+    for (int g = 0; g < ngNew; ++g) {
+      const int* c = H->child + 2 * (size_t)(groupOff + g);
+      const int prevOff = lvl == 1 ? -1 : H->groupOff[lvl - 2];
+      real mass = prevOff < 0 ? mlEffMass(S->invMass[c[0]]) : H->massG[prevOff + c[0]];
+      if (c[1] >= 0) mass += prevOff < 0 ? mlEffMass(S->invMass[c[1]]) : H->massG[prevOff + c[1]];
+      H->massG[groupOff + g] = mass;
+      H->invMassG[groupOff + g] = 1.0 / mass;
+    }
+    /* colouring of the crossing manifolds, :257-327 */
+    unsigned char* slot = H->slot + (size_t)(lvl - 1) * nm;
+    for (int g = 0; g < ngNew; ++g) mask[g] = 0;
+    int remaining = 1, prevRemaining = -1, maxc = -1;
+    for (int round = 0; round < 64 && remaining > 0; ++round) {
+      for (int g = 0; g < ngNew; ++g) winner[g] = -1;
+      for (int idx = 0; idx < nm; ++idx) {
+        if (!elig[idx] || slot[idx] != ML_SLOT_SKIP) continue;
+        const Manifold* m = &S->manifolds[idx];
+        const int gA = H->grp[S->realIdx[m->bodyA]];
+        const int gB = (m->bodyB >= 0) ? H->grp[S->realIdx[m->bodyB]] : -1;
+        if (gB == gA) continue;
+        const long long key = colorKey(idx);
+        if (key > winner[gA]) winner[gA] = key;
+        if (gB >= 0 && key > winner[gB]) winner[gB] = key;
+      }
+      int rem = 0;
+      for (int idx = 0; idx < nm; ++idx) {
+        if (!elig[idx] || slot[idx] != ML_SLOT_SKIP) continue;
+        const Manifold* m = &S->manifolds[idx];
+        const int gA = H->grp[S->realIdx[m->bodyA]];
+        const int gB = (m->bodyB >= 0) ? H->grp[S->realIdx[m->bodyB]] : -1;
+        if (gB == gA) continue;
+        const long long key = colorKey(idx);
+        if (winner[gA] != key || (gB >= 0 && winner[gB] != key)) {
+          ++rem;
+          continue;
+        }
+        uint64_t forbidden = mask[gA];
+        if (gB >= 0) forbidden |= mask[gB];
+        int c = 0;
+        while (c < ML_SLOT_SKIP - 1 && (forbidden & ((uint64_t)1 << c))) ++c;
+        slot[idx] = (unsigned char)c;
+        mask[gA] |= (uint64_t)1 << c;
+        if (gB >= 0) mask[gB] |= (uint64_t)1 << c;
+      }
+      if (rem == prevRemaining) break;
+      prevRemaining = rem;
+      remaining = rem;
+    }
+    for (int idx = 0; idx < nm; ++idx)
+      if (slot[idx] != ML_SLOT_SKIP && slot[idx] > maxc) maxc = slot[idx];
+    H->numColors[lvl - 1] = maxc + 1;
+    H->groupOff[lvl - 1] = groupOff;
+    H->numGroups[lvl - 1] = ngNew;
+    H->parentOff[lvl - 1] = parentOff;
+    H->numLevels = lvl;
+    parentOff += ngPrev;
+    groupOff += ngNew;
+    ngPrev = ngNew;
+    if (ngNew <= 32) break;
+  }
+  free(winner); free(mate); free(mask); free(elig);
+}
+/* One coarse row, the body of ml_coarse_pgs (src/solver_multilevel.hpp:405-466): translation
+ * only, e = 0, on the shared impulse accumulator. */
+static void mlCoarseRow(Sim* S, MlHier* H, int lvl, int idx) {
+  const int off = H->groupOff[lvl - 1];
+  const Manifold* m = &S->manifolds[idx];
+  const int gA = H->grp[S->realIdx[m->bodyA]];
+  const int gB = (m->bodyB >= 0) ? H->grp[S->realIdx[m->bodyB]] : -1;
+  const real invN = 1.0 / (real)m->numPoints;
+  const V3 Nsum = m->normalSum;
+  const real lenN = sqrt(dot3(Nsum, Nsum));
+  if (lenN < 1e-9) return;
+  const V3 rAavg = scale3(m->rASum, invN);
+  const V3 rBavg = scale3(m->rBSum, invN);
+  const V3 diffCenters = (gB < 0) ? rAavg : sub3(rAavg, rBavg);
+  V3* velG = H->velG + off;
+  const V3 vA = velG[gA];
+  const V3 vB = (gB >= 0) ? velG[gB] : scale3(m->wallVelSum, invN);
+  const real vn = dot3(sub3(vA, vB), Nsum);
+  const real alignment = dot3(Nsum, diffCenters);
+  const real sgn = (alignment > 0.0) ? 1.0 : -1.0;
+  const real invMA = H->invMassG[off + gA];
+  const real invMB = (gB >= 0) ? H->invMassG[off + gB] : 0.0;
+  const real w = dot3(Nsum, Nsum) * (invMA + invMB);
+  if (w <= 0.0) return;
+  const real vtil = sgn * vn;
+  const real dp = vtil / w;
+  const real pOld = S->lambdaAcc[idx];
+  real pNew = pOld + dp;
+  if (pNew < 0.0) pNew = 0.0;
+  const real d = pNew - pOld;
+  if (d == 0.0) return;
+  S->lambdaAcc[idx] = pNew;
+  const real appr = fabs(d) * w / lenN;
+  if (appr > S->maxApproachQS) S->maxApproachQS = appr;
+  const real lambda = -sgn * d;
+  const V3 J = scale3(Nsum, lambda);
+  velG[gA].x += J.x * invMA;
+  velG[gA].y += J.y * invMA;
+  velG[gA].z += J.z * invMA;
+  if (gB >= 0) {
+    velG[gB].x -= J.x * invMB;
+    velG[gB].y -= J.y * invMB;
+    velG[gB].z -= J.z * invMB;
+  }
+}
+/* multilevelCoarseCycleKokkos, src/solver_multilevel.hpp:354-483: per level, compose the body
+ * map, restrict (V = sum m v / sum m, summed over the tree), `coarseSweeps` coloured coarse
+ * sweeps, prolongate (every member takes its group's velocity change). */
+static void mlCoarseCycle(Sim* S, MlHier* H, int coarseSweeps) {
+  const int n = S->numReal, nm = S->nm;
+  for (int i = 0; i < n; ++i) H->grp[i] = i;
+  for (int lvl = 1; lvl <= H->numLevels; ++lvl) {
+    const int off = H->groupOff[lvl - 1], ng = H->numGroups[lvl - 1];
+    const int* parent = H->parent + H->parentOff[lvl - 1];
+    for (int i = 0; i < n; ++i) H->grp[i] = parent[H->grp[i]];
+    /* restrict: momenta of the bodies, then up the tree to this level */
+    for (int i = 0; i < n; ++i) {
+      const real m = mlEffMass(S->invMass[i]);
+      H->mom[i] = mk3(m * S->velPred[i].x, m * S->velPred[i].y, m * S->velPred[i].z);
+    }
+    for (int l = 1; l <= lvl; ++l) {
+      const int lo = H->groupOff[l - 1];
+      const V3* below = l == 1 ? H->mom : H->mom + n + H->groupOff[l - 2];
+      for (int g = 0; g < H->numGroups[l - 1]; ++g) {
+        const int* c = H->child + 2 * (size_t)(lo + g);
+        V3 p = below[c[0]];
+        if (c[1] >= 0) p = add3(p, below[c[1]]);
+        H->mom[n + lo + g] = p;
+      }
+    }
+    V3* velG = H->velG + off;
+    V3* velG0 = H->velG0 + off;
+    for (int g = 0; g < ng; ++g) {
+      const real invM = 1.0 / H->massG[off + g];
+      const V3 p = H->mom[n + off + g];
+      velG[g] = mk3(p.x * invM, p.y * invM, p.z * invM);
+      velG0[g] = velG[g];
+    }
+    const unsigned char* slot = H->slot + (size_t)(lvl - 1) * nm;
+    for (int s = 0; s < coarseSweeps; ++s)
+      for (int color = 0; color < H->numColors[lvl - 1]; ++color)
+        for (int idx = 0; idx < nm; ++idx)
+          if (slot[idx] == color) mlCoarseRow(S, H, lvl, idx);
+    for (int i = 0; i < n; ++i) {
+      const int g = H->grp[i];
+      S->velPred[i].x += velG[g].x - velG0[g].x;
+      S->velPred[i].y += velG[g].y - velG0[g].y;
+      S->velPred[i].z += velG[g].z - velG0[g].z;
+    }
+  }
+}
+/* the fine smoothing sweep of the multilevel loop tracks, besides the full residual, the
+ * quasi-static one: corrections of rows with |vn0| <= 4 vRest |N| (src/solver_velocity.hpp:
+ * 721-723, 827-831) */
+static void pgsSolveOneQS(Sim* S, int idx, real restVelThreshold) {
+  const real before = S->maxApproach;
+  S->maxApproach = 0.0;
+  pgsSolveOne(S, idx, restVelThreshold);
+  const real a = S->maxApproach;
+  S->maxApproach = a > before ? a : before;
+  const V3 Nsum = S->manifolds[idx].normalSum;
+  if (fabs(S->vn0[idx]) <= 4.0 * restVelThreshold * sqrt(dot3(Nsum, Nsum)) && a > S->maxApproachQS)
+    S->maxApproachQS = a;
 }
 
 static void fillRad(Sim* S, int n) {
@@ -2202,7 +2499,6 @@ static void orderedStabilization(Sim* S, V3 gHat, real vRestS);
 
 static int demStep(Sim* S) {
   memset(&S->stats, 0, sizeof S->stats);
-  if (S->stabMode == 2) return fail("stabilization mode 'multilevel' is not implemented");
   /* growth ramp, src/sim.hpp:100-108 */
   if (S->growthFactor != -1.0 && S->growthRate != 0.0) {
     S->growthFactor *= exp(S->growthRate * S->dt);
@@ -2294,6 +2590,20 @@ static int demStep(Sim* S) {
           S->stats.sweeps_stab++;
           if (S->maxApproach <= vRestS) break;
         }
+      } else if (smode == 2) {
+        /* src/sim.hpp:303-355: hierarchy once, then fine sweep + coarse cycle until the
+         * quasi-static residual drains */
+        MlHier H;
+        mlBuild(S, &H, 8.0 * S->dt * gMagP, S->mlGates, gHat);
+        S->stats.ml_levels = H.numLevels;
+        for (int it = 0; it < 2 * S->velIters; ++it) {
+          S->maxApproachQS = 0.0;
+          runClasses(S, &VC, pgsSolveOneQS, vRestS);
+          S->stats.sweeps_stab++;
+          if (H.numLevels > 0) mlCoarseCycle(S, &H, 2);
+          if (S->maxApproachQS <= vRestS) break;
+        }
+        mlFree(&H);
       } else if (smode == 3) {
         for (int it = 0; it < 256; ++it) {
           S->maxApproach = 0.0;

@@ -38,6 +38,7 @@ typedef struct pdref_stats {
   long long uncolored_pos; /* sum over rounds of uncoloured contacts entering the round */
   double max_overlap;
   double max_approach;
+  int ml_levels;           /* multilevel stabilization: coarse levels built (0 otherwise) */
 } pdref_stats;
 
 void* pdref_create(int capacity);

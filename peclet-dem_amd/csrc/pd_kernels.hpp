@@ -2434,6 +2434,315 @@ __global__ __launch_bounds__(64) void k_vsweep_serial(const Dev* __restrict__ Dp
   }
 }
 
+// ============================================================================================
+// stabilization mode "multilevel" — src/solver_multilevel.hpp, driven by src/sim.hpp:303-355.
+// Aggregation levels over the quasi-static rows (one round of random-priority matching per
+// level), a colouring of each level's crossing rows, and per loop iteration a fine sweep that
+// also tracks the quasi-static residual, then per level restrict / coarse sweeps / prolongate.
+// A group's mass and momentum are summed over the aggregation tree (a level-l group = its
+// leader's level-(l-1) sum plus its mate's); the coarse colour 62, whose rows may share a
+// group, runs in ascending manifold order.
+// ============================================================================================
+constexpr int ML_MAX_LEVELS = 10;        // kMlMaxLevels, src/solver_multilevel.hpp:44
+constexpr unsigned char ML_SKIP = 63;    // kMlSlotSkip, :45
+constexpr int ML_GATE_PERSISTENT = 1, ML_GATE_CONE = 2, ML_GATE_SLIP = 4;  // :82-84
+struct MlDev {
+  unsigned char* elig;         // [rows] coarse-eligible
+  unsigned char* slot;         // [rows] this level's colour (ML_SKIP = none)
+  int* grp;                    // [numReal] body -> group of the current level
+  unsigned long long* winner;  // [numReal] largest key + 1 (0 = none)
+  int* mate;                   // [numReal]
+  int* lead;                   // [numReal + 1] leader flags
+  int* scan;                   // [numReal + 1] their exclusive scan
+  int* parent;                 // pooled by level: finer id -> group
+  int2* child;                 // pooled by group: (leader, mate or -1)
+  double* massG;
+  double* invMassG;
+  double4* velG;
+  double4* velG0;
+  double4* mom;                // [numReal + group pool] momentum tree
+  unsigned long long* mask;    // [numReal] committed colours per group
+  int* col;                    // [manifolds] this level's colour per manifold (-1 none)
+};
+// mldetail::effMass, src/solver_multilevel.hpp:68-70
+__device__ __forceinline__ double mlEffMass(double invMass) { return 1.0 / fmax(invMass, 1e-30); }
+// mldetail::eligible, src/solver_multilevel.hpp:88-119 (rows are the coloured manifolds)
+__global__ __launch_bounds__(TPB) void k_ml_elig(const Dev* __restrict__ Dp, MlDev ml, int rows, double qsThr,
+                                                 int gates, d3 gHat) {
+  const Dev& D = *Dp;
+  const int k = gtid();
+  if (k >= rows) return;
+  const int m = D.vPerm[k];
+  unsigned char e = D.mNp[m] > 0;
+  if (e && (gates & ML_GATE_PERSISTENT) && !(D.vFlag[k] & F_PERSIST)) e = 0;
+  if (e && fabs(D.vVn0[k]) > qsThr) e = 0;
+  if (e && (gates & ML_GATE_SLIP)) {
+    const d3 vt = xyz(D.vVt0[k]);
+    if (dot3(vt, vt) > qsThr * qsThr) e = 0;
+  }
+  if (e && D.vAB[k].y >= 0 && (gates & ML_GATE_CONE)) {
+    const int2 ab = D.mAB[m];
+    const d3 dx = sub3(xyz(D.posPred[ab.x]), xyz(D.posPred[ab.y]));
+    const double up = -(dx.x * gHat.x + dx.y * gHat.y + dx.z * gHat.z);
+    e = fabs(up) > 0.3 * sqrt(dot3(dx, dx));
+  }
+  ml.elig[k] = e;
+}
+__global__ void k_ml_iota(int* a, int n) {
+  const int i = gtid();
+  if (i < n) a[i] = i;
+}
+// ml_match_contend / ml_match_commit, src/solver_multilevel.hpp:169-207
+__global__ __launch_bounds__(TPB) void k_ml_match_contend(const Dev* __restrict__ Dp, MlDev ml, int rows) {
+  const Dev& D = *Dp;
+  const int k = gtid();
+  if (k >= rows || !ml.elig[k]) return;
+  const int2 r = D.vAB[k];
+  if (r.y < 0) return;
+  const int gA = ml.grp[r.x], gB = ml.grp[r.y];
+  if (gA == gB) return;
+  const unsigned long long key = colorKey1(D.vPerm[k]);
+  atomicMax(&ml.winner[gA], key);
+  atomicMax(&ml.winner[gB], key);
+}
+__global__ __launch_bounds__(TPB) void k_ml_match_commit(const Dev* __restrict__ Dp, MlDev ml, int rows) {
+  const Dev& D = *Dp;
+  const int k = gtid();
+  if (k >= rows || !ml.elig[k]) return;
+  const int2 r = D.vAB[k];
+  if (r.y < 0) return;
+  const int gA = ml.grp[r.x], gB = ml.grp[r.y];
+  if (gA == gB) return;
+  const unsigned long long key = colorKey1(D.vPerm[k]);
+  if (ml.winner[gA] == key && ml.winner[gB] == key) {
+    ml.mate[gA] = gB;
+    ml.mate[gB] = gA;
+  }
+}
+// ml_compact, :209-232: leaders are the smaller id of a matched pair and the singletons
+__global__ void k_ml_lead(MlDev ml, int ng) {
+  const int g = gtid();
+  if (g < ng) ml.lead[g] = ml.mate[g] >= g ? 1 : 0;
+}
+__global__ void k_ml_parent(MlDev ml, int ng, int parentOff, int groupOff) {
+  const int g = gtid();
+  if (g >= ng) return;
+  const int mt = ml.mate[g];
+  if (mt >= g) {
+    const int id = ml.scan[g];
+    ml.parent[parentOff + g] = id;
+    ml.child[groupOff + id] = make_int2(g, mt != g ? mt : -1);
+  } else {
+    ml.parent[parentOff + g] = ml.scan[mt];
+  }
+}
+// ml_grp_compose, :236-242 (and ml_cycle_compose, :376-381)
+__global__ void k_ml_compose(MlDev ml, int n, int parentOff) {
+  const int i = gtid();
+  if (i < n) ml.grp[i] = ml.parent[parentOff + ml.grp[i]];
+}
+// group masses, :243-256, summed over the tree
+__global__ __launch_bounds__(TPB) void k_ml_mass(const Dev* __restrict__ Dp, MlDev ml, int ng, int groupOff,
+                                                 int prevOff) {
+  const Dev& D = *Dp;
+  const int g = gtid();
+  if (g >= ng) return;
+  const int2 c = ml.child[groupOff + g];
+  double m = prevOff < 0 ? mlEffMass(D.bc[c.x].iim.w) : ml.massG[prevOff + c.x];
+  if (c.y >= 0) m += prevOff < 0 ? mlEffMass(D.bc[c.y].iim.w) : ml.massG[prevOff + c.y];
+  ml.massG[groupOff + g] = m;
+  ml.invMassG[groupOff + g] = 1.0 / m;
+}
+// ml_color_contend / ml_color_commit, :266-312: one round of the crossing-row colouring; the
+// losers of the round are counted into *rem
+__global__ __launch_bounds__(TPB) void k_ml_col_contend(const Dev* __restrict__ Dp, MlDev ml, int rows) {
+  const Dev& D = *Dp;
+  const int k = gtid();
+  if (k >= rows || !ml.elig[k] || ml.slot[k] != ML_SKIP) return;
+  const int2 r = D.vAB[k];
+  const int gA = ml.grp[r.x];
+  const int gB = r.y >= 0 ? ml.grp[r.y] : -1;
+  if (gB == gA) return;
+  const unsigned long long key = colorKey1(D.vPerm[k]);
+  atomicMax(&ml.winner[gA], key);
+  if (gB >= 0) atomicMax(&ml.winner[gB], key);
+}
+__global__ __launch_bounds__(TPB) void k_ml_col_commit(const Dev* __restrict__ Dp, MlDev ml, int rows,
+                                                       int* rem) {
+  const Dev& D = *Dp;
+  __shared__ int sh[WAVES];
+  const int k = gtid();
+  int lost = 0;
+  if (k < rows && ml.elig[k] && ml.slot[k] == ML_SKIP) {
+    const int2 r = D.vAB[k];
+    const int gA = ml.grp[r.x];
+    const int gB = r.y >= 0 ? ml.grp[r.y] : -1;
+    if (gB != gA) {
+      const unsigned long long key = colorKey1(D.vPerm[k]);
+      if (ml.winner[gA] != key || (gB >= 0 && ml.winner[gB] != key)) {
+        lost = 1;
+      } else {
+        // the only winner at both groups this round: nothing else writes their masks
+        unsigned long long forbidden = ml.mask[gA];
+        if (gB >= 0) forbidden |= ml.mask[gB];
+        int c = 0;
+        while (c < ML_SKIP - 1 && (forbidden & (1ull << c))) ++c;
+        ml.slot[k] = (unsigned char)c;
+        ml.mask[gA] |= 1ull << c;
+        if (gB >= 0) ml.mask[gB] |= 1ull << c;
+      }
+    }
+  }
+  blockAtomicAddI(rem, lost, sh);
+}
+// the level's colour per manifold, for the colour-class placement of its rows
+__global__ __launch_bounds__(TPB) void k_ml_col_of(const Dev* __restrict__ Dp, MlDev ml, int nm) {
+  const Dev& D = *Dp;
+  const int m = gtid();
+  if (m >= nm) return;
+  int c = -1;
+  if (D.mColor[m] >= 0) {
+    const int k = D.vRowOf[m];
+    if (k >= 0 && ml.slot[k] != ML_SKIP) c = ml.slot[k];
+  }
+  ml.col[m] = c;
+}
+// restrict, src/solver_multilevel.hpp:382-405: body momenta, then up the tree
+__global__ __launch_bounds__(TPB) void k_ml_mom0(const Dev* __restrict__ Dp, MlDev ml, int n) {
+  const Dev& D = *Dp;
+  const int i = gtid();
+  if (i >= n) return;
+  const double m = mlEffMass(D.bc[i].iim.w);
+  const double4 v = D.vw[i].v;
+  ml.mom[i] = make_double4(m * v.x, m * v.y, m * v.z, 0.0);
+}
+__global__ void k_ml_tree(MlDev ml, int ng, int groupOff, int belowOff, int n) {
+  const int g = gtid();
+  if (g >= ng) return;
+  const int2 c = ml.child[groupOff + g];
+  double4 p = ml.mom[belowOff + c.x];
+  if (c.y >= 0) {
+    const double4 q = ml.mom[belowOff + c.y];
+    p.x += q.x;
+    p.y += q.y;
+    p.z += q.z;
+  }
+  ml.mom[n + groupOff + g] = p;
+}
+__global__ void k_ml_restrict(MlDev ml, int ng, int groupOff, int n) {
+  const int g = gtid();
+  if (g >= ng) return;
+  const double invM = 1.0 / ml.massG[groupOff + g];
+  const double4 p = ml.mom[n + groupOff + g];
+  const double4 v = make_double4(p.x * invM, p.y * invM, p.z * invM, 0.0);
+  ml.velG[groupOff + g] = v;
+  ml.velG0[groupOff + g] = v;
+}
+// ml_coarse_pgs body, :405-466: translation only, e = 0, on the shared impulse accumulator
+__device__ void mlCoarseRow(const Dev& D, const MlDev& ml, int off, int k,
+                            unsigned long long* qsBits) {
+  const int2 r = D.vAB[k];
+  const int gA = ml.grp[r.x];
+  const int gB = r.y >= 0 ? ml.grp[r.y] : -1;
+  const double4 n4 = D.vN[k], ra4 = D.vRA[k];
+  const d3 Nsum = xyz(n4);
+  const double lenN = ra4.w;
+  if (lenN < 1e-9) return;
+  const double invN = D.vTB[k].w;
+  const double sgn = D.vRB[k].w;  // the sign of Nsum . (rA - rB), as in the fine row
+  const double4 vA4 = ml.velG[off + gA];
+  const d3 vA = xyz(vA4);
+  const d3 vB = gB >= 0 ? xyz(ml.velG[off + gB]) : scale3(xyz(D.vWV[k]), invN);
+  const double vn = dot3(sub3(vA, vB), Nsum);
+  const double invMA = ml.invMassG[off + gA];
+  const double invMB = gB >= 0 ? ml.invMassG[off + gB] : 0.0;
+  const double w = dot3(Nsum, Nsum) * (invMA + invMB);
+  if (w <= 0.0) return;
+  const double vtil = sgn * vn;
+  const double dp = vtil / w;
+  const double pOld = D.vLam[k];
+  double pNew = pOld + dp;
+  if (pNew < 0.0) pNew = 0.0;
+  const double d = pNew - pOld;
+  if (d == 0.0) return;
+  D.vLam[k] = pNew;
+  atomicMaxD(qsBits, fabs(d) * w / lenN);
+  const double lambda = -sgn * d;
+  const d3 J = scale3(Nsum, lambda);
+  double4 a = vA4;
+  a.x += J.x * invMA;
+  a.y += J.y * invMA;
+  a.z += J.z * invMA;
+  ml.velG[off + gA] = a;
+  if (gB >= 0) {
+    double4 b = ml.velG[off + gB];
+    b.x -= J.x * invMB;
+    b.y -= J.y * invMB;
+    b.z -= J.z * invMB;
+    ml.velG[off + gB] = b;
+  }
+}
+// one coarse colour class (below 62: the rows touch distinct groups)
+__global__ __launch_bounds__(TPB) void k_ml_coarse(const Dev* __restrict__ Dp, MlDev ml, const int* perm,
+                                                   int b, int e, int off,
+                                                   unsigned long long* qsBits) {
+  const Dev& D = *Dp;
+  const int q = b + gtid();
+  if (q >= e) return;
+  mlCoarseRow(D, ml, off, D.vRowOf[perm[q]], qsBits);
+}
+__global__ __launch_bounds__(64) void k_ml_coarse_serial(const Dev* __restrict__ Dp, MlDev ml, const int* perm,
+                                                         int b, int e, int off,
+                                                         unsigned long long* qsBits) {
+  const Dev& D = *Dp;
+  if (gtid() != 0) return;
+  for (int q = b; q < e; ++q) mlCoarseRow(D, ml, off, D.vRowOf[perm[q]], qsBits);
+}
+// ml_prolongate, :468-478: every member takes its group's velocity change
+__global__ __launch_bounds__(TPB) void k_ml_prolong(const Dev* __restrict__ Dp, MlDev ml, int n, int off) {
+  const Dev& D = *Dp;
+  const int i = gtid();
+  if (i >= n) return;
+  const int g = ml.grp[i];
+  const double4 a = ml.velG[off + g], a0 = ml.velG0[off + g];
+  double4 v = D.vw[i].v;
+  v.x += a.x - a0.x;
+  v.y += a.y - a0.y;
+  v.z += a.z - a0.z;
+  D.vw[i].v = v;
+}
+// the fine smoothing sweep of the multilevel loop: PGS rows that also report the quasi-static
+// residual, corrections of rows with |vn0| <= 4 vRest |N| (src/solver_velocity.hpp:721-723,
+// 827-831)
+__device__ __forceinline__ double mlQs(const Dev& D, int k, double a, double thr) {
+  return fabs(D.vVn0[k]) <= 4.0 * thr * D.vRA[k].w ? a : 0.0;
+}
+__global__ __launch_bounds__(TPB) void k_vsweep_qs(const Dev* __restrict__ Dp, Params P, int rowBegin, int rowEnd,
+                                                   double thr, unsigned long long* qsBits) {
+  const Dev& D = *Dp;
+  __shared__ unsigned long long sh[WAVES];
+  const int k = rowBegin + gtid();
+  double a = 0.0, q = 0.0;
+  if (k < rowEnd) {
+    a = pgsRow(D, P, k, thr);
+    q = mlQs(D, k, a, thr);
+  }
+  blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
+  __syncthreads();
+  blockAtomicMaxU64(qsBits, dbits(q), sh);
+}
+__global__ __launch_bounds__(64) void k_vsweep_qs_serial(const Dev* __restrict__ Dp, Params P, int rowBegin,
+                                                         int rowEnd, double thr,
+                                                         unsigned long long* qsBits) {
+  const Dev& D = *Dp;
+  if (gtid() != 0) return;
+  for (int k = rowBegin; k < rowEnd; ++k) {
+    const double a = pgsRow(D, P, k, thr);
+    atomicMaxD(D.maxApproachBits, a);
+    atomicMaxD(qsBits, mlQs(D, k, a, thr));
+  }
+}
+
 // ---- small problems: a whole adaptive sweep loop in one workgroup ---------------------------
 // Rows of one colour are body-disjoint, so the colours run in order separated by workgroup
 // barriers and each sweep's residual is a block reduction; the stop rule is the loop's own.  The

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
@@ -370,6 +371,8 @@ struct pd_sim {
     jobV.stream = st;
     jobP.stream = sideStream;
     ensureSlots(cap > 0 ? cap : 1, 0);
+    // src/sim.hpp:318-321: the multilevel gate mask comes from the environment, default slip
+    if (const char* e = std::getenv("PECLET_DEM_ML_GATES")) mlGates = std::atoi(e);
     red.ensure(8, st);
     bits.ensure(4, st);
     counters.ensure(16, st);
@@ -1413,6 +1416,201 @@ struct pd_sim {
       if (maxApproach <= vRestS) break;
     }
   }
+  // stabilization mode 2 ("multilevel"), src/sim.hpp:303-355 with src/solver_multilevel.hpp:
+  // the hierarchy is built once, then each iteration runs one fine sweep (tracking the
+  // quasi-static residual) and the coarse cycle, until that residual is at most vRestS.
+  DBuf<unsigned char> mlElig, mlSlot;
+  DBuf<int> mlGrp, mlMate, mlLead, mlScan, mlParent, mlCol, mlPerm, mlRowOfTmp, mlCnt;
+  DBuf<int> mlBlockCnt, mlColCnt, mlColStart, mlClsCnt;
+  DBuf<int2> mlChild;
+  DBuf<double> mlMassG, mlInvMassG;
+  DBuf<double4> mlVelG, mlVelG0, mlMom;
+  DBuf<unsigned long long> mlWinner, mlMask;
+  int mlGates = ML_GATE_SLIP;
+  struct MlLevel {
+    int groupOff, numGroups, parentOff;
+    std::vector<int> starts;  // colour classes of the level's crossing rows (65 entries)
+  };
+  MlDev mlDev(int level) {
+    MlDev m{};
+    m.elig = mlElig.p;
+    m.slot = mlSlot.p + (size_t)level * nRowsV;
+    m.grp = mlGrp.p;
+    m.winner = mlWinner.p;
+    m.mate = mlMate.p;
+    m.lead = mlLead.p;
+    m.scan = mlScan.p;
+    m.parent = mlParent.p;
+    m.child = mlChild.p;
+    m.massG = mlMassG.p;
+    m.invMassG = mlInvMassG.p;
+    m.velG = mlVelG.p;
+    m.velG0 = mlVelG0.p;
+    m.mom = mlMom.p;
+    m.mask = mlMask.p;
+    m.col = mlCol.p;
+    return m;
+  }
+  // buildContactHierarchyKokkos, src/solver_multilevel.hpp:126-347
+  std::vector<MlLevel> mlBuild(double qsThr, d3 gHat) {
+    std::vector<MlLevel> L;
+    const int n = numReal, R = nRowsV;
+    if (nm <= 0 || n <= 0 || R <= 0) return L;
+    // the pools are sized by the capacity, as the reference's (src/particles.hpp:244-248); a
+    // level needs room for the finer level's parents and its own groups
+    const long parentCap = 5L * capacity, groupCap = 4L * capacity;
+    const size_t parentNeed = std::min<long>(parentCap, 10L * n) + 1;
+    const size_t groupNeed = std::min<long>(groupCap, 10L * n) + 1;
+    mlElig.ensure(R + 1, st);
+    mlSlot.ensure((size_t)ML_MAX_LEVELS * R + 1, st);
+    mlGrp.ensure(n + 1, st);
+    mlMate.ensure(n + 1, st);
+    mlLead.ensure(n + 1, st);
+    mlScan.ensure(n + 1, st);
+    mlWinner.ensure(n + 1, st);
+    mlMask.ensure(n + 1, st);
+    mlParent.ensure(parentNeed, st);
+    mlChild.ensure(groupNeed, st);
+    mlMassG.ensure(groupNeed, st);
+    mlInvMassG.ensure(groupNeed, st);
+    mlVelG.ensure(groupNeed, st);
+    mlVelG0.ensure(groupNeed, st);
+    mlMom.ensure(n + groupNeed, st);
+    mlCol.ensure(nm + 1, st);
+    mlRowOfTmp.ensure(nm + 1, st);
+    mlCnt.ensure(64, st);
+    mlBlockCnt.ensure(64 * CP_BLOCKS, st);
+    mlColCnt.ensure(64, st);
+    mlColStart.ensure(65, st);
+    mlClsCnt.ensure(CLS_BLOCKS, st);
+    k_ml_elig<<<nblk(R), TPB, 0, st>>>(dD, mlDev(0), R, qsThr, mlGates, gHat);
+    k_fill_u8<<<nblk((long)ML_MAX_LEVELS * R), TPB, 0, st>>>(mlSlot.p, ML_SKIP,
+                                                             (long)ML_MAX_LEVELS * R);
+    k_ml_iota<<<nblk(n), TPB, 0, st>>>(mlGrp.p, n);
+    int ngPrev = n, parentOff = 0, groupOff = 0;
+    for (int lvl = 1; lvl <= ML_MAX_LEVELS; ++lvl) {
+      if ((long)parentOff + ngPrev > parentCap) break;
+      const MlDev m = mlDev(lvl - 1);
+      // matching
+      k_fill_u64<<<nblk(ngPrev), TPB, 0, st>>>(mlWinner.p, 0ull, ngPrev);
+      k_ml_iota<<<nblk(ngPrev), TPB, 0, st>>>(mlMate.p, ngPrev);
+      k_ml_match_contend<<<nblk(R), TPB, 0, st>>>(dD, m, R);
+      k_ml_match_commit<<<nblk(R), TPB, 0, st>>>(dD, m, R);
+      k_ml_lead<<<nblk(ngPrev), TPB, 0, st>>>(m, ngPrev);
+      scanQuiet(mlLead.p, mlScan.p, ngPrev);
+      const int ngNew = read1(mlScan.p + ngPrev);
+      if (ngNew >= ngPrev || ngNew > (9 * ngPrev) / 10) break;
+      if ((long)groupOff + ngNew > groupCap) break;
+      k_ml_parent<<<nblk(ngPrev), TPB, 0, st>>>(m, ngPrev, parentOff, groupOff);
+      k_ml_compose<<<nblk(n), TPB, 0, st>>>(m, n, parentOff);
+      k_ml_mass<<<nblk(ngNew), TPB, 0, st>>>(dD, m, ngNew, groupOff,
+                                             lvl == 1 ? -1 : L.back().groupOff);
+      // colouring of the crossing rows: at most 64 rounds, run in batches whose loser counts
+      // are read once per batch (a round after the last contender changes nothing)
+      k_fill_u64<<<nblk(ngNew), TPB, 0, st>>>(mlMask.p, 0ull, ngNew);
+      int round = 0, remaining = 1, prevRemaining = -1, batch = 1;
+      while (round < 64 && remaining > 0) {
+        const int B = std::min(batch, 64 - round);
+        zero(mlCnt.p, B);
+        for (int j = 0; j < B; ++j) {
+          k_fill_u64<<<nblk(ngNew), TPB, 0, st>>>(mlWinner.p, 0ull, ngNew);
+          k_ml_col_contend<<<nblk(R), TPB, 0, st>>>(dD, m, R);
+          k_ml_col_commit<<<nblk(R), TPB, 0, st>>>(dD, m, R, mlCnt.p + j);
+        }
+        int cnt[64];
+        readN(cnt, mlCnt.p, B);
+        bool stop = false;
+        for (int j = 0; j < B && !stop; ++j) {
+          ++round;
+          // a round that left as many losers as the last one ends the colouring (the
+          // reference's stall rule); that cannot happen while a contender is left, since the
+          // largest key always wins
+          if (cnt[j] == prevRemaining) stop = true;
+          prevRemaining = cnt[j];
+          remaining = cnt[j];
+          if (remaining <= 0) stop = true;
+        }
+        if (stop) break;
+        batch *= 2;
+      }
+      // the level's rows by colour (colour 62, whose rows may share a group, in manifold order)
+      MlLevel lv{groupOff, ngNew, parentOff, {}};
+      mlPerm.ensure((size_t)lvl * nm + 1, st, (size_t)(lvl - 1) * nm);
+      int* perm = mlPerm.p + (size_t)(lvl - 1) * nm;
+      k_ml_col_of<<<nblk(nm), TPB, 0, st>>>(dD, m, nm);
+      const int nbc = std::min(nblk(nm), CP_BLOCKS);
+      k_color_count<<<nbc, TPB, 0, st>>>(mlCol.p, nm, mlBlockCnt.p);
+      k_color_offsets<<<64, TPB, 0, st>>>(mlBlockCnt.p, nbc, mlColCnt.p);
+      k_color_starts<<<1, 64, 0, st>>>(mlColCnt.p, mlColStart.p);
+      k_color_place<<<nbc, TPB, 0, st>>>(mlCol.p, nm, mlBlockCnt.p, mlColStart.p, perm,
+                                         mlRowOfTmp.p);
+      lv.starts.resize(65);
+      readN(lv.starts.data(), mlColStart.p, 65);
+      if (lv.starts[63] > lv.starts[62]) {
+        const int nb = std::max(1, std::min(CLS_BLOCKS, nblk(nm)));
+        k_class_chunk_count<<<nb, 1024, 0, st>>>(mlCol.p, nm, 62, mlClsCnt.p);
+        k_class_chunk_place<<<nb, 1024, 0, st>>>(mlCol.p, nm, 62, lv.starts[62], mlClsCnt.p,
+                                                 perm, mlRowOfTmp.p);
+      }
+      HIPCHECK(hipGetLastError());
+      L.push_back(std::move(lv));
+      parentOff += ngPrev;
+      groupOff += ngNew;
+      ngPrev = ngNew;
+      if (ngNew <= 32) break;
+    }
+    return L;
+  }
+  // multilevelCoarseCycleKokkos, src/solver_multilevel.hpp:354-483 (2 coarse sweeps per level)
+  void mlCoarseCycle(const std::vector<MlLevel>& L, unsigned long long* qsBits) {
+    const int n = numReal;
+    k_ml_iota<<<nblk(n), TPB, 0, st>>>(mlGrp.p, n);
+    for (int lvl = 1; lvl <= (int)L.size(); ++lvl) {
+      const MlLevel& lv = L[lvl - 1];
+      const MlDev m = mlDev(lvl - 1);
+      const int off = lv.groupOff, ng = lv.numGroups;
+      k_ml_compose<<<nblk(n), TPB, 0, st>>>(m, n, lv.parentOff);
+      k_ml_mom0<<<nblk(n), TPB, 0, st>>>(dD, m, n);
+      for (int l = 1; l <= lvl; ++l)
+        k_ml_tree<<<nblk(L[l - 1].numGroups), TPB, 0, st>>>(
+            m, L[l - 1].numGroups, L[l - 1].groupOff, l == 1 ? 0 : n + L[l - 2].groupOff, n);
+      k_ml_restrict<<<nblk(ng), TPB, 0, st>>>(m, ng, off, n);
+      const int* perm = mlPerm.p + (size_t)(lvl - 1) * nm;
+      for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < 63; ++c) {
+          const int b = lv.starts[c], e = lv.starts[c + 1];
+          if (e <= b) continue;
+          if (c == 62)
+            k_ml_coarse_serial<<<1, 64, 0, st>>>(dD, m, perm, b, e, off, qsBits);
+          else
+            k_ml_coarse<<<nblk(e - b), TPB, 0, st>>>(dD, m, perm, b, e, off, qsBits);
+        }
+      k_ml_prolong<<<nblk(n), TPB, 0, st>>>(dD, m, n, off);
+    }
+    HIPCHECK(hipGetLastError());
+  }
+  void multilevelStabilization(const std::vector<int>& vs, int numColors, d3 gHat,
+                               double vRestS, double qsThr) {
+    const std::vector<MlLevel> L = mlBuild(qsThr, gHat);
+    stats.ml_levels = (int)L.size();
+    unsigned long long* qsBits = bits.p + 3;
+    Params P = params(0.0, 0.0);
+    for (int it = 0; it < 2 * velIters; ++it) {
+      zero(qsBits, 1);
+      for (int c = 0; c < numColors; ++c) {
+        const int b = vs[c], e = vs[c + 1];
+        if (e <= b) continue;
+        if (c == 62)
+          k_vsweep_qs_serial<<<1, 64, 0, st>>>(dD, P, b, e, vRestS, qsBits);
+        else
+          k_vsweep_qs<<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, vRestS, qsBits);
+      }
+      stats.sweeps_stab++;
+      if (!L.empty()) mlCoarseCycle(L, qsBits);
+      HIPCHECK(hipGetLastError());
+      if (bitsd(read1(qsBits)) <= vRestS) break;
+    }
+  }
   static ClassStarts classStarts(const std::vector<int>& s) {
     ClassStarts c{};
     for (int k = 0; k < 65 && k < (int)s.size(); ++k) c.s[k] = s[k];
@@ -1483,7 +1681,6 @@ struct pd_sim {
     vFusedRec = false;
     timedNow = timing > 0 && timingSubstep % timingEvery == 0;
     ++timingSubstep;
-    if (stabMode == 2) throw std::runtime_error("stabilization mode 'multilevel' is not implemented");
     // a frictional wall drives friction even for frictionless bodies (src/sim.hpp:169-170)
     double wallFrictionMax = 0.0;
     for (const WallP& w : wallsHost) wallFrictionMax = std::max(wallFrictionMax, w.friction);
@@ -1667,7 +1864,7 @@ struct pd_sim {
     const d3 gHat = usePersist ? d3{g[0] / gMagP, g[1] / gMagP, g[2] / gMagP} : d3{0, 0, 0};
     const double vRest = 2.0 * dt * std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
     const double vRestS = 2.0 * dt * gMagP;
-    const bool hostLoop = velLeftover > 0 || (usePGS && (stabMode == 3 || stabMode == 4)) || !useGS ||
+    const bool hostLoop = velLeftover > 0 || (usePGS && stabMode >= 2) || !useGS ||
                           legacyFriction;
     const bool smallV = nRowsV <= blockRows;
     // the whole velocity stage in one launch on LDS-staged bodies
@@ -1836,6 +2033,9 @@ struct pd_sim {
                                                        nullptr);
           if (stabMode == 4) {
             orderedStabilization(gHat, vRestS);
+          } else if (stabMode == 2) {
+            multilevelStabilization(vs, numColors, gHat, vRestS, 8.0 * dt * gMagP);
+            maxApproach = readMaxApproach();
           } else {
             const int cap = stabMode == 1 ? 2 * velIters : 256;
             for (int it = 0; it < cap; ++it) {

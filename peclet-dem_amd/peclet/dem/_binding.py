@@ -39,6 +39,7 @@ class Stats(C.Structure):
         ("uncolored_pos", C.c_longlong),
         ("max_overlap", C.c_double),
         ("max_approach", C.c_double),
+        ("ml_levels", C.c_int),
     ]
 
     def as_dict(self):

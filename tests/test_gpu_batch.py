@@ -49,7 +49,9 @@ def test_batch_reports_a_failing_system(hip_lib):
     sc = make_scene("bed", 10, 0)
     apply_scene(batch.systems[0], sc)
     apply_scene(batch.systems[1], sc)
-    batch.systems[1].set_stabilization_mode("multilevel")
+    # an open recording whose masses change fails the system's next substep
+    batch.systems[1].tape_begin(4)
+    batch.systems[1].set_inv_mass(np.full(10, 2.0))
     with pytest.raises(RuntimeError, match="system 1"):
         batch.step(1e-5, 1)
     batch.close()

@@ -13,7 +13,7 @@ RTOL = 1e-12
 INT_STATS = ["num_real", "num_particles", "num_pairs", "num_contacts", "num_manifolds",
              "active_manifolds", "num_colors_vel", "num_colors_pos", "rounds_vel", "rounds_pos",
              "sweeps_vel", "sweeps_stab", "sweeps_pos", "stab_ran", "uncolored_vel",
-             "uncolored_pos"]
+             "uncolored_pos", "ml_levels"]
 
 
 def hip_sim(capacity):
@@ -77,7 +77,7 @@ def assert_ledger(h, r, tag):
 
 
 def run_pair(kind, n, steps, seed=0, check_topology_every=1, block_rows=None, tuning=None,
-             **over):
+             log=None, **over):
     scene = make_scene(kind, n, seed)
     scene.update(over)
     h = apply_scene(hip_sim(n), scene)
@@ -92,6 +92,8 @@ def run_pair(kind, n, steps, seed=0, check_topology_every=1, block_rows=None, tu
         h.step(scene["dt"])
         r.step(scene["dt"])
         assert_stats(h, r, tag)
+        if log is not None:
+            log.append(r.get_stats())
         if check_topology_every and k % check_topology_every == 0:
             assert_topology(h, r, tag, colours=scene.get("use_gs", True))
         assert_state(h, r, tag)
@@ -172,6 +174,28 @@ def test_cylinders_one_launch_per_colour(hip_lib):
 @pytest.mark.gpu
 def test_escalate_stabilization(hip_lib):
     run_pair("bed", 200, 6, stab="escalate")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gates,kind,n,steps,levels", [
+    (None, "bed", 200, 8, 1), (None, "bed_periodic", 400, 12, 2),
+    (None, "bed_periodic", 4000, 5, 2), ("1", "bed", 200, 8, 2),
+    ("1", "bed_periodic", 400, 12, 5), ("2", "bed_periodic", 400, 12, 1),
+    ("7", "bed_periodic", 400, 12, 0)])
+def test_multilevel_stabilization(hip_lib, monkeypatch, gates, kind, n, steps, levels):
+    """Stabilization mode "multilevel": the aggregation levels, their colourings, the coarse
+    sweeps and the quasi-static stop equal the oracle's every substep, under the default slip
+    gate and the persistent (1), cone (2) and all (7) gate masks (PECLET_DEM_ML_GATES, read when
+    a simulation is created).  `levels`: the deepest hierarchy the run builds, so the coarse
+    path runs where it should."""
+    if gates is None:
+        monkeypatch.delenv("PECLET_DEM_ML_GATES", raising=False)
+    else:
+        monkeypatch.setenv("PECLET_DEM_ML_GATES", gates)
+    log = []
+    run_pair(kind, n, steps, stab="multilevel", log=log)
+    assert max(st["ml_levels"] for st in log) == levels
+    assert sum(st["stab_ran"] for st in log) > 0
 
 
 @pytest.mark.gpu

@@ -1005,3 +1005,117 @@ def test_column_two_substeps_warm_start_and_side_flags(mode):
     np.testing.assert_allclose(s.get_velocities()[:, 2], v2, rtol=0, atol=1e-12 * g * dt)
     assert st2 > 0 and all(persistent) and any(prev)
     assert np.array_equal(s.debug_grounded(), level)
+
+
+# ---- stabilization mode "multilevel" (src/solver_multilevel.hpp, src/sim.hpp:303-355) ----
+def _ml_pair(layout, mode="multilevel", dt=1e-3, g=10.0, u=5e-2, u_upper=None):
+    """Two unit-mass spheres (radius 0.5) touching each other, the lower one (or both, side by
+    side) touching the floor, moving down at u (quasi-static: u + g dt <= 8 g dt), frictionless,
+    e = 0, one velocity iteration (so the stabilization loop is capped at two)."""
+    s = oracle_sim(2)
+    s.set_sphere_shape(0.5)
+    s.set_domain((-5, -5, -1), (5, 5, 10))
+    s.enable_periodicity(False, False, False)
+    s.add_plane(0, 0, 0, 0, 0, 1)
+    if layout == "vertical":
+        s.set_positions(np.array([[0.0, 0.0, 0.5], [0.0, 0.0, 1.5]]))
+    else:
+        s.set_positions(np.array([[0.0, 0.0, 0.5], [1.0, 0.0, 0.5]]))
+    s.set_inv_mass(np.ones(2))
+    s.set_inv_inertia(np.full((2, 3), 10.0))
+    s.set_velocities(np.array([[0.0, 0.0, -u], [0.0, 0.0, -(u if u_upper is None else u_upper)]]))
+    s.set_gravity(0, 0, -g)
+    s.set_material_params(0.0, 0.0, 0.0)
+    s.set_solver_iterations(4, 1)
+    s.set_stabilization_mode(mode)
+    return s, u + g * dt
+
+
+def test_multilevel_two_sphere_column_closed_form(monkeypatch):
+    """Stacked pair on the floor, approach a = u + g dt.  The main sweep leaves the pair at
+    -a/2 (in either colour order); the gate fires; one level aggregates the pair (mass 2); the
+    first loop iteration's fine sweep leaves it at -a/4 (momentum -a/2), and the coarse floor row
+    stops the aggregate: floor impulse 2a in total, the pair row's 0.75a, both bodies at rest;
+    the second iteration changes nothing.  Plain sweeps (escalate) and the one-sided pass end
+    elsewhere."""
+    monkeypatch.delenv("PECLET_DEM_ML_GATES", raising=False)
+    s, a = _ml_pair("vertical")
+    s.step(1e-3)
+    st = s.get_stats()
+    assert (st["stab_ran"], st["ml_levels"], st["sweeps_vel"], st["sweeps_stab"]) == (1, 1, 1, 2)
+    ids, _ = s.debug_manifolds()
+    keys, lam, _, _ = s.debug_ledger()
+    floor = [k for k in range(len(ids)) if ids[k, 1] < 0]
+    assert len(ids) == 2 and len(floor) == 1
+    # the ledger is sorted by key; the wall row's key sorts after the pair's (body B = -1)
+    lam_floor = lam[np.argmax(keys)]
+    lam_pair = lam[np.argmin(keys)]
+    assert abs(lam_floor - 2.0 * a) <= 1e-12 * a and abs(lam_pair - 0.75 * a) <= 1e-12 * a
+    np.testing.assert_allclose(s.get_velocities(), 0.0, atol=1e-12 * a)
+    for other in ("escalate", "onesided"):
+        o, _ = _ml_pair("vertical", other)
+        o.step(1e-3)
+        assert o.get_stats()["ml_levels"] == 0
+        assert np.abs(o.get_velocities()[:, 2]).max() > 0.1 * a
+
+
+@pytest.mark.parametrize("gates,layout,levels", [
+    (None, "vertical", 1), (None, "horizontal", 1),  # slip gate: vt0 = 0, both rows eligible
+    ("1", "vertical", 0),                            # persistent: nothing persists on a first substep
+    ("2", "vertical", 1), ("2", "horizontal", 0),    # cone: only a support-oriented pair merges
+    ("4", "vertical", 1)])
+def test_multilevel_eligibility_gates(monkeypatch, gates, layout, levels):
+    if gates is None:
+        monkeypatch.delenv("PECLET_DEM_ML_GATES", raising=False)
+    else:
+        monkeypatch.setenv("PECLET_DEM_ML_GATES", gates)
+    s, _ = _ml_pair(layout)
+    s.step(1e-3)
+    st = s.get_stats()
+    assert st["stab_ran"] == 1 and st["ml_levels"] == levels
+
+
+def test_multilevel_ballistic_pair_never_merges(monkeypatch):
+    """A pair approaching faster than 8 g dt is ballistic: it is not aggregated (the floor row
+    of the resting lower sphere stays quasi-static)."""
+    monkeypatch.delenv("PECLET_DEM_ML_GATES", raising=False)
+    s, _ = _ml_pair("vertical", u=0.0, u_upper=0.2)  # the pair approaches at 20 g dt
+    s.step(1e-3)
+    st = s.get_stats()
+    assert st["stab_ran"] == 1 and st["ml_levels"] == 0
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_multilevel_conserves_momentum_without_walls(monkeypatch, seed):
+    """Every coarse impulse acts on two aggregates with opposite signs, the restriction is
+    mass-weighted and the prolongation adds one velocity change to every member, so in a
+    wall-free periodic cluster the substep changes the total momentum by exactly the gravity
+    impulse, while the hierarchy is non-trivial."""
+    monkeypatch.delenv("PECLET_DEM_ML_GATES", raising=False)
+    rng = np.random.default_rng(seed)
+    dt, g, side = 1e-3, 10.0, 4
+    n = side ** 3
+    s = oracle_sim(n)
+    s.set_sphere_shape(0.5)
+    box = side * 0.99
+    s.set_domain((0, 0, 0), (box, box, box))
+    s.enable_periodicity(True, True, True)
+    idx = np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    s.set_positions((idx + 0.5) * 0.99 + rng.uniform(-0.004, 0.004, (n, 3)))
+    inv_m = rng.uniform(0.5, 2.0, n)
+    s.set_inv_mass(inv_m)
+    s.set_inv_inertia(np.full((n, 3), 10.0))
+    v = rng.normal(0.0, 3.0 * dt * g, (n, 3))
+    s.set_velocities(v)
+    s.set_gravity(0, 0, -g)
+    s.set_material_params(0.0, 0.0, 0.5)
+    s.set_solver_iterations(4, 2)
+    s.set_stabilization_mode("multilevel")
+    m = 1.0 / inv_m
+    p0 = (v * m[:, None]).sum(0)
+    s.step(dt)
+    st = s.get_stats()
+    assert st["stab_ran"] == 1 and st["ml_levels"] >= 2
+    p1 = (s.get_velocities() * m[:, None]).sum(0)
+    drift = p1 - p0 - np.array([0.0, 0.0, -g * dt]) * m.sum()
+    assert np.abs(drift).max() <= 1e-13 * np.abs(v * m[:, None]).sum()
