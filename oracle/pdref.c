@@ -27,8 +27,10 @@
  * closed forms and independent algorithms; the Jacobi solves by binary restitution, symmetric
  * supports and the geometric convergence of the position average; the escalate and ordered
  * loops, warm start, persistence, grounded levels and the one-sided side flags by a
- * one-dimensional column model over two substeps; the Hertz shape paths by the SDF-sphere
- * restitution and cube stick/slide closed forms.  No built function is left unpinned.
+ * one-dimensional column model over two substeps; the multilevel stabilization by a two-sphere
+ * closed form, its eligibility gates and momentum conservation in a wall-free cluster; the Hertz
+ * shape paths by the SDF-sphere restitution and cube stick/slide closed forms.  No built
+ * function is left unpinned.
  */
 #include "pdref.h"
 
