@@ -254,7 +254,9 @@ int pd_set_timing(pd_sim* s, int level);
  * in global memory) when its rows touch at most "velocity_loop_lds_keys" distinct bodies
  * (default and maximum 12288), 0 keeps them all in global memory; "vsweep_waves" = 3 (default)
  * or 4 selects the per-colour PGS sweep compiled for that many waves per SIMD (4: at most 128
- * VGPRs, with some register spilling).  Results do not depend on any of them. */
+ * VGPRs, with some register spilling); "velocity_pack_bodies" = 1 (default) lets the per-colour
+ * velocity launches read each body's velocity and constants from one 128-byte record packed for
+ * the stage, 0 from the separate arrays.  Results do not depend on any of them. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 /* Autotuner (no reference counterpart): chooses candidate_skin_permille, block_solver_rows,
  * velocity_loop_lds and vsweep_waves for this simulation by timing its own substeps (coordinate

@@ -1787,12 +1787,14 @@ __device__ __forceinline__ int ledgerFind(const unsigned long long* keys, const 
 
 // velocity rows: gather the manifold (colour-sorted), warm start, persistence, vn0 / vt0
 // (gatherWarmLambdaKokkos :113, markPersistentManifoldsKokkos :78, computeVn0Kokkos :390)
-__global__ __launch_bounds__(TPB) void k_vrows(const Dev* __restrict__ Dp, Params P, int numRows, int ledgerBuckets,
-                                               int usePGS) {
+// One thread per manifold, in manifold order: the manifold sums and the bodies are read
+// coalesced (manifolds follow the pair order) and the row is written at the manifold's row.
+__global__ __launch_bounds__(TPB) void k_vrows(const Dev* __restrict__ Dp, Params P, int numManifolds,
+                                               int ledgerBuckets, int usePGS) {
   const Dev& D = *Dp;
-  const int k = gtid();
-  if (k >= numRows) return;
-  const int m = D.vPerm[k];
+  const int m = gtid();
+  if (m >= numManifolds || D.mColor[m] < 0) return;
+  const int k = D.vRowOf[m];
   const int2 r = D.mReal[m];
   const int realA = r.x, realB = r.y;
   const int np = D.mNp[m];
@@ -1964,7 +1966,60 @@ struct GBodies {
   }
   __device__ Body64 load(int real, int) const { return vw[real]; }
   __device__ void store(int real, int, const Body64& b) const { vw[real] = b; }
+  __device__ BodyC cload(const Dev& D, int real) const { return D.bc[real]; }
 };
+// One 128-byte record per body for the velocity stage of large beds: velocity, angular
+// velocity, orientation and inverse mass / inertia in one cache line, so a row's body gather
+// fetches two whole lines instead of four half-used ones (the two 64-byte records of the
+// separate arrays).  Packed from vw / bc before the stage and unpacked after it.
+struct alignas(128) VBody {
+  double4 v, w, q, iim;
+};
+struct PBodies {
+  VBody* vb;
+  __device__ void row(const Dev& D, int k, int2& r, int2& sl) const {
+    r = D.vAB[k];
+    sl = make_int2(0, 0);
+  }
+  __device__ Body64 load(int real, int) const {
+    Body64 b;
+    b.v = vb[real].v;
+    b.w = vb[real].w;
+    return b;
+  }
+  __device__ void store(int real, int, const Body64& b) const {
+    vb[real].v = b.v;
+    vb[real].w = b.w;
+  }
+  __device__ BodyC cload(const Dev&, int real) const {
+    BodyC c;
+    c.q = vb[real].q;
+    c.iim = vb[real].iim;
+    return c;
+  }
+};
+__global__ void k_vb_pack(const Dev* __restrict__ Dp, VBody* vb, int n) {
+  const Dev& D = *Dp;
+  const int i = gtid();
+  if (i >= n) return;
+  const Body64 b = D.vw[i];
+  const BodyC c = D.bc[i];
+  VBody o;
+  o.v = b.v;
+  o.w = b.w;
+  o.q = c.q;
+  o.iim = c.iim;
+  vb[i] = o;
+}
+__global__ void k_vb_unpack(const Dev* __restrict__ Dp, const VBody* vb, int n) {
+  const Dev& D = *Dp;
+  const int i = gtid();
+  if (i >= n) return;
+  Body64 b;
+  b.v = vb[i].v;
+  b.w = vb[i].w;
+  D.vw[i] = b;
+}
 // LDS slot codes of a row's bodies: a body without an LDS slot stays in global memory
 constexpr int VL_GLOBAL = 0xFFFE, VL_NONE = 0xFFFF;
 struct LBodies {
@@ -2001,11 +2056,12 @@ struct LBodies {
     bv[3 * s + 1] = make_double2(o.v.z, o.w.x);
     bv[3 * s + 2] = make_double2(o.w.y, o.w.z);
   }
+  __device__ BodyC cload(const Dev& D, int real) const { return D.bc[real]; }
 };
 
 // warmStartApplyKokkos, src/solver_velocity.hpp:498-605 (one row)
 template <class BS>
-__device__ void warmApplyRowB(const Dev& D, int k, int2 r, int2 sl, const BS& bs) {
+__device__ __forceinline__ void warmApplyRowB(const Dev& D, int k, int2 r, int2 sl, const BS& bs) {
   const double p = D.vLam[k];
   const double4 lt4 = D.vLamT[k];
   d3 lt = xyz(lt4);
@@ -2019,7 +2075,7 @@ __device__ void warmApplyRowB(const Dev& D, int k, int2 r, int2 sl, const BS& bs
   const double sgn = rb4.w;
   const double lambda = -sgn * p;
   const d3 Jlin = scale3(Nsum, lambda);
-  const BodyC ca = D.bc[realA];
+  const BodyC ca = bs.cload(D, realA);
   const double invMassA = ca.iim.w;
   Body64 ba = bs.load(realA, sl.x);
   const q4 qA = toq(ca.q);
@@ -2037,7 +2093,7 @@ __device__ void warmApplyRowB(const Dev& D, int k, int2 r, int2 sl, const BS& bs
   BodyC cb;
   double invMassB = 0.0;
   if (realB >= 0) {
-    cb = D.bc[realB];
+    cb = bs.cload(D, realB);
     invMassB = cb.iim.w;
     bb = bs.load(realB, sl.y);
     bb.v.x += -Jlin.x * invMassB;
@@ -2077,7 +2133,7 @@ __device__ void warmApplyRowB(const Dev& D, int k, int2 r, int2 sl, const BS& bs
   bs.store(realA, sl.x, ba);
   if (realB >= 0) bs.store(realB, sl.y, bb);
 }
-__device__ void warmApplyRow(const Dev& D, int k) {
+__device__ __forceinline__ void warmApplyRow(const Dev& D, int k) {
   const GBodies g{D.vw};
   int2 r, sl;
   g.row(D, k, r, sl);
@@ -2086,11 +2142,11 @@ __device__ void warmApplyRow(const Dev& D, int k) {
 
 // PGSManifoldSweep::solveOne, src/solver_velocity.hpp:641-858 (one row)
 template <class BS>
-__device__ double pgsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
+__device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
                           double restVelThreshold, const BS& bs) {
   double amax = 0.0;
   const int realA = r.x, realB = r.y;
-  const BodyC ca = D.bc[realA];
+  const BodyC ca = bs.cload(D, realA);
   const double invMassA = ca.iim.w;
   const d3 invIA = xyz(ca.iim);
   const q4 qA = toq(ca.q);
@@ -2104,7 +2160,7 @@ __device__ double pgsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
   const double4 tb4 = D.vTB[k];
   const double invN = tb4.w;
   if (realB >= 0) {
-    cb = D.bc[realB];
+    cb = bs.cload(D, realB);
     invMassB = cb.iim.w;
     invIB = xyz(cb.iim);
     qB = toq(cb.q);
@@ -2253,7 +2309,7 @@ __device__ double pgsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
   if (dirtyB) bs.store(realB, sl.y, bb);
   return amax;
 }
-__device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
+__device__ __forceinline__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
   const GBodies g{D.vw};
   int2 r, sl;
   g.row(D, k, r, sl);
@@ -2262,10 +2318,10 @@ __device__ double pgsRow(const Dev& D, const Params& P, int k, double restVelThr
 
 // solveVelocityColoredGSKokkos one-shot restitution (g = 0), src/solver_velocity.hpp:989-1133
 template <class BS>
-__device__ double gsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
+__device__ __forceinline__ double gsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
                          double restVelThreshold, const BS& bs) {
   const int realA = r.x, realB = r.y;
-  const BodyC ca = D.bc[realA];
+  const BodyC ca = bs.cload(D, realA);
   const double invMassA = ca.iim.w;
   const d3 invIA = xyz(ca.iim);
   const q4 qA = toq(ca.q);
@@ -2279,7 +2335,7 @@ __device__ double gsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
   const double4 tb4 = D.vTB[k];
   const double invN = tb4.w;
   if (realB >= 0) {
-    cb = D.bc[realB];
+    cb = bs.cload(D, realB);
     invMassB = cb.iim.w;
     invIB = xyz(cb.iim);
     qB = toq(cb.q);
@@ -2335,12 +2391,44 @@ __device__ double gsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
   }
   return amax;
 }
-__device__ double gsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
+__device__ __forceinline__ double gsRow(const Dev& D, const Params& P, int k, double restVelThreshold) {
   const GBodies g{D.vw};
   int2 r, sl;
   g.row(D, k, r, sl);
   return gsRowB(D, P, k, r, sl, restVelThreshold, g);
 }
+// the same rows on packed body records: RowOf<false> reads vw / bc, RowOf<true> one VBody each
+template <bool PK>
+struct RowOf {
+  __device__ __forceinline__ static void warm(const Dev& D, int k, VBody*) { warmApplyRow(D, k); }
+  __device__ __forceinline__ static double pgs(const Dev& D, const Params& P, int k, double thr, VBody*) {
+    return pgsRow(D, P, k, thr);
+  }
+  __device__ __forceinline__ static double gs(const Dev& D, const Params& P, int k, double thr, VBody*) {
+    return gsRow(D, P, k, thr);
+  }
+};
+template <>
+struct RowOf<true> {
+  __device__ __forceinline__ static void warm(const Dev& D, int k, VBody* vb) {
+    const PBodies p{vb};
+    int2 r, sl;
+    p.row(D, k, r, sl);
+    warmApplyRowB(D, k, r, sl, p);
+  }
+  __device__ static double pgs(const Dev& D, const Params& P, int k, double thr, VBody* vb) {
+    const PBodies p{vb};
+    int2 r, sl;
+    p.row(D, k, r, sl);
+    return pgsRowB(D, P, k, r, sl, thr, p);
+  }
+  __device__ static double gs(const Dev& D, const Params& P, int k, double thr, VBody* vb) {
+    const PBodies p{vb};
+    int2 r, sl;
+    p.row(D, k, r, sl);
+    return gsRowB(D, P, k, r, sl, thr, p);
+  }
+};
 
 // computeSideFlagsKokkos, src/solver_velocity.hpp:456-495 (rows are active, non-duplicate)
 template <class BS>
@@ -2359,12 +2447,16 @@ __device__ __forceinline__ void sideFlagsRow(const Dev& D, int k, d3 gHat, doubl
   D.vFlag[k] = fl;
 }
 // `need` (optional) gates the launch like `run`
+// vb (optional): the stage's packed body records
 __global__ void k_side_flags(const Dev* __restrict__ Dp, int numRows, d3 gHat, double riseThr, double approachThr,
-                             const int* run, const int* need) {
+                             const int* run, const int* need, VBody* vb = nullptr) {
   const Dev& D = *Dp;
   const int k = gtid();
   if (k >= numRows || !*run || (need && !*need)) return;
-  sideFlagsRow(D, k, gHat, riseThr, approachThr, GBodies{D.vw});
+  if (vb)
+    sideFlagsRow(D, k, gHat, riseThr, approachThr, PBodies{vb});
+  else
+    sideFlagsRow(D, k, gHat, riseThr, approachThr, GBodies{D.vw});
 }
 
 // mode: 0 = warm start, 1 = PGS sweep, 2 = one-shot GS sweep
@@ -2391,46 +2483,54 @@ __global__ void k_stab_gate(const unsigned long long* bits, int* ctl, int mode, 
   ctl[CTL_STOP_S] = run ? 0 : 1;
 }
 
-template <int MODE>
+// PK: bodies in the packed records vb (else vw / bc)
+template <int MODE, bool PK = false>
 __global__ __launch_bounds__(TPB) void k_vsweep(const Dev* __restrict__ Dp, Params P, int rowBegin, int rowEnd,
-                                                double thr, const int* stop) {
+                                                double thr, const int* stop, VBody* vb = nullptr) {
   const Dev& D = *Dp;
   if (*stop) return;
   __shared__ unsigned long long sh[WAVES];
   const int k = rowBegin + gtid();
   double a = 0.0;
   if (k < rowEnd) {
-    if (MODE == 0) warmApplyRow(D, k);
-    else if (MODE == 1) a = pgsRow(D, P, k, thr);
-    else a = gsRow(D, P, k, thr);
+    if (MODE == 0) RowOf<PK>::warm(D, k, vb);
+    else if (MODE == 1) a = RowOf<PK>::pgs(D, P, k, thr, vb);
+    else a = RowOf<PK>::gs(D, P, k, thr, vb);
   }
   if (MODE != 0) blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
 }
 // the PGS sweep compiled for at least 4 waves per SIMD (at most 128 VGPRs instead of ~153):
 // more rows in flight per CU to hide the body gathers' latency, against register pressure; the
 // autotuner picks between the two
+template <bool PK = false>
 __global__ __launch_bounds__(TPB, 4) void k_vsweep_o4(const Dev* __restrict__ Dp, Params P,
                                                       int rowBegin, int rowEnd, double thr,
-                                                      const int* stop) {
+                                                      const int* stop, VBody* vb = nullptr) {
   const Dev& D = *Dp;
   if (*stop) return;
   __shared__ unsigned long long sh[WAVES];
   const int k = rowBegin + gtid();
   double a = 0.0;
-  if (k < rowEnd) a = pgsRow(D, P, k, thr);
+  if (k < rowEnd) a = RowOf<PK>::pgs(D, P, k, thr, vb);
   blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
 }
 // colour 62 may hold manifolds sharing a body (mask saturation): one thread, in row order (the
 // class's rows are placed in manifold order)
 __global__ __launch_bounds__(64) void k_vsweep_serial(const Dev* __restrict__ Dp, Params P, int rowBegin,
                                                       int rowEnd, int mode, double thr,
-                                                      const int* stop) {
+                                                      const int* stop, VBody* vb = nullptr) {
   const Dev& D = *Dp;
   if (gtid() != 0 || *stop) return;
   for (int k = rowBegin; k < rowEnd; ++k) {
-    if (mode == 0) warmApplyRow(D, k);
-    else if (mode == 1) atomicMaxD(D.maxApproachBits, pgsRow(D, P, k, thr));
-    else atomicMaxD(D.maxApproachBits, gsRow(D, P, k, thr));
+    if (vb) {
+      if (mode == 0) RowOf<true>::warm(D, k, vb);
+      else if (mode == 1) atomicMaxD(D.maxApproachBits, RowOf<true>::pgs(D, P, k, thr, vb));
+      else atomicMaxD(D.maxApproachBits, RowOf<true>::gs(D, P, k, thr, vb));
+    } else {
+      if (mode == 0) warmApplyRow(D, k);
+      else if (mode == 1) atomicMaxD(D.maxApproachBits, pgsRow(D, P, k, thr));
+      else atomicMaxD(D.maxApproachBits, gsRow(D, P, k, thr));
+    }
   }
 }
 
@@ -3190,13 +3290,14 @@ __global__ void k_jp_init_contacts(int* colour, int n, int* work) {
 // src/solver_position.hpp:276-312).  Row: pN = (n, wTotal), pRA = (rA, invMassA),
 // pRB = (rB or plane point, invMassB).
 // zeroBits (optional): the position loop's residual, cleared before the loop's first sweep
-__global__ __launch_bounds__(TPB) void k_prows(const Dev* __restrict__ Dp, int numRows,
+// One thread per contact, in contact order (coalesced reads), written at the contact's row.
+__global__ __launch_bounds__(TPB) void k_prows(const Dev* __restrict__ Dp, int numContacts,
                                                unsigned long long* zeroBits = nullptr) {
   const Dev& D = *Dp;
-  const int k = gtid();
-  if (zeroBits && k == 0) *zeroBits = 0ull;
-  if (k >= numRows) return;
-  const int c = D.pPerm[k];
+  const int c = gtid();
+  if (zeroBits && c == 0) *zeroBits = 0ull;
+  if (c >= numContacts || D.cColor[c] < 0) return;
+  const int k = D.pRowOf[c];
   const int2 ab = D.cAB[c];
   const int idA = ab.x, idB = ab.y;
   const double4 n4 = D.cN[c], ra4 = D.cRA[c], rb4 = D.cRB[c];

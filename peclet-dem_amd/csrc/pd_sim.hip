@@ -1419,6 +1419,10 @@ struct pd_sim {
   // stabilization mode 2 ("multilevel"), src/sim.hpp:303-355 with src/solver_multilevel.hpp:
   // the hierarchy is built once, then each iteration runs one fine sweep (tracking the
   // quasi-static residual) and the coarse cycle, until that residual is at most vRestS.
+  // packed body records of the velocity stage (large beds), active between pack and unpack
+  DBuf<VBody> vbody;
+  VBody* vbActive = nullptr;
+  bool packBodies = true;
   DBuf<unsigned char> mlElig, mlSlot;
   DBuf<int> mlGrp, mlMate, mlLead, mlScan, mlParent, mlCol, mlPerm, mlRowOfTmp, mlCnt;
   DBuf<int> mlBlockCnt, mlColCnt, mlColStart, mlClsCnt;
@@ -1625,20 +1629,31 @@ struct pd_sim {
     for (int c = 0; c < numColors; ++c) {
       const int b = vs[c], e = vs[c + 1];
       if (e <= b) continue;
+      VBody* vb = vbActive;
       if (c == 62) {
-        k_vsweep_serial<<<1, 1, 0, st>>>(dD, P, b, e, mode, thr, stop);
+        k_vsweep_serial<<<1, 1, 0, st>>>(dD, P, b, e, mode, thr, stop, vb);
       } else {
         if (mode == 0) {
-          k_vsweep<0><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
+          if (vb)
+            k_vsweep<0, true><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
+          else
+            k_vsweep<0><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
         } else if (mode == 1) {
           kernBegin();
-          if (vsweepWaves4)
+          if (vsweepWaves4 && vb)
+            k_vsweep_o4<true><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
+          else if (vsweepWaves4)
             k_vsweep_o4<<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
+          else if (vb)
+            k_vsweep<1, true><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
           else
             k_vsweep<1><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
           tagSweepRec(kernEnd(K_VSWEEP, e - b));
         } else {
-          k_vsweep<2><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
+          if (vb)
+            k_vsweep<2, true><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
+          else
+            k_vsweep<2><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
         }
       }
     }
@@ -1675,6 +1690,7 @@ struct pd_sim {
   static constexpr int POS_TAIL_ROWS = 8192;
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
+    vbActive = nullptr;  // a substep cut short by an error may have left the stage's records
     vBlockRec[0] = vBlockRec[1] = pBlockRec = -1;
     sweepRecs.clear();
     curSweepTag = -1;
@@ -1876,8 +1892,16 @@ struct pd_sim {
     refreshDev();
 
     if (nRowsV > 0)
-      k_vrows<<<nblk(nRowsV), TPB, 0, st>>>(dD, P, nRowsV, prevCount > 0 ? ledgerBuckets : 0,
-                                            usePGS ? 1 : 0);
+      k_vrows<<<nblk(nm), TPB, 0, st>>>(dD, P, nm, prevCount > 0 ? ledgerBuckets : 0,
+                                        usePGS ? 1 : 0);
+    // the per-colour launches of a large bed read each row's bodies from one 128-byte record
+    // per body, packed here and unpacked when the stage ends
+    if (packBodies && !hostLoop && !fusedV && !smallV && nRowsV > 0 && numReal > 0 &&
+        (usePGS || velIters > 0)) {
+      vbody.ensure(numReal, st);
+      k_vb_pack<<<nblk(numReal), TPB, 0, st>>>(dD, vbody.p, numReal);
+      vbActive = vbody.p;
+    }
     if (usePGS) {
       // grounded levels, double-buffered (src/contact_preprocessing.hpp:213-253); the decay ran
       // in the predict launch
@@ -1986,7 +2010,7 @@ struct pd_sim {
           if (nRowsV > 0)
             k_side_flags<<<nblk(nRowsV), TPB, 0, st>>>(dD, nRowsV, gHat, 8.0 * dt * gMagP,
                                                        8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN,
-                                                       nullptr);
+                                                       nullptr, vbActive);
           sweepLoop(2 * velIters, ctl.p + CTL_STOP_S, [&, ss = 0]() mutable {
             k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S);
             curSweepTag = 4096 + ss++;
@@ -1996,6 +2020,10 @@ struct pd_sim {
                                            vRestS, 0);
           });
         }
+      }
+      if (vbActive) {
+        k_vb_unpack<<<nblk(numReal), TPB, 0, st>>>(dD, vbody.p, numReal);
+        vbActive = nullptr;
       }
       HIPCHECK(hipGetLastError());
     } else {
@@ -2137,7 +2165,7 @@ struct pd_sim {
     // with the batched position loop ahead, the rows launch also clears its residual
     const bool posBatches = useGS && posLeftover == 0 && nRowsP > blockRows;
     if (nRowsP > 0)
-      k_prows<<<nblk(nRowsP), TPB, 0, st>>>(dD, nRowsP, posBatches ? bits.p : nullptr);
+      k_prows<<<nblk(nc), TPB, 0, st>>>(dD, nc, posBatches ? bits.p : nullptr);
     mark(ST_PSWEEP);
     const double posTol = 1e-4 * baseRadius * globalScale;
     const int* finalCtl = nullptr;
@@ -3574,6 +3602,9 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     } else if (k == "vsweep_waves") {
       if (value != 3 && value != 4) throw ArgError("vsweep_waves must be 3 or 4");
       s->vsweepWaves4 = value == 4;
+    } else if (k == "velocity_pack_bodies") {
+      if (value != 0 && value != 1) throw ArgError("velocity_pack_bodies must be 0 or 1");
+      s->packBodies = value == 1;
     } else if (k == "timing_every") {
       if (value < 1) throw ArgError("timing_every must be >= 1");
       s->timingEvery = (int)std::min<long>(value, 1L << 30);

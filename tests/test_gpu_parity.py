@@ -135,6 +135,15 @@ def test_velocity_loop_body_storage(hip_lib, kind, tuning):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["bed", "bed_periodic", "gas"])
+@pytest.mark.parametrize("pack", [0, 1])
+def test_per_colour_launches_body_records(hip_lib, kind, pack):
+    # one launch per colour and sweep (the large-bed path), with each body's velocity and
+    # constants packed into one record for the stage (1) or read from the separate arrays (0)
+    run_pair(kind, 400, 15, tuning={"block_solver_rows": 0, "velocity_pack_bodies": pack})
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n,steps", [(2500, 4), (4000, 3)])
 def test_velocity_stage_large_beds(hip_lib, n, steps):
     # 2500 spheres: ~4500-6700 rows over ~2900 bodies, so the staged stage keeps 2400 bodies in
