@@ -171,6 +171,15 @@ struct Dev {
 constexpr unsigned char F_PERSIST = 1, F_SIDE1 = 2, F_SIDE2 = 4, F_UP = 8, F_DOWN = 16;
 
 __device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup i to XCD i mod 8), each XCD with
+// its own L2.  Launched over a grid that is a multiple of 8 (xcdGrid), this maps the workgroups
+// so that XCD x takes the x-th contiguous eighth of the range: rows placed in edge order then
+// keep their bodies in one XCD's L2 across the colour launches of a sweep.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcdTid() {
+  const int per = gridDim.x / NUM_XCD;
+  return ((blockIdx.x % NUM_XCD) * per + blockIdx.x / NUM_XCD) * blockDim.x + threadIdx.x;
+}
 
 // ---- wave64 / block reductions: one global atomic per block instead of one per thread ----
 // block-aggregated append of `item` (when `take`) to list[*count]; every thread must call it
@@ -1624,15 +1633,18 @@ __global__ void k_jp_counts(int* cnt, const int* src, int first, int n) {
 }
 // Colour classes: pass 1 counts per (block, colour) with LDS atomics; k_color_offsets turns the
 // counts into per-block bases; pass 2 places every edge at base + its rank inside the block.  Both
-// passes walk the same grid-stride chunks, so the per-block counts match.  The order inside a
-// class is free (the class is body-disjoint).
+// passes walk the same contiguous chunk per block, so the per-block counts match and a class's
+// rows follow the edge order up to the shuffle inside one block's pass over 256 edges.  The order
+// inside a class is otherwise free (the class is body-disjoint).
 constexpr int CP_BLOCKS = 1024;
 // blockCnt is colour-major: blockCnt[c * CP_BLOCKS + b]
 __global__ void k_color_count(const int* colour, int numEdges, int* blockCnt) {
   __shared__ int cnt[64];
   if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
   __syncthreads();
-  for (int e = gtid(); e < numEdges; e += gridDim.x * blockDim.x) {
+  const int chunk = (numEdges + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * chunk, hi = min(numEdges, lo + chunk);
+  for (int e = lo + (int)threadIdx.x; e < hi; e += blockDim.x) {
     const int c = colour[e];
     if (c >= 0) atomicAdd(&cnt[c], 1);
   }
@@ -1739,7 +1751,9 @@ __global__ void k_color_place(const int* colour, int numEdges, const int* blockB
     base[threadIdx.x] = colStart[threadIdx.x] + blockBase[threadIdx.x * CP_BLOCKS + blockIdx.x];
   }
   __syncthreads();
-  for (int e = gtid(); e < numEdges; e += gridDim.x * blockDim.x) {
+  const int chunk = (numEdges + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * chunk, hi = min(numEdges, lo + chunk);
+  for (int e = lo + (int)threadIdx.x; e < hi; e += blockDim.x) {
     const int c = colour[e];
     if (c >= 0) {
       const int k = base[c] + atomicAdd(&cnt[c], 1);
@@ -2490,7 +2504,7 @@ __global__ __launch_bounds__(TPB) void k_vsweep(const Dev* __restrict__ Dp, Para
   const Dev& D = *Dp;
   if (*stop) return;
   __shared__ unsigned long long sh[WAVES];
-  const int k = rowBegin + gtid();
+  const int k = rowBegin + xcdTid();
   double a = 0.0;
   if (k < rowEnd) {
     if (MODE == 0) RowOf<PK>::warm(D, k, vb);
@@ -2509,7 +2523,7 @@ __global__ __launch_bounds__(TPB, 4) void k_vsweep_o4(const Dev* __restrict__ Dp
   const Dev& D = *Dp;
   if (*stop) return;
   __shared__ unsigned long long sh[WAVES];
-  const int k = rowBegin + gtid();
+  const int k = rowBegin + xcdTid();
   double a = 0.0;
   if (k < rowEnd) a = RowOf<PK>::pgs(D, P, k, thr, vb);
   blockAtomicMaxU64(D.maxApproachBits, dbits(a), sh);
@@ -3364,7 +3378,7 @@ __global__ __launch_bounds__(TPB) void k_psweep(const Dev* __restrict__ Dp, int 
   const Dev& D = *Dp;
   if (*stop) return;
   __shared__ unsigned long long sh[WAVES];
-  const int k = rowBegin + gtid();
+  const int k = rowBegin + xcdTid();
   const double o = (k < rowEnd) ? posRow(D, k) : 0.0;
   blockAtomicMaxU64(D.maxOverlapBits, dbits(o), sh);
 }

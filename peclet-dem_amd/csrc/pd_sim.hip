@@ -48,6 +48,8 @@ struct ArgError : std::runtime_error {
 
 // every kernel bounds-checks its index, so an empty range still launches one block
 inline int nblk(long n) { return n > 0 ? (int)((n + TPB - 1) / TPB) : 1; }
+// grid of a launch mapped with xcdTid: a multiple of the 8 XCDs
+inline int xcdGrid(long n) { return (nblk(n) + NUM_XCD - 1) / NUM_XCD * NUM_XCD; }
 
 // Device allocation is serialised process-wide, and a buffer replaced by growth is not freed at
 // once: it is retired and freed at a point where no stream can still read it (after a device
@@ -1635,25 +1637,25 @@ struct pd_sim {
       } else {
         if (mode == 0) {
           if (vb)
-            k_vsweep<0, true><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
+            k_vsweep<0, true><<<xcdGrid(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
           else
-            k_vsweep<0><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
+            k_vsweep<0><<<xcdGrid(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
         } else if (mode == 1) {
           kernBegin();
           if (vsweepWaves4 && vb)
-            k_vsweep_o4<true><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
+            k_vsweep_o4<true><<<xcdGrid(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
           else if (vsweepWaves4)
-            k_vsweep_o4<<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
+            k_vsweep_o4<<<xcdGrid(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
           else if (vb)
-            k_vsweep<1, true><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
+            k_vsweep<1, true><<<xcdGrid(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
           else
-            k_vsweep<1><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
+            k_vsweep<1><<<xcdGrid(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
           tagSweepRec(kernEnd(K_VSWEEP, e - b));
         } else {
           if (vb)
-            k_vsweep<2, true><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
+            k_vsweep<2, true><<<xcdGrid(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop, vb);
           else
-            k_vsweep<2><<<nblk(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
+            k_vsweep<2><<<xcdGrid(e - b), TPB, 0, st>>>(dD, P, b, e, thr, stop);
         }
       }
     }
@@ -2208,7 +2210,7 @@ struct pd_sim {
               k_psweep_serial<<<1, 1, 0, st>>>(dD, cb, ce, ctl.p + CTL_STOP_P);
             } else {
               kernBegin();
-              k_psweep<<<nblk(ce - cb), TPB, 0, st>>>(dD, cb, ce, ctl.p + CTL_STOP_P);
+              k_psweep<<<xcdGrid(ce - cb), TPB, 0, st>>>(dD, cb, ce, ctl.p + CTL_STOP_P);
               if (timing >= 2) {
                 curSweepTag = 2 * 4096 + done + j;
                 tagSweepRec(kernEnd(K_PSWEEP, ce - cb));
@@ -2243,7 +2245,7 @@ struct pd_sim {
           if (c == 62)
             k_psweep_serial<<<1, 1, 0, st>>>(dD, b, e, ctl.p + CTL_NONE);
           else
-            k_psweep<<<nblk(e - b), TPB, 0, st>>>(dD, b, e, ctl.p + CTL_NONE);
+            k_psweep<<<xcdGrid(e - b), TPB, 0, st>>>(dD, b, e, ctl.p + CTL_NONE);
         }
         jacobiPositionPass(1);
         stats.sweeps_pos++;
