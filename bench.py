@@ -487,6 +487,8 @@ def main():
     ap.add_argument("--only", choices=["headline", "pour", "cylinders", "hertz", "batched"],
                     default=None, help="run one leg and print its JSON (profiling)")
     ap.add_argument("--adjoint-n", type=int, default=50000)
+    ap.add_argument("--adjoint-case", choices=["both", "loose_start", "dense_bed"], default="both",
+                    help="with --adjoint-only: which start to run")
     ap.add_argument("--adjoint-substeps", type=int, default=16)
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=24)
@@ -499,7 +501,8 @@ def main():
         sys.exit(2)
     if args.adjoint_only:
         print(json.dumps({k: run_adjoint(args.adjoint_n, args.adjoint_substeps, 8, 8, d)
-                          for k, d in (("loose_start", False), ("dense_bed", True))}))
+                          for k, d in (("loose_start", False), ("dense_bed", True))
+                          if args.adjoint_case in ("both", k)}))
         return
     if args.n is None:
         args.n = 200000 if args.workload == "cylinders" else 1048576

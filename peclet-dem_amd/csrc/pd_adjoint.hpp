@@ -631,6 +631,307 @@ struct ElPgs {
       st3(out + 9, wB);
     }
   }
+  // Reverse of operator() for T = double: xb = J^T yb with J the Jacobian the forward-mode
+  // evaluation defines (same branches, the same always-applied updates and the same sticking
+  // tangent), assembled in one backward pass instead of NI forward-mode columns.  M_A, M_B are
+  // the bodies' world inverse inertia (angDelta is M J, genInvMass tau^T M tau).
+  PD_HD void vjpRev(const double (&x)[NI], const double (&yb)[NO], double (&xb)[NI]) const {
+    using pd::add3;
+    using pd::cross3;
+    using pd::dot3;
+    using pd::scale3;
+    using pd::sub3;
+    for (int i = 0; i < NI; ++i) xb[i] = 0.0;
+    for (int i = 0; i < NO; ++i) xb[i] = yb[i];  // the identity part (unchanged outputs)
+    const d3 N{x[16], x[17], x[18]}, TA{x[19], x[20], x[21]}, TB{x[22], x[23], x[24]};
+    const d3 rA{x[25], x[26], x[27]}, rB{x[28], x[29], x[30]};
+    const double Nsq = dot3(N, N);
+    const double lenN = sqrt(Nsq);
+    if (lenN < 1e-9) return;
+    const double invLenN = lenN > 0.0 ? 1.0 / lenN : 0.0;
+    const d3 dc = B.hasB ? sub3(rA, rB) : rA;
+    const double sgn = dot3(N, dc) > 0.0 ? 1.0 : -1.0;
+    const d3 vA{x[0], x[1], x[2]}, wA{x[3], x[4], x[5]};
+    const d3 vB = B.hasB ? d3{x[6], x[7], x[8]} : B.vWall;
+    const d3 wB = B.hasB ? d3{x[9], x[10], x[11]} : d3{0.0, 0.0, 0.0};
+    const double wAn = Nsq * B.invMassA + pd::genInvMass(TA, B.invIA, B.qA);
+    const double wBn = Nsq * B.invMassB + pd::genInvMass(TB, B.invIB, B.qB);
+    const bool applyA = sf != 2, applyB = sf != 1;
+    const double wTotal = (applyA ? wAn : 0.0) + (applyB ? wBn : 0.0);
+    const double growth = dot3(scale3(dc, growthRate), N);
+    const double vn = dot3(vA, N) + dot3(wA, TA) + dot3(vB, pd::neg3(N)) + dot3(wB, TB) + growth;
+    if (!(wTotal > 0.0)) return;
+    const double vn0 = x[31];
+    const bool slowN = fabs(vn0) < restVelThreshold * lenN;
+    const bool useE = sf == 0 && !slowN;
+    const double e = x[37];
+    const double v0til = sgn * vn0;
+    const double target = v0til > 0.0 ? -(useE ? e : 0.0) * v0til : 0.0;
+    const double dp = (sgn * vn - target) / wTotal;
+    const double pOld = x[12];
+    const double pNew0 = pOld + dp;
+    const bool clamped = pNew0 < 0.0;
+    const double pNew = clamped ? 0.0 : pNew0;
+    const double dApplied = pNew - pOld;
+    const bool applied = dApplied != 0.0 || !clamped;
+    const double lamNow = applied ? pNew : pOld;
+    const double lambda = -sgn * dApplied;
+    d3 vA1 = vA, wA1 = wA, vB1 = vB, wB1 = wB;
+    if (applied) {
+      const d3 Jlin = scale3(N, lambda);
+      if (applyA) {
+        vA1 = add3(vA, scale3(Jlin, B.invMassA));
+        wA1 = add3(wA, pd::angDelta(B.qA, scale3(TA, lambda), B.invIA));
+      }
+      if (B.hasB && applyB) {
+        vB1 = add3(vB, scale3(pd::neg3(Jlin), B.invMassB));
+        wB1 = add3(wB, pd::angDelta(B.qB, scale3(TB, lambda), B.invIB));
+      }
+    }
+    // adjoints of the stage values
+    d3 gvA1{yb[0], yb[1], yb[2]}, gwA1{yb[3], yb[4], yb[5]};
+    d3 gvB1{0, 0, 0}, gwB1{0, 0, 0};
+    if (B.hasB) {
+      gvB1 = d3{yb[6], yb[7], yb[8]};
+      gwB1 = d3{yb[9], yb[10], yb[11]};
+    }
+    d3 gN{0, 0, 0}, gTA{0, 0, 0}, gTB{0, 0, 0}, grA{0, 0, 0}, grB{0, 0, 0};
+    d3 gltOld{yb[13], yb[14], yb[15]}, gvt0{0, 0, 0};
+    double glamNow = 0.0, glenN = 0.0, ginvLenN = 0.0, gmu = 0.0, gpimp = 0.0, gvn0 = 0.0, ge = 0.0;
+    const double mu = x[36];
+    const d3 ltOld{x[13], x[14], x[15]};
+    const bool haveOld = ltOld.x != 0.0 || ltOld.y != 0.0 || ltOld.z != 0.0;
+    if (mu > 0.0 || haveOld) {
+      // ---- forward of the cone part
+      const d3 nhat = scale3(N, invLenN);
+      const d3 vrel = sub3(add3(vA1, cross3(wA1, rA)), add3(vB1, cross3(wB1, rB)));
+      const double vrn = dot3(vrel, nhat);
+      const d3 vt = sub3(vrel, scale3(nhat, vrn));
+      const bool betaOn = restT != 0.0 && sf == 0 && !slowN;
+      const d3 vt0{x[32], x[33], x[34]};
+      const double v0n = dot3(vt0, nhat);
+      d3 vtErr = vt;
+      if (betaOn) vtErr = add3(vt, scale3(sub3(vt0, scale3(nhat, v0n)), restT));
+      const double vtLen = sqrt(dot3(vtErr, vtErr));
+      // which update moved the impulse: 1 the sliding update, 2 the sticking tangent, 0 none
+      int upd = 0;
+      d3 that{0, 0, 0}, cA{0, 0, 0}, cB{0, 0, 0};
+      double wT = 0.0, c = 0.0, invLen = 0.0, wTc = 0.0;
+      d3 ltN0 = ltOld;  // value
+      if (vtLen > 1e-9) {
+        invLen = 1.0 / vtLen;
+        that = scale3(vtErr, invLen);
+        cA = cross3(rA, that);
+        cB = cross3(rB, that);
+        const double wAt = sf == 2 ? 0.0 : B.invMassA + pd::genInvMass(cA, B.invIA, B.qA);
+        const double wBt = (sf == 1 || !B.hasB) ? 0.0 : B.invMassB + pd::genInvMass(cB, B.invIB, B.qB);
+        wT = wAt + wBt;
+        if (wT > 1e-9) {
+          c = -vtLen / wT;
+          ltN0 = add3(ltOld, scale3(that, c));
+          upd = 1;
+        }
+      } else {
+        const d3 ax = fabs(nhat.x) < 0.9 ? d3{1.0, 0.0, 0.0} : d3{0.0, 1.0, 0.0};
+        d3 th = cross3(nhat, ax);
+        th = scale3(th, 1.0 / sqrt(dot3(th, th)));
+        const double wAt = sf == 2 ? 0.0 : B.invMassA + pd::genInvMass(cross3(rA, th), B.invIA, B.qA);
+        const double wBt =
+            (sf == 1 || !B.hasB) ? 0.0 : B.invMassB + pd::genInvMass(cross3(rB, th), B.invIB, B.qB);
+        wTc = wAt + wBt;
+        if (wTc > 1e-9) upd = 2;  // value unchanged, tangents of lt - vtErr / wT
+      }
+      const double l0n = dot3(ltN0, nhat);
+      const d3 ltN1 = sub3(ltN0, scale3(nhat, l0n));
+      const bool lamPos = lamNow >= 0.0;
+      const double tm = lamPos ? lamNow : 0.0;
+      double nTot = tm * lenN;
+      const bool pimpOn = slowN && x[35] >= 0.0;
+      if (slowN) nTot = nTot + (pimpOn ? x[35] : 0.0);
+      const double bound = mu * nTot;
+      const double ltLen = sqrt(dot3(ltN1, ltN1));
+      // cone: 0 inside, 1 scaled onto it, 2 cleared (zero bound)
+      const int cone = ltLen > bound ? (bound > 0.0 ? 1 : 2) : 0;
+      const double s = cone == 1 ? bound / ltLen : 1.0;
+      const d3 ltN2 = cone == 1 ? scale3(ltN1, s) : (cone == 2 ? d3{0, 0, 0} : ltN1);
+      const d3 dApp = sub3(ltN2, ltOld);
+      // ---- reverse of the cone part
+      d3 gdApp{0, 0, 0};
+      d3 gltN2{yb[13], yb[14], yb[15]};
+      gltOld = d3{0, 0, 0};
+      if (applyA) {
+        gdApp = add3(gdApp, scale3(gvA1, B.invMassA));
+        const d3 u = pd::angDelta(B.qA, gwA1, B.invIA);  // M_A is symmetric
+        grA = add3(grA, cross3(dApp, u));
+        gdApp = add3(gdApp, cross3(u, rA));
+      }
+      if (B.hasB && applyB) {
+        gdApp = sub3(gdApp, scale3(gvB1, B.invMassB));
+        const d3 u = pd::angDelta(B.qB, gwB1, B.invIB);
+        const d3 ndApp = pd::neg3(dApp);
+        grB = add3(grB, cross3(ndApp, u));
+        gdApp = sub3(gdApp, cross3(u, rB));
+      }
+      gltN2 = add3(gltN2, gdApp);
+      gltOld = sub3(gltOld, gdApp);
+      d3 gltN1{0, 0, 0};
+      double gbound = 0.0;
+      if (cone == 1) {
+        gltN1 = scale3(gltN2, s);
+        const double gs = dot3(gltN2, ltN1);
+        gbound += gs / ltLen;
+        const double gltLen = -gs * s / ltLen;
+        gltN1 = add3(gltN1, scale3(ltN1, gltLen / ltLen));
+      } else if (cone == 0) {
+        gltN1 = gltN2;
+      }
+      gmu += gbound * nTot;
+      const double gnTot = gbound * mu;
+      if (pimpOn) gpimp += gnTot;
+      glenN += gnTot * tm;
+      if (lamPos) glamNow += gnTot * lenN;
+      // ltN1 = ltN0 - nhat (ltN0 . nhat)
+      d3 gnhat{0, 0, 0};
+      const double g1n = dot3(gltN1, nhat);
+      d3 gltN0 = sub3(gltN1, scale3(nhat, g1n));
+      gnhat = sub3(gnhat, add3(scale3(gltN1, l0n), scale3(ltN0, g1n)));
+      d3 gvtErr{0, 0, 0};
+      double gvtLen = 0.0;
+      gltOld = add3(gltOld, gltN0);
+      if (upd == 1) {
+        // ltN0 = ltOld + that c, c = -vtLen / wT
+        d3 gthat = scale3(gltN0, c);
+        const double gc = dot3(gltN0, that);
+        gvtLen += -gc / wT;
+        const double gwT = -gc * c / wT;
+        if (sf != 2) {
+          const d3 gcA = scale3(pd::angDelta(B.qA, cA, B.invIA), 2.0 * gwT);
+          grA = add3(grA, cross3(that, gcA));
+          gthat = add3(gthat, cross3(gcA, rA));
+        }
+        if (!(sf == 1 || !B.hasB)) {
+          const d3 gcB = scale3(pd::angDelta(B.qB, cB, B.invIB), 2.0 * gwT);
+          grB = add3(grB, cross3(that, gcB));
+          gthat = add3(gthat, cross3(gcB, rB));
+        }
+        // that = vtErr / vtLen (as vtErr * (1 / vtLen))
+        gvtErr = add3(gvtErr, scale3(gthat, invLen));
+        const double ginv = dot3(gthat, vtErr);
+        gvtLen += -ginv * invLen * invLen;
+      } else if (upd == 2) {
+        gvtErr = sub3(gvtErr, scale3(gltN0, 1.0 / wTc));
+      }
+      if (vtLen > 0.0) gvtErr = add3(gvtErr, scale3(vtErr, gvtLen / vtLen));
+      // vtErr = vt + (vt0 - nhat (vt0 . nhat)) beta
+      d3 gvt = gvtErr;
+      if (betaOn) {
+        const d3 gv0p = scale3(gvtErr, restT);
+        const double gpn = dot3(gv0p, nhat);
+        gvt0 = add3(gvt0, sub3(gv0p, scale3(nhat, gpn)));
+        gnhat = sub3(gnhat, add3(scale3(gv0p, v0n), scale3(vt0, gpn)));
+      }
+      // vt = vrel - nhat (vrel . nhat)
+      const double gtn = dot3(gvt, nhat);
+      const d3 gvrel = sub3(gvt, scale3(nhat, gtn));
+      gnhat = sub3(gnhat, add3(scale3(gvt, vrn), scale3(vrel, gtn)));
+      // vrel = vA1 + wA1 x rA - vB1 - wB1 x rB
+      gvA1 = add3(gvA1, gvrel);
+      gwA1 = add3(gwA1, cross3(rA, gvrel));
+      grA = add3(grA, cross3(gvrel, wA1));
+      if (B.hasB) {
+        gvB1 = sub3(gvB1, gvrel);
+        const d3 ng = pd::neg3(gvrel);
+        gwB1 = add3(gwB1, cross3(rB, ng));
+        grB = add3(grB, cross3(ng, wB1));
+      } else {
+        grB = add3(grB, cross3(pd::neg3(gvrel), wB1));
+      }
+      // nhat = N invLenN
+      gN = add3(gN, scale3(gnhat, invLenN));
+      ginvLenN += dot3(gnhat, N);
+    }
+    // ---- reverse of the normal part
+    double gpNew = 0.0, gpOld = 0.0;
+    // out[12] and lamNow: pNew if applied, else pOld
+    if (applied) gpNew += yb[12] + glamNow;
+    else gpOld += yb[12] + glamNow;
+    d3 gvA = gvA1, gwA = gwA1, gvB = gvB1, gwB = gwB1;
+    if (applied) {
+      d3 gJ{0, 0, 0};
+      double glambda = 0.0;
+      if (applyA) {
+        gJ = add3(gJ, scale3(gvA1, B.invMassA));
+        const d3 t = pd::angDelta(B.qA, gwA1, B.invIA);
+        gTA = add3(gTA, scale3(t, lambda));
+        glambda += dot3(t, TA);
+      }
+      if (B.hasB && applyB) {
+        gJ = sub3(gJ, scale3(gvB1, B.invMassB));
+        const d3 t = pd::angDelta(B.qB, gwB1, B.invIB);
+        gTB = add3(gTB, scale3(t, lambda));
+        glambda += dot3(t, TB);
+      }
+      gN = add3(gN, scale3(gJ, lambda));
+      glambda += dot3(gJ, N);
+      const double gdApplied = -sgn * glambda;
+      gpNew += gdApplied;
+      gpOld -= gdApplied;
+    }
+    double gvn = 0.0, gwTotal = 0.0;
+    if (!clamped) {
+      gpOld += gpNew;
+      const double gdp = gpNew;
+      gvn = sgn * gdp / wTotal;
+      const double gtarget = -gdp / wTotal;
+      gwTotal = -gdp * dp / wTotal;
+      if (v0til > 0.0 && useE) {
+        ge += -gtarget * v0til;
+        gvn0 += sgn * (-e * gtarget);
+      }
+    }
+    // wTotal
+    double gwAn = applyA ? gwTotal : 0.0, gwBn = applyB ? gwTotal : 0.0;
+    double gNsq = gwAn * B.invMassA + gwBn * B.invMassB;
+    gTA = add3(gTA, scale3(pd::angDelta(B.qA, TA, B.invIA), 2.0 * gwAn));
+    gTB = add3(gTB, scale3(pd::angDelta(B.qB, TB, B.invIB), 2.0 * gwBn));
+    // vn = vA.N + wA.TA - vB.N + wB.TB + growth
+    gvA = add3(gvA, scale3(N, gvn));
+    gwA = add3(gwA, scale3(TA, gvn));
+    gN = add3(gN, scale3(sub3(vA, vB), gvn));
+    gTA = add3(gTA, scale3(wA, gvn));
+    gTB = add3(gTB, scale3(wB, gvn));
+    if (B.hasB) {
+      gvB = sub3(gvB, scale3(N, gvn));
+      gwB = add3(gwB, scale3(TB, gvn));
+    }
+    // growth = (dc growthRate) . N
+    gN = add3(gN, scale3(dc, growthRate * gvn));
+    const d3 gdc = scale3(N, growthRate * gvn);
+    grA = add3(grA, gdc);
+    if (B.hasB) grB = sub3(grB, gdc);
+    // lenN, invLenN, Nsq
+    glenN += -ginvLenN * invLenN * invLenN;
+    gN = add3(gN, scale3(N, 2.0 * gNsq + glenN / lenN));
+    // scatter
+    xb[0] = gvA.x; xb[1] = gvA.y; xb[2] = gvA.z;
+    xb[3] = gwA.x; xb[4] = gwA.y; xb[5] = gwA.z;
+    if (B.hasB) {
+      xb[6] = gvB.x; xb[7] = gvB.y; xb[8] = gvB.z;
+      xb[9] = gwB.x; xb[10] = gwB.y; xb[11] = gwB.z;
+    }
+    xb[12] = gpOld;
+    xb[13] = gltOld.x; xb[14] = gltOld.y; xb[15] = gltOld.z;
+    xb[16] = gN.x; xb[17] = gN.y; xb[18] = gN.z;
+    xb[19] = gTA.x; xb[20] = gTA.y; xb[21] = gTA.z;
+    xb[22] = gTB.x; xb[23] = gTB.y; xb[24] = gTB.z;
+    xb[25] = grA.x; xb[26] = grA.y; xb[27] = grA.z;
+    xb[28] = grB.x; xb[29] = grB.y; xb[30] = grB.z;
+    xb[31] = gvn0;
+    xb[32] = gvt0.x; xb[33] = gvt0.y; xb[34] = gvt0.z;
+    xb[35] = gpimp;
+    xb[36] = gmu;
+    xb[37] = ge;
+  }
 };
 
 // applyVelocityAndPredictPositionKokkos, src/integration.hpp:112-148:
@@ -1404,6 +1705,36 @@ __global__ __launch_bounds__(TPB) void b_vpass(RB<double> B, GB G, RK K, Topo t,
     } else if (c < 16) G.lam[4 * k + c - 12] = xb[c];
     else G.rc[22 * k + c - 16] += xb[c];
   }
+}
+
+// The reverse PGS sweeps (mode 1 main loop, 2 stabilization): one thread per row, the row's
+// vector-Jacobian product in one backward pass (ElPgs::vjpRev) instead of forward-mode columns.
+__global__ __launch_bounds__(TPB) void b_vpass_pgs(RB<double> B, GB G, RK K, Topo t, int mode, int r0,
+                                                   int r1, const double* tape) {
+  const int k = r0 + gtid();
+  if (k >= r1) return;
+  const int2 r = t.vAB[k];
+  double x[VR_NI], yb[VR_NO], xb[VR_NI];
+  for (int c = 0; c < 16; ++c) x[c] = tape[16 * (size_t)k + c];
+  for (int c = 0; c < 20; ++c) x[16 + c] = B.rc[20 * k + c];
+  x[36] = B.par[0];
+  x[37] = B.par[1];
+  for (int c = 0; c < 6; ++c) yb[c] = G.vc[6 * r.x + c];
+  for (int c = 0; c < 6; ++c) yb[6 + c] = r.y >= 0 ? G.vc[6 * r.y + c] : 0.0;
+  for (int c = 0; c < 4; ++c) yb[12 + c] = G.lam[4 * k + c];
+  ElPgs f;
+  f.B = rowBodies(B, K, r);
+  f.growthRate = K.P.growthRate;
+  f.restVelThreshold = K.vRest;
+  f.restT = K.P.restT;
+  const unsigned char fl = t.vFlag[k];
+  f.sf = mode == 2 ? ((fl & F_SIDE1) ? 1 : ((fl & F_SIDE2) ? 2 : 0)) : 0;
+  f.vjpRev(x, yb, xb);
+  for (int c = 0; c < 6; ++c) G.vc[6 * r.x + c] = xb[c];
+  if (r.y >= 0)
+    for (int c = 0; c < 6; ++c) G.vc[6 * r.y + c] = xb[6 + c];
+  for (int c = 0; c < 4; ++c) G.lam[4 * k + c] = xb[12 + c];
+  for (int c = 16; c < VR_NI; ++c) G.rc[22 * k + c - 16] += xb[c];
 }
 
 // warm-start values -> the previous ledger; the velocity-stage start -> the predicted velocities

@@ -402,9 +402,13 @@ void reverseStep(pd_sim& s, const TapeStep& S, ad::RB<double> B, GWS& W, double*
       const int mode = p == 0 ? 0 : (p <= S.sweepsV ? 1 : 2);
       for (int c = S.nColV - 1; c >= 0; --c) {
         const int r0 = S.vs[c], r1 = S.vs[c + 1];
-        if (r1 > r0)
+        if (r1 <= r0) continue;
+        if (mode == 0)
           ad::b_vpass<<<nb((long)(r1 - r0) * ad::VP_SPLIT), TPB, 0, st>>>(
               B, G, K, t, mode, r0, r1, W.vtape.p + stride * p);
+        else
+          ad::b_vpass_pgs<<<nb(r1 - r0), TPB, 0, st>>>(B, G, K, t, mode, r0, r1,
+                                                       W.vtape.p + stride * p);
       }
     }
   }
