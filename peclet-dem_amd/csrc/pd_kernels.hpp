@@ -3382,6 +3382,27 @@ __global__ __launch_bounds__(TPB) void k_psweep(const Dev* __restrict__ Dp, int 
   const double o = (k < rowEnd) ? posRow(D, k) : 0.0;
   blockAtomicMaxU64(D.maxOverlapBits, dbits(o), sh);
 }
+// colour 62 in waves: a row's wave is one more than the latest earlier row of the class sharing
+// a body, so the rows of a wave are body-disjoint and every row sees the same bodies as in the
+// serial pass in row order; the waves run in order, separated by workgroup barriers
+__global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psweep_waves(
+    const Dev* __restrict__ Dp, const int* __restrict__ perm, const int* __restrict__ waveStart,
+    int numWaves, const int* stop) {
+  const Dev& D = *Dp;
+  __shared__ unsigned long long sh[BLOCK_SOLVER_THREADS / 64];
+  if (*stop) return;
+  unsigned long long m = 0;
+  for (int w = 0; w < numWaves; ++w) {
+    const int b = waveStart[w], e = waveStart[w + 1];
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+      const unsigned long long a = dbits(posRow(D, perm[i]));
+      m = a > m ? a : m;
+    }
+    __syncthreads();
+  }
+  const unsigned long long M = blockMaxAll(m, sh);
+  if (threadIdx.x == 0 && M) atomicMax(D.maxOverlapBits, M);
+}
 // colour 62: one thread, in row order (the class's rows are placed in contact order)
 __global__ void k_psweep_serial(const Dev* __restrict__ Dp, int rowBegin, int rowEnd, const int* stop) {
   const Dev& D = *Dp;

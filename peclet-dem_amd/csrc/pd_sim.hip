@@ -1689,6 +1689,55 @@ struct pd_sim {
     if (rec >= 0 && curSweepTag >= 0) sweepRecs.push_back({rec, curSweepTag});
   }
   int posSweepsPrev = 1;  // sweeps of the last batched position loop (first batch size)
+  // The contact colouring's last class (62) may hold rows sharing a body (shell shapes: the many
+  // contacts of one pair), which run in row order.  From waveMinRows rows on they are scheduled
+  // in waves on the host (one pass over the class) and run by one workgroup, wave after wave.
+  long waveMinRows = 4096;
+  DBuf<int> pWavePerm, pWaveStart;
+  std::vector<int> waveLast, waveStamp, waveOf, waveCnt, wavePermHost;
+  int waveGen = 0;
+  // rows [b, b + count) with bodies ends[]: waves in row order into perm / starts; returns the
+  // wave count
+  int serialClassWaves(const int2* ends, int count, int numIds, int rowBegin, DBuf<int>& perm,
+                       DBuf<int>& starts) {
+    if ((int)waveLast.size() < numIds) {
+      waveLast.assign(numIds, 0);
+      waveStamp.assign(numIds, 0);
+      waveGen = 0;
+    }
+    if (++waveGen == 0x7fffffff) {
+      std::fill(waveStamp.begin(), waveStamp.end(), 0);
+      waveGen = 1;
+    }
+    waveOf.resize(count);
+    int depth = 0;
+    auto last = [&](int id) { return waveStamp[id] == waveGen ? waveLast[id] : 0; };
+    for (int r = 0; r < count; ++r) {
+      const int a = ends[r].x, b = ends[r].y;
+      int w = last(a);
+      if (b >= 0) w = std::max(w, last(b));
+      ++w;
+      waveOf[r] = w - 1;
+      waveLast[a] = w;
+      waveStamp[a] = waveGen;
+      if (b >= 0) {
+        waveLast[b] = w;
+        waveStamp[b] = waveGen;
+      }
+      depth = std::max(depth, w);
+    }
+    waveCnt.assign(depth + 1, 0);
+    for (int r = 0; r < count; ++r) waveCnt[waveOf[r] + 1]++;
+    for (int w = 0; w < depth; ++w) waveCnt[w + 1] += waveCnt[w];
+    wavePermHost.resize(count);
+    std::vector<int> fill(waveCnt.begin(), waveCnt.end() - 1);
+    for (int r = 0; r < count; ++r) wavePermHost[fill[waveOf[r]]++] = rowBegin + r;
+    perm.ensure(count + 1, st);
+    starts.ensure(depth + 2, st);
+    writeN(perm.p, wavePermHost.data(), count);
+    writeN(starts.p, waveCnt.data(), depth + 1);
+    return depth;
+  }
   static constexpr int POS_TAIL_ROWS = 8192;
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
@@ -2198,6 +2247,15 @@ struct pd_sim {
       int cFrom = numPosColors;
       while (cFrom > 0 && ps[numPosColors] - ps[cFrom - 1] <= POS_TAIL_ROWS) --cFrom;
       const ClassStarts pcs = classStarts(ps);
+      int waves62 = 0;
+      if (numPosColors == 63 && ps[63] - ps[62] >= std::max(1L, waveMinRows)) {
+        cFrom = 63;  // colour 62 leaves the tail launch: it runs in waves
+        const int cnt62 = ps[63] - ps[62];
+        std::vector<int2> ends(cnt62);
+        readN(ends.data(), pAB.p + ps[62], cnt62);
+        waves62 = serialClassWaves(ends.data(), cnt62, numParticles, ps[62], pWavePerm,
+                                   pWaveStart);
+      }
       int done = 0, batch = std::max(1, std::min(posIters, posSweepsPrev));
       // (the residual was cleared by the rows launch)
       while (done < posIters) {
@@ -2206,7 +2264,10 @@ struct pd_sim {
           for (int c = 0; c < cFrom; ++c) {
             const int cb = ps[c], ce = ps[c + 1];
             if (ce <= cb) continue;
-            if (c == 62) {
+            if (c == 62 && waves62 > 0) {
+              k_psweep_waves<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(dD, pWavePerm.p, pWaveStart.p,
+                                                                 waves62, ctl.p + CTL_STOP_P);
+            } else if (c == 62) {
               k_psweep_serial<<<1, 1, 0, st>>>(dD, cb, ce, ctl.p + CTL_STOP_P);
             } else {
               kernBegin();
@@ -3604,6 +3665,9 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     } else if (k == "vsweep_waves") {
       if (value != 3 && value != 4) throw ArgError("vsweep_waves must be 3 or 4");
       s->vsweepWaves4 = value == 4;
+    } else if (k == "serial_class_waves_min_rows") {
+      if (value < 0) throw ArgError("serial_class_waves_min_rows must be >= 0");
+      s->waveMinRows = value;
     } else if (k == "velocity_pack_bodies") {
       if (value != 0 && value != 1) throw ArgError("velocity_pack_bodies must be 0 or 1");
       s->packBodies = value == 1;

@@ -174,10 +174,16 @@ def apply_cylinders(sim, sc, dt):
 
 @pytest.mark.gpu
 @pytest.mark.timeout(600)
-def test_hollow_cylinders_256_point_shell(hip_lib, oracle_threads):
+@pytest.mark.parametrize("tuning", [{}, {"block_solver_rows": 0, "serial_class_waves_min_rows": 1}],
+                         ids=["default", "per-colour-waves"])
+def test_hollow_cylinders_256_point_shell(hip_lib, oracle_threads, tuning):
+    # per-colour-waves: the position sweeps run one launch per colour and the contact colour
+    # 62 (the many contacts of one pair of shells share both bodies) runs in waves
     n, dt = 125, 5e-6
     sc = cylinder_scene(n, 3, 3.9e-3, 0.2, 5)
     h = apply_cylinders(hip_sim(n), sc, dt)
+    for key, value in tuning.items():
+        h.set_tuning(key, value)
     r = apply_cylinders(oracle_sim(n), sc, dt)
     assert np.array_equal(h.get_shell_points(), r.get_shell_points())
     assert len(h.get_shell_points()) == 256
