@@ -2879,6 +2879,34 @@ __device__ __forceinline__ unsigned long long blockMaxAll(unsigned long long v,
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = sh[w] > m ? sh[w] : m;
   return m;
 }
+// colour 62 of the manifold colouring in waves (see k_psweep_waves): mode 0 warm start, 1 PGS,
+// 2 one-shot GS; PK: bodies in the packed records vb
+template <bool PK>
+__global__ __launch_bounds__(VBLOCK_THREADS) void k_vsweep_waves(
+    const Dev* __restrict__ Dp, Params P, const int* __restrict__ perm,
+    const int* __restrict__ waveStart, int numWaves, int mode, double thr, const int* stop,
+    VBody* vb) {
+  const Dev& D = *Dp;
+  __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
+  if (*stop) return;
+  unsigned long long m = 0;
+  for (int w = 0; w < numWaves; ++w) {
+    const int b = waveStart[w], e = waveStart[w + 1];
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) {
+      const int k = perm[i];
+      double a = 0.0;
+      if (mode == 0) RowOf<PK>::warm(D, k, vb);
+      else if (mode == 1) a = RowOf<PK>::pgs(D, P, k, thr, vb);
+      else a = RowOf<PK>::gs(D, P, k, thr, vb);
+      const unsigned long long ab = dbits(a);
+      m = ab > m ? ab : m;
+    }
+    __syncthreads();
+  }
+  const unsigned long long M = blockMaxAll(m, sh);
+  if (mode != 0 && threadIdx.x == 0 && M) atomicMax(D.maxApproachBits, M);
+}
+
 // The sweep loop proper (MODE 1 = PGS, 2 = one-shot GS) over rows whose bodies live in `bs`;
 // returns after the sweep that meets the stop rule or after `iters` sweeps.
 template <int MODE, class BS>

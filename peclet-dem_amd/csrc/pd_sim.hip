@@ -1632,7 +1632,14 @@ struct pd_sim {
       const int b = vs[c], e = vs[c + 1];
       if (e <= b) continue;
       VBody* vb = vbActive;
-      if (c == 62) {
+      if (c == 62 && vWaves62 > 0) {
+        if (vb)
+          k_vsweep_waves<true><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, vWavePerm.p, vWaveStart.p,
+                                                             vWaves62, mode, thr, stop, vb);
+        else
+          k_vsweep_waves<false><<<1, VBLOCK_THREADS, 0, st>>>(dD, P, vWavePerm.p, vWaveStart.p,
+                                                              vWaves62, mode, thr, stop, nullptr);
+      } else if (c == 62) {
         k_vsweep_serial<<<1, 1, 0, st>>>(dD, P, b, e, mode, thr, stop, vb);
       } else {
         if (mode == 0) {
@@ -1693,7 +1700,8 @@ struct pd_sim {
   // contacts of one pair), which run in row order.  From waveMinRows rows on they are scheduled
   // in waves on the host (one pass over the class) and run by one workgroup, wave after wave.
   long waveMinRows = 4096;
-  DBuf<int> pWavePerm, pWaveStart;
+  DBuf<int> pWavePerm, pWaveStart, vWavePerm, vWaveStart;
+  int vWaves62 = 0;  // waves of the manifold colouring's class 62 this substep (0: row order)
   std::vector<int> waveLast, waveStamp, waveOf, waveCnt, wavePermHost;
   int waveGen = 0;
   // rows [b, b + count) with bodies ends[]: waves in row order into perm / starts; returns the
@@ -1945,6 +1953,14 @@ struct pd_sim {
     if (nRowsV > 0)
       k_vrows<<<nblk(nm), TPB, 0, st>>>(dD, P, nm, prevCount > 0 ? ledgerBuckets : 0,
                                         usePGS ? 1 : 0);
+    // a large manifold class 62 (rows sharing bodies) runs in waves in the per-colour sweeps
+    vWaves62 = 0;
+    if (numColors == 63 && vs[63] - vs[62] >= std::max(1L, waveMinRows)) {
+      const int cnt62 = vs[63] - vs[62];
+      std::vector<int2> ends(cnt62);
+      readN(ends.data(), vAB.p + vs[62], cnt62);
+      vWaves62 = serialClassWaves(ends.data(), cnt62, numReal, vs[62], vWavePerm, vWaveStart);
+    }
     // the per-colour launches of a large bed read each row's bodies from one 128-byte record
     // per body, packed here and unpacked when the stage ends
     if (packBodies && !hostLoop && !fusedV && !smallV && nRowsV > 0 && numReal > 0 &&

@@ -108,6 +108,25 @@ def make_scene(kind, n=256, seed=0):
             sc["pos"][:, 0] %= L
             sc["pos"][:, 1] %= L
         return sc
+    if kind == "big_among_small":
+        # a sphere of ten radii in a bed of unit ones, the nearest of which overlap it by up to
+        # half a radius: on the first substep it holds more than 62 active manifolds, so both
+        # colourings' last class (62) holds rows that share the big sphere
+        r = 1e-3
+        pos, v, L = lattice_bed(n, r, seed, vel=0.02, layers=12)
+        big = np.array([[L / 2, L / 2, 11.5 * r]])
+        keep = np.linalg.norm(pos - big[0], axis=1) > 10.5 * r
+        pos = np.concatenate([pos[keep], big])
+        m_ = len(pos)
+        scale = np.concatenate([rng.uniform(0.97, 1.03, m_ - 1), [10.0]])
+        rad = r * scale
+        m = 2500.0 * 4.0 / 3.0 * np.pi * rad ** 3
+        return dict(shape=(SPHERE, r), domain=(L, L, 60 * r), periodic=(True, True, False),
+                    planes=[(0, 0, 0, 0, 0, 1)], pos=pos, scale=scale, inv_mass=1.0 / m,
+                    inv_inertia=np.repeat((1.0 / (0.4 * m * rad ** 2))[:, None], 3, 1),
+                    vel=np.concatenate([v[keep], np.zeros((1, 3))]),
+                    angvel=rng.normal(0, 5.0, (m_, 3)), gravity=(0, 0, -9.81),
+                    material=(0.3, 0.2, 0.5), iters=(12, 8), dt=1e-5)
     if kind == "column_small_box":
         # a 2 x 2 column of spheres in a periodic box two diameters wide: every sphere has three
         # periodic images

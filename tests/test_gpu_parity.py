@@ -144,6 +144,19 @@ def test_per_colour_launches_body_records(hip_lib, kind, pack):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tuning", [{}, {"block_solver_rows": 0}, {"block_solver_rows": 0,
+                                     "serial_class_waves_min_rows": 1}],
+                         ids=["default", "per-colour", "per-colour-waves"])
+def test_big_sphere_last_colour_class(hip_lib, tuning):
+    # both colourings' class 62 holds rows sharing the big sphere (the first substep's manifolds,
+    # every substep's contacts): one row after another in one workgroup's loop, one thread in a
+    # per-colour launch, or in waves of body-disjoint rows
+    log = []
+    run_pair("big_among_small", 4000, 4, tuning=tuning, log=log)
+    assert log[0]["num_colors_vel"] == 63 and log[0]["num_colors_pos"] == 63
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n,steps", [(2500, 4), (4000, 3)])
 def test_velocity_stage_large_beds(hip_lib, n, steps):
     # 2500 spheres: ~4500-6700 rows over ~2900 bodies, so the staged stage keeps 2400 bodies in
