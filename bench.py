@@ -102,19 +102,21 @@ def kernel_bytes(name, units, log):
 
 
 def make_workload(kind, n, seed, column):
-    from peclet.dem.workloads import cylinder_bed, lattice_bed, sphere_bed
+    from peclet.dem.workloads import cylinder_bed, cylinder_pack, lattice_bed, sphere_bed
 
     if kind == "settled":
         return lattice_bed(n, 0.16, seed)
     if kind == "cylinders":
         return cylinder_bed(n, 0.2, seed)
+    if kind == "cylinders_packed":
+        return cylinder_pack(n, 0.2, seed)
     return sphere_bed(n, 0.8e-3, 1.2e-3, (0.16, 0.16, 0.4), seed, column=column)
 
 
 def setup(sim, kind, bed, dt):
     from peclet.dem.workloads import setup_cylinder_bed, setup_gravity_bed
 
-    if kind == "cylinders":
+    if kind in ("cylinders", "cylinders_packed"):
         return setup_cylinder_bed(sim, bed, dt, 0.4, 0.2)
     return setup_gravity_bed(sim, bed, dt, 0.5, 0.3)
 
@@ -126,6 +128,10 @@ WORKLOAD_TEXT = {
     "pour": ("gravity pour, {n} polydisperse spheres r~U[0.8,1.2] mm, 160x160 mm periodic xy + "
              "floor, random start seeded across the periodic seams in a {col} mm column, "
              "dt=5e-6 s, mu=0.5, e=0.3, iters pos 12 / vel 8"),
+    "cylinders_packed": ("{n} hollow cylinders r_out 2 mm, r_in 1.2 mm, L 4 mm, 256-point seeded "
+                         "shell, packed at rest on a 4.1 mm jittered lattice (the contact-rich state "
+                         "of a settled bed), random orientations, 200x200 mm periodic xy + floor, "
+                         "dt=5e-6 s, mu=0.4, e=0.2, iters pos 12 / vel 8"),
     "cylinders": ("{n} hollow cylinders r_out 2 mm, r_in 1.2 mm, L 4 mm, 256-point seeded shell, "
                   "random orientations, 200x200 mm periodic xy + floor, random start, dt=5e-6 s, "
                   "mu=0.4, e=0.2, iters pos 12 / vel 8"),
@@ -460,6 +466,8 @@ def main():
     ap.add_argument("--pour-steps", type=int, default=500,
                     help="steps of the pour leg (the whole 500-step job by default)")
     ap.add_argument("--cyl-steps", type=int, default=2, help="steps of the cylinder leg")
+    ap.add_argument("--packed-substeps", type=int, default=4,
+                    help="timed substeps of the packed-cylinder leg")
     ap.add_argument("--hertz-steps", type=int, default=4, help="steps of the Hertz leg")
     ap.add_argument("--no-pour", action="store_true")
     ap.add_argument("--no-cylinders", action="store_true")
@@ -484,7 +492,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the forward+adjoint leg and print its JSON")
-    ap.add_argument("--only", choices=["headline", "pour", "cylinders", "hertz", "batched"],
+    ap.add_argument("--only", choices=["headline", "pour", "cylinders", "cylinders_packed", "hertz",
+                                       "batched"],
                     default=None, help="run one leg and print its JSON (profiling)")
     ap.add_argument("--adjoint-n", type=int, default=50000)
     ap.add_argument("--adjoint-case", choices=["both", "loose_start", "dense_bed"], default="both",
@@ -528,6 +537,11 @@ def main():
                             args.cyl_steps * SUBSTEPS_PER_STEP, 1, args.timing_level)
         print(json.dumps(r))
         return
+    if args.only == "cylinders_packed":
+        _, _, r = plain_leg("cylinders_packed", 200000, 3, args.column, dt, 1,
+                            args.packed_substeps, 1, args.timing_level)
+        print(json.dumps(r))
+        return
     if args.only == "hertz":
         print(json.dumps(hertz_leg(1048576, args.seed, args.hertz_steps)))
         return
@@ -561,6 +575,11 @@ def main():
                                             args.cyl_steps * SUBSTEPS_PER_STEP, 1,
                                             args.timing_level)
         print(f"# cylinders: {legs['cylinders']['value']:.3e} particle-substeps/s",
+              file=sys.stderr, flush=True)
+        # the same cylinders packed at rest: every contact colour up to the last class (62)
+        _, _, legs["cylinders_packed"] = plain_leg("cylinders_packed", 200000, 3, args.column, dt,
+                                                   1, args.packed_substeps, 1, args.timing_level)
+        print(f"# cylinders packed: {legs['cylinders_packed']['value']:.3e} particle-substeps/s",
               file=sys.stderr, flush=True)
     if not args.no_hertz:
         legs["hertz"] = hertz_leg(1048576, args.seed, args.hertz_steps)

@@ -258,8 +258,9 @@ int pd_set_timing(pd_sim* s, int level);
  * velocity launches read each body's velocity and constants from one 128-byte record packed for
  * the stage, 0 from the separate arrays; "serial_class_waves_min_rows" (default 4096): a last
  * contact colour class (62, whose rows may share bodies) of at least that many rows runs in
- * waves of body-disjoint rows instead of one row after another.  Results do not depend on any
- * of them. */
+ * waves of body-disjoint rows instead of one row after another, a wave of at least
+ * "serial_class_wave_launch_rows" rows (default 512) in a launch of its own and runs of smaller
+ * waves in one workgroup.  Results do not depend on any of them. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 /* Autotuner (no reference counterpart): chooses candidate_skin_permille, block_solver_rows,
  * velocity_loop_lds and vsweep_waves for this simulation by timing its own substeps (coordinate

@@ -3431,6 +3431,16 @@ __global__ __launch_bounds__(BLOCK_SOLVER_THREADS) void k_psweep_waves(
   const unsigned long long M = blockMaxAll(m, sh);
   if (threadIdx.x == 0 && M) atomicMax(D.maxOverlapBits, M);
 }
+// one wave of colour 62 with many rows (body-disjoint): the rows perm[b, e) over the grid
+__global__ __launch_bounds__(TPB) void k_psweep_perm(const Dev* __restrict__ Dp, const int* __restrict__ perm,
+                                                     int b, int e, const int* stop) {
+  const Dev& D = *Dp;
+  if (*stop) return;
+  __shared__ unsigned long long sh[WAVES];
+  const int i = b + gtid();
+  const double o = (i < e) ? posRow(D, perm[i]) : 0.0;
+  blockAtomicMaxU64(D.maxOverlapBits, dbits(o), sh);
+}
 // colour 62: one thread, in row order (the class's rows are placed in contact order)
 __global__ void k_psweep_serial(const Dev* __restrict__ Dp, int rowBegin, int rowEnd, const int* stop) {
   const Dev& D = *Dp;

@@ -1701,6 +1701,11 @@ struct pd_sim {
   // in waves on the host (one pass over the class) and run by one workgroup, wave after wave.
   long waveMinRows = 4096;
   DBuf<int> pWavePerm, pWaveStart, vWavePerm, vWaveStart;
+  // the contact class's waves as launches: runs of small waves in one workgroup (x = first
+  // wave, y = end wave, z = 0), a wave of at least waveLaunchRows rows over the grid (z = 1)
+  long waveLaunchRows = 512;
+  std::vector<int3> pWaveSegs;
+  std::vector<int> pWaveStartHost;
   int vWaves62 = 0;  // waves of the manifold colouring's class 62 this substep (0: row order)
   std::vector<int> waveLast, waveStamp, waveOf, waveCnt, wavePermHost;
   int waveGen = 0;
@@ -2271,6 +2276,15 @@ struct pd_sim {
         readN(ends.data(), pAB.p + ps[62], cnt62);
         waves62 = serialClassWaves(ends.data(), cnt62, numParticles, ps[62], pWavePerm,
                                    pWaveStart);
+        pWaveStartHost.assign(waveCnt.begin(), waveCnt.begin() + waves62 + 1);
+        pWaveSegs.clear();
+        for (int w = 0; w < waves62; ++w) {
+          const bool big = pWaveStartHost[w + 1] - pWaveStartHost[w] >= waveLaunchRows;
+          if (!big && !pWaveSegs.empty() && pWaveSegs.back().z == 0 && pWaveSegs.back().y == w)
+            pWaveSegs.back().y = w + 1;
+          else
+            pWaveSegs.push_back(make_int3(w, w + 1, big ? 1 : 0));
+        }
       }
       int done = 0, batch = std::max(1, std::min(posIters, posSweepsPrev));
       // (the residual was cleared by the rows launch)
@@ -2281,8 +2295,16 @@ struct pd_sim {
             const int cb = ps[c], ce = ps[c + 1];
             if (ce <= cb) continue;
             if (c == 62 && waves62 > 0) {
-              k_psweep_waves<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(dD, pWavePerm.p, pWaveStart.p,
-                                                                 waves62, ctl.p + CTL_STOP_P);
+              for (const int3 g : pWaveSegs) {
+                if (g.z) {
+                  const int wb = pWaveStartHost[g.x], we = pWaveStartHost[g.y];
+                  k_psweep_perm<<<nblk(we - wb), TPB, 0, st>>>(dD, pWavePerm.p, wb, we,
+                                                               ctl.p + CTL_STOP_P);
+                } else {
+                  k_psweep_waves<<<1, BLOCK_SOLVER_THREADS, 0, st>>>(
+                      dD, pWavePerm.p, pWaveStart.p + g.x, g.y - g.x, ctl.p + CTL_STOP_P);
+                }
+              }
             } else if (c == 62) {
               k_psweep_serial<<<1, 1, 0, st>>>(dD, cb, ce, ctl.p + CTL_STOP_P);
             } else {
@@ -3684,6 +3706,9 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     } else if (k == "serial_class_waves_min_rows") {
       if (value < 0) throw ArgError("serial_class_waves_min_rows must be >= 0");
       s->waveMinRows = value;
+    } else if (k == "serial_class_wave_launch_rows") {
+      if (value < 1) throw ArgError("serial_class_wave_launch_rows must be >= 1");
+      s->waveLaunchRows = value;
     } else if (k == "velocity_pack_bodies") {
       if (value != 0 && value != 1) throw ArgError("velocity_pack_bodies must be 0 or 1");
       s->packBodies = value == 1;

@@ -292,6 +292,40 @@ def cylinder_bed(n, box_xy, seed, r_out=2e-3, r_in=1.2e-3, length=4e-3, shell_po
     }
 
 
+def cylinder_pack(n, box_xy, seed, pitch=4.1e-3, r_out=2e-3, r_in=1.2e-3, length=4e-3,
+                  shell_points=256, rho=2500.0):
+    """Hollow cylinders packed on a jittered cubic lattice of the given pitch (smaller than the
+    bounding diameter, so neighbours interpenetrate their bounding spheres and most touch) over
+    the floor of a periodic ``box_xy`` box, random orientations, at rest: the contact-rich state
+    of a settled bed.  Same dict as :func:`cylinder_bed`."""
+    rb = math.sqrt(r_out ** 2 + (0.5 * length) ** 2)
+    side = max(1, int(box_xy / pitch))
+    rng = np.random.default_rng(seed)
+    k = np.arange(n)
+    iz, rem = np.divmod(k, side * side)
+    iy, ix = np.divmod(rem, side)
+    pos = np.stack([(ix + 0.5) * pitch, (iy + 0.5) * pitch, rb + iz * pitch], 1)
+    pos += rng.uniform(-0.05, 0.05, pos.shape) * pitch
+    pos[:, 2] = np.maximum(pos[:, 2], rb)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    vol = math.pi * (r_out ** 2 - r_in ** 2) * length
+    m = rho * vol
+    i_axis = 0.5 * m * (r_out ** 2 + r_in ** 2)
+    i_diam = m * (3.0 * (r_out ** 2 + r_in ** 2) + length ** 2) / 12.0
+    L = side * pitch
+    return {
+        "pos": pos,
+        "quat": q,
+        "inv_mass": np.full(n, 1.0 / m),
+        "inv_inertia": np.tile([1.0 / i_diam, 1.0 / i_axis, 1.0 / i_diam], (n, 1)),
+        "shell": cylinder_shell(shell_points, r_out, r_in, length),
+        "shape": (2, r_out, length, r_out - r_in),
+        "domain": (float(L), float(L), float(pos[:, 2].max() + 4 * rb)),
+        "base_radius": r_out,
+    }
+
+
 def setup_cylinder_bed(sim, bed, dt, mu, e, iters=(12, 8)):
     sim.initialize_shape(*bed["shape"])
     sim.set_shell_points(bed["shell"])

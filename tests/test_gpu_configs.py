@@ -174,11 +174,14 @@ def apply_cylinders(sim, sc, dt):
 
 @pytest.mark.gpu
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("tuning", [{}, {"block_solver_rows": 0, "serial_class_waves_min_rows": 1}],
-                         ids=["default", "per-colour-waves"])
+@pytest.mark.parametrize("tuning", [{}, {"block_solver_rows": 0, "serial_class_waves_min_rows": 1},
+                                    {"block_solver_rows": 0, "serial_class_waves_min_rows": 1,
+                                     "serial_class_wave_launch_rows": 8}],
+                         ids=["default", "per-colour-waves", "per-colour-wave-launches"])
 def test_hollow_cylinders_256_point_shell(hip_lib, oracle_threads, tuning):
     # per-colour-waves: the position sweeps run one launch per colour and the contact colour
-    # 62 (the many contacts of one pair of shells share both bodies) runs in waves
+    # 62 (the many contacts of one pair of shells share both bodies) runs in waves, in one
+    # workgroup or (wave-launches) each wave of at least 8 rows in a launch of its own
     n, dt = 125, 5e-6
     sc = cylinder_scene(n, 3, 3.9e-3, 0.2, 5)
     h = apply_cylinders(hip_sim(n), sc, dt)
