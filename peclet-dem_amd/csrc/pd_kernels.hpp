@@ -131,7 +131,6 @@ struct Dev {
   PD_G double4* vRA;
   PD_G double4* vRB;
   PD_G double4* vVt0;
-  PD_G double4* vPre;  // per row, constant in the solve: (wA_n, wB_n, growth term, 1 / lenN)
   PD_G double4* vLamT;
   PD_G double4* vWV;   // boundary rows: wall velocity sum (the body-B velocity is vWV / points)
   PD_G double* vVn0;
@@ -1857,7 +1856,7 @@ __global__ __launch_bounds__(TPB) void k_vrows(const Dev* __restrict__ Dp, Param
     }
   }
   D.vLam[k] = lam;
-  D.vLamT[k] = lamT;
+  D.vLamT[k] = make_double4(lamT.x, lamT.y, lamT.z, pimp);  // .w: the warm-start position impulse
   D.vFlag[k] = flag;
   // computeVn0Kokkos, src/solver_velocity.hpp:390-447
   const Body64 ba = D.vw[realA];
@@ -1874,24 +1873,6 @@ __global__ __launch_bounds__(TPB) void k_vrows(const Dev* __restrict__ Dp, Param
   double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
   vn += dot3(vGrowth, Nsum);
   D.vVn0[k] = vn;
-  {
-    // the row's constants of the PGS / GS updates, evaluated as they are there: body masses,
-    // inertia and orientation do not change during the velocity solve
-    const BodyC ca = D.bc[realA];
-    const double Nsq = dot3(Nsum, Nsum);
-    const double wAn = Nsq * ca.iim.w + genInvMass(TauA, xyz(ca.iim), toq(ca.q));
-    double invMassB = 0.0;
-    d3 invIB{0, 0, 0};
-    q4 qB{0, 0, 0, 1};
-    if (realB >= 0) {
-      const BodyC cb = D.bc[realB];
-      invMassB = cb.iim.w;
-      invIB = xyz(cb.iim);
-      qB = toq(cb.q);
-    }
-    const double wBn = Nsq * invMassB + genInvMass(TauB, invIB, qB);
-    D.vPre[k] = make_double4(wAn, wBn, dot3(vGrowth, Nsum), lenN > 0.0 ? 1.0 / lenN : 0.0);
-  }
   d3 vt{0, 0, 0};
   if (lenN > 1e-9) {
     const d3 nhat = scale3(Nsum, 1.0 / lenN);
@@ -2016,14 +1997,11 @@ __global__ void k_vb_pack(const Dev* __restrict__ Dp, VBody* vb, int n) {
   const Dev& D = *Dp;
   const int i = gtid();
   if (i >= n) return;
-  const Body64 b = D.vw[i];
-  const BodyC c = D.bc[i];
-  VBody o;
-  o.v = b.v;
-  o.w = b.w;
-  o.q = c.q;
-  o.iim = c.iim;
-  vb[i] = o;
+  // field by field: a local 128-byte-aligned record would be placed in scratch memory
+  vb[i].v = D.vw[i].v;
+  vb[i].w = D.vw[i].w;
+  vb[i].q = D.bc[i].q;
+  vb[i].iim = D.bc[i].iim;
 }
 __global__ void k_vb_unpack(const Dev* __restrict__ Dp, const VBody* vb, int n) {
   const Dev& D = *Dp;
@@ -2124,7 +2102,7 @@ __device__ __forceinline__ void warmApplyRowB(const Dev& D, int k, int2 r, int2 
       const d3 nhat = scale3(Nsum, 1.0 / lenN2);
       lt = sub3(lt, scale3(nhat, dot3(lt, nhat)));
     }
-    D.vLamT[k] = tod4(lt, 0.0);
+    D.vLamT[k] = tod4(lt, lt4.w);
     ba.v.x += lt.x * invMassA;
     ba.v.y += lt.y * invMassA;
     ba.v.z += lt.z * invMassA;
@@ -2154,6 +2132,23 @@ __device__ __forceinline__ void warmApplyRow(const Dev& D, int k) {
   warmApplyRowB(D, k, r, sl, g);
 }
 
+// A velocity row's constants in the PGS / GS updates (body masses, inertia and orientation do
+// not change during the velocity solve), evaluated by the row from the bodies it loads.
+struct RowPre {
+  double wAn, wBn, growth, invLenN;
+};
+__device__ __forceinline__ RowPre rowPre(const Params& P, int realB, d3 Nsum, d3 TauA, d3 TauB,
+                                         d3 rAavg, d3 rBavg, double lenN, double invMassA,
+                                         d3 invIA, q4 qA, double invMassB, d3 invIB, q4 qB) {
+  RowPre r;
+  const double Nsq = dot3(Nsum, Nsum);
+  r.wAn = Nsq * invMassA + genInvMass(TauA, invIA, qA);
+  r.wBn = Nsq * invMassB + genInvMass(TauB, invIB, qB);
+  const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
+  r.growth = dot3(scale3(diffCenters, P.growthRate), Nsum);
+  r.invLenN = lenN > 0.0 ? 1.0 / lenN : 0.0;
+  return r;
+}
 // PGSManifoldSweep::solveOne, src/solver_velocity.hpp:641-858 (one row)
 template <class BS>
 __device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, int2 r, int2 sl,
@@ -2191,12 +2186,14 @@ __device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, 
   const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
   const double lenN = ra4.w;
   if (lenN < 1e-9) return 0.0;
-  const double4 pre = D.vPre[k];
+  // the row's constants, from the bodies just loaded (cheaper than reading them back)
+  const RowPre pre = rowPre(P, realB, Nsum, TauA, TauB, rAavg, rBavg, lenN, invMassA, invIA, qA,
+                            invMassB, invIB, qB);
   double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
-  vn += pre.z;  // growth term dot3(diffCenters * growthRate, Nsum)
+  vn += pre.growth;  // dot3(diffCenters * growthRate, Nsum)
   const double sgn = rb4.w;
-  double wA_n = pre.x;  // Nsq * invMassA + genInvMass(TauA, invIA, qA)
-  double wB_n = pre.y;
+  double wA_n = pre.wAn;  // Nsq * invMassA + genInvMass(TauA, invIA, qA)
+  double wB_n = pre.wBn;
   bool applyA = true, applyB = true;
   const unsigned char fl = D.vFlag[k];
   const int sf = (fl & F_SIDE1) ? 1 : ((fl & F_SIDE2) ? 2 : 0);
@@ -2251,11 +2248,11 @@ __device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, 
   }
   // friction cone, src/solver_velocity.hpp:760-855
   double mu = ta4.w;
-  const double4 lto4 = D.vLamT[k];
+  const double4 lto4 = D.vLamT[k];  // .w: the warm-start position impulse
   const d3 ltOld = xyz(lto4);
   const bool haveOld = nonzero3(ltOld);
   if (mu > 0.0 || haveOld) {
-    const d3 nhat = scale3(Nsum, pre.w);  // 1 / lenN
+    const d3 nhat = scale3(Nsum, pre.invLenN);
     const d3 vA2 = xyz(ba.v), wA2 = xyz(ba.w);
     d3 vB2{0, 0, 0}, wB2{0, 0, 0};
     if (realB >= 0) {
@@ -2269,9 +2266,8 @@ __device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, 
     double beta = P.restT;
     if (beta != 0.0 && (sf != 0 || fabs(vn0) < restVelThreshold * lenN)) beta = 0.0;
     d3 vtErr = vt;
-    const double4 vt04 = D.vVt0[k];
     if (beta != 0.0) {
-      d3 v0 = xyz(vt04);
+      d3 v0 = xyz(D.vVt0[k]);
       v0 = sub3(v0, scale3(nhat, dot3(v0, nhat)));
       vtErr = add3(vt, scale3(v0, beta));
     }
@@ -2288,13 +2284,13 @@ __device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, 
     }
     ltNew = sub3(ltNew, scale3(nhat, dot3(ltNew, nhat)));
     double nTot = fmax(lamNow, 0.0) * lenN;
-    if (fabs(vn0) < restVelThreshold * lenN) nTot += fmax(vt04.w, 0.0);
+    if (fabs(vn0) < restVelThreshold * lenN) nTot += fmax(lto4.w, 0.0);
     const double bound = mu * nTot;
     const double ltLen = sqrt(dot3(ltNew, ltNew));
     if (ltLen > bound) ltNew = (bound > 0.0) ? scale3(ltNew, bound / ltLen) : d3{0.0, 0.0, 0.0};
     const d3 dApp = sub3(ltNew, ltOld);
     if (nonzero3(dApp)) {
-      D.vLamT[k] = tod4(ltNew, 0.0);
+      D.vLamT[k] = tod4(ltNew, lto4.w);
       const double dLen = sqrt(dot3(dApp, dApp));
       if (wT > 1e-9 && dLen * wT > amax) amax = dLen * wT;
       if (applyA) {
@@ -2367,16 +2363,17 @@ __device__ __forceinline__ double gsRowB(const Dev& D, const Params& P, int k, i
   const double lenN = ra4.w;
   if (lenN < 1e-9) return 0.0;
   const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
-  const double4 pre = D.vPre[k];
+  const RowPre pre = rowPre(P, realB, Nsum, TauA, TauB, rAavg, rBavg, lenN, invMassA, invIA, qA,
+                            invMassB, invIB, qB);
   double vn = dot3(vA, Nsum) + dot3(wA, TauA) + dot3(vB, neg3(Nsum)) + dot3(wB, TauB);
-  vn += pre.z;  // growth term
+  vn += pre.growth;
   const double alignment = dot3(Nsum, diffCenters);
   if (alignment > 0.0) {
     if (vn < 0.0) return 0.0;
   } else {
     if (vn > 0.0) return 0.0;
   }
-  const double wA_n = pre.x, wB_n = pre.y;
+  const double wA_n = pre.wAn, wB_n = pre.wBn;
   const double wTotal = wA_n + wB_n;
   if (wTotal <= 0.0) return 0.0;
   const double amax = fabs(vn) / lenN;

@@ -304,7 +304,7 @@ struct pd_sim {
   DBuf<int> pairOfContact;  // point shapes: the pair of each pair contact
   DBuf<int2> pairs, cAB, mAB, mReal, mRange, mRange2, vAB, pAB;
   DBuf<int> ncnt, noff, cSlot, mCnt, mStart, mNp, mColor, cColor, vPerm, vRowOf, pPerm, pRowOf;
-  DBuf<double4> cN, cRA, cRB, cBV, mNs, mTA, mTB, mRA, mRB, mWV, vN, vTA, vTB, vRA, vRB, vVt0, vPre,
+  DBuf<double4> cN, cRA, cRB, cBV, mNs, mTA, mTB, mRA, mRB, mWV, vN, vTA, vTB, vRA, vRB, vVt0,
       vLamT, vWV, pN,
       pRA, pRB, lLamT;
   DBuf<double> vVn0, vLam, pLam, lLam;
@@ -430,7 +430,7 @@ struct pd_sim {
     cColor.release(); vPerm.release(); vRowOf.release(); pPerm.release(); pRowOf.release();
     cN.release(); cRA.release(); cRB.release(); mNs.release(); mTA.release(); mTB.release();
     mRA.release(); mRB.release(); vN.release(); vTA.release(); vTB.release(); vRA.release();
-    vRB.release(); vVt0.release(); vPre.release(); vLamT.release(); pN.release(); pRA.release(); pRB.release();
+    vRB.release(); vVt0.release(); vLamT.release(); pN.release(); pRA.release(); pRB.release();
     lLamT.release(); vVn0.release(); vLam.release(); pLam.release(); lLam.release();
     mKey.release(); lKey.release(); uKey.release();
     red.release(); bits.release(); lIdx.release(); bucketCnt.release(); bucketStart.release();
@@ -635,7 +635,7 @@ struct pd_sim {
     gset(d.mRB, mRB.p); gset(d.mNp, mNp.p); gset(d.mKey, mKey.p); gset(d.mCode, mCode.p); gset(d.mColor, mColor.p);
     gset(d.cColor, cColor.p); gset(d.vPerm, vPerm.p);
     gset(d.vRowOf, vRowOf.p); gset(d.pPerm, pPerm.p); gset(d.pRowOf, pRowOf.p); gset(d.vAB, vAB.p); gset(d.vN, vN.p);
-    gset(d.vTA, vTA.p); gset(d.vTB, vTB.p); gset(d.vRA, vRA.p); gset(d.vRB, vRB.p); gset(d.vVt0, vVt0.p); gset(d.vPre, vPre.p);
+    gset(d.vTA, vTA.p); gset(d.vTB, vTB.p); gset(d.vRA, vRA.p); gset(d.vRB, vRB.p); gset(d.vVt0, vVt0.p);
     gset(d.vLamT, vLamT.p); gset(d.vVn0, vVn0.p); gset(d.vLam, vLam.p); gset(d.vFlag, vFlag.p); gset(d.pAB, pAB.p);
     gset(d.pN, pN.p); gset(d.pRA, pRA.p); gset(d.pRB, pRB.p); gset(d.pLam, pLam.p); gset(d.lKey, lKey.p);
     gset(d.lLam, lLam.p); gset(d.lLamT, lLamT.p); gset(d.lIdx, lIdx.p); gset(d.uKey, uKey.p);
@@ -1696,6 +1696,8 @@ struct pd_sim {
     if (rec >= 0 && curSweepTag >= 0) sweepRecs.push_back({rec, curSweepTag});
   }
   int posSweepsPrev = 1;  // sweeps of the last batched position loop (first batch size)
+  // contact lists of at least this many contacts start their colouring before the velocity stage
+  static constexpr int EARLY_CONTACT_COLOURING = 1 << 18;
   // The contact colouring's last class (62) may hold rows sharing a body (shell shapes: the many
   // contacts of one pair), which run in row order.  From waveMinRows rows on they are scheduled
   // in waves on the host (one pass over the class) and run by one workgroup, wave after wave.
@@ -1912,6 +1914,26 @@ struct pd_sim {
     }
     mark(ST_COLOR_V);
     HIPCHECK(hipEventRecord(evContacts, st));
+    // The contact colouring (src/solver_position.hpp:155-241) needs only the contact list: it
+    // runs on the side stream.  A large contact list starts it once the manifold colouring is
+    // done (the two colourings would slow each other), beside the whole velocity stage; a small
+    // one after the velocity stage's launches are enqueued, so that enqueueing it does not hold
+    // back a short stage.
+    const bool colourP = useGS && nc > 0 && numParticles > 0;
+    bool colourPStarted = false;
+    auto startContactColouring = [&]() {
+      cColor.ensure(nc + 1, st);
+      pPerm.ensure(nc + 1, st);
+      pRowOf.ensure(nc + 1, st);
+      jobP.work[0].ensure(nc + 1, st);
+      jobP.work[1].ensure(nc + 1, st);
+      refreshDev();
+      // ordered after the contact list only (evContacts was recorded behind the manifolds)
+      HIPCHECK(hipStreamWaitEvent(jobP.stream, evContacts, 0));
+      k_jp_init_contacts<<<nblk(nc), TPB, 0, jobP.stream>>>(cColor.p, nc, jobP.work[0].p);
+      colourBegin(jobP, cAB.p, cColor.p, nc, numParticles, nullptr, nc, pPerm.p, pRowOf.p);
+      colourPStarted = true;
+    };
 
     // manifold colouring, src/solver_velocity.hpp:277-374
     vPerm.ensure(nm + 1, st);
@@ -1936,6 +1958,7 @@ struct pd_sim {
     } else {
       vs.assign(65, 0);
     }
+    if (colourP && nc >= EARLY_CONTACT_COLOURING) startContactColouring();
     nRowsV = vs[numColors];
     mark(ST_PREP);
 
@@ -1951,7 +1974,7 @@ struct pd_sim {
     const bool fusedV = !hostLoop && smallV && velIters > 0 && ldsLoop && nRowsV <= VL_ROWS;
     const size_t R = nRowsV + 1;
     vAB.ensure(R, st); vN.ensure(R, st); vTA.ensure(R, st); vTB.ensure(R, st); vRA.ensure(R, st);
-    vRB.ensure(R, st); vVt0.ensure(R, st); vPre.ensure(R, st); vLamT.ensure(R, st); vVn0.ensure(R, st);
+    vRB.ensure(R, st); vVt0.ensure(R, st); vLamT.ensure(R, st); vVn0.ensure(R, st);
     vLam.ensure(R, st); vFlag.ensure(R, st); vWV.ensure(R, st);
     refreshDev();
 
@@ -2149,23 +2172,7 @@ struct pd_sim {
         }
       }
     }
-    // The contact colouring (src/solver_position.hpp:155-241) needs only the contact list: it
-    // runs on the side stream, overlapping the velocity stages, whose sweep loops occupy one CU.
-    // It is enqueued here, behind the velocity loops, so the host spends the time it takes to
-    // enqueue it while the main stream has work queued.
-    const bool colourP = useGS && nc > 0 && numParticles > 0;
-    if (colourP) {
-      cColor.ensure(nc + 1, st);
-      pPerm.ensure(nc + 1, st);
-      pRowOf.ensure(nc + 1, st);
-      jobP.work[0].ensure(nc + 1, st);
-      jobP.work[1].ensure(nc + 1, st);
-      refreshDev();
-      // ordered after the contact list only (evContacts was recorded behind the manifolds)
-      HIPCHECK(hipStreamWaitEvent(jobP.stream, evContacts, 0));
-      k_jp_init_contacts<<<nblk(nc), TPB, 0, jobP.stream>>>(cColor.p, nc, jobP.work[0].p);
-      colourBegin(jobP, cAB.p, cColor.p, nc, numParticles, nullptr, nc, pPerm.p, pRowOf.p);
-    }
+    if (colourP && !colourPStarted) startContactColouring();
     mark(ST_LEDGER);
     if (usePGS && nm > 0) {
       // commitPairKeysLambdaKokkos, src/contact_preprocessing.hpp:160-192
