@@ -925,7 +925,9 @@ __global__ __launch_bounds__(TPB) void k_bp_compact(const Dev* __restrict__ Dp, 
   const int n = D.pairCnt[i];
   if (n == 0) return;
   int2* seg = D.pairs + D.pairStart[i];
-  constexpr int R = 8;  // short rows are sorted in registers
+  // rows of up to 16 partners (a dense bed's half-stencil rows hold ~12) are sorted in
+  // registers; longer ones in place in global memory
+  constexpr int R = 16;
   if (n <= R) {
     int v[R];
 #pragma unroll
