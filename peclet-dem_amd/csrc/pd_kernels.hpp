@@ -1632,6 +1632,229 @@ __global__ void k_jp_counts(int* cnt, const int* src, int first, int n) {
   const int i = threadIdx.x;
   if (i < n) cnt[i] = i == 0 ? (src ? *src : first) : 0;
 }
+
+// ---- the same rounds over ordered incidence lists ----------------------------------------------
+// In the contend / commit rounds an edge is coloured in the first round in which its priority is
+// the largest among the uncoloured edges at each of its bodies.  The edges at one body are thus
+// coloured at most one per round and in descending priority.  With each body's entering edges
+// sorted by descending priority and a cursor at its first uncoloured one, an edge is coloured in
+// the round after the one that made it the cursor edge of the last of its distinct bodies (round
+// 0 if it starts as the cursor edge of all of them), which is what these kernels run: a round
+// colours its list with the same first fit over its bodies' masks, moves those bodies' cursors
+// one place on and lists the edges that thereby became cursor edges at all their bodies.  Every
+// round colours the same edges with the same colours as a contend / commit round, and each edge
+// is read once instead of once per round it waits.  Within a round no two listed edges share a
+// body, so a body's mask and cursor have one writer per round.
+// Layout: a body's mask, cursor and list end in one 16-byte record; a list entry holds the edge
+// and its other body (-1 for an edge of one body); a round's list holds (edge, body, other body),
+// so a round reads no edge ends.
+struct alignas(16) JpBody {
+  unsigned long long mask;
+  int cursor, end;
+};
+constexpr int JPI_THREADS = 1024;  // sort and round kernels: one list append per 1024 threads
+__device__ __forceinline__ bool jpTwoBodies(int2 ab) { return ab.y >= 0 && ab.y != ab.x; }
+// entry (edge f, other body) has become the cursor entry at one more of f's bodies: true for the
+// last of them (an edge of one body arrives once)
+__device__ __forceinline__ bool jpArrive(int2 entry, int* arrived) {
+  return entry.y < 0 || atomicAdd(&arrived[entry.x], 1) == 1;
+}
+// block-aggregated append of up to two items per thread (one atomic per block); every thread of
+// the block must call it
+__device__ __forceinline__ void blockAppend2(bool t0, int4 i0, bool t1, int4 i1, int4* list,
+                                             int* count, int* shWave, int* shBase) {
+  const unsigned long long m0 = __ballot(t0), m1 = __ballot(t1);
+  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+  if (lane == 0) shWave[wave] = __popcll(m0) + __popcll(m1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      const int c = shWave[w];
+      shWave[w] = tot;
+      tot += c;
+    }
+    *shBase = tot ? atomicAdd(count, tot) : 0;
+  }
+  __syncthreads();
+  const int base = *shBase + shWave[wave];
+  const unsigned long long below = (1ull << lane) - 1ull;
+  if (t0) list[base + __popcll(m0 & below)] = i0;
+  if (t1) list[base + __popcll(m0) + __popcll(m1 & below)] = i1;
+  __syncthreads();  // the slots are reused by the next call
+}
+// incidence counts of the entering edges (deg[] is zero on entry); arrived[e] = 0
+__global__ void k_jpi_count(const int2* ends, const int* work, const int* count, int nHost,
+                            int* deg, int* arrived) {
+  const int n = count ? *count : nHost;
+  for (int t = gtid(); t < n; t += gridDim.x * blockDim.x) {
+    const int e = work[t];
+    const int2 ab = ends[e];
+    atomicAdd(&deg[ab.x], 1);
+    if (jpTwoBodies(ab)) atomicAdd(&deg[ab.y], 1);
+    arrived[e] = 0;
+  }
+}
+// every entering edge into the (unordered) lists of its bodies at start[]; deg[] returns to zero
+__global__ void k_jpi_fill(const int2* ends, const int* work, const int* count, int nHost,
+                           const int* start, int* deg, int* lists) {
+  const int n = count ? *count : nHost;
+  for (int t = gtid(); t < n; t += gridDim.x * blockDim.x) {
+    const int e = work[t];
+    const int2 ab = ends[e];
+    lists[start[ab.x] + atomicSub(&deg[ab.x], 1) - 1] = e;
+    if (jpTwoBodies(ab)) lists[start[ab.y] + atomicSub(&deg[ab.y], 1) - 1] = e;
+  }
+}
+// the list entry of edge f at body x
+__device__ __forceinline__ int2 jpEntry(const int2* ends, int f, int x) {
+  const int2 ab = ends[f];
+  return make_int2(f, jpTwoBodies(ab) ? (ab.x == x ? ab.y : ab.x) : -1);
+}
+// n <= R edges of body x sorted by descending priority in registers (a priority's low word is its
+// edge + 1); returns the first entry
+template <int R>
+__device__ __forceinline__ int2 jpiSortRegs(const int2* ends, const int* src, int2* dst, int n,
+                                            int x) {
+  unsigned long long v[R];
+#pragma unroll
+  for (int a = 0; a < R; ++a) v[a] = a < n ? colorKey1(src[a]) : 0ull;
+  // odd-even transposition sort (fixed indices: stays in registers)
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int a = r & 1; a + 1 < R; a += 2) {
+      const unsigned long long hi = v[a] > v[a + 1] ? v[a] : v[a + 1];
+      const unsigned long long lo = v[a] > v[a + 1] ? v[a + 1] : v[a];
+      v[a] = hi;
+      v[a + 1] = lo;
+    }
+  int2 head = make_int2(0, 0);
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+    if (a < n) {
+      const int2 en = jpEntry(ends, (int)(unsigned)(v[a] - 1ull), x);
+      dst[a] = en;
+      if (a == 0) head = en;
+    }
+  return head;
+}
+// One thread per body: sorts a list of up to 16 edges, sets the body's record (mask 0, cursor on
+// the first entry) and lists that entry's edge for round 0 if it leads at all its bodies; longer
+// lists go to longList for k_jpi_sort_long.
+__global__ __launch_bounds__(JPI_THREADS) void k_jpi_sort(const int2* ends, const int* start,
+                                                          const int* lists, int2* sorted,
+                                                          int numBodies, JpBody* body,
+                                                          int* arrived, int4* next,
+                                                          int* nextCount, int* longList,
+                                                          int* longCount) {
+  __shared__ int shWave[JPI_THREADS / 64], shBase, shCnt, shBase2;
+  // (no early exit: every thread reaches the block-aggregated appends)
+  const int b = gtid();
+  const int s = b < numBodies ? start[b] : 0;
+  const int n = b < numBodies ? start[b + 1] - s : 0;
+  bool lead = false, isLong = false;
+  int2 head = make_int2(0, 0);
+  if (n > 0) {
+    JpBody rec;
+    rec.mask = 0ull;
+    rec.cursor = s;
+    rec.end = s + n;
+    body[b] = rec;
+    if (n <= 8) {
+      head = jpiSortRegs<8>(ends, lists + s, sorted + s, n, b);
+    } else if (n <= 16) {
+      head = jpiSortRegs<16>(ends, lists + s, sorted + s, n, b);
+    } else {
+      isLong = true;
+    }
+    if (!isLong) lead = jpArrive(head, arrived);
+  }
+  blockAppend2(lead, make_int4(head.x, b, head.y, 0), false, int4{}, next, nextCount, shWave,
+               &shBase);
+  blockAppend(isLong, b, longList, longCount, &shCnt, &shBase2);
+}
+// One workgroup per long list (grid-stride over longList): each edge's place is the number of
+// the list's edges of higher priority, counted against the list staged through LDS in chunks.
+constexpr int JPI_CHUNK = 2048;
+constexpr int JPI_LONG_GRID = 512;
+__global__ __launch_bounds__(TPB) void k_jpi_sort_long(const int2* ends, const int* start,
+                                                       const int* lists, int2* sorted,
+                                                       const int* longList, const int* longCount,
+                                                       int* arrived, int4* next, int* nextCount) {
+  __shared__ unsigned long long sk[JPI_CHUNK];
+  const int nl = *longCount;
+  for (int i = blockIdx.x; i < nl; i += gridDim.x) {
+    const int b = longList[i];
+    const int s = start[b], n = start[b + 1] - s;
+    // block-uniform trip counts: every thread reaches each barrier
+    for (int base = 0; base < n; base += TPB) {
+      const int k = base + (int)threadIdx.x;
+      const unsigned long long key = k < n ? colorKey1(lists[s + k]) : 0ull;
+      int rank = 0;
+      for (int c0 = 0; c0 < n; c0 += JPI_CHUNK) {
+        const int m = min(JPI_CHUNK, n - c0);
+        __syncthreads();
+        for (int q = threadIdx.x; q < m; q += TPB) sk[q] = colorKey1(lists[s + c0 + q]);
+        __syncthreads();
+        if (k < n)
+          for (int q = 0; q < m; ++q) rank += sk[q] > key ? 1 : 0;
+      }
+      if (k < n) {
+        const int2 en = jpEntry(ends, (int)(unsigned)(key - 1ull), b);
+        sorted[s + rank] = en;
+        if (rank == 0 && jpArrive(en, arrived))
+          next[atomicAdd(nextCount, 1)] = make_int4(en.x, b, en.y, 0);
+      }
+    }
+  }
+}
+// One round: colour the listed edges, advance their bodies' cursors, list the next round's edges.
+__global__ __launch_bounds__(JPI_THREADS) void k_jpi_round(const int4* work, const int* count,
+                                                           int* colour, const int2* sorted,
+                                                           JpBody* body, int* arrived,
+                                                           int4* next, int* nextCount) {
+  __shared__ int shWave[JPI_THREADS / 64], shBase;
+  const int n = *count;
+  // block-uniform trip count: every thread reaches each block-aggregated append
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const int t = base + (int)threadIdx.x;
+    bool takeA = false, takeB = false;
+    int4 nA{}, nB{};
+    if (t < n) {
+      const int4 w = work[t];  // (edge, body, other body or -1)
+      const JpBody A = body[w.y];
+      JpBody B{};
+      if (w.z >= 0) B = body[w.z];
+      // the first colour free at both bodies; 62 when none of 0..61 is
+      const unsigned long long freeBits = ~(A.mask | B.mask);
+      const int c = freeBits ? min(__ffsll((long long)freeBits) - 1, 62) : 62;
+      colour[w.x] = c;
+      const unsigned long long bit = 1ull << c;
+      JpBody A1 = A;
+      A1.mask |= bit;
+      A1.cursor += 1;
+      body[w.y] = A1;
+      if (A1.cursor < A1.end) {
+        const int2 en = sorted[A1.cursor];
+        takeA = jpArrive(en, arrived);
+        nA = make_int4(en.x, w.y, en.y, 0);
+      }
+      if (w.z >= 0) {
+        JpBody B1 = B;
+        B1.mask |= bit;
+        B1.cursor += 1;
+        body[w.z] = B1;
+        if (B1.cursor < B1.end) {
+          const int2 en = sorted[B1.cursor];
+          takeB = jpArrive(en, arrived);
+          nB = make_int4(en.x, w.z, en.y, 0);
+        }
+      }
+    }
+    blockAppend2(takeA, nA, takeB, nB, next, nextCount, shWave, &shBase);
+  }
+}
 // Colour classes: pass 1 counts per (block, colour) with LDS atomics; k_color_offsets turns the
 // counts into per-block bases; pass 2 places every edge at base + its rank inside the block.  Both
 // passes walk the same contiguous chunk per block, so the per-block counts match and a class's

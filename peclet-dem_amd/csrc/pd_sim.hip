@@ -1138,6 +1138,8 @@ struct pd_sim {
 
   static constexpr int JP_MAX_BATCH = 32;
   static constexpr long JP_GRID_CAP = 2048;
+  // colourings of at least this many edges run their rounds over ordered incidence lists
+  long incMinEdges = 1L << 16;
   // One colouring in progress: its stream, its own work lists, counters, per-body winner / mask
   // slots, class buffers and pinned read-back area, so the manifold and contact colourings can
   // run on different streams at once.
@@ -1171,9 +1173,22 @@ struct pd_sim {
     const int* hostStarts = nullptr;
     const int* postSeqHost = nullptr;  // the read-back's sequence word and number
     int postSeq = 0;
+    // rounds over ordered incidence lists (k_jpi_*) instead of contend / commit rounds
+    bool inc = false;
+    DBuf<int> iDeg, iStart, iList, iArrived, iLong, iScan;
+    DBuf<int2> iSorted;
+    DBuf<JpBody> iBody;
+    DBuf<int4> iWork[2];
+    const int* hostEntering = nullptr;  // the entering count in the first read-back (device count)
+    long listSize = 0;                  // uncoloured edges entering the next round
+    long frontier = 0;                  // edges listed for the next batch's first round
+    std::vector<int> fhist, fseen;      // edges coloured per round, last / current colouring
     void release() {
       work[0].release(); work[1].release(); cnt.release(); blockCnt.release(); colCnt.release();
       colStart.release(); winner.release(); mask.release(); clsCnt.release();
+      iDeg.release(); iStart.release(); iList.release(); iSorted.release(); iBody.release();
+      iArrived.release(); iLong.release(); iScan.release(); iWork[0].release();
+      iWork[1].release();
     }
   };
   ColourJob jobV, jobP;  // manifold colouring (main stream), contact colouring (side stream)
@@ -1191,7 +1206,7 @@ struct pd_sim {
                    const int* enteringDev, int enteringHost, int* perm, int* rowOf) {
     J.winner.ensure(numBodies + 1, st);
     J.mask.ensure(numBodies + 1, st);
-    J.cnt.ensure(JP_MAX_BATCH + 1, st);
+    J.cnt.ensure(JP_MAX_BATCH + 2, st);  // (the last slot: the incidence path's long-list count)
     J.blockCnt.ensure(64 * CP_BLOCKS, st);
     J.colCnt.ensure(64, st);
     J.colStart.ensure(65, st);
@@ -1214,7 +1229,87 @@ struct pd_sim {
     J.enteringDev = enteringDev;
     J.first = true;
     J.noSmall = false;
-    colourLaunch(J);
+    J.inc = numEdges >= incMinEdges;
+    J.fseen.clear();
+    if (J.inc) {
+      J.globalDirty = true;  // the masks are left as this colouring's, the winner slots untouched
+      J.hostEntering = nullptr;
+      J.batch = std::min<int>(JP_MAX_BATCH, std::max<int>(4, (int)J.fhist.size() + 2));
+      colourLaunchInc(J);
+    } else {
+      colourLaunch(J);
+    }
+  }
+  // Enqueue one batch of incidence-list rounds (k_jpi_*), the class kernels and the read-back of
+  // the per-round counts and class starts.  The first batch builds the lists: incidence counts,
+  // their scan, the unordered lists, then the per-body sorts, which also list round 0's edges.
+  // cnt[j] is the size of round j's list; the host replays the reference loop on the counts.
+  void colourLaunchInc(ColourJob& J) {
+    hipStream_t s = J.stream;
+    const int B = (int)std::min<long>(J.batch, J.maxRounds - J.rounds);
+    J.B = B;
+    int* longCnt = J.cnt.p + JP_MAX_BATCH + 1;
+    auto jblk = [](long n) { return n > 0 ? (int)((n + JPI_THREADS - 1) / JPI_THREADS) : 1; };
+    if (J.first) {
+      const int nb = J.numBodies;
+      const long ne = J.numEdges;
+      J.iDeg.ensure(nb + 1, s);
+      J.iStart.ensure(nb + 1, s);
+      J.iBody.ensure(nb + 1, s);
+      J.iLong.ensure(nb + 1, s);
+      J.iList.ensure(2 * ne + 1, s);
+      J.iSorted.ensure(2 * ne + 1, s);
+      J.iArrived.ensure(ne + 1, s);
+      J.iWork[0].ensure(ne + 1, s);
+      J.iWork[1].ensure(ne + 1, s);
+      const int nbs = std::max(1, (nb + SCAN_TILE - 1) / SCAN_TILE);
+      J.iScan.ensure(nbs + 1, s);
+      const int gridE = (int)std::min<long>(nblk(ne), JP_GRID_CAP);
+      const int* work0 = J.work[0].p;
+      k_fill_i32<<<nblk(nb), TPB, 0, s>>>(J.iDeg.p, 0, nb);
+      k_jpi_count<<<gridE, TPB, 0, s>>>(J.ends, work0, J.enteringDev, (int)J.entering, J.iDeg.p,
+                                        J.iArrived.p);
+      if (nbs > 1) k_scan_reduce<<<nbs, TPB, 0, s>>>(J.iDeg.p, nb, J.iScan.p, nullptr);
+      k_scan_down2<<<nbs, TPB, 0, s>>>(J.iDeg.p, nb, J.iScan.p, J.iStart.p, nullptr);
+      k_jpi_fill<<<gridE, TPB, 0, s>>>(J.ends, work0, J.enteringDev, (int)J.entering,
+                                       J.iStart.p, J.iDeg.p, J.iList.p);
+      k_jp_counts<<<1, 64, 0, s>>>(J.cnt.p, nullptr, 0, JP_MAX_BATCH + 2);
+      J.cur = 0;
+      k_jpi_sort<<<jblk(nb), JPI_THREADS, 0, s>>>(J.ends, J.iStart.p, J.iList.p, J.iSorted.p, nb,
+                                                  J.iBody.p, J.iArrived.p, J.iWork[0].p, J.cnt.p,
+                                                  J.iLong.p, longCnt);
+      k_jpi_sort_long<<<JPI_LONG_GRID, TPB, 0, s>>>(J.ends, J.iStart.p, J.iList.p, J.iSorted.p,
+                                                    J.iLong.p, longCnt, J.iArrived.p,
+                                                    J.iWork[0].p, J.cnt.p);
+    } else {
+      k_jp_counts<<<1, 64, 0, s>>>(J.cnt.p, nullptr, (int)J.frontier, B + 1);
+    }
+    for (int j = 0; j < B; ++j) {
+      const int r = J.rounds + j;
+      long est = std::max<long>(1, J.numEdges / 8);
+      if (r < (int)J.fhist.size()) est = J.fhist[r];
+      else if (j == 0 && !J.first) est = J.frontier;
+      const int grid = (int)std::min<long>(jblk(est + est / 4 + 64), JP_GRID_CAP / 4);
+      k_jpi_round<<<grid, JPI_THREADS, 0, s>>>(J.iWork[(J.cur + j) & 1].p, J.cnt.p + j, J.colour,
+                                               J.iSorted.p, J.iBody.p, J.iArrived.p,
+                                               J.iWork[(J.cur + j + 1) & 1].p, J.cnt.p + j + 1);
+    }
+    const int nbc = std::min(nblk(J.numEdges), CP_BLOCKS);
+    k_color_count<<<nbc, TPB, 0, s>>>(J.colour, J.numEdges, J.blockCnt.p);
+    k_color_offsets<<<64, TPB, 0, s>>>(J.blockCnt.p, nbc, J.colCnt.p);
+    k_color_starts<<<1, 64, 0, s>>>(J.colCnt.p, J.colStart.p);
+    k_color_place<<<nbc, TPB, 0, s>>>(J.colour, J.numEdges, J.blockCnt.p, J.colStart.p, J.perm,
+                                      J.rowOf);
+    HIPCHECK(hipGetLastError());
+    PostBatch rb = postBatch(J.pinned, J.pinnedDev, 256);
+    J.hostCnt = rb.add(J.cnt.p, B + 1);
+    J.hostStarts = rb.add(J.colStart.p, 65);
+    if (J.first && J.enteringDev) J.hostEntering = rb.add(J.enteringDev, 1);
+    rb.post(s);
+    J.postSeqHost = rb.seqHost;
+    J.postSeq = rb.seq;
+    if (J.first) J.listSize = J.enteringDev ? -1 : J.entering;
+    J.first = false;
   }
   // enqueue one batch of rounds, the class kernels and the read-back of counts and class starts
   void colourLaunch(ColourJob& J) {
@@ -1294,7 +1389,34 @@ struct pd_sim {
   // Returns the uncoloured leftovers after the round cap.
   int colourFinish(ColourJob& J, std::vector<int>& starts, int& numColors, int& rounds,
                    long long& uncol) {
-    for (;;) {
+    while (J.inc) {
+      waitPosted(J.postSeqHost, J.postSeq, J.stream);
+      if (J.listSize < 0) J.listSize = *J.hostEntering;
+      for (int j = 0; j < J.B && J.remaining > 0; ++j) {
+        const long size = J.listSize;
+        J.uncol += size;
+        J.seen.push_back((int)size);
+        J.fseen.push_back(J.hostCnt[j]);
+        ++J.rounds;
+        const long rem = size - J.hostCnt[j];
+        if (rem == J.prevRemaining && rem > 0)
+          throw std::runtime_error("colouring stalled: a round coloured no edge");
+        J.prevRemaining = (int)rem;
+        J.remaining = (int)rem;
+        J.listSize = rem;
+      }
+      J.frontier = J.hostCnt[J.B];
+      J.entering = J.remaining;
+      J.cur = (J.cur + J.B) & 1;
+      if (J.remaining <= 0 || J.rounds >= J.maxRounds) {
+        starts.assign(J.hostStarts, J.hostStarts + 65);
+        J.fhist.swap(J.fseen);
+        break;
+      }
+      J.batch = std::min(J.batch * 2, JP_MAX_BATCH);
+      colourLaunchInc(J);
+    }
+    for (; !J.inc;) {
       waitPosted(J.postSeqHost, J.postSeq, J.stream);
       // a small-colouring kernel that could not take its list changed nothing: replay the
       // rounds before it and continue with launch-per-round batches
@@ -3716,6 +3838,9 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     } else if (k == "serial_class_wave_launch_rows") {
       if (value < 1) throw ArgError("serial_class_wave_launch_rows must be >= 1");
       s->waveLaunchRows = value;
+    } else if (k == "colouring_incidence_min_edges") {
+      if (value < 0) throw ArgError("colouring_incidence_min_edges must be >= 0");
+      s->incMinEdges = value;
     } else if (k == "velocity_pack_bodies") {
       if (value != 0 && value != 1) throw ArgError("velocity_pack_bodies must be 0 or 1");
       s->packBodies = value == 1;

@@ -173,6 +173,21 @@ def test_bed_small_periodic_box_twins(hip_lib):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kind,n,steps", [("bed", 300, 12), ("bed_periodic", 4000, 3),
+                                          ("bed_small_box", 24, 10), ("gas", 300, 10),
+                                          ("cylinders", 27, 6), ("big_among_small", 4000, 3)])
+def test_colouring_by_incidence_lists(hip_lib, kind, n, steps):
+    # every colouring (manifolds and contacts) runs its rounds over per-body edge lists sorted by
+    # priority; bodies with more than 16 edges (the big sphere, cylinders touching at many shell
+    # points) have their lists sorted by one workgroup each.  Colours, round counts and the
+    # per-round uncoloured totals are compared with the oracle's every substep.
+    log = []
+    run_pair(kind, n, steps, tuning={"colouring_incidence_min_edges": 0}, log=log)
+    if kind == "big_among_small":
+        assert log[0]["num_colors_pos"] == 63
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("block_rows", [None, 0])
 def test_gas_one_shot_gs(hip_lib, block_rows):
     run_pair("gas", 300, 20, block_rows=block_rows)
