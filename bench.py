@@ -479,7 +479,8 @@ def main():
                     help="also run a plain copy of MIB MiB (scales the HBM byte counters)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--tuning", action="append", default=[], metavar="NAME=VALUE",
-                    help="any library tuning knob (pd_set_tuning) for the headline")
+                    help="any library tuning knob (pd_set_tuning) for the headline, or for "
+                         "the leg that --only runs")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the headline and the batch with the library's default knobs "
                          "instead of autotuning them in the untimed warm-up")
@@ -526,7 +527,7 @@ def main():
     if args.only == "pour":
         sim, _, r = plain_leg("pour", 1048576, args.seed, args.column, dt, 8,
                               args.pour_steps * SUBSTEPS_PER_STEP, args.timing_every,
-                              args.timing_level)
+                              args.timing_level, tuning)
         sim.close()
         if args.copy_probe:
             copy_probe(args.copy_probe)
@@ -534,12 +535,12 @@ def main():
         return
     if args.only == "cylinders":
         _, _, r = plain_leg("cylinders", 200000, 3, args.column, dt, SUBSTEPS_PER_STEP,
-                            args.cyl_steps * SUBSTEPS_PER_STEP, 1, args.timing_level)
+                            args.cyl_steps * SUBSTEPS_PER_STEP, 1, args.timing_level, tuning)
         print(json.dumps(r))
         return
     if args.only == "cylinders_packed":
         _, _, r = plain_leg("cylinders_packed", 200000, 3, args.column, dt, 1,
-                            args.packed_substeps, 1, args.timing_level)
+                            args.packed_substeps, 1, args.timing_level, tuning)
         print(json.dumps(r))
         return
     if args.only == "hertz":

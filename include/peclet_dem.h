@@ -260,9 +260,10 @@ int pd_set_timing(pd_sim* s, int level);
  * contact colour class (62, whose rows may share bodies) of at least that many rows runs in
  * waves of body-disjoint rows instead of one row after another, a wave of at least
  * "serial_class_wave_launch_rows" rows (default 512) in a launch of its own and runs of smaller
- * waves in one workgroup; "colouring_incidence_min_edges" (default 65536): a colouring of at
- * least that many edges runs its Jones-Plassmann rounds over per-body edge lists sorted by
- * priority (one visit per edge) instead of contend / commit passes over every uncoloured edge.
+ * waves in one workgroup; "colouring_incidence_min_edges" (default 65536): a colouring whose
+ * uncoloured edges number at least that many and at least a quarter of its bodies runs its
+ * Jones-Plassmann rounds over per-body edge lists sorted by priority (one visit per edge)
+ * instead of contend / commit passes over every uncoloured edge (0: every colouring does).
  * Results do not depend on any of them. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 /* Autotuner (no reference counterpart): chooses candidate_skin_permille, block_solver_rows,

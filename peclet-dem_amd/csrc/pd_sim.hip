@@ -1138,7 +1138,7 @@ struct pd_sim {
 
   static constexpr int JP_MAX_BATCH = 32;
   static constexpr long JP_GRID_CAP = 2048;
-  // colourings of at least this many edges run their rounds over ordered incidence lists
+  // large colourings run their rounds over ordered incidence lists (colourBegin has the rule)
   long incMinEdges = 1L << 16;
   // One colouring in progress: its stream, its own work lists, counters, per-body winner / mask
   // slots, class buffers and pinned read-back area, so the manifold and contact colourings can
@@ -1229,7 +1229,12 @@ struct pd_sim {
     J.enteringDev = enteringDev;
     J.first = true;
     J.noSmall = false;
-    J.inc = numEdges >= incMinEdges;
+    // the incidence path pays passes over every body: it is taken for colourings whose entering
+    // edges (this one's for a host count, else the job's last colouring's) number at least
+    // incMinEdges and a quarter of the bodies; incMinEdges = 0 takes it for every colouring
+    const long estEntering =
+        enteringDev ? (J.hist.empty() ? (long)numEdges : (long)J.hist[0]) : (long)enteringHost;
+    J.inc = incMinEdges == 0 || (estEntering >= incMinEdges && 4 * estEntering >= numBodies);
     J.fseen.clear();
     if (J.inc) {
       J.globalDirty = true;  // the masks are left as this colouring's, the winner slots untouched
