@@ -251,10 +251,14 @@ def run_batched(num, seed, steps, seq_steps, workers, autotune=True):
                         "r~U[0.8,1.2] mm, mu~U[0.2,0.8], e~U[0.1,0.6], phi0=0.1, dt=1e-5 s"}
 
 
+ADJOINT_REPEATS = 3  # timed runs per form; the line reports their medians
+
+
 def run_adjoint(n, substeps, every, warm, dense):
     """Forward + adjoint form: particle-substeps/s of a recorded forward run plus its adjoint
     (gradient of the mean bed height w.r.t. initial velocities, friction and restitution),
-    against the plain forward run of the same substeps from the same state."""
+    against the plain forward run of the same substeps from the same state (medians of
+    ADJOINT_REPEATS timed runs, after one untimed recording and adjoint)."""
     from peclet.dem import Simulation
     from peclet.dem.workloads import lattice_bed, setup_gravity_bed, sphere_bed
 
@@ -276,30 +280,44 @@ def run_adjoint(n, substeps, every, warm, dense):
         sim.synchronize()
         return sim
 
-    sim = fresh()
-    t0 = time.perf_counter()
-    sim.step_n(dt, substeps)
-    sim.synchronize()
-    t_plain = time.perf_counter() - t0
-    sim.close()
-    sim = fresh()
-    t0 = time.perf_counter()
-    sim.tape_begin(every)
-    sim.step_n(dt, substeps)
-    sim.tape_end()
-    t_fwd = time.perf_counter() - t0
     grad = np.zeros((n, 13))
     grad[:, 2] = 1.0 / n
-    t0 = time.perf_counter()
-    g0, gp = sim.tape_adjoint(grad)
-    t_adj = time.perf_counter() - t0
-    contacts = sim.num_contacts()
-    fwd_bytes = substep_bytes(sim.get_stats()) * substeps
+
+    def recorded(sim):
+        t0 = time.perf_counter()
+        sim.tape_begin(every)
+        sim.step_n(dt, substeps)
+        sim.tape_end()
+        sim.synchronize()
+        t1 = time.perf_counter()
+        g = sim.tape_adjoint(grad)
+        return t1 - t0, time.perf_counter() - t1, g
+
+    # one untimed recording and adjoint first (first launches of the adjoint kernels, workspace
+    # allocation), then the median of `REPEATS` timed runs of each form, each from a fresh start
+    sim = fresh()
+    recorded(sim)
     sim.close()
+    plain, fwd, adj = [], [], []
+    for _ in range(ADJOINT_REPEATS):
+        sim = fresh()
+        t0 = time.perf_counter()
+        sim.step_n(dt, substeps)
+        sim.synchronize()
+        plain.append(time.perf_counter() - t0)
+        sim.close()
+        sim = fresh()
+        tf, ta, (g0, gp) = recorded(sim)
+        fwd.append(tf)
+        adj.append(ta)
+        contacts = sim.num_contacts()
+        fwd_bytes = substep_bytes(sim.get_stats()) * substeps
+        sim.close()
+    t_plain, t_fwd, t_adj = (float(np.median(x)) for x in (plain, fwd, adj))
     return {"value": n * substeps / (t_fwd + t_adj), "unit": "particle-substeps/s",
             "workload": what + f", dt=1e-5 s, mu=0.5, e=0.3, {warm} substeps before the "
                                f"recording, {substeps} recorded, checkpoint every {every}",
-            "contacts_at_end": contacts, "forward_ms": 1e3 * t_plain,
+            "repeats": ADJOINT_REPEATS, "contacts_at_end": contacts, "forward_ms": 1e3 * t_plain,
             "recorded_forward_ms": 1e3 * t_fwd, "adjoint_ms": 1e3 * t_adj,
             "cost_vs_forward": (t_fwd + t_adj) / t_plain,
             # the forward substeps' algorithmic bytes (last substep's counters) over the whole

@@ -328,18 +328,37 @@ void replayStep(pd_sim& s, const TapeStep& S, ad::RB<T> B, double* vtape, double
   HIPCHECK(hipGetLastError());
 }
 
+// the per-substep adjoint buffers, carved from one allocation so that one fill clears them all
+struct GCarve {
+  double *xs, *xt, *v0, *vh, *q0, *vc, *xr, *qr, *cg, *ms, *rc, *lam, *pc, *pl;
+  size_t total;  // doubles used
+};
+inline GCarve carveG(double* base, size_t slots, size_t nc, size_t nm, size_t rows, size_t prows) {
+  GCarve c{};
+  size_t o = 0;
+  // each piece starts on a 64-byte boundary
+  auto take = [&](size_t n) {
+    double* p = base ? base + o : nullptr;
+    o += (n + 7) / 8 * 8;
+    return p;
+  };
+  c.xs = take(3 * slots); c.xt = take(3 * slots); c.v0 = take(3 * slots); c.vh = take(6 * slots);
+  c.q0 = take(4 * slots); c.vc = take(6 * slots); c.xr = take(3 * slots); c.qr = take(4 * slots);
+  c.cg = take(10 * nc); c.ms = take(15 * nm); c.rc = take(22 * rows); c.lam = take(4 * rows);
+  c.pc = take(10 * prows); c.pl = take(prows);
+  c.total = o;
+  return c;
+}
 struct GWS {  // adjoint workspace
-  DBuf<double> X, xs, xt, v0, vh, q0, vc, xr, qr, cg, ms, rc, lam, pc, pl, La, Lb, par;
+  DBuf<double> X, Z, La, Lb, par;
   DBuf<double> vtape, ptape;
   size_t nL = 1;
   void ensure(const AdjTape& A, hipStream_t st) {
-    size_t slots = 1, ncm = 1, nmm = 1, rows = 1, prows = 1, vt = 1, pt = 1;
+    size_t nmm = 1, vt = 1, pt = 1, z = 1;
     for (const TapeStep& S : A.steps) {
-      slots = std::max(slots, (size_t)S.t.nSlots);
-      ncm = std::max(ncm, (size_t)S.t.nc);
+      z = std::max(z, carveG(nullptr, S.t.nSlots, std::max(1, S.t.nc), std::max(1, S.t.nm),
+                             std::max(1, S.t.nRows), std::max(1, S.t.nPRows)).total);
       nmm = std::max(nmm, (size_t)S.t.nm);
-      rows = std::max(rows, (size_t)S.t.nRows);
-      prows = std::max(prows, (size_t)S.t.nPRows);
       const size_t passes = 1 + S.sweepsV + (S.stabRan ? S.sweepsS : 0);
       vt = std::max(vt, passes * 16 * (size_t)S.t.nRows);
       pt = std::max(pt, (size_t)S.sweepsP * 7 * (size_t)S.t.nPRows);
@@ -347,17 +366,12 @@ struct GWS {  // adjoint workspace
     for (const TapeCkpt& c : A.ckpts) nmm = std::max(nmm, (size_t)c.nL);
     nL = nmm;
     X.ensure(13 * (size_t)std::max(1, A.numReal), st);
-    xs.ensure(3 * slots, st); xt.ensure(3 * slots, st); v0.ensure(3 * slots, st);
-    vh.ensure(6 * slots, st); q0.ensure(4 * slots, st); vc.ensure(6 * slots, st);
-    xr.ensure(3 * slots, st); qr.ensure(4 * slots, st); cg.ensure(10 * ncm, st);
-    ms.ensure(15 * nmm, st); rc.ensure(22 * rows, st); lam.ensure(4 * rows, st);
-    pc.ensure(10 * prows, st); pl.ensure(prows, st); La.ensure(5 * nmm, st);
+    Z.ensure(z, st);
+    La.ensure(5 * nmm, st);
     Lb.ensure(5 * nmm, st); par.ensure(2, st); vtape.ensure(vt, st); ptape.ensure(pt, st);
   }
   void release() {
-    X.release(); xs.release(); xt.release(); v0.release(); vh.release(); q0.release();
-    vc.release(); xr.release(); qr.release(); cg.release(); ms.release(); rc.release();
-    lam.release(); pc.release(); pl.release(); La.release(); Lb.release(); par.release();
+    X.release(); Z.release(); La.release(); Lb.release(); par.release();
     vtape.release(); ptape.release();
   }
 };
@@ -370,18 +384,14 @@ void reverseStep(pd_sim& s, const TapeStep& S, ad::RB<double> B, GWS& W, double*
   const ad::RK K = constants(*s.tape, S);
   const ad::Topo& t = S.t;
   const int nR = K.P.numReal;
-  const size_t sl = (size_t)t.nSlots;
-  s.zero(W.xs.p, 3 * sl); s.zero(W.xt.p, 3 * sl); s.zero(W.v0.p, 3 * sl);
-  s.zero(W.vh.p, 6 * sl); s.zero(W.q0.p, 4 * sl); s.zero(W.vc.p, 6 * sl);
-  s.zero(W.xr.p, 3 * sl); s.zero(W.qr.p, 4 * sl);
-  s.zero(W.cg.p, 10 * (size_t)std::max(1, t.nc)); s.zero(W.ms.p, 15 * (size_t)std::max(1, t.nm));
-  s.zero(W.rc.p, 22 * (size_t)std::max(1, t.nRows)); s.zero(W.lam.p, 4 * (size_t)std::max(1, t.nRows));
-  s.zero(W.pc.p, 10 * (size_t)std::max(1, t.nPRows)); s.zero(W.pl.p, (size_t)std::max(1, t.nPRows));
+  const GCarve C = carveG(W.Z.p, (size_t)t.nSlots, std::max(1, t.nc), std::max(1, t.nm),
+                         std::max(1, t.nRows), std::max(1, t.nPRows));
+  s.zero(W.Z.p, C.total);
   s.zero(gLp, 5 * (size_t)std::max(1, nLp));
   ad::GB G{};
-  G.X = W.X.p; G.xs = W.xs.p; G.xt = W.xt.p; G.v0 = W.v0.p; G.vh = W.vh.p; G.q0 = W.q0.p;
-  G.vc = W.vc.p; G.xr = W.xr.p; G.qr = W.qr.p; G.cg = W.cg.p; G.ms = W.ms.p; G.rc = W.rc.p;
-  G.lam = W.lam.p; G.pc = W.pc.p; G.pl = W.pl.p; G.L = gL; G.Lp = gLp;
+  G.X = W.X.p; G.xs = C.xs; G.xt = C.xt; G.v0 = C.v0; G.vh = C.vh; G.q0 = C.q0;
+  G.vc = C.vc; G.xr = C.xr; G.qr = C.qr; G.cg = C.cg; G.ms = C.ms; G.rc = C.rc;
+  G.lam = C.lam; G.pc = C.pc; G.pl = C.pl; G.L = gL; G.Lp = gLp;
   if (t.nm > 0) ad::b_ledger<<<nb(t.nm), TPB, 0, st>>>(B, G, K, t);
   ad::b_final<<<nb(nR), TPB, 0, st>>>(G, K);
   if (t.nPRows > 0) {

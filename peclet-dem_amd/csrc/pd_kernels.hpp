@@ -242,6 +242,14 @@ __global__ void k_fill_u64(unsigned long long* a, unsigned long long v, int n) {
   const int i = gtid();
   if (i < n) a[i] = v;
 }
+// zero two arrays of n elements in one launch
+__global__ void k_zero2_u64(unsigned long long* a, unsigned long long* b, int n) {
+  const int i = gtid();
+  if (i < n) {
+    a[i] = 0ull;
+    b[i] = 0ull;
+  }
+}
 __global__ void k_fill_u8(unsigned char* a, unsigned char v, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -339,6 +347,49 @@ __global__ void k_scan_down2(const int* in, int n, const int* blockSums, int* ou
       if (base + k < n) out[base + k] = res[k];
   }
   if ((int)blockIdx.x == last && threadIdx.x == 0) out[n] = prefix + tot;
+}
+// The same exclusive scan (total into out[n]) in one workgroup of SCAN1_THREADS, for lists of up
+// to SCAN1_MAX elements: tiles of 4 elements per thread with a running carry, so a mid-sized scan
+// costs one launch instead of two.
+constexpr int SCAN1_THREADS = 1024;
+constexpr int SCAN1_MAX = 32768;
+__global__ __launch_bounds__(SCAN1_THREADS) void k_scan_single(const int* in, int n, int* out,
+                                                               const int* nDev) {
+  __shared__ int waveTot[SCAN1_THREADS / 64];
+  if (nDev) n = *nDev;
+  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+  int carry = 0;
+  for (int base = 0; base < n; base += SCAN1_THREADS * 4) {
+    const int i0 = base + (int)threadIdx.x * 4;
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? in[i0 + k] : 0;
+    const int own = v[0] + v[1] + v[2] + v[3];
+    // inclusive scan across the wave, then across the waves' totals
+    int inc = own;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int x = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += x;
+    }
+    if (lane == 63) waveTot[wave] = inc;
+    __syncthreads();
+    int before = 0, tot = 0;
+    for (int w = 0; w < SCAN1_THREADS / 64; ++w) {
+      const int t = waveTot[w];
+      if (w < wave) before += t;
+      tot += t;
+    }
+    int ex = carry + before + inc - own;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (i0 + k < n) out[i0 + k] = ex;
+      ex += v[k];
+    }
+    carry += tot;
+    __syncthreads();  // waveTot is rewritten by the next tile
+  }
+  if (threadIdx.x == 0) out[n] = carry;
 }
 
 // ============================================================================================
@@ -2146,6 +2197,51 @@ __global__ void k_ground_sweep_list(const Dev* __restrict__ Dp, const int* list,
       const int lvl = gOld[r.x] - 1;
       if (lvl > 0) atomicMax(&gNew[r.y], lvl);
     }
+  }
+}
+// The seed and the eight sweeps above in one workgroup (a short support list): each sweep is
+// separated from the next by a workgroup barrier, so every sweep reads the levels its
+// predecessor finished, as the launch-per-sweep form does; the result ends in `ga` (eight
+// swaps).  The levels a sweep reads were written by atomics of the previous sweep, so they are
+// read with device-scope loads (no stale copies in the vector cache).
+constexpr int GROUND_BLOCK_THREADS = 1024;
+constexpr int GROUND_BLOCK_MAX = 16384;  // manifolds (the support list's bound)
+__device__ __forceinline__ int loadLevel(const int* g, int i) {
+  return __hip_atomic_load(g + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__global__ __launch_bounds__(GROUND_BLOCK_THREADS) void k_ground_block(
+    const Dev* __restrict__ Dp, const int* list, const int* count, int* ga, int* gb) {
+  const Dev& D = *Dp;
+  const int n = *count;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const int2 r = D.mReal[list[q]];
+    gb[r.x] = ga[r.x];
+    if (r.y >= 0) gb[r.y] = ga[r.y];
+  }
+  __syncthreads();
+  int* gOld = ga;
+  int* gNew = gb;
+  for (int sweep = 0; sweep < 8; ++sweep) {
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      const int m = list[q];
+      const int2 r = D.mReal[m];
+      atomicMax(&gNew[r.x], loadLevel(gOld, r.x));
+      if (r.y >= 0) atomicMax(&gNew[r.y], loadLevel(gOld, r.y));
+      const unsigned char code = D.mCode[m];
+      if (code == 3) {
+        atomicMax(&gNew[r.x], 255);
+      } else if (code == 1) {
+        const int lvl = loadLevel(gOld, r.y) - 1;
+        if (lvl > 0) atomicMax(&gNew[r.x], lvl);
+      } else {
+        const int lvl = loadLevel(gOld, r.x) - 1;
+        if (lvl > 0) atomicMax(&gNew[r.y], lvl);
+      }
+    }
+    __syncthreads();
+    int* t = gOld;
+    gOld = gNew;
+    gNew = t;
   }
 }
 __global__ void k_ground_sweep(const Dev* __restrict__ Dp, int numManifolds, const int* gOld, int* gNew) {

@@ -584,6 +584,11 @@ struct pd_sim {
       return;
     }
     const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb > 1 && n <= SCAN1_MAX) {
+      k_scan_single<<<1, SCAN1_THREADS, 0, st>>>(in, n, out, nullptr);
+      HIPCHECK(hipGetLastError());
+      return;
+    }
     scanTmp.ensure(nb + 1, st);
     if (nb > 1) k_scan_reduce<<<nb, TPB, 0, st>>>(in, n, scanTmp.p, nullptr);
     k_scan_down2<<<nb, TPB, 0, st>>>(in, n, scanTmp.p, out, nullptr);
@@ -592,6 +597,11 @@ struct pd_sim {
   // the same scan over a count that lives on the device (*nDev <= nMax)
   void scanDev(const int* in, int* out, const int* nDev, int nMax) {
     const int nb = std::max(1, (nMax + SCAN_TILE - 1) / SCAN_TILE);
+    if (nb > 1 && nMax <= SCAN1_MAX) {
+      k_scan_single<<<1, SCAN1_THREADS, 0, st>>>(in, nMax, out, nDev);
+      HIPCHECK(hipGetLastError());
+      return;
+    }
     scanTmp.ensure(nb + 1, st);
     if (nb > 1) k_scan_reduce<<<nb, TPB, 0, st>>>(in, nMax, scanTmp.p, nDev);
     k_scan_down2<<<nb, TPB, 0, st>>>(in, nMax, scanTmp.p, out, nDev);
@@ -1346,8 +1356,7 @@ struct pd_sim {
     const bool place = J.first && J.smallAt == 0;
     const int loadGlobal = J.globalDirty ? 0 : 1;
     if (launchRounds && J.globalDirty) {
-      HIPCHECK(hipMemsetAsync(J.mask.p, 0, J.numBodies * sizeof(unsigned long long), s));
-      HIPCHECK(hipMemsetAsync(J.winner.p, 0, J.numBodies * sizeof(unsigned long long), s));
+      k_zero2_u64<<<nblk(J.numBodies), TPB, 0, s>>>(J.mask.p, J.winner.p, J.numBodies);
     }
     // after this batch the global slots are valid for every body a later round can touch: zeroed
     // above, or written back by the LDS kernel for the bodies of the edges it leaves
@@ -2129,7 +2138,10 @@ struct pd_sim {
       // in the predict launch
       int* ga = ground.p;
       int* gb = ground2.p;
-      if (nm > 0) {
+      if (nm > 0 && nm <= GROUND_BLOCK_MAX) {
+        // a short support list: the seed and the eight sweeps in one workgroup
+        k_ground_block<<<1, GROUND_BLOCK_THREADS, 0, st>>>(dD, supList.p, supCnt.p, ga, gb);
+      } else if (nm > 0) {
         const int gg = std::min(nblk(nm), 1024);
         k_ground_seed<<<gg, TPB, 0, st>>>(dD, supList.p, supCnt.p, ga, gb);
         for (int s = 0; s < 8; ++s) {
