@@ -263,7 +263,9 @@ int pd_set_timing(pd_sim* s, int level);
  * waves in one workgroup; "colouring_incidence_min_edges" (default 65536): a colouring whose
  * uncoloured edges number at least that many and at least a quarter of its bodies runs its
  * Jones-Plassmann rounds over per-body edge lists sorted by priority (one visit per edge)
- * instead of contend / commit passes over every uncoloured edge (0: every colouring does).
+ * instead of contend / commit passes over every uncoloured edge (0: every colouring does);
+ * "position_tail_rows" (default 2048): in per-colour position sweeps, the trailing contact colours
+ * holding at most that many rows together run in one workgroup launch that also ends the sweep.
  * Results do not depend on any of them. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 /* Autotuner (no reference counterpart): chooses candidate_skin_permille, block_solver_rows,

@@ -2798,13 +2798,16 @@ enum { CTL_STOP_V = 0, CTL_SWEEPS_V, CTL_STAB_RUN, CTL_STOP_S, CTL_SWEEPS_S, CTL
 __global__ void k_sweep_begin(unsigned long long* bits, const int* stop) {
   if (!*stop) *bits = 0ull;
 }
-// after one sweep: count it and stop once its max is within the tolerance
-__global__ void k_sweep_check(const unsigned long long* bits, int* stop, int* count, double thr,
-                              int strict) {
+// after one sweep: count it and stop once its max is within the tolerance; with `next` set (a
+// sweep of the same loop follows) a sweep that does not stop also clears the accumulator for the
+// next one, the work of k_sweep_begin
+__global__ void k_sweep_check(unsigned long long* bits, int* stop, int* count, double thr,
+                              int strict, int next) {
   if (*stop) return;
   *count += 1;
   const double m = bitsd(*bits);
   if (strict ? (m < thr) : (m <= thr)) *stop = 1;
+  else if (next) *bits = 0ull;
 }
 // stabilization gate (src/sim.hpp:279-283): run iff mode != 0 and the residual exceeds vRestS
 __global__ void k_stab_gate(const unsigned long long* bits, int* ctl, int mode, double vRestS,

@@ -1889,7 +1889,8 @@ struct pd_sim {
     writeN(starts.p, waveCnt.data(), depth + 1);
     return depth;
   }
-  static constexpr int POS_TAIL_ROWS = 8192;
+  // trailing contact colours holding at most this many rows together run in one workgroup
+  long posTailRows = 2048;
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
     vbActive = nullptr;  // a substep cut short by an error may have left the stage's records
@@ -2218,12 +2219,13 @@ struct pd_sim {
               ctl.p + CTL_SWEEPS_V, nullptr);
         vBlockRec[0] = kernEnd(K_VBLOCK, nRowsV);
       } else if (!smallV) sweepLoop(velIters, ctl.p + CTL_STOP_V, [&, sv = 0]() mutable {
-        k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V);
+        // (each check clears the accumulator for the sweep after it)
+        if (sv == 0) k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V);
         curSweepTag = sv++;
         velocitySweep(vs, numColors, usePGS ? 1 : 2, vRest, ctl.p + CTL_STOP_V);
         curSweepTag = -1;
         k_sweep_check<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_V, ctl.p + CTL_SWEEPS_V,
-                                       stopThr, 0);
+                                       stopThr, 0, sv < velIters ? 1 : 0);
       });
       mark(ST_STAB);
       if (usePGS) {
@@ -2246,12 +2248,12 @@ struct pd_sim {
                                                        8.0 * dt * gMagP, ctl.p + CTL_STAB_RUN,
                                                        nullptr, vbActive);
           sweepLoop(2 * velIters, ctl.p + CTL_STOP_S, [&, ss = 0]() mutable {
-            k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S);
+            if (ss == 0) k_sweep_begin<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S);
             curSweepTag = 4096 + ss++;
             velocitySweep(vs, numColors, 1, vRestS, ctl.p + CTL_STOP_S);
             curSweepTag = -1;
             k_sweep_check<<<1, 1, 0, st>>>(bits.p + 1, ctl.p + CTL_STOP_S, ctl.p + CTL_SWEEPS_S,
-                                           vRestS, 0);
+                                           vRestS, 0, ss < 2 * velIters ? 1 : 0);
           });
         }
       }
@@ -2409,10 +2411,10 @@ struct pd_sim {
       // batches of device-gated sweeps, the first as long as the previous substep's loop; each
       // batch ends with one read of the loop counters and residuals, which is also the substep's
       // final read-back once the loop has stopped
-      // the trailing colours holding at most POS_TAIL_ROWS rows together run in one workgroup
+      // the trailing colours holding at most posTailRows rows together run in one workgroup
       // launch that also ends the sweep (stop check, accumulator cleared for the next sweep)
       int cFrom = numPosColors;
-      while (cFrom > 0 && ps[numPosColors] - ps[cFrom - 1] <= POS_TAIL_ROWS) --cFrom;
+      while (cFrom > 0 && ps[numPosColors] - ps[cFrom - 1] <= posTailRows) --cFrom;
       const ClassStarts pcs = classStarts(ps);
       int waves62 = 0;
       if (numPosColors == 63 && ps[63] - ps[62] >= std::max(1L, waveMinRows)) {
@@ -3858,6 +3860,9 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     } else if (k == "colouring_incidence_min_edges") {
       if (value < 0) throw ArgError("colouring_incidence_min_edges must be >= 0");
       s->incMinEdges = value;
+    } else if (k == "position_tail_rows") {
+      if (value < 0) throw ArgError("position_tail_rows must be >= 0");
+      s->posTailRows = value;
     } else if (k == "velocity_pack_bodies") {
       if (value != 0 && value != 1) throw ArgError("velocity_pack_bodies must be 0 or 1");
       s->packBodies = value == 1;

@@ -144,6 +144,14 @@ def test_per_colour_launches_body_records(hip_lib, kind, pack):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tail", [0, 64, 100000])
+def test_position_tail_split(hip_lib, tail):
+    # per-colour position sweeps with no trailing colours in the one-workgroup tail launch (0),
+    # the last small colours there (64) and every colour there (100000)
+    run_pair("bed_periodic", 400, 12, tuning={"block_solver_rows": 0, "position_tail_rows": tail})
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tuning", [{}, {"block_solver_rows": 0}, {"block_solver_rows": 0,
                                      "serial_class_waves_min_rows": 1}],
                          ids=["default", "per-colour", "per-colour-waves"])
