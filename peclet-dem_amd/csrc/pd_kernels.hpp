@@ -1860,6 +1860,40 @@ __global__ __launch_bounds__(TPB) void k_jpi_sort_long(const int2* ends, const i
     }
   }
 }
+// One listed edge: its colour (the first fit over its bodies' masks), its bodies' cursors moved
+// on, and the entries that thereby reached their last body (take*, n*) for the next list.
+__device__ __forceinline__ void jpiColourOne(const int4 w, int* colour, const int2* sorted,
+                                             JpBody* body, int* arrived, bool& takeA, int4& nA,
+                                             bool& takeB, int4& nB) {
+  const JpBody A = body[w.y];
+  JpBody B{};
+  if (w.z >= 0) B = body[w.z];
+  // the first colour free at both bodies; 62 when none of 0..61 is
+  const unsigned long long freeBits = ~(A.mask | B.mask);
+  const int c = freeBits ? min(__ffsll((long long)freeBits) - 1, 62) : 62;
+  colour[w.x] = c;
+  const unsigned long long bit = 1ull << c;
+  JpBody A1 = A;
+  A1.mask |= bit;
+  A1.cursor += 1;
+  body[w.y] = A1;
+  if (A1.cursor < A1.end) {
+    const int2 en = sorted[A1.cursor];
+    takeA = jpArrive(en, arrived);
+    nA = make_int4(en.x, w.y, en.y, 0);
+  }
+  if (w.z >= 0) {
+    JpBody B1 = B;
+    B1.mask |= bit;
+    B1.cursor += 1;
+    body[w.z] = B1;
+    if (B1.cursor < B1.end) {
+      const int2 en = sorted[B1.cursor];
+      takeB = jpArrive(en, arrived);
+      nB = make_int4(en.x, w.z, en.y, 0);
+    }
+  }
+}
 // One round: colour the listed edges, advance their bodies' cursors, list the next round's edges.
 __global__ __launch_bounds__(JPI_THREADS) void k_jpi_round(const int4* work, const int* count,
                                                            int* colour, const int2* sorted,
@@ -1872,38 +1906,40 @@ __global__ __launch_bounds__(JPI_THREADS) void k_jpi_round(const int4* work, con
     const int t = base + (int)threadIdx.x;
     bool takeA = false, takeB = false;
     int4 nA{}, nB{};
-    if (t < n) {
-      const int4 w = work[t];  // (edge, body, other body or -1)
-      const JpBody A = body[w.y];
-      JpBody B{};
-      if (w.z >= 0) B = body[w.z];
-      // the first colour free at both bodies; 62 when none of 0..61 is
-      const unsigned long long freeBits = ~(A.mask | B.mask);
-      const int c = freeBits ? min(__ffsll((long long)freeBits) - 1, 62) : 62;
-      colour[w.x] = c;
-      const unsigned long long bit = 1ull << c;
-      JpBody A1 = A;
-      A1.mask |= bit;
-      A1.cursor += 1;
-      body[w.y] = A1;
-      if (A1.cursor < A1.end) {
-        const int2 en = sorted[A1.cursor];
-        takeA = jpArrive(en, arrived);
-        nA = make_int4(en.x, w.y, en.y, 0);
-      }
-      if (w.z >= 0) {
-        JpBody B1 = B;
-        B1.mask |= bit;
-        B1.cursor += 1;
-        body[w.z] = B1;
-        if (B1.cursor < B1.end) {
-          const int2 en = sorted[B1.cursor];
-          takeB = jpArrive(en, arrived);
-          nB = make_int4(en.x, w.z, en.y, 0);
-        }
-      }
-    }
+    if (t < n) jpiColourOne(work[t], colour, sorted, body, arrived, takeA, nA, takeB, nB);
     blockAppend2(takeA, nA, takeB, nB, next, nextCount, shWave, &shBase);
+  }
+}
+// Rounds j0 .. j1-1 in one workgroup (short lists): the same round body, rounds separated by
+// workgroup barriers, the next list counted in LDS and each round's list size written to cnt[]
+// as the launched rounds do.  Round j0 reads listA, the rounds then alternate between the two.
+__global__ __launch_bounds__(JPI_THREADS) void k_jpi_tail(int4* listA, int4* listB, int* cnt,
+                                                          int j0, int j1, int* colour,
+                                                          const int2* sorted, JpBody* body,
+                                                          int* arrived) {
+  __shared__ int shN, shNext;
+  if (threadIdx.x == 0) shN = cnt[j0];
+  __syncthreads();
+  for (int j = j0; j < j1; ++j) {
+    const int n = shN;
+    if (n == 0) break;  // (block-uniform) later rounds keep the zero counts they hold
+    const int4* work = ((j - j0) & 1) ? listB : listA;
+    int4* next = ((j - j0) & 1) ? listA : listB;
+    if (threadIdx.x == 0) shNext = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      bool takeA = false, takeB = false;
+      int4 nA{}, nB{};
+      jpiColourOne(work[t], colour, sorted, body, arrived, takeA, nA, takeB, nB);
+      if (takeA) next[atomicAdd(&shNext, 1)] = nA;
+      if (takeB) next[atomicAdd(&shNext, 1)] = nB;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      cnt[j + 1] = shNext;
+      shN = shNext;
+    }
+    __syncthreads();
   }
 }
 // Colour classes: pass 1 counts per (block, colour) with LDS atomics; k_color_offsets turns the

@@ -1147,6 +1147,8 @@ struct pd_sim {
   }
 
   static constexpr int JP_MAX_BATCH = 32;
+  // incidence rounds expected to list at most this many edges run in one workgroup (k_jpi_tail)
+  static constexpr long JPI_TAIL_EDGES = 4096;
   static constexpr long JP_GRID_CAP = 2048;
   // large colourings run their rounds over ordered incidence lists (colourBegin has the rule)
   long incMinEdges = 1L << 16;
@@ -1304,6 +1306,13 @@ struct pd_sim {
       long est = std::max<long>(1, J.numEdges / 8);
       if (r < (int)J.fhist.size()) est = J.fhist[r];
       else if (j == 0 && !J.first) est = J.frontier;
+      if (est <= JPI_TAIL_EDGES && r < (int)J.fhist.size()) {
+        // the rest of the batch is expected to hold short lists: one workgroup runs them
+        k_jpi_tail<<<1, JPI_THREADS, 0, s>>>(J.iWork[(J.cur + j) & 1].p,
+                                             J.iWork[(J.cur + j + 1) & 1].p, J.cnt.p, j, B,
+                                             J.colour, J.iSorted.p, J.iBody.p, J.iArrived.p);
+        break;
+      }
       const int grid = (int)std::min<long>(jblk(est + est / 4 + 64), JP_GRID_CAP / 4);
       k_jpi_round<<<grid, JPI_THREADS, 0, s>>>(J.iWork[(J.cur + j) & 1].p, J.cnt.p + j, J.colour,
                                                J.iSorted.p, J.iBody.p, J.iArrived.p,
