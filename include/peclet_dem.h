@@ -265,8 +265,11 @@ int pd_set_timing(pd_sim* s, int level);
  * Jones-Plassmann rounds over per-body edge lists sorted by priority (one visit per edge)
  * instead of contend / commit passes over every uncoloured edge (0: every colouring does);
  * "position_tail_rows" (default 2048): in per-colour position sweeps, the trailing contact colours
- * holding at most that many rows together run in one workgroup launch that also ends the sweep.
- * Results do not depend on any of them. */
+ * holding at most that many rows together run in one workgroup launch that also ends the sweep;
+ * "adjoint_keep_segment" = 1 (default) lets pd_tape_adjoint replay each substep of a checkpoint
+ * segment once, into workspaces and tapes of its own, when a segment's worth fits in a quarter of
+ * the free device memory; 0 replays the segment for its states and each substep again right
+ * before its reverse.  Results do not depend on any of them. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 /* Autotuner (no reference counterpart): chooses candidate_skin_permille, block_solver_rows,
  * velocity_loop_lds and vsweep_waves for this simulation by timing its own substeps (coordinate

@@ -232,20 +232,35 @@ namespace adjrun {
 
 inline int nb(long n) { return n > 0 ? (int)((n + TPB - 1) / TPB) : 1; }
 
+// the next n elements of type E from a carving cursor, 256-byte aligned (a null cursor measures)
+template <class E>
+E* carvePiece(char*& cur, size_t n) {
+  const size_t at = ((size_t)cur + 255) / 256 * 256;
+  cur = (char*)(at + n * sizeof(E));
+  return (E*)at;
+}
 template <class T>
 struct WS {  // replay workspace
   DBuf<T> xs, xt, v0, vh, q0, vc, xr, qr, cg, ms, rc, lam, pc, pl, par;
   DBuf<ad::SlotK> sk;
   DBuf<int> realOf, gcnt, goff, cMan;
-  void ensure(const AdjTape& A, hipStream_t st) {
+  struct Sizes {
     size_t slots = 1, ncm = 1, nmm = 1, rows = 1, prows = 1;
+  };
+  static Sizes sizes(const AdjTape& A) {
+    Sizes z;
     for (const TapeStep& S : A.steps) {
-      slots = std::max(slots, (size_t)S.t.nSlots);
-      ncm = std::max(ncm, (size_t)S.t.nc);
-      nmm = std::max(nmm, (size_t)S.t.nm);
-      rows = std::max(rows, (size_t)S.t.nRows);
-      prows = std::max(prows, (size_t)S.t.nPRows);
+      z.slots = std::max(z.slots, (size_t)S.t.nSlots);
+      z.ncm = std::max(z.ncm, (size_t)S.t.nc);
+      z.nmm = std::max(z.nmm, (size_t)S.t.nm);
+      z.rows = std::max(z.rows, (size_t)S.t.nRows);
+      z.prows = std::max(z.prows, (size_t)S.t.nPRows);
     }
+    return z;
+  }
+  void ensure(const AdjTape& A, hipStream_t st) {
+    const Sizes z = sizes(A);
+    const size_t slots = z.slots, ncm = z.ncm, nmm = z.nmm, rows = z.rows, prows = z.prows;
     xs.ensure(3 * slots, st); xt.ensure(3 * slots, st); v0.ensure(3 * slots, st);
     vh.ensure(6 * slots, st); q0.ensure(4 * slots, st); vc.ensure(6 * slots, st);
     xr.ensure(3 * slots, st); qr.ensure(4 * slots, st); cg.ensure(10 * ncm, st);
@@ -259,6 +274,32 @@ struct WS {  // replay workspace
     xr.release(); qr.release(); cg.release(); ms.release(); rc.release(); lam.release();
     pc.release(); pl.release(); par.release(); sk.release(); realOf.release(); gcnt.release();
     goff.release(); cMan.release();
+  }
+  // the same buffers carved from caller-owned memory (`cur` advances past them); such a
+  // workspace owns nothing and is not released
+  void carve(const AdjTape& A, char*& cur) {
+    const Sizes z = sizes(A);
+    auto put = [&](auto& b, size_t n) {
+      using E = std::remove_reference_t<decltype(*b.p)>;
+      b.p = carvePiece<E>(cur, n);
+      b.cap = n;
+    };
+    put(xs, 3 * z.slots); put(xt, 3 * z.slots); put(v0, 3 * z.slots); put(vh, 6 * z.slots);
+    put(q0, 4 * z.slots); put(vc, 6 * z.slots); put(xr, 3 * z.slots); put(qr, 4 * z.slots);
+    put(cg, 10 * z.ncm); put(ms, 15 * z.nmm); put(rc, 20 * z.rows); put(lam, 4 * z.rows);
+    put(pc, 10 * z.prows); put(pl, z.prows); put(par, 2); put(sk, z.slots);
+    put(realOf, z.slots); put(gcnt, (size_t)A.numReal + 1); put(goff, (size_t)A.numReal + 1);
+    put(cMan, z.ncm);
+  }
+  // bytes of one carved workspace and its tape pair of vt + pt doubles, a multiple of 256 (so
+  // consecutive ones carved from a 256-byte aligned base carve exactly as measured)
+  static size_t carveBytes(const AdjTape& A, size_t vt, size_t pt) {
+    WS<T> w;
+    char* cur = nullptr;
+    w.carve(A, cur);
+    (void)carvePiece<double>(cur, vt);
+    (void)carvePiece<double>(cur, pt);
+    return ((size_t)cur + 255) / 256 * 256;
   }
   ad::RB<T> rb(T* X, T* Xn, T* L, const T* Lp) {
     ad::RB<T> B{};
@@ -352,7 +393,7 @@ inline GCarve carveG(double* base, size_t slots, size_t nc, size_t nm, size_t ro
 struct GWS {  // adjoint workspace
   DBuf<double> X, Z, La, Lb, par;
   DBuf<double> vtape, ptape;
-  size_t nL = 1;
+  size_t nL = 1, vtN = 1, ptN = 1;  // (tape lengths, doubles)
   void ensure(const AdjTape& A, hipStream_t st) {
     size_t nmm = 1, vt = 1, pt = 1, z = 1;
     for (const TapeStep& S : A.steps) {
@@ -365,6 +406,8 @@ struct GWS {  // adjoint workspace
     }
     for (const TapeCkpt& c : A.ckpts) nmm = std::max(nmm, (size_t)c.nL);
     nL = nmm;
+    vtN = vt;
+    ptN = pt;
     X.ensure(13 * (size_t)std::max(1, A.numReal), st);
     Z.ensure(z, st);
     La.ensure(5 * nmm, st);
@@ -378,8 +421,8 @@ struct GWS {  // adjoint workspace
 
 // reverse of one substep: G.X holds the adjoint of the end state on entry and of the start state
 // on exit; gL the adjoint of this substep's ledger, gLp (zeroed here) receives the previous one's
-void reverseStep(pd_sim& s, const TapeStep& S, ad::RB<double> B, GWS& W, double* gL,
-                 double* gLp, int nLp) {
+void reverseStep(pd_sim& s, const TapeStep& S, ad::RB<double> B, GWS& W, const double* vtape,
+                 const double* ptape, double* gL, double* gLp, int nLp) {
   hipStream_t st = s.st;
   const ad::RK K = constants(*s.tape, S);
   const ad::Topo& t = S.t;
@@ -400,7 +443,7 @@ void reverseStep(pd_sim& s, const TapeStep& S, ad::RB<double> B, GWS& W, double*
       for (int c = S.nColP - 1; c >= 0; --c) {
         const int r0 = S.ps[c], r1 = S.ps[c + 1];
         if (r1 > r0)
-          ad::b_ppass<<<nb(r1 - r0), TPB, 0, st>>>(B, G, t, r0, r1, W.ptape.p + stride * it);
+          ad::b_ppass<<<nb(r1 - r0), TPB, 0, st>>>(B, G, t, r0, r1, ptape + stride * it);
       }
     ad::b_prows<<<nb(t.nPRows), TPB, 0, st>>>(B, G, t);
   }
@@ -415,10 +458,10 @@ void reverseStep(pd_sim& s, const TapeStep& S, ad::RB<double> B, GWS& W, double*
         if (r1 <= r0) continue;
         if (mode == 0)
           ad::b_vpass<<<nb((long)(r1 - r0) * ad::VP_SPLIT), TPB, 0, st>>>(
-              B, G, K, t, mode, r0, r1, W.vtape.p + stride * p);
+              B, G, K, t, mode, r0, r1, vtape + stride * p);
         else
           ad::b_vpass_pgs<<<nb(r1 - r0), TPB, 0, st>>>(B, G, K, t, mode, r0, r1,
-                                                       W.vtape.p + stride * p);
+                                                       vtape + stride * p);
       }
     }
   }
@@ -547,6 +590,34 @@ void adjointAll(pd_sim& s, AdjTape* A, const std::vector<double>& gXend, std::ve
   double* gL = G.La.p;   // adjoint of the current substep's ledger
   double* gLp = G.Lb.p;  // adjoint of the previous one
   s.zero(gL, 5 * G.nL);
+  // A segment's substeps are replayed once each into workspaces and tapes of their own when
+  // those fit in a quarter of the free device memory; otherwise the segment is replayed for its
+  // states and each substep again, with its tapes, right before its reverse.
+  const int segMax = std::min(every, nS);
+  bool keep = false;
+  if (segMax > 1 && s.adjKeepSegment) {
+    size_t freeB = 0, totalB = 0;
+    HIPCHECK(hipMemGetInfo(&freeB, &totalB));
+    keep = (size_t)segMax * W.carveBytes(*A, G.vtN, G.ptN) <= freeB / 4;
+  }
+  // one allocation for the segment's workspaces and tapes (one per substep)
+  DBuf<char> segArena;
+  std::vector<WS<double>> Wk;
+  std::vector<double*> vtk, ptk;
+  if (keep) {
+    const size_t per = W.carveBytes(*A, G.vtN, G.ptN);
+    segArena.ensure(per * segMax, s.st);
+    Wk.resize(segMax);
+    vtk.resize(segMax);
+    ptk.resize(segMax);
+    for (int i = 0; i < segMax; ++i) {
+      char* cur = segArena.p + per * i;
+      Wk[i].carve(*A, cur);
+      vtk[i] = carvePiece<double>(cur, G.vtN);
+      ptk[i] = carvePiece<double>(cur, G.ptN);
+      s.writeN(Wk[i].par.p, par, 2);
+    }
+  }
   const int nSeg = (nS + every - 1) / every;
   for (int seg = nSeg - 1; seg >= 0; --seg) {
     const int s0 = seg * every, s1 = std::min(nS, s0 + every);
@@ -556,6 +627,19 @@ void adjointAll(pd_sim& s, AdjTape* A, const std::vector<double>& gXend, std::ve
       HIPCHECK(hipMemcpyAsync(Ls.p, c.L, sizeof(double) * 5 * c.nL, hipMemcpyDeviceToDevice, s.st));
     auto Xat = [&](int k) { return Xs.p + (size_t)(k - s0) * nX; };
     auto Lat = [&](int k) { return Ls.p + (size_t)(k - s0 + 1) * 5 * G.nL; };  // L_k, k >= s0-1
+    if (keep) {
+      // one replay per substep, each into its own workspace and tapes, then the reverses
+      for (int k = s0; k < s1; ++k)
+        replayStep<double>(s, A->steps[k], Wk[k - s0].rb(Xat(k), Xat(k + 1), Lat(k), Lat(k - 1)),
+                           vtk[k - s0], ptk[k - s0]);
+      for (int k = s1 - 1; k >= s0; --k) {
+        const int nLp = k == 0 ? A->nKey0 : A->steps[k - 1].t.nm;
+        reverseStep(s, A->steps[k], Wk[k - s0].rb(Xat(k), Xat(k + 1), Lat(k), Lat(k - 1)), G,
+                    vtk[k - s0], ptk[k - s0], gL, gLp, nLp);
+        std::swap(gL, gLp);
+      }
+      continue;
+    }
     for (int k = s0; k < s1 - 1; ++k)
       replayStep<double>(s, A->steps[k], W.rb(Xat(k), Xat(k + 1), Lat(k), Lat(k - 1)), nullptr,
                          nullptr);
@@ -564,13 +648,15 @@ void adjointAll(pd_sim& s, AdjTape* A, const std::vector<double>& gXend, std::ve
       replayStep<double>(s, S, W.rb(Xat(k), Xat(k + 1), Lat(k), Lat(k - 1)), G.vtape.p,
                          G.ptape.p);
       const int nLp = k == 0 ? A->nKey0 : A->steps[k - 1].t.nm;
-      reverseStep(s, S, W.rb(Xat(k), Xat(k + 1), Lat(k), Lat(k - 1)), G, gL, gLp, nLp);
+      reverseStep(s, S, W.rb(Xat(k), Xat(k + 1), Lat(k), Lat(k - 1)), G, G.vtape.p, G.ptape.p,
+                  gL, gLp, nLp);
       std::swap(gL, gLp);
     }
   }
   gX0.resize(nX);
   s.readN(gX0.data(), G.X.p, nX);
   s.readN(gpar, G.par.p, 2);
+  segArena.release();  // (the carved workspaces own nothing)
   W.release();
   G.release();
   Xs.release();

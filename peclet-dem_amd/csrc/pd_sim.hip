@@ -1900,6 +1900,8 @@ struct pd_sim {
   }
   // trailing contact colours holding at most this many rows together run in one workgroup
   long posTailRows = 2048;
+  // the adjoint replays each substep of a checkpoint segment once, into workspaces of its own
+  bool adjKeepSegment = true;
   void demStep() {
     std::memset(&stats, 0, sizeof stats);
     vbActive = nullptr;  // a substep cut short by an error may have left the stage's records
@@ -3869,6 +3871,9 @@ int pd_set_tuning(pd_sim* s, const char* name, long value) {
     } else if (k == "colouring_incidence_min_edges") {
       if (value < 0) throw ArgError("colouring_incidence_min_edges must be >= 0");
       s->incMinEdges = value;
+    } else if (k == "adjoint_keep_segment") {
+      if (value != 0 && value != 1) throw ArgError("adjoint_keep_segment must be 0 or 1");
+      s->adjKeepSegment = value == 1;
     } else if (k == "position_tail_rows") {
       if (value < 0) throw ArgError("position_tail_rows must be >= 0");
       s->posTailRows = value;

@@ -54,6 +54,26 @@ def test_tape_replay_is_the_stepped_state(hip_lib, kind, every):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kind,every", [("bed", 4), ("bed_periodic", 3)])
+def test_segment_replayed_once_or_twice_gives_the_same_adjoint(hip_lib, kind, every):
+    # each substep of a checkpoint segment replayed once into its own workspace (default), or
+    # the segment replayed for its states and each substep again before its reverse.  The
+    # replays are bit-equal either way; the reverse sums scattered adjoints with atomic adds, so
+    # two adjoints agree to rounding (the same bound as two runs of one form)
+    print(kind, every, flush=True)
+    scene, h = recorded(kind, 300, 4, 10, every)
+    gE = np.random.default_rng(3).normal(size=(h.num_particles(), 13))
+    g_once, p_once = h.tape_adjoint(gE)
+    g_again, _ = h.tape_adjoint(gE)
+    h.set_tuning("adjoint_keep_segment", 0)
+    g_twice, p_twice = h.tape_adjoint(gE)
+    scale = np.abs(g_twice).max()
+    spread = np.abs(g_once - g_again).max()
+    assert np.abs(g_once - g_twice).max() <= max(1e-12 * scale, 4 * spread)
+    np.testing.assert_allclose(p_once, p_twice, rtol=1e-10)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("kind", ["bed", "bed_periodic"])
 def test_adjoint_is_the_transpose_of_the_tangent(hip_lib, kind):
     print(kind, flush=True)
