@@ -217,6 +217,9 @@ def copy_probe(mib):
     print(f"# copy probe: 3 copies of {mib} MiB", file=sys.stderr, flush=True)
 
 
+BATCH_ROUNDS = 3  # alternating sequential / batched timing rounds of the batched leg
+
+
 def run_batched(num, seed, steps, seq_steps, workers, autotune=True):
     """Config-4 style ensemble: aggregate particle-substeps/s of one batch, against the same
     systems stepped one after another (a single worker)."""
@@ -232,14 +235,22 @@ def run_batched(num, seed, steps, seq_steps, workers, autotune=True):
     batch.step(dt, 2)  # warm-up: buffers reach their working size
     if autotune:
         batch.autotune(dt, 2)  # the largest system of each size class is tuned
-    batch.set_workers(1)
-    t0 = time.perf_counter()
-    batch.step(dt, seq_steps)
-    t_seq = time.perf_counter() - t0
-    batch.set_workers(workers)
-    t0 = time.perf_counter()
-    batch.step(dt, steps)
-    t_b = time.perf_counter() - t0
+    # the one-worker (sequential) and the batched timings alternate over BATCH_ROUNDS rounds, so
+    # a drift of the box's clocks or host load over the leg reaches both alike
+    t_seq = t_b = 0.0
+    per_round = max(1, steps // BATCH_ROUNDS)
+    steps = per_round * BATCH_ROUNDS
+    seq_per_round = max(1, seq_steps // BATCH_ROUNDS)
+    seq_steps = seq_per_round * BATCH_ROUNDS
+    for _ in range(BATCH_ROUNDS):
+        batch.set_workers(1)
+        t0 = time.perf_counter()
+        batch.step(dt, seq_per_round)
+        t_seq += time.perf_counter() - t0
+        batch.set_workers(workers)
+        t0 = time.perf_counter()
+        batch.step(dt, per_round)
+        t_b += time.perf_counter() - t0
     # algorithmic bytes of one batch substep from each system's last substep counters
     bps = sum(substep_bytes(sim.get_stats()) for sim in batch.systems)
     batch.close()
@@ -570,7 +581,7 @@ def main():
         return
     if args.only == "batched":
         print(json.dumps(run_batched(args.batch_systems, 4, args.batch_steps,
-                                     max(2, args.batch_steps // 6), args.batch_workers)))
+                                     max(2, args.batch_steps // 2), args.batch_workers)))
         return
 
     sim, bed, head = plain_leg(args.workload, args.n, args.seed, args.column, dt,
@@ -612,7 +623,7 @@ def main():
     batched = None
     if not args.no_batched:
         batched = run_batched(args.batch_systems, 4, args.batch_steps,
-                              max(2, args.batch_steps // 6), args.batch_workers,
+                              max(2, args.batch_steps // 2), args.batch_workers,
                               not args.no_autotune)
         print(f"# batched: {batched['value']:.3e} particle-substeps/s "
               f"({batched['speedup_vs_sequential']:.2f}x sequential)", file=sys.stderr, flush=True)

@@ -269,7 +269,9 @@ int pd_set_timing(pd_sim* s, int level);
  * "adjoint_keep_segment" = 1 (default) lets pd_tape_adjoint replay each substep of a checkpoint
  * segment once, into workspaces and tapes of its own, when a segment's worth fits in a quarter of
  * the free device memory; 0 replays the segment for its states and each substep again right
- * before its reverse.  Results do not depend on any of them. */
+ * before its reverse; "side_stream_min_contacts" (default 0): contact lists of at least that
+ * many contacts are coloured on the simulation's second stream, beside the velocity stage,
+ * shorter ones on its main stream.  Results do not depend on any of them. */
 int pd_set_tuning(pd_sim* s, const char* name, long value);
 /* Autotuner (no reference counterpart): chooses candidate_skin_permille, block_solver_rows,
  * velocity_loop_lds and vsweep_waves for this simulation by timing its own substeps (coordinate

@@ -144,6 +144,15 @@ def test_per_colour_launches_body_records(hip_lib, kind, pack):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("block_rows", [None, 0])
+def test_contact_colouring_on_the_main_stream(hip_lib, block_rows):
+    # the contact colouring runs on the second stream by default, beside the velocity stage;
+    # here on the main stream behind it
+    run_pair("bed_periodic", 400, 10, block_rows=block_rows,
+             tuning={"side_stream_min_contacts": 1 << 30})
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tail", [0, 64, 100000])
 def test_position_tail_split(hip_lib, tail):
     # per-colour position sweeps with no trailing colours in the one-workgroup tail launch (0),
