@@ -349,8 +349,9 @@ __global__ void k_scan_down2(const int* in, int n, const int* blockSums, int* ou
   if ((int)blockIdx.x == last && threadIdx.x == 0) out[n] = prefix + tot;
 }
 // The same exclusive scan (total into out[n]) in one workgroup of SCAN1_THREADS, for lists of up
-// to SCAN1_MAX elements: tiles of 4 elements per thread with a running carry, so a mid-sized scan
-// costs one launch instead of two.
+// to SCAN1_MAX elements (`in` and `out` distinct): each thread sums a contiguous chunk, one
+// workgroup scan of the chunk sums, then each thread writes its chunk's prefixes.  One launch
+// and one barrier instead of the two-kernel scan.
 constexpr int SCAN1_THREADS = 1024;
 constexpr int SCAN1_MAX = 32768;
 __global__ __launch_bounds__(SCAN1_THREADS) void k_scan_single(const int* in, int n, int* out,
@@ -358,38 +359,33 @@ __global__ __launch_bounds__(SCAN1_THREADS) void k_scan_single(const int* in, in
   __shared__ int waveTot[SCAN1_THREADS / 64];
   if (nDev) n = *nDev;
   const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
-  int carry = 0;
-  for (int base = 0; base < n; base += SCAN1_THREADS * 4) {
-    const int i0 = base + (int)threadIdx.x * 4;
-    int v[4];
+  const int per = (n + SCAN1_THREADS - 1) / SCAN1_THREADS;
+  const int b0 = (int)threadIdx.x * per;
+  const int e0 = min(b0 + per, n);
+  int own = 0;
+  for (int i = b0; i < e0; ++i) own += in[i];
+  // inclusive scan across the wave, then across the waves' totals
+  int inc = own;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? in[i0 + k] : 0;
-    const int own = v[0] + v[1] + v[2] + v[3];
-    // inclusive scan across the wave, then across the waves' totals
-    int inc = own;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int x = __shfl_up(inc, off, 64);
-      if (lane >= off) inc += x;
-    }
-    if (lane == 63) waveTot[wave] = inc;
-    __syncthreads();
-    int before = 0, tot = 0;
-    for (int w = 0; w < SCAN1_THREADS / 64; ++w) {
-      const int t = waveTot[w];
-      if (w < wave) before += t;
-      tot += t;
-    }
-    int ex = carry + before + inc - own;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (i0 + k < n) out[i0 + k] = ex;
-      ex += v[k];
-    }
-    carry += tot;
-    __syncthreads();  // waveTot is rewritten by the next tile
+  for (int off = 1; off < 64; off <<= 1) {
+    const int x = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += x;
   }
-  if (threadIdx.x == 0) out[n] = carry;
+  if (lane == 63) waveTot[wave] = inc;
+  __syncthreads();
+  int before = 0, tot = 0;
+  for (int w = 0; w < SCAN1_THREADS / 64; ++w) {
+    const int t = waveTot[w];
+    if (w < wave) before += t;
+    tot += t;
+  }
+  int ex = before + inc - own;
+  for (int i = b0; i < e0; ++i) {
+    const int v = in[i];
+    out[i] = ex;
+    ex += v;
+  }
+  if (threadIdx.x == 0) out[n] = tot;
 }
 
 // ============================================================================================

@@ -477,10 +477,18 @@ struct pd_sim {
   // Wait until the posted word *seqp equals seq: the k_post that writes it wrote its values
   // first.  Polling the pinned word wakes the host as soon as the values land; the stream is
   // queried now and then so that a failed stream raises instead of spinning.
+  // Polls the posted word; the stream is queried (to raise on a failed stream) only every
+  // ~2 ms of waiting, since the runtime call takes locks that the other threads of a batch need
+  // for their launches.
   static void waitPosted(const volatile int* seqp, int seq, hipStream_t s) {
+    auto last = std::chrono::steady_clock::now();
     for (long spin = 1;; ++spin) {
       if (*seqp == seq) break;
-      if ((spin & 4095) == 0) {
+      __builtin_ia32_pause();
+      if ((spin & 1023) == 0) {
+        const auto now = std::chrono::steady_clock::now();
+        if (now - last < std::chrono::milliseconds(2)) continue;
+        last = now;
         const hipError_t e = hipStreamQuery(s);
         if (e == hipSuccess) {
           if (*seqp == seq) break;
