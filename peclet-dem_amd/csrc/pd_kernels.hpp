@@ -444,6 +444,14 @@ __device__ __forceinline__ void ghostShifts(const Params& P, d3 p, int& sx, int&
 // counters, the overlap accumulator; the grounded-level decay of the substep
 // (src/contact_preprocessing.hpp:213-253, first step) when `ground` is given; and each real's
 // ghost count (as k_ghost_count).
+// One 128-byte record per body for the velocity stage of large beds: velocity, angular
+// velocity, orientation and inverse mass / inertia in one cache line, so a row's body gather
+// fetches two whole lines instead of four half-used ones (the two 64-byte records of the
+// separate arrays).  Written by the predict launch (or packed from vw / bc before the stage) and
+// unpacked after the stage.
+struct alignas(128) VBody {
+  double4 v, w, q, iim;
+};
 struct SubstepStart {
   int* candFlag;
   int* counters;
@@ -454,6 +462,7 @@ struct SubstepStart {
   int* ground;  // nullptr: no grounded levels this substep
   int decay;
   int* ghostCnt;  // ghost count per real (src/periodicity.hpp:43-94, pass 1), from pos
+  VBody* vb;      // non-null: also write each real's packed velocity-stage record
 };
 // predictVelocityKokkos, src/integration.hpp:44-93 (+ realIndices and broadphase radius fill)
 __global__ void k_predict(const Dev* __restrict__ Dp, Params P, SubstepStart S) {
@@ -508,6 +517,12 @@ __global__ void k_predict(const Dev* __restrict__ Dp, Params P, SubstepStart S) 
   out.v = tod4(v, 0.0);
   out.w = tod4(wpred, 0.0);
   D.vw[i] = out;
+  if (S.vb) {  // (field by field, as k_vb_pack)
+    S.vb[i].v = out.v;
+    S.vb[i].w = out.w;
+    S.vb[i].q = b.q;
+    S.vb[i].iim = b.iim;
+  }
   d3 x = xyz(D.pos[i]);
   if (invM > 0.0) x = add3(x, scale3(v, dt));
   const double rad = D.scale[i] * P.gs * P.baseRadius;
@@ -2315,13 +2330,6 @@ struct GBodies {
   __device__ Body64 load(int real, int) const { return vw[real]; }
   __device__ void store(int real, int, const Body64& b) const { vw[real] = b; }
   __device__ BodyC cload(const Dev& D, int real) const { return D.bc[real]; }
-};
-// One 128-byte record per body for the velocity stage of large beds: velocity, angular
-// velocity, orientation and inverse mass / inertia in one cache line, so a row's body gather
-// fetches two whole lines instead of four half-used ones (the two 64-byte records of the
-// separate arrays).  Packed from vw / bc before the stage and unpacked after it.
-struct alignas(128) VBody {
-  double4 v, w, q, iim;
 };
 struct PBodies {
   VBody* vb;

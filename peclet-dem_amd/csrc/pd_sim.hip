@@ -1578,6 +1578,8 @@ struct pd_sim {
   DBuf<VBody> vbody;
   VBody* vbActive = nullptr;
   bool packBodies = true;
+  bool vbUsedLast = false;     // the previous substep's velocity stage used the records
+  bool vbFromPredict = false;  // this substep's predict launch wrote them
   DBuf<unsigned char> mlElig, mlSlot;
   DBuf<int> mlGrp, mlMate, mlLead, mlScan, mlParent, mlCol, mlPerm, mlRowOfTmp, mlCnt;
   DBuf<int> mlBlockCnt, mlColCnt, mlColStart, mlClsCnt;
@@ -1966,8 +1968,15 @@ struct pd_sim {
     ghostOff.ensure(numReal + 1, st);
     slotInfo.ensure(4, st);
     SubstepStart S0{candFlag.p, counters.p, supCnt.p, ctl.p, CTL_COUNT, bits.p, nullptr, 8,
-                    ghostCnt.p};
+                    ghostCnt.p, nullptr};
     if (usePersist && useGS) S0.ground = ground.p;  // the grounded levels are updated this substep
+    // the velocity stage's packed records are written by the predict launch when the previous
+    // substep's stage used them (nothing changes the reals' velocities in between)
+    vbFromPredict = packBodies && vbUsedLast && numReal > 0;
+    if (vbFromPredict) {
+      vbody.ensure(numReal, st);
+      S0.vb = vbody.p;
+    }
     kernBegin();
     k_predict<<<std::max(1, nblk(numReal)), TPB, 0, st>>>(dD, P, S0);
     kernEnd(K_PREDICT, numReal);
@@ -2152,13 +2161,15 @@ struct pd_sim {
       vWaves62 = serialClassWaves(ends.data(), cnt62, numReal, vs[62], vWavePerm, vWaveStart);
     }
     // the per-colour launches of a large bed read each row's bodies from one 128-byte record
-    // per body, packed here and unpacked when the stage ends
-    if (packBodies && !hostLoop && !fusedV && !smallV && nRowsV > 0 && numReal > 0 &&
-        (usePGS || velIters > 0)) {
+    // per body, written by the predict launch or packed here, and unpacked when the stage ends
+    const bool usePack = packBodies && !hostLoop && !fusedV && !smallV && nRowsV > 0 &&
+                         numReal > 0 && (usePGS || velIters > 0);
+    if (usePack) {
       vbody.ensure(numReal, st);
-      k_vb_pack<<<nblk(numReal), TPB, 0, st>>>(dD, vbody.p, numReal);
+      if (!vbFromPredict) k_vb_pack<<<nblk(numReal), TPB, 0, st>>>(dD, vbody.p, numReal);
       vbActive = vbody.p;
     }
+    vbUsedLast = usePack;
     if (usePGS) {
       // grounded levels, double-buffered (src/contact_preprocessing.hpp:213-253); the decay ran
       // in the predict launch
