@@ -168,6 +168,8 @@ struct Dev {
 
 // velocity-row flag bits
 constexpr unsigned char F_PERSIST = 1, F_SIDE1 = 2, F_SIDE2 = 4, F_UP = 8, F_DOWN = 16;
+// a one-point manifold's row: its tangent lever arms are recomputed instead of read (rowTaus)
+constexpr unsigned char F_ONE = 32;
 
 __device__ __forceinline__ int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 // Workgroups are dispatched to the 8 XCDs in turn (workgroup i to XCD i mod 8), each XCD with
@@ -2156,11 +2158,15 @@ __global__ __launch_bounds__(TPB) void k_vrows(const Dev* __restrict__ Dp, Param
   D.vAB[k] = r;
   D.vWV[k] = D.mWV[m];
   D.vN[k] = tod4(Nsum, restitution);
-  D.vTA[k] = tod4(TauA, mu);
-  D.vTB[k] = tod4(TauB, invN);
-  D.vRA[k] = tod4(rAavg, lenN);
-  D.vRB[k] = tod4(rBavg, sgn);
   unsigned char flag = 0;
+  if (np == 1) {
+    flag |= F_ONE;  // TauA / TauB are recomputed by the rows (rowTaus)
+  } else {
+    D.vTA[k] = tod4(TauA, 0.0);
+    D.vTB[k] = tod4(TauB, invN);
+  }
+  D.vRA[k] = tod4(rAavg, mu);
+  D.vRB[k] = tod4(rBavg, sgn);
   const unsigned char code = D.mCode[m];
   if (code == 1) flag |= F_UP;
   if (code == 2) flag |= F_DOWN;
@@ -2412,6 +2418,27 @@ struct LBodies {
   __device__ BodyC cload(const Dev& D, int real) const { return D.bc[real]; }
 };
 
+// A velocity row's constants as k_vrows lays them out: vN = (Nsum, restitution), vRA = (rAavg,
+// mu), vRB = (rBavg, sgn), vTA = TauA, vTB = (TauB, 1 / points); |Nsum| is recomputed (the same
+// expression k_vrows evaluates).  A one-point row (F_ONE) reads neither vTA nor vTB: its sums are
+// one-term sums, TauA = 0 + rA x n and TauB = 0 + rB x (-n) as k_mreduce formed them from the
+// contact's own arm and normal, and rA = 0 + arm, Nsum = 0 + n differ from those at most in the
+// sign of zero components, which moves no bit of the products' 0 + sums; 1 / points = 1.
+__device__ __forceinline__ double rowLenN(d3 Nsum) { return sqrt(dot3(Nsum, Nsum)); }
+__device__ __forceinline__ void rowTaus(const Dev& D, int k, unsigned char fl, d3 Nsum, d3 rAavg,
+                                        d3 rBavg, d3& TauA, d3& TauB, double& invN) {
+  if (fl & F_ONE) {
+    const d3 z{0.0, 0.0, 0.0};
+    TauA = add3(z, cross3(rAavg, Nsum));
+    TauB = add3(z, cross3(rBavg, neg3(Nsum)));
+    invN = 1.0;
+  } else {
+    TauA = xyz(D.vTA[k]);
+    const double4 tb4 = D.vTB[k];
+    TauB = xyz(tb4);
+    invN = tb4.w;
+  }
+}
 // warmStartApplyKokkos, src/solver_velocity.hpp:498-605 (one row)
 template <class BS>
 __device__ __forceinline__ void warmApplyRowB(const Dev& D, int k, int2 r, int2 sl, const BS& bs) {
@@ -2421,11 +2448,14 @@ __device__ __forceinline__ void warmApplyRowB(const Dev& D, int k, int2 r, int2 
   const bool hasT = nonzero3(lt);
   if (p == 0.0 && !hasT) return;
   const int realA = r.x, realB = r.y;
-  const d3 Nsum = xyz(D.vN[k]), TauA = xyz(D.vTA[k]), TauB = xyz(D.vTB[k]);
+  const d3 Nsum = xyz(D.vN[k]);
   const d3 rAavg = xyz(D.vRA[k]);
   const double4 rb4 = D.vRB[k];
   const d3 rBavg = xyz(rb4);
   const double sgn = rb4.w;
+  d3 TauA, TauB;
+  double invN;
+  rowTaus(D, k, D.vFlag[k], Nsum, rAavg, rBavg, TauA, TauB, invN);
   const double lambda = -sgn * p;
   const d3 Jlin = scale3(Nsum, lambda);
   const BodyC ca = bs.cload(D, realA);
@@ -2527,8 +2557,13 @@ __device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, 
   Body64 ba = bs.load(realA, sl.x);
   Body64 bb;
   d3 vB{0, 0, 0}, wB{0, 0, 0};
-  const double4 tb4 = D.vTB[k];
-  const double invN = tb4.w;
+  const unsigned char fl = D.vFlag[k];
+  const double4 n4 = D.vN[k], ra4 = D.vRA[k], rb4 = D.vRB[k];
+  const d3 Nsum = xyz(n4);
+  const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
+  d3 TauA, TauB;
+  double invN;
+  rowTaus(D, k, fl, Nsum, rAavg, rBavg, TauA, TauB, invN);
   if (realB >= 0) {
     cb = bs.cload(D, realB);
     invMassB = cb.iim.w;
@@ -2541,11 +2576,8 @@ __device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, 
     vB = scale3(xyz(D.vWV[k]), invN);
   }
   const d3 vA = xyz(ba.v), wA = xyz(ba.w);
-  const double4 n4 = D.vN[k], ta4 = D.vTA[k], ra4 = D.vRA[k], rb4 = D.vRB[k];
   double restitution = n4.w;
-  const d3 Nsum = xyz(n4), TauA = xyz(ta4), TauB = xyz(tb4);
-  const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
-  const double lenN = ra4.w;
+  const double lenN = rowLenN(Nsum);
   if (lenN < 1e-9) return 0.0;
   // the row's constants, from the bodies just loaded (cheaper than reading them back)
   const RowPre pre = rowPre(P, realB, Nsum, TauA, TauB, rAavg, rBavg, lenN, invMassA, invIA, qA,
@@ -2556,7 +2588,6 @@ __device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, 
   double wA_n = pre.wAn;  // Nsq * invMassA + genInvMass(TauA, invIA, qA)
   double wB_n = pre.wBn;
   bool applyA = true, applyB = true;
-  const unsigned char fl = D.vFlag[k];
   const int sf = (fl & F_SIDE1) ? 1 : ((fl & F_SIDE2) ? 2 : 0);
   if (sf == 1) {
     wB_n = 0.0;
@@ -2608,7 +2639,7 @@ __device__ __forceinline__ double pgsRowB(const Dev& D, const Params& P, int k, 
     }
   }
   // friction cone, src/solver_velocity.hpp:760-855
-  double mu = ta4.w;
+  double mu = ra4.w;
   const double4 lto4 = D.vLamT[k];  // .w: the warm-start position impulse
   const d3 ltOld = xyz(lto4);
   const bool haveOld = nonzero3(ltOld);
@@ -2703,8 +2734,13 @@ __device__ __forceinline__ double gsRowB(const Dev& D, const Params& P, int k, i
   Body64 ba = bs.load(realA, sl.x);
   Body64 bb;
   d3 vB{0, 0, 0}, wB{0, 0, 0};
-  const double4 tb4 = D.vTB[k];
-  const double invN = tb4.w;
+  const unsigned char fl = D.vFlag[k];
+  const double4 n4 = D.vN[k], ra4 = D.vRA[k], rb4 = D.vRB[k];
+  const d3 Nsum = xyz(n4);
+  const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
+  d3 TauA, TauB;
+  double invN;
+  rowTaus(D, k, fl, Nsum, rAavg, rBavg, TauA, TauB, invN);
   if (realB >= 0) {
     cb = bs.cload(D, realB);
     invMassB = cb.iim.w;
@@ -2717,11 +2753,8 @@ __device__ __forceinline__ double gsRowB(const Dev& D, const Params& P, int k, i
     vB = scale3(xyz(D.vWV[k]), invN);
   }
   const d3 vA = xyz(ba.v), wA = xyz(ba.w);
-  const double4 n4 = D.vN[k], ta4 = D.vTA[k], ra4 = D.vRA[k], rb4 = D.vRB[k];
   double restitution = n4.w;
-  const d3 Nsum = xyz(n4), TauA = xyz(ta4), TauB = xyz(tb4);
-  const d3 rAavg = xyz(ra4), rBavg = xyz(rb4);
-  const double lenN = ra4.w;
+  const double lenN = rowLenN(Nsum);
   if (lenN < 1e-9) return 0.0;
   const d3 diffCenters = (realB < 0) ? rAavg : sub3(rAavg, rBavg);
   const RowPre pre = rowPre(P, realB, Nsum, TauA, TauB, rAavg, rBavg, lenN, invMassA, invIA, qA,
@@ -3119,11 +3152,10 @@ __device__ void mlCoarseRow(const Dev& D, const MlDev& ml, int off, int k,
   const int2 r = D.vAB[k];
   const int gA = ml.grp[r.x];
   const int gB = r.y >= 0 ? ml.grp[r.y] : -1;
-  const double4 n4 = D.vN[k], ra4 = D.vRA[k];
-  const d3 Nsum = xyz(n4);
-  const double lenN = ra4.w;
+  const d3 Nsum = xyz(D.vN[k]);
+  const double lenN = rowLenN(Nsum);
   if (lenN < 1e-9) return;
-  const double invN = D.vTB[k].w;
+  const double invN = (D.vFlag[k] & F_ONE) ? 1.0 : D.vTB[k].w;
   const double sgn = D.vRB[k].w;  // the sign of Nsum . (rA - rB), as in the fine row
   const double4 vA4 = ml.velG[off + gA];
   const d3 vA = xyz(vA4);
@@ -3190,7 +3222,7 @@ __global__ __launch_bounds__(TPB) void k_ml_prolong(const Dev* __restrict__ Dp, 
 // residual, corrections of rows with |vn0| <= 4 vRest |N| (src/solver_velocity.hpp:721-723,
 // 827-831)
 __device__ __forceinline__ double mlQs(const Dev& D, int k, double a, double thr) {
-  return fabs(D.vVn0[k]) <= 4.0 * thr * D.vRA[k].w ? a : 0.0;
+  return fabs(D.vVn0[k]) <= 4.0 * thr * rowLenN(xyz(D.vN[k])) ? a : 0.0;
 }
 __global__ __launch_bounds__(TPB) void k_vsweep_qs(const Dev* __restrict__ Dp, Params P, int rowBegin, int rowEnd,
                                                    double thr, unsigned long long* qsBits) {
