@@ -8,8 +8,9 @@
 //   contacts (narrowphase order): cAB int2, cN (n, dist), cRA (rA, e_b), cRB (rB, mu_b)
 //   manifolds (key order): mAB, mReal int2, mRange int2, mNs (Nsum, e_sum), mTA (tauA, mu_sum),
 //                          mTB, mRA, mRB double4, mNp int
-//   velocity rows (colour-sorted): vAB int2, vN, vTA, vTB, vRA, vRB, vVt0, vLamT double4, vVn0,
-//                                  vLam double, vFlag uchar
+//   velocity rows (colour-sorted): vAB int2, vN (Nsum, e), vRA (rA, mu), vRB (rB, sign),
+//                                  vTA, vTB (tauB, 1/points) (several-point manifolds only),
+//                                  vVt0, vLamT double4, vVn0, vLam double, vFlag uchar
 //   position rows (colour-sorted): pAB int2, pN, pRA, pRB double4, pLam double
 // Every kernel cites the reference function it implements; the canonical orders are those of the
 // CPU oracle (ghost slots by real index, pairs lexicographic, contacts pair-major then planes,
