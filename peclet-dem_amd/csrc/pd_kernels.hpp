@@ -573,15 +573,13 @@ __global__ void k_ghost_emit(const Dev* __restrict__ Dp, Params P, const int* of
   int sx, sy, sz;
   ghostShifts(P, p, sx, sy, sz);
   if (!sx && !sy && !sz) return;
-  const int ax[2] = {0, sx}, ay[2] = {0, sy}, az[2] = {0, sz};
   const int nx = sx == 0 ? 1 : 2, ny = sy == 0 ? 1 : 2, nz = sz == 0 ? 1 : 2;
   const double4 pp = D.posPred[i];
   const d3 pPred = xyz(pp);
-  const BodyC b = D.bc[i];
-  BodyC bg;
-  bg.q = b.q;
-  bg.iim = make_double4(0.0, 0.0, 0.0, b.iim.w);
-  const Body64 v0 = D.vw0[i], v1 = D.vw[i];
+  // (records copied member by member: whole local records would sit in scratch)
+  const double4 bq = D.bc[i].q;
+  const double4 giim = make_double4(0.0, 0.0, 0.0, D.bc[i].iim.w);
+  const double4 v0v = D.vw0[i].v, v0w = D.vw0[i].w, v1v = D.vw[i].v, v1w = D.vw[i].w;
   const double4 qp = D.quatPred[i];
   const double sc = D.scale[i];
   const unsigned char mat = D.matId[i];
@@ -589,15 +587,19 @@ __global__ void k_ghost_emit(const Dev* __restrict__ Dp, Params P, const int* of
   for (int a = 0; a < nx; ++a)
     for (int bb = 0; bb < ny; ++bb)
       for (int c = 0; c < nz; ++c) {
-        const int ix = ax[a], iy = ay[bb], iz = az[c];
+        // (selects, not a small array indexed by the loop counters: that would sit in scratch)
+        const int ix = a ? sx : 0, iy = bb ? sy : 0, iz = c ? sz : 0;
         if (ix == 0 && iy == 0 && iz == 0) continue;
         if (slot >= capSlots) return;
         const d3 shift{ix * P.dsize.x, iy * P.dsize.y, iz * P.dsize.z};
         D.pos[slot] = tod4(add3(p, shift), 0.0);
         D.posPred[slot] = tod4(add3(pPred, shift), pp.w);
-        D.bc[slot] = bg;
-        D.vw0[slot] = v0;
-        D.vw[slot] = v1;
+        D.bc[slot].q = bq;
+        D.bc[slot].iim = giim;
+        D.vw0[slot].v = v0v;
+        D.vw0[slot].w = v0w;
+        D.vw[slot].v = v1v;
+        D.vw[slot].w = v1w;
         D.quatPred[slot] = qp;
         D.scale[slot] = sc;
         D.matId[slot] = mat;
@@ -1382,13 +1384,14 @@ __global__ __launch_bounds__(TPB) void k_mreduce(const Dev* __restrict__ Dp, int
       const int2 cab = D.cAB[c];
       const double4 cra = D.cRA[c], crb = D.cRB[c];
       const bool flip = (cab.y >= 0 && cab.y < cab.x);
-      const d3 nvec = xyz(cn);
-      na = flip ? nvec : neg3(nvec);
+      const double sn = flip ? 1.0 : -1.0;
+      na = d3{sn * cn.x, sn * cn.y, sn * cn.z};  // (exact: a sign change)
       const d3 shift{cn.x * cn.w * 0.5, cn.y * cn.w * 0.5, cn.z * cn.w * 0.5};
       const d3 rAmid = sub3(xyz(cra), shift);
       const d3 rBmid = add3(xyz(crb), shift);
-      rcan = flip ? rBmid : rAmid;
-      roth = flip ? rAmid : rBmid;
+      // (component selects: a select of whole records went through scratch)
+      rcan = d3{flip ? rBmid.x : rAmid.x, flip ? rBmid.y : rAmid.y, flip ? rBmid.z : rAmid.z};
+      roth = d3{flip ? rAmid.x : rBmid.x, flip ? rAmid.y : rBmid.y, flip ? rAmid.z : rBmid.z};
       tca = cross3(rcan, na);
       tcb = cross3(roth, neg3(na));
       e = cra.w;
