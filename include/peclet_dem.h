@@ -266,6 +266,8 @@ int pd_set_timing(pd_sim* s, int level);
  * instead of contend / commit passes over every uncoloured edge (0: every colouring does);
  * "position_tail_rows" (default 2048): in per-colour position sweeps, the trailing contact colours
  * holding at most that many rows together run in one workgroup launch that also ends the sweep;
+ * "velocity_tail_rows" (default 0, off): the same for the per-colour velocity sweeps (without a
+ * last manifold class 62);
  * "adjoint_keep_segment" = 1 (default) lets pd_tape_adjoint replay each substep of a checkpoint
  * segment once, into workspaces and tapes of its own, when a segment's worth fits in a quarter of
  * the free device memory; 0 replays the segment for its states and each substep again right

@@ -3304,6 +3304,33 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_vsweep_waves(
   if (mode != 0 && threadIdx.x == 0 && M) atomicMax(D.maxApproachBits, M);
 }
 
+// The trailing small colour classes [cFrom, numColors) of a per-colour sweep (none of them the
+// last class, 62) in one workgroup with a barrier between classes: one launch instead of one
+// per small class
+template <bool PK>
+__global__ __launch_bounds__(VBLOCK_THREADS) void k_vsweep_tail(
+    const Dev* __restrict__ Dp, Params P, ClassStarts cs, int cFrom, int numColors, int mode,
+    double thr, const int* stop, VBody* vb) {
+  const Dev& D = *Dp;
+  __shared__ unsigned long long sh[VBLOCK_THREADS / 64];
+  if (*stop) return;
+  unsigned long long m = 0;
+  for (int c = cFrom; c < numColors; ++c) {
+    const int b = cs.s[c], e = cs.s[c + 1];
+    for (int k = b + (int)threadIdx.x; k < e; k += blockDim.x) {
+      double a = 0.0;
+      if (mode == 0) RowOf<PK>::warm(D, k, vb);
+      else if (mode == 1) a = RowOf<PK>::pgs(D, P, k, thr, vb);
+      else a = RowOf<PK>::gs(D, P, k, thr, vb);
+      const unsigned long long ab = dbits(a);
+      m = ab > m ? ab : m;
+    }
+    __syncthreads();
+  }
+  const unsigned long long M = blockMaxAll(m, sh);
+  if (mode != 0 && threadIdx.x == 0 && M) atomicMax(D.maxApproachBits, M);
+}
+
 // The sweep loop proper (MODE 1 = PGS, 2 = one-shot GS) over rows whose bodies live in `bs`;
 // returns after the sweep that meets the stop rule or after `iters` sweeps.
 template <int MODE, class BS>

@@ -161,6 +161,15 @@ def test_position_tail_split(hip_lib, tail):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["bed_periodic", "gas"])
+@pytest.mark.parametrize("tail", [0, 64, 100000])
+def test_velocity_tail_split(hip_lib, kind, tail):
+    # per-colour velocity sweeps (PGS, warm start, stabilization; one-shot GS for the gas) with
+    # none, the last small or every colour in the one-workgroup tail launch
+    run_pair(kind, 400, 12, tuning={"block_solver_rows": 0, "velocity_tail_rows": tail})
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tuning", [{}, {"block_solver_rows": 0}, {"block_solver_rows": 0,
                                      "serial_class_waves_min_rows": 1}],
                          ids=["default", "per-colour", "per-colour-waves"])
