@@ -195,6 +195,45 @@ __device__ __forceinline__ void blockAppend(bool take, int item, int* list, int*
   __syncthreads();
   if (take) list[*shBase + local] = item;
 }
+// Chunked appends: a workgroup walks one contiguous chunk of the index range (chunkOf), counts
+// what it appends, takes its base with one atomic and places its items in a second walk by their
+// rank within the workgroup (blockRank).  Thousands of workgroups each taking one atomic per
+// 256 items on the same counter serialise across the XCDs; a few hundred taking one each do not.
+__device__ __forceinline__ int2 chunkOf(int n) {
+  const int per = ((n + (int)gridDim.x - 1) / (int)gridDim.x + (int)blockDim.x - 1) /
+                  (int)blockDim.x * (int)blockDim.x;
+  const int c0 = (int)blockIdx.x * per;
+  return make_int2(c0, min(c0 + per, n));
+}
+// exclusive rank of `take` among the workgroup's threads and the workgroup total; every thread
+// of the workgroup must call it
+__device__ __forceinline__ int blockRank(bool take, int* shWave, int& total) {
+  const unsigned long long m = __ballot(take);
+  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+  if (lane == 0) shWave[wave] = __popcll(m);
+  __syncthreads();
+  int before = 0;
+  total = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    const int c = shWave[w];
+    if (w < wave) before += c;
+    total += c;
+  }
+  __syncthreads();
+  return before + __popcll(m & ((1ull << lane) - 1ull));
+}
+// block sum of v (every thread); the result is valid in every thread
+__device__ __forceinline__ int blockSumI(int v, int* shWave) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+  if (lane == 0) shWave[wave] = v;
+  __syncthreads();
+  int t = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += shWave[w];
+  __syncthreads();
+  return t;
+}
+constexpr int CHUNK_GRID = 1024;  // workgroups of the chunked-append kernels
 constexpr int WAVES = TPB / 64;
 __device__ __forceinline__ unsigned long long waveMaxU64(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -1455,24 +1494,48 @@ __device__ __forceinline__ unsigned char manifoldKeyCode(const Dev& D, int m, d3
 }
 // (also rowOf[m] = -1: rows are placed for the coloured manifolds only; and each manifold's
 // ledger key and support code with the support list, as k_mkeys, in the same pass)
-__global__ void k_jp_init_manifolds(const Dev* __restrict__ Dp, int numManifolds, int* work, int* count,
-                                    int* rowOf, d3 gHat, int* supList, int* supCount) {
+__global__ __launch_bounds__(TPB) void k_jp_init_manifolds(const Dev* __restrict__ Dp,
+                                                            int numManifolds, int* work,
+                                                            int* count, int* rowOf, d3 gHat,
+                                                            int* supList, int* supCount) {
   const Dev& D = *Dp;
-  __shared__ int shCnt, shBase, shCnt2, shBase2;
-  const int m = gtid();
-  int c = -2;
-  unsigned char code = 0;
-  if (m < numManifolds) {
+  __shared__ int shWave[WAVES], shBase, shBase2;
+  const int2 ch = chunkOf(numManifolds);
+  // pass 1: colour state, row slot, key and support code; count the list entries
+  int nWork = 0, nSup = 0;
+  for (int m = ch.x + (int)threadIdx.x; m < ch.y; m += blockDim.x) {
     const int2 r = D.mReal[m];
-    c = -1;
+    int c = -1;
     if (D.mNp[m] <= 0) c = -2;
     else if (D.mAB[m].y >= 0 && r.x > r.y) c = -2;
     D.mColor[m] = c;
     rowOf[m] = -1;
-    code = manifoldKeyCode(D, m, gHat);
+    const unsigned char code = manifoldKeyCode(D, m, gHat);
+    nWork += c == -1 ? 1 : 0;
+    nSup += code != 0 ? 1 : 0;
   }
-  blockAppend(c == -1, m, work, count, &shCnt, &shBase);
-  blockAppend(code != 0, m, supList, supCount, &shCnt2, &shBase2);
+  const int totWork = blockSumI(nWork, shWave);
+  const int totSup = blockSumI(nSup, shWave);
+  if (threadIdx.x == 0) {
+    shBase = totWork ? atomicAdd(count, totWork) : 0;
+    shBase2 = totSup ? atomicAdd(supCount, totSup) : 0;
+  }
+  __syncthreads();
+  // pass 2: the entries in chunk order (block-uniform trip count: every thread ranks)
+  int runWork = shBase, runSup = shBase2;
+  for (int m0 = ch.x; m0 < ch.y; m0 += blockDim.x) {
+    const int m = m0 + (int)threadIdx.x;
+    const bool in = m < ch.y;
+    const bool w = in && D.mColor[m] == -1;
+    const bool sp = in && D.mCode[m] != 0;
+    int tw, ts;
+    const int rw = blockRank(w, shWave, tw);
+    const int rs = blockRank(sp, shWave, ts);
+    if (w) work[runWork + rw] = m;
+    if (sp) supList[runSup + rs] = m;
+    runWork += tw;
+    runSup += ts;
+  }
 }
 // One JP round over the compacted work list.  winner[] holds, per body, 1 + the largest priority
 // among its uncoloured edges; a committing edge clears the slots of its two bodies, a losing edge
@@ -3676,24 +3739,30 @@ __global__ __launch_bounds__(VBLOCK_THREADS) void k_ordered_block(
 // bucketCnt is all zero on entry (k_ledger_scatter clears what this pass counts).  Each live key
 // takes its place in its bucket from the count's atomic; the dead keys (~0) share one region,
 // their places handed out per block (one atomic per block, then the block's own order).
-__global__ void k_ledger_count(const Dev* __restrict__ Dp, int numManifolds, int numReal,
-                               int* rank) {
+__global__ __launch_bounds__(TPB) void k_ledger_count(const Dev* __restrict__ Dp, int numManifolds,
+                                                      int numReal, int* rank) {
   const Dev& D = *Dp;
-  __shared__ int shCnt, shBase;
-  const int m = gtid();
-  bool dead = false;
-  if (m < numManifolds) {
+  __shared__ int shWave[WAVES], shBase;
+  const int2 ch = chunkOf(numManifolds);
+  // live entries: rank within their bucket; dead ones (key ~0) counted for the chunk
+  int dead = 0;
+  for (int m = ch.x + (int)threadIdx.x; m < ch.y; m += blockDim.x) {
     const unsigned long long key = D.mKey[m];
-    if (key == ~0ull) dead = true;
+    if (key == ~0ull) ++dead;
     else rank[m] = atomicAdd(&D.bucketCnt[(int)(key >> 32)], 1);
   }
-  if (threadIdx.x == 0) shCnt = 0;
+  const int tot = blockSumI(dead, shWave);
+  if (threadIdx.x == 0) shBase = tot ? atomicAdd(&D.bucketCnt[numReal], tot) : 0;
   __syncthreads();
-  const int local = dead ? atomicAdd(&shCnt, 1) : 0;
-  __syncthreads();
-  if (threadIdx.x == 0 && shCnt) shBase = atomicAdd(&D.bucketCnt[numReal], shCnt);
-  __syncthreads();
-  if (dead) rank[m] = shBase + local;
+  int run = shBase;
+  for (int m0 = ch.x; m0 < ch.y; m0 += blockDim.x) {  // (block-uniform trip count)
+    const int m = m0 + (int)threadIdx.x;
+    const bool isDead = m < ch.y && D.mKey[m] == ~0ull;
+    int t;
+    const int r = blockRank(isDead, shWave, t);
+    if (isDead) rank[m] = run + r;
+    run += t;
+  }
 }
 __global__ void k_ledger_scatter(const Dev* __restrict__ Dp, int numManifolds, int numReal,
                                  const int* rank) {

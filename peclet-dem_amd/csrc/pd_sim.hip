@@ -2136,7 +2136,7 @@ struct pd_sim {
     const double gMag0 = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
     const d3 gHat0 = (usePersist && gMag0 > 0.0) ? d3{g[0] / gMag0, g[1] / gMag0, g[2] / gMag0}
                                                  : d3{0, 0, 0};
-    k_jp_init_manifolds<<<std::max(1, nblk(nm)), TPB, 0, st>>>(
+    k_jp_init_manifolds<<<std::max(1, std::min(nblk(nm), CHUNK_GRID)), TPB, 0, st>>>(
         dD, nm, jobV.work[0].p, counters.p + 1, vRowOf.p, gHat0, supList.p, supCnt.p);
     std::vector<int> vs;
     int numColors = 0;
@@ -2383,7 +2383,7 @@ struct pd_sim {
       // cut short by an error leaves it dirty
       if (ledgerCntDirty) zero(bucketCnt.p, bucketCnt.cap);
       ledgerCntDirty = true;
-      k_ledger_count<<<nblk(nm), TPB, 0, st>>>(dD, nm, numReal, lRank.p);
+      k_ledger_count<<<std::min(nblk(nm), CHUNK_GRID), TPB, 0, st>>>(dD, nm, numReal, lRank.p);
       scanQuiet(bucketCnt.p, bucketStart.p, numReal + 1);
       k_ledger_scatter<<<nblk(nm), TPB, 0, st>>>(dD, nm, numReal, lRank.p);
       ledgerCntDirty = false;
