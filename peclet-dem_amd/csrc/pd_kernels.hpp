@@ -255,7 +255,10 @@ __device__ __forceinline__ void blockAtomicMaxU64(unsigned long long* addr, unsi
   if (threadIdx.x == 0) {
     unsigned long long m = sh[0];
     for (int w = 1; w < WAVES; ++w) m = sh[w] > m ? sh[w] : m;
-    if (m) atomicMax(addr, m);
+    // the maximum only grows during a launch: a block below the value already there skips its
+    // atomic (a stale, lower read only costs an atomic that changes nothing)
+    if (m && m > __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(addr, m);
   }
 }
 __device__ __forceinline__ void blockAtomicAddI(int* addr, int v, int* sh) {
