@@ -127,6 +127,29 @@ def make_scene(kind, n=256, seed=0):
                     vel=np.concatenate([v[keep], np.zeros((1, 3))]),
                     angvel=rng.normal(0, 5.0, (m_, 3)), gravity=(0, 0, -9.81),
                     material=(0.3, 0.2, 0.5), iters=(12, 8), dt=1e-5)
+    if kind == "sphere_in_shell":
+        # a sphere of forty radii wrapped in one layer of n - 1 unit spheres on a Fibonacci
+        # lattice, each overlapping it by 0.2 radii: thousands of contacts and manifolds share the
+        # big sphere (its edge lists span several LDS chunks of the per-body sort)
+        r = 1e-3
+        m_small = n - 1
+        k = np.arange(m_small) + 0.5
+        phi = np.arccos(1.0 - 2.0 * k / m_small)
+        theta = np.pi * (1.0 + 5 ** 0.5) * k
+        L = 0.12
+        c = np.array([L / 2, L / 2, L / 2])
+        R = (40.0 + 1.0 - 0.2) * r
+        shell = c + R * np.stack([np.cos(theta) * np.sin(phi), np.sin(theta) * np.sin(phi),
+                                  np.cos(phi)], 1)
+        pos = np.concatenate([shell, c[None]])
+        scale = np.concatenate([np.ones(m_small), [40.0]])
+        rad = r * scale
+        m = 2500.0 * 4.0 / 3.0 * np.pi * rad ** 3
+        return dict(shape=(SPHERE, r), domain=(L, L, L), periodic=(False, False, False),
+                    planes=[], pos=pos, scale=scale, inv_mass=1.0 / m,
+                    inv_inertia=np.repeat((1.0 / (0.4 * m * rad ** 2))[:, None], 3, 1),
+                    vel=rng.normal(0, 0.01, (n, 3)), angvel=rng.normal(0, 1.0, (n, 3)),
+                    gravity=(0, 0, 0), material=(0.3, 0.2, 0.5), iters=(12, 8), dt=1e-5)
     if kind == "column_small_box":
         # a 2 x 2 column of spheres in a periodic box two diameters wide: every sphere has three
         # periodic images

@@ -183,6 +183,19 @@ def test_big_sphere_last_colour_class(hip_lib, tuning):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tuning", [{}, {"colouring_incidence_min_edges": 0}],
+                         ids=["contend-commit", "incidence"])
+def test_sphere_in_shell_thousands_of_edges_at_one_body(hip_lib, tuning):
+    # ~5000 contacts and manifolds share one sphere: ~5000 colouring rounds, the last class
+    # holding thousands of rows at that sphere, and (incidence rounds) a per-body list sorted by
+    # one workgroup against the list staged through LDS in several chunks
+    log = []
+    run_pair("sphere_in_shell", 5001, 2, tuning=tuning, log=log)
+    assert log[0]["num_contacts"] > 4096 and log[0]["num_colors_pos"] == 63
+    assert log[0]["rounds_pos"] > 4096
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n,steps", [(2500, 4), (4000, 3)])
 def test_velocity_stage_large_beds(hip_lib, n, steps):
     # 2500 spheres: ~4500-6700 rows over ~2900 bodies, so the staged stage keeps 2400 bodies in
