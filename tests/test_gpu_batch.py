@@ -8,7 +8,9 @@ from scenes import apply_scene, make_scene
 
 
 @pytest.mark.gpu
-def test_batch_systems_equal_independent_runs(hip_lib):
+@pytest.mark.parametrize("tuning", [{}, {"colouring_incidence_min_edges": 0}],
+                         ids=["default", "incidence-colouring"])
+def test_batch_systems_equal_independent_runs(hip_lib, tuning):
     from peclet.dem import Batch, Simulation
 
     specs = [("bed_periodic", 300, 1, dict()), ("bed", 150, 2, dict(material=(0.5, 0.0, 0.3))),
@@ -23,6 +25,8 @@ def test_batch_systems_equal_independent_runs(hip_lib):
     for sim, sc in zip(batch.systems, scenes):
         apply_scene(sim, sc)
         alone.append(apply_scene(Simulation(len(sc["pos"])), sc))
+        for key, value in tuning.items():
+            sim.set_tuning(key, value)  # (the systems stepped alone keep the defaults)
     dt = scenes[0]["dt"]  # the batch steps every system with the same dt
     for _ in range(4):
         batch.step(dt, 2)
