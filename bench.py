@@ -75,12 +75,12 @@ def substep_bytes(st):
     G = st["num_particles"] - N
     P, Cn, M = st["num_pairs"], st["num_contacts"], st["num_manifolds"]
     b = 684 * N + 460 * G + 152 * P + 236 * Cn + 1661 * M
-    # the colourings as the incidence-list rounds move them: 248 B per coloured edge and 40 B per
+    # the colourings as the incidence-list rounds move them: 256 B per coloured edge and 40 B per
     # body, whichever round form ran (the contend / commit rounds move more)
     if st["active_manifolds"] > 0:
-        b += 248 * st["active_manifolds"] + 40 * N
+        b += 256 * st["active_manifolds"] + 40 * N
     if Cn > 0:
-        b += 248 * Cn + 40 * (N + G)
+        b += 256 * Cn + 40 * (N + G)
     b += 601 * (st["sweeps_vel"] + st["sweeps_stab"]) * st["active_manifolds"]
     b += 388 * st["sweeps_pos"] * Cn
     return b

@@ -1817,27 +1817,31 @@ __device__ __forceinline__ void blockAppend2(bool t0, int4 i0, bool t1, int4 i1,
   if (t1) list[base + __popcll(m0) + __popcll(m1 & below)] = i1;
   __syncthreads();  // the slots are reused by the next call
 }
-// incidence counts of the entering edges (deg[] is zero on entry); arrived[e] = 0
+// incidence counts of the entering edges (deg[] is zero on entry), each entering edge's place in
+// the lists of its bodies (slot[t]: at ends.x, at ends.y or -1), arrived[e] = 0
 __global__ void k_jpi_count(const int2* ends, const int* work, const int* count, int nHost,
-                            int* deg, int* arrived) {
+                            int* deg, int2* slot, int* arrived) {
   const int n = count ? *count : nHost;
   for (int t = gtid(); t < n; t += gridDim.x * blockDim.x) {
     const int e = work[t];
     const int2 ab = ends[e];
-    atomicAdd(&deg[ab.x], 1);
-    if (jpTwoBodies(ab)) atomicAdd(&deg[ab.y], 1);
+    const int sa = atomicAdd(&deg[ab.x], 1);
+    const int sb = jpTwoBodies(ab) ? atomicAdd(&deg[ab.y], 1) : -1;
+    slot[t] = make_int2(sa, sb);
     arrived[e] = 0;
   }
 }
-// every entering edge into the (unordered) lists of its bodies at start[]; deg[] returns to zero
+// every entering edge into the (unordered) lists of its bodies at start[] + its slot (the lists
+// are put in priority order afterwards, so the order of the slots does not matter)
 __global__ void k_jpi_fill(const int2* ends, const int* work, const int* count, int nHost,
-                           const int* start, int* deg, int* lists) {
+                           const int* start, const int2* slot, int* lists) {
   const int n = count ? *count : nHost;
   for (int t = gtid(); t < n; t += gridDim.x * blockDim.x) {
     const int e = work[t];
     const int2 ab = ends[e];
-    lists[start[ab.x] + atomicSub(&deg[ab.x], 1) - 1] = e;
-    if (jpTwoBodies(ab)) lists[start[ab.y] + atomicSub(&deg[ab.y], 1) - 1] = e;
+    const int2 sl = slot[t];
+    lists[start[ab.x] + sl.x] = e;
+    if (sl.y >= 0) lists[start[ab.y] + sl.y] = e;
   }
 }
 // the list entry of edge f at body x

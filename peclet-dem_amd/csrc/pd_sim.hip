@@ -1196,7 +1196,7 @@ struct pd_sim {
     // rounds over ordered incidence lists (k_jpi_*) instead of contend / commit rounds
     bool inc = false;
     DBuf<int> iDeg, iStart, iList, iArrived, iLong, iScan;
-    DBuf<int2> iSorted;
+    DBuf<int2> iSorted, iSlot;
     DBuf<JpBody> iBody;
     DBuf<int4> iWork[2];
     const int* hostEntering = nullptr;  // the entering count in the first read-back (device count)
@@ -1208,7 +1208,7 @@ struct pd_sim {
       colStart.release(); winner.release(); mask.release(); clsCnt.release();
       iDeg.release(); iStart.release(); iList.release(); iSorted.release(); iBody.release();
       iArrived.release(); iLong.release(); iScan.release(); iWork[0].release();
-      iWork[1].release();
+      iWork[1].release(); iSlot.release();
     }
   };
   ColourJob jobV, jobP;  // manifold colouring (main stream), contact colouring (side stream)
@@ -1284,6 +1284,7 @@ struct pd_sim {
       J.iLong.ensure(nb + 1, s);
       J.iList.ensure(2 * ne + 1, s);
       J.iSorted.ensure(2 * ne + 1, s);
+      J.iSlot.ensure(ne + 1, s);
       J.iArrived.ensure(ne + 1, s);
       J.iWork[0].ensure(ne + 1, s);
       J.iWork[1].ensure(ne + 1, s);
@@ -1293,11 +1294,11 @@ struct pd_sim {
       const int* work0 = J.work[0].p;
       k_fill_i32<<<nblk(nb), TPB, 0, s>>>(J.iDeg.p, 0, nb);
       k_jpi_count<<<gridE, TPB, 0, s>>>(J.ends, work0, J.enteringDev, (int)J.entering, J.iDeg.p,
-                                        J.iArrived.p);
+                                        J.iSlot.p, J.iArrived.p);
       if (nbs > 1) k_scan_reduce<<<nbs, TPB, 0, s>>>(J.iDeg.p, nb, J.iScan.p, nullptr);
       k_scan_down2<<<nbs, TPB, 0, s>>>(J.iDeg.p, nb, J.iScan.p, J.iStart.p, nullptr);
       k_jpi_fill<<<gridE, TPB, 0, s>>>(J.ends, work0, J.enteringDev, (int)J.entering,
-                                       J.iStart.p, J.iDeg.p, J.iList.p);
+                                       J.iStart.p, J.iSlot.p, J.iList.p);
       k_jp_counts<<<1, 64, 0, s>>>(J.cnt.p, nullptr, 0, JP_MAX_BATCH + 2);
       J.cur = 0;
       k_jpi_sort<<<jblk(nb), JPI_THREADS, 0, s>>>(J.ends, J.iStart.p, J.iList.p, J.iSorted.p, nb,
