@@ -1822,13 +1822,35 @@ __device__ __forceinline__ void blockAppend2(bool t0, int4 i0, bool t1, int4 i1,
 __global__ void k_jpi_count(const int2* ends, const int* work, const int* count, int nHost,
                             int* deg, int2* slot, int* arrived) {
   const int n = count ? *count : nHost;
-  for (int t = gtid(); t < n; t += gridDim.x * blockDim.x) {
-    const int e = work[t];
-    const int2 ab = ends[e];
-    const int sa = atomicAdd(&deg[ab.x], 1);
-    const int sb = jpTwoBodies(ab) ? atomicAdd(&deg[ab.y], 1) : -1;
-    slot[t] = make_int2(sa, sb);
-    arrived[e] = 0;
+  const int lane = (int)(threadIdx.x & 63);
+  // wave-uniform trip count; a wave's edges are consecutive list entries, so edges listed next
+  // to each other with the same first body (the usual case: edges follow the pair order) take
+  // their places at that body with one atomic per run
+  for (int t0 = gtid() - lane; t0 < n; t0 += gridDim.x * blockDim.x) {
+    const int t = t0 + lane;
+    const bool act = t < n;  // the active lanes are a prefix of the wave
+    int e = 0;
+    int2 ab = make_int2(-1, -1);
+    if (act) {
+      e = work[t];
+      ab = ends[e];
+    }
+    const int prev = __shfl_up(ab.x, 1);
+    const bool head = act && (lane == 0 || prev != ab.x);
+    const unsigned long long heads = __ballot(head);
+    const int nAct = __popcll(__ballot(act));
+    // this lane's run: from its head lane hl to the next head (or the last active lane)
+    const unsigned long long upTo = heads & ((2ull << lane) - 1ull);
+    const int hl = (act && upTo) ? 63 - __clzll(upTo) : 0;
+    const unsigned long long after = heads & ~((2ull << hl) - 1ull);
+    const int runEnd = after ? __ffsll((long long)after) - 1 : nAct;
+    int base = head ? atomicAdd(&deg[ab.x], runEnd - hl) : 0;
+    base = __shfl(base, hl);
+    if (act) {
+      const int sb = jpTwoBodies(ab) ? atomicAdd(&deg[ab.y], 1) : -1;
+      slot[t] = make_int2(base + (lane - hl), sb);
+      arrived[e] = 0;
+    }
   }
 }
 // every entering edge into the (unordered) lists of its bodies at start[] + its slot (the lists
